@@ -1,0 +1,142 @@
+"""Benchmark: particle-steps/s of the plain delta-SPH step on the 3-D dam break.
+
+Workload: tank 3.22 x 1.0 x 1.0 m, water column 1.2 x 1.0 x 0.55 m, dx 0.008 m
+(1,293,750 fluid particles on a lattice + 3 dummy layers on the floor and the
+four walls, 1,854,987 particles in all), Wendland C2 with h = 1.3 dx, Tait EOS
+gamma 7, rho0 1000, c0 = 10 sqrt(2 g H), delta 0.1, alpha 0.02, CFL 0.2, FP32.
+One step = one full predictor-corrector step (two half steps: hash, radix
+sort, cell table + reorder, wall pass, gradient pass, continuity+momentum,
+integrator) for every particle; particles are resident in HBM before timing.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--dx 0.008] [--no-cpu-baseline]
+
+Prints one JSON line.  Only one GPU is used: --gpus above 1 exits.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
+
+
+def step_bytes(case):
+    """Algorithmic HBM bytes per particle-step of the step's kernel schedule:
+    640 + 48 G/N + 16 f_wall + 204 f_fluid (compulsory traffic, neighbour
+    reads served by caches, sort at its single-pass bound)."""
+    n = case.n
+    G = case.cpu.grid_num
+    ff = case.json.water_ptc_num / n
+    return 640.0 + 48.0 * G / n + 16.0 * (1.0 - ff) + 204.0 * ff
+
+
+def governing_bytes(case):
+    """Algorithmic bytes of one continuity+momentum launch: per particle it
+    reads (x,rho) 16, (v,p) 16, (grad rho, div v) 16, type 4, hash 4 and writes
+    (a, drho/dt) 16 = 72 B; plus one pass over the cell table, 4 (G+1) B."""
+    return 72.0 * case.n + 4.0 * (case.cpu.grid_num + 1)
+
+
+def cpu_baseline(case, steps=3, warmup=1):
+    """FP32 CPU oracle (OpenMP over particles) on this host's cores, same case."""
+    import oracle
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    o = oracle.Oracle(case.params, fp64=False)
+    o.upload(case.host)
+    o.step(warmup)
+    t0 = time.perf_counter()
+    o.step(steps)
+    dt = time.perf_counter() - t0
+    o.close()
+    return {"value": case.n * steps / dt, "unit": "particle-steps/s", "cores": threads, "kind": "port",
+            "sample": f"{steps} steps (after {warmup} warm-up) of the same {case.n}-particle 3-D dam break, "
+                      f"FP32 oracle, OpenMP {threads} threads"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--dx", type=float, default=0.008)
+    ap.add_argument("--profile-steps", type=int, default=3)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); the step is single-GPU.",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import zsph_amd as z
+    from zsph_amd import cases
+
+    case = cases.dambreak3d(dx=args.dx)
+    print(f"[bench] 3-D dam break dx={args.dx}: N={case.n} (fluid {case.json.water_ptc_num}), "
+          f"grid {case.cpu.grid_xdim}x{case.cpu.grid_ydim}x{case.cpu.grid_zdim}", file=sys.stderr, flush=True)
+    s = z.Solver(case, device=0)
+    s.upload(case.host)
+    s.step(args.warmup)
+    s.sync()
+    print(f"[bench] warm-up {args.warmup} steps done", file=sys.stderr, flush=True)
+
+    # timed region: K graph-replayed steps, synchronised on both sides
+    s.sync()
+    t0 = time.perf_counter()
+    s.step(args.steps)
+    s.sync()
+    wall = time.perf_counter() - t0
+    # the same K steps again between HIP events on the solver's stream
+    ev_ms = s.time_steps(args.steps)
+    print(f"[bench] {args.steps} steps: host {wall * 1e3 / args.steps:.3f} ms/step, "
+          f"events {ev_ms / args.steps:.3f} ms/step", file=sys.stderr, flush=True)
+
+    # per-kernel times (HIP events around every stage on the same stream)
+    prof = s.profile_steps(args.profile_steps)
+    launches = 2 * args.profile_steps
+    gov_ms = prof["governing"] / launches
+    killed = s.killed()
+    s.close()
+
+    ms_step = wall * 1e3 / args.steps
+    value = case.n * args.steps / wall
+    gbytes = governing_bytes(case)
+    gov_gbs = gbytes / (gov_ms * 1e-3) / 1e9
+    b_step = step_bytes(case)
+    out = {
+        "metric": "particle-steps/s, 1.3M 3D dam-break",
+        "value": value,
+        "unit": "particle-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (lattice dam break generated in-process)",
+        "config": {"workload": f"3-D dam break 3.22x1.0x1.0 m, column 1.2x1.0x0.55 m, dx {args.dx} m, "
+                               f"{case.n} particles ({case.json.water_ptc_num} fluid), delta-SPH Antuono, "
+                               f"Tait gamma 7, CFL 0.2",
+                   "particles": case.n, "fluid_particles": case.json.water_ptc_num,
+                   "fluid_particle_steps_per_s": case.json.water_ptc_num * args.steps / wall,
+                   "event_ms_per_step": ev_ms / args.steps, "removed_particles": killed},
+        "roofline": {"bound": "hbm", "kernel": "k_governing", "achieved": gov_gbs, "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": gov_gbs / HBM_PEAK_GBS, "traffic": None,
+                     "bytes_per_launch": gbytes, "avg_launch_ms": gov_ms},
+        "step_roofline": {"bytes_per_particle_step": b_step, "achieved": b_step * value / 1e9,
+                          "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": b_step * value / 1e9 / HBM_PEAK_GBS},
+        "stage_ms_per_step": {k: v / args.profile_steps for k, v in prof.items()},
+    }
+    if not args.no_cpu_baseline:
+        print("[bench] CPU baseline (FP32 oracle)...", file=sys.stderr, flush=True)
+        out["cpu_baseline"] = cpu_baseline(case)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

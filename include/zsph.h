@@ -1,0 +1,299 @@
+/*
+ * zsph.h - C ABI of the MI355X (gfx950) delta-SPH step library (libzsph.so).
+ *
+ * Every entry point replaces one host function or kernel launch of the
+ * reference single-GPU driver (single_gpu_sph_src/deltasph/main.cu:58-90);
+ * the reference symbol is cited on each declaration.  The ABI uses plain C
+ * types only: host arrays are caller-owned, device memory is owned by an
+ * opaque context, every call returns 0 on success or a negative ZSPH_E* code
+ * (the library never exits the process; zsph_last_error() has the text).
+ *
+ * Host array layout (the reference's cpu_output_t, data_structure.cuh:146-156):
+ *   pos  float[3N]  (x,y,z)        vel  float[3N]
+ *   rhop float[2N]  (rho,p)        acc_drhodt float[4N] (ax,ay,az,drho/dt)
+ *   type int[N]  (1 fluid, 0 fixed dummy wall, 2 floating-body dummy,
+ *                 3 moving-wall dummy (piston), 100 removed)
+ *   table int[N] (original id - current index, sort.cu:107)
+ *   hash  int[N]
+ */
+#ifndef ZSPH_H
+#define ZSPH_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define ZSPH_OK 0
+#define ZSPH_E_ARG (-1)       /* invalid argument / inconsistent parameters */
+#define ZSPH_E_HIP (-2)       /* HIP runtime error                          */
+#define ZSPH_E_NOMEM (-3)     /* device or host allocation failed           */
+#define ZSPH_E_DT (-4)        /* dt above the stability bound (check.cu:3-18) */
+#define ZSPH_E_IO (-5)        /* file could not be read / written / parsed  */
+#define ZSPH_E_STATE (-6)     /* call out of order (e.g. sort before hash)  */
+#define ZSPH_E_NODEV (-7)     /* no HIP device                              */
+
+#define ZSPH_PATH_MAX 256
+
+/* Particle types (the reference only distinguishes type==1 from the rest). */
+#define ZSPH_FLUID 1
+#define ZSPH_WALL 0
+#define ZSPH_BODY 2
+#define ZSPH_PISTON 3
+#define ZSPH_KILLED 100
+
+/* Density-diffusion form. */
+#define ZSPH_DIFF_ANTUONO 0 /* equation/delta_ns.cu (deltasph executable)  */
+#define ZSPH_DIFF_MC 1      /* equation/simple_ns.cu (simplesph executable) */
+
+/*
+ * Case file contents: the reference's cpu_json_t (data_structure.cuh:6-51,
+ * keys read at io/read_json.cu:12-49) plus optional extension keys.
+ */
+typedef struct zsph_json {
+    float g, rho0, dx, h_factor, r_factor, eta_factor, cs_factor, delta, alpha;
+    float xmin, xmax, ymin, ymax, zmin, zmax;
+    float grid_layer_factor, grid_size_factor;
+    float dt;
+    int start_step, current_step, end_step, output_step;
+    int ptc_num, water_ptc_num, air_ptc_num, wall_ptc_num, rigid_ptc_num;
+    int thread_num, gpu_id;
+    char input_path[ZSPH_PATH_MAX];
+    char output_path[ZSPH_PATH_MAX];
+    char git_hash[64];
+    /* extensions; absent keys keep the reference behaviour */
+    int dim;         /* 3 (reference) or 2 (x-z plane, y == 0)                  */
+    float gamma;     /* EOS exponent; 1 = linear p = c^2 (rho - rho0) (reference) */
+    float c0;        /* > 0: explicit sound speed, else cs_factor*sqrt(|g|(zmax-zmin)) */
+    float cfl;       /* > 0: dt = cfl * h / c0, else "dt" key                     */
+    int diffusion;   /* ZSPH_DIFF_ANTUONO or ZSPH_DIFF_MC                          */
+    int output_fields; /* ZSPH_F_* mask written by the VTK writer                  */
+    int binary;      /* 1 = big-endian binary legacy VTK (ZSPH_BINARY), 0 = ASCII */
+} zsph_json;
+
+/* Derived host parameters: the reference's cpu_param_t (data_structure.cuh:53-105). */
+typedef struct zsph_cpu_param {
+    float g, m, rho0, dx, h, r, eta, cs, adh, delta, alpha;
+    float dt;
+    int start_step, current_step, end_step, output_step;
+    float xmin, xmax, ymin, ymax, zmin, zmax;
+    int grid_factor;
+    float grid_size;
+    float grid_xmin, grid_ymin, grid_zmin;
+    float grid_xmax, grid_ymax, grid_zmax;
+    int grid_xdim, grid_ydim, grid_zdim;
+    int grid_num, grid_hash_min, grid_hash_max;
+    int ptc_num, water_ptc_num, air_ptc_num, wall_ptc_num, rigid_ptc_num;
+    int thread_num, gpu_id;
+    char input_path[ZSPH_PATH_MAX];
+    char output_path[ZSPH_PATH_MAX];
+    char git_hash[64];
+    int dim;
+    float gamma;
+    int diffusion;
+    int output_fields;
+    int binary;
+} zsph_cpu_param;
+
+/*
+ * Device parameter block: a superset of the reference's gpu_param_t
+ * (data_structure.cuh:107-134), which the reference keeps in __constant__
+ * memory (deltasph/main.cu:47).  Here it is copied into the context and
+ * passed to every kernel by value.
+ */
+typedef struct zsph_params {
+    float half_dt, h, g, m, rho0, adh, h2, eta2, cs2, alpha, delta_h_cs, h_cs_rho_m;
+    float grid_size;
+    int grid_factor;
+    float grid_xmin, grid_ymin, grid_zmin;
+    int grid_xdim, grid_ydim, grid_zdim;
+    int grid_hash_min, grid_hash_max;
+    int ptc_num;
+    /* extensions */
+    int dim;        /* 2 or 3                                               */
+    float gamma;    /* EOS exponent (1 = reference linear EOS)             */
+    int diffusion;  /* ZSPH_DIFF_*                                          */
+    float rc2;      /* FP32 cut-off (2h)^2 = 4*h2: neighbour iff r^2 <= rc2 */
+} zsph_params;
+
+/* Host particle arrays (any pointer may be NULL to skip that field). */
+typedef struct zsph_host_state {
+    float *pos;
+    float *vel;
+    float *rhop;
+    float *acc_drhodt;
+    int *type;
+    int *table;
+    int *hash;
+} zsph_host_state;
+
+/* Field mask for zsph_download / the VTK writer (replaces ZSPH_OUTPUT_*). */
+#define ZSPH_F_POS 1
+#define ZSPH_F_VEL 2
+#define ZSPH_F_RHOP 4
+#define ZSPH_F_ACC 8
+#define ZSPH_F_TYPE 16
+#define ZSPH_F_TABLE 32
+#define ZSPH_F_HASH 64
+#define ZSPH_F_ALL 127
+
+typedef struct zsph_ctx zsph_ctx;
+
+/* ------------------------------------------------------------------------ */
+/* Host set-up (CPU only)                                                    */
+/* ------------------------------------------------------------------------ */
+
+/* Fill *j with the defaults of every optional key (dim 3, gamma 1, ...). */
+void zsph_json_defaults(zsph_json *j);
+
+/* io/read_json.cu:5-51 read_json.  Reads a flat JSON object. */
+int zsph_read_json(const char *path, zsph_json *out);
+
+/* lib/init.cu:153-219 set_cpu_param + lib/check.cu:3-18 check_dt
+ * (returns ZSPH_E_DT instead of exit(1)). */
+int zsph_set_cpu_param(zsph_cpu_param *out, const zsph_json *in);
+
+/* lib/init.cu:221-251 set_gpu_param. */
+void zsph_set_gpu_param(zsph_params *out, const zsph_cpu_param *in);
+
+/*
+ * Dam-break case generator (the reference reads input.vtk instead,
+ * io/read_vkt.cu:3-92; no case files ship with it).  Lattice fluid column
+ * [0,cw]x[0,wy]x[0,ch] at (i+1/2)dx in a tank [0,lx]x[0,wy]x[0,lz] with
+ * `layers` dummy layers on the floor and the four side walls (dim 2: x-z
+ * plane, floor and two walls).  Hydrostatic initial density through the EOS.
+ * Writes into *j the domain, counts and dt derived from the geometry.  Call
+ * with st == NULL to get the particle count in *n_out, then again with
+ * arrays of that size.
+ */
+typedef struct zsph_dambreak {
+    int dim;
+    float dx;
+    float lx, wy, lz;   /* tank interior                              */
+    float cw, ch;       /* water column width (x) and height (z)      */
+    int layers;         /* dummy wall layers (3 in the configs)       */
+    float wall_height;  /* side walls up to this z (<= 0: lz)          */
+} zsph_dambreak;
+int zsph_case_dambreak(const zsph_dambreak *c, zsph_json *j, zsph_host_state *st, int *n_out);
+
+/* Legacy VTK (io/write_vtk.cu:29-208 write_vtk, io/read_vkt.cu:3-92 read_vtk).
+ * The writer emits every field in `fields`; binary = big-endian. The reader
+ * accepts float or double arrays, BINARY or ASCII, fields p, rho, vel, acc,
+ * type, table (missing p/rho/vel/acc default to 0/rho0/0/0 like the reference). */
+int zsph_write_vtk(const char *path, const zsph_host_state *st, int n, int fields, int binary);
+int zsph_read_vtk(const char *path, zsph_host_state *st, int n, float rho0);
+int zsph_vtk_count(const char *path, int *n_out);
+
+/* ------------------------------------------------------------------------ */
+/* Device context                                                            */
+/* ------------------------------------------------------------------------ */
+
+/* lib/init.cu:38-115 alloc_gpu_ptc_data x2 + alloc_gpu_tmp_data, plus the
+ * cudaMemcpyToSymbol(par/tmp_data) of deltasph/main.cu:47-48 and the device
+ * checks of lib/check.cu:19-59.  device < 0 keeps the current device. */
+int zsph_create(const zsph_params *p, int device, zsph_ctx **out);
+
+/* lib/init.cu:117-134 delete_gpu_*_data. */
+void zsph_destroy(zsph_ctx *ctx);
+
+/* lib/check.cu:96-105 check_gerr, without the exit: synchronise the
+ * context's stream and return the first pending error. */
+int zsph_sync(zsph_ctx *ctx);
+const char *zsph_last_error(zsph_ctx *ctx);
+
+/* lib/init.cu:136-151 cpu_to_gpu: pos, vel, rhop, type, table into buffer
+ * `buf` (0 = "old", 1 = "new").  Saved predictor state is cleared. */
+int zsph_upload(zsph_ctx *ctx, int buf, const zsph_host_state *st);
+
+/* lib/gmemcpy.cu:3-36 gpu_to_cpu, with the ZSPH_OUTPUT_* macros replaced by
+ * the `fields` mask (ZSPH_F_HASH reads the sorted hash of the last sort). */
+int zsph_download(zsph_ctx *ctx, int buf, zsph_host_state *st, int fields);
+
+/* lib/hash.cu:3-26 cuda_ptc_hash: cell hash of buffer `buf`, index = i;
+ * particles outside the grid are removed (type 100, parked at the grid origin). */
+int zsph_hash(zsph_ctx *ctx, int buf);
+
+/* lib/sort.cu:3-6 cuda_sort_index (thrust::sort_by_key): stable LSD radix
+ * sort of (hash, index) on the device. */
+int zsph_sort_index(zsph_ctx *ctx);
+
+/* lib/sort.cu:8-84 cuda_sort_data: cell start/end tables of the sorted
+ * hashes, and the gather of buffer `from` into buffer `to` in cell order. */
+int zsph_sort_data(zsph_ctx *ctx, int from, int to);
+
+/* equation/delta_ns.cu:5-177 and equation/simple_ns.cu:3-97
+ * cuda_boundary_ns: dummy-wall pressure extrapolation, then (delta form)
+ * the renormalised density gradient and velocity divergence; (MC form) the
+ * velocity divergence only. */
+int zsph_boundary(zsph_ctx *ctx, int buf);
+
+/* equation/delta_ns.cu:179-267 and equation/simple_ns.cu:99-180
+ * cuda_govering_ns: continuity + momentum -> acc_drhodt. */
+int zsph_governing(zsph_ctx *ctx, int buf);
+
+/* lib/time_int.cu:3-48 cuda_prediction and :50-91 cuda_correction. */
+int zsph_prediction(zsph_ctx *ctx, int buf);
+int zsph_correction(zsph_ctx *ctx, int buf);
+
+/* deltasph/main.cu:58-90: `nsteps` full predictor-corrector steps starting
+ * and ending in buffer 0, replayed from a captured HIP graph (no host
+ * synchronisation inside; call zsph_sync to wait). */
+int zsph_step(zsph_ctx *ctx, int nsteps);
+
+/* ------------------------------------------------------------------------ */
+/* Diagnostics and raw access (no counterpart in the reference)             */
+/* ------------------------------------------------------------------------ */
+
+/* Internal device arrays, for tests and checkpoints. Element counts:
+ * POSR/VELP/TPOSR/TVELP/ACC/GD: 4N floats; TYPE/TABLE/KEY/INDEX: N ints;
+ * CELL_START: grid_num+1 ints. */
+#define ZSPH_A_POSR 0       /* (x,y,z,rho)      per buffer */
+#define ZSPH_A_VELP 1       /* (vx,vy,vz,p)     per buffer */
+#define ZSPH_A_TPOSR 2      /* saved (x,rho)^n  per buffer */
+#define ZSPH_A_TVELP 3      /* saved (v,p)^n    per buffer */
+#define ZSPH_A_TYPE 4       /* per buffer                  */
+#define ZSPH_A_TABLE 5      /* per buffer                  */
+#define ZSPH_A_KEY 6        /* cell hash (sorted after zsph_sort_index) */
+#define ZSPH_A_INDEX 7      /* sort permutation            */
+#define ZSPH_A_CELL_START 8 /* first sorted index of each cell; [G] = N */
+#define ZSPH_A_ACC 9        /* (ax,ay,az,drho/dt)          */
+#define ZSPH_A_GD 10        /* (grad rho^L, div v)         */
+int zsph_array_get(zsph_ctx *ctx, int which, int buf, void *host, size_t bytes);
+int zsph_array_set(zsph_ctx *ctx, int which, int buf, const void *host, size_t bytes);
+
+/* Per-particle neighbour count and an order-independent 64-bit digest of
+ * the neighbour-index set of buffer `buf` (after zsph_sort_data), using the
+ * same cell walk and r^2 <= rc2 predicate as the physics kernels. */
+int zsph_neighbor_stats(zsph_ctx *ctx, int buf, int *count, uint64_t *digest);
+
+/* Timing on the context's stream with HIP events (no counterpart in the
+ * reference, which prints host wall time per half step, main.cu:91-95).
+ * zsph_time_steps: `nsteps` graph-replayed steps bracketed by two events;
+ * *ms = elapsed milliseconds.  zsph_profile_steps: `nsteps` steps issued
+ * stage by stage (same kernels, no graph) with an event pair around every
+ * stage; stage_ms[ZSPH_ST_*] accumulates milliseconds over all half steps. */
+#define ZSPH_ST_HASH 0
+#define ZSPH_ST_SORT 1
+#define ZSPH_ST_CELLS 2
+#define ZSPH_ST_REORDER 3
+#define ZSPH_ST_WALL 4
+#define ZSPH_ST_LDG 5
+#define ZSPH_ST_GOVERNING 6
+#define ZSPH_ST_INTEGRATE 7
+#define ZSPH_ST_COUNT 8
+int zsph_time_steps(zsph_ctx *ctx, int nsteps, float *ms);
+int zsph_profile_steps(zsph_ctx *ctx, int nsteps, float *stage_ms);
+
+/* Number of particles removed by zsph_hash since zsph_create. */
+int zsph_killed(zsph_ctx *ctx, int *killed);
+
+/* sizeof of the ABI structs, so bindings can check their mirrors:
+ * 0 zsph_json, 1 zsph_cpu_param, 2 zsph_params, 3 zsph_host_state, 4 zsph_dambreak. */
+size_t zsph_sizeof(int which);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* ZSPH_H */

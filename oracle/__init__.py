@@ -1,0 +1,138 @@
+"""CPU oracle of the delta-SPH step -- TEST INFRASTRUCTURE ONLY.
+
+ctypes front end of oracle/liboracle.so (built from zsph_oracle.hpp by
+oracle/Makefile).  Only tests/, the smoke check and the bench's CPU-baseline
+leg use it, as the checker; the product package never imports it.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "liboracle.so")
+_LIB = None
+
+A_ACC_SCALE = 100   # per particle (sum |acc terms|, sum |drho terms|)
+A_GD_SCALE = 101    # per particle (sum |grad rho terms|, sum |div v terms|)
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} missing: run make -C oracle")
+        from zsph_amd import _abi as abi
+        L = C.CDLL(LIB_PATH)
+        vp = C.c_void_p
+        P = C.POINTER
+
+        def sig(name, res, args):
+            f = getattr(L, name)
+            f.restype, f.argtypes = res, args
+
+        sig("orc_create", vp, [P(abi.Params), C.c_int])
+        sig("orc_destroy", None, [vp])
+        sig("orc_upload", C.c_int, [vp, C.c_int, P(abi.HostState)])
+        sig("orc_download", C.c_int, [vp, C.c_int, P(abi.HostState), C.c_int])
+        for n in ("orc_hash", "orc_boundary", "orc_wall", "orc_ldg", "orc_governing", "orc_prediction",
+                  "orc_correction", "orc_step"):
+            sig(n, C.c_int, [vp, C.c_int])
+        sig("orc_sort_index", C.c_int, [vp])
+        sig("orc_sort_data", C.c_int, [vp, C.c_int, C.c_int])
+        sig("orc_killed", C.c_int, [vp, P(C.c_longlong)])
+        sig("orc_array_get", C.c_int, [vp, C.c_int, C.c_int, P(C.c_double), C.c_size_t])
+        sig("orc_array_set", C.c_int, [vp, C.c_int, C.c_int, P(C.c_double), C.c_size_t])
+        sig("orc_neighbor_stats", C.c_int, [vp, C.c_int, C.c_int, P(C.c_int), P(C.c_uint64)])
+        for n in ("orc_kernel_W", "orc_kernel_F", "orc_eos_p", "orc_eos_rho"):
+            sig(n, C.c_double, [P(abi.Params), C.c_double])
+        sig("orc_cell", C.c_int, [C.c_float, C.c_float, C.c_float, C.c_int])
+        _LIB = L
+    return _LIB
+
+
+class Oracle:
+    """Same call sequence as zsph_amd.Solver, on the CPU (fp64=True: parity oracle)."""
+
+    def __init__(self, params, fp64=True):
+        self.params = params
+        self.n = params.ptc_num
+        self.grid_num = params.grid_xdim * params.grid_ydim * params.grid_zdim
+        self._h = lib().orc_create(C.byref(params), 1 if fp64 else 0)
+
+    def close(self):
+        if self._h:
+            lib().orc_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _ok(self, rc, what):
+        if rc != 0:
+            raise RuntimeError(f"oracle {what} failed ({rc})")
+
+    def upload(self, host, buf=0):
+        st = host.state()
+        self._ok(lib().orc_upload(self._h, buf, C.byref(st)), "upload")
+
+    def download(self, buf=0, fields=127):
+        from zsph_amd import HostArrays
+        h = HostArrays(self.n)
+        st = h.state()
+        self._ok(lib().orc_download(self._h, buf, C.byref(st), fields), "download")
+        return h
+
+    def hash(self, buf=0): self._ok(lib().orc_hash(self._h, buf), "hash")
+    def sort_index(self): self._ok(lib().orc_sort_index(self._h), "sort_index")
+    def sort_data(self, src=0, dst=1): self._ok(lib().orc_sort_data(self._h, src, dst), "sort_data")
+    def boundary(self, buf=1): self._ok(lib().orc_boundary(self._h, buf), "boundary")
+    def wall(self, buf=1): self._ok(lib().orc_wall(self._h, buf), "wall")
+    def ldg(self, buf=1): self._ok(lib().orc_ldg(self._h, buf), "ldg")
+    def governing(self, buf=1): self._ok(lib().orc_governing(self._h, buf), "governing")
+    def prediction(self, buf=1): self._ok(lib().orc_prediction(self._h, buf), "prediction")
+    def correction(self, buf=0): self._ok(lib().orc_correction(self._h, buf), "correction")
+    def step(self, n=1): self._ok(lib().orc_step(self._h, n), "step")
+
+    def killed(self):
+        k = C.c_longlong(0)
+        self._ok(lib().orc_killed(self._h, C.byref(k)), "killed")
+        return k.value
+
+    def get(self, which, buf=0):
+        from zsph_amd import A_CELL_START
+        if which == A_CELL_START:
+            count = self.grid_num + 1
+            shape = (count,)
+        elif which in (A_ACC_SCALE, A_GD_SCALE):
+            count, shape = 2 * self.n, (self.n, 2)
+        elif which in (0, 1, 2, 3, 9, 10):
+            count, shape = 4 * self.n, (self.n, 4)
+        else:
+            count, shape = self.n, (self.n,)
+        out = np.empty(count, np.float64)
+        self._ok(lib().orc_array_get(self._h, which, buf, out.ctypes.data_as(C.POINTER(C.c_double)), count),
+                 "array_get")
+        return out.reshape(shape)
+
+    def set(self, which, arr, buf=0):
+        a = np.ascontiguousarray(arr, dtype=np.float64).ravel()
+        self._ok(lib().orc_array_set(self._h, which, buf, a.ctypes.data_as(C.POINTER(C.c_double)), a.size),
+                 "array_set")
+
+    def neighbor_stats(self, buf=1, brute=False):
+        cnt = np.empty(self.n, np.int32)
+        dig = np.empty(self.n, np.uint64)
+        self._ok(lib().orc_neighbor_stats(self._h, buf, 1 if brute else 0, cnt.ctypes.data_as(C.POINTER(C.c_int)),
+                                          dig.ctypes.data_as(C.POINTER(C.c_uint64))), "neighbor_stats")
+        return cnt, dig
+
+
+def kernel_W(params, r): return lib().orc_kernel_W(C.byref(params), r)
+def kernel_F(params, r): return lib().orc_kernel_F(C.byref(params), r)
+def eos_p(params, rho): return lib().orc_eos_p(C.byref(params), rho)
+def eos_rho(params, p): return lib().orc_eos_rho(C.byref(params), p)
+def cell(x, o, size, dim): return lib().orc_cell(x, o, size, dim)

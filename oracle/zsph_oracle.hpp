@@ -1,0 +1,437 @@
+// CPU oracle of the delta-SPH step: TEST INFRASTRUCTURE ONLY.
+//
+// A serial/OpenMP restatement of the reference's equations, templated on
+// the arithmetic type (double = the parity oracle, float = the CPU
+// baseline).  It is imported only by tests/, the smoke check and the
+// bench's cpu_baseline leg; the product library never links it.
+//
+// Followed reference lines (relative to the reference root):
+//   hash            single_gpu_sph_src/lib/hash.cu:3-26
+//   sort            single_gpu_sph_src/lib/sort.cu:3-6 (defined stable)
+//   cells, reorder  single_gpu_sph_src/lib/sort.cu:8-84
+//   boundary        single_gpu_sph_src/equation/delta_ns.cu:5-177,
+//                   single_gpu_sph_src/equation/simple_ns.cu:3-97
+//   governing       single_gpu_sph_src/equation/delta_ns.cu:179-267,
+//                   single_gpu_sph_src/equation/simple_ns.cu:99-180
+//   integrator      single_gpu_sph_src/lib/time_int.cu:3-91
+//   step order      single_gpu_sph_src/deltasph/main.cu:58-90
+// Intended-equation decisions (not the reference's defects): full +-k cell
+// rows in every pass; the Antuono psi term uses (G_i + G_j).x_ij; wall
+// (rho, p) are final before any pass reads them; the last sorted particle
+// belongs to its cell; per-axis grid bounds.
+//
+// The integer stages (cell hash, neighbour predicate) are evaluated in FP32
+// on FP32 positions in every instantiation, so they are bit-identical to a
+// correct FP32 implementation.  Physics uses the template type.
+#pragma once
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <numeric>
+#include <vector>
+
+#include "zsph.h"
+
+namespace orc {
+
+// FP32 neighbour predicate inputs: r^2 = fma(dz,dz,fma(dy,dy,dx*dx)).
+inline float r2_f32(float dx, float dy, float dz) { return std::fmaf(dz, dz, std::fmaf(dy, dy, dx * dx)); }
+
+// floor((x - o) / size) in IEEE FP32; false outside [0, dim) or NaN.
+inline bool cell_f32(float x, float o, float size, int dim, int *c) {
+    volatile float num = x - o;  // keep the FP32 rounding of the difference
+    const float t = std::floor(num / size);
+    if (!(t >= 0.0f && t < (float)dim)) return false;
+    *c = (int)t;
+    return true;
+}
+
+template <class R>
+struct Buffer {
+    std::vector<R> posr, velp, tposr, tvelp;  // 4 per particle: (x,y,z,rho) (v,p)
+    std::vector<int> type, table;
+    void resize(size_t n) {
+        posr.assign(4 * n, R(0)); velp.assign(4 * n, R(0));
+        tposr.assign(4 * n, R(0)); tvelp.assign(4 * n, R(0));
+        type.assign(n, 0); table.assign(n, 0);
+    }
+};
+
+template <class R>
+struct Oracle {
+    zsph_params p{};
+    int N = 0, G = 0, X = 1, Y = 1, Z = 1, k = 1;
+    Buffer<R> buf[2];
+    std::vector<uint32_t> key;  // unsorted after hash, sorted after sort
+    std::vector<int> idx;       // permutation after sort
+    std::vector<int> cs;        // G+1
+    std::vector<R> acc, gd;     // 4 per particle
+    std::vector<R> acc_scale, gd_scale;  // sum |term| per particle: (acc, drho), (G, D)
+    long long killed = 0;
+    bool hashed = false, sorted = false;
+
+    explicit Oracle(const zsph_params &prm) : p(prm) {
+        N = p.ptc_num;
+        X = p.grid_xdim; Y = p.grid_ydim; Z = p.grid_zdim;
+        G = X * Y * Z;
+        k = p.grid_factor;
+        buf[0].resize(N); buf[1].resize(N);
+        key.assign(N, 0); idx.assign(N, 0); cs.assign((size_t)G + 1, 0);
+        acc.assign(4 * (size_t)N, R(0)); gd.assign(4 * (size_t)N, R(0));
+        acc_scale.assign(2 * (size_t)N, R(0)); gd_scale.assign(2 * (size_t)N, R(0));
+    }
+
+    // ---- constants in the working precision --------------------------------
+    R rho0() const { return R(p.rho0); }
+    R c2() const { return R(p.cs2); }
+    R gam() const { return R(p.gamma); }
+    R Bcoef() const { return R(p.rho0) * R(p.cs2) / R(p.gamma); }
+
+    // EOS p(rho): linear c^2 (rho - rho0) for gamma == 1 (time_int.cu:39-40),
+    // Tait B ((rho/rho0)^gamma - 1) otherwise.
+    R eos_p(R rho) const {
+        if (p.gamma == 1.0f) return c2() * (rho - rho0());
+        return Bcoef() * (std::pow(rho / rho0(), gam()) - R(1));
+    }
+    // Inverse EOS (delta_ns.cu:129 for gamma == 1).
+    R eos_rho(R pr) const {
+        if (p.gamma == 1.0f) return pr / c2() + rho0();
+        return rho0() * std::pow(R(1) + pr / Bcoef(), R(1) / gam());
+    }
+    // Wendland C2: W = aD (1-q/2)^4 (2q+1), F = (1/r) dW/dr = -5 aD (1-q/2)^3 / h^2.
+    R kernel_W(R r) const {
+        const R q = r / R(p.h);
+        const R t = R(1) - q / R(2);
+        return R(p.adh) * t * t * t * t * (R(2) * q + R(1));
+    }
+    R kernel_F(R r) const {
+        const R q = r / R(p.h);
+        const R t = R(1) - q / R(2);
+        return R(-5) * R(p.adh) * t * t * t / (R(p.h) * R(p.h));
+    }
+
+    // ---- neighbour walk --------------------------------------------------------
+    // Candidates of particle i (sorted order) in buffer b: the (2k+1)^3 cells
+    // around its cell, clipped to the grid, ascending within each x-run.
+    template <class F>
+    void walk(const Buffer<R> &b, int i, F &&f) const {
+        const uint32_t h = key[i];
+        const int cz = (int)(h / (uint32_t)(X * Y));
+        const int cy = (int)((h / (uint32_t)X) % (uint32_t)Y);
+        const int cx = (int)(h % (uint32_t)X);
+        const int x0 = std::max(cx - k, 0), x1 = std::min(cx + k, X - 1);
+        for (int z = std::max(cz - k, 0); z <= std::min(cz + k, Z - 1); ++z)
+            for (int y = std::max(cy - k, 0); y <= std::min(cy + k, Y - 1); ++y) {
+                const int row = (z * Y + y) * X;
+                for (int j = cs[row + x0]; j < cs[row + x1 + 1]; ++j) f(j);
+            }
+        (void)b;
+    }
+    bool is_nbr(const Buffer<R> &b, int i, int j) const {
+        const float dx = (float)b.posr[4 * i] - (float)b.posr[4 * j];
+        const float dy = (float)b.posr[4 * i + 1] - (float)b.posr[4 * j + 1];
+        const float dz = (float)b.posr[4 * i + 2] - (float)b.posr[4 * j + 2];
+        return r2_f32(dx, dy, dz) <= p.rc2;
+    }
+
+    // ---- stages ----------------------------------------------------------------
+    // hash.cu:3-26
+    void hash(int bi) {
+        Buffer<R> &b = buf[bi];
+        long long kills = 0;
+#pragma omp parallel for schedule(static) reduction(+ : kills)
+        for (int i = 0; i < N; ++i) {
+            int cx, cy, cz;
+            const bool in = cell_f32((float)b.posr[4 * i], p.grid_xmin, p.grid_size, X, &cx) &&
+                            cell_f32((float)b.posr[4 * i + 1], p.grid_ymin, p.grid_size, Y, &cy) &&
+                            cell_f32((float)b.posr[4 * i + 2], p.grid_zmin, p.grid_size, Z, &cz);
+            if (in) {
+                key[i] = (uint32_t)(cz * X * Y + cy * X + cx);
+            } else {
+                b.type[i] = ZSPH_KILLED;
+                b.posr[4 * i] = R(p.grid_xmin); b.posr[4 * i + 1] = R(p.grid_ymin);
+                b.posr[4 * i + 2] = R(p.grid_zmin); b.posr[4 * i + 3] = R(p.rho0);
+                for (int c = 0; c < 4; ++c) b.velp[4 * i + c] = R(0);
+                key[i] = 0;
+                ++kills;
+            }
+            idx[i] = i;
+        }
+        killed += kills;
+        hashed = true;
+        sorted = false;
+    }
+    // sort.cu:3-6, defined as a stable ascending sort of (hash, index).
+    void sort_index() {
+        std::vector<int> perm(N);
+        std::iota(perm.begin(), perm.end(), 0);
+        std::stable_sort(perm.begin(), perm.end(), [&](int a, int c) { return key[a] < key[c]; });
+        std::vector<uint32_t> k2(N);
+        std::vector<int> i2(N);
+        for (int t = 0; t < N; ++t) { k2[t] = key[perm[t]]; i2[t] = idx[perm[t]]; }
+        key.swap(k2);
+        idx.swap(i2);
+        hashed = false;
+        sorted = true;
+    }
+    // sort.cu:8-84: cell c holds sorted positions [cs[c], cs[c+1]).
+    void sort_data(int from, int to) {
+        for (int c = 0; c <= G; ++c)
+            cs[c] = (int)(std::lower_bound(key.begin(), key.end(), (uint32_t)c) - key.begin());
+        const Buffer<R> &a = buf[from];
+        Buffer<R> &b = buf[to];
+#pragma omp parallel for schedule(static)
+        for (int t = 0; t < N; ++t) {
+            const int s = idx[t];
+            for (int c = 0; c < 4; ++c) {
+                b.posr[4 * t + c] = a.posr[4 * s + c];
+                b.velp[4 * t + c] = a.velp[4 * s + c];
+                b.tposr[4 * t + c] = a.tposr[4 * s + c];
+                b.tvelp[4 * t + c] = a.tvelp[4 * s + c];
+            }
+            b.type[t] = a.type[s];
+            b.table[t] = a.table[s] + s - t;
+        }
+    }
+
+    // Dummy pressure (delta_ns.cu:81-85, :116-131): Shepard sum over fluid
+    // neighbours of p_j - rho_j (a_w - g).x_wj with a_w = 0, g = (0,0,g).
+    void wall(int bi) {
+        Buffer<R> &b = buf[bi];
+        std::vector<R> newr(N), newp(N);
+#pragma omp parallel for schedule(static)
+        for (int i = 0; i < N; ++i) {
+            if (b.type[i] == ZSPH_FLUID) continue;
+            R P = 0, S = 0;
+            walk(b, i, [&](int j) {
+                if (b.type[j] != ZSPH_FLUID || !is_nbr(b, i, j)) return;
+                const R dx = b.posr[4 * i] - b.posr[4 * j];
+                const R dy = b.posr[4 * i + 1] - b.posr[4 * j + 1];
+                const R dz = b.posr[4 * i + 2] - b.posr[4 * j + 2];
+                const R W = kernel_W(std::sqrt(dx * dx + dy * dy + dz * dz));
+                const R gx = 0, gy = 0, gz = R(p.g);
+                const R aw_minus_g_dot_x = (R(0) - gx) * dx + (R(0) - gy) * dy + (R(0) - gz) * dz;
+                P += (b.velp[4 * j + 3] - b.posr[4 * j + 3] * aw_minus_g_dot_x) * W;
+                S += W;
+            });
+            R pw = (std::fabs(S) > R(1e-8)) ? P / S : R(0);
+            if (pw < R(0)) pw = R(0);
+            newp[i] = pw;
+            newr[i] = eos_rho(pw);
+        }
+        for (int i = 0; i < N; ++i)
+            if (b.type[i] != ZSPH_FLUID) { b.posr[4 * i + 3] = newr[i]; b.velp[4 * i + 3] = newp[i]; }
+    }
+
+    // Solve L x = g by Gaussian elimination with partial pivoting; returns det.
+    static R solve3(R A[3][3], R g[3], R out[3]) {
+        R M[3][4];
+        for (int r = 0; r < 3; ++r) { for (int c = 0; c < 3; ++c) M[r][c] = A[r][c]; M[r][3] = g[r]; }
+        R det = 1;
+        for (int c = 0; c < 3; ++c) {
+            int piv = c;
+            for (int r = c + 1; r < 3; ++r) if (std::fabs(M[r][c]) > std::fabs(M[piv][c])) piv = r;
+            if (piv != c) { for (int t = 0; t < 4; ++t) std::swap(M[c][t], M[piv][t]); det = -det; }
+            det *= M[c][c];
+            if (M[c][c] == R(0)) return R(0);
+            for (int r = c + 1; r < 3; ++r) {
+                const R f = M[r][c] / M[c][c];
+                for (int t = c; t < 4; ++t) M[r][t] -= f * M[c][t];
+            }
+        }
+        for (int r = 2; r >= 0; --r) {
+            R s = M[r][3];
+            for (int t = r + 1; t < 3; ++t) s -= M[r][t] * out[t];
+            out[r] = s / M[r][r];
+        }
+        return det;
+    }
+
+    // delta_ns.cu:5-177 (after the wall pass): L_i, D_i = div v, G_i = L_i^-1 grad rho.
+    // MC form (simple_ns.cu:3-97): D only.
+    void ldg(int bi) {
+        const Buffer<R> &b = buf[bi];
+        const bool delta = p.diffusion == ZSPH_DIFF_ANTUONO;
+        const R m = R(p.m);
+#pragma omp parallel for schedule(static)
+        for (int i = 0; i < N; ++i) {
+            for (int c = 0; c < 4; ++c) gd[4 * i + c] = 0;
+            gd_scale[2 * i] = gd_scale[2 * i + 1] = 0;
+            if (b.type[i] != ZSPH_FLUID) continue;
+            R L[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+            R g[3] = {0, 0, 0};
+            R D = 0, gs = 0, ds = 0;
+            const R rhoi = b.posr[4 * i + 3];
+            walk(b, i, [&](int j) {
+                if (j == i || !is_nbr(b, i, j)) return;
+                const R x[3] = {b.posr[4 * i] - b.posr[4 * j], b.posr[4 * i + 1] - b.posr[4 * j + 1],
+                                b.posr[4 * i + 2] - b.posr[4 * j + 2]};
+                const R r = std::sqrt(x[0] * x[0] + x[1] * x[1] + x[2] * x[2]);
+                const R F = kernel_F(r);
+                const R rhoj = b.posr[4 * j + 3];
+                const R Vj = m / rhoj;
+                for (int a = 0; a < 3; ++a)
+                    for (int c = 0; c < 3; ++c) L[a][c] -= F * Vj * x[a] * x[c];
+                for (int a = 0; a < 3; ++a) g[a] += F * (rhoj - rhoi) * Vj * x[a];
+                gs += std::fabs(F * (rhoj - rhoi) * Vj) * r;
+                if (b.type[j] == ZSPH_FLUID) {
+                    R dvdx = 0;
+                    for (int a = 0; a < 3; ++a) dvdx += (b.velp[4 * i + a] - b.velp[4 * j + a]) * x[a];
+                    D -= F * dvdx * Vj;
+                    ds += std::fabs(F * dvdx * Vj);
+                }
+            });
+            R G[3] = {0, 0, 0};
+            if (delta) {
+                if (p.dim == 2) L[1][1] += R(1);
+                R Lc[3][3];
+                std::memcpy(Lc, L, sizeof(L));
+                R sol[3] = {0, 0, 0};
+                const R det = solve3(Lc, g, sol);
+                if (std::fabs(det) > R(1.8e-8)) { G[0] = sol[0]; G[1] = sol[1]; G[2] = sol[2]; }
+                else { G[0] = g[0]; G[1] = g[1]; G[2] = g[2]; }
+            }
+            gd[4 * i] = G[0]; gd[4 * i + 1] = G[1]; gd[4 * i + 2] = G[2]; gd[4 * i + 3] = D;
+            gd_scale[2 * i] = delta ? gs : R(0);
+            gd_scale[2 * i + 1] = ds;
+        }
+    }
+
+    // delta_ns.cu:179-267 / simple_ns.cu:99-180.
+    void governing(int bi) {
+        const Buffer<R> &b = buf[bi];
+        const bool delta = p.diffusion == ZSPH_DIFF_ANTUONO;
+        const R m = R(p.m), eta2 = R(p.eta2), dhc = R(p.delta_h_cs), hcrm = R(p.h_cs_rho_m), al = R(p.alpha);
+#pragma omp parallel for schedule(static)
+        for (int i = 0; i < N; ++i) {
+            const int ti = b.type[i];
+            for (int c = 0; c < 4; ++c) acc[4 * i + c] = 0;
+            acc_scale[2 * i] = acc_scale[2 * i + 1] = 0;
+            if (ti != ZSPH_FLUID && ti != ZSPH_BODY && ti != ZSPH_PISTON) continue;
+            const bool fi = ti == ZSPH_FLUID;
+            const R rhoi = b.posr[4 * i + 3], pi = b.velp[4 * i + 3];
+            R a[3] = {0, 0, 0}, dr = 0, as = 0, drs = 0;
+            walk(b, i, [&](int j) {
+                if (j == i) return;
+                const bool fj = b.type[j] == ZSPH_FLUID;
+                if (!fi && !fj) return;
+                if (!is_nbr(b, i, j)) return;
+                const R x[3] = {b.posr[4 * i] - b.posr[4 * j], b.posr[4 * i + 1] - b.posr[4 * j + 1],
+                                b.posr[4 * i + 2] - b.posr[4 * j + 2]};
+                const R r2 = x[0] * x[0] + x[1] * x[1] + x[2] * x[2];
+                const R r = std::sqrt(r2);
+                const R F = kernel_F(r);
+                const R rhoj = b.posr[4 * j + 3], pj = b.velp[4 * j + 3];
+                R dvdx = 0;
+                for (int c = 0; c < 3; ++c) dvdx += (b.velp[4 * i + c] - b.velp[4 * j + c]) * x[c];
+                const R cont = rhoi * dvdx * F * m / rhoj;
+                dr += cont;
+                drs += std::fabs(cont);
+                R pt = -(pi + pj) * m * F / (rhoi * rhoj);
+                R ptabs = std::fabs(pt);
+                if (fi && fj) {
+                    if (delta) {
+                        R gx = 0;
+                        for (int c = 0; c < 3; ++c) gx += (gd[4 * i + c] + gd[4 * j + c]) * x[c];
+                        const R d1 = dhc * F * R(2) * (rhoi - rhoj) * r2 / (r2 + eta2) * m / rhoj;
+                        const R d2 = -dhc * F * gx * m / rhoj;
+                        dr += d1 + d2;
+                        drs += std::fabs(d1) + std::fabs(d2);
+                    }
+                    const R v = hcrm * F * (gd[4 * i + 3] + gd[4 * j + 3] + al * dvdx / (r2 + eta2)) / (rhoi * rhoj);
+                    pt += v;
+                    ptabs += std::fabs(v);
+                }
+                if (!delta) {
+                    const R d = (rhoi - rhoj) * r2 * F * dhc / (r2 + eta2) * m / rhoj;
+                    dr += d;
+                    drs += std::fabs(d);
+                }
+                for (int c = 0; c < 3; ++c) a[c] += pt * x[c];
+                as += ptabs * r;
+            });
+            if (fi) a[2] += R(p.g);
+            acc[4 * i] = a[0]; acc[4 * i + 1] = a[1]; acc[4 * i + 2] = a[2]; acc[4 * i + 3] = dr;
+            acc_scale[2 * i] = as + (fi ? std::fabs(R(p.g)) : R(0));
+            acc_scale[2 * i + 1] = drs;
+        }
+    }
+
+    // time_int.cu:3-48
+    void prediction(int bi) {
+        Buffer<R> &b = buf[bi];
+        const R hdt = R(p.half_dt);
+#pragma omp parallel for schedule(static)
+        for (int i = 0; i < N; ++i) {
+            if (b.type[i] != ZSPH_FLUID) continue;
+            for (int c = 0; c < 4; ++c) { b.tposr[4 * i + c] = b.posr[4 * i + c]; b.tvelp[4 * i + c] = b.velp[4 * i + c]; }
+            for (int c = 0; c < 3; ++c) {
+                b.posr[4 * i + c] = b.tposr[4 * i + c] + b.tvelp[4 * i + c] * hdt;
+                b.velp[4 * i + c] = b.tvelp[4 * i + c] + acc[4 * i + c] * hdt;
+            }
+            R rho = b.tposr[4 * i + 3] + acc[4 * i + 3] * hdt;
+            if (rho < rho0()) rho = rho0();
+            b.posr[4 * i + 3] = rho;
+            b.velp[4 * i + 3] = eos_p(rho);
+        }
+    }
+    // time_int.cu:50-91 (dt = 2 half_dt)
+    void correction(int bi) {
+        Buffer<R> &b = buf[bi];
+        const R dt = R(p.half_dt) * R(2);
+#pragma omp parallel for schedule(static)
+        for (int i = 0; i < N; ++i) {
+            if (b.type[i] != ZSPH_FLUID) continue;
+            R v_star[3] = {b.velp[4 * i], b.velp[4 * i + 1], b.velp[4 * i + 2]};
+            for (int c = 0; c < 3; ++c) {
+                b.posr[4 * i + c] = b.tposr[4 * i + c] + v_star[c] * dt;
+                b.velp[4 * i + c] = b.tvelp[4 * i + c] + acc[4 * i + c] * dt;
+            }
+            R rho = b.tposr[4 * i + 3] + acc[4 * i + 3] * dt;
+            if (rho < rho0()) rho = rho0();
+            b.posr[4 * i + 3] = rho;
+            b.velp[4 * i + 3] = eos_p(rho);
+        }
+    }
+
+    void boundary(int bi) { wall(bi); ldg(bi); }
+
+    // main.cu:58-90: half-step A old->new + prediction, B new->old + correction.
+    void step(int nsteps) {
+        for (int s = 0; s < nsteps; ++s) {
+            hash(0); sort_index(); sort_data(0, 1); boundary(1); governing(1); prediction(1);
+            hash(1); sort_index(); sort_data(1, 0); boundary(0); governing(0); correction(0);
+        }
+    }
+
+    static uint64_t mix64(uint64_t z) {
+        z += 0x9E3779B97F4A7C15ull;
+        z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+        z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+        return z ^ (z >> 31);
+    }
+    // Neighbour count and digest sum mix64(j): by the cell walk, or by brute
+    // force over all j (O(N^2), small cases only).
+    void nbr_stats(int bi, bool brute, int *count, uint64_t *digest) const {
+        const Buffer<R> &b = buf[bi];
+#pragma omp parallel for schedule(dynamic, 64)
+        for (int i = 0; i < N; ++i) {
+            int c = 0;
+            uint64_t h = 0;
+            auto visit = [&](int j) {
+                if (j == i || !is_nbr(b, i, j)) return;
+                ++c;
+                h += mix64((uint64_t)j);
+            };
+            if (brute)
+                for (int j = 0; j < N; ++j) visit(j);
+            else
+                walk(b, i, visit);
+            count[i] = c;
+            digest[i] = h;
+        }
+    }
+};
+
+}  // namespace orc

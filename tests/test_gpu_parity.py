@@ -1,0 +1,226 @@
+"""GPU parity: the HIP kernels of libzsph against the FP64 CPU oracle on the
+same seeded inputs.  Integer stages (cell hash, sort order, cell table,
+gathers, neighbour sets) must be bit-exact; floating-point results follow the
+tolerances in tests/helpers.py (1e-5 relative, sums measured against the sum
+of their absolute terms).
+"""
+import sys
+
+import numpy as np
+import pytest
+
+import oracle
+import zsph_amd as z
+from oracle import A_ACC_SCALE, A_GD_SCALE
+from zsph_amd import (A_ACC, A_CELL_START, A_GD, A_INDEX, A_KEY, A_POSR, A_TABLE, A_TPOSR, A_TVELP, A_TYPE,
+                      A_VELP, FLUID, KILLED, WALL, cases)
+from tests.helpers import REL, assert_close_scaled, jitter, p_scale
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+CASES = {
+    "2d": lambda: jitter(cases.dambreak2d(), seed=12345),
+    "2d-mc-linear": lambda: jitter(cases.dambreak2d(gamma=1.0, diffusion=z.DIFF_MC), seed=4),
+    "3d-small": lambda: jitter(cases.dambreak3d(dx=0.04, lx=0.8, wy=0.4, lz=0.5, cw=0.3, ch=0.3), seed=12345),
+    "3d-mid": lambda: jitter(cases.dambreak3d(dx=0.02), seed=12345),
+}
+
+
+def pair(name):
+    c = CASES[name]()
+    say(f"[case {name}] N={c.n} grid={c.cpu.grid_xdim}x{c.cpu.grid_ydim}x{c.cpu.grid_zdim}")
+    s = z.Solver(c)
+    s.upload(c.host)
+    o = oracle.Oracle(c.params)
+    o.upload(c.host)
+    return c, s, o
+
+
+@pytest.mark.parametrize("name", ["2d", "3d-small", "3d-mid"])
+def test_hash_sort_reorder_bitexact(name):
+    c, s, o = pair(name)
+    s.hash(0); o.hash(0)
+    assert np.array_equal(s.get(A_KEY), o.get(A_KEY).astype(np.uint32))
+    s.sort_index(); o.sort_index()
+    assert np.array_equal(s.get(A_KEY), o.get(A_KEY).astype(np.uint32))
+    assert np.array_equal(s.get(A_INDEX), o.get(A_INDEX).astype(np.int32))
+    s.sort_data(0, 1); o.sort_data(0, 1)
+    assert np.array_equal(s.get(A_CELL_START), o.get(A_CELL_START).astype(np.int32))
+    for a in (A_POSR, A_VELP, A_TPOSR, A_TVELP):
+        assert np.array_equal(s.get(a, 1), o.get(a, 1).astype(np.float32)), a
+    for a in (A_TYPE, A_TABLE):
+        assert np.array_equal(s.get(a, 1), o.get(a, 1).astype(np.int32)), a
+    s.sync()
+
+
+@pytest.mark.parametrize("name", ["2d", "3d-small", "3d-mid"])
+def test_neighbour_sets_bitexact(name):
+    c, s, o = pair(name)
+    for x in (s, o):
+        x.hash(0); x.sort_index(); x.sort_data(0, 1)
+    cg, dg = s.neighbor_stats(1)
+    co, do = o.neighbor_stats(1)
+    assert np.array_equal(cg, co) and np.array_equal(dg, do)
+    if c.n < 20000:
+        cb, db = o.neighbor_stats(1, brute=True)
+        assert np.array_equal(cg, cb) and np.array_equal(dg, db)
+    say(f"  neighbours: mean {cg.mean():.1f} max {cg.max()}")
+
+
+@pytest.mark.parametrize("name", ["2d", "2d-mc-linear", "3d-small", "3d-mid"])
+def test_boundary_and_governing_within_tolerance(name):
+    c, s, o = pair(name)
+    for x in (s, o):
+        x.hash(0); x.sort_index(); x.sort_data(0, 1)
+    # wall pass + gradient pass
+    s.boundary(1)
+    o.wall(1)
+    typ = o.get(A_TYPE, 1)
+    wall = typ != FLUID
+    gp, op = s.get(A_POSR, 1), o.get(A_POSR, 1)
+    gv, ov = s.get(A_VELP, 1), o.get(A_VELP, 1)
+    ps = p_scale(c)
+    assert_close_scaled(gv[wall, 3], ov[wall, 3], np.maximum(np.abs(ov[wall, 3]), ps), "wall p")
+    assert_close_scaled(gp[wall, 3], op[wall, 3], np.abs(op[wall, 3]), "wall rho")
+    # identical inputs for the gradient pass: the GPU's wall values
+    o.set(A_POSR, gp, 1); o.set(A_VELP, gv, 1)
+    o.ldg(1)
+    gg, og = s.get(A_GD), o.get(A_GD)
+    sc = o.get(A_GD_SCALE)
+    f = typ == FLUID
+    if c.params.diffusion == z.DIFF_ANTUONO:
+        # G = L^-1 (sum): scale by the summed magnitude times |L^-1| ~ the G magnitude
+        gsc = np.maximum(sc[:, 0:1], np.abs(og[:, :3]).max(axis=1, keepdims=True))
+        assert_close_scaled(gg[f, :3], og[f, :3], np.broadcast_to(gsc[f], gg[f, :3].shape), "grad rho^L",
+                            rel=2e-5)
+    assert_close_scaled(gg[f, 3], og[f, 3], sc[f, 1], "div v")
+    # governing on identical inputs (the GPU's gradient pass output)
+    o.set(A_GD, gg)
+    s.governing(1)
+    o.governing(1)
+    ga, oa = s.get(A_ACC), o.get(A_ACC)
+    asc = o.get(A_ACC_SCALE)
+    err_a = assert_close_scaled(ga[f, :3], oa[f, :3], np.broadcast_to(asc[f, 0:1], ga[f, :3].shape), "acc")
+    err_d = assert_close_scaled(ga[f, 3], oa[f, 3], asc[f, 1], "drho/dt")
+    say(f"  max scaled err acc {err_a:.2e} drho {err_d:.2e}")
+    assert np.all(ga[typ == WALL] == 0.0)
+
+
+@pytest.mark.parametrize("name", ["2d", "3d-small"])
+def test_predictor_corrector(name):
+    c, s, o = pair(name)
+    for x in (s, o):
+        x.hash(0); x.sort_index(); x.sort_data(0, 1); x.boundary(1)
+    o.set(A_POSR, s.get(A_POSR, 1), 1); o.set(A_VELP, s.get(A_VELP, 1), 1); o.set(A_GD, s.get(A_GD))
+    s.governing(1)
+    acc = s.get(A_ACC)
+    o.set(A_ACC, acc)
+    s.prediction(1); o.prediction(1)
+    typ = o.get(A_TYPE, 1)
+    f = typ == FLUID
+    dx = c.json.dx
+    for a, what in ((A_POSR, "pred x,rho"), (A_VELP, "pred v,p")):
+        g, r = s.get(a, 1), o.get(a, 1)
+        if a == A_POSR:
+            assert_close_scaled(g[f, :3], r[f, :3], np.abs(r[f, :3]) + dx, what)
+            assert_close_scaled(g[f, 3], r[f, 3], np.abs(r[f, 3]), what)
+        else:
+            vs = np.abs(r[f, :3]) + np.abs(acc[f, :3]) * c.params.half_dt + 1e-3
+            assert_close_scaled(g[f, :3], r[f, :3], vs, what)
+            assert_close_scaled(g[f, 3], r[f, 3], np.maximum(np.abs(r[f, 3]), p_scale(c)), what)
+    assert np.array_equal(s.get(A_TPOSR, 1)[f], o.get(A_TPOSR, 1)[f].astype(np.float32))
+    # corrector from the same state
+    o.set(A_POSR, s.get(A_POSR, 1), 1); o.set(A_VELP, s.get(A_VELP, 1), 1)
+    s.correction(1); o.correction(1)
+    g, r = s.get(A_POSR, 1), o.get(A_POSR, 1)
+    assert_close_scaled(g[f, :3], r[f, :3], np.abs(r[f, :3]) + dx, "corr x")
+    assert_close_scaled(g[f, 3], r[f, 3], np.abs(r[f, 3]), "corr rho")
+    # dummies never move
+    w = ~f
+    assert np.array_equal(s.get(A_POSR, 1)[w, :3], o.get(A_POSR, 1)[w, :3].astype(np.float32))
+
+
+def by_original_id(h):
+    """Host arrays reordered by original particle id (table + index)."""
+    oid = h.table.astype(np.int64) + np.arange(h.n)
+    order = np.argsort(oid)
+    assert np.array_equal(oid[order], np.arange(h.n))
+    return order
+
+
+@pytest.mark.parametrize("name", ["2d", "3d-small", "3d-mid"])
+def test_full_step_matches_oracle(name):
+    c, s, o = pair(name)
+    s.step(1)
+    s.sync()
+    o.step(1)
+    g = s.download(0)
+    r = o.download(0)
+    og, orr = by_original_id(g), by_original_id(r)
+    f = r.type[orr] == FLUID
+    assert np.array_equal(g.type[og], r.type[orr])
+    rho_g, rho_r = g.rhop[og, 0][f], r.rhop[orr, 0][f]
+    assert_close_scaled(rho_g, rho_r, np.abs(rho_r), "rho after one step")
+    assert_close_scaled(g.rhop[og, 1][f], r.rhop[orr, 1][f], np.maximum(np.abs(r.rhop[orr, 1][f]), p_scale(c)),
+                        "p after one step")
+    assert_close_scaled(g.pos[og][f], r.pos[orr][f], np.abs(r.pos[orr][f]) + c.json.dx, "x after one step")
+    asc = o.get(A_ACC_SCALE)  # buffer-0 order, like the downloaded arrays
+    sc = asc[orr, 0][f]
+    assert_close_scaled(g.acc[og, :3][f], r.acc[orr, :3][f], np.broadcast_to(sc[:, None], (f.sum(), 3)),
+                        "acc after one step", rel=3e-5)
+    assert s.killed() == o.killed() == 0
+
+
+def test_removed_particles_match():
+    c = CASES["3d-small"]()
+    c.host.pos[3] = (50.0, 0.1, 0.1)
+    c.host.pos[11] = (0.1, 0.1, -5.0)
+    say(f"[case removed] N={c.n}")
+    s = z.Solver(c); s.upload(c.host)
+    o = oracle.Oracle(c.params); o.upload(c.host)
+    s.hash(0); o.hash(0)
+    assert s.killed() == o.killed() == 2
+    assert np.array_equal(s.get(A_TYPE, 0), o.get(A_TYPE, 0).astype(np.int32))
+    assert np.array_equal(s.get(A_POSR, 0), o.get(A_POSR, 0).astype(np.float32))
+    assert np.array_equal(s.get(A_KEY), o.get(A_KEY).astype(np.uint32))
+
+
+def test_graph_replay_is_deterministic_and_equals_stages():
+    c = CASES["3d-small"]()
+    say(f"[case determinism] N={c.n}")
+    runs = []
+    for mode in ("graph", "graph", "stages"):
+        s = z.Solver(c)
+        s.upload(c.host)
+        if mode == "graph":
+            s.step(3)
+        else:
+            for _ in range(3):
+                s.hash(0); s.sort_index(); s.sort_data(0, 1); s.boundary(1); s.governing(1); s.prediction(1)
+                s.hash(1); s.sort_index(); s.sort_data(1, 0); s.boundary(0); s.governing(0); s.correction(0)
+        s.sync()
+        runs.append((s.get(A_POSR, 0), s.get(A_VELP, 0), s.get(A_ACC)))
+        s.close()
+    for a, b in zip(runs[0], runs[1]):
+        assert np.array_equal(a, b)
+    for a, b in zip(runs[0], runs[2]):
+        assert np.array_equal(a, b)
+
+
+def test_out_of_order_calls_are_errors():
+    c = CASES["2d"]()
+    s = z.Solver(c)
+    s.upload(c.host)
+    with pytest.raises(z.ZsphError):
+        s.sort_index()
+    with pytest.raises(z.ZsphError):
+        s.boundary(1)
+    s.hash(0)
+    with pytest.raises(z.ZsphError):
+        s.sort_data(0, 0)

@@ -1,0 +1,247 @@
+"""zsph_amd: MI355X (gfx950) delta-SPH step behind the reference's host interface.
+
+Python host mirror of the reference driver's entry points
+(single_gpu_sph_src/deltasph/main.cu:58-90, header/lib.cuh:7-36,
+header/equation.cuh:8-17) over the C ABI of ``libzsph.so`` (include/zsph.h).
+Every compute call runs the hand-written HIP kernels of the library; there is
+no CPU fallback: without the built library or a GPU the calls raise.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+from . import _abi as abi
+from ._abi import (A_ACC, A_CELL_START, A_GD, A_INDEX, A_KEY, A_POSR, A_TABLE, A_TPOSR, A_TVELP,
+                   A_TYPE, A_VELP, BODY, DIFF_ANTUONO, DIFF_MC, F_ACC, F_ALL, F_HASH, F_POS, F_RHOP,
+                   F_TABLE, F_TYPE, F_VEL, FLUID, KILLED, PISTON, WALL)
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libzsph.so")
+_LIB = None
+
+
+def lib():
+    """The loaded libzsph.so (raises if it has not been built)."""
+    global _LIB
+    if _LIB is None:
+        _LIB = abi.load(LIB_PATH)
+    return _LIB
+
+
+class ZsphError(RuntimeError):
+    def __init__(self, code, msg=""):
+        super().__init__(f"zsph error {code} ({abi.ERRORS.get(code, 'unknown')}){': ' + msg if msg else ''}")
+        self.code = code
+
+
+def _ptr(a, ctype):
+    return a.ctypes.data_as(C.POINTER(ctype)) if a is not None else None
+
+
+class HostArrays:
+    """Host particle arrays in the reference's cpu_output_t layout (data_structure.cuh:146-156)."""
+
+    def __init__(self, n):
+        self.n = n
+        self.pos = np.zeros((n, 3), np.float32)
+        self.vel = np.zeros((n, 3), np.float32)
+        self.rhop = np.zeros((n, 2), np.float32)
+        self.acc = np.zeros((n, 4), np.float32)
+        self.type = np.zeros(n, np.int32)
+        self.table = np.zeros(n, np.int32)
+        self.hash = np.zeros(n, np.int32)
+
+    def state(self):
+        return abi.HostState(_ptr(self.pos, C.c_float), _ptr(self.vel, C.c_float), _ptr(self.rhop, C.c_float),
+                             _ptr(self.acc, C.c_float), _ptr(self.type, C.c_int), _ptr(self.table, C.c_int),
+                             _ptr(self.hash, C.c_int))
+
+    def copy(self):
+        h = HostArrays(self.n)
+        for k in ("pos", "vel", "rhop", "acc", "type", "table", "hash"):
+            getattr(h, k)[...] = getattr(self, k)
+        return h
+
+
+class Case:
+    """Case parameters (cpu_json_t -> cpu_param_t -> gpu_param_t, lib/init.cu:153-251) and initial particles."""
+
+    def __init__(self, json, host):
+        self.json = json
+        self.host = host
+        self.cpu = abi.CpuParam()
+        self.params = abi.Params()
+        self.derive()
+
+    def derive(self):
+        rc = lib().zsph_set_cpu_param(C.byref(self.cpu), C.byref(self.json))
+        if rc != abi.OK:
+            raise ZsphError(rc, "zsph_set_cpu_param")
+        lib().zsph_set_gpu_param(C.byref(self.params), C.byref(self.cpu))
+        return self
+
+    @property
+    def n(self):
+        return self.host.n
+
+    @staticmethod
+    def defaults():
+        j = abi.Json()
+        lib().zsph_json_defaults(C.byref(j))
+        return j
+
+    @classmethod
+    def dambreak(cls, dim, dx, lx, wy, lz, cw, ch, layers=3, wall_height=0.0, gamma=7.0, cfl=0.2,
+                 diffusion=DIFF_ANTUONO, **overrides):
+        """Lattice dam break (zsph_case_dambreak).  Extra keyword arguments set zsph_json fields."""
+        j = cls.defaults()
+        j.gamma, j.cfl, j.diffusion = gamma, cfl, diffusion
+        for k, v in overrides.items():
+            setattr(j, k, v)
+        d = abi.DamBreak(dim, dx, lx, wy, lz, cw, ch, layers, wall_height)
+        n = C.c_int(0)
+        rc = lib().zsph_case_dambreak(C.byref(d), C.byref(j), None, C.byref(n))
+        if rc != abi.OK:
+            raise ZsphError(rc, "zsph_case_dambreak")
+        host = HostArrays(n.value)
+        st = host.state()
+        rc = lib().zsph_case_dambreak(C.byref(d), C.byref(j), C.byref(st), C.byref(n))
+        if rc != abi.OK:
+            raise ZsphError(rc, "zsph_case_dambreak")
+        return cls(j, host)
+
+    @classmethod
+    def from_json(cls, path):
+        """Case file + <input_path>/input.vtk, like the reference's main (deltasph/main.cu:13-20)."""
+        j = abi.Json()
+        rc = lib().zsph_read_json(path.encode(), C.byref(j))
+        if rc != abi.OK:
+            raise ZsphError(rc, f"zsph_read_json({path})")
+        host = HostArrays(j.ptc_num)
+        vtk = os.path.join(j.input_path.decode(), "input.vtk")
+        st = host.state()
+        rc = lib().zsph_read_vtk(vtk.encode(), C.byref(st), j.ptc_num, j.rho0)
+        if rc != abi.OK:
+            raise ZsphError(rc, f"zsph_read_vtk({vtk})")
+        return cls(j, host)
+
+
+_ARRAY_SHAPES = {A_POSR: (4, np.float32), A_VELP: (4, np.float32), A_TPOSR: (4, np.float32),
+                 A_TVELP: (4, np.float32), A_TYPE: (1, np.int32), A_TABLE: (1, np.int32),
+                 A_KEY: (1, np.uint32), A_INDEX: (1, np.int32), A_ACC: (4, np.float32),
+                 A_GD: (4, np.float32)}
+
+
+class Solver:
+    """One device context (zsph_ctx): the reference's device buffers and kernels."""
+
+    def __init__(self, case, device=0):
+        self.case = case
+        self.params = case.params if isinstance(case, Case) else case
+        self.n = self.params.ptc_num
+        self.grid_num = self.params.grid_xdim * self.params.grid_ydim * self.params.grid_zdim
+        self._ctx = C.c_void_p()
+        rc = lib().zsph_create(C.byref(self.params), device, C.byref(self._ctx))
+        if rc != abi.OK:
+            raise ZsphError(rc, "zsph_create")
+
+    def _chk(self, rc, what):
+        if rc != abi.OK:
+            raise ZsphError(rc, f"{what}: {lib().zsph_last_error(self._ctx).decode()}")
+
+    def close(self):
+        if self._ctx:
+            lib().zsph_destroy(self._ctx)
+            self._ctx = C.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # lib/init.cu:136-151 / lib/gmemcpy.cu:3-36
+    def upload(self, host=None, buf=0):
+        host = host if host is not None else self.case.host
+        st = host.state()
+        self._chk(lib().zsph_upload(self._ctx, buf, C.byref(st)), "zsph_upload")
+
+    def download(self, buf=0, fields=F_ALL):
+        h = HostArrays(self.n)
+        st = h.state()
+        self._chk(lib().zsph_download(self._ctx, buf, C.byref(st), fields), "zsph_download")
+        return h
+
+    # the reference's per-stage entry points
+    def hash(self, buf=0):
+        self._chk(lib().zsph_hash(self._ctx, buf), "zsph_hash")
+
+    def sort_index(self):
+        self._chk(lib().zsph_sort_index(self._ctx), "zsph_sort_index")
+
+    def sort_data(self, src=0, dst=1):
+        self._chk(lib().zsph_sort_data(self._ctx, src, dst), "zsph_sort_data")
+
+    def boundary(self, buf=1):
+        self._chk(lib().zsph_boundary(self._ctx, buf), "zsph_boundary")
+
+    def governing(self, buf=1):
+        self._chk(lib().zsph_governing(self._ctx, buf), "zsph_governing")
+
+    def prediction(self, buf=1):
+        self._chk(lib().zsph_prediction(self._ctx, buf), "zsph_prediction")
+
+    def correction(self, buf=0):
+        self._chk(lib().zsph_correction(self._ctx, buf), "zsph_correction")
+
+    def step(self, nsteps=1):
+        self._chk(lib().zsph_step(self._ctx, nsteps), "zsph_step")
+
+    def sync(self):
+        self._chk(lib().zsph_sync(self._ctx), "zsph_sync")
+
+    # raw arrays
+    def get(self, which, buf=0):
+        if which == A_CELL_START:
+            out = np.empty(self.grid_num + 1, np.int32)
+        else:
+            w, dt = _ARRAY_SHAPES[which]
+            out = np.empty((self.n, w) if w > 1 else self.n, dt)
+        self._chk(lib().zsph_array_get(self._ctx, which, buf, out.ctypes.data, out.nbytes), "zsph_array_get")
+        return out
+
+    def set(self, which, arr, buf=0):
+        w, dt = _ARRAY_SHAPES[which]
+        a = np.ascontiguousarray(arr, dtype=dt)
+        self._chk(lib().zsph_array_set(self._ctx, which, buf, a.ctypes.data, a.nbytes), "zsph_array_set")
+
+    def neighbor_stats(self, buf=1):
+        cnt = np.empty(self.n, np.int32)
+        dig = np.empty(self.n, np.uint64)
+        self._chk(lib().zsph_neighbor_stats(self._ctx, buf, _ptr(cnt, C.c_int), _ptr(dig, C.c_uint64)),
+                  "zsph_neighbor_stats")
+        return cnt, dig
+
+    def time_steps(self, nsteps):
+        """Milliseconds of `nsteps` graph-replayed steps, between two HIP events on the solver's stream."""
+        ms = C.c_float(0.0)
+        self._chk(lib().zsph_time_steps(self._ctx, nsteps, C.byref(ms)), "zsph_time_steps")
+        return ms.value
+
+    def profile_steps(self, nsteps):
+        """Per-stage milliseconds (dict over abi.STAGES) of `nsteps` steps issued stage by stage."""
+        out = (C.c_float * len(abi.STAGES))()
+        self._chk(lib().zsph_profile_steps(self._ctx, nsteps, out), "zsph_profile_steps")
+        return dict(zip(abi.STAGES, list(out)))
+
+    def killed(self):
+        k = C.c_int(0)
+        self._chk(lib().zsph_killed(self._ctx, C.byref(k)), "zsph_killed")
+        return k.value

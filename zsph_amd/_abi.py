@@ -1,0 +1,138 @@
+"""ctypes mirror of include/zsph.h (the C ABI of libzsph.so).
+
+Struct layouts are checked against the library's own ``zsph_sizeof`` at
+load time, so a header change that is not mirrored here fails loudly.
+"""
+import ctypes as C
+import os
+
+PATH_MAX = 256
+
+OK = 0
+E_ARG, E_HIP, E_NOMEM, E_DT, E_IO, E_STATE, E_NODEV = -1, -2, -3, -4, -5, -6, -7
+ERRORS = {E_ARG: "invalid argument", E_HIP: "HIP runtime error", E_NOMEM: "out of memory",
+          E_DT: "dt above the stability bound", E_IO: "I/O error", E_STATE: "call out of order",
+          E_NODEV: "no HIP device"}
+
+FLUID, WALL, BODY, PISTON, KILLED = 1, 0, 2, 3, 100
+DIFF_ANTUONO, DIFF_MC = 0, 1
+
+F_POS, F_VEL, F_RHOP, F_ACC, F_TYPE, F_TABLE, F_HASH = 1, 2, 4, 8, 16, 32, 64
+F_ALL = 127
+
+A_POSR, A_VELP, A_TPOSR, A_TVELP, A_TYPE, A_TABLE = 0, 1, 2, 3, 4, 5
+A_KEY, A_INDEX, A_CELL_START, A_ACC, A_GD = 6, 7, 8, 9, 10
+
+_f, _i = C.c_float, C.c_int
+
+
+class Json(C.Structure):
+    _fields_ = [(n, _f) for n in ("g", "rho0", "dx", "h_factor", "r_factor", "eta_factor",
+                                  "cs_factor", "delta", "alpha")] + \
+               [(n, _f) for n in ("xmin", "xmax", "ymin", "ymax", "zmin", "zmax")] + \
+               [("grid_layer_factor", _f), ("grid_size_factor", _f), ("dt", _f)] + \
+               [(n, _i) for n in ("start_step", "current_step", "end_step", "output_step",
+                                  "ptc_num", "water_ptc_num", "air_ptc_num", "wall_ptc_num",
+                                  "rigid_ptc_num", "thread_num", "gpu_id")] + \
+               [("input_path", C.c_char * PATH_MAX), ("output_path", C.c_char * PATH_MAX),
+                ("git_hash", C.c_char * 64)] + \
+               [("dim", _i), ("gamma", _f), ("c0", _f), ("cfl", _f), ("diffusion", _i),
+                ("output_fields", _i), ("binary", _i)]
+
+
+class CpuParam(C.Structure):
+    _fields_ = [(n, _f) for n in ("g", "m", "rho0", "dx", "h", "r", "eta", "cs", "adh", "delta",
+                                  "alpha", "dt")] + \
+               [(n, _i) for n in ("start_step", "current_step", "end_step", "output_step")] + \
+               [(n, _f) for n in ("xmin", "xmax", "ymin", "ymax", "zmin", "zmax")] + \
+               [("grid_factor", _i), ("grid_size", _f)] + \
+               [(n, _f) for n in ("grid_xmin", "grid_ymin", "grid_zmin", "grid_xmax", "grid_ymax",
+                                  "grid_zmax")] + \
+               [(n, _i) for n in ("grid_xdim", "grid_ydim", "grid_zdim", "grid_num", "grid_hash_min",
+                                  "grid_hash_max", "ptc_num", "water_ptc_num", "air_ptc_num",
+                                  "wall_ptc_num", "rigid_ptc_num", "thread_num", "gpu_id")] + \
+               [("input_path", C.c_char * PATH_MAX), ("output_path", C.c_char * PATH_MAX),
+                ("git_hash", C.c_char * 64)] + \
+               [("dim", _i), ("gamma", _f), ("diffusion", _i), ("output_fields", _i), ("binary", _i)]
+
+
+class Params(C.Structure):
+    _fields_ = [(n, _f) for n in ("half_dt", "h", "g", "m", "rho0", "adh", "h2", "eta2", "cs2",
+                                  "alpha", "delta_h_cs", "h_cs_rho_m", "grid_size")] + \
+               [("grid_factor", _i)] + \
+               [(n, _f) for n in ("grid_xmin", "grid_ymin", "grid_zmin")] + \
+               [(n, _i) for n in ("grid_xdim", "grid_ydim", "grid_zdim", "grid_hash_min",
+                                  "grid_hash_max", "ptc_num", "dim")] + \
+               [("gamma", _f), ("diffusion", _i), ("rc2", _f)]
+
+
+class HostState(C.Structure):
+    _fields_ = [("pos", C.POINTER(_f)), ("vel", C.POINTER(_f)), ("rhop", C.POINTER(_f)),
+                ("acc_drhodt", C.POINTER(_f)), ("type", C.POINTER(_i)), ("table", C.POINTER(_i)),
+                ("hash", C.POINTER(_i))]
+
+
+class DamBreak(C.Structure):
+    _fields_ = [("dim", _i), ("dx", _f), ("lx", _f), ("wy", _f), ("lz", _f), ("cw", _f), ("ch", _f),
+                ("layers", _i), ("wall_height", _f)]
+
+
+def _sig(lib, name, res, args):
+    fn = getattr(lib, name)
+    fn.restype = res
+    fn.argtypes = args
+    return fn
+
+
+def load(path):
+    """Load libzsph.so at `path` and declare every exported entry point."""
+    if not os.path.exists(path):
+        raise RuntimeError(
+            f"libzsph.so not found at {path}: build it first (python -c 'import __graft_entry__ as g; g.build()')")
+    lib = C.CDLL(path)
+    P = C.POINTER
+    vp = C.c_void_p
+    _sig(lib, "zsph_sizeof", C.c_size_t, [_i])
+    for which, cls in enumerate((Json, CpuParam, Params, HostState, DamBreak)):
+        got = lib.zsph_sizeof(which)
+        if got != C.sizeof(cls):
+            raise RuntimeError(f"ABI mismatch: sizeof({cls.__name__}) = {C.sizeof(cls)} here, {got} in {path}")
+    _sig(lib, "zsph_json_defaults", None, [P(Json)])
+    _sig(lib, "zsph_read_json", _i, [C.c_char_p, P(Json)])
+    _sig(lib, "zsph_set_cpu_param", _i, [P(CpuParam), P(Json)])
+    _sig(lib, "zsph_set_gpu_param", None, [P(Params), P(CpuParam)])
+    _sig(lib, "zsph_case_dambreak", _i, [P(DamBreak), P(Json), P(HostState), P(_i)])
+    _sig(lib, "zsph_write_vtk", _i, [C.c_char_p, P(HostState), _i, _i, _i])
+    _sig(lib, "zsph_read_vtk", _i, [C.c_char_p, P(HostState), _i, _f])
+    _sig(lib, "zsph_vtk_count", _i, [C.c_char_p, P(_i)])
+    _sig(lib, "zsph_create", _i, [P(Params), _i, P(vp)])
+    _sig(lib, "zsph_destroy", None, [vp])
+    _sig(lib, "zsph_sync", _i, [vp])
+    _sig(lib, "zsph_last_error", C.c_char_p, [vp])
+    _sig(lib, "zsph_upload", _i, [vp, _i, P(HostState)])
+    _sig(lib, "zsph_download", _i, [vp, _i, P(HostState), _i])
+    for name in ("zsph_hash", "zsph_boundary", "zsph_governing", "zsph_prediction", "zsph_correction",
+                 "zsph_step"):
+        _sig(lib, name, _i, [vp, _i])
+    _sig(lib, "zsph_sort_index", _i, [vp])
+    _sig(lib, "zsph_sort_data", _i, [vp, _i, _i])
+    _sig(lib, "zsph_array_get", _i, [vp, _i, _i, vp, C.c_size_t])
+    _sig(lib, "zsph_array_set", _i, [vp, _i, _i, vp, C.c_size_t])
+    _sig(lib, "zsph_neighbor_stats", _i, [vp, _i, P(_i), P(C.c_uint64)])
+    _sig(lib, "zsph_killed", _i, [vp, P(_i)])
+    _sig(lib, "zsph_time_steps", _i, [vp, _i, P(_f)])
+    _sig(lib, "zsph_profile_steps", _i, [vp, _i, P(_f)])
+    return lib
+
+
+# Every symbol include/zsph.h declares (checked by the CPU test suite).
+EXPORTS = (
+    "zsph_json_defaults", "zsph_read_json", "zsph_set_cpu_param", "zsph_set_gpu_param",
+    "zsph_case_dambreak", "zsph_write_vtk", "zsph_read_vtk", "zsph_vtk_count", "zsph_create",
+    "zsph_destroy", "zsph_sync", "zsph_last_error", "zsph_upload", "zsph_download", "zsph_hash",
+    "zsph_sort_index", "zsph_sort_data", "zsph_boundary", "zsph_governing", "zsph_prediction",
+    "zsph_correction", "zsph_step", "zsph_array_get", "zsph_array_set", "zsph_neighbor_stats",
+    "zsph_killed", "zsph_sizeof", "zsph_time_steps", "zsph_profile_steps",
+)
+
+STAGES = ("hash", "sort", "cells", "reorder", "wall", "ldg", "governing", "integrate")

@@ -1,0 +1,211 @@
+// zsph_run: the host driver (single_gpu_sph_src/deltasph/main.cu:8-151) over
+// the C ABI.  Reads a case file (io/read_json.cu keys) and its
+// <input_path>/input.vtk, or generates a dam break; steps on the GPU and
+// hands output frames to a pool of writer threads (io/mul_thread.cu:3-80),
+// here a blocking queue that is drained and joined before exit.
+//
+//   zsph_run case.json
+//   zsph_run --dambreak {2,3} [--dx DX] [--steps N] [--output-step K] [--out DIR] [--gamma G]
+#include <chrono>
+#include <condition_variable>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <deque>
+#include <mutex>
+#include <string>
+#include <thread>
+#include <vector>
+
+#include "zsph.h"
+
+namespace {
+
+struct Frame {
+    int step = 0;
+    int n = 0;
+    std::vector<float> pos, vel, rhop, acc;
+    std::vector<int> type, table, hash;
+};
+
+class Writers {
+  public:
+    Writers(int nthreads, std::string dir, int fields, int binary)
+        : dir_(std::move(dir)), fields_(fields), binary_(binary) {
+        for (int t = 0; t < (nthreads < 1 ? 1 : nthreads); ++t) pool_.emplace_back([this] { run(); });
+    }
+    ~Writers() {
+        {
+            std::lock_guard<std::mutex> g(m_);
+            done_ = true;
+        }
+        cv_.notify_all();
+        for (auto &t : pool_) t.join();
+    }
+    void push(Frame &&f) {
+        {
+            std::unique_lock<std::mutex> g(m_);
+            room_.wait(g, [this] { return q_.size() < pool_.size() + 1; });
+            q_.push_back(std::move(f));
+        }
+        cv_.notify_one();
+    }
+    int errors() const { return errors_; }
+
+  private:
+    void run() {
+        for (;;) {
+            Frame f;
+            {
+                std::unique_lock<std::mutex> g(m_);
+                cv_.wait(g, [this] { return done_ || !q_.empty(); });
+                if (q_.empty()) return;
+                f = std::move(q_.front());
+                q_.pop_front();
+            }
+            room_.notify_one();
+            zsph_host_state st{f.pos.data(), f.vel.data(), f.rhop.data(), f.acc.data(),
+                               f.type.data(), f.table.data(), f.hash.data()};
+            const std::string path = dir_ + "/zsph" + std::to_string(f.step) + ".vtk";
+            if (zsph_write_vtk(path.c_str(), &st, f.n, fields_, binary_) != ZSPH_OK) {
+                std::lock_guard<std::mutex> g(m_);
+                ++errors_;
+            }
+        }
+    }
+    std::string dir_;
+    int fields_, binary_;
+    std::vector<std::thread> pool_;
+    std::deque<Frame> q_;
+    std::mutex m_;
+    std::condition_variable cv_, room_;
+    bool done_ = false;
+    int errors_ = 0;
+};
+
+int usage() {
+    std::fprintf(stderr,
+                 "usage: zsph_run case.json\n"
+                 "       zsph_run --dambreak {2,3} [--dx DX] [--steps N] [--output-step K] [--out DIR] [--gamma G]\n");
+    return 2;
+}
+
+}  // namespace
+
+int main(int argc, char **argv) {
+    if (argc < 2) return usage();
+    zsph_json j;
+    zsph_json_defaults(&j);
+    std::vector<float> pos, vel, rhop, acc;
+    std::vector<int> type, table;
+    int n = 0;
+    if (std::strcmp(argv[1], "--dambreak") == 0) {
+        if (argc < 3) return usage();
+        zsph_dambreak c{};
+        c.dim = std::atoi(argv[2]);
+        c.layers = 3;
+        if (c.dim == 2) {
+            c.dx = 0.005f; c.lx = 1.61f; c.wy = 0.0f; c.lz = 0.8f; c.cw = 0.4f; c.ch = 0.3f;
+        } else {
+            c.dx = 0.008f; c.lx = 3.22f; c.wy = 1.0f; c.lz = 1.0f; c.cw = 1.2f; c.ch = 0.55f;
+        }
+        j.cfl = 0.2f;
+        j.gamma = 7.0f;
+        j.end_step = 100;
+        j.output_step = 50;
+        for (int a = 3; a + 1 < argc; a += 2) {
+            const std::string k = argv[a];
+            const char *v = argv[a + 1];
+            if (k == "--dx") c.dx = (float)std::atof(v);
+            else if (k == "--steps") j.end_step = std::atoi(v);
+            else if (k == "--output-step") j.output_step = std::atoi(v);
+            else if (k == "--out") std::snprintf(j.output_path, sizeof(j.output_path), "%s", v);
+            else if (k == "--gamma") j.gamma = (float)std::atof(v);
+            else return usage();
+        }
+        if (zsph_case_dambreak(&c, &j, nullptr, &n) != ZSPH_OK) return 1;
+        pos.resize(3 * (size_t)n); vel.resize(3 * (size_t)n); rhop.resize(2 * (size_t)n);
+        acc.resize(4 * (size_t)n); type.resize(n); table.resize(n);
+        zsph_host_state st{pos.data(), vel.data(), rhop.data(), acc.data(), type.data(), table.data(), nullptr};
+        if (zsph_case_dambreak(&c, &j, &st, &n) != ZSPH_OK) return 1;
+    } else {
+        if (zsph_read_json(argv[1], &j) != ZSPH_OK) {
+            std::fprintf(stderr, "cannot read %s\n", argv[1]);
+            return 1;
+        }
+        n = j.ptc_num;
+        pos.resize(3 * (size_t)n); vel.resize(3 * (size_t)n); rhop.resize(2 * (size_t)n);
+        acc.resize(4 * (size_t)n); type.resize(n); table.resize(n);
+        zsph_host_state st{pos.data(), vel.data(), rhop.data(), acc.data(), type.data(), table.data(), nullptr};
+        const std::string vtk = std::string(j.input_path) + "/input.vtk";
+        if (zsph_read_vtk(vtk.c_str(), &st, n, j.rho0) != ZSPH_OK) {
+            std::fprintf(stderr, "cannot read %s (ptc_num %d)\n", vtk.c_str(), n);
+            return 1;
+        }
+    }
+    zsph_cpu_param cp;
+    int r = zsph_set_cpu_param(&cp, &j);
+    if (r == ZSPH_E_DT) {
+        std::fprintf(stderr, "* Dt is greater than the dt_max\n");
+        return 1;
+    }
+    if (r != ZSPH_OK) {
+        std::fprintf(stderr, "invalid case parameters (%d)\n", r);
+        return 1;
+    }
+    zsph_params gp;
+    zsph_set_gpu_param(&gp, &cp);
+    std::printf("* ptc_num %d  grid %d x %d x %d  dt %.4g  h %.4g  c0 %.4g\n", cp.ptc_num, cp.grid_xdim,
+                cp.grid_ydim, cp.grid_zdim, cp.dt, cp.h, cp.cs);
+    zsph_ctx *ctx = nullptr;
+    if ((r = zsph_create(&gp, cp.gpu_id, &ctx)) != ZSPH_OK) {
+        std::fprintf(stderr, "zsph_create failed (%d)\n", r);
+        return 1;
+    }
+    zsph_host_state st{pos.data(), vel.data(), rhop.data(), acc.data(), type.data(), table.data(), nullptr};
+    if (zsph_upload(ctx, 0, &st) != ZSPH_OK) {
+        std::fprintf(stderr, "%s\n", zsph_last_error(ctx));
+        zsph_destroy(ctx);
+        return 1;
+    }
+    int status = 0;
+    {
+        Writers w(cp.thread_num, cp.output_path, cp.output_fields | ZSPH_F_POS, cp.binary);
+        auto t0 = std::chrono::steady_clock::now();
+        for (int step = cp.start_step; step < cp.end_step; ++step) {
+            if (zsph_step(ctx, 1) != ZSPH_OK) {
+                std::fprintf(stderr, "step %d: %s\n", step, zsph_last_error(ctx));
+                status = 1;
+                break;
+            }
+            if (cp.output_step > 0 && step % cp.output_step == 0) {
+                Frame f;
+                f.step = step;
+                f.n = n;
+                f.pos.resize(3 * (size_t)n); f.vel.resize(3 * (size_t)n); f.rhop.resize(2 * (size_t)n);
+                f.acc.resize(4 * (size_t)n); f.type.resize(n); f.table.resize(n); f.hash.resize(n);
+                zsph_host_state fs{f.pos.data(), f.vel.data(), f.rhop.data(), f.acc.data(),
+                                   f.type.data(), f.table.data(), f.hash.data()};
+                if (zsph_download(ctx, 0, &fs, ZSPH_F_ALL) != ZSPH_OK) {
+                    std::fprintf(stderr, "download: %s\n", zsph_last_error(ctx));
+                    status = 1;
+                    break;
+                }
+                const double el = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+                std::printf("step %d  t=%.5f s  wall %.2f s\n", step, (step + 1) * cp.dt, el);
+                std::fflush(stdout);
+                w.push(std::move(f));
+            }
+        }
+        if (zsph_sync(ctx) != ZSPH_OK) {
+            std::fprintf(stderr, "%s\n", zsph_last_error(ctx));
+            status = 1;
+        }
+        if (w.errors()) status = 1;
+    }
+    int killed = 0;
+    zsph_killed(ctx, &killed);
+    std::printf("* done, removed particles: %d\n", killed);
+    zsph_destroy(ctx);
+    return status;
+}

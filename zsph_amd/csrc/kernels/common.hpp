@@ -1,0 +1,157 @@
+// Device-side definitions shared by the gfx950 kernels of libzsph.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "zsph.h"
+
+namespace zs {
+
+// Kernel view of zsph_params plus derived constants, passed by value.
+struct Prm {
+    float half_dt, dt, h, inv_h, g, m, rho0, adh, h2, eta2, cs2, alpha, delta_h_cs, h_cs_rho_m;
+    float fk;        // -5 adh / h^2 : F(q) = fk (1 - q/2)^3
+    float rc2;       // neighbour iff r^2 <= rc2
+    float gamma, B;  // EOS: p = B ((rho/rho0)^gamma - 1), B = rho0 c^2 / gamma
+    float inv_gamma;
+    float grid_size;
+    float gx0, gy0, gz0;
+    int k;           // cells searched each side
+    int X, Y, Z, XY;
+    int G;           // number of cells
+    int N;
+    int dim;
+    int diffusion;
+};
+
+// Host-side construction (also used by the C ABI layer).
+inline Prm make_prm(const zsph_params &p) {
+    Prm q{};
+    q.half_dt = p.half_dt;
+    q.dt = p.half_dt * 2.0f;
+    q.h = p.h;
+    q.inv_h = 1.0f / p.h;
+    q.g = p.g;
+    q.m = p.m;
+    q.rho0 = p.rho0;
+    q.adh = p.adh;
+    q.h2 = p.h2;
+    q.eta2 = p.eta2;
+    q.cs2 = p.cs2;
+    q.alpha = p.alpha;
+    q.delta_h_cs = p.delta_h_cs;
+    q.h_cs_rho_m = p.h_cs_rho_m;
+    q.fk = -5.0f * p.adh / p.h2;
+    q.rc2 = p.rc2;
+    q.gamma = p.gamma;
+    q.B = p.rho0 * p.cs2 / p.gamma;
+    q.inv_gamma = 1.0f / p.gamma;
+    q.grid_size = p.grid_size;
+    q.gx0 = p.grid_xmin;
+    q.gy0 = p.grid_ymin;
+    q.gz0 = p.grid_zmin;
+    q.k = p.grid_factor;
+    q.X = p.grid_xdim;
+    q.Y = p.grid_ydim;
+    q.Z = p.grid_zdim;
+    q.XY = p.grid_xdim * p.grid_ydim;
+    q.G = p.grid_xdim * p.grid_ydim * p.grid_zdim;
+    q.N = p.ptc_num;
+    q.dim = p.dim;
+    q.diffusion = p.diffusion;
+    return q;
+}
+
+// One particle buffer (the reference's gpu_ptc_t, data_structure.cuh:158-168,
+// with (pos,rho) and (vel,p) packed into float4 for 16-byte coalesced loads).
+struct Buf {
+    float4 *posr;   // x, y, z, rho
+    float4 *velp;   // vx, vy, vz, p
+    float4 *tposr;  // saved x, rho at the start of the step
+    float4 *tvelp;  // saved v, p
+    int *type;
+    int *table;
+};
+
+#define ZS_BLOCK 256
+
+// Blocks are dispatched round-robin over the 8 XCDs; remap so each XCD
+// (own L2) works on one contiguous range of particles.
+__device__ __forceinline__ int xcd_block(int nblocks) {
+    const int b = blockIdx.x;
+    const int per = nblocks >> 3, rem = nblocks & 7;
+    const int x = b & 7, local = b >> 3;
+    return x * per + (x < rem ? x : rem) + local;
+}
+
+// Cell coordinate along one axis: floor((x - o) / size) with IEEE FP32
+// division; returns false outside [0, dim).  NaN positions fail.
+__host__ __device__ inline bool cell_axis(float x, float o, float size, int dim, int *c) {
+#ifdef __HIP_DEVICE_COMPILE__
+    const float t = floorf(__fdiv_rn(x - o, size));
+#else
+    const float t = floorf((x - o) / size);
+#endif
+    if (!(t >= 0.0f && t < (float)dim)) return false;
+    *c = (int)t;
+    return true;
+}
+
+// EOS p(rho).  gamma == 1 is the reference's linear form c^2 (rho - rho0)
+// (lib/time_int.cu:39-40); gamma == 7 uses the expanded (1+e)^7 - 1 so the
+// small-compression regime keeps full relative precision.
+__host__ __device__ inline float eos_p(const Prm &q, float rho) {
+    if (q.gamma == 1.0f) return q.cs2 * (rho - q.rho0);
+    const float e = (rho - q.rho0) / q.rho0;
+    if (q.gamma == 7.0f) {
+        const float poly = 7.0f + e * (21.0f + e * (35.0f + e * (35.0f + e * (21.0f + e * (7.0f + e)))));
+        return q.B * e * poly;
+    }
+    return q.B * expm1f(q.gamma * log1pf(e));
+}
+
+// Inverse EOS rho(p) for dummy particles (delta_ns.cu:129 for gamma == 1).
+__host__ __device__ inline float eos_rho(const Prm &q, float p) {
+    if (q.gamma == 1.0f) return p / q.cs2 + q.rho0;
+    return q.rho0 + q.rho0 * expm1f(q.inv_gamma * log1pf(p / q.B));
+}
+
+// r^2 in one fixed FP32 form (shared with the oracle): fma(dz,dz,fma(dy,dy,dx*dx)).
+__host__ __device__ inline float dist2(float dx, float dy, float dz) {
+    return __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx));
+}
+
+// Decode a linear cell index into the clamped search box.
+struct CellBox {
+    int xlo, xhi, ylo, yhi, zlo, zhi;
+};
+__device__ __forceinline__ CellBox cell_box(const Prm &q, uint32_t key) {
+    const int cz = (int)(key / (uint32_t)q.XY);
+    const int rem = (int)key - cz * q.XY;
+    const int cy = rem / q.X;
+    const int cx = rem - cy * q.X;
+    CellBox b;
+    b.xlo = max(cx - q.k, 0); b.xhi = min(cx + q.k, q.X - 1);
+    b.ylo = max(cy - q.k, 0); b.yhi = min(cy + q.k, q.Y - 1);
+    b.zlo = max(cz - q.k, 0); b.zhi = min(cz + q.k, q.Z - 1);
+    return b;
+}
+
+// Visit every candidate j of the (2k+1)^3 cell box in ascending sorted
+// order within each x-run; f(j) decides on r^2 itself.  The x-run of
+// cells [xlo, xhi] in one (y,z) row is one contiguous particle range.
+template <class F>
+__device__ __forceinline__ void for_each_candidate(const Prm &q, uint32_t key, const int *__restrict__ cs,
+                                                   F &&f) {
+    const CellBox b = cell_box(q, key);
+    for (int cz = b.zlo; cz <= b.zhi; ++cz) {
+        for (int cy = b.ylo; cy <= b.yhi; ++cy) {
+            const int row = cz * q.XY + cy * q.X;
+            const int jb = cs[row + b.xlo];
+            const int je = cs[row + b.xhi + 1];
+            for (int j = jb; j < je; ++j) f(j);
+        }
+    }
+}
+
+}  // namespace zs

@@ -1,0 +1,32 @@
+// Host launchers of the gfx950 kernels (one translation unit each).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "common.hpp"
+
+namespace zs {
+
+// nbr_build.hip
+void launch_hash(hipStream_t s, const Prm &q, const Buf &b, uint32_t *key, int *idx, int *killed);
+int radix_blocks(int n);
+size_t radix_hist_words(int n, int bits);
+// Sorts (key[0], val[0]) over the low `keybits` bits; returns the slot (0/1)
+// holding the result, -1 for an unsupported digit width.  identity_vals:
+// val[0] is known to be 0..n-1 and is not read.
+int launch_sort(hipStream_t s, int n, int keybits, int bits, uint32_t *key[2], int *val[2], uint32_t *hist,
+                bool identity_vals);
+void launch_cell_start(hipStream_t s, const uint32_t *skey, int n, int G, int *cs);
+void launch_reorder(hipStream_t s, int n, const int *perm, const Buf &a, const Buf &b);
+
+// physics.hip
+void launch_wall(hipStream_t s, const Prm &q, const Buf &b, const uint32_t *skey, const int *cs);
+void launch_ldg(hipStream_t s, const Prm &q, const Buf &b, const uint32_t *skey, const int *cs, float4 *gd);
+void launch_governing(hipStream_t s, const Prm &q, const Buf &b, const uint32_t *skey, const int *cs,
+                      const float4 *gd, float4 *acc);
+void launch_predict(hipStream_t s, const Prm &q, const Buf &b, const float4 *acc);
+void launch_correct(hipStream_t s, const Prm &q, const Buf &b, const float4 *acc);
+void launch_nbr_stats(hipStream_t s, const Prm &q, const Buf &b, const uint32_t *skey, const int *cs, int *count,
+                      uint64_t *digest);
+
+}  // namespace zs

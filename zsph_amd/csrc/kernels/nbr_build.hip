@@ -1,0 +1,257 @@
+// Neighbour build for gfx950: cell hash, stable LSD radix sort of
+// (hash, index) with wave64 ballot ranking, cell-start table, and the
+// cell-order gather of the particle buffer.
+//   hash      <- single_gpu_sph_src/lib/hash.cu:3-26   (cuda_ptc_hash)
+//   sort      <- single_gpu_sph_src/lib/sort.cu:3-6    (thrust::sort_by_key)
+//   cells +
+//   reorder   <- single_gpu_sph_src/lib/sort.cu:8-84   (cuda_sort_data)
+#include "common.hpp"
+#include "launch.hpp"
+
+namespace zs {
+
+// ---------------------------------------------------------------------------
+// Cell hash.  H = (cz*Y + cy)*X + cx; a particle whose cell lies outside the
+// grid on any axis is removed: type 100, parked at the grid origin with
+// v = 0, (rho, p) = (rho0, 0) and H = 0 (hash.cu:16-23).
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(ZS_BLOCK) void k_hash(Prm q, Buf b, uint32_t *__restrict__ key,
+                                                   int *__restrict__ idx, int *__restrict__ killed) {
+    const int i = xcd_block(gridDim.x) * ZS_BLOCK + threadIdx.x;
+    if (i >= q.N) return;
+    const float4 x = b.posr[i];
+    int cx, cy, cz;
+    const bool in = cell_axis(x.x, q.gx0, q.grid_size, q.X, &cx) &&
+                    cell_axis(x.y, q.gy0, q.grid_size, q.Y, &cy) &&
+                    cell_axis(x.z, q.gz0, q.grid_size, q.Z, &cz);
+    uint32_t h = 0;
+    if (in) {
+        h = (uint32_t)((cz * q.Y + cy) * q.X + cx);
+    } else {
+        b.type[i] = ZSPH_KILLED;
+        b.posr[i] = make_float4(q.gx0, q.gy0, q.gz0, q.rho0);
+        b.velp[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        atomicAdd(killed, 1);
+    }
+    key[i] = h;
+    idx[i] = i;
+}
+
+// ---------------------------------------------------------------------------
+// Radix sort.  Tile = 256 threads x RS_ITEMS keys; wave w of a block owns the
+// contiguous keys [tile + w*64*RS_ITEMS, +64*RS_ITEMS) and walks them 64 at a
+// time, so (wave, item, lane) order equals input order and the scatter is
+// stable.  Equal digits inside a wave are found with BITS ballots.
+// ---------------------------------------------------------------------------
+constexpr int RS_ITEMS = 16;
+constexpr int RS_TILE = ZS_BLOCK * RS_ITEMS;
+constexpr int RS_WAVES = ZS_BLOCK / 64;
+
+__device__ __forceinline__ uint32_t lane_rank(uint64_t m) {
+    return __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+}
+
+// Mask of the lanes (among `valid`) whose BITS-bit digit equals this lane's.
+template <int BITS>
+__device__ __forceinline__ uint64_t match_digit(uint32_t d, uint64_t valid) {
+    uint64_t m = valid;
+#pragma unroll
+    for (int bit = 0; bit < BITS; ++bit) {
+        const bool set = (d >> bit) & 1u;
+        const uint64_t bal = __ballot(set);
+        m &= set ? bal : ~bal;
+    }
+    return m;
+}
+
+template <int BITS>
+__global__ __launch_bounds__(ZS_BLOCK) void k_radix_hist(const uint32_t *__restrict__ kin, int n, int shift,
+                                                         uint32_t *__restrict__ hist, int nblk) {
+    constexpr int R = 1 << BITS;
+    __shared__ uint32_t h[R];
+    for (int d = threadIdx.x; d < R; d += ZS_BLOCK) h[d] = 0;
+    __syncthreads();
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int base = blockIdx.x * RS_TILE + w * 64 * RS_ITEMS;
+    for (int it = 0; it < RS_ITEMS; ++it) {
+        const int e = base + it * 64 + lane;
+        const bool valid = e < n;
+        const uint32_t d = valid ? (kin[e] >> shift) & (R - 1) : 0u;
+        const uint64_t m = match_digit<BITS>(d, __ballot(valid));
+        if (valid && lane_rank(m) == 0) atomicAdd(&h[d], (uint32_t)__popcll(m));
+    }
+    __syncthreads();
+    for (int d = threadIdx.x; d < R; d += ZS_BLOCK) hist[d * nblk + blockIdx.x] = h[d];
+}
+
+// Exclusive scan of the digit-major histogram (R*nblk entries) in one block.
+__global__ __launch_bounds__(1024) void k_radix_scan(uint32_t *__restrict__ a, int total) {
+    __shared__ uint32_t part[1024];
+    const int t = threadIdx.x;
+    const int chunk = (total + 1023) / 1024;
+    const int b = min(t * chunk, total), e = min(b + chunk, total);
+    uint32_t s = 0;
+    for (int i = b; i < e; ++i) s += a[i];
+    part[t] = s;
+    __syncthreads();
+    for (int off = 1; off < 1024; off <<= 1) {
+        const uint32_t v = (t >= off) ? part[t - off] : 0u;
+        __syncthreads();
+        part[t] += v;
+        __syncthreads();
+    }
+    uint32_t run = part[t] - s;  // exclusive prefix of this chunk
+    for (int i = b; i < e; ++i) {
+        const uint32_t v = a[i];
+        a[i] = run;
+        run += v;
+    }
+}
+
+template <int BITS>
+__global__ __launch_bounds__(ZS_BLOCK) void k_radix_scatter(const uint32_t *__restrict__ kin,
+                                                            const int *__restrict__ vin,
+                                                            uint32_t *__restrict__ kout,
+                                                            int *__restrict__ vout, int n, int shift,
+                                                            const uint32_t *__restrict__ scanned, int nblk) {
+    constexpr int R = 1 << BITS;
+    __shared__ uint32_t cnt[RS_WAVES][R];
+    __shared__ uint32_t goff[R];
+    for (int d = threadIdx.x; d < R; d += ZS_BLOCK) {
+#pragma unroll
+        for (int w = 0; w < RS_WAVES; ++w) cnt[w][d] = 0;
+        goff[d] = scanned[d * nblk + blockIdx.x];
+    }
+    __syncthreads();
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int base = blockIdx.x * RS_TILE + w * 64 * RS_ITEMS;
+    uint32_t k[RS_ITEMS];
+    int v[RS_ITEMS];
+    uint32_t loc[RS_ITEMS];
+#pragma unroll
+    for (int it = 0; it < RS_ITEMS; ++it) {
+        const int e = base + it * 64 + lane;
+        const bool valid = e < n;
+        k[it] = valid ? kin[e] : 0u;
+        v[it] = valid ? (vin ? vin[e] : e) : 0;
+    }
+#pragma unroll
+    for (int it = 0; it < RS_ITEMS; ++it) {
+        const int e = base + it * 64 + lane;
+        const bool valid = e < n;
+        const uint32_t d = (k[it] >> shift) & (R - 1);
+        const uint64_t m = match_digit<BITS>(d, __ballot(valid));
+        const uint32_t r = lane_rank(m);
+        const uint32_t before = valid ? cnt[w][d] : 0u;
+        __builtin_amdgcn_wave_barrier();
+        if (valid && r == (uint32_t)__popcll(m) - 1u) cnt[w][d] = before + (uint32_t)__popcll(m);
+        __builtin_amdgcn_wave_barrier();
+        loc[it] = before + r;
+    }
+    __syncthreads();
+    for (int d = threadIdx.x; d < R; d += ZS_BLOCK) {
+        uint32_t run = 0;
+#pragma unroll
+        for (int ww = 0; ww < RS_WAVES; ++ww) {
+            const uint32_t c = cnt[ww][d];
+            cnt[ww][d] = run;
+            run += c;
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int it = 0; it < RS_ITEMS; ++it) {
+        const int e = base + it * 64 + lane;
+        if (e < n) {
+            const uint32_t d = (k[it] >> shift) & (R - 1);
+            const uint32_t dst = goff[d] + cnt[w][d] + loc[it];
+            kout[dst] = k[it];
+            vout[dst] = v[it];
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Cell table: cs[c] = first sorted position with hash >= c (lower bound), so
+// cell c holds [cs[c], cs[c+1]) and empty cells have cs[c] == cs[c+1];
+// cs[G] = N.  Every entry is rewritten on every call.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(ZS_BLOCK) void k_cell_start(const uint32_t *__restrict__ skey, int n, int G,
+                                                         int *__restrict__ cs) {
+    const int c = blockIdx.x * ZS_BLOCK + threadIdx.x;
+    if (c > G) return;
+    int lo = 0, hi = n;
+    while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (skey[mid] < (uint32_t)c) lo = mid + 1;
+        else hi = mid;
+    }
+    cs[c] = lo;
+}
+
+// Gather buffer `a` into cell order in buffer `b`; table'[k] = table[p] + p - k
+// keeps table + index equal to the original particle id (sort.cu:107).
+__global__ __launch_bounds__(ZS_BLOCK) void k_reorder(int n, const int *__restrict__ perm, Buf a, Buf b) {
+    const int k = xcd_block(gridDim.x) * ZS_BLOCK + threadIdx.x;
+    if (k >= n) return;
+    const int p = perm[k];
+    b.posr[k] = a.posr[p];
+    b.velp[k] = a.velp[p];
+    b.tposr[k] = a.tposr[p];
+    b.tvelp[k] = a.tvelp[p];
+    b.type[k] = a.type[p];
+    b.table[k] = a.table[p] + p - k;
+}
+
+// ---------------------------------------------------------------------------
+// Launchers
+// ---------------------------------------------------------------------------
+static inline int nblocks(int n) { return (n + ZS_BLOCK - 1) / ZS_BLOCK; }
+
+void launch_hash(hipStream_t s, const Prm &q, const Buf &b, uint32_t *key, int *idx, int *killed) {
+    if (q.N == 0) return;
+    k_hash<<<nblocks(q.N), ZS_BLOCK, 0, s>>>(q, b, key, idx, killed);
+}
+
+int radix_blocks(int n) { return (n + RS_TILE - 1) / RS_TILE; }
+
+size_t radix_hist_words(int n, int bits) { return ((size_t)1 << bits) * (size_t)radix_blocks(n); }
+
+template <int BITS>
+static int sort_passes(hipStream_t s, int n, int keybits, uint32_t *key[2], int *val[2], uint32_t *hist,
+                       bool identity_vals) {
+    const int nblk = radix_blocks(n);
+    const int R = 1 << BITS;
+    int src = 0;
+    for (int shift = 0; shift < keybits; shift += BITS) {
+        k_radix_hist<BITS><<<nblk, ZS_BLOCK, 0, s>>>(key[src], n, shift, hist, nblk);
+        k_radix_scan<<<1, 1024, 0, s>>>(hist, R * nblk);
+        k_radix_scatter<BITS><<<nblk, ZS_BLOCK, 0, s>>>(key[src], (identity_vals && shift == 0) ? nullptr : val[src],
+                                                        key[1 - src], val[1 - src], n, shift, hist, nblk);
+        src = 1 - src;
+    }
+    return src;
+}
+
+int launch_sort(hipStream_t s, int n, int keybits, int bits, uint32_t *key[2], int *val[2], uint32_t *hist,
+                bool identity_vals) {
+    if (n == 0) return 0;
+    switch (bits) {
+    case 4: return sort_passes<4>(s, n, keybits, key, val, hist, identity_vals);
+    case 6: return sort_passes<6>(s, n, keybits, key, val, hist, identity_vals);
+    case 7: return sort_passes<7>(s, n, keybits, key, val, hist, identity_vals);
+    case 8: return sort_passes<8>(s, n, keybits, key, val, hist, identity_vals);
+    default: return -1;
+    }
+}
+
+void launch_cell_start(hipStream_t s, const uint32_t *skey, int n, int G, int *cs) {
+    k_cell_start<<<(G + 1 + ZS_BLOCK - 1) / ZS_BLOCK, ZS_BLOCK, 0, s>>>(skey, n, G, cs);
+}
+
+void launch_reorder(hipStream_t s, int n, const int *perm, const Buf &a, const Buf &b) {
+    if (n == 0) return;
+    k_reorder<<<nblocks(n), ZS_BLOCK, 0, s>>>(n, perm, a, b);
+}
+
+}  // namespace zs

@@ -1,0 +1,491 @@
+// C ABI of libzsph: device context, transfers, the per-stage entry points of
+// the reference driver (single_gpu_sph_src/deltasph/main.cu:58-90) and the
+// graph-captured full step.  Host code only; kernels live in kernels/.
+#include <hip/hip_runtime.h>
+
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "kernels/common.hpp"
+#include "kernels/launch.hpp"
+#include "zsph.h"
+
+struct zsph_ctx {
+    zsph_params p{};
+    zs::Prm q{};
+    int device = 0;
+    hipStream_t stream = nullptr;
+    zs::Buf buf[2]{};
+    uint32_t *key[2]{};
+    int *idx[2]{};
+    int sorted_slot = -1;  // slot of (key, idx) holding the sorted result
+    int hashed = 0;        // key[0]/idx[0] hold a fresh (unsorted) hash
+    int *cell_start = nullptr;
+    float4 *acc = nullptr;
+    float4 *gd = nullptr;
+    uint32_t *hist = nullptr;
+    int *killed = nullptr;
+    int keybits = 1;
+    int radix_bits = 8;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t graph_exec = nullptr;
+    std::string err;
+    std::vector<void *> allocs;
+};
+
+namespace {
+
+int set_err(zsph_ctx *c, int code, const char *fmt, ...) {
+    if (c) {
+        char buf[512];
+        va_list ap;
+        va_start(ap, fmt);
+        std::vsnprintf(buf, sizeof(buf), fmt, ap);
+        va_end(ap);
+        c->err = buf;
+    }
+    return code;
+}
+
+#define HIPCHK(ctx, call)                                                                       \
+    do {                                                                                        \
+        hipError_t e_ = (call);                                                                 \
+        if (e_ != hipSuccess)                                                                   \
+            return set_err(ctx, ZSPH_E_HIP, "%s failed: %s", #call, hipGetErrorString(e_));     \
+    } while (0)
+
+template <class T>
+int dalloc(zsph_ctx *c, T **p, size_t count) {
+    void *d = nullptr;
+    const size_t bytes = count ? count * sizeof(T) : sizeof(T);
+    hipError_t e = hipMalloc(&d, bytes);
+    if (e != hipSuccess) return set_err(c, ZSPH_E_NOMEM, "hipMalloc(%zu) failed: %s", bytes, hipGetErrorString(e));
+    c->allocs.push_back(d);
+    *p = static_cast<T *>(d);
+    e = hipMemset(d, 0, bytes);
+    if (e != hipSuccess) return set_err(c, ZSPH_E_HIP, "hipMemset failed: %s", hipGetErrorString(e));
+    return ZSPH_OK;
+}
+
+int check_launch(zsph_ctx *c, const char *what) {
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return set_err(c, ZSPH_E_HIP, "%s launch failed: %s", what, hipGetErrorString(e));
+    return ZSPH_OK;
+}
+
+bool valid_buf(int b) { return b == 0 || b == 1; }
+
+// Optional per-stage event timing (zsph_profile_steps).
+struct Prof {
+    hipStream_t s;
+    std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> ev;
+    hipEvent_t open_ev = nullptr;
+    int open_stage = -1;
+    void begin(int st) {
+        hipEvent_t e;
+        (void)hipEventCreate(&e);
+        (void)hipEventRecord(e, s);
+        open_ev = e;
+        open_stage = st;
+    }
+    void end() {
+        hipEvent_t e;
+        (void)hipEventCreate(&e);
+        (void)hipEventRecord(e, s);
+        ev.push_back({open_stage, {open_ev, e}});
+    }
+};
+struct Scope {
+    Prof *p;
+    Scope(Prof *pp, int st) : p(pp) { if (p) p->begin(st); }
+    ~Scope() { if (p) p->end(); }
+};
+
+// The stages of one half step, in the reference's order (main.cu:63-72).
+int stage_hash(zsph_ctx *c, int b, Prof *pf = nullptr) {
+    {
+        Scope sc(pf, ZSPH_ST_HASH);
+        zs::launch_hash(c->stream, c->q, c->buf[b], c->key[0], c->idx[0], c->killed);
+    }
+    c->hashed = 1;
+    c->sorted_slot = -1;
+    return check_launch(c, "hash");
+}
+int stage_sort(zsph_ctx *c, Prof *pf = nullptr) {
+    if (!c->hashed) return set_err(c, ZSPH_E_STATE, "zsph_sort_index called before zsph_hash");
+    int slot;
+    {
+        Scope sc(pf, ZSPH_ST_SORT);
+        slot = zs::launch_sort(c->stream, c->q.N, c->keybits, c->radix_bits, c->key, c->idx, c->hist, true);
+    }
+    if (slot < 0) return set_err(c, ZSPH_E_ARG, "unsupported radix width %d", c->radix_bits);
+    c->sorted_slot = slot;
+    c->hashed = 0;
+    return check_launch(c, "radix sort");
+}
+int stage_sort_data(zsph_ctx *c, int from, int to, Prof *pf = nullptr) {
+    if (c->sorted_slot < 0) return set_err(c, ZSPH_E_STATE, "zsph_sort_data called before zsph_sort_index");
+    if (from == to) return set_err(c, ZSPH_E_ARG, "zsph_sort_data needs two different buffers");
+    {
+        Scope sc(pf, ZSPH_ST_CELLS);
+        zs::launch_cell_start(c->stream, c->key[c->sorted_slot], c->q.N, c->q.G, c->cell_start);
+    }
+    {
+        Scope sc(pf, ZSPH_ST_REORDER);
+        zs::launch_reorder(c->stream, c->q.N, c->idx[c->sorted_slot], c->buf[from], c->buf[to]);
+    }
+    return check_launch(c, "sort_data");
+}
+int stage_boundary(zsph_ctx *c, int b, Prof *pf = nullptr) {
+    if (c->sorted_slot < 0) return set_err(c, ZSPH_E_STATE, "zsph_boundary called before zsph_sort_data");
+    const uint32_t *sk = c->key[c->sorted_slot];
+    {
+        Scope sc(pf, ZSPH_ST_WALL);
+        zs::launch_wall(c->stream, c->q, c->buf[b], sk, c->cell_start);
+    }
+    {
+        Scope sc(pf, ZSPH_ST_LDG);
+        zs::launch_ldg(c->stream, c->q, c->buf[b], sk, c->cell_start, c->gd);
+    }
+    return check_launch(c, "boundary");
+}
+int stage_governing(zsph_ctx *c, int b, Prof *pf = nullptr) {
+    if (c->sorted_slot < 0) return set_err(c, ZSPH_E_STATE, "zsph_governing called before zsph_sort_data");
+    {
+        Scope sc(pf, ZSPH_ST_GOVERNING);
+        zs::launch_governing(c->stream, c->q, c->buf[b], c->key[c->sorted_slot], c->cell_start, c->gd, c->acc);
+    }
+    return check_launch(c, "governing");
+}
+
+int half_step(zsph_ctx *c, int from, int to, bool predictor, Prof *pf = nullptr) {
+    int r;
+    if ((r = stage_hash(c, from, pf))) return r;
+    if ((r = stage_sort(c, pf))) return r;
+    if ((r = stage_sort_data(c, from, to, pf))) return r;
+    if ((r = stage_boundary(c, to, pf))) return r;
+    if ((r = stage_governing(c, to, pf))) return r;
+    {
+        Scope sc(pf, ZSPH_ST_INTEGRATE);
+        if (predictor)
+            zs::launch_predict(c->stream, c->q, c->buf[to], c->acc);
+        else
+            zs::launch_correct(c->stream, c->q, c->buf[to], c->acc);
+    }
+    return check_launch(c, predictor ? "prediction" : "correction");
+}
+
+}  // namespace
+
+extern "C" {
+
+int zsph_create(const zsph_params *p, int device, zsph_ctx **out) {
+    if (!p || !out) return ZSPH_E_ARG;
+    *out = nullptr;
+    if (p->ptc_num < 0 || p->grid_xdim < 1 || p->grid_ydim < 1 || p->grid_zdim < 1 || !(p->h > 0.0f) ||
+        !(p->grid_size > 0.0f) || p->grid_factor < 1 || (p->dim != 2 && p->dim != 3) || !(p->gamma > 0.0f) ||
+        !(p->rc2 > 0.0f) || !(p->rho0 > 0.0f) || !(p->cs2 > 0.0f))
+        return ZSPH_E_ARG;
+    const long long G = (long long)p->grid_xdim * p->grid_ydim * p->grid_zdim;
+    if (G > (1ll << 30) || G - 1 != (long long)p->grid_hash_max) return ZSPH_E_ARG;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return ZSPH_E_NODEV;
+    zsph_ctx *c = new zsph_ctx();
+    c->p = *p;
+    c->q = zs::make_prm(*p);
+    if (device >= 0) {
+        if (device >= ndev) { delete c; return ZSPH_E_NODEV; }
+        if (hipSetDevice(device) != hipSuccess) { delete c; return ZSPH_E_HIP; }
+    }
+    hipGetDevice(&c->device);
+    int r = ZSPH_OK;
+    auto fail = [&](int code) {
+        zsph_destroy(c);
+        return code;
+    };
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) return fail(ZSPH_E_HIP);
+    const size_t N = (size_t)p->ptc_num;
+    for (int b = 0; b < 2; ++b) {
+        if ((r = dalloc(c, &c->buf[b].posr, N)) || (r = dalloc(c, &c->buf[b].velp, N)) ||
+            (r = dalloc(c, &c->buf[b].tposr, N)) || (r = dalloc(c, &c->buf[b].tvelp, N)) ||
+            (r = dalloc(c, &c->buf[b].type, N)) || (r = dalloc(c, &c->buf[b].table, N)))
+            return fail(r);
+        if ((r = dalloc(c, &c->key[b], N)) || (r = dalloc(c, &c->idx[b], N))) return fail(r);
+    }
+    int kb = 1;
+    while (kb < 31 && ((long long)1 << kb) < G) ++kb;
+    c->keybits = kb;
+    if ((r = dalloc(c, &c->cell_start, (size_t)G + 1)) || (r = dalloc(c, &c->acc, N)) ||
+        (r = dalloc(c, &c->gd, N)) || (r = dalloc(c, &c->hist, zs::radix_hist_words((int)N, 8))) ||
+        (r = dalloc(c, &c->killed, 1)))
+        return fail(r);
+    *out = c;
+    return ZSPH_OK;
+}
+
+void zsph_destroy(zsph_ctx *c) {
+    if (!c) return;
+    if (c->stream) hipStreamSynchronize(c->stream);
+    if (c->graph_exec) hipGraphExecDestroy(c->graph_exec);
+    if (c->graph) hipGraphDestroy(c->graph);
+    for (void *d : c->allocs) hipFree(d);
+    if (c->stream) hipStreamDestroy(c->stream);
+    delete c;
+}
+
+int zsph_sync(zsph_ctx *c) {
+    if (!c) return ZSPH_E_ARG;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    HIPCHK(c, hipGetLastError());
+    return ZSPH_OK;
+}
+
+const char *zsph_last_error(zsph_ctx *c) { return c ? c->err.c_str() : "null context"; }
+
+int zsph_upload(zsph_ctx *c, int b, const zsph_host_state *st) {
+    if (!c || !st || !valid_buf(b) || !st->pos || !st->type) return set_err(c, ZSPH_E_ARG, "zsph_upload: bad argument");
+    const int N = c->q.N;
+    std::vector<float4> posr(N), velp(N);
+    for (int i = 0; i < N; ++i) {
+        posr[i] = make_float4(st->pos[3 * i], st->pos[3 * i + 1], st->pos[3 * i + 2],
+                              st->rhop ? st->rhop[2 * i] : c->q.rho0);
+        velp[i] = st->vel ? make_float4(st->vel[3 * i], st->vel[3 * i + 1], st->vel[3 * i + 2], 0.0f)
+                          : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        velp[i].w = st->rhop ? st->rhop[2 * i + 1] : 0.0f;
+    }
+    const zs::Buf &d = c->buf[b];
+    HIPCHK(c, hipMemcpyAsync(d.posr, posr.data(), N * sizeof(float4), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipMemcpyAsync(d.velp, velp.data(), N * sizeof(float4), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipMemcpyAsync(d.tposr, posr.data(), N * sizeof(float4), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipMemcpyAsync(d.tvelp, velp.data(), N * sizeof(float4), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipMemcpyAsync(d.type, st->type, N * sizeof(int), hipMemcpyHostToDevice, c->stream));
+    if (st->table) {
+        HIPCHK(c, hipMemcpyAsync(d.table, st->table, N * sizeof(int), hipMemcpyHostToDevice, c->stream));
+    } else {
+        HIPCHK(c, hipMemsetAsync(d.table, 0, N * sizeof(int), c->stream));
+    }
+    if (st->acc_drhodt)
+        HIPCHK(c, hipMemcpyAsync(c->acc, st->acc_drhodt, N * sizeof(float4), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    c->hashed = 0;
+    c->sorted_slot = -1;
+    return ZSPH_OK;
+}
+
+int zsph_download(zsph_ctx *c, int b, zsph_host_state *st, int fields) {
+    if (!c || !st || !valid_buf(b)) return set_err(c, ZSPH_E_ARG, "zsph_download: bad argument");
+    const int N = c->q.N;
+    const zs::Buf &d = c->buf[b];
+    std::vector<float4> tmp(N);
+    if ((fields & ZSPH_F_POS) || (fields & ZSPH_F_RHOP)) {
+        HIPCHK(c, hipMemcpyAsync(tmp.data(), d.posr, N * sizeof(float4), hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        for (int i = 0; i < N; ++i) {
+            if ((fields & ZSPH_F_POS) && st->pos) {
+                st->pos[3 * i] = tmp[i].x; st->pos[3 * i + 1] = tmp[i].y; st->pos[3 * i + 2] = tmp[i].z;
+            }
+            if ((fields & ZSPH_F_RHOP) && st->rhop) st->rhop[2 * i] = tmp[i].w;
+        }
+    }
+    if ((fields & ZSPH_F_VEL) || (fields & ZSPH_F_RHOP)) {
+        HIPCHK(c, hipMemcpyAsync(tmp.data(), d.velp, N * sizeof(float4), hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        for (int i = 0; i < N; ++i) {
+            if ((fields & ZSPH_F_VEL) && st->vel) {
+                st->vel[3 * i] = tmp[i].x; st->vel[3 * i + 1] = tmp[i].y; st->vel[3 * i + 2] = tmp[i].z;
+            }
+            if ((fields & ZSPH_F_RHOP) && st->rhop) st->rhop[2 * i + 1] = tmp[i].w;
+        }
+    }
+    if ((fields & ZSPH_F_ACC) && st->acc_drhodt)
+        HIPCHK(c, hipMemcpyAsync(st->acc_drhodt, c->acc, N * sizeof(float4), hipMemcpyDeviceToHost, c->stream));
+    if ((fields & ZSPH_F_TYPE) && st->type)
+        HIPCHK(c, hipMemcpyAsync(st->type, d.type, N * sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    if ((fields & ZSPH_F_TABLE) && st->table)
+        HIPCHK(c, hipMemcpyAsync(st->table, d.table, N * sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    if ((fields & ZSPH_F_HASH) && st->hash) {
+        const uint32_t *src = c->key[c->sorted_slot >= 0 ? c->sorted_slot : 0];
+        HIPCHK(c, hipMemcpyAsync(st->hash, src, N * sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    }
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return ZSPH_OK;
+}
+
+int zsph_hash(zsph_ctx *c, int b) {
+    if (!c || !valid_buf(b)) return set_err(c, ZSPH_E_ARG, "zsph_hash: bad buffer");
+    return stage_hash(c, b);
+}
+int zsph_sort_index(zsph_ctx *c) {
+    if (!c) return ZSPH_E_ARG;
+    return stage_sort(c);
+}
+int zsph_sort_data(zsph_ctx *c, int from, int to) {
+    if (!c || !valid_buf(from) || !valid_buf(to)) return set_err(c, ZSPH_E_ARG, "zsph_sort_data: bad buffer");
+    return stage_sort_data(c, from, to);
+}
+int zsph_boundary(zsph_ctx *c, int b) {
+    if (!c || !valid_buf(b)) return set_err(c, ZSPH_E_ARG, "zsph_boundary: bad buffer");
+    return stage_boundary(c, b);
+}
+int zsph_governing(zsph_ctx *c, int b) {
+    if (!c || !valid_buf(b)) return set_err(c, ZSPH_E_ARG, "zsph_governing: bad buffer");
+    return stage_governing(c, b);
+}
+int zsph_prediction(zsph_ctx *c, int b) {
+    if (!c || !valid_buf(b)) return set_err(c, ZSPH_E_ARG, "zsph_prediction: bad buffer");
+    zs::launch_predict(c->stream, c->q, c->buf[b], c->acc);
+    return check_launch(c, "prediction");
+}
+int zsph_correction(zsph_ctx *c, int b) {
+    if (!c || !valid_buf(b)) return set_err(c, ZSPH_E_ARG, "zsph_correction: bad buffer");
+    zs::launch_correct(c->stream, c->q, c->buf[b], c->acc);
+    return check_launch(c, "correction");
+}
+
+int zsph_step(zsph_ctx *c, int nsteps) {
+    if (!c || nsteps < 0) return set_err(c, ZSPH_E_ARG, "zsph_step: bad argument");
+    if (nsteps == 0 || c->q.N == 0) return ZSPH_OK;
+    if (!c->graph_exec) {
+        HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+        int r = half_step(c, 0, 1, true);
+        if (!r) r = half_step(c, 1, 0, false);
+        hipGraph_t g = nullptr;
+        hipError_t e = hipStreamEndCapture(c->stream, &g);
+        if (r) {
+            if (g) hipGraphDestroy(g);
+            return r;
+        }
+        if (e != hipSuccess) return set_err(c, ZSPH_E_HIP, "graph capture failed: %s", hipGetErrorString(e));
+        c->graph = g;
+        e = hipGraphInstantiate(&c->graph_exec, g, nullptr, nullptr, 0);
+        if (e != hipSuccess) return set_err(c, ZSPH_E_HIP, "graph instantiate failed: %s", hipGetErrorString(e));
+    }
+    for (int s = 0; s < nsteps; ++s) HIPCHK(c, hipGraphLaunch(c->graph_exec, c->stream));
+    // After a step the sorted tables describe buffer 0.
+    c->hashed = 0;
+    c->sorted_slot = (c->keybits + c->radix_bits - 1) / c->radix_bits % 2;
+    return ZSPH_OK;
+}
+
+int zsph_array_get(zsph_ctx *c, int which, int b, void *host, size_t bytes) {
+    if (!c || !host || !valid_buf(b)) return set_err(c, ZSPH_E_ARG, "zsph_array_get: bad argument");
+    const size_t N = (size_t)c->q.N;
+    const void *src = nullptr;
+    size_t need = 0;
+    switch (which) {
+    case ZSPH_A_POSR: src = c->buf[b].posr; need = N * 16; break;
+    case ZSPH_A_VELP: src = c->buf[b].velp; need = N * 16; break;
+    case ZSPH_A_TPOSR: src = c->buf[b].tposr; need = N * 16; break;
+    case ZSPH_A_TVELP: src = c->buf[b].tvelp; need = N * 16; break;
+    case ZSPH_A_TYPE: src = c->buf[b].type; need = N * 4; break;
+    case ZSPH_A_TABLE: src = c->buf[b].table; need = N * 4; break;
+    case ZSPH_A_KEY: src = c->key[c->sorted_slot >= 0 ? c->sorted_slot : 0]; need = N * 4; break;
+    case ZSPH_A_INDEX: src = c->idx[c->sorted_slot >= 0 ? c->sorted_slot : 0]; need = N * 4; break;
+    case ZSPH_A_CELL_START: src = c->cell_start; need = ((size_t)c->q.G + 1) * 4; break;
+    case ZSPH_A_ACC: src = c->acc; need = N * 16; break;
+    case ZSPH_A_GD: src = c->gd; need = N * 16; break;
+    default: return set_err(c, ZSPH_E_ARG, "zsph_array_get: unknown array %d", which);
+    }
+    if (bytes != need) return set_err(c, ZSPH_E_ARG, "zsph_array_get: %zu bytes given, %zu needed", bytes, need);
+    if (need) HIPCHK(c, hipMemcpyAsync(host, src, need, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return ZSPH_OK;
+}
+
+int zsph_array_set(zsph_ctx *c, int which, int b, const void *host, size_t bytes) {
+    if (!c || !host || !valid_buf(b)) return set_err(c, ZSPH_E_ARG, "zsph_array_set: bad argument");
+    const size_t N = (size_t)c->q.N;
+    void *dst = nullptr;
+    size_t need = 0;
+    switch (which) {
+    case ZSPH_A_POSR: dst = c->buf[b].posr; need = N * 16; break;
+    case ZSPH_A_VELP: dst = c->buf[b].velp; need = N * 16; break;
+    case ZSPH_A_TPOSR: dst = c->buf[b].tposr; need = N * 16; break;
+    case ZSPH_A_TVELP: dst = c->buf[b].tvelp; need = N * 16; break;
+    case ZSPH_A_TYPE: dst = c->buf[b].type; need = N * 4; break;
+    case ZSPH_A_TABLE: dst = c->buf[b].table; need = N * 4; break;
+    case ZSPH_A_ACC: dst = c->acc; need = N * 16; break;
+    case ZSPH_A_GD: dst = c->gd; need = N * 16; break;
+    default: return set_err(c, ZSPH_E_ARG, "zsph_array_set: array %d is not writable", which);
+    }
+    if (bytes != need) return set_err(c, ZSPH_E_ARG, "zsph_array_set: %zu bytes given, %zu needed", bytes, need);
+    if (need) HIPCHK(c, hipMemcpyAsync(dst, host, need, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return ZSPH_OK;
+}
+
+int zsph_neighbor_stats(zsph_ctx *c, int b, int *count, uint64_t *digest) {
+    if (!c || !valid_buf(b) || !count || !digest) return set_err(c, ZSPH_E_ARG, "zsph_neighbor_stats: bad argument");
+    if (c->sorted_slot < 0) return set_err(c, ZSPH_E_STATE, "zsph_neighbor_stats called before zsph_sort_data");
+    const size_t N = (size_t)c->q.N;
+    int *dc = nullptr;
+    uint64_t *dd = nullptr;
+    HIPCHK(c, hipMalloc(&dc, N ? N * 4 : 4));
+    hipError_t e = hipMalloc(&dd, N ? N * 8 : 8);
+    if (e != hipSuccess) {
+        (void)hipFree(dc);
+        return set_err(c, ZSPH_E_NOMEM, "hipMalloc failed");
+    }
+    zs::launch_nbr_stats(c->stream, c->q, c->buf[b], c->key[c->sorted_slot], c->cell_start, dc, dd);
+    e = hipGetLastError();
+    if (e == hipSuccess && N) e = hipMemcpyAsync(count, dc, N * 4, hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess && N) e = hipMemcpyAsync(digest, dd, N * 8, hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    (void)hipFree(dc);
+    (void)hipFree(dd);
+    if (e != hipSuccess) return set_err(c, ZSPH_E_HIP, "neighbour stats failed: %s", hipGetErrorString(e));
+    return ZSPH_OK;
+}
+
+int zsph_time_steps(zsph_ctx *c, int nsteps, float *ms) {
+    if (!c || !ms || nsteps < 1) return set_err(c, ZSPH_E_ARG, "zsph_time_steps: bad argument");
+    int r = zsph_step(c, 1);  // make sure the graph exists before the timed region
+    if (r) return r;
+    hipEvent_t e0, e1;
+    HIPCHK(c, hipEventCreate(&e0));
+    HIPCHK(c, hipEventCreate(&e1));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    HIPCHK(c, hipEventRecord(e0, c->stream));
+    r = zsph_step(c, nsteps);
+    HIPCHK(c, hipEventRecord(e1, c->stream));
+    HIPCHK(c, hipEventSynchronize(e1));
+    HIPCHK(c, hipEventElapsedTime(ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    return r;
+}
+
+int zsph_profile_steps(zsph_ctx *c, int nsteps, float *stage_ms) {
+    if (!c || !stage_ms || nsteps < 1) return set_err(c, ZSPH_E_ARG, "zsph_profile_steps: bad argument");
+    Prof pf;
+    pf.s = c->stream;
+    int r = ZSPH_OK;
+    for (int s = 0; s < nsteps && !r; ++s) {
+        r = half_step(c, 0, 1, true, &pf);
+        if (!r) r = half_step(c, 1, 0, false, &pf);
+    }
+    hipError_t e = hipStreamSynchronize(c->stream);
+    for (int k = 0; k < ZSPH_ST_COUNT; ++k) stage_ms[k] = 0.0f;
+    for (auto &x : pf.ev) {
+        float t = 0.0f;
+        if (e == hipSuccess) e = hipEventElapsedTime(&t, x.second.first, x.second.second);
+        stage_ms[x.first] += t;
+        (void)hipEventDestroy(x.second.first);
+        (void)hipEventDestroy(x.second.second);
+    }
+    if (r) return r;
+    if (e != hipSuccess) return set_err(c, ZSPH_E_HIP, "profile: %s", hipGetErrorString(e));
+    return ZSPH_OK;
+}
+
+int zsph_killed(zsph_ctx *c, int *killed) {
+    if (!c || !killed) return ZSPH_E_ARG;
+    HIPCHK(c, hipMemcpyAsync(killed, c->killed, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return ZSPH_OK;
+}
+
+}  // extern "C"
