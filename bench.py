@@ -63,6 +63,7 @@ def main():
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--dx", type=float, default=0.008)
+    ap.add_argument("--gsf", type=float, default=1.0, help="grid_size_factor: cell edge = 2h / gsf")
     ap.add_argument("--profile-steps", type=int, default=3)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     args = ap.parse_args()
@@ -74,7 +75,7 @@ def main():
     import zsph_amd as z
     from zsph_amd import cases
 
-    case = cases.dambreak3d(dx=args.dx)
+    case = cases.dambreak3d(dx=args.dx, grid_size_factor=args.gsf)
     print(f"[bench] 3-D dam break dx={args.dx}: N={case.n} (fluid {case.json.water_ptc_num}), "
           f"grid {case.cpu.grid_xdim}x{case.cpu.grid_ydim}x{case.cpu.grid_zdim}", file=sys.stderr, flush=True)
     s = z.Solver(case, device=0)

@@ -29,6 +29,9 @@ CASES = {
     "2d-mc-linear": lambda: jitter(cases.dambreak2d(gamma=1.0, diffusion=z.DIFF_MC), seed=4),
     "3d-small": lambda: jitter(cases.dambreak3d(dx=0.04, lx=0.8, wy=0.4, lz=0.5, cw=0.3, ch=0.3), seed=12345),
     "3d-mid": lambda: jitter(cases.dambreak3d(dx=0.02), seed=12345),
+    # cells of edge h (grid_size_factor 2): 5x5x5 cell box
+    "3d-k2": lambda: jitter(cases.dambreak3d(dx=0.03, lx=0.9, wy=0.5, lz=0.6, cw=0.4, ch=0.35,
+                                             grid_size_factor=2.0), seed=99, amp=0.3),
 }
 
 
@@ -42,7 +45,7 @@ def pair(name):
     return c, s, o
 
 
-@pytest.mark.parametrize("name", ["2d", "3d-small", "3d-mid"])
+@pytest.mark.parametrize("name", ["2d", "3d-small", "3d-mid", "3d-k2"])
 def test_hash_sort_reorder_bitexact(name):
     c, s, o = pair(name)
     s.hash(0); o.hash(0)
@@ -59,7 +62,7 @@ def test_hash_sort_reorder_bitexact(name):
     s.sync()
 
 
-@pytest.mark.parametrize("name", ["2d", "3d-small", "3d-mid"])
+@pytest.mark.parametrize("name", ["2d", "3d-small", "3d-mid", "3d-k2"])
 def test_neighbour_sets_bitexact(name):
     c, s, o = pair(name)
     for x in (s, o):
@@ -73,7 +76,7 @@ def test_neighbour_sets_bitexact(name):
     say(f"  neighbours: mean {cg.mean():.1f} max {cg.max()}")
 
 
-@pytest.mark.parametrize("name", ["2d", "2d-mc-linear", "3d-small", "3d-mid"])
+@pytest.mark.parametrize("name", ["2d", "2d-mc-linear", "3d-small", "3d-mid", "3d-k2"])
 def test_boundary_and_governing_within_tolerance(name):
     c, s, o = pair(name)
     for x in (s, o):
@@ -154,7 +157,7 @@ def by_original_id(h):
     return order
 
 
-@pytest.mark.parametrize("name", ["2d", "3d-small", "3d-mid"])
+@pytest.mark.parametrize("name", ["2d", "3d-small", "3d-mid", "3d-k2"])
 def test_full_step_matches_oracle(name):
     c, s, o = pair(name)
     s.step(1)

@@ -15,6 +15,9 @@ struct Prm {
     float gamma, B;  // EOS: p = B ((rho/rho0)^gamma - 1), B = rho0 c^2 / gamma
     float inv_gamma;
     float grid_size;
+    float inv_cell;  // 1 / grid_size
+    float trim2;     // (cut-off in cell units)^2, padded: row-trimming radius
+    float trim_pad;  // extra half-width (cell units) kept around each trimmed row
     float gx0, gy0, gz0;
     int k;           // cells searched each side
     int X, Y, Z, XY;
@@ -47,6 +50,12 @@ inline Prm make_prm(const zsph_params &p) {
     q.B = p.rho0 * p.cs2 / p.gamma;
     q.inv_gamma = 1.0f / p.gamma;
     q.grid_size = p.grid_size;
+    q.inv_cell = 1.0f / p.grid_size;
+    // Cut-off radius in cell units, padded by 0.1 % plus 1e-3 cells: far
+    // above the FP32 rounding of r^2 and of the cell-unit coordinates, so
+    // trimming never drops a pair the r^2 <= rc2 predicate accepts.
+    q.trim2 = p.rc2 * q.inv_cell * q.inv_cell * 1.002f;
+    q.trim_pad = 1e-3f;
     q.gx0 = p.grid_xmin;
     q.gy0 = p.grid_ymin;
     q.gz0 = p.grid_zmin;
@@ -150,6 +159,60 @@ __device__ __forceinline__ void for_each_candidate(const Prm &q, uint32_t key, c
             const int jb = cs[row + b.xlo];
             const int je = cs[row + b.xhi + 1];
             for (int j = jb; j < je; ++j) f(j);
+        }
+    }
+}
+
+// Visit every neighbour j != i (FP32 r^2 <= rc2) of particle i at xi.
+// Each (y,z) cell row of the box is first trimmed to the x-cells that can
+// hold a point within the cut-off of xi (a row whose perpendicular distance
+// exceeds the cut-off is skipped); the trimming works in cell units with a
+// margin, so it only ever keeps extra candidates and the neighbour set is
+// decided by the r^2 predicate alone.  Candidates are tested 32 at a time
+// into a hit mask, then the hits are visited: the wave runs the interaction
+// body once per hit of its busiest lane, not once per candidate.
+// f(j, dx, dy, dz, r2, xj) with x_ij = xi - xj.
+template <class F>
+__device__ __forceinline__ void for_each_neighbor(const Prm &q, int i, float4 xi, uint32_t key,
+                                                  const int *__restrict__ cs, const float4 *__restrict__ posr,
+                                                  F &&f) {
+    const CellBox b = cell_box(q, key);
+    const float fx = (xi.x - q.gx0) * q.inv_cell;
+    const float fy = (xi.y - q.gy0) * q.inv_cell;
+    const float fz = (xi.z - q.gz0) * q.inv_cell;
+    for (int cz = b.zlo; cz <= b.zhi; ++cz) {
+        const float dzc = fmaxf(0.0f, fmaxf((float)cz - fz, fz - (float)(cz + 1)));
+        const float remz = q.trim2 - dzc * dzc;
+        if (remz < 0.0f) continue;
+        for (int cy = b.ylo; cy <= b.yhi; ++cy) {
+            const float dyc = fmaxf(0.0f, fmaxf((float)cy - fy, fy - (float)(cy + 1)));
+            const float rem = remz - dyc * dyc;
+            if (rem < 0.0f) continue;
+            const float w = sqrtf(rem) + q.trim_pad;
+            const int xa = max(b.xlo, (int)floorf(fx - w));
+            const int xb = min(b.xhi, (int)floorf(fx + w));
+            if (xa > xb) continue;
+            const int row = cz * q.XY + cy * q.X;
+            const int jb = cs[row + xa];
+            const int je = cs[row + xb + 1];
+            for (int base = jb; base < je; base += 32) {
+                const int cnt = min(32, je - base);
+                uint32_t m = 0;
+                for (int t = 0; t < cnt; ++t) {
+                    const float4 xj = posr[base + t];
+                    const float r2 = dist2(xi.x - xj.x, xi.y - xj.y, xi.z - xj.z);
+                    m |= (r2 <= q.rc2 ? 1u : 0u) << t;
+                }
+                if (i >= base && i < base + cnt) m &= ~(1u << (i - base));
+                while (m) {
+                    const int t = __builtin_ctz(m);
+                    m &= m - 1u;
+                    const int j = base + t;
+                    const float4 xj = posr[j];
+                    const float dx = xi.x - xj.x, dy = xi.y - xj.y, dz = xi.z - xj.z;
+                    f(j, dx, dy, dz, dist2(dx, dy, dz), xj);
+                }
+            }
         }
     }
 }

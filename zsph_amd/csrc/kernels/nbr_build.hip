@@ -84,28 +84,32 @@ __global__ __launch_bounds__(ZS_BLOCK) void k_radix_hist(const uint32_t *__restr
     for (int d = threadIdx.x; d < R; d += ZS_BLOCK) hist[d * nblk + blockIdx.x] = h[d];
 }
 
-// Exclusive scan of the digit-major histogram (R*nblk entries) in one block.
-__global__ __launch_bounds__(1024) void k_radix_scan(uint32_t *__restrict__ a, int total) {
-    __shared__ uint32_t part[1024];
+// Row scan: block d turns hist[d][0..nblk) into its exclusive prefix (the
+// offset of each tile among digit-d keys) and writes the digit total.
+__global__ __launch_bounds__(ZS_BLOCK) void k_radix_rowscan(uint32_t *__restrict__ hist, int nblk,
+                                                            uint32_t *__restrict__ total) {
+    __shared__ uint32_t part[ZS_BLOCK];
+    uint32_t *row = hist + (size_t)blockIdx.x * nblk;
     const int t = threadIdx.x;
-    const int chunk = (total + 1023) / 1024;
-    const int b = min(t * chunk, total), e = min(b + chunk, total);
+    const int chunk = (nblk + ZS_BLOCK - 1) / ZS_BLOCK;
+    const int b = min(t * chunk, nblk), e = min(b + chunk, nblk);
     uint32_t s = 0;
-    for (int i = b; i < e; ++i) s += a[i];
+    for (int i = b; i < e; ++i) s += row[i];
     part[t] = s;
     __syncthreads();
-    for (int off = 1; off < 1024; off <<= 1) {
+    for (int off = 1; off < ZS_BLOCK; off <<= 1) {
         const uint32_t v = (t >= off) ? part[t - off] : 0u;
         __syncthreads();
         part[t] += v;
         __syncthreads();
     }
-    uint32_t run = part[t] - s;  // exclusive prefix of this chunk
+    uint32_t run = part[t] - s;
     for (int i = b; i < e; ++i) {
-        const uint32_t v = a[i];
-        a[i] = run;
+        const uint32_t v = row[i];
+        row[i] = run;
         run += v;
     }
+    if (t == ZS_BLOCK - 1) total[blockIdx.x] = part[t];
 }
 
 template <int BITS>
@@ -113,14 +117,41 @@ __global__ __launch_bounds__(ZS_BLOCK) void k_radix_scatter(const uint32_t *__re
                                                             const int *__restrict__ vin,
                                                             uint32_t *__restrict__ kout,
                                                             int *__restrict__ vout, int n, int shift,
-                                                            const uint32_t *__restrict__ scanned, int nblk) {
+                                                            const uint32_t *__restrict__ scanned, int nblk,
+                                                            const uint32_t *__restrict__ total) {
     constexpr int R = 1 << BITS;
+    constexpr int PER = (R + ZS_BLOCK - 1) / ZS_BLOCK;
     __shared__ uint32_t cnt[RS_WAVES][R];
     __shared__ uint32_t goff[R];
-    for (int d = threadIdx.x; d < R; d += ZS_BLOCK) {
+    __shared__ uint32_t part[ZS_BLOCK];
+    // digit base = exclusive scan of the digit totals; thread t owns digits [t*PER, t*PER+PER)
+    {
+        const int t = threadIdx.x;
+        uint32_t s = 0;
 #pragma unroll
-        for (int w = 0; w < RS_WAVES; ++w) cnt[w][d] = 0;
-        goff[d] = scanned[d * nblk + blockIdx.x];
+        for (int k = 0; k < PER; ++k) {
+            const int d = t * PER + k;
+            if (d < R) s += total[d];
+        }
+        part[t] = s;
+        __syncthreads();
+        for (int off = 1; off < ZS_BLOCK; off <<= 1) {
+            const uint32_t v = (t >= off) ? part[t - off] : 0u;
+            __syncthreads();
+            part[t] += v;
+            __syncthreads();
+        }
+        uint32_t run = part[t] - s;
+#pragma unroll
+        for (int k = 0; k < PER; ++k) {
+            const int d = t * PER + k;
+            if (d < R) {
+                goff[d] = run + scanned[(size_t)d * nblk + blockIdx.x];
+                run += total[d];
+#pragma unroll
+                for (int w = 0; w < RS_WAVES; ++w) cnt[w][d] = 0;
+            }
+        }
     }
     __syncthreads();
     const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
@@ -215,19 +246,20 @@ void launch_hash(hipStream_t s, const Prm &q, const Buf &b, uint32_t *key, int *
 
 int radix_blocks(int n) { return (n + RS_TILE - 1) / RS_TILE; }
 
-size_t radix_hist_words(int n, int bits) { return ((size_t)1 << bits) * (size_t)radix_blocks(n); }
+size_t radix_hist_words(int n, int bits) { return ((size_t)1 << bits) * ((size_t)radix_blocks(n) + 1); }
 
 template <int BITS>
 static int sort_passes(hipStream_t s, int n, int keybits, uint32_t *key[2], int *val[2], uint32_t *hist,
                        bool identity_vals) {
     const int nblk = radix_blocks(n);
     const int R = 1 << BITS;
+    uint32_t *total = hist + (size_t)R * nblk;
     int src = 0;
     for (int shift = 0; shift < keybits; shift += BITS) {
         k_radix_hist<BITS><<<nblk, ZS_BLOCK, 0, s>>>(key[src], n, shift, hist, nblk);
-        k_radix_scan<<<1, 1024, 0, s>>>(hist, R * nblk);
+        k_radix_rowscan<<<R, ZS_BLOCK, 0, s>>>(hist, nblk, total);
         k_radix_scatter<BITS><<<nblk, ZS_BLOCK, 0, s>>>(key[src], (identity_vals && shift == 0) ? nullptr : val[src],
-                                                        key[1 - src], val[1 - src], n, shift, hist, nblk);
+                                                        key[1 - src], val[1 - src], n, shift, hist, nblk, total);
         src = 1 - src;
     }
     return src;
@@ -237,10 +269,10 @@ int launch_sort(hipStream_t s, int n, int keybits, int bits, uint32_t *key[2], i
                 bool identity_vals) {
     if (n == 0) return 0;
     switch (bits) {
-    case 4: return sort_passes<4>(s, n, keybits, key, val, hist, identity_vals);
-    case 6: return sort_passes<6>(s, n, keybits, key, val, hist, identity_vals);
-    case 7: return sort_passes<7>(s, n, keybits, key, val, hist, identity_vals);
     case 8: return sort_passes<8>(s, n, keybits, key, val, hist, identity_vals);
+    case 9: return sort_passes<9>(s, n, keybits, key, val, hist, identity_vals);
+    case 10: return sort_passes<10>(s, n, keybits, key, val, hist, identity_vals);
+    case 11: return sort_passes<11>(s, n, keybits, key, val, hist, identity_vals);
     default: return -1;
     }
 }

@@ -28,19 +28,14 @@ __global__ __launch_bounds__(ZS_BLOCK) void k_wall(Prm q, Buf b, const uint32_t 
     if (b.type[i] == ZSPH_FLUID) return;
     const float4 xi = b.posr[i];
     float P = 0.0f, S = 0.0f;
-    for_each_candidate(q, skey[i], cs, [&](int j) {
+    for_each_neighbor(q, i, xi, skey[i], cs, b.posr, [&](int j, float dx, float dy, float dz, float r2, float4 xj) {
         if (b.type[j] != ZSPH_FLUID) return;
-        const float4 xj = b.posr[j];
-        const float dx = xi.x - xj.x, dy = xi.y - xj.y, dz = xi.z - xj.z;
-        const float r2 = dist2(dx, dy, dz);
-        if (r2 <= q.rc2) {
-            const float qq = sqrtf(r2) * q.inv_h;
-            const float t = 1.0f - 0.5f * qq;
-            const float W = q.adh * t * t * t * t * (2.0f * qq + 1.0f);
-            const float pj = b.velp[j].w;
-            P += (pj + xj.w * q.g * dz) * W;
-            S += W;
-        }
+        const float qq = sqrtf(r2) * q.inv_h;
+        const float t = 1.0f - 0.5f * qq;
+        const float W = q.adh * t * t * t * t * (2.0f * qq + 1.0f);
+        const float pj = b.velp[j].w;
+        P += (pj + xj.w * q.g * dz) * W;
+        S += W;
     });
     float p = (fabsf(S) > 1e-8f) ? P / S : 0.0f;
     p = fmaxf(p, 0.0f);
@@ -65,27 +60,21 @@ __global__ __launch_bounds__(ZS_BLOCK) void k_ldg(Prm q, Buf b, const uint32_t *
     const float4 vi = b.velp[i];
     float lxx = 0, lxy = 0, lxz = 0, lyy = 0, lyz = 0, lzz = 0;
     float gx = 0, gy = 0, gz = 0, D = 0;
-    for_each_candidate(q, skey[i], cs, [&](int j) {
-        if (j == i) return;
-        const float4 xj = b.posr[j];
-        const float dx = xi.x - xj.x, dy = xi.y - xj.y, dz = xi.z - xj.z;
-        const float r2 = dist2(dx, dy, dz);
-        if (r2 <= q.rc2) {
-            const float qq = sqrtf(r2) * q.inv_h;
-            const float t = 1.0f - 0.5f * qq;
-            const float F = q.fk * t * t * t;
-            const float FV = F * q.m / xj.w;
-            if (delta) {
-                lxx -= FV * dx * dx; lxy -= FV * dx * dy; lxz -= FV * dx * dz;
-                lyy -= FV * dy * dy; lyz -= FV * dy * dz; lzz -= FV * dz * dz;
-                const float c = FV * (xj.w - xi.w);
-                gx += c * dx; gy += c * dy; gz += c * dz;
-            }
-            if (b.type[j] == ZSPH_FLUID) {
-                const float4 vj = b.velp[j];
-                const float dvdx = (vi.x - vj.x) * dx + (vi.y - vj.y) * dy + (vi.z - vj.z) * dz;
-                D -= FV * dvdx;
-            }
+    for_each_neighbor(q, i, xi, skey[i], cs, b.posr, [&](int j, float dx, float dy, float dz, float r2, float4 xj) {
+        const float qq = sqrtf(r2) * q.inv_h;
+        const float t = 1.0f - 0.5f * qq;
+        const float F = q.fk * t * t * t;
+        const float FV = F * q.m / xj.w;
+        if (delta) {
+            lxx -= FV * dx * dx; lxy -= FV * dx * dy; lxz -= FV * dx * dz;
+            lyy -= FV * dy * dy; lyz -= FV * dy * dz; lzz -= FV * dz * dz;
+            const float c = FV * (xj.w - xi.w);
+            gx += c * dx; gy += c * dy; gz += c * dz;
+        }
+        if (b.type[j] == ZSPH_FLUID) {
+            const float4 vj = b.velp[j];
+            const float dvdx = (vi.x - vj.x) * dx + (vi.y - vj.y) * dy + (vi.z - vj.z) * dz;
+            D -= FV * dvdx;
         }
     });
     if (delta) {
@@ -131,38 +120,32 @@ __global__ __launch_bounds__(ZS_BLOCK) void k_governing(Prm q, Buf b, const uint
     const float4 gi = gd[i];
     const float rhoi = xi.w, pi = vi.w;
     float ax = 0, ay = 0, az = 0, dr = 0;
-    for_each_candidate(q, skey[i], cs, [&](int j) {
-        if (j == i) return;
+    for_each_neighbor(q, i, xi, skey[i], cs, b.posr, [&](int j, float dx, float dy, float dz, float r2, float4 xj) {
         const int tj = b.type[j];
         const bool fj = tj == ZSPH_FLUID;
         if (!fi && !fj) return;
-        const float4 xj = b.posr[j];
-        const float dx = xi.x - xj.x, dy = xi.y - xj.y, dz = xi.z - xj.z;
-        const float r2 = dist2(dx, dy, dz);
-        if (r2 <= q.rc2) {
-            const float qq = sqrtf(r2) * q.inv_h;
-            const float t = 1.0f - 0.5f * qq;
-            const float F = q.fk * t * t * t;
-            const float4 vj = b.velp[j];
-            const float rhoj = xj.w, pj = vj.w;
-            const float Vj = q.m / rhoj;
-            const float dvdx = (vi.x - vj.x) * dx + (vi.y - vj.y) * dy + (vi.z - vj.z) * dz;
-            dr += rhoi * dvdx * F * Vj;
-            const float irr = 1.0f / (rhoi * rhoj);
-            float pt = -(pi + pj) * q.m * F * irr;
-            if (fi && fj) {
-                const float ire = 1.0f / (r2 + q.eta2);
-                if (delta) {
-                    const float4 gj = gd[j];
-                    const float psi = 2.0f * (rhoi - rhoj) * r2 * ire -
-                                      ((gi.x + gj.x) * dx + (gi.y + gj.y) * dy + (gi.z + gj.z) * dz);
-                    dr += q.delta_h_cs * F * psi * Vj;
-                }
-                pt += q.h_cs_rho_m * F * (gi.w + gd[j].w + q.alpha * dvdx * ire) * irr;
+        const float qq = sqrtf(r2) * q.inv_h;
+        const float t = 1.0f - 0.5f * qq;
+        const float F = q.fk * t * t * t;
+        const float4 vj = b.velp[j];
+        const float rhoj = xj.w, pj = vj.w;
+        const float Vj = q.m / rhoj;
+        const float dvdx = (vi.x - vj.x) * dx + (vi.y - vj.y) * dy + (vi.z - vj.z) * dz;
+        dr += rhoi * dvdx * F * Vj;
+        const float irr = 1.0f / (rhoi * rhoj);
+        float pt = -(pi + pj) * q.m * F * irr;
+        if (fi && fj) {
+            const float ire = 1.0f / (r2 + q.eta2);
+            const float4 gj = gd[j];
+            if (delta) {
+                const float psi = 2.0f * (rhoi - rhoj) * r2 * ire -
+                                  ((gi.x + gj.x) * dx + (gi.y + gj.y) * dy + (gi.z + gj.z) * dz);
+                dr += q.delta_h_cs * F * psi * Vj;
             }
-            if (!delta) dr += (rhoi - rhoj) * r2 * F * q.delta_h_cs / (r2 + q.eta2) * Vj;
-            ax += pt * dx; ay += pt * dy; az += pt * dz;
+            pt += q.h_cs_rho_m * F * (gi.w + gj.w + q.alpha * dvdx * ire) * irr;
         }
+        if (!delta) dr += (rhoi - rhoj) * r2 * F * q.delta_h_cs / (r2 + q.eta2) * Vj;
+        ax += pt * dx; ay += pt * dy; az += pt * dz;
     });
     if (fi) az += q.g;
     acc[i] = make_float4(ax, ay, az, dr);
@@ -217,14 +200,9 @@ __global__ __launch_bounds__(ZS_BLOCK) void k_nbr_stats(Prm q, Buf b, const uint
     const float4 xi = b.posr[i];
     int c = 0;
     uint64_t h = 0;
-    for_each_candidate(q, skey[i], cs, [&](int j) {
-        if (j == i) return;
-        const float4 xj = b.posr[j];
-        const float r2 = dist2(xi.x - xj.x, xi.y - xj.y, xi.z - xj.z);
-        if (r2 <= q.rc2) {
-            ++c;
-            h += mix64((uint64_t)j);
-        }
+    for_each_neighbor(q, i, xi, skey[i], cs, b.posr, [&](int j, float, float, float, float, float4) {
+        ++c;
+        h += mix64((uint64_t)j);
     });
     count[i] = c;
     digest[i] = h;

@@ -5,6 +5,7 @@
 
 #include <cstdarg>
 #include <cstdio>
+#include <algorithm>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -200,7 +201,7 @@ int zsph_create(const zsph_params *p, int device, zsph_ctx **out) {
         if (device >= ndev) { delete c; return ZSPH_E_NODEV; }
         if (hipSetDevice(device) != hipSuccess) { delete c; return ZSPH_E_HIP; }
     }
-    hipGetDevice(&c->device);
+    (void)hipGetDevice(&c->device);
     int r = ZSPH_OK;
     auto fail = [&](int code) {
         zsph_destroy(c);
@@ -218,8 +219,11 @@ int zsph_create(const zsph_params *p, int device, zsph_ctx **out) {
     int kb = 1;
     while (kb < 31 && ((long long)1 << kb) < G) ++kb;
     c->keybits = kb;
+    // fewest passes of at most 11-bit digits, spread evenly (19 bits: 2 x 10)
+    const int passes = (kb + 10) / 11;
+    c->radix_bits = std::max(8, (kb + passes - 1) / passes);
     if ((r = dalloc(c, &c->cell_start, (size_t)G + 1)) || (r = dalloc(c, &c->acc, N)) ||
-        (r = dalloc(c, &c->gd, N)) || (r = dalloc(c, &c->hist, zs::radix_hist_words((int)N, 8))) ||
+        (r = dalloc(c, &c->gd, N)) || (r = dalloc(c, &c->hist, zs::radix_hist_words((int)N, 11))) ||
         (r = dalloc(c, &c->killed, 1)))
         return fail(r);
     *out = c;
@@ -228,11 +232,11 @@ int zsph_create(const zsph_params *p, int device, zsph_ctx **out) {
 
 void zsph_destroy(zsph_ctx *c) {
     if (!c) return;
-    if (c->stream) hipStreamSynchronize(c->stream);
-    if (c->graph_exec) hipGraphExecDestroy(c->graph_exec);
-    if (c->graph) hipGraphDestroy(c->graph);
-    for (void *d : c->allocs) hipFree(d);
-    if (c->stream) hipStreamDestroy(c->stream);
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    if (c->graph_exec) (void)hipGraphExecDestroy(c->graph_exec);
+    if (c->graph) (void)hipGraphDestroy(c->graph);
+    for (void *d : c->allocs) (void)hipFree(d);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
 }
 
@@ -355,7 +359,7 @@ int zsph_step(zsph_ctx *c, int nsteps) {
         hipGraph_t g = nullptr;
         hipError_t e = hipStreamEndCapture(c->stream, &g);
         if (r) {
-            if (g) hipGraphDestroy(g);
+            if (g) (void)hipGraphDestroy(g);
             return r;
         }
         if (e != hipSuccess) return set_err(c, ZSPH_E_HIP, "graph capture failed: %s", hipGetErrorString(e));
