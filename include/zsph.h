@@ -286,6 +286,25 @@ int zsph_neighbor_stats(zsph_ctx *ctx, int buf, int *count, uint64_t *digest);
 int zsph_time_steps(zsph_ctx *ctx, int nsteps, float *ms);
 int zsph_profile_steps(zsph_ctx *ctx, int nsteps, float *stage_ms);
 
+/* Run diagnostics of buffer `buf` (no counterpart in the reference, which
+ * only prints timings).  Sums are accumulated in FP64 per block and added
+ * in a fixed order, so repeated calls give identical results. */
+typedef struct zsph_diag {
+    int n_fluid;        /* particles of type 1                              */
+    int n_killed;       /* removed since zsph_create                        */
+    double mass_fluid;  /* n_fluid * m                                      */
+    double front_x;     /* max x over fluid particles with z < front_zmax   */
+    double zmax_fluid;  /* max z over fluid particles                       */
+    double kinetic;     /* sum 1/2 m |v|^2 over fluid                       */
+    double momentum[3]; /* sum m v over fluid                               */
+} zsph_diag;
+int zsph_diagnostics(zsph_ctx *ctx, int buf, float front_zmax, zsph_diag *out);
+
+/* Fluid pressure at `npts` points (xyz triples): Shepard average
+ * sum p_j W / sum W over fluid particles within 2h; 0 where there are none.
+ * Buffer `buf` must be in cell order (after zsph_sort_data or zsph_step). */
+int zsph_probe_pressure(zsph_ctx *ctx, int buf, const float *pts, int npts, float *p_out);
+
 /* Number of particles removed by zsph_hash since zsph_create. */
 int zsph_killed(zsph_ctx *ctx, int *killed);
 

@@ -47,6 +47,8 @@ def lib():
         for n in ("orc_kernel_W", "orc_kernel_F", "orc_eos_p", "orc_eos_rho"):
             sig(n, C.c_double, [P(abi.Params), C.c_double])
         sig("orc_cell", C.c_int, [C.c_float, C.c_float, C.c_float, C.c_int])
+        sig("orc_diagnostics", C.c_int, [vp, C.c_int, C.c_float, P(abi.Diag)])
+        sig("orc_probe_pressure", C.c_int, [vp, C.c_int, P(C.c_float), C.c_int, P(C.c_double)])
         _LIB = L
     return _LIB
 
@@ -122,6 +124,19 @@ class Oracle:
         a = np.ascontiguousarray(arr, dtype=np.float64).ravel()
         self._ok(lib().orc_array_set(self._h, which, buf, a.ctypes.data_as(C.POINTER(C.c_double)), a.size),
                  "array_set")
+
+    def diagnostics(self, buf=0, front_zmax=float("inf")):
+        from zsph_amd import _abi as abi
+        d = abi.Diag()
+        self._ok(lib().orc_diagnostics(self._h, buf, front_zmax, C.byref(d)), "diagnostics")
+        return d.as_dict()
+
+    def probe_pressure(self, points, buf=0):
+        pts = np.ascontiguousarray(points, np.float32).reshape(-1, 3)
+        out = np.zeros(len(pts), np.float64)
+        self._ok(lib().orc_probe_pressure(self._h, buf, pts.ctypes.data_as(C.POINTER(C.c_float)), len(pts),
+                                          out.ctypes.data_as(C.POINTER(C.c_double))), "probe_pressure")
+        return out
 
     def neighbor_stats(self, buf=1, brute=False):
         cnt = np.empty(self.n, np.int32)

@@ -154,6 +154,11 @@ int orc_neighbor_stats(void *h, int b, int brute, int *count, uint64_t *digest) 
     return ORC_CALL(o.nbr_stats(b, brute != 0, count, digest));
 }
 
+int orc_diagnostics(void *h, int b, float front_zmax, zsph_diag *d) { return ORC_CALL(o.diagnostics(b, front_zmax, d)); }
+int orc_probe_pressure(void *h, int b, const float *pts, int np, double *out) {
+    return ORC_CALL(o.probes(b, pts, np, out));
+}
+
 // Scalar kernels of the oracle, exposed for closed-form tests (a particle-
 // free oracle carries only the constants).
 static orc::Oracle<double> lite(const zsph_params *p) {

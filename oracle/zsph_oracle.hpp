@@ -405,6 +405,46 @@ struct Oracle {
         }
     }
 
+    // Run diagnostics (count, kinetic energy, momentum, front, top) of buffer bi.
+    void diagnostics(int bi, float front_zmax, zsph_diag *d) const {
+        const Buffer<R> &b = buf[bi];
+        std::memset(d, 0, sizeof(*d));
+        d->front_x = d->zmax_fluid = -INFINITY;
+        const double m = p.m;
+        for (int i = 0; i < N; ++i) {
+            if (b.type[i] != ZSPH_FLUID) continue;
+            ++d->n_fluid;
+            const double u[3] = {(double)b.velp[4 * i], (double)b.velp[4 * i + 1], (double)b.velp[4 * i + 2]};
+            d->kinetic += 0.5 * m * (u[0] * u[0] + u[1] * u[1] + u[2] * u[2]);
+            for (int c = 0; c < 3; ++c) d->momentum[c] += m * u[c];
+            const double x = (double)b.posr[4 * i], z = (double)b.posr[4 * i + 2];
+            if ((float)z < front_zmax) d->front_x = std::max(d->front_x, x);
+            d->zmax_fluid = std::max(d->zmax_fluid, z);
+        }
+        d->mass_fluid = d->n_fluid * m;
+        d->n_killed = (int)killed;
+    }
+    // Shepard fluid pressure at points (brute force over all particles).
+    void probes(int bi, const float *pts, int np, double *out) const {
+        const Buffer<R> &b = buf[bi];
+        for (int k = 0; k < np; ++k) {
+            R pw = 0, ws = 0;
+            for (int j = 0; j < N; ++j) {
+                if (b.type[j] != ZSPH_FLUID) continue;
+                const float dx = pts[3 * k] - (float)b.posr[4 * j];
+                const float dy = pts[3 * k + 1] - (float)b.posr[4 * j + 1];
+                const float dz = pts[3 * k + 2] - (float)b.posr[4 * j + 2];
+                if (!(r2_f32(dx, dy, dz) <= p.rc2)) continue;
+                const R ex = R(pts[3 * k]) - b.posr[4 * j], ey = R(pts[3 * k + 1]) - b.posr[4 * j + 1],
+                        ez = R(pts[3 * k + 2]) - b.posr[4 * j + 2];
+                const R W = kernel_W(std::sqrt(ex * ex + ey * ey + ez * ez));
+                pw += b.velp[4 * j + 3] * W;
+                ws += W;
+            }
+            out[k] = (ws > R(1e-8)) ? (double)(pw / ws) : 0.0;
+        }
+    }
+
     static uint64_t mix64(uint64_t z) {
         z += 0x9E3779B97F4A7C15ull;
         z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;

@@ -241,6 +241,18 @@ class Solver:
         self._chk(lib().zsph_profile_steps(self._ctx, nsteps, out), "zsph_profile_steps")
         return dict(zip(abi.STAGES, list(out)))
 
+    def diagnostics(self, buf=0, front_zmax=float("inf")):
+        d = abi.Diag()
+        self._chk(lib().zsph_diagnostics(self._ctx, buf, front_zmax, C.byref(d)), "zsph_diagnostics")
+        return d.as_dict()
+
+    def probe_pressure(self, points, buf=0):
+        pts = np.ascontiguousarray(points, np.float32).reshape(-1, 3)
+        out = np.zeros(len(pts), np.float32)
+        self._chk(lib().zsph_probe_pressure(self._ctx, buf, _ptr(pts, C.c_float), len(pts), _ptr(out, C.c_float)),
+                  "zsph_probe_pressure")
+        return out
+
     def killed(self):
         k = C.c_int(0)
         self._chk(lib().zsph_killed(self._ctx, C.byref(k)), "zsph_killed")

@@ -72,6 +72,16 @@ class HostState(C.Structure):
                 ("hash", C.POINTER(_i))]
 
 
+class Diag(C.Structure):
+    _fields_ = [("n_fluid", _i), ("n_killed", _i), ("mass_fluid", C.c_double), ("front_x", C.c_double),
+                ("zmax_fluid", C.c_double), ("kinetic", C.c_double), ("momentum", C.c_double * 3)]
+
+    def as_dict(self):
+        return {"n_fluid": self.n_fluid, "n_killed": self.n_killed, "mass_fluid": self.mass_fluid,
+                "front_x": self.front_x, "zmax_fluid": self.zmax_fluid, "kinetic": self.kinetic,
+                "momentum": list(self.momentum)}
+
+
 class DamBreak(C.Structure):
     _fields_ = [("dim", _i), ("dx", _f), ("lx", _f), ("wy", _f), ("lz", _f), ("cw", _f), ("ch", _f),
                 ("layers", _i), ("wall_height", _f)]
@@ -121,6 +131,8 @@ def load(path):
     _sig(lib, "zsph_neighbor_stats", _i, [vp, _i, P(_i), P(C.c_uint64)])
     _sig(lib, "zsph_killed", _i, [vp, P(_i)])
     _sig(lib, "zsph_time_steps", _i, [vp, _i, P(_f)])
+    _sig(lib, "zsph_diagnostics", _i, [vp, _i, _f, P(Diag)])
+    _sig(lib, "zsph_probe_pressure", _i, [vp, _i, P(_f), _i, P(_f)])
     _sig(lib, "zsph_profile_steps", _i, [vp, _i, P(_f)])
     return lib
 
@@ -132,7 +144,8 @@ EXPORTS = (
     "zsph_destroy", "zsph_sync", "zsph_last_error", "zsph_upload", "zsph_download", "zsph_hash",
     "zsph_sort_index", "zsph_sort_data", "zsph_boundary", "zsph_governing", "zsph_prediction",
     "zsph_correction", "zsph_step", "zsph_array_get", "zsph_array_set", "zsph_neighbor_stats",
-    "zsph_killed", "zsph_sizeof", "zsph_time_steps", "zsph_profile_steps",
+    "zsph_killed", "zsph_sizeof", "zsph_time_steps", "zsph_profile_steps", "zsph_diagnostics",
+    "zsph_probe_pressure",
 )
 
 STAGES = ("hash", "sort", "cells", "reorder", "wall", "ldg", "governing", "integrate")

@@ -29,4 +29,10 @@ void launch_correct(hipStream_t s, const Prm &q, const Buf &b, const float4 *acc
 void launch_nbr_stats(hipStream_t s, const Prm &q, const Buf &b, const uint32_t *skey, const int *cs, int *count,
                       uint64_t *digest);
 
+// diag.hip
+constexpr int DIAG_D = 5;  // per-block FP64 partials: count, kinetic, momentum x/y/z
+int diag_blocks(int n);
+void launch_diag(hipStream_t s, const Prm &q, const Buf &b, float front_zmax, double *part, float *pmax);
+void launch_probe(hipStream_t s, const Prm &q, const Buf &b, const float *pts, int npts, float *out);
+
 }  // namespace zs

@@ -6,6 +6,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <algorithm>
+#include <cmath>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -29,6 +30,8 @@ struct zsph_ctx {
     float4 *gd = nullptr;
     uint32_t *hist = nullptr;
     int *killed = nullptr;
+    double *diag_part = nullptr;  // per-block diagnostics partials
+    float *diag_max = nullptr;
     int keybits = 1;
     int radix_bits = 8;
     hipGraph_t graph = nullptr;
@@ -224,7 +227,9 @@ int zsph_create(const zsph_params *p, int device, zsph_ctx **out) {
     c->radix_bits = std::max(8, (kb + passes - 1) / passes);
     if ((r = dalloc(c, &c->cell_start, (size_t)G + 1)) || (r = dalloc(c, &c->acc, N)) ||
         (r = dalloc(c, &c->gd, N)) || (r = dalloc(c, &c->hist, zs::radix_hist_words((int)N, 11))) ||
-        (r = dalloc(c, &c->killed, 1)))
+        (r = dalloc(c, &c->killed, 1)) ||
+        (r = dalloc(c, &c->diag_part, (size_t)zs::diag_blocks((int)N) * zs::DIAG_D)) ||
+        (r = dalloc(c, &c->diag_max, (size_t)zs::diag_blocks((int)N) * 2)))
         return fail(r);
     *out = c;
     return ZSPH_OK;
@@ -482,6 +487,58 @@ int zsph_profile_steps(zsph_ctx *c, int nsteps, float *stage_ms) {
     }
     if (r) return r;
     if (e != hipSuccess) return set_err(c, ZSPH_E_HIP, "profile: %s", hipGetErrorString(e));
+    return ZSPH_OK;
+}
+
+int zsph_diagnostics(zsph_ctx *c, int b, float front_zmax, zsph_diag *out) {
+    if (!c || !out || !valid_buf(b)) return set_err(c, ZSPH_E_ARG, "zsph_diagnostics: bad argument");
+    std::memset(out, 0, sizeof(*out));
+    out->front_x = out->zmax_fluid = -INFINITY;
+    const int nb = c->q.N ? zs::diag_blocks(c->q.N) : 0;
+    std::vector<double> part((size_t)nb * zs::DIAG_D);
+    std::vector<float> mx((size_t)nb * 2);
+    zs::launch_diag(c->stream, c->q, c->buf[b], front_zmax, c->diag_part, c->diag_max);
+    int r = check_launch(c, "diagnostics");
+    if (r) return r;
+    if (nb) {
+        HIPCHK(c, hipMemcpyAsync(part.data(), c->diag_part, part.size() * sizeof(double), hipMemcpyDeviceToHost,
+                                 c->stream));
+        HIPCHK(c, hipMemcpyAsync(mx.data(), c->diag_max, mx.size() * sizeof(float), hipMemcpyDeviceToHost, c->stream));
+    }
+    HIPCHK(c, hipMemcpyAsync(&out->n_killed, c->killed, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    double cnt = 0.0;
+    for (int k = 0; k < nb; ++k) {
+        const double *v = &part[(size_t)k * zs::DIAG_D];
+        cnt += v[0];
+        out->kinetic += v[1];
+        out->momentum[0] += v[2];
+        out->momentum[1] += v[3];
+        out->momentum[2] += v[4];
+        out->front_x = std::max(out->front_x, (double)mx[2 * k]);
+        out->zmax_fluid = std::max(out->zmax_fluid, (double)mx[2 * k + 1]);
+    }
+    out->n_fluid = (int)cnt;
+    out->mass_fluid = cnt * (double)c->p.m;
+    return ZSPH_OK;
+}
+
+int zsph_probe_pressure(zsph_ctx *c, int b, const float *pts, int npts, float *p_out) {
+    if (!c || !valid_buf(b) || npts < 0 || (npts && (!pts || !p_out)))
+        return set_err(c, ZSPH_E_ARG, "zsph_probe_pressure: bad argument");
+    if (npts == 0) return ZSPH_OK;
+    float *d = nullptr;
+    HIPCHK(c, hipMalloc(&d, sizeof(float) * 4 * (size_t)npts));
+    hipError_t e = hipMemcpyAsync(d, pts, sizeof(float) * 3 * (size_t)npts, hipMemcpyHostToDevice, c->stream);
+    if (e == hipSuccess) {
+        zs::launch_probe(c->stream, c->q, c->buf[b], d, npts, d + 3 * (size_t)npts);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(p_out, d + 3 * (size_t)npts, sizeof(float) * npts, hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    (void)hipFree(d);
+    if (e != hipSuccess) return set_err(c, ZSPH_E_HIP, "probe pressure failed: %s", hipGetErrorString(e));
     return ZSPH_OK;
 }
 
