@@ -187,6 +187,36 @@ int zsph_read_vtk(const char *path, zsph_host_state *st, int n, float rho0);
 int zsph_vtk_count(const char *path, int *n_out);
 
 /* ------------------------------------------------------------------------ */
+/* Rigid body (6-DOF), FP64                                                  */
+/* ------------------------------------------------------------------------ */
+
+/* The reference's rigid_t (Test_6DOF/header/data_structure.hpp:36-50) in
+ * double precision.  q is scalar-first; I is the inertia tensor about the
+ * centre of mass in the body frame (row-major); F, T are the applied force
+ * and torque about c in the world frame. */
+typedef struct zsph_rigid {
+    double c[3], vel[3], omega[3], q[4];
+    double F[3], T[3], acc[3], alpha[3];
+    double I[9];
+    double mass;
+} zsph_rigid;
+
+/* Test_6DOF/equation/rigid_info.cpp:3-39 gravity_center + inertia_tensor:
+ * c = mean of the n points, I = (mass/n) sum(|r|^2 1 - r r^T); sets r->c,
+ * r->I, r->mass, leaves the rest untouched. */
+int zsph_rigid_mass_properties(const float *pts, int n, double mass, zsph_rigid *r);
+
+/* Test_6DOF/equation/acc_alpha.cpp:3-35 cal_acc_alpha: a = F/M,
+ * alpha = R I^-1 R^T (T - w x (R I R^T w)). */
+void zsph_rigid_acc_alpha(zsph_rigid *r);
+
+/* Test_6DOF/equation/time_int.cpp:3-22 time_int: explicit Euler of the
+ * points (x += (v + w x (x - c)) dt), then q += 1/2 (w (x) q) dt with
+ * renormalisation, c += v dt, v += a dt, w += alpha dt; acc and alpha are
+ * reset to 0.  pts may be NULL (n = 0). */
+void zsph_rigid_euler_step(zsph_rigid *r, float *pts, int n, double dt);
+
+/* ------------------------------------------------------------------------ */
 /* Device context                                                            */
 /* ------------------------------------------------------------------------ */
 
@@ -309,7 +339,8 @@ int zsph_probe_pressure(zsph_ctx *ctx, int buf, const float *pts, int npts, floa
 int zsph_killed(zsph_ctx *ctx, int *killed);
 
 /* sizeof of the ABI structs, so bindings can check their mirrors:
- * 0 zsph_json, 1 zsph_cpu_param, 2 zsph_params, 3 zsph_host_state, 4 zsph_dambreak. */
+ * 0 zsph_json, 1 zsph_cpu_param, 2 zsph_params, 3 zsph_host_state, 4 zsph_dambreak,
+ * 5 zsph_rigid, 6 zsph_diag. */
 size_t zsph_sizeof(int which);
 
 #ifdef __cplusplus

@@ -82,6 +82,12 @@ class Diag(C.Structure):
                 "momentum": list(self.momentum)}
 
 
+class Rigid(C.Structure):
+    _fields_ = [(n, C.c_double * 3) for n in ("c", "vel", "omega")] + [("q", C.c_double * 4)] + \
+               [(n, C.c_double * 3) for n in ("F", "T", "acc", "alpha")] + \
+               [("I", C.c_double * 9), ("mass", C.c_double)]
+
+
 class DamBreak(C.Structure):
     _fields_ = [("dim", _i), ("dx", _f), ("lx", _f), ("wy", _f), ("lz", _f), ("cw", _f), ("ch", _f),
                 ("layers", _i), ("wall_height", _f)]
@@ -103,7 +109,7 @@ def load(path):
     P = C.POINTER
     vp = C.c_void_p
     _sig(lib, "zsph_sizeof", C.c_size_t, [_i])
-    for which, cls in enumerate((Json, CpuParam, Params, HostState, DamBreak)):
+    for which, cls in enumerate((Json, CpuParam, Params, HostState, DamBreak, Rigid, Diag)):
         got = lib.zsph_sizeof(which)
         if got != C.sizeof(cls):
             raise RuntimeError(f"ABI mismatch: sizeof({cls.__name__}) = {C.sizeof(cls)} here, {got} in {path}")
@@ -115,6 +121,9 @@ def load(path):
     _sig(lib, "zsph_write_vtk", _i, [C.c_char_p, P(HostState), _i, _i, _i])
     _sig(lib, "zsph_read_vtk", _i, [C.c_char_p, P(HostState), _i, _f])
     _sig(lib, "zsph_vtk_count", _i, [C.c_char_p, P(_i)])
+    _sig(lib, "zsph_rigid_mass_properties", _i, [P(_f), _i, C.c_double, P(Rigid)])
+    _sig(lib, "zsph_rigid_acc_alpha", None, [P(Rigid)])
+    _sig(lib, "zsph_rigid_euler_step", None, [P(Rigid), P(_f), _i, C.c_double])
     _sig(lib, "zsph_create", _i, [P(Params), _i, P(vp)])
     _sig(lib, "zsph_destroy", None, [vp])
     _sig(lib, "zsph_sync", _i, [vp])
@@ -145,7 +154,7 @@ EXPORTS = (
     "zsph_sort_index", "zsph_sort_data", "zsph_boundary", "zsph_governing", "zsph_prediction",
     "zsph_correction", "zsph_step", "zsph_array_get", "zsph_array_set", "zsph_neighbor_stats",
     "zsph_killed", "zsph_sizeof", "zsph_time_steps", "zsph_profile_steps", "zsph_diagnostics",
-    "zsph_probe_pressure",
+    "zsph_probe_pressure", "zsph_rigid_mass_properties", "zsph_rigid_acc_alpha", "zsph_rigid_euler_step",
 )
 
 STAGES = ("hash", "sort", "cells", "reorder", "wall", "ldg", "governing", "integrate")

@@ -173,6 +173,8 @@ extern "C" size_t zsph_sizeof(int which) {
     case 2: return sizeof(zsph_params);
     case 3: return sizeof(zsph_host_state);
     case 4: return sizeof(zsph_dambreak);
+    case 5: return sizeof(zsph_rigid);
+    case 6: return sizeof(zsph_diag);
     default: return 0;
     }
 }
