@@ -63,7 +63,7 @@ def main():
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--dx", type=float, default=0.008)
-    ap.add_argument("--gsf", type=float, default=1.0, help="grid_size_factor: cell edge = 2h / gsf")
+    ap.add_argument("--gsf", type=float, default=2.0, help="grid_size_factor: cell edge = 2h / gsf")
     ap.add_argument("--profile-steps", type=int, default=3)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     args = ap.parse_args()
@@ -122,7 +122,7 @@ def main():
         "data": "synthetic (lattice dam break generated in-process)",
         "config": {"workload": f"3-D dam break 3.22x1.0x1.0 m, column 1.2x1.0x0.55 m, dx {args.dx} m, "
                                f"{case.n} particles ({case.json.water_ptc_num} fluid), delta-SPH Antuono, "
-                               f"Tait gamma 7, CFL 0.2",
+                               f"Tait gamma 7, CFL 0.2, cell edge 2h/{args.gsf:g}",
                    "particles": case.n, "fluid_particles": case.json.water_ptc_num,
                    "fluid_particle_steps_per_s": case.json.water_ptc_num * args.steps / wall,
                    "event_ms_per_step": ev_ms / args.steps, "removed_particles": killed},

@@ -107,7 +107,7 @@ class Oracle:
     def get(self, which, buf=0):
         from zsph_amd import A_CELL_START
         if which == A_CELL_START:
-            count = self.grid_num + 1
+            count = 2 * (self.grid_num + 1)
             shape = (count,)
         elif which in (A_ACC_SCALE, A_GD_SCALE):
             count, shape = 2 * self.n, (self.n, 2)

@@ -58,7 +58,7 @@ int download(orc::Oracle<R> &o, int bi, zsph_host_state *st, int fields) {
             for (int c = 0; c < 4; ++c) st->acc_drhodt[4 * i + c] = (float)o.acc[4 * i + c];
         if ((fields & ZSPH_F_TYPE) && st->type) st->type[i] = b.type[i];
         if ((fields & ZSPH_F_TABLE) && st->table) st->table[i] = b.table[i];
-        if ((fields & ZSPH_F_HASH) && st->hash) st->hash[i] = (int)o.key[i];
+        if ((fields & ZSPH_F_HASH) && st->hash) st->hash[i] = (int)(o.key[i] & (o.flag - 1u));
     }
     return ZSPH_OK;
 }

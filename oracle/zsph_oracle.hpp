@@ -62,6 +62,7 @@ template <class R>
 struct Oracle {
     zsph_params p{};
     int N = 0, G = 0, X = 1, Y = 1, Z = 1, k = 1;
+    uint32_t flag = 2;  // sort-key bit of non-fluid particles: smallest power of two >= G
     Buffer<R> buf[2];
     std::vector<uint32_t> key;  // unsorted after hash, sorted after sort
     std::vector<int> idx;       // permutation after sort
@@ -77,7 +78,8 @@ struct Oracle {
         G = X * Y * Z;
         k = p.grid_factor;
         buf[0].resize(N); buf[1].resize(N);
-        key.assign(N, 0); idx.assign(N, 0); cs.assign((size_t)G + 1, 0);
+        while (flag < (uint32_t)G) flag <<= 1;
+        key.assign(N, 0); idx.assign(N, 0); cs.assign(2 * ((size_t)G + 1), 0);
         acc.assign(4 * (size_t)N, R(0)); gd.assign(4 * (size_t)N, R(0));
         acc_scale.assign(2 * (size_t)N, R(0)); gd_scale.assign(2 * (size_t)N, R(0));
     }
@@ -116,7 +118,7 @@ struct Oracle {
     // around its cell, clipped to the grid, ascending within each x-run.
     template <class F>
     void walk(const Buffer<R> &b, int i, F &&f) const {
-        const uint32_t h = key[i];
+        const uint32_t h = key[i] & (flag - 1u);
         const int cz = (int)(h / (uint32_t)(X * Y));
         const int cy = (int)((h / (uint32_t)X) % (uint32_t)Y);
         const int cx = (int)(h % (uint32_t)X);
@@ -124,7 +126,8 @@ struct Oracle {
         for (int z = std::max(cz - k, 0); z <= std::min(cz + k, Z - 1); ++z)
             for (int y = std::max(cy - k, 0); y <= std::min(cy + k, Y - 1); ++y) {
                 const int row = (z * Y + y) * X;
-                for (int j = cs[row + x0]; j < cs[row + x1 + 1]; ++j) f(j);
+                for (int j = cs[row + x0]; j < cs[row + x1 + 1]; ++j) f(j);                    // fluid
+                for (int j = cs[G + 1 + row + x0]; j < cs[G + 1 + row + x1 + 1]; ++j) f(j);  // others
             }
         (void)b;
     }
@@ -147,13 +150,13 @@ struct Oracle {
                             cell_f32((float)b.posr[4 * i + 1], p.grid_ymin, p.grid_size, Y, &cy) &&
                             cell_f32((float)b.posr[4 * i + 2], p.grid_zmin, p.grid_size, Z, &cz);
             if (in) {
-                key[i] = (uint32_t)(cz * X * Y + cy * X + cx);
+                key[i] = (uint32_t)(cz * X * Y + cy * X + cx) | (b.type[i] == ZSPH_FLUID ? 0u : flag);
             } else {
                 b.type[i] = ZSPH_KILLED;
                 b.posr[4 * i] = R(p.grid_xmin); b.posr[4 * i + 1] = R(p.grid_ymin);
                 b.posr[4 * i + 2] = R(p.grid_zmin); b.posr[4 * i + 3] = R(p.rho0);
                 for (int c = 0; c < 4; ++c) b.velp[4 * i + c] = R(0);
-                key[i] = 0;
+                key[i] = flag;
                 ++kills;
             }
             idx[i] = i;
@@ -177,8 +180,11 @@ struct Oracle {
     }
     // sort.cu:8-84: cell c holds sorted positions [cs[c], cs[c+1]).
     void sort_data(int from, int to) {
-        for (int c = 0; c <= G; ++c)
-            cs[c] = (int)(std::lower_bound(key.begin(), key.end(), (uint32_t)c) - key.begin());
+        // fluid starts of cells 0..G, then the starts of the other particles
+        for (int e = 0; e <= 2 * G + 1; ++e) {
+            const uint32_t t = (e <= G) ? (uint32_t)e : (flag | (uint32_t)(e - G - 1));
+            cs[e] = (int)(std::lower_bound(key.begin(), key.end(), t) - key.begin());
+        }
         const Buffer<R> &a = buf[from];
         Buffer<R> &b = buf[to];
 #pragma omp parallel for schedule(static)

@@ -103,6 +103,9 @@ def test_hash_sort_cells_reorder():
     cell = [np.floor((pos[:, a] - np.float32(g)) / np.float32(p.grid_size)).astype(np.int64)
             for a, g in enumerate((p.grid_xmin, p.grid_ymin, p.grid_zmin))]
     want = cell[2] * p.grid_xdim * p.grid_ydim + cell[1] * p.grid_xdim + cell[0]
+    # sort key: the non-fluid flag (smallest power of two >= G) above the cell bits
+    flag = 1 << max(1, int(np.ceil(np.log2(p.grid_num))))
+    want = want + np.where(c.host.type == FLUID, 0, flag)
     assert np.array_equal(key, want)
     o.sort_index()
     perm = o.get(A_INDEX).astype(np.int64)
@@ -110,7 +113,10 @@ def test_hash_sort_cells_reorder():
     assert np.array_equal(o.get(A_KEY).astype(np.int64), want[perm])
     o.sort_data(0, 1)
     cs = o.get(A_CELL_START).astype(np.int64)
-    assert np.array_equal(cs, np.searchsorted(want[perm], np.arange(p.grid_num + 1), side="left"))
+    G = p.grid_num
+    targets = np.concatenate([np.arange(G + 1), flag + np.arange(G + 1)])
+    assert np.array_equal(cs, np.searchsorted(want[perm], targets, side="left"))
+    assert cs[G] == (c.host.type == FLUID).sum() and cs[2 * G + 1] == c.n
     posr = o.get(A_POSR, 1)
     assert np.array_equal(posr[:, :3].astype(np.float32), pos[perm])
     table = o.get(A_TABLE, 1).astype(np.int64)
@@ -127,7 +133,8 @@ def test_out_of_grid_particles_are_removed():
     assert o.killed() == 2
     t = o.get(A_TYPE)
     assert t[5] == KILLED and t[7] == KILLED
-    assert o.get(A_KEY)[5] == 0
+    flag = 1 << max(1, int(np.ceil(np.log2(c.cpu.grid_num))))
+    assert o.get(A_KEY)[5] == flag  # cell 0, sorted with the non-fluid particles
     posr = o.get(A_POSR)
     assert tuple(posr[5, :3].astype(np.float32)) == (np.float32(c.cpu.grid_xmin), np.float32(c.cpu.grid_ymin),
                                                       np.float32(c.cpu.grid_zmin))

@@ -210,7 +210,7 @@ class Solver:
     # raw arrays
     def get(self, which, buf=0):
         if which == A_CELL_START:
-            out = np.empty(self.grid_num + 1, np.int32)
+            out = np.empty(2 * (self.grid_num + 1), np.int32)
         else:
             w, dt = _ARRAY_SHAPES[which]
             out = np.empty((self.n, w) if w > 1 else self.n, dt)

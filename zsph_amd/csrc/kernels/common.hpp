@@ -22,6 +22,7 @@ struct Prm {
     int k;           // cells searched each side
     int X, Y, Z, XY;
     int G;           // number of cells
+    uint32_t flag;   // sort-key bit of non-fluid particles (1 << cell bits)
     int N;
     int dim;
     int diffusion;
@@ -65,6 +66,9 @@ inline Prm make_prm(const zsph_params &p) {
     q.Z = p.grid_zdim;
     q.XY = p.grid_xdim * p.grid_ydim;
     q.G = p.grid_xdim * p.grid_ydim * p.grid_zdim;
+    int cb = 1;
+    while (cb < 30 && (1 << cb) < q.G) ++cb;
+    q.flag = 1u << cb;
     q.N = p.ptc_num;
     q.dim = p.dim;
     q.diffusion = p.diffusion;
@@ -146,40 +150,73 @@ __device__ __forceinline__ CellBox cell_box(const Prm &q, uint32_t key) {
     return b;
 }
 
-// Visit every candidate j of the (2k+1)^3 cell box in ascending sorted
-// order within each x-run; f(j) decides on r^2 itself.  The x-run of
-// cells [xlo, xhi] in one (y,z) row is one contiguous particle range.
-template <class F>
-__device__ __forceinline__ void for_each_candidate(const Prm &q, uint32_t key, const int *__restrict__ cs,
-                                                   F &&f) {
-    const CellBox b = cell_box(q, key);
-    for (int cz = b.zlo; cz <= b.zhi; ++cz) {
-        for (int cy = b.ylo; cy <= b.yhi; ++cy) {
-            const int row = cz * q.XY + cy * q.X;
-            const int jb = cs[row + b.xlo];
-            const int je = cs[row + b.xhi + 1];
-            for (int j = jb; j < je; ++j) f(j);
-        }
-    }
-}
+// Sort keys put every fluid particle before every other particle: key =
+// cell | (type != fluid ? flag : 0), flag = 1 << cell bits.  The cell table
+// cs has 2 (G + 1) entries: fluid particles of cell c are [cs[c], cs[c+1]),
+// the others (dummies, removed) are [cs[G+1+c], cs[G+1+c+1]).  Fluid
+// particles therefore occupy [0, cs[G]) and dummies [cs[G], N).
 
-// Visit every neighbour j != i (FP32 r^2 <= rc2) of particle i at xi.
-// Each (y,z) cell row of the box is first trimmed to the x-cells that can
-// hold a point within the cut-off of xi (a row whose perpendicular distance
-// exceeds the cut-off is skipped); the trimming works in cell units with a
-// margin, so it only ever keeps extra candidates and the neighbour set is
-// decided by the r^2 predicate alone.  Candidates are tested 32 at a time
-// into a hit mask, then the hits are visited: the wave runs the interaction
-// body once per hit of its busiest lane, not once per candidate.
+// Per-lane queue of neighbour indices in LDS (QCAP entries per thread,
+// stored [slot][thread] so the lanes of a wave hit consecutive banks).
+constexpr int QCAP = 32;
+constexpr int QCHUNK = 8;  // candidates tested per lane between queue checks
+
+// Visit every neighbour j != i (FP32 r^2 <= rc2) of particle i at xi whose
+// segment is enabled (FLUID_J: j < cs[G]; DUMMY_J: j >= cs[G]).
+// * Each (y,z) cell row of the box is first trimmed to the x-cells that can
+//   hold a point within the cut-off of xi, and rows beyond the cut-off are
+//   skipped.  The trimming works in cell units with a margin, so it only ever
+//   keeps extra candidates; the r^2 predicate alone decides the set.
+// * Candidates are tested QCHUNK at a time per lane; hits go to the lane's
+//   LDS queue, and whenever every active lane holds at least 4 queued hits
+//   the wave processes 4 of each with all lanes busy.  The remainder is
+//   processed at the end.
 // f(j, dx, dy, dz, r2, xj) with x_ij = xi - xj.
-template <class F>
-__device__ __forceinline__ void for_each_neighbor(const Prm &q, int i, float4 xi, uint32_t key,
+template <bool FLUID_J, bool DUMMY_J, class F>
+__device__ __forceinline__ void for_each_neighbor(const Prm &q, int i, float4 xi, uint32_t cell,
                                                   const int *__restrict__ cs, const float4 *__restrict__ posr,
-                                                  F &&f) {
-    const CellBox b = cell_box(q, key);
+                                                  int *__restrict__ qbuf, F &&f) {
+    const CellBox b = cell_box(q, cell);
     const float fx = (xi.x - q.gx0) * q.inv_cell;
     const float fy = (xi.y - q.gy0) * q.inv_cell;
     const float fz = (xi.z - q.gz0) * q.inv_cell;
+    int *myq = qbuf + threadIdx.x;
+    uint32_t head = 0, tail = 0;
+    auto visit = [&](int j) {
+        const float4 xj = posr[j];
+        const float dx = xi.x - xj.x, dy = xi.y - xj.y, dz = xi.z - xj.z;
+        f(j, dx, dy, dz, dist2(dx, dy, dz), xj);
+    };
+    auto drain_all = [&]() {
+        while (head != tail) {
+            visit(myq[(head & (QCAP - 1)) * ZS_BLOCK]);
+            ++head;
+        }
+    };
+    auto scan = [&](int jb, int je) {
+        for (int base = jb; base < je; base += QCHUNK) {
+            if (__any(tail - head > (uint32_t)(QCAP - QCHUNK))) drain_all();
+#pragma unroll
+            for (int t = 0; t < QCHUNK; ++t) {
+                const int j = base + t;
+                if (j < je) {
+                    const float4 xj = posr[j];
+                    const float r2 = dist2(xi.x - xj.x, xi.y - xj.y, xi.z - xj.z);
+                    if (r2 <= q.rc2 && j != i) {
+                        myq[(tail & (QCAP - 1)) * ZS_BLOCK] = j;
+                        ++tail;
+                    }
+                }
+            }
+            while (__all(tail - head >= 4u)) {
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    visit(myq[(head & (QCAP - 1)) * ZS_BLOCK]);
+                    ++head;
+                }
+            }
+        }
+    };
     for (int cz = b.zlo; cz <= b.zhi; ++cz) {
         const float dzc = fmaxf(0.0f, fmaxf((float)cz - fz, fz - (float)(cz + 1)));
         const float remz = q.trim2 - dzc * dzc;
@@ -193,28 +230,11 @@ __device__ __forceinline__ void for_each_neighbor(const Prm &q, int i, float4 xi
             const int xb = min(b.xhi, (int)floorf(fx + w));
             if (xa > xb) continue;
             const int row = cz * q.XY + cy * q.X;
-            const int jb = cs[row + xa];
-            const int je = cs[row + xb + 1];
-            for (int base = jb; base < je; base += 32) {
-                const int cnt = min(32, je - base);
-                uint32_t m = 0;
-                for (int t = 0; t < cnt; ++t) {
-                    const float4 xj = posr[base + t];
-                    const float r2 = dist2(xi.x - xj.x, xi.y - xj.y, xi.z - xj.z);
-                    m |= (r2 <= q.rc2 ? 1u : 0u) << t;
-                }
-                if (i >= base && i < base + cnt) m &= ~(1u << (i - base));
-                while (m) {
-                    const int t = __builtin_ctz(m);
-                    m &= m - 1u;
-                    const int j = base + t;
-                    const float4 xj = posr[j];
-                    const float dx = xi.x - xj.x, dy = xi.y - xj.y, dz = xi.z - xj.z;
-                    f(j, dx, dy, dz, dist2(dx, dy, dz), xj);
-                }
-            }
+            if (FLUID_J) scan(cs[row + xa], cs[row + xb + 1]);
+            if (DUMMY_J) scan(cs[q.G + 1 + row + xa], cs[q.G + 1 + row + xb + 1]);
         }
     }
+    drain_all();
 }
 
 }  // namespace zs

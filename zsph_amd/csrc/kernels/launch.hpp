@@ -16,7 +16,8 @@ size_t radix_hist_words(int n, int bits);
 // val[0] is known to be 0..n-1 and is not read.
 int launch_sort(hipStream_t s, int n, int keybits, int bits, uint32_t *key[2], int *val[2], uint32_t *hist,
                 bool identity_vals);
-void launch_cell_start(hipStream_t s, const uint32_t *skey, int n, int G, int *cs);
+// cs[0..G]: fluid cell starts; cs[G+1..2G+1]: dummy cell starts (key | flag).
+void launch_cell_start(hipStream_t s, const uint32_t *skey, int n, int G, uint32_t flag, int *cs);
 void launch_reorder(hipStream_t s, int n, const int *perm, const Buf &a, const Buf &b);
 
 // physics.hip
@@ -24,8 +25,8 @@ void launch_wall(hipStream_t s, const Prm &q, const Buf &b, const uint32_t *skey
 void launch_ldg(hipStream_t s, const Prm &q, const Buf &b, const uint32_t *skey, const int *cs, float4 *gd);
 void launch_governing(hipStream_t s, const Prm &q, const Buf &b, const uint32_t *skey, const int *cs,
                       const float4 *gd, float4 *acc);
-void launch_predict(hipStream_t s, const Prm &q, const Buf &b, const float4 *acc);
-void launch_correct(hipStream_t s, const Prm &q, const Buf &b, const float4 *acc);
+void launch_predict(hipStream_t s, const Prm &q, const Buf &b, const int *cs, const float4 *acc);
+void launch_correct(hipStream_t s, const Prm &q, const Buf &b, const int *cs, const float4 *acc);
 void launch_nbr_stats(hipStream_t s, const Prm &q, const Buf &b, const uint32_t *skey, const int *cs, int *count,
                       uint64_t *digest);
 

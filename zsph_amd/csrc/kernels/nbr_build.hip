@@ -13,7 +13,8 @@ namespace zs {
 // ---------------------------------------------------------------------------
 // Cell hash.  H = (cz*Y + cy)*X + cx; a particle whose cell lies outside the
 // grid on any axis is removed: type 100, parked at the grid origin with
-// v = 0, (rho, p) = (rho0, 0) and H = 0 (hash.cu:16-23).
+// v = 0, (rho, p) = (rho0, 0) and H = 0 (hash.cu:16-23).  The sort key adds
+// the non-fluid flag above the cell bits, so fluid sorts first.
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(ZS_BLOCK) void k_hash(Prm q, Buf b, uint32_t *__restrict__ key,
                                                    int *__restrict__ idx, int *__restrict__ killed) {
@@ -25,15 +26,19 @@ __global__ __launch_bounds__(ZS_BLOCK) void k_hash(Prm q, Buf b, uint32_t *__res
                     cell_axis(x.y, q.gy0, q.grid_size, q.Y, &cy) &&
                     cell_axis(x.z, q.gz0, q.grid_size, q.Z, &cz);
     uint32_t h = 0;
+    int t;
     if (in) {
         h = (uint32_t)((cz * q.Y + cy) * q.X + cx);
+        t = b.type[i];
     } else {
+        t = ZSPH_KILLED;
         b.type[i] = ZSPH_KILLED;
         b.posr[i] = make_float4(q.gx0, q.gy0, q.gz0, q.rho0);
         b.velp[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        b.tvelp[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
         atomicAdd(killed, 1);
     }
-    key[i] = h;
+    key[i] = h | (t == ZSPH_FLUID ? 0u : q.flag);
     idx[i] = i;
 }
 
@@ -203,21 +208,24 @@ __global__ __launch_bounds__(ZS_BLOCK) void k_radix_scatter(const uint32_t *__re
 }
 
 // ---------------------------------------------------------------------------
-// Cell table: cs[c] = first sorted position with hash >= c (lower bound), so
-// cell c holds [cs[c], cs[c+1]) and empty cells have cs[c] == cs[c+1];
-// cs[G] = N.  Every entry is rewritten on every call.
+// Cell table: entry e < G+1 is the first sorted position with key >= e (the
+// fluid particles of cell c are [cs[c], cs[c+1]), so cs[G] = fluid count);
+// entry G+1+c is the first position with key >= flag | c (the dummies of
+// cell c; cs[2G+1] = N).  Empty cells have equal bounds.  Every entry is
+// rewritten on every call.
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(ZS_BLOCK) void k_cell_start(const uint32_t *__restrict__ skey, int n, int G,
-                                                         int *__restrict__ cs) {
-    const int c = blockIdx.x * ZS_BLOCK + threadIdx.x;
-    if (c > G) return;
+                                                         uint32_t flag, int *__restrict__ cs) {
+    const int e = blockIdx.x * ZS_BLOCK + threadIdx.x;
+    if (e > 2 * G + 1) return;
+    const uint32_t target = (e <= G) ? (uint32_t)e : (flag | (uint32_t)(e - G - 1));
     int lo = 0, hi = n;
     while (lo < hi) {
         const int mid = (lo + hi) >> 1;
-        if (skey[mid] < (uint32_t)c) lo = mid + 1;
+        if (skey[mid] < target) lo = mid + 1;
         else hi = mid;
     }
-    cs[c] = lo;
+    cs[e] = lo;
 }
 
 // Gather buffer `a` into cell order in buffer `b`; table'[k] = table[p] + p - k
@@ -277,8 +285,8 @@ int launch_sort(hipStream_t s, int n, int keybits, int bits, uint32_t *key[2], i
     }
 }
 
-void launch_cell_start(hipStream_t s, const uint32_t *skey, int n, int G, int *cs) {
-    k_cell_start<<<(G + 1 + ZS_BLOCK - 1) / ZS_BLOCK, ZS_BLOCK, 0, s>>>(skey, n, G, cs);
+void launch_cell_start(hipStream_t s, const uint32_t *skey, int n, int G, uint32_t flag, int *cs) {
+    k_cell_start<<<(2 * G + 2 + ZS_BLOCK - 1) / ZS_BLOCK, ZS_BLOCK, 0, s>>>(skey, n, G, flag, cs);
 }
 
 void launch_reorder(hipStream_t s, int n, const int *perm, const Buf &a, const Buf &b) {

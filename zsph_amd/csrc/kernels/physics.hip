@@ -6,6 +6,10 @@
 //   governing   <- equation/delta_ns.cu:179-267, equation/simple_ns.cu:99-180
 //   predict /
 //   correct     <- lib/time_int.cu:3-48, :50-91
+// Particles are in cell order with every fluid particle first: fluid
+// [0, Nf), dummies [Nf, N), Nf = cs[G].  Each kernel launches over the
+// segment it updates, and a neighbour's kind follows from its index, so no
+// type array is read inside a neighbour loop.
 // The reference's boundary kernel writes wall (rho, p) while other threads
 // of the same launch read them; here the wall pass is its own launch, and the
 // gradient pass runs only after every wall value is final.
@@ -18,25 +22,32 @@ static inline int nblocks(int n) { return (n + ZS_BLOCK - 1) / ZS_BLOCK; }
 
 __device__ __forceinline__ float *wptr(float4 *a, int i) { return reinterpret_cast<float *>(a + i) + 3; }
 
-// Dummy particles (type != 1): Shepard extrapolation of fluid pressure with
-// the hydrostatic correction, p_w = max(sum (p_j + rho_j g z_wj) W / sum W, 0),
-// rho_w from the inverse EOS (delta_ns.cu:81-85, :116-131).
+__device__ __forceinline__ uint32_t cell_of(const Prm &q, uint32_t key) { return key & (q.flag - 1u); }
+
+// Dummy particles: Shepard extrapolation of fluid pressure with the
+// hydrostatic and wall-acceleration correction,
+//   p_w = max(sum (p_j - rho_j (a_w - g) . x_wj) W / sum W, 0),
+// rho_w from the inverse EOS (delta_ns.cu:81-85, :116-131; a_w = 0 there).
+// a_w is the dummy's prescribed acceleration, kept in tvelp.xyz (zero for
+// fixed walls).
 __global__ __launch_bounds__(ZS_BLOCK) void k_wall(Prm q, Buf b, const uint32_t *__restrict__ skey,
                                                    const int *__restrict__ cs) {
-    const int i = xcd_block(gridDim.x) * ZS_BLOCK + threadIdx.x;
+    __shared__ int qbuf[QCAP * ZS_BLOCK];
+    const int nf = cs[q.G];
+    const int i = nf + xcd_block(gridDim.x) * ZS_BLOCK + threadIdx.x;
     if (i >= q.N) return;
-    if (b.type[i] == ZSPH_FLUID) return;
     const float4 xi = b.posr[i];
+    const float4 aw = b.tvelp[i];
     float P = 0.0f, S = 0.0f;
-    for_each_neighbor(q, i, xi, skey[i], cs, b.posr, [&](int j, float dx, float dy, float dz, float r2, float4 xj) {
-        if (b.type[j] != ZSPH_FLUID) return;
-        const float qq = sqrtf(r2) * q.inv_h;
-        const float t = 1.0f - 0.5f * qq;
-        const float W = q.adh * t * t * t * t * (2.0f * qq + 1.0f);
-        const float pj = b.velp[j].w;
-        P += (pj + xj.w * q.g * dz) * W;
-        S += W;
-    });
+    for_each_neighbor<true, false>(q, i, xi, cell_of(q, skey[i]), cs, b.posr, qbuf,
+                                   [&](int j, float dx, float dy, float dz, float r2, float4 xj) {
+                                       const float qq = sqrtf(r2) * q.inv_h;
+                                       const float t = 1.0f - 0.5f * qq;
+                                       const float W = q.adh * t * t * t * t * (2.0f * qq + 1.0f);
+                                       const float ax = aw.x * dx + aw.y * dy + (aw.z - q.g) * dz;
+                                       P += (b.velp[j].w - xj.w * ax) * W;
+                                       S += W;
+                                   });
     float p = (fabsf(S) > 1e-8f) ? P / S : 0.0f;
     p = fmaxf(p, 0.0f);
     *wptr(b.posr, i) = eos_rho(q, p);
@@ -49,9 +60,11 @@ __global__ __launch_bounds__(ZS_BLOCK) void k_wall(Prm q, Buf b, const uint32_t 
 // not depend on j).  MC form: D only.  Output gd = (G, D); dummies get 0.
 __global__ __launch_bounds__(ZS_BLOCK) void k_ldg(Prm q, Buf b, const uint32_t *__restrict__ skey,
                                                   const int *__restrict__ cs, float4 *__restrict__ gd) {
+    __shared__ int qbuf[QCAP * ZS_BLOCK];
+    const int nf = cs[q.G];
     const int i = xcd_block(gridDim.x) * ZS_BLOCK + threadIdx.x;
     if (i >= q.N) return;
-    if (b.type[i] != ZSPH_FLUID) {
+    if (i >= nf) {
         gd[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
         return;
     }
@@ -60,23 +73,22 @@ __global__ __launch_bounds__(ZS_BLOCK) void k_ldg(Prm q, Buf b, const uint32_t *
     const float4 vi = b.velp[i];
     float lxx = 0, lxy = 0, lxz = 0, lyy = 0, lyz = 0, lzz = 0;
     float gx = 0, gy = 0, gz = 0, D = 0;
-    for_each_neighbor(q, i, xi, skey[i], cs, b.posr, [&](int j, float dx, float dy, float dz, float r2, float4 xj) {
-        const float qq = sqrtf(r2) * q.inv_h;
-        const float t = 1.0f - 0.5f * qq;
-        const float F = q.fk * t * t * t;
-        const float FV = F * q.m / xj.w;
-        if (delta) {
-            lxx -= FV * dx * dx; lxy -= FV * dx * dy; lxz -= FV * dx * dz;
-            lyy -= FV * dy * dy; lyz -= FV * dy * dz; lzz -= FV * dz * dz;
-            const float c = FV * (xj.w - xi.w);
-            gx += c * dx; gy += c * dy; gz += c * dz;
-        }
-        if (b.type[j] == ZSPH_FLUID) {
-            const float4 vj = b.velp[j];
-            const float dvdx = (vi.x - vj.x) * dx + (vi.y - vj.y) * dy + (vi.z - vj.z) * dz;
-            D -= FV * dvdx;
-        }
-    });
+    for_each_neighbor<true, true>(q, i, xi, cell_of(q, skey[i]), cs, b.posr, qbuf,
+                                  [&](int j, float dx, float dy, float dz, float r2, float4 xj) {
+                                      const float qq = sqrtf(r2) * q.inv_h;
+                                      const float t = 1.0f - 0.5f * qq;
+                                      const float FV = q.fk * t * t * t * q.m / xj.w;
+                                      if (delta) {
+                                          lxx -= FV * dx * dx; lxy -= FV * dx * dy; lxz -= FV * dx * dz;
+                                          lyy -= FV * dy * dy; lyz -= FV * dy * dz; lzz -= FV * dz * dz;
+                                          const float c = FV * (xj.w - xi.w);
+                                          gx += c * dx; gy += c * dy; gz += c * dz;
+                                      }
+                                      if (j < nf) {
+                                          const float4 vj = b.velp[j];
+                                          D -= FV * ((vi.x - vj.x) * dx + (vi.y - vj.y) * dy + (vi.z - vj.z) * dz);
+                                      }
+                                  });
     if (delta) {
         if (q.dim == 2) lyy += 1.0f;  // x-z plane: unit yy entry keeps L invertible
         const float c00 = lyy * lzz - lyz * lyz;
@@ -99,31 +111,24 @@ __global__ __launch_bounds__(ZS_BLOCK) void k_ldg(Prm q, Buf b, const uint32_t *
     gd[i] = make_float4(gx, gy, gz, D);
 }
 
-// Continuity + momentum (delta_ns.cu:204-265; MC form simple_ns.cu:123-177).
-// Fluid i: all neighbours; moving dummies (body, piston): fluid neighbours
-// only, without gravity, so their sum is the fluid force on the body.
-// Fixed dummies and removed particles get 0.
+// Continuity + momentum of fluid particles (delta_ns.cu:204-265; MC form
+// simple_ns.cu:123-177).  Pressure and continuity over all neighbours,
+// delta term and viscosity over fluid pairs, gravity added.
 __global__ __launch_bounds__(ZS_BLOCK) void k_governing(Prm q, Buf b, const uint32_t *__restrict__ skey,
                                                         const int *__restrict__ cs, const float4 *__restrict__ gd,
                                                         float4 *__restrict__ acc) {
+    __shared__ int qbuf[QCAP * ZS_BLOCK];
+    const int nf = cs[q.G];
     const int i = xcd_block(gridDim.x) * ZS_BLOCK + threadIdx.x;
-    if (i >= q.N) return;
-    const int ti = b.type[i];
-    if (ti != ZSPH_FLUID && ti != ZSPH_BODY && ti != ZSPH_PISTON) {
-        acc[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-        return;
-    }
-    const bool fi = ti == ZSPH_FLUID;
+    if (i >= nf) return;
     const bool delta = q.diffusion == ZSPH_DIFF_ANTUONO;
     const float4 xi = b.posr[i];
     const float4 vi = b.velp[i];
     const float4 gi = gd[i];
     const float rhoi = xi.w, pi = vi.w;
     float ax = 0, ay = 0, az = 0, dr = 0;
-    for_each_neighbor(q, i, xi, skey[i], cs, b.posr, [&](int j, float dx, float dy, float dz, float r2, float4 xj) {
-        const int tj = b.type[j];
-        const bool fj = tj == ZSPH_FLUID;
-        if (!fi && !fj) return;
+    for_each_neighbor<true, true>(q, i, xi, cell_of(q, skey[i]), cs, b.posr, qbuf,
+                                  [&](int j, float dx, float dy, float dz, float r2, float4 xj) {
         const float qq = sqrtf(r2) * q.inv_h;
         const float t = 1.0f - 0.5f * qq;
         const float F = q.fk * t * t * t;
@@ -134,7 +139,7 @@ __global__ __launch_bounds__(ZS_BLOCK) void k_governing(Prm q, Buf b, const uint
         dr += rhoi * dvdx * F * Vj;
         const float irr = 1.0f / (rhoi * rhoj);
         float pt = -(pi + pj) * q.m * F * irr;
-        if (fi && fj) {
+        if (j < nf) {
             const float ire = 1.0f / (r2 + q.eta2);
             const float4 gj = gd[j];
             if (delta) {
@@ -147,15 +152,46 @@ __global__ __launch_bounds__(ZS_BLOCK) void k_governing(Prm q, Buf b, const uint
         if (!delta) dr += (rhoi - rhoj) * r2 * F * q.delta_h_cs / (r2 + q.eta2) * Vj;
         ax += pt * dx; ay += pt * dy; az += pt * dz;
     });
-    if (fi) az += q.g;
+    acc[i] = make_float4(ax, ay, az + q.g, dr);
+}
+
+// Dummies: moving ones (body, piston) get the pressure force of their fluid
+// neighbours, without gravity (their sum is the fluid force on the body);
+// fixed walls and removed particles get 0.
+__global__ __launch_bounds__(ZS_BLOCK) void k_governing_dummy(Prm q, Buf b, const uint32_t *__restrict__ skey,
+                                                              const int *__restrict__ cs, float4 *__restrict__ acc) {
+    __shared__ int qbuf[QCAP * ZS_BLOCK];
+    const int nf = cs[q.G];
+    const int i = nf + blockIdx.x * ZS_BLOCK + threadIdx.x;
+    if (i >= q.N) return;
+    const int ti = b.type[i];
+    float ax = 0, ay = 0, az = 0, dr = 0;
+    if (ti == ZSPH_BODY || ti == ZSPH_PISTON) {
+        const float4 xi = b.posr[i];
+        const float4 vi = b.velp[i];
+        const float rhoi = xi.w, pi = vi.w;
+        for_each_neighbor<true, false>(q, i, xi, cell_of(q, skey[i]), cs, b.posr, qbuf,
+                                       [&](int j, float dx, float dy, float dz, float r2, float4 xj) {
+            const float qq = sqrtf(r2) * q.inv_h;
+            const float t = 1.0f - 0.5f * qq;
+            const float F = q.fk * t * t * t;
+            const float4 vj = b.velp[j];
+            const float rhoj = xj.w;
+            const float dvdx = (vi.x - vj.x) * dx + (vi.y - vj.y) * dy + (vi.z - vj.z) * dz;
+            dr += rhoi * dvdx * F * q.m / rhoj;
+            const float pt = -(pi + vj.w) * q.m * F / (rhoi * rhoj);
+            ax += pt * dx; ay += pt * dy; az += pt * dz;
+        });
+    }
     acc[i] = make_float4(ax, ay, az, dr);
 }
 
 // Midpoint predictor (time_int.cu:3-48), fluid only: save state n, then
 // x += dt/2 v, v += dt/2 a, rho = max(rho + dt/2 drho, rho0), p = EOS(rho).
-__global__ __launch_bounds__(ZS_BLOCK) void k_predict(Prm q, Buf b, const float4 *__restrict__ acc) {
+__global__ __launch_bounds__(ZS_BLOCK) void k_predict(Prm q, Buf b, const int *__restrict__ cs,
+                                                      const float4 *__restrict__ acc) {
     const int i = blockIdx.x * ZS_BLOCK + threadIdx.x;
-    if (i >= q.N || b.type[i] != ZSPH_FLUID) return;
+    if (i >= cs[q.G]) return;
     float4 x = b.posr[i], v = b.velp[i];
     const float4 a = acc[i];
     b.tposr[i] = x;
@@ -170,9 +206,10 @@ __global__ __launch_bounds__(ZS_BLOCK) void k_predict(Prm q, Buf b, const float4
 
 // Midpoint corrector (time_int.cu:50-91): x = x^n + dt v*, v = v^n + dt a*,
 // rho = max(rho^n + dt drho*, rho0), p = EOS(rho).
-__global__ __launch_bounds__(ZS_BLOCK) void k_correct(Prm q, Buf b, const float4 *__restrict__ acc) {
+__global__ __launch_bounds__(ZS_BLOCK) void k_correct(Prm q, Buf b, const int *__restrict__ cs,
+                                                      const float4 *__restrict__ acc) {
     const int i = blockIdx.x * ZS_BLOCK + threadIdx.x;
-    if (i >= q.N || b.type[i] != ZSPH_FLUID) return;
+    if (i >= cs[q.G]) return;
     float4 x = b.tposr[i], v = b.tvelp[i];
     const float4 vs = b.velp[i];
     const float4 a = acc[i];
@@ -191,19 +228,22 @@ __device__ __forceinline__ uint64_t mix64(uint64_t z) {
     return z ^ (z >> 31);
 }
 
-// Neighbour count and set digest sum_j mix64(j) (mod 2^64) per particle.
+// Neighbour count and set digest sum_j mix64(j) (mod 2^64) per particle,
+// through the same walk as the physics kernels.
 __global__ __launch_bounds__(ZS_BLOCK) void k_nbr_stats(Prm q, Buf b, const uint32_t *__restrict__ skey,
                                                         const int *__restrict__ cs, int *__restrict__ count,
                                                         uint64_t *__restrict__ digest) {
+    __shared__ int qbuf[QCAP * ZS_BLOCK];
     const int i = blockIdx.x * ZS_BLOCK + threadIdx.x;
     if (i >= q.N) return;
     const float4 xi = b.posr[i];
     int c = 0;
     uint64_t h = 0;
-    for_each_neighbor(q, i, xi, skey[i], cs, b.posr, [&](int j, float, float, float, float, float4) {
-        ++c;
-        h += mix64((uint64_t)j);
-    });
+    for_each_neighbor<true, true>(q, i, xi, cell_of(q, skey[i]), cs, b.posr, qbuf,
+                                  [&](int j, float, float, float, float, float4) {
+                                      ++c;
+                                      h += mix64((uint64_t)j);
+                                  });
     count[i] = c;
     digest[i] = h;
 }
@@ -216,13 +256,15 @@ void launch_ldg(hipStream_t s, const Prm &q, const Buf &b, const uint32_t *skey,
 }
 void launch_governing(hipStream_t s, const Prm &q, const Buf &b, const uint32_t *skey, const int *cs,
                       const float4 *gd, float4 *acc) {
-    if (q.N) k_governing<<<nblocks(q.N), ZS_BLOCK, 0, s>>>(q, b, skey, cs, gd, acc);
+    if (!q.N) return;
+    k_governing<<<nblocks(q.N), ZS_BLOCK, 0, s>>>(q, b, skey, cs, gd, acc);
+    k_governing_dummy<<<nblocks(q.N), ZS_BLOCK, 0, s>>>(q, b, skey, cs, acc);
 }
-void launch_predict(hipStream_t s, const Prm &q, const Buf &b, const float4 *acc) {
-    if (q.N) k_predict<<<nblocks(q.N), ZS_BLOCK, 0, s>>>(q, b, acc);
+void launch_predict(hipStream_t s, const Prm &q, const Buf &b, const int *cs, const float4 *acc) {
+    if (q.N) k_predict<<<nblocks(q.N), ZS_BLOCK, 0, s>>>(q, b, cs, acc);
 }
-void launch_correct(hipStream_t s, const Prm &q, const Buf &b, const float4 *acc) {
-    if (q.N) k_correct<<<nblocks(q.N), ZS_BLOCK, 0, s>>>(q, b, acc);
+void launch_correct(hipStream_t s, const Prm &q, const Buf &b, const int *cs, const float4 *acc) {
+    if (q.N) k_correct<<<nblocks(q.N), ZS_BLOCK, 0, s>>>(q, b, cs, acc);
 }
 void launch_nbr_stats(hipStream_t s, const Prm &q, const Buf &b, const uint32_t *skey, const int *cs, int *count,
                       uint64_t *digest) {

@@ -135,7 +135,7 @@ int stage_sort_data(zsph_ctx *c, int from, int to, Prof *pf = nullptr) {
     if (from == to) return set_err(c, ZSPH_E_ARG, "zsph_sort_data needs two different buffers");
     {
         Scope sc(pf, ZSPH_ST_CELLS);
-        zs::launch_cell_start(c->stream, c->key[c->sorted_slot], c->q.N, c->q.G, c->cell_start);
+        zs::launch_cell_start(c->stream, c->key[c->sorted_slot], c->q.N, c->q.G, c->q.flag, c->cell_start);
     }
     {
         Scope sc(pf, ZSPH_ST_REORDER);
@@ -175,9 +175,9 @@ int half_step(zsph_ctx *c, int from, int to, bool predictor, Prof *pf = nullptr)
     {
         Scope sc(pf, ZSPH_ST_INTEGRATE);
         if (predictor)
-            zs::launch_predict(c->stream, c->q, c->buf[to], c->acc);
+            zs::launch_predict(c->stream, c->q, c->buf[to], c->cell_start, c->acc);
         else
-            zs::launch_correct(c->stream, c->q, c->buf[to], c->acc);
+            zs::launch_correct(c->stream, c->q, c->buf[to], c->cell_start, c->acc);
     }
     return check_launch(c, predictor ? "prediction" : "correction");
 }
@@ -219,13 +219,15 @@ int zsph_create(const zsph_params *p, int device, zsph_ctx **out) {
             return fail(r);
         if ((r = dalloc(c, &c->key[b], N)) || (r = dalloc(c, &c->idx[b], N))) return fail(r);
     }
+    // sort keys: cell bits + the non-fluid flag bit above them
     int kb = 1;
-    while (kb < 31 && ((long long)1 << kb) < G) ++kb;
+    while ((1u << kb) < c->q.flag) ++kb;
+    ++kb;
     c->keybits = kb;
     // fewest passes of at most 11-bit digits, spread evenly (19 bits: 2 x 10)
     const int passes = (kb + 10) / 11;
     c->radix_bits = std::max(8, (kb + passes - 1) / passes);
-    if ((r = dalloc(c, &c->cell_start, (size_t)G + 1)) || (r = dalloc(c, &c->acc, N)) ||
+    if ((r = dalloc(c, &c->cell_start, 2 * ((size_t)G + 1))) || (r = dalloc(c, &c->acc, N)) ||
         (r = dalloc(c, &c->gd, N)) || (r = dalloc(c, &c->hist, zs::radix_hist_words((int)N, 11))) ||
         (r = dalloc(c, &c->killed, 1)) ||
         (r = dalloc(c, &c->diag_part, (size_t)zs::diag_blocks((int)N) * zs::DIAG_D)) ||
@@ -320,6 +322,11 @@ int zsph_download(zsph_ctx *c, int b, zsph_host_state *st, int fields) {
         HIPCHK(c, hipMemcpyAsync(st->hash, src, N * sizeof(int), hipMemcpyDeviceToHost, c->stream));
     }
     HIPCHK(c, hipStreamSynchronize(c->stream));
+    if ((fields & ZSPH_F_HASH) && st->hash) {
+        // the cell index only (the sort key's non-fluid flag is internal)
+        const int mask = (int)(c->q.flag - 1u);
+        for (int i = 0; i < N; ++i) st->hash[i] &= mask;
+    }
     return ZSPH_OK;
 }
 
@@ -345,12 +352,14 @@ int zsph_governing(zsph_ctx *c, int b) {
 }
 int zsph_prediction(zsph_ctx *c, int b) {
     if (!c || !valid_buf(b)) return set_err(c, ZSPH_E_ARG, "zsph_prediction: bad buffer");
-    zs::launch_predict(c->stream, c->q, c->buf[b], c->acc);
+    if (c->sorted_slot < 0) return set_err(c, ZSPH_E_STATE, "zsph_prediction called before zsph_sort_data");
+    zs::launch_predict(c->stream, c->q, c->buf[b], c->cell_start, c->acc);
     return check_launch(c, "prediction");
 }
 int zsph_correction(zsph_ctx *c, int b) {
     if (!c || !valid_buf(b)) return set_err(c, ZSPH_E_ARG, "zsph_correction: bad buffer");
-    zs::launch_correct(c->stream, c->q, c->buf[b], c->acc);
+    if (c->sorted_slot < 0) return set_err(c, ZSPH_E_STATE, "zsph_correction called before zsph_sort_data");
+    zs::launch_correct(c->stream, c->q, c->buf[b], c->cell_start, c->acc);
     return check_launch(c, "correction");
 }
 
@@ -393,7 +402,7 @@ int zsph_array_get(zsph_ctx *c, int which, int b, void *host, size_t bytes) {
     case ZSPH_A_TABLE: src = c->buf[b].table; need = N * 4; break;
     case ZSPH_A_KEY: src = c->key[c->sorted_slot >= 0 ? c->sorted_slot : 0]; need = N * 4; break;
     case ZSPH_A_INDEX: src = c->idx[c->sorted_slot >= 0 ? c->sorted_slot : 0]; need = N * 4; break;
-    case ZSPH_A_CELL_START: src = c->cell_start; need = ((size_t)c->q.G + 1) * 4; break;
+    case ZSPH_A_CELL_START: src = c->cell_start; need = 2 * ((size_t)c->q.G + 1) * 4; break;
     case ZSPH_A_ACC: src = c->acc; need = N * 16; break;
     case ZSPH_A_GD: src = c->gd; need = N * 16; break;
     default: return set_err(c, ZSPH_E_ARG, "zsph_array_get: unknown array %d", which);
