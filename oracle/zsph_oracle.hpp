@@ -182,7 +182,7 @@ struct Oracle {
     void sort_data(int from, int to) {
         // fluid starts of cells 0..G, then the starts of the other particles
         for (int e = 0; e <= 2 * G + 1; ++e) {
-            const uint32_t t = (e <= G) ? (uint32_t)e : (flag | (uint32_t)(e - G - 1));
+            const uint32_t t = (e <= G) ? (uint32_t)e : (flag + (uint32_t)(e - G - 1));  // c runs to G; G may equal flag
             cs[e] = (int)(std::lower_bound(key.begin(), key.end(), t) - key.begin());
         }
         const Buffer<R> &a = buf[from];

@@ -140,9 +140,15 @@ def test_out_of_grid_particles_are_removed():
                                                       np.float32(c.cpu.grid_zmin))
 
 
-@pytest.mark.parametrize("dim", [2, 3])
+@pytest.mark.parametrize("dim", [2, 3, "2-pow2"])
 def test_cell_walk_equals_brute_force(dim):
-    c = jitter(cases.dambreak2d(dx=0.02) if dim == 2 else small3d(), seed=7, amp=0.3)
+    # "2-pow2": the 2-D dam break of config 0, whose grid has exactly 2^13 cells
+    # (the dummy-table sentinel must still be N there)
+    if dim == "2-pow2":
+        c = jitter(cases.dambreak2d(), seed=12345)
+        assert c.cpu.grid_num == 8192
+    else:
+        c = jitter(cases.dambreak2d(dx=0.02) if dim == 2 else small3d(), seed=7, amp=0.3)
     o = stage(c, upto="sort")
     cw, dw = o.neighbor_stats(1, brute=False)
     cb, db = o.neighbor_stats(1, brute=True)

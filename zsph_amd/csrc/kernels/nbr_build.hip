@@ -218,7 +218,8 @@ __global__ __launch_bounds__(ZS_BLOCK) void k_cell_start(const uint32_t *__restr
                                                          uint32_t flag, int *__restrict__ cs) {
     const int e = blockIdx.x * ZS_BLOCK + threadIdx.x;
     if (e > 2 * G + 1) return;
-    const uint32_t target = (e <= G) ? (uint32_t)e : (flag | (uint32_t)(e - G - 1));
+    // flag + c, not flag | c: c runs to G, and G == flag when G is a power of two
+    const uint32_t target = (e <= G) ? (uint32_t)e : (flag + (uint32_t)(e - G - 1));
     int lo = 0, hi = n;
     while (lo < hi) {
         const int mid = (lo + hi) >> 1;
