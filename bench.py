@@ -66,6 +66,7 @@ def main():
     ap.add_argument("--gsf", type=float, default=2.0, help="grid_size_factor: cell edge = 2h / gsf")
     ap.add_argument("--profile-steps", type=int, default=3)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-autotune", dest="autotune", action="store_false")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); the step is single-GPU.",
@@ -80,6 +81,9 @@ def main():
           f"grid {case.cpu.grid_xdim}x{case.cpu.grid_ydim}x{case.cpu.grid_zdim}", file=sys.stderr, flush=True)
     s = z.Solver(case, device=0)
     s.upload(case.host)
+    # pick the neighbour-walk variant and sort digit width on this case (state restored after)
+    tune = s.autotune(steps=3) if args.autotune else s.get_tuning()
+    print(f"[bench] tuning: {tune}", file=sys.stderr, flush=True)
     s.step(args.warmup)
     s.sync()
     print(f"[bench] warm-up {args.warmup} steps done", file=sys.stderr, flush=True)
@@ -125,7 +129,8 @@ def main():
                                f"Tait gamma 7, CFL 0.2, cell edge 2h/{args.gsf:g}",
                    "particles": case.n, "fluid_particles": case.json.water_ptc_num,
                    "fluid_particle_steps_per_s": case.json.water_ptc_num * args.steps / wall,
-                   "event_ms_per_step": ev_ms / args.steps, "removed_particles": killed},
+                   "event_ms_per_step": ev_ms / args.steps, "removed_particles": killed,
+                   "tuning": tune},
         "roofline": {"bound": "hbm", "kernel": "k_governing", "achieved": gov_gbs, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": gov_gbs / HBM_PEAK_GBS, "traffic": None,
                      "bytes_per_launch": gbytes, "avg_launch_ms": gov_ms},

@@ -335,6 +335,23 @@ int zsph_diagnostics(zsph_ctx *ctx, int buf, float front_zmax, zsph_diag *out);
  * Buffer `buf` must be in cell order (after zsph_sort_data or zsph_step). */
 int zsph_probe_pressure(zsph_ctx *ctx, int buf, const float *pts, int npts, float *p_out);
 
+/* Launch tuning (no counterpart in the reference, which launches 256-thread
+ * blocks, deltasph/main.cu:55-56).  Results do not depend on it: both
+ * neighbour walks visit neighbours in the same order, and the stable sort
+ * defines one order whatever its digit width. */
+#define ZSPH_WALK_MASK 0  /* per-row hit bitmask, then the hits            */
+#define ZSPH_WALK_QUEUE 1 /* per-lane LDS queue of hits, drained wave-wide */
+typedef struct zsph_tuning {
+    int walk;        /* ZSPH_WALK_*                         */
+    int radix_bits;  /* sort digit width, 8..11             */
+} zsph_tuning;
+int zsph_set_tuning(zsph_ctx *ctx, const zsph_tuning *t);
+int zsph_get_tuning(zsph_ctx *ctx, zsph_tuning *t);
+/* Times every tuning candidate over `steps` graph-replayed steps starting
+ * from a saved copy of buffer 0, restores that state, and keeps the
+ * fastest; *best and *best_ms (may be NULL) receive it. */
+int zsph_autotune(zsph_ctx *ctx, int steps, zsph_tuning *best, float *best_ms);
+
 /* Number of particles removed by zsph_hash since zsph_create. */
 int zsph_killed(zsph_ctx *ctx, int *killed);
 

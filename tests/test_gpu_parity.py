@@ -216,6 +216,34 @@ def test_graph_replay_is_deterministic_and_equals_stages():
         assert np.array_equal(a, b)
 
 
+@pytest.mark.parametrize("name", ["2d", "3d-k2"])
+def test_results_do_not_depend_on_tuning(name):
+    c = CASES[name]()
+    say(f"[case tuning {name}] N={c.n}")
+    ref = None
+    for walk in (z._abi.WALK_MASK, z._abi.WALK_QUEUE):
+        for bits in (8, 11):
+            s = z.Solver(c)
+            s.set_tuning(walk, bits)
+            s.upload(c.host)
+            s.step(3)
+            s.sync()
+            got = (s.get(A_POSR, 0), s.get(A_VELP, 0), s.get(A_ACC), s.get(A_TABLE, 0))
+            s.close()
+            if ref is None:
+                ref = got
+            else:
+                for a, b in zip(ref, got):
+                    assert np.array_equal(a, b), (walk, bits)
+    # the autotuner restores the state it timed from
+    s = z.Solver(c)
+    s.upload(c.host)
+    before = s.get(A_POSR, 0)
+    t = s.autotune(steps=2)
+    assert t["walk"] in (0, 1) and 8 <= t["radix_bits"] <= 11
+    assert np.array_equal(s.get(A_POSR, 0), before) and s.killed() == 0
+
+
 def test_out_of_order_calls_are_errors():
     c = CASES["2d"]()
     s = z.Solver(c)

@@ -229,6 +229,22 @@ class Solver:
                   "zsph_neighbor_stats")
         return cnt, dig
 
+    def set_tuning(self, walk, radix_bits):
+        t = abi.Tuning(walk, radix_bits)
+        self._chk(lib().zsph_set_tuning(self._ctx, C.byref(t)), "zsph_set_tuning")
+
+    def get_tuning(self):
+        t = abi.Tuning()
+        self._chk(lib().zsph_get_tuning(self._ctx, C.byref(t)), "zsph_get_tuning")
+        return {"walk": t.walk, "radix_bits": t.radix_bits}
+
+    def autotune(self, steps=5):
+        """Time every tuning candidate from the current buffer-0 state (restored afterwards); keep the fastest."""
+        t = abi.Tuning()
+        ms = C.c_float(0.0)
+        self._chk(lib().zsph_autotune(self._ctx, steps, C.byref(t), C.byref(ms)), "zsph_autotune")
+        return {"walk": t.walk, "radix_bits": t.radix_bits, "ms_per_step": ms.value / steps}
+
     def time_steps(self, nsteps):
         """Milliseconds of `nsteps` graph-replayed steps, between two HIP events on the solver's stream."""
         ms = C.c_float(0.0)

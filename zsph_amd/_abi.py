@@ -88,6 +88,13 @@ class Rigid(C.Structure):
                [("I", C.c_double * 9), ("mass", C.c_double)]
 
 
+class Tuning(C.Structure):
+    _fields_ = [("walk", _i), ("radix_bits", _i)]
+
+
+WALK_MASK, WALK_QUEUE = 0, 1
+
+
 class DamBreak(C.Structure):
     _fields_ = [("dim", _i), ("dx", _f), ("lx", _f), ("wy", _f), ("lz", _f), ("cw", _f), ("ch", _f),
                 ("layers", _i), ("wall_height", _f)]
@@ -143,6 +150,9 @@ def load(path):
     _sig(lib, "zsph_diagnostics", _i, [vp, _i, _f, P(Diag)])
     _sig(lib, "zsph_probe_pressure", _i, [vp, _i, P(_f), _i, P(_f)])
     _sig(lib, "zsph_profile_steps", _i, [vp, _i, P(_f)])
+    _sig(lib, "zsph_set_tuning", _i, [vp, P(Tuning)])
+    _sig(lib, "zsph_get_tuning", _i, [vp, P(Tuning)])
+    _sig(lib, "zsph_autotune", _i, [vp, _i, P(Tuning), P(_f)])
     return lib
 
 
@@ -155,6 +165,7 @@ EXPORTS = (
     "zsph_correction", "zsph_step", "zsph_array_get", "zsph_array_set", "zsph_neighbor_stats",
     "zsph_killed", "zsph_sizeof", "zsph_time_steps", "zsph_profile_steps", "zsph_diagnostics",
     "zsph_probe_pressure", "zsph_rigid_mass_properties", "zsph_rigid_acc_alpha", "zsph_rigid_euler_step",
+    "zsph_set_tuning", "zsph_get_tuning", "zsph_autotune",
 )
 
 STAGES = ("hash", "sort", "cells", "reorder", "wall", "ldg", "governing", "integrate")

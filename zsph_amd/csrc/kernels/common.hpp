@@ -156,67 +156,20 @@ __device__ __forceinline__ CellBox cell_box(const Prm &q, uint32_t key) {
 // the others (dummies, removed) are [cs[G+1+c], cs[G+1+c+1]).  Fluid
 // particles therefore occupy [0, cs[G]) and dummies [cs[G], N).
 
-// Per-lane queue of neighbour indices in LDS (QCAP entries per thread,
-// stored [slot][thread] so the lanes of a wave hit consecutive banks).
-constexpr int QCAP = 32;
-constexpr int QCHUNK = 8;  // candidates tested per lane between queue checks
-
-// Visit every neighbour j != i (FP32 r^2 <= rc2) of particle i at xi whose
-// segment is enabled (FLUID_J: j < cs[G]; DUMMY_J: j >= cs[G]).
-// * Each (y,z) cell row of the box is first trimmed to the x-cells that can
-//   hold a point within the cut-off of xi, and rows beyond the cut-off are
-//   skipped.  The trimming works in cell units with a margin, so it only ever
-//   keeps extra candidates; the r^2 predicate alone decides the set.
-// * Candidates are tested QCHUNK at a time per lane; hits go to the lane's
-//   LDS queue, and whenever every active lane holds at least 4 queued hits
-//   the wave processes 4 of each with all lanes busy.  The remainder is
-//   processed at the end.
-// f(j, dx, dy, dz, r2, xj) with x_ij = xi - xj.
-template <bool FLUID_J, bool DUMMY_J, class F>
-__device__ __forceinline__ void for_each_neighbor(const Prm &q, int i, float4 xi, uint32_t cell,
-                                                  const int *__restrict__ cs, const float4 *__restrict__ posr,
-                                                  int *__restrict__ qbuf, F &&f) {
+// Calls g(fb, fe, db, de) for every (y,z) cell row of the (2k+1)^3 box
+// around xi's cell, trimmed to the x-cells that can hold a point within the
+// cut-off of xi (rows beyond the cut-off are skipped): [fb, fe) are the
+// row's fluid candidates, [db, de) its other particles (empty when the
+// segment is not requested).  The trimming works in cell units with a
+// margin, so it only ever keeps extra candidates; the r^2 predicate alone
+// decides the neighbour set.
+template <bool FLUID_J, bool DUMMY_J, class G>
+__device__ __forceinline__ void for_each_row(const Prm &q, float4 xi, uint32_t cell, const int *__restrict__ cs,
+                                             G &&g) {
     const CellBox b = cell_box(q, cell);
     const float fx = (xi.x - q.gx0) * q.inv_cell;
     const float fy = (xi.y - q.gy0) * q.inv_cell;
     const float fz = (xi.z - q.gz0) * q.inv_cell;
-    int *myq = qbuf + threadIdx.x;
-    uint32_t head = 0, tail = 0;
-    auto visit = [&](int j) {
-        const float4 xj = posr[j];
-        const float dx = xi.x - xj.x, dy = xi.y - xj.y, dz = xi.z - xj.z;
-        f(j, dx, dy, dz, dist2(dx, dy, dz), xj);
-    };
-    auto drain_all = [&]() {
-        while (head != tail) {
-            visit(myq[(head & (QCAP - 1)) * ZS_BLOCK]);
-            ++head;
-        }
-    };
-    auto scan = [&](int jb, int je) {
-        for (int base = jb; base < je; base += QCHUNK) {
-            if (__any(tail - head > (uint32_t)(QCAP - QCHUNK))) drain_all();
-#pragma unroll
-            for (int t = 0; t < QCHUNK; ++t) {
-                const int j = base + t;
-                if (j < je) {
-                    const float4 xj = posr[j];
-                    const float r2 = dist2(xi.x - xj.x, xi.y - xj.y, xi.z - xj.z);
-                    if (r2 <= q.rc2 && j != i) {
-                        myq[(tail & (QCAP - 1)) * ZS_BLOCK] = j;
-                        ++tail;
-                    }
-                }
-            }
-            while (__all(tail - head >= 4u)) {
-#pragma unroll
-                for (int u = 0; u < 4; ++u) {
-                    visit(myq[(head & (QCAP - 1)) * ZS_BLOCK]);
-                    ++head;
-                }
-            }
-        }
-    };
     for (int cz = b.zlo; cz <= b.zhi; ++cz) {
         const float dzc = fmaxf(0.0f, fmaxf((float)cz - fz, fz - (float)(cz + 1)));
         const float remz = q.trim2 - dzc * dzc;
@@ -230,11 +183,102 @@ __device__ __forceinline__ void for_each_neighbor(const Prm &q, int i, float4 xi
             const int xb = min(b.xhi, (int)floorf(fx + w));
             if (xa > xb) continue;
             const int row = cz * q.XY + cy * q.X;
-            if (FLUID_J) scan(cs[row + xa], cs[row + xb + 1]);
-            if (DUMMY_J) scan(cs[q.G + 1 + row + xa], cs[q.G + 1 + row + xb + 1]);
+            int fb = 0, fe = 0, db = 0, de = 0;
+            if (FLUID_J) { fb = cs[row + xa]; fe = cs[row + xb + 1]; }
+            if (DUMMY_J) { db = cs[q.G + 1 + row + xa]; de = cs[q.G + 1 + row + xb + 1]; }
+            g(fb, fe, db, de);
         }
     }
-    drain_all();
+}
+
+// Neighbour-walk variants (chosen per context by measurement):
+//  WALK_MASK:  each row's candidates (fluid range, then the others) are
+//              tested 32 at a time into a per-lane hit mask, then the hits
+//              are visited; the wave runs the interaction body once per hit
+//              of its busiest lane in that chunk.
+//  WALK_QUEUE: hits go to a per-lane LDS queue; whenever every active lane
+//              holds >= 4 queued hits the wave processes 4 of each with all
+//              lanes busy, the rest at the end.
+constexpr int WALK_MASK = 0;
+constexpr int WALK_QUEUE = 1;
+constexpr int QCAP = 32;   // queue entries per thread, stored [slot][thread]
+constexpr int QCHUNK = 8;  // candidates tested per lane between queue checks
+template <int WALK>
+struct WalkLds {
+    static constexpr int ints = (WALK == WALK_QUEUE) ? QCAP * ZS_BLOCK : 1;
+};
+
+// Visit every neighbour j != i (FP32 r^2 <= rc2) of particle i at xi whose
+// segment is enabled (FLUID_J: j < cs[G]; DUMMY_J: j >= cs[G]).
+// f(j, dx, dy, dz, r2, xj) with x_ij = xi - xj.  qbuf: WalkLds<WALK>::ints
+// ints of LDS per block (unused by WALK_MASK).
+template <int WALK, bool FLUID_J, bool DUMMY_J, class F>
+__device__ __forceinline__ void for_each_neighbor(const Prm &q, int i, float4 xi, uint32_t cell,
+                                                  const int *__restrict__ cs, const float4 *__restrict__ posr,
+                                                  int *__restrict__ qbuf, F &&f) {
+    auto visit = [&](int j) {
+        const float4 xj = posr[j];
+        const float dx = xi.x - xj.x, dy = xi.y - xj.y, dz = xi.z - xj.z;
+        f(j, dx, dy, dz, dist2(dx, dy, dz), xj);
+    };
+    auto hit = [&](int j) {
+        const float4 xj = posr[j];
+        return dist2(xi.x - xj.x, xi.y - xj.y, xi.z - xj.z) <= q.rc2 && j != i;
+    };
+    if (WALK == WALK_MASK) {
+        for_each_row<FLUID_J, DUMMY_J>(q, xi, cell, cs, [&](int fb, int fe, int db, int de) {
+            const int nfl = fe - fb;
+            const int total = nfl + (de - db);
+            for (int base = 0; base < total; base += 32) {
+                const int cnt = min(32, total - base);
+                uint32_t m = 0;
+                for (int t = 0; t < cnt; ++t) {
+                    const int u = base + t;
+                    const int j = (u < nfl) ? fb + u : db + (u - nfl);
+                    m |= (hit(j) ? 1u : 0u) << t;
+                }
+                while (m) {
+                    const int u = base + __builtin_ctz(m);
+                    m &= m - 1u;
+                    visit((u < nfl) ? fb + u : db + (u - nfl));
+                }
+            }
+        });
+    } else {
+        int *myq = qbuf + threadIdx.x;
+        uint32_t head = 0, tail = 0;
+        auto drain_all = [&]() {
+            while (head != tail) {
+                visit(myq[(head & (QCAP - 1)) * ZS_BLOCK]);
+                ++head;
+            }
+        };
+        auto scan = [&](int jb, int je) {
+            for (int base = jb; base < je; base += QCHUNK) {
+                if (__any(tail - head > (uint32_t)(QCAP - QCHUNK))) drain_all();
+#pragma unroll
+                for (int t = 0; t < QCHUNK; ++t) {
+                    const int j = base + t;
+                    if (j < je && hit(j)) {
+                        myq[(tail & (QCAP - 1)) * ZS_BLOCK] = j;
+                        ++tail;
+                    }
+                }
+                while (__all(tail - head >= 4u)) {
+#pragma unroll
+                    for (int u = 0; u < 4; ++u) {
+                        visit(myq[(head & (QCAP - 1)) * ZS_BLOCK]);
+                        ++head;
+                    }
+                }
+            }
+        };
+        for_each_row<FLUID_J, DUMMY_J>(q, xi, cell, cs, [&](int fb, int fe, int db, int de) {
+            if (FLUID_J) scan(fb, fe);
+            if (DUMMY_J) scan(db, de);
+        });
+        drain_all();
+    }
 }
 
 }  // namespace zs

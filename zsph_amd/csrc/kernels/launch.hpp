@@ -20,15 +20,16 @@ int launch_sort(hipStream_t s, int n, int keybits, int bits, uint32_t *key[2], i
 void launch_cell_start(hipStream_t s, const uint32_t *skey, int n, int G, uint32_t flag, int *cs);
 void launch_reorder(hipStream_t s, int n, const int *perm, const Buf &a, const Buf &b);
 
-// physics.hip
-void launch_wall(hipStream_t s, const Prm &q, const Buf &b, const uint32_t *skey, const int *cs);
-void launch_ldg(hipStream_t s, const Prm &q, const Buf &b, const uint32_t *skey, const int *cs, float4 *gd);
-void launch_governing(hipStream_t s, const Prm &q, const Buf &b, const uint32_t *skey, const int *cs,
+// physics.hip; walk = WALK_MASK or WALK_QUEUE (common.hpp)
+void launch_wall(hipStream_t s, int walk, const Prm &q, const Buf &b, const uint32_t *skey, const int *cs);
+void launch_ldg(hipStream_t s, int walk, const Prm &q, const Buf &b, const uint32_t *skey, const int *cs,
+                float4 *gd);
+void launch_governing(hipStream_t s, int walk, const Prm &q, const Buf &b, const uint32_t *skey, const int *cs,
                       const float4 *gd, float4 *acc);
 void launch_predict(hipStream_t s, const Prm &q, const Buf &b, const int *cs, const float4 *acc);
 void launch_correct(hipStream_t s, const Prm &q, const Buf &b, const int *cs, const float4 *acc);
-void launch_nbr_stats(hipStream_t s, const Prm &q, const Buf &b, const uint32_t *skey, const int *cs, int *count,
-                      uint64_t *digest);
+void launch_nbr_stats(hipStream_t s, int walk, const Prm &q, const Buf &b, const uint32_t *skey, const int *cs,
+                      int *count, uint64_t *digest);
 
 // diag.hip
 constexpr int DIAG_D = 5;  // per-block FP64 partials: count, kinetic, momentum x/y/z

@@ -30,16 +30,17 @@ __device__ __forceinline__ uint32_t cell_of(const Prm &q, uint32_t key) { return
 // rho_w from the inverse EOS (delta_ns.cu:81-85, :116-131; a_w = 0 there).
 // a_w is the dummy's prescribed acceleration, kept in tvelp.xyz (zero for
 // fixed walls).
+template <int WALK>
 __global__ __launch_bounds__(ZS_BLOCK) void k_wall(Prm q, Buf b, const uint32_t *__restrict__ skey,
                                                    const int *__restrict__ cs) {
-    __shared__ int qbuf[QCAP * ZS_BLOCK];
+    __shared__ int qbuf[WalkLds<WALK>::ints];
     const int nf = cs[q.G];
     const int i = nf + xcd_block(gridDim.x) * ZS_BLOCK + threadIdx.x;
     if (i >= q.N) return;
     const float4 xi = b.posr[i];
     const float4 aw = b.tvelp[i];
     float P = 0.0f, S = 0.0f;
-    for_each_neighbor<true, false>(q, i, xi, cell_of(q, skey[i]), cs, b.posr, qbuf,
+    for_each_neighbor<WALK, true, false>(q, i, xi, cell_of(q, skey[i]), cs, b.posr, qbuf,
                                    [&](int j, float dx, float dy, float dz, float r2, float4 xj) {
                                        const float qq = sqrtf(r2) * q.inv_h;
                                        const float t = 1.0f - 0.5f * qq;
@@ -58,9 +59,10 @@ __global__ __launch_bounds__(ZS_BLOCK) void k_wall(Prm q, Buf b, const uint32_t 
 // G_i = L_i^-1 sum F (rho_j - rho_i) V_j x_ij (delta_ns.cu:75-79, :86-90,
 // :99-114, :133-174).  L^-1 is applied once to the accumulated sum (it does
 // not depend on j).  MC form: D only.  Output gd = (G, D); dummies get 0.
+template <int WALK>
 __global__ __launch_bounds__(ZS_BLOCK) void k_ldg(Prm q, Buf b, const uint32_t *__restrict__ skey,
                                                   const int *__restrict__ cs, float4 *__restrict__ gd) {
-    __shared__ int qbuf[QCAP * ZS_BLOCK];
+    __shared__ int qbuf[WalkLds<WALK>::ints];
     const int nf = cs[q.G];
     const int i = xcd_block(gridDim.x) * ZS_BLOCK + threadIdx.x;
     if (i >= q.N) return;
@@ -73,7 +75,7 @@ __global__ __launch_bounds__(ZS_BLOCK) void k_ldg(Prm q, Buf b, const uint32_t *
     const float4 vi = b.velp[i];
     float lxx = 0, lxy = 0, lxz = 0, lyy = 0, lyz = 0, lzz = 0;
     float gx = 0, gy = 0, gz = 0, D = 0;
-    for_each_neighbor<true, true>(q, i, xi, cell_of(q, skey[i]), cs, b.posr, qbuf,
+    for_each_neighbor<WALK, true, true>(q, i, xi, cell_of(q, skey[i]), cs, b.posr, qbuf,
                                   [&](int j, float dx, float dy, float dz, float r2, float4 xj) {
                                       const float qq = sqrtf(r2) * q.inv_h;
                                       const float t = 1.0f - 0.5f * qq;
@@ -114,10 +116,11 @@ __global__ __launch_bounds__(ZS_BLOCK) void k_ldg(Prm q, Buf b, const uint32_t *
 // Continuity + momentum of fluid particles (delta_ns.cu:204-265; MC form
 // simple_ns.cu:123-177).  Pressure and continuity over all neighbours,
 // delta term and viscosity over fluid pairs, gravity added.
+template <int WALK>
 __global__ __launch_bounds__(ZS_BLOCK) void k_governing(Prm q, Buf b, const uint32_t *__restrict__ skey,
                                                         const int *__restrict__ cs, const float4 *__restrict__ gd,
                                                         float4 *__restrict__ acc) {
-    __shared__ int qbuf[QCAP * ZS_BLOCK];
+    __shared__ int qbuf[WalkLds<WALK>::ints];
     const int nf = cs[q.G];
     const int i = xcd_block(gridDim.x) * ZS_BLOCK + threadIdx.x;
     if (i >= nf) return;
@@ -127,7 +130,7 @@ __global__ __launch_bounds__(ZS_BLOCK) void k_governing(Prm q, Buf b, const uint
     const float4 gi = gd[i];
     const float rhoi = xi.w, pi = vi.w;
     float ax = 0, ay = 0, az = 0, dr = 0;
-    for_each_neighbor<true, true>(q, i, xi, cell_of(q, skey[i]), cs, b.posr, qbuf,
+    for_each_neighbor<WALK, true, true>(q, i, xi, cell_of(q, skey[i]), cs, b.posr, qbuf,
                                   [&](int j, float dx, float dy, float dz, float r2, float4 xj) {
         const float qq = sqrtf(r2) * q.inv_h;
         const float t = 1.0f - 0.5f * qq;
@@ -158,9 +161,10 @@ __global__ __launch_bounds__(ZS_BLOCK) void k_governing(Prm q, Buf b, const uint
 // Dummies: moving ones (body, piston) get the pressure force of their fluid
 // neighbours, without gravity (their sum is the fluid force on the body);
 // fixed walls and removed particles get 0.
+template <int WALK>
 __global__ __launch_bounds__(ZS_BLOCK) void k_governing_dummy(Prm q, Buf b, const uint32_t *__restrict__ skey,
                                                               const int *__restrict__ cs, float4 *__restrict__ acc) {
-    __shared__ int qbuf[QCAP * ZS_BLOCK];
+    __shared__ int qbuf[WalkLds<WALK>::ints];
     const int nf = cs[q.G];
     const int i = nf + blockIdx.x * ZS_BLOCK + threadIdx.x;
     if (i >= q.N) return;
@@ -170,7 +174,7 @@ __global__ __launch_bounds__(ZS_BLOCK) void k_governing_dummy(Prm q, Buf b, cons
         const float4 xi = b.posr[i];
         const float4 vi = b.velp[i];
         const float rhoi = xi.w, pi = vi.w;
-        for_each_neighbor<true, false>(q, i, xi, cell_of(q, skey[i]), cs, b.posr, qbuf,
+        for_each_neighbor<WALK, true, false>(q, i, xi, cell_of(q, skey[i]), cs, b.posr, qbuf,
                                        [&](int j, float dx, float dy, float dz, float r2, float4 xj) {
             const float qq = sqrtf(r2) * q.inv_h;
             const float t = 1.0f - 0.5f * qq;
@@ -230,16 +234,17 @@ __device__ __forceinline__ uint64_t mix64(uint64_t z) {
 
 // Neighbour count and set digest sum_j mix64(j) (mod 2^64) per particle,
 // through the same walk as the physics kernels.
+template <int WALK>
 __global__ __launch_bounds__(ZS_BLOCK) void k_nbr_stats(Prm q, Buf b, const uint32_t *__restrict__ skey,
                                                         const int *__restrict__ cs, int *__restrict__ count,
                                                         uint64_t *__restrict__ digest) {
-    __shared__ int qbuf[QCAP * ZS_BLOCK];
+    __shared__ int qbuf[WalkLds<WALK>::ints];
     const int i = blockIdx.x * ZS_BLOCK + threadIdx.x;
     if (i >= q.N) return;
     const float4 xi = b.posr[i];
     int c = 0;
     uint64_t h = 0;
-    for_each_neighbor<true, true>(q, i, xi, cell_of(q, skey[i]), cs, b.posr, qbuf,
+    for_each_neighbor<WALK, true, true>(q, i, xi, cell_of(q, skey[i]), cs, b.posr, qbuf,
                                   [&](int j, float, float, float, float, float4) {
                                       ++c;
                                       h += mix64((uint64_t)j);
@@ -248,17 +253,26 @@ __global__ __launch_bounds__(ZS_BLOCK) void k_nbr_stats(Prm q, Buf b, const uint
     digest[i] = h;
 }
 
-void launch_wall(hipStream_t s, const Prm &q, const Buf &b, const uint32_t *skey, const int *cs) {
-    if (q.N) k_wall<<<nblocks(q.N), ZS_BLOCK, 0, s>>>(q, b, skey, cs);
+#define ZS_WALK_LAUNCH(kern, grid, ...)                                          \
+    do {                                                                         \
+        if (walk == WALK_QUEUE)                                                  \
+            kern<WALK_QUEUE><<<(grid), ZS_BLOCK, 0, s>>>(__VA_ARGS__);           \
+        else                                                                     \
+            kern<WALK_MASK><<<(grid), ZS_BLOCK, 0, s>>>(__VA_ARGS__);            \
+    } while (0)
+
+void launch_wall(hipStream_t s, int walk, const Prm &q, const Buf &b, const uint32_t *skey, const int *cs) {
+    if (q.N) ZS_WALK_LAUNCH(k_wall, nblocks(q.N), q, b, skey, cs);
 }
-void launch_ldg(hipStream_t s, const Prm &q, const Buf &b, const uint32_t *skey, const int *cs, float4 *gd) {
-    if (q.N) k_ldg<<<nblocks(q.N), ZS_BLOCK, 0, s>>>(q, b, skey, cs, gd);
+void launch_ldg(hipStream_t s, int walk, const Prm &q, const Buf &b, const uint32_t *skey, const int *cs,
+                float4 *gd) {
+    if (q.N) ZS_WALK_LAUNCH(k_ldg, nblocks(q.N), q, b, skey, cs, gd);
 }
-void launch_governing(hipStream_t s, const Prm &q, const Buf &b, const uint32_t *skey, const int *cs,
+void launch_governing(hipStream_t s, int walk, const Prm &q, const Buf &b, const uint32_t *skey, const int *cs,
                       const float4 *gd, float4 *acc) {
     if (!q.N) return;
-    k_governing<<<nblocks(q.N), ZS_BLOCK, 0, s>>>(q, b, skey, cs, gd, acc);
-    k_governing_dummy<<<nblocks(q.N), ZS_BLOCK, 0, s>>>(q, b, skey, cs, acc);
+    ZS_WALK_LAUNCH(k_governing, nblocks(q.N), q, b, skey, cs, gd, acc);
+    ZS_WALK_LAUNCH(k_governing_dummy, nblocks(q.N), q, b, skey, cs, acc);
 }
 void launch_predict(hipStream_t s, const Prm &q, const Buf &b, const int *cs, const float4 *acc) {
     if (q.N) k_predict<<<nblocks(q.N), ZS_BLOCK, 0, s>>>(q, b, cs, acc);
@@ -266,9 +280,9 @@ void launch_predict(hipStream_t s, const Prm &q, const Buf &b, const int *cs, co
 void launch_correct(hipStream_t s, const Prm &q, const Buf &b, const int *cs, const float4 *acc) {
     if (q.N) k_correct<<<nblocks(q.N), ZS_BLOCK, 0, s>>>(q, b, cs, acc);
 }
-void launch_nbr_stats(hipStream_t s, const Prm &q, const Buf &b, const uint32_t *skey, const int *cs, int *count,
-                      uint64_t *digest) {
-    if (q.N) k_nbr_stats<<<nblocks(q.N), ZS_BLOCK, 0, s>>>(q, b, skey, cs, count, digest);
+void launch_nbr_stats(hipStream_t s, int walk, const Prm &q, const Buf &b, const uint32_t *skey, const int *cs,
+                      int *count, uint64_t *digest) {
+    if (q.N) ZS_WALK_LAUNCH(k_nbr_stats, nblocks(q.N), q, b, skey, cs, count, digest);
 }
 
 }  // namespace zs

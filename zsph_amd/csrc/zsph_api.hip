@@ -34,6 +34,7 @@ struct zsph_ctx {
     float *diag_max = nullptr;
     int keybits = 1;
     int radix_bits = 8;
+    int walk = zs::WALK_MASK;  // neighbour-walk variant (tuning)
     hipGraph_t graph = nullptr;
     hipGraphExec_t graph_exec = nullptr;
     std::string err;
@@ -148,11 +149,11 @@ int stage_boundary(zsph_ctx *c, int b, Prof *pf = nullptr) {
     const uint32_t *sk = c->key[c->sorted_slot];
     {
         Scope sc(pf, ZSPH_ST_WALL);
-        zs::launch_wall(c->stream, c->q, c->buf[b], sk, c->cell_start);
+        zs::launch_wall(c->stream, c->walk, c->q, c->buf[b], sk, c->cell_start);
     }
     {
         Scope sc(pf, ZSPH_ST_LDG);
-        zs::launch_ldg(c->stream, c->q, c->buf[b], sk, c->cell_start, c->gd);
+        zs::launch_ldg(c->stream, c->walk, c->q, c->buf[b], sk, c->cell_start, c->gd);
     }
     return check_launch(c, "boundary");
 }
@@ -160,7 +161,7 @@ int stage_governing(zsph_ctx *c, int b, Prof *pf = nullptr) {
     if (c->sorted_slot < 0) return set_err(c, ZSPH_E_STATE, "zsph_governing called before zsph_sort_data");
     {
         Scope sc(pf, ZSPH_ST_GOVERNING);
-        zs::launch_governing(c->stream, c->q, c->buf[b], c->key[c->sorted_slot], c->cell_start, c->gd, c->acc);
+        zs::launch_governing(c->stream, c->walk, c->q, c->buf[b], c->key[c->sorted_slot], c->cell_start, c->gd, c->acc);
     }
     return check_launch(c, "governing");
 }
@@ -388,6 +389,97 @@ int zsph_step(zsph_ctx *c, int nsteps) {
     return ZSPH_OK;
 }
 
+int zsph_set_tuning(zsph_ctx *c, const zsph_tuning *t) {
+    if (!c || !t) return ZSPH_E_ARG;
+    if ((t->walk != ZSPH_WALK_MASK && t->walk != ZSPH_WALK_QUEUE) || t->radix_bits < 8 || t->radix_bits > 11)
+        return set_err(c, ZSPH_E_ARG, "zsph_set_tuning: walk %d / radix_bits %d out of range", t->walk, t->radix_bits);
+    if (t->walk == c->walk && t->radix_bits == c->radix_bits) return ZSPH_OK;
+    c->walk = t->walk;
+    c->radix_bits = t->radix_bits;
+    // the captured step embeds the old launches
+    if (c->graph_exec) (void)hipGraphExecDestroy(c->graph_exec);
+    if (c->graph) (void)hipGraphDestroy(c->graph);
+    c->graph_exec = nullptr;
+    c->graph = nullptr;
+    c->sorted_slot = -1;
+    c->hashed = 0;
+    return ZSPH_OK;
+}
+
+int zsph_get_tuning(zsph_ctx *c, zsph_tuning *t) {
+    if (!c || !t) return ZSPH_E_ARG;
+    t->walk = c->walk;
+    t->radix_bits = c->radix_bits;
+    return ZSPH_OK;
+}
+
+int zsph_autotune(zsph_ctx *c, int steps, zsph_tuning *best, float *best_ms) {
+    if (!c || steps < 1) return set_err(c, ZSPH_E_ARG, "zsph_autotune: bad argument");
+    const size_t N = (size_t)c->q.N;
+    if (N == 0) return ZSPH_OK;
+    // save buffer 0 and the removal counter; every trial starts from them
+    const zs::Buf &b0 = c->buf[0];
+    void *const arrays[6] = {b0.posr, b0.velp, b0.tposr, b0.tvelp, b0.type, b0.table};
+    const size_t bytes[6] = {N * 16, N * 16, N * 16, N * 16, N * 4, N * 4};
+    size_t total = 4;
+    for (size_t k : bytes) total += k;
+    char *save = nullptr;
+    HIPCHK(c, hipMalloc(&save, total));
+    auto copy_all = [&](bool to_save) -> hipError_t {
+        size_t off = 0;
+        hipError_t e = hipSuccess;
+        for (int k = 0; k < 6 && e == hipSuccess; ++k) {
+            e = to_save ? hipMemcpyAsync(save + off, arrays[k], bytes[k], hipMemcpyDeviceToDevice, c->stream)
+                        : hipMemcpyAsync(arrays[k], save + off, bytes[k], hipMemcpyDeviceToDevice, c->stream);
+            off += bytes[k];
+        }
+        if (e == hipSuccess)
+            e = to_save ? hipMemcpyAsync(save + off, c->killed, 4, hipMemcpyDeviceToDevice, c->stream)
+                        : hipMemcpyAsync(c->killed, save + off, 4, hipMemcpyDeviceToDevice, c->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+        return e;
+    };
+    hipError_t e = copy_all(true);
+    if (e != hipSuccess) {
+        (void)hipFree(save);
+        return set_err(c, ZSPH_E_HIP, "autotune save: %s", hipGetErrorString(e));
+    }
+    zsph_tuning cand[8];
+    int nc = 0;
+    const int passes = (c->keybits + 10) / 11;  // fewest passes of <= 11-bit digits
+    const int even = std::max(8, (c->keybits + passes - 1) / passes);
+    for (int walk = 0; walk < 2; ++walk) {
+        cand[nc++] = {walk, even};
+        if (even != 8) cand[nc++] = {walk, 8};
+    }
+    zsph_tuning keep{c->walk, c->radix_bits};
+    float best_t = 1e30f;
+    int r = ZSPH_OK;
+    for (int k = 0; k < nc && !r; ++k) {
+        if ((r = zsph_set_tuning(c, &cand[k]))) break;
+        if ((e = copy_all(false)) != hipSuccess) {
+            r = set_err(c, ZSPH_E_HIP, "autotune restore: %s", hipGetErrorString(e));
+            break;
+        }
+        float ms = 0.0f;
+        if ((r = zsph_time_steps(c, steps, &ms))) break;
+        if (ms < best_t) {
+            best_t = ms;
+            keep = cand[k];
+        }
+    }
+    if (!r) r = zsph_set_tuning(c, &keep);
+    e = copy_all(false);
+    (void)hipFree(save);
+    c->sorted_slot = -1;  // buffer 0 is back to its unsorted starting state
+    c->hashed = 0;
+    if (r) return r;
+    if (e != hipSuccess) return set_err(c, ZSPH_E_HIP, "autotune restore: %s", hipGetErrorString(e));
+    if (best) *best = keep;
+    if (best_ms) *best_ms = best_t;
+    return ZSPH_OK;
+}
+
 int zsph_array_get(zsph_ctx *c, int which, int b, void *host, size_t bytes) {
     if (!c || !host || !valid_buf(b)) return set_err(c, ZSPH_E_ARG, "zsph_array_get: bad argument");
     const size_t N = (size_t)c->q.N;
@@ -447,7 +539,7 @@ int zsph_neighbor_stats(zsph_ctx *c, int b, int *count, uint64_t *digest) {
         (void)hipFree(dc);
         return set_err(c, ZSPH_E_NOMEM, "hipMalloc failed");
     }
-    zs::launch_nbr_stats(c->stream, c->q, c->buf[b], c->key[c->sorted_slot], c->cell_start, dc, dd);
+    zs::launch_nbr_stats(c->stream, c->walk, c->q, c->buf[b], c->key[c->sorted_slot], c->cell_start, dc, dd);
     e = hipGetLastError();
     if (e == hipSuccess && N) e = hipMemcpyAsync(count, dc, N * 4, hipMemcpyDeviceToHost, c->stream);
     if (e == hipSuccess && N) e = hipMemcpyAsync(digest, dd, N * 8, hipMemcpyDeviceToHost, c->stream);
