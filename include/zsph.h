@@ -341,6 +341,9 @@ int zsph_probe_pressure(zsph_ctx *ctx, int buf, const float *pts, int npts, floa
  * defines one order whatever its digit width. */
 #define ZSPH_WALK_MASK 0  /* per-row hit bitmask, then the hits            */
 #define ZSPH_WALK_QUEUE 1 /* per-lane LDS queue of hits, drained wave-wide */
+#define ZSPH_WALK_LIST 2  /* gradient pass records neighbour lists (in walk
+                           * order) that the governing pass reuses; takes
+                           * 512 B of device memory per particle          */
 typedef struct zsph_tuning {
     int walk;        /* ZSPH_WALK_*                         */
     int radix_bits;  /* sort digit width, 8..11             */

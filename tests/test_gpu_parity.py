@@ -221,7 +221,7 @@ def test_results_do_not_depend_on_tuning(name):
     c = CASES[name]()
     say(f"[case tuning {name}] N={c.n}")
     ref = None
-    for walk in (z._abi.WALK_MASK, z._abi.WALK_QUEUE):
+    for walk in (z._abi.WALK_MASK, z._abi.WALK_QUEUE, z._abi.WALK_LIST):
         for bits in (8, 11):
             s = z.Solver(c)
             s.set_tuning(walk, bits)

@@ -92,7 +92,7 @@ class Tuning(C.Structure):
     _fields_ = [("walk", _i), ("radix_bits", _i)]
 
 
-WALK_MASK, WALK_QUEUE = 0, 1
+WALK_MASK, WALK_QUEUE, WALK_LIST = 0, 1, 2
 
 
 class DamBreak(C.Structure):

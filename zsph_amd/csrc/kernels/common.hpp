@@ -199,8 +199,22 @@ __device__ __forceinline__ void for_each_row(const Prm &q, float4 xi, uint32_t c
 //  WALK_QUEUE: hits go to a per-lane LDS queue; whenever every active lane
 //              holds >= 4 queued hits the wave processes 4 of each with all
 //              lanes busy, the rest at the end.
+//  WALK_LIST:  the gradient pass records each fluid particle's neighbours
+//              (in walk order) in a global list, then runs its interactions
+//              from the list with every lane busy; the continuity+momentum
+//              pass of the same half step reads the list and tests no
+//              candidates.  Particles with more than LIST_MAX neighbours
+//              fall back to the walk (same order, same results).
 constexpr int WALK_MASK = 0;
 constexpr int WALK_QUEUE = 1;
+constexpr int WALK_LIST = 2;
+constexpr int LIST_MAX = 128;
+// List layout: 64-particle blocks (one wave), slot-major inside a block, so
+// the lanes of a wave reading their k-th neighbours touch 256 contiguous bytes.
+__device__ __forceinline__ size_t list_at(int i, int k) {
+    return ((size_t)(i >> 6) * LIST_MAX + (size_t)k) * 64 + (size_t)(i & 63);
+}
+inline size_t list_words(int n) { return ((size_t)(n + 63) >> 6) * LIST_MAX * 64; }
 constexpr int QCAP = 32;   // queue entries per thread, stored [slot][thread]
 constexpr int QCHUNK = 8;  // candidates tested per lane between queue checks
 template <int WALK>
@@ -225,7 +239,7 @@ __device__ __forceinline__ void for_each_neighbor(const Prm &q, int i, float4 xi
         const float4 xj = posr[j];
         return dist2(xi.x - xj.x, xi.y - xj.y, xi.z - xj.z) <= q.rc2 && j != i;
     };
-    if (WALK == WALK_MASK) {
+    if (WALK != WALK_QUEUE) {  // WALK_MASK (also the walk WALK_LIST falls back to)
         for_each_row<FLUID_J, DUMMY_J>(q, xi, cell, cs, [&](int fb, int fe, int db, int de) {
             const int nfl = fe - fb;
             const int total = nfl + (de - db);
@@ -278,6 +292,59 @@ __device__ __forceinline__ void for_each_neighbor(const Prm &q, int i, float4 xi
             if (DUMMY_J) scan(db, de);
         });
         drain_all();
+    }
+}
+
+// WALK_LIST, recording side: stores the neighbours of i in walk order (the
+// first LIST_MAX of them) and returns how many there are.
+template <bool FLUID_J, bool DUMMY_J>
+__device__ __forceinline__ int build_list(const Prm &q, int i, float4 xi, uint32_t cell, const int *__restrict__ cs,
+                                          const float4 *__restrict__ posr, int *__restrict__ list) {
+    int cnt = 0;
+    auto seg = [&](int jb, int je) {
+        for (int j = jb; j < je; ++j) {
+            const float4 xj = posr[j];
+            if (dist2(xi.x - xj.x, xi.y - xj.y, xi.z - xj.z) <= q.rc2 && j != i) {
+                if (cnt < LIST_MAX) list[list_at(i, cnt)] = j;
+                ++cnt;
+            }
+        }
+    };
+    for_each_row<FLUID_J, DUMMY_J>(q, xi, cell, cs, [&](int fb, int fe, int db, int de) {
+        if (FLUID_J) seg(fb, fe);
+        if (DUMMY_J) seg(db, de);
+    });
+    return cnt;
+}
+
+// WALK_LIST, visiting side: f for each recorded neighbour of i, in walk
+// order; past LIST_MAX the walk is repeated and the recorded ones skipped.
+template <bool FLUID_J, bool DUMMY_J, class F>
+__device__ __forceinline__ void visit_list(const Prm &q, int i, float4 xi, uint32_t cell, const int *__restrict__ cs,
+                                           const float4 *__restrict__ posr, const int *__restrict__ list, int cnt,
+                                           F &&f) {
+    auto visit = [&](int j) {
+        const float4 xj = posr[j];
+        const float dx = xi.x - xj.x, dy = xi.y - xj.y, dz = xi.z - xj.z;
+        f(j, dx, dy, dz, dist2(dx, dy, dz), xj);
+    };
+    const int n = min(cnt, LIST_MAX);
+    for (int k = 0; k < n; ++k) visit(list[list_at(i, k)]);
+    if (cnt > LIST_MAX) {
+        int seen = 0;
+        auto seg = [&](int jb, int je) {
+            for (int j = jb; j < je; ++j) {
+                const float4 xj = posr[j];
+                if (dist2(xi.x - xj.x, xi.y - xj.y, xi.z - xj.z) <= q.rc2 && j != i) {
+                    if (seen >= LIST_MAX) visit(j);
+                    ++seen;
+                }
+            }
+        };
+        for_each_row<FLUID_J, DUMMY_J>(q, xi, cell, cs, [&](int fb, int fe, int db, int de) {
+            if (FLUID_J) seg(fb, fe);
+            if (DUMMY_J) seg(db, de);
+        });
     }
 }
 

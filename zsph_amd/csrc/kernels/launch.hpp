@@ -20,12 +20,15 @@ int launch_sort(hipStream_t s, int n, int keybits, int bits, uint32_t *key[2], i
 void launch_cell_start(hipStream_t s, const uint32_t *skey, int n, int G, uint32_t flag, int *cs);
 void launch_reorder(hipStream_t s, int n, const int *perm, const Buf &a, const Buf &b);
 
-// physics.hip; walk = WALK_MASK or WALK_QUEUE (common.hpp)
+// physics.hip; walk = WALK_MASK, WALK_QUEUE or WALK_LIST (common.hpp).
+// WALK_LIST: launch_ldg records the neighbour lists (nlist: list_words(N)
+// ints, ncount: N ints) that launch_governing then reads; a caller without a
+// current list passes another walk to launch_governing.
 void launch_wall(hipStream_t s, int walk, const Prm &q, const Buf &b, const uint32_t *skey, const int *cs);
 void launch_ldg(hipStream_t s, int walk, const Prm &q, const Buf &b, const uint32_t *skey, const int *cs,
-                float4 *gd);
+                float4 *gd, int *nlist, int *ncount);
 void launch_governing(hipStream_t s, int walk, const Prm &q, const Buf &b, const uint32_t *skey, const int *cs,
-                      const float4 *gd, float4 *acc);
+                      const float4 *gd, float4 *acc, const int *nlist, const int *ncount);
 void launch_predict(hipStream_t s, const Prm &q, const Buf &b, const int *cs, const float4 *acc);
 void launch_correct(hipStream_t s, const Prm &q, const Buf &b, const int *cs, const float4 *acc);
 void launch_nbr_stats(hipStream_t s, int walk, const Prm &q, const Buf &b, const uint32_t *skey, const int *cs,

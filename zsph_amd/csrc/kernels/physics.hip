@@ -61,7 +61,8 @@ __global__ __launch_bounds__(ZS_BLOCK) void k_wall(Prm q, Buf b, const uint32_t 
 // not depend on j).  MC form: D only.  Output gd = (G, D); dummies get 0.
 template <int WALK>
 __global__ __launch_bounds__(ZS_BLOCK) void k_ldg(Prm q, Buf b, const uint32_t *__restrict__ skey,
-                                                  const int *__restrict__ cs, float4 *__restrict__ gd) {
+                                                  const int *__restrict__ cs, float4 *__restrict__ gd,
+                                                  int *__restrict__ nlist, int *__restrict__ ncount) {
     __shared__ int qbuf[WalkLds<WALK>::ints];
     const int nf = cs[q.G];
     const int i = xcd_block(gridDim.x) * ZS_BLOCK + threadIdx.x;
@@ -75,22 +76,29 @@ __global__ __launch_bounds__(ZS_BLOCK) void k_ldg(Prm q, Buf b, const uint32_t *
     const float4 vi = b.velp[i];
     float lxx = 0, lxy = 0, lxz = 0, lyy = 0, lyz = 0, lzz = 0;
     float gx = 0, gy = 0, gz = 0, D = 0;
-    for_each_neighbor<WALK, true, true>(q, i, xi, cell_of(q, skey[i]), cs, b.posr, qbuf,
-                                  [&](int j, float dx, float dy, float dz, float r2, float4 xj) {
-                                      const float qq = sqrtf(r2) * q.inv_h;
-                                      const float t = 1.0f - 0.5f * qq;
-                                      const float FV = q.fk * t * t * t * q.m / xj.w;
-                                      if (delta) {
-                                          lxx -= FV * dx * dx; lxy -= FV * dx * dy; lxz -= FV * dx * dz;
-                                          lyy -= FV * dy * dy; lyz -= FV * dy * dz; lzz -= FV * dz * dz;
-                                          const float c = FV * (xj.w - xi.w);
-                                          gx += c * dx; gy += c * dy; gz += c * dz;
-                                      }
-                                      if (j < nf) {
-                                          const float4 vj = b.velp[j];
-                                          D -= FV * ((vi.x - vj.x) * dx + (vi.y - vj.y) * dy + (vi.z - vj.z) * dz);
-                                      }
-                                  });
+    auto body = [&](int j, float dx, float dy, float dz, float r2, float4 xj) {
+        const float qq = sqrtf(r2) * q.inv_h;
+        const float t = 1.0f - 0.5f * qq;
+        const float FV = q.fk * t * t * t * q.m / xj.w;
+        if (delta) {
+            lxx -= FV * dx * dx; lxy -= FV * dx * dy; lxz -= FV * dx * dz;
+            lyy -= FV * dy * dy; lyz -= FV * dy * dz; lzz -= FV * dz * dz;
+            const float c = FV * (xj.w - xi.w);
+            gx += c * dx; gy += c * dy; gz += c * dz;
+        }
+        if (j < nf) {
+            const float4 vj = b.velp[j];
+            D -= FV * ((vi.x - vj.x) * dx + (vi.y - vj.y) * dy + (vi.z - vj.z) * dz);
+        }
+    };
+    const uint32_t cell = cell_of(q, skey[i]);
+    if (WALK == WALK_LIST) {
+        const int cnt = build_list<true, true>(q, i, xi, cell, cs, b.posr, nlist);
+        ncount[i] = cnt;
+        visit_list<true, true>(q, i, xi, cell, cs, b.posr, nlist, cnt, body);
+    } else {
+        for_each_neighbor<WALK, true, true>(q, i, xi, cell, cs, b.posr, qbuf, body);
+    }
     if (delta) {
         if (q.dim == 2) lyy += 1.0f;  // x-z plane: unit yy entry keeps L invertible
         const float c00 = lyy * lzz - lyz * lyz;
@@ -119,7 +127,8 @@ __global__ __launch_bounds__(ZS_BLOCK) void k_ldg(Prm q, Buf b, const uint32_t *
 template <int WALK>
 __global__ __launch_bounds__(ZS_BLOCK) void k_governing(Prm q, Buf b, const uint32_t *__restrict__ skey,
                                                         const int *__restrict__ cs, const float4 *__restrict__ gd,
-                                                        float4 *__restrict__ acc) {
+                                                        float4 *__restrict__ acc, const int *__restrict__ nlist,
+                                                        const int *__restrict__ ncount) {
     __shared__ int qbuf[WalkLds<WALK>::ints];
     const int nf = cs[q.G];
     const int i = xcd_block(gridDim.x) * ZS_BLOCK + threadIdx.x;
@@ -130,8 +139,7 @@ __global__ __launch_bounds__(ZS_BLOCK) void k_governing(Prm q, Buf b, const uint
     const float4 gi = gd[i];
     const float rhoi = xi.w, pi = vi.w;
     float ax = 0, ay = 0, az = 0, dr = 0;
-    for_each_neighbor<WALK, true, true>(q, i, xi, cell_of(q, skey[i]), cs, b.posr, qbuf,
-                                  [&](int j, float dx, float dy, float dz, float r2, float4 xj) {
+    auto body = [&](int j, float dx, float dy, float dz, float r2, float4 xj) {
         const float qq = sqrtf(r2) * q.inv_h;
         const float t = 1.0f - 0.5f * qq;
         const float F = q.fk * t * t * t;
@@ -154,7 +162,12 @@ __global__ __launch_bounds__(ZS_BLOCK) void k_governing(Prm q, Buf b, const uint
         }
         if (!delta) dr += (rhoi - rhoj) * r2 * F * q.delta_h_cs / (r2 + q.eta2) * Vj;
         ax += pt * dx; ay += pt * dy; az += pt * dz;
-    });
+    };
+    const uint32_t cell = cell_of(q, skey[i]);
+    if (WALK == WALK_LIST)
+        visit_list<true, true>(q, i, xi, cell, cs, b.posr, nlist, ncount[i], body);
+    else
+        for_each_neighbor<WALK, true, true>(q, i, xi, cell, cs, b.posr, qbuf, body);
     acc[i] = make_float4(ax, ay, az + q.g, dr);
 }
 
@@ -253,6 +266,7 @@ __global__ __launch_bounds__(ZS_BLOCK) void k_nbr_stats(Prm q, Buf b, const uint
     digest[i] = h;
 }
 
+// Walks other than the queue use the mask walk where no list applies.
 #define ZS_WALK_LAUNCH(kern, grid, ...)                                          \
     do {                                                                         \
         if (walk == WALK_QUEUE)                                                  \
@@ -265,13 +279,20 @@ void launch_wall(hipStream_t s, int walk, const Prm &q, const Buf &b, const uint
     if (q.N) ZS_WALK_LAUNCH(k_wall, nblocks(q.N), q, b, skey, cs);
 }
 void launch_ldg(hipStream_t s, int walk, const Prm &q, const Buf &b, const uint32_t *skey, const int *cs,
-                float4 *gd) {
-    if (q.N) ZS_WALK_LAUNCH(k_ldg, nblocks(q.N), q, b, skey, cs, gd);
+                float4 *gd, int *nlist, int *ncount) {
+    if (!q.N) return;
+    if (walk == WALK_LIST)
+        k_ldg<WALK_LIST><<<nblocks(q.N), ZS_BLOCK, 0, s>>>(q, b, skey, cs, gd, nlist, ncount);
+    else
+        ZS_WALK_LAUNCH(k_ldg, nblocks(q.N), q, b, skey, cs, gd, nlist, ncount);
 }
 void launch_governing(hipStream_t s, int walk, const Prm &q, const Buf &b, const uint32_t *skey, const int *cs,
-                      const float4 *gd, float4 *acc) {
+                      const float4 *gd, float4 *acc, const int *nlist, const int *ncount) {
     if (!q.N) return;
-    ZS_WALK_LAUNCH(k_governing, nblocks(q.N), q, b, skey, cs, gd, acc);
+    if (walk == WALK_LIST)
+        k_governing<WALK_LIST><<<nblocks(q.N), ZS_BLOCK, 0, s>>>(q, b, skey, cs, gd, acc, nlist, ncount);
+    else
+        ZS_WALK_LAUNCH(k_governing, nblocks(q.N), q, b, skey, cs, gd, acc, nlist, ncount);
     ZS_WALK_LAUNCH(k_governing_dummy, nblocks(q.N), q, b, skey, cs, acc);
 }
 void launch_predict(hipStream_t s, const Prm &q, const Buf &b, const int *cs, const float4 *acc) {

@@ -35,6 +35,9 @@ struct zsph_ctx {
     int keybits = 1;
     int radix_bits = 8;
     int walk = zs::WALK_MASK;  // neighbour-walk variant (tuning)
+    int *nlist = nullptr;      // WALK_LIST neighbour lists (list_words(N) ints) and counts
+    int *ncount = nullptr;
+    bool list_valid = false;   // the lists describe the current sorted buffer
     hipGraph_t graph = nullptr;
     hipGraphExec_t graph_exec = nullptr;
     std::string err;
@@ -142,6 +145,7 @@ int stage_sort_data(zsph_ctx *c, int from, int to, Prof *pf = nullptr) {
         Scope sc(pf, ZSPH_ST_REORDER);
         zs::launch_reorder(c->stream, c->q.N, c->idx[c->sorted_slot], c->buf[from], c->buf[to]);
     }
+    c->list_valid = false;
     return check_launch(c, "sort_data");
 }
 int stage_boundary(zsph_ctx *c, int b, Prof *pf = nullptr) {
@@ -153,15 +157,19 @@ int stage_boundary(zsph_ctx *c, int b, Prof *pf = nullptr) {
     }
     {
         Scope sc(pf, ZSPH_ST_LDG);
-        zs::launch_ldg(c->stream, c->walk, c->q, c->buf[b], sk, c->cell_start, c->gd);
+        zs::launch_ldg(c->stream, c->walk, c->q, c->buf[b], sk, c->cell_start, c->gd, c->nlist, c->ncount);
     }
+    c->list_valid = (c->walk == zs::WALK_LIST);
     return check_launch(c, "boundary");
 }
 int stage_governing(zsph_ctx *c, int b, Prof *pf = nullptr) {
     if (c->sorted_slot < 0) return set_err(c, ZSPH_E_STATE, "zsph_governing called before zsph_sort_data");
     {
         Scope sc(pf, ZSPH_ST_GOVERNING);
-        zs::launch_governing(c->stream, c->walk, c->q, c->buf[b], c->key[c->sorted_slot], c->cell_start, c->gd, c->acc);
+        // the list walk needs this half step's lists; otherwise walk the cells
+        const int walk = (c->walk == zs::WALK_LIST && !c->list_valid) ? zs::WALK_MASK : c->walk;
+        zs::launch_governing(c->stream, walk, c->q, c->buf[b], c->key[c->sorted_slot], c->cell_start, c->gd, c->acc,
+                             c->nlist, c->ncount);
     }
     return check_launch(c, "governing");
 }
@@ -391,9 +399,15 @@ int zsph_step(zsph_ctx *c, int nsteps) {
 
 int zsph_set_tuning(zsph_ctx *c, const zsph_tuning *t) {
     if (!c || !t) return ZSPH_E_ARG;
-    if ((t->walk != ZSPH_WALK_MASK && t->walk != ZSPH_WALK_QUEUE) || t->radix_bits < 8 || t->radix_bits > 11)
+    if (t->walk < ZSPH_WALK_MASK || t->walk > ZSPH_WALK_LIST || t->radix_bits < 8 || t->radix_bits > 11)
         return set_err(c, ZSPH_E_ARG, "zsph_set_tuning: walk %d / radix_bits %d out of range", t->walk, t->radix_bits);
     if (t->walk == c->walk && t->radix_bits == c->radix_bits) return ZSPH_OK;
+    if (t->walk == ZSPH_WALK_LIST && !c->nlist) {
+        int r;
+        if ((r = dalloc(c, &c->nlist, zs::list_words(c->q.N))) || (r = dalloc(c, &c->ncount, (size_t)c->q.N)))
+            return r;
+    }
+    c->list_valid = false;
     c->walk = t->walk;
     c->radix_bits = t->radix_bits;
     // the captured step embeds the old launches
@@ -448,7 +462,7 @@ int zsph_autotune(zsph_ctx *c, int steps, zsph_tuning *best, float *best_ms) {
     int nc = 0;
     const int passes = (c->keybits + 10) / 11;  // fewest passes of <= 11-bit digits
     const int even = std::max(8, (c->keybits + passes - 1) / passes);
-    for (int walk = 0; walk < 2; ++walk) {
+    for (int walk = ZSPH_WALK_MASK; walk <= ZSPH_WALK_LIST; ++walk) {
         cand[nc++] = {walk, even};
         if (even != 8) cand[nc++] = {walk, 8};
     }
