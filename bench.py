@@ -67,6 +67,7 @@ def main():
     ap.add_argument("--profile-steps", type=int, default=3)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-autotune", dest="autotune", action="store_false")
+    ap.add_argument("--walk", type=int, default=None, help="force a neighbour walk (0 mask, 1 queue, 2 list)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); the step is single-GPU.",
@@ -82,6 +83,9 @@ def main():
     s = z.Solver(case, device=0)
     s.upload(case.host)
     # pick the neighbour-walk variant and sort digit width on this case (state restored after)
+    if args.walk is not None:
+        s.set_tuning(args.walk, s.get_tuning()["radix_bits"])
+        args.autotune = False
     tune = s.autotune(steps=3) if args.autotune else s.get_tuning()
     print(f"[bench] tuning: {tune}", file=sys.stderr, flush=True)
     s.step(args.warmup)
