@@ -335,6 +335,63 @@ int zsph_diagnostics(zsph_ctx *ctx, int buf, float front_zmax, zsph_diag *out);
  * Buffer `buf` must be in cell order (after zsph_sort_data or zsph_step). */
 int zsph_probe_pressure(zsph_ctx *ctx, int buf, const float *pts, int npts, float *p_out);
 
+/* ------------------------------------------------------------------------ */
+/* Moving dummies: floating body (6-DOF coupling) and piston wavemaker.     */
+/* The reference has the rigid-body equations only as a standalone test    */
+/* (Test_6DOF/) and zero wall-acceleration placeholders in the dummy        */
+/* extrapolation (equation/delta_ns.cu:83); the coupling here is new.       */
+/* ------------------------------------------------------------------------ */
+
+/* Make the type-2 (ZSPH_BODY) particles of buffer 0 one rigid body with
+ * state *r (c, vel, omega, q, I about c in the body frame, mass).  Their
+ * body-frame offsets are taken from their current positions.  Each half step
+ * then: sums the fluid's force and torque on the body particles (their
+ * governing accelerations, gravity excluded), adds M g, integrates the FP64
+ * state with the fluid's midpoint predictor-corrector, and moves the body
+ * particles rigidly (x = c + R r0, v = V + w x R r0, wall acceleration
+ * A + alpha x R r0 + w x (w x R r0) for the dummy pressure). */
+int zsph_set_body(zsph_ctx *ctx, const zsph_rigid *r);
+/* Current body state; F, T are the fluid force and torque of the last half
+ * step (without gravity), acc and alpha the resulting accelerations. */
+int zsph_get_body(zsph_ctx *ctx, zsph_rigid *r);
+
+/* Piston: type-3 (ZSPH_PISTON) particles of buffer 0 move rigidly along
+ * `axis` (0 x, 1 y, 2 z) with displacement
+ *   d(t) = amplitude (sin(2 pi frequency t + phase) - sin(phase)),
+ * x(t) = x0 + d(t) e, where x0 is chosen so that x(t_call) is the position at
+ * the call (t_call = the simulated time, zsph_get_time).  Velocity d'(t) e;
+ * the wall acceleration d''(t) e enters the dummy pressure.
+ * zsph_upload into buffer 0 ends both the body and the piston set-up (new
+ * particles); call zsph_set_body / zsph_set_piston again after it. */
+typedef struct zsph_piston {
+    double amplitude, frequency, phase;
+    int axis;
+} zsph_piston;
+int zsph_set_piston(zsph_ctx *ctx, const zsph_piston *p);
+
+/* Simulated time (s); zsph_step advances it by dt per step. */
+int zsph_set_time(zsph_ctx *ctx, double t);
+int zsph_get_time(zsph_ctx *ctx, double *t);
+
+/* Force and torque (about the body centre c) of the fluid on the body
+ * particles of buffer `buf`, from the last zsph_governing, FP64 sums of
+ * FP32 terms: the body-force reduction on its own. */
+int zsph_body_reduce(zsph_ctx *ctx, int buf, double F[3], double T[3]);
+
+/* Wave-tank case (config 2): fluid of depth `depth` in [0,lx]x[0,wy], a
+ * piston of `layers` dummy layers behind x = 0 over the tank height `lz`,
+ * fixed floor and walls elsewhere, and a floating box [box_lo, box_hi]
+ * (a `layers`-thick shell of type-2 particles on the fluid's lattice, no
+ * fluid inside).  Same two-call protocol as zsph_case_dambreak. */
+typedef struct zsph_wavetank {
+    float dx;
+    float lx, wy, lz, depth;
+    float box_lo[3], box_hi[3];  /* box_hi <= box_lo on any axis: no box */
+    int layers;
+    float piston_travel;         /* floor extends this far behind the piston */
+} zsph_wavetank;
+int zsph_case_wavetank(const zsph_wavetank *c, zsph_json *j, zsph_host_state *st, int *n_out);
+
 /* Launch tuning (no counterpart in the reference, which launches 256-thread
  * blocks, deltasph/main.cu:55-56).  Results do not depend on it: both
  * neighbour walks visit neighbours in the same order, and the stable sort
@@ -360,7 +417,7 @@ int zsph_killed(zsph_ctx *ctx, int *killed);
 
 /* sizeof of the ABI structs, so bindings can check their mirrors:
  * 0 zsph_json, 1 zsph_cpu_param, 2 zsph_params, 3 zsph_host_state, 4 zsph_dambreak,
- * 5 zsph_rigid, 6 zsph_diag. */
+ * 5 zsph_rigid, 6 zsph_diag, 7 zsph_tuning, 8 zsph_piston, 9 zsph_wavetank. */
 size_t zsph_sizeof(int which);
 
 #ifdef __cplusplus

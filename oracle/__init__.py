@@ -49,6 +49,14 @@ def lib():
         sig("orc_cell", C.c_int, [C.c_float, C.c_float, C.c_float, C.c_int])
         sig("orc_diagnostics", C.c_int, [vp, C.c_int, C.c_float, P(abi.Diag)])
         sig("orc_probe_pressure", C.c_int, [vp, C.c_int, P(C.c_float), C.c_int, P(C.c_double)])
+        sig("orc_set_body", C.c_int, [vp, P(abi.Rigid)])
+        sig("orc_get_body", C.c_int, [vp, P(abi.Rigid)])
+        sig("orc_set_piston", C.c_int, [vp, C.c_double, C.c_double, C.c_double, C.c_int])
+        sig("orc_set_time", C.c_int, [vp, C.c_double])
+        sig("orc_get_time", C.c_int, [vp, P(C.c_double)])
+        sig("orc_body_reduce", C.c_int, [vp, C.c_int, P(C.c_double), P(C.c_double)])
+        sig("orc_rigid_acc_alpha", None, [P(abi.Rigid), C.c_double])
+        sig("orc_rigid_advance", None, [P(abi.Rigid), P(abi.Rigid), C.c_double, P(abi.Rigid)])
         _LIB = L
     return _LIB
 
@@ -145,9 +153,50 @@ class Oracle:
                                           dig.ctypes.data_as(C.POINTER(C.c_uint64))), "neighbor_stats")
         return cnt, dig
 
+    # ---- moving dummies (buffer 0) -------------------------------------------
+    def set_body(self, rigid):
+        self._ok(lib().orc_set_body(self._h, C.byref(rigid)), "set_body")
+
+    def get_body(self):
+        from zsph_amd import _abi as abi
+        r = abi.Rigid()
+        self._ok(lib().orc_get_body(self._h, C.byref(r)), "get_body")
+        return r
+
+    def set_piston(self, amplitude, frequency, phase=0.0, axis=0):
+        self._ok(lib().orc_set_piston(self._h, amplitude, frequency, phase, axis), "set_piston")
+
+    def set_time(self, t):
+        self._ok(lib().orc_set_time(self._h, t), "set_time")
+
+    def time(self):
+        t = C.c_double(0.0)
+        self._ok(lib().orc_get_time(self._h, C.byref(t)), "get_time")
+        return t.value
+
+    def body_reduce(self, buf=1):
+        F = (C.c_double * 3)()
+        T = (C.c_double * 3)()
+        self._ok(lib().orc_body_reduce(self._h, buf, F, T), "body_reduce")
+        return np.array(F[:]), np.array(T[:])
+
 
 def kernel_W(params, r): return lib().orc_kernel_W(C.byref(params), r)
 def kernel_F(params, r): return lib().orc_kernel_F(C.byref(params), r)
 def eos_p(params, rho): return lib().orc_eos_p(C.byref(params), rho)
 def eos_rho(params, p): return lib().orc_eos_rho(C.byref(params), p)
 def cell(x, o, size, dim): return lib().orc_cell(x, o, size, dim)
+
+
+
+def rigid_acc_alpha(rigid, g=0.0):
+    """The oracle's FP64 accelerations of `rigid` (abi.Rigid, in place), gravity g along z."""
+    lib().orc_rigid_acc_alpha(C.byref(rigid), g)
+
+
+def rigid_advance(s0, rates, dt):
+    """(c, v, w, q) of s0 advanced by dt with the rates (v, a, w, alpha, q) of `rates`."""
+    from zsph_amd import _abi as abi
+    out = abi.Rigid()
+    lib().orc_rigid_advance(C.byref(s0), C.byref(rates), dt, C.byref(out))
+    return out

@@ -159,6 +159,25 @@ int orc_probe_pressure(void *h, int b, const float *pts, int np, double *out) {
     return ORC_CALL(o.probes(b, pts, np, out));
 }
 
+// Moving dummies (floating body, piston) on buffer 0; time in seconds.
+int orc_set_body(void *h, const zsph_rigid *r) {
+    return dispatch<double>(static_cast<Handle *>(h), [&](auto &o) { return o.set_body(*r); });
+}
+int orc_get_body(void *h, zsph_rigid *r) { return ORC_CALL(*r = o.bs); }
+int orc_set_piston(void *h, double amp, double freq, double phase, int axis) {
+    return dispatch<double>(static_cast<Handle *>(h), [&](auto &o) { return o.set_piston(amp, freq, phase, axis); });
+}
+int orc_set_time(void *h, double t) { return ORC_CALL(o.t = o.t0 = t); }
+int orc_get_time(void *h, double *t) { return ORC_CALL(*t = o.t); }
+int orc_body_reduce(void *h, int b, double *F, double *T) { return ORC_CALL(o.body_force(b, F, T)); }
+
+// The oracle's rigid-body accelerations (gravity g along z) and midpoint
+// building block, for tests against the reference's Test_6DOF routines.
+void orc_rigid_acc_alpha(zsph_rigid *r, double g) { orc::Oracle<double>::body_accelerations(*r, g); }
+void orc_rigid_advance(const zsph_rigid *s0, const zsph_rigid *rates, double dt, zsph_rigid *out) {
+    orc::Oracle<double>::body_advance(*s0, *rates, dt, *out);
+}
+
 // Scalar kernels of the oracle, exposed for closed-form tests (a particle-
 // free oracle carries only the constants).
 static orc::Oracle<double> lite(const zsph_params *p) {

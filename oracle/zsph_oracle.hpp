@@ -155,7 +155,7 @@ struct Oracle {
                 b.type[i] = ZSPH_KILLED;
                 b.posr[4 * i] = R(p.grid_xmin); b.posr[4 * i + 1] = R(p.grid_ymin);
                 b.posr[4 * i + 2] = R(p.grid_zmin); b.posr[4 * i + 3] = R(p.rho0);
-                for (int c = 0; c < 4; ++c) b.velp[4 * i + c] = R(0);
+                for (int c = 0; c < 4; ++c) b.velp[4 * i + c] = b.tvelp[4 * i + c] = R(0);
                 key[i] = flag;
                 ++kills;
             }
@@ -202,7 +202,9 @@ struct Oracle {
     }
 
     // Dummy pressure (delta_ns.cu:81-85, :116-131): Shepard sum over fluid
-    // neighbours of p_j - rho_j (a_w - g).x_wj with a_w = 0, g = (0,0,g).
+    // neighbours of p_j - rho_j (a_w - g).x_wj, g = (0,0,g).  a_w is the
+    // dummy's prescribed acceleration kept in tvelp.xyz (zero in the
+    // reference, which has no moving dummies; zero here for fixed walls).
     void wall(int bi) {
         Buffer<R> &b = buf[bi];
         std::vector<R> newr(N), newp(N);
@@ -217,7 +219,8 @@ struct Oracle {
                 const R dz = b.posr[4 * i + 2] - b.posr[4 * j + 2];
                 const R W = kernel_W(std::sqrt(dx * dx + dy * dy + dz * dz));
                 const R gx = 0, gy = 0, gz = R(p.g);
-                const R aw_minus_g_dot_x = (R(0) - gx) * dx + (R(0) - gy) * dy + (R(0) - gz) * dz;
+                const R *aw = &b.tvelp[4 * i];
+                const R aw_minus_g_dot_x = (aw[0] - gx) * dx + (aw[1] - gy) * dy + (aw[2] - gz) * dz;
                 P += (b.velp[4 * j + 3] - b.posr[4 * j + 3] * aw_minus_g_dot_x) * W;
                 S += W;
             });
@@ -364,8 +367,208 @@ struct Oracle {
         }
     }
 
+    // ---- moving dummies: floating body and piston ---------------------------------
+    // Rigid equations of Test_6DOF/equation/acc_alpha.cpp:3-35 and
+    // time_int.cpp:3-22 in FP64, integrated with the fluid's midpoint
+    // scheme; restated here independently of the product (body-frame
+    // Gaussian elimination instead of an explicit inverse).
+    zsph_rigid bs{}, bs0{};  // body state now / at the start of the step
+    double t = 0.0, t0 = 0.0;
+    bool has_body = false, has_piston = false;
+    double p_amp = 0.0, p_w = 0.0, p_phase = 0.0;
+    int p_axis = 0;
+
+    static void rotm(const double q[4], double M[3][3]) {
+        const double a = q[0], b = q[1], c = q[2], d = q[3];
+        M[0][0] = a * a + b * b - c * c - d * d; M[0][1] = 2 * (b * c - a * d); M[0][2] = 2 * (b * d + a * c);
+        M[1][0] = 2 * (b * c + a * d); M[1][1] = a * a - b * b + c * c - d * d; M[1][2] = 2 * (c * d - a * b);
+        M[2][0] = 2 * (b * d - a * c); M[2][1] = 2 * (c * d + a * b); M[2][2] = a * a - b * b - c * c + d * d;
+    }
+    static void mv(const double M[3][3], const double v[3], double o[3], bool transpose = false) {
+        for (int r = 0; r < 3; ++r) {
+            o[r] = 0.0;
+            for (int c = 0; c < 3; ++c) o[r] += (transpose ? M[c][r] : M[r][c]) * v[c];
+        }
+    }
+    static void crs(const double a[3], const double b[3], double o[3]) {
+        const double x = a[1] * b[2] - a[2] * b[1], y = a[2] * b[0] - a[0] * b[2], z = a[0] * b[1] - a[1] * b[0];
+        o[0] = x; o[1] = y; o[2] = z;
+    }
+    // Solve A x = y (3x3, partial pivoting).
+    static void gauss3(const double Ain[9], const double y[3], double x[3]) {
+        double A[3][4];
+        for (int r = 0; r < 3; ++r) { for (int c = 0; c < 3; ++c) A[r][c] = Ain[3 * r + c]; A[r][3] = y[r]; }
+        for (int k = 0; k < 3; ++k) {
+            int piv = k;
+            for (int r = k + 1; r < 3; ++r) if (std::fabs(A[r][k]) > std::fabs(A[piv][k])) piv = r;
+            for (int c = 0; c < 4; ++c) std::swap(A[k][c], A[piv][c]);
+            for (int r = k + 1; r < 3; ++r) {
+                const double f = A[r][k] / A[k][k];
+                for (int c = k; c < 4; ++c) A[r][c] -= f * A[k][c];
+            }
+        }
+        for (int k = 2; k >= 0; --k) {
+            double v = A[k][3];
+            for (int c = k + 1; c < 3; ++c) v -= A[k][c] * x[c];
+            x[k] = v / A[k][k];
+        }
+    }
+    // a = (F + M g) / M;  alpha: I (R^T alpha) = R^T (T - w x (R I R^T w))   (acc_alpha.cpp:5-31)
+    static void body_accelerations(zsph_rigid &s, double g) {
+        for (int k = 0; k < 3; ++k) s.acc[k] = s.F[k] / s.mass;
+        s.acc[2] += g;
+        double Rm[3][3], wb[3], Iwb[3], Lw[3], wxL[3], rhs[3], tb[3], ab[3];
+        rotm(s.q, Rm);
+        mv(Rm, s.omega, wb, true);  // w in the body frame
+        for (int r = 0; r < 3; ++r) Iwb[r] = s.I[3 * r] * wb[0] + s.I[3 * r + 1] * wb[1] + s.I[3 * r + 2] * wb[2];
+        mv(Rm, Iwb, Lw);            // angular momentum, world frame
+        crs(s.omega, Lw, wxL);
+        for (int k = 0; k < 3; ++k) rhs[k] = s.T[k] - wxL[k];
+        mv(Rm, rhs, tb, true);
+        gauss3(s.I, tb, ab);
+        mv(Rm, ab, s.alpha);
+    }
+    // (c, v, w, q) = s0 + dt * rates(s): c' = v, v' = a, w' = alpha, q' = 1/2 (0,w) q.
+    static void body_advance(const zsph_rigid &s0, const zsph_rigid &r, double dt, zsph_rigid &out) {
+        const double w[3] = {r.omega[0], r.omega[1], r.omega[2]};
+        const double *q = r.q;
+        const double dq[4] = {0.5 * (-w[0] * q[1] - w[1] * q[2] - w[2] * q[3]),
+                              0.5 * (w[0] * q[0] + w[1] * q[3] - w[2] * q[2]),
+                              0.5 * (w[1] * q[0] + w[2] * q[1] - w[0] * q[3]),
+                              0.5 * (w[2] * q[0] + w[0] * q[2] - w[1] * q[1])};
+        zsph_rigid o = r;
+        for (int k = 0; k < 3; ++k) {
+            o.c[k] = s0.c[k] + r.vel[k] * dt;
+            o.vel[k] = s0.vel[k] + r.acc[k] * dt;
+            o.omega[k] = s0.omega[k] + r.alpha[k] * dt;
+        }
+        double nq = 0.0;
+        for (int k = 0; k < 4; ++k) { o.q[k] = s0.q[k] + dq[k] * dt; nq += o.q[k] * o.q[k]; }
+        nq = std::sqrt(nq);
+        for (int k = 0; k < 4; ++k) o.q[k] /= nq;
+        out = o;
+    }
+    // Fluid force and torque on the body particles of buffer bi (last governing).
+    void body_force(int bi, double F[3], double T[3]) const {
+        const Buffer<R> &b = buf[bi];
+        for (int k = 0; k < 3; ++k) F[k] = T[k] = 0.0;
+        for (int i = 0; i < N; ++i) {
+            if (b.type[i] != ZSPH_BODY) continue;
+            double f[3], d[3], tq[3];
+            for (int k = 0; k < 3; ++k) {
+                f[k] = (double)p.m * (double)acc[4 * i + k];
+                d[k] = (double)b.posr[4 * i + k] - bs.c[k];
+            }
+            crs(d, f, tq);
+            for (int k = 0; k < 3; ++k) { F[k] += f[k]; T[k] += tq[k]; }
+        }
+    }
+    void piston_motion(double tt, double &d, double &dv, double &da) const {
+        const double ph = p_w * tt + p_phase;
+        d = p_amp * (std::sin(ph) - std::sin(p_phase));
+        dv = p_amp * p_w * std::cos(ph);
+        da = -p_amp * p_w * p_w * std::sin(ph);
+    }
+    // Body and piston dummies of buffer bi at the current time.
+    void kinematics(int bi) {
+        Buffer<R> &b = buf[bi];
+        double Rm[3][3];
+        rotm(bs.q, Rm);
+        double d = 0, dv = 0, da = 0;
+        if (has_piston) piston_motion(t, d, dv, da);
+        for (int i = 0; i < N; ++i) {
+            double x[3], v[3], a[3];
+            if (b.type[i] == ZSPH_BODY && has_body) {
+                const double r0[3] = {(double)b.tposr[4 * i], (double)b.tposr[4 * i + 1], (double)b.tposr[4 * i + 2]};
+                double r[3], wr[3], wwr[3], ar[3];
+                mv(Rm, r0, r);
+                crs(bs.omega, r, wr);
+                crs(bs.omega, wr, wwr);
+                crs(bs.alpha, r, ar);
+                for (int k = 0; k < 3; ++k) {
+                    x[k] = bs.c[k] + r[k];
+                    v[k] = bs.vel[k] + wr[k];
+                    a[k] = bs.acc[k] + ar[k] + wwr[k];
+                }
+            } else if (b.type[i] == ZSPH_PISTON && has_piston) {
+                for (int k = 0; k < 3; ++k) { x[k] = (double)b.tposr[4 * i + k]; v[k] = a[k] = 0.0; }
+                x[p_axis] += d; v[p_axis] = dv; a[p_axis] = da;
+            } else {
+                continue;
+            }
+            for (int k = 0; k < 3; ++k) { b.posr[4 * i + k] = R(x[k]); b.velp[4 * i + k] = R(v[k]); b.tvelp[4 * i + k] = R(a[k]); }
+        }
+    }
+    // Body force, accelerations, midpoint update and time of one half step.
+    void body_half(int bi, bool predictor) {
+        if (!has_body && !has_piston) return;
+        if (predictor) t0 = t;
+        const double dt = predictor ? (double)p.half_dt : (double)p.half_dt * 2.0;
+        t = t0 + dt;
+        if (has_body) {
+            body_force(bi, bs.F, bs.T);
+            body_accelerations(bs, (double)p.g);
+            if (predictor) bs0 = bs;
+            body_advance(bs0, bs, dt, bs);
+        }
+    }
+    // Body-frame offsets r0 = R^T (x - c) (stored in FP32, as the product
+    // stores them), v = V + w x d, a_w = A + alpha x d + w x (w x d) on buffer 0.
+    int set_body(const zsph_rigid &r) {
+        bs = r;
+        double qn = 0.0;
+        for (int k = 0; k < 4; ++k) qn += r.q[k] * r.q[k];
+        qn = std::sqrt(qn);
+        for (int k = 0; k < 4; ++k) bs.q[k] = r.q[k] / qn;
+        double Rm[3][3];
+        rotm(bs.q, Rm);
+        Buffer<R> &b = buf[0];
+        int n = 0;
+        for (int i = 0; i < N; ++i) {
+            if (b.type[i] != ZSPH_BODY) continue;
+            ++n;
+            double d[3], r0[3], wd[3], wwd[3], ad[3];
+            for (int k = 0; k < 3; ++k) d[k] = (double)b.posr[4 * i + k] - bs.c[k];
+            mv(Rm, d, r0, true);
+            crs(bs.omega, d, wd);
+            crs(bs.omega, wd, wwd);
+            crs(bs.alpha, d, ad);
+            for (int k = 0; k < 3; ++k) {
+                b.tposr[4 * i + k] = R((float)r0[k]);
+                b.velp[4 * i + k] = R(bs.vel[k] + wd[k]);
+                b.tvelp[4 * i + k] = R(bs.acc[k] + ad[k] + wwd[k]);
+            }
+        }
+        if (!n) return ZSPH_E_ARG;
+        bs0 = bs;
+        has_body = true;
+        return ZSPH_OK;
+    }
+    int set_piston(double amp, double freq, double phase, int axis) {
+        p_amp = amp; p_w = 2.0 * M_PI * freq; p_phase = phase; p_axis = axis;
+        double d, dv, da;
+        piston_motion(t, d, dv, da);
+        Buffer<R> &b = buf[0];
+        int n = 0;
+        for (int i = 0; i < N; ++i) {
+            if (b.type[i] != ZSPH_PISTON) continue;
+            ++n;
+            for (int k = 0; k < 3; ++k) {
+                b.tposr[4 * i + k] = b.posr[4 * i + k];
+                b.velp[4 * i + k] = b.tvelp[4 * i + k] = R(0);
+            }
+            b.tposr[4 * i + axis] = R((float)((double)(float)b.posr[4 * i + axis] - d));
+            b.velp[4 * i + axis] = R(dv);
+            b.tvelp[4 * i + axis] = R(da);
+        }
+        if (!n) return ZSPH_E_ARG;
+        has_piston = true;
+        return ZSPH_OK;
+    }
+
     // time_int.cu:3-48
     void prediction(int bi) {
+        body_half(bi, true);
         Buffer<R> &b = buf[bi];
         const R hdt = R(p.half_dt);
 #pragma omp parallel for schedule(static)
@@ -381,9 +584,11 @@ struct Oracle {
             b.posr[4 * i + 3] = rho;
             b.velp[4 * i + 3] = eos_p(rho);
         }
+        if (has_body || has_piston) kinematics(bi);
     }
     // time_int.cu:50-91 (dt = 2 half_dt)
     void correction(int bi) {
+        body_half(bi, false);
         Buffer<R> &b = buf[bi];
         const R dt = R(p.half_dt) * R(2);
 #pragma omp parallel for schedule(static)
@@ -399,6 +604,7 @@ struct Oracle {
             b.posr[4 * i + 3] = rho;
             b.velp[4 * i + 3] = eos_p(rho);
         }
+        if (has_body || has_piston) kinematics(bi);
     }
 
     void boundary(int bi) { wall(bi); ldg(bi); }

@@ -112,6 +112,27 @@ class Case:
         return cls(j, host)
 
     @classmethod
+    def wavetank(cls, dx, lx, wy, lz, depth, box_lo=(0.0, 0.0, 0.0), box_hi=(0.0, 0.0, 0.0), layers=3,
+                 piston_travel=0.0, gamma=7.0, cfl=0.2, diffusion=DIFF_ANTUONO, **overrides):
+        """Wave tank with a piston and a floating box (zsph_case_wavetank).  box_hi <= box_lo: no box."""
+        j = cls.defaults()
+        j.gamma, j.cfl, j.diffusion = gamma, cfl, diffusion
+        for k, v in overrides.items():
+            setattr(j, k, v)
+        w = abi.WaveTank(dx, lx, wy, lz, depth, (C.c_float * 3)(*box_lo), (C.c_float * 3)(*box_hi), layers,
+                         piston_travel)
+        n = C.c_int(0)
+        rc = lib().zsph_case_wavetank(C.byref(w), C.byref(j), None, C.byref(n))
+        if rc != abi.OK:
+            raise ZsphError(rc, "zsph_case_wavetank")
+        host = HostArrays(n.value)
+        st = host.state()
+        rc = lib().zsph_case_wavetank(C.byref(w), C.byref(j), C.byref(st), C.byref(n))
+        if rc != abi.OK:
+            raise ZsphError(rc, "zsph_case_wavetank")
+        return cls(j, host)
+
+    @classmethod
     def from_json(cls, path):
         """Case file + <input_path>/input.vtk, like the reference's main (deltasph/main.cu:13-20)."""
         j = abi.Json()
@@ -273,3 +294,49 @@ class Solver:
         k = C.c_int(0)
         self._chk(lib().zsph_killed(self._ctx, C.byref(k)), "zsph_killed")
         return k.value
+
+    # ---- moving dummies -------------------------------------------------------
+    def set_body(self, rigid):
+        """Make the type-2 particles of buffer 0 one rigid body with state `rigid` (abi.Rigid)."""
+        self._chk(lib().zsph_set_body(self._ctx, C.byref(rigid)), "zsph_set_body")
+
+    def get_body(self):
+        r = abi.Rigid()
+        self._chk(lib().zsph_get_body(self._ctx, C.byref(r)), "zsph_get_body")
+        return r
+
+    def set_piston(self, amplitude, frequency, phase=0.0, axis=0):
+        p = abi.Piston(amplitude, frequency, phase, axis)
+        self._chk(lib().zsph_set_piston(self._ctx, C.byref(p)), "zsph_set_piston")
+
+    def set_time(self, t):
+        self._chk(lib().zsph_set_time(self._ctx, t), "zsph_set_time")
+
+    def time(self):
+        t = C.c_double(0.0)
+        self._chk(lib().zsph_get_time(self._ctx, C.byref(t)), "zsph_get_time")
+        return t.value
+
+    def body_reduce(self, buf=1):
+        """Fluid force and torque on the body from the last governing pass on `buf`."""
+        F = (C.c_double * 3)()
+        T = (C.c_double * 3)()
+        self._chk(lib().zsph_body_reduce(self._ctx, buf, F, T), "zsph_body_reduce")
+        return np.array(F[:]), np.array(T[:])
+
+
+def rigid_state(c, mass, inertia, vel=(0.0, 0.0, 0.0), omega=(0.0, 0.0, 0.0), q=(1.0, 0.0, 0.0, 0.0)):
+    """An abi.Rigid with centre c, mass, 3x3 inertia about c (body frame), velocities and orientation."""
+    r = abi.Rigid()
+    r.c[:] = [float(x) for x in c]
+    r.vel[:] = [float(x) for x in vel]
+    r.omega[:] = [float(x) for x in omega]
+    r.q[:] = [float(x) for x in q]
+    r.I[:] = [float(x) for x in np.asarray(inertia, np.float64).reshape(9)]
+    r.mass = float(mass)
+    return r
+
+
+def rigid_dict(r):
+    return {k: np.array(getattr(r, k)[:]) for k in ("c", "vel", "omega", "q", "F", "T", "acc", "alpha", "I")} | \
+        {"mass": r.mass}

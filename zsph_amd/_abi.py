@@ -100,6 +100,15 @@ class DamBreak(C.Structure):
                 ("layers", _i), ("wall_height", _f)]
 
 
+class Piston(C.Structure):
+    _fields_ = [("amplitude", C.c_double), ("frequency", C.c_double), ("phase", C.c_double), ("axis", _i)]
+
+
+class WaveTank(C.Structure):
+    _fields_ = [("dx", _f), ("lx", _f), ("wy", _f), ("lz", _f), ("depth", _f), ("box_lo", _f * 3),
+                ("box_hi", _f * 3), ("layers", _i), ("piston_travel", _f)]
+
+
 def _sig(lib, name, res, args):
     fn = getattr(lib, name)
     fn.restype = res
@@ -116,7 +125,8 @@ def load(path):
     P = C.POINTER
     vp = C.c_void_p
     _sig(lib, "zsph_sizeof", C.c_size_t, [_i])
-    for which, cls in enumerate((Json, CpuParam, Params, HostState, DamBreak, Rigid, Diag)):
+    for which, cls in enumerate((Json, CpuParam, Params, HostState, DamBreak, Rigid, Diag, Tuning, Piston,
+                                 WaveTank)):
         got = lib.zsph_sizeof(which)
         if got != C.sizeof(cls):
             raise RuntimeError(f"ABI mismatch: sizeof({cls.__name__}) = {C.sizeof(cls)} here, {got} in {path}")
@@ -153,6 +163,13 @@ def load(path):
     _sig(lib, "zsph_set_tuning", _i, [vp, P(Tuning)])
     _sig(lib, "zsph_get_tuning", _i, [vp, P(Tuning)])
     _sig(lib, "zsph_autotune", _i, [vp, _i, P(Tuning), P(_f)])
+    _sig(lib, "zsph_set_body", _i, [vp, P(Rigid)])
+    _sig(lib, "zsph_get_body", _i, [vp, P(Rigid)])
+    _sig(lib, "zsph_set_piston", _i, [vp, P(Piston)])
+    _sig(lib, "zsph_set_time", _i, [vp, C.c_double])
+    _sig(lib, "zsph_get_time", _i, [vp, P(C.c_double)])
+    _sig(lib, "zsph_body_reduce", _i, [vp, _i, P(C.c_double), P(C.c_double)])
+    _sig(lib, "zsph_case_wavetank", _i, [P(WaveTank), P(Json), P(HostState), P(_i)])
     return lib
 
 
@@ -165,7 +182,8 @@ EXPORTS = (
     "zsph_correction", "zsph_step", "zsph_array_get", "zsph_array_set", "zsph_neighbor_stats",
     "zsph_killed", "zsph_sizeof", "zsph_time_steps", "zsph_profile_steps", "zsph_diagnostics",
     "zsph_probe_pressure", "zsph_rigid_mass_properties", "zsph_rigid_acc_alpha", "zsph_rigid_euler_step",
-    "zsph_set_tuning", "zsph_get_tuning", "zsph_autotune",
+    "zsph_set_tuning", "zsph_get_tuning", "zsph_autotune", "zsph_set_body", "zsph_get_body",
+    "zsph_set_piston", "zsph_set_time", "zsph_get_time", "zsph_body_reduce", "zsph_case_wavetank",
 )
 
 STAGES = ("hash", "sort", "cells", "reorder", "wall", "ldg", "governing", "integrate")

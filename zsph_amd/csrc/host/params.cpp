@@ -175,6 +175,9 @@ extern "C" size_t zsph_sizeof(int which) {
     case 4: return sizeof(zsph_dambreak);
     case 5: return sizeof(zsph_rigid);
     case 6: return sizeof(zsph_diag);
+    case 7: return sizeof(zsph_tuning);
+    case 8: return sizeof(zsph_piston);
+    case 9: return sizeof(zsph_wavetank);
     default: return 0;
     }
 }

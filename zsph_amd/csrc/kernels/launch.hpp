@@ -40,4 +40,20 @@ int diag_blocks(int n);
 void launch_diag(hipStream_t s, const Prm &q, const Buf &b, float front_zmax, double *part, float *pmax);
 void launch_probe(hipStream_t s, const Prm &q, const Buf &b, const float *pts, int npts, float *out);
 
+// body.hip: floating body and piston.  Device-resident FP64 state, so a
+// captured step advances it with every replay.
+struct DevBody {
+    zsph_rigid s;       // state at time t
+    zsph_rigid s0;      // state at the start of the step (time t0)
+    double t, t0;       // simulated time now and at the start of the step
+    double amp, pw, phase;  // piston: d(t) = amp (sin(pw t + phase) - sin(phase)), pw = 2 pi f
+    int axis, has_body, has_piston, pad;
+};
+constexpr int BODY_BLOCKS = 256;  // fixed partial count: the force sum has a fixed order
+constexpr int BODY_D = 6;         // F x/y/z, T x/y/z
+void launch_body_reduce(hipStream_t s, const Prm &q, const Buf &b, const int *cs, const float4 *acc,
+                        const DevBody *body, double *part);
+void launch_body_update(hipStream_t s, const Prm &q, DevBody *body, const double *part, bool predictor);
+void launch_kinematics(hipStream_t s, const Prm &q, const Buf &b, const int *cs, const DevBody *body);
+
 }  // namespace zs

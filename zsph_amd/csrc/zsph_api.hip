@@ -11,6 +11,7 @@
 #include <string>
 #include <vector>
 
+#include "host/rigid.hpp"
 #include "kernels/common.hpp"
 #include "kernels/launch.hpp"
 #include "zsph.h"
@@ -38,6 +39,10 @@ struct zsph_ctx {
     int *nlist = nullptr;      // WALK_LIST neighbour lists (list_words(N) ints) and counts
     int *ncount = nullptr;
     bool list_valid = false;   // the lists describe the current sorted buffer
+    zs::DevBody *body = nullptr;  // moving dummies: FP64 body state, piston, device time
+    double *body_part = nullptr;  // BODY_BLOCKS x BODY_D force/torque partials
+    bool has_body = false, has_piston = false;
+    double t = 0.0, t0 = 0.0;     // simulated time while no dummies move (else on the device)
     hipGraph_t graph = nullptr;
     hipGraphExec_t graph_exec = nullptr;
     std::string err;
@@ -85,6 +90,14 @@ int check_launch(zsph_ctx *c, const char *what) {
 }
 
 bool valid_buf(int b) { return b == 0 || b == 1; }
+
+// Copy on the context's stream and wait for it: ordered with the kernels on
+// that (non-blocking) stream, which a plain hipMemcpy is not.
+int copy_sync(zsph_ctx *c, void *dst, const void *src, size_t bytes, hipMemcpyKind kind) {
+    HIPCHK(c, hipMemcpyAsync(dst, src, bytes, kind, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return ZSPH_OK;
+}
 
 // Optional per-stage event timing (zsph_profile_steps).
 struct Prof {
@@ -174,6 +187,30 @@ int stage_governing(zsph_ctx *c, int b, Prof *pf = nullptr) {
     return check_launch(c, "governing");
 }
 
+// Integrator of one half step on buffer b: the body's force sum and FP64
+// update, the fluid's predictor or corrector, then the moving dummies placed
+// at the new time.  Without moving dummies only the fluid kernel runs.
+int stage_integrate(zsph_ctx *c, int b, bool predictor, Prof *pf = nullptr) {
+    Scope sc(pf, ZSPH_ST_INTEGRATE);
+    const bool moving = c->has_body || c->has_piston;
+    if (c->has_body)
+        zs::launch_body_reduce(c->stream, c->q, c->buf[b], c->cell_start, c->acc, c->body, c->body_part);
+    if (moving) zs::launch_body_update(c->stream, c->q, c->body, c->body_part, predictor);
+    if (predictor)
+        zs::launch_predict(c->stream, c->q, c->buf[b], c->cell_start, c->acc);
+    else
+        zs::launch_correct(c->stream, c->q, c->buf[b], c->cell_start, c->acc);
+    if (moving) zs::launch_kinematics(c->stream, c->q, c->buf[b], c->cell_start, c->body);
+    return check_launch(c, predictor ? "prediction" : "correction");
+}
+
+// Host clock of a half step while no dummies move (the device keeps it otherwise).
+void host_time(zsph_ctx *c, bool predictor) {
+    if (c->has_body || c->has_piston) return;
+    if (predictor) c->t0 = c->t;
+    c->t = c->t0 + (predictor ? (double)c->q.half_dt : (double)c->q.dt);
+}
+
 int half_step(zsph_ctx *c, int from, int to, bool predictor, Prof *pf = nullptr) {
     int r;
     if ((r = stage_hash(c, from, pf))) return r;
@@ -181,14 +218,14 @@ int half_step(zsph_ctx *c, int from, int to, bool predictor, Prof *pf = nullptr)
     if ((r = stage_sort_data(c, from, to, pf))) return r;
     if ((r = stage_boundary(c, to, pf))) return r;
     if ((r = stage_governing(c, to, pf))) return r;
-    {
-        Scope sc(pf, ZSPH_ST_INTEGRATE);
-        if (predictor)
-            zs::launch_predict(c->stream, c->q, c->buf[to], c->cell_start, c->acc);
-        else
-            zs::launch_correct(c->stream, c->q, c->buf[to], c->cell_start, c->acc);
-    }
-    return check_launch(c, predictor ? "prediction" : "correction");
+    return stage_integrate(c, to, predictor, pf);
+}
+
+void drop_graph(zsph_ctx *c) {
+    if (c->graph_exec) (void)hipGraphExecDestroy(c->graph_exec);
+    if (c->graph) (void)hipGraphDestroy(c->graph);
+    c->graph_exec = nullptr;
+    c->graph = nullptr;
 }
 
 }  // namespace
@@ -240,7 +277,8 @@ int zsph_create(const zsph_params *p, int device, zsph_ctx **out) {
         (r = dalloc(c, &c->gd, N)) || (r = dalloc(c, &c->hist, zs::radix_hist_words((int)N, 11))) ||
         (r = dalloc(c, &c->killed, 1)) ||
         (r = dalloc(c, &c->diag_part, (size_t)zs::diag_blocks((int)N) * zs::DIAG_D)) ||
-        (r = dalloc(c, &c->diag_max, (size_t)zs::diag_blocks((int)N) * 2)))
+        (r = dalloc(c, &c->diag_max, (size_t)zs::diag_blocks((int)N) * 2)) || (r = dalloc(c, &c->body, 1)) ||
+        (r = dalloc(c, &c->body_part, (size_t)zs::BODY_BLOCKS * zs::BODY_D)))
         return fail(r);
     *out = c;
     return ZSPH_OK;
@@ -292,6 +330,17 @@ int zsph_upload(zsph_ctx *c, int b, const zsph_host_state *st) {
     HIPCHK(c, hipStreamSynchronize(c->stream));
     c->hashed = 0;
     c->sorted_slot = -1;
+    if (b == 0 && (c->has_body || c->has_piston)) {
+        // new particles in buffer 0: the moving-dummy set-up no longer applies
+        zs::DevBody hb;
+        if (int e_ = copy_sync(c, &hb, c->body, sizeof(hb), hipMemcpyDeviceToHost)) return e_;
+        c->t = hb.t;
+        c->t0 = hb.t0;
+        hb.has_body = hb.has_piston = 0;
+        if (int e_ = copy_sync(c, c->body, &hb, sizeof(hb), hipMemcpyHostToDevice)) return e_;
+        c->has_body = c->has_piston = false;
+        drop_graph(c);
+    }
     return ZSPH_OK;
 }
 
@@ -362,14 +411,14 @@ int zsph_governing(zsph_ctx *c, int b) {
 int zsph_prediction(zsph_ctx *c, int b) {
     if (!c || !valid_buf(b)) return set_err(c, ZSPH_E_ARG, "zsph_prediction: bad buffer");
     if (c->sorted_slot < 0) return set_err(c, ZSPH_E_STATE, "zsph_prediction called before zsph_sort_data");
-    zs::launch_predict(c->stream, c->q, c->buf[b], c->cell_start, c->acc);
-    return check_launch(c, "prediction");
+    host_time(c, true);
+    return stage_integrate(c, b, true);
 }
 int zsph_correction(zsph_ctx *c, int b) {
     if (!c || !valid_buf(b)) return set_err(c, ZSPH_E_ARG, "zsph_correction: bad buffer");
     if (c->sorted_slot < 0) return set_err(c, ZSPH_E_STATE, "zsph_correction called before zsph_sort_data");
-    zs::launch_correct(c->stream, c->q, c->buf[b], c->cell_start, c->acc);
-    return check_launch(c, "correction");
+    host_time(c, false);
+    return stage_integrate(c, b, false);
 }
 
 int zsph_step(zsph_ctx *c, int nsteps) {
@@ -390,7 +439,11 @@ int zsph_step(zsph_ctx *c, int nsteps) {
         e = hipGraphInstantiate(&c->graph_exec, g, nullptr, nullptr, 0);
         if (e != hipSuccess) return set_err(c, ZSPH_E_HIP, "graph instantiate failed: %s", hipGetErrorString(e));
     }
-    for (int s = 0; s < nsteps; ++s) HIPCHK(c, hipGraphLaunch(c->graph_exec, c->stream));
+    for (int s = 0; s < nsteps; ++s) {
+        HIPCHK(c, hipGraphLaunch(c->graph_exec, c->stream));
+        host_time(c, true);
+        host_time(c, false);
+    }
     // After a step the sorted tables describe buffer 0.
     c->hashed = 0;
     c->sorted_slot = (c->keybits + c->radix_bits - 1) / c->radix_bits % 2;
@@ -410,11 +463,7 @@ int zsph_set_tuning(zsph_ctx *c, const zsph_tuning *t) {
     c->list_valid = false;
     c->walk = t->walk;
     c->radix_bits = t->radix_bits;
-    // the captured step embeds the old launches
-    if (c->graph_exec) (void)hipGraphExecDestroy(c->graph_exec);
-    if (c->graph) (void)hipGraphDestroy(c->graph);
-    c->graph_exec = nullptr;
-    c->graph = nullptr;
+    drop_graph(c);  // the captured step embeds the old launches
     c->sorted_slot = -1;
     c->hashed = 0;
     return ZSPH_OK;
@@ -435,8 +484,9 @@ int zsph_autotune(zsph_ctx *c, int steps, zsph_tuning *best, float *best_ms) {
     const zs::Buf &b0 = c->buf[0];
     void *const arrays[6] = {b0.posr, b0.velp, b0.tposr, b0.tvelp, b0.type, b0.table};
     const size_t bytes[6] = {N * 16, N * 16, N * 16, N * 16, N * 4, N * 4};
-    size_t total = 4;
+    size_t total = 256 + sizeof(zs::DevBody);  // + removal counter (padded) + body state
     for (size_t k : bytes) total += k;
+    const double t_keep = c->t, t0_keep = c->t0;
     char *save = nullptr;
     HIPCHK(c, hipMalloc(&save, total));
     auto copy_all = [&](bool to_save) -> hipError_t {
@@ -450,6 +500,10 @@ int zsph_autotune(zsph_ctx *c, int steps, zsph_tuning *best, float *best_ms) {
         if (e == hipSuccess)
             e = to_save ? hipMemcpyAsync(save + off, c->killed, 4, hipMemcpyDeviceToDevice, c->stream)
                         : hipMemcpyAsync(c->killed, save + off, 4, hipMemcpyDeviceToDevice, c->stream);
+        off += 256;
+        if (e == hipSuccess)
+            e = to_save ? hipMemcpyAsync(save + off, c->body, sizeof(zs::DevBody), hipMemcpyDeviceToDevice, c->stream)
+                        : hipMemcpyAsync(c->body, save + off, sizeof(zs::DevBody), hipMemcpyDeviceToDevice, c->stream);
         if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
         return e;
     };
@@ -485,6 +539,8 @@ int zsph_autotune(zsph_ctx *c, int steps, zsph_tuning *best, float *best_ms) {
     if (!r) r = zsph_set_tuning(c, &keep);
     e = copy_all(false);
     (void)hipFree(save);
+    c->t = t_keep;
+    c->t0 = t0_keep;
     c->sorted_slot = -1;  // buffer 0 is back to its unsorted starting state
     c->hashed = 0;
     if (r) return r;
@@ -588,7 +644,9 @@ int zsph_profile_steps(zsph_ctx *c, int nsteps, float *stage_ms) {
     pf.s = c->stream;
     int r = ZSPH_OK;
     for (int s = 0; s < nsteps && !r; ++s) {
+        host_time(c, true);
         r = half_step(c, 0, 1, true, &pf);
+        host_time(c, false);
         if (!r) r = half_step(c, 1, 0, false, &pf);
     }
     hipError_t e = hipStreamSynchronize(c->stream);
@@ -661,6 +719,180 @@ int zsph_killed(zsph_ctx *c, int *killed) {
     if (!c || !killed) return ZSPH_E_ARG;
     HIPCHK(c, hipMemcpyAsync(killed, c->killed, sizeof(int), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
+    return ZSPH_OK;
+}
+
+// ---- moving dummies --------------------------------------------------------
+
+namespace {
+
+// Device body record as the host sees it; the time comes from the host clock
+// while nothing moves.
+int body_fetch(zsph_ctx *c, zs::DevBody *hb) {
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    if (int e_ = copy_sync(c, hb, c->body, sizeof(*hb), hipMemcpyDeviceToHost)) return e_;
+    if (!c->has_body && !c->has_piston) {
+        hb->t = c->t;
+        hb->t0 = c->t0;
+    }
+    return ZSPH_OK;
+}
+
+// Buffer 0's particle arrays on the host, edited by the set-up calls.
+struct HostArrays {
+    std::vector<float4> posr, velp, tposr, tvelp;
+    std::vector<int> type;
+};
+int fetch_arrays(zsph_ctx *c, HostArrays &h) {
+    const size_t N = (size_t)c->q.N;
+    const zs::Buf &d = c->buf[0];
+    h.posr.resize(N); h.velp.resize(N); h.tposr.resize(N); h.tvelp.resize(N); h.type.resize(N);
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    if (int e_ = copy_sync(c, h.posr.data(), d.posr, N * sizeof(float4), hipMemcpyDeviceToHost)) return e_;
+    if (int e_ = copy_sync(c, h.velp.data(), d.velp, N * sizeof(float4), hipMemcpyDeviceToHost)) return e_;
+    if (int e_ = copy_sync(c, h.tposr.data(), d.tposr, N * sizeof(float4), hipMemcpyDeviceToHost)) return e_;
+    if (int e_ = copy_sync(c, h.tvelp.data(), d.tvelp, N * sizeof(float4), hipMemcpyDeviceToHost)) return e_;
+    if (int e_ = copy_sync(c, h.type.data(), d.type, N * sizeof(int), hipMemcpyDeviceToHost)) return e_;
+    return ZSPH_OK;
+}
+int store_arrays(zsph_ctx *c, const HostArrays &h, const zs::DevBody &hb) {
+    const size_t N = (size_t)c->q.N;
+    const zs::Buf &d = c->buf[0];
+    if (int e_ = copy_sync(c, d.velp, h.velp.data(), N * sizeof(float4), hipMemcpyHostToDevice)) return e_;
+    if (int e_ = copy_sync(c, d.tposr, h.tposr.data(), N * sizeof(float4), hipMemcpyHostToDevice)) return e_;
+    if (int e_ = copy_sync(c, d.tvelp, h.tvelp.data(), N * sizeof(float4), hipMemcpyHostToDevice)) return e_;
+    if (int e_ = copy_sync(c, c->body, &hb, sizeof(hb), hipMemcpyHostToDevice)) return e_;
+    drop_graph(c);  // the captured step does not contain the moving-dummy kernels
+    return ZSPH_OK;
+}
+
+}  // namespace
+
+int zsph_set_body(zsph_ctx *c, const zsph_rigid *r) {
+    using namespace zs::rb;
+    if (!c || !r) return set_err(c, ZSPH_E_ARG, "zsph_set_body: null argument");
+    const double qn = std::sqrt(r->q[0] * r->q[0] + r->q[1] * r->q[1] + r->q[2] * r->q[2] + r->q[3] * r->q[3]);
+    M3 I;
+    for (int k = 0; k < 9; ++k) I.m[k] = r->I[k];
+    const double detI = I.m[0] * (I.m[4] * I.m[8] - I.m[5] * I.m[7]) - I.m[1] * (I.m[3] * I.m[8] - I.m[5] * I.m[6]) +
+                        I.m[2] * (I.m[3] * I.m[7] - I.m[4] * I.m[6]);
+    if (!(r->mass > 0.0) || !(qn > 0.0) || !std::isfinite(qn) || !(std::fabs(detI) > 0.0) || !std::isfinite(detI))
+        return set_err(c, ZSPH_E_ARG, "zsph_set_body: needs mass > 0, a non-zero quaternion and an invertible I");
+    zs::DevBody hb;
+    HostArrays h;
+    int e;
+    if ((e = body_fetch(c, &hb)) || (e = fetch_arrays(c, h))) return e;
+    hb.s = *r;
+    for (int k = 0; k < 4; ++k) hb.s.q[k] = r->q[k] / qn;
+    const M3 R = rotation(qget(hb.s));
+    const M3 Rt = transpose(R);
+    const V3 cc = v3(hb.s.c), V = v3(hb.s.vel), w = v3(hb.s.omega), A = v3(hb.s.acc), al = v3(hb.s.alpha);
+    int nbody = 0;
+    for (size_t i = 0; i < h.type.size(); ++i) {
+        if (h.type[i] != ZSPH_BODY) continue;
+        ++nbody;
+        const V3 d = V3{h.posr[i].x, h.posr[i].y, h.posr[i].z} - cc;
+        const V3 r0 = mul(Rt, d);
+        const V3 v = V + cross(w, d);
+        const V3 a = A + cross(al, d) + cross(w, cross(w, d));
+        h.tposr[i] = make_float4((float)r0.x, (float)r0.y, (float)r0.z, 0.0f);
+        h.velp[i].x = (float)v.x; h.velp[i].y = (float)v.y; h.velp[i].z = (float)v.z;
+        h.tvelp[i].x = (float)a.x; h.tvelp[i].y = (float)a.y; h.tvelp[i].z = (float)a.z;
+    }
+    if (nbody == 0) return set_err(c, ZSPH_E_ARG, "zsph_set_body: buffer 0 holds no type-2 (body) particles");
+    hb.s0 = hb.s;
+    hb.has_body = 1;
+    if ((e = store_arrays(c, h, hb))) return e;
+    c->has_body = true;
+    return ZSPH_OK;
+}
+
+int zsph_get_body(zsph_ctx *c, zsph_rigid *r) {
+    if (!c || !r) return set_err(c, ZSPH_E_ARG, "zsph_get_body: null argument");
+    if (!c->has_body) return set_err(c, ZSPH_E_STATE, "zsph_get_body: no body (zsph_set_body)");
+    zs::DevBody hb;
+    int e = body_fetch(c, &hb);
+    if (e) return e;
+    *r = hb.s;
+    return ZSPH_OK;
+}
+
+int zsph_set_piston(zsph_ctx *c, const zsph_piston *p) {
+    if (!c || !p) return set_err(c, ZSPH_E_ARG, "zsph_set_piston: null argument");
+    if (p->axis < 0 || p->axis > 2 || !(p->frequency >= 0.0) || !std::isfinite(p->frequency) ||
+        !std::isfinite(p->amplitude) || !std::isfinite(p->phase))
+        return set_err(c, ZSPH_E_ARG, "zsph_set_piston: axis 0..2, finite amplitude/phase, frequency >= 0");
+    zs::DevBody hb;
+    HostArrays h;
+    int e;
+    if ((e = body_fetch(c, &hb)) || (e = fetch_arrays(c, h))) return e;
+    hb.amp = p->amplitude;
+    hb.pw = 2.0 * M_PI * p->frequency;
+    hb.phase = p->phase;
+    hb.axis = p->axis;
+    const double ph = hb.pw * hb.t + hb.phase;
+    const double d = hb.amp * (std::sin(ph) - std::sin(hb.phase));
+    const double dv = hb.amp * hb.pw * std::cos(ph), da = -hb.amp * hb.pw * hb.pw * std::sin(ph);
+    int np = 0;
+    for (size_t i = 0; i < h.type.size(); ++i) {
+        if (h.type[i] != ZSPH_PISTON) continue;
+        ++np;
+        float rest[3] = {h.posr[i].x, h.posr[i].y, h.posr[i].z};
+        float v[3] = {0.0f, 0.0f, 0.0f}, a[3] = {0.0f, 0.0f, 0.0f};
+        rest[p->axis] = (float)((double)rest[p->axis] - d);
+        v[p->axis] = (float)dv;
+        a[p->axis] = (float)da;
+        h.tposr[i] = make_float4(rest[0], rest[1], rest[2], 0.0f);
+        h.velp[i].x = v[0]; h.velp[i].y = v[1]; h.velp[i].z = v[2];
+        h.tvelp[i].x = a[0]; h.tvelp[i].y = a[1]; h.tvelp[i].z = a[2];
+    }
+    if (np == 0) return set_err(c, ZSPH_E_ARG, "zsph_set_piston: buffer 0 holds no type-3 (piston) particles");
+    hb.has_piston = 1;
+    if ((e = store_arrays(c, h, hb))) return e;
+    c->has_piston = true;
+    return ZSPH_OK;
+}
+
+int zsph_set_time(zsph_ctx *c, double t) {
+    if (!c || !std::isfinite(t)) return set_err(c, ZSPH_E_ARG, "zsph_set_time: bad argument");
+    c->t = c->t0 = t;
+    if (c->has_body || c->has_piston) {
+        zs::DevBody hb;
+        int e = body_fetch(c, &hb);
+        if (e) return e;
+        hb.t = hb.t0 = t;
+        if (int e_ = copy_sync(c, c->body, &hb, sizeof(hb), hipMemcpyHostToDevice)) return e_;
+    }
+    return ZSPH_OK;
+}
+
+int zsph_get_time(zsph_ctx *c, double *t) {
+    if (!c || !t) return set_err(c, ZSPH_E_ARG, "zsph_get_time: null argument");
+    zs::DevBody hb;
+    int e = body_fetch(c, &hb);
+    if (e) return e;
+    *t = hb.t;
+    return ZSPH_OK;
+}
+
+int zsph_body_reduce(zsph_ctx *c, int b, double F[3], double T[3]) {
+    if (!c || !valid_buf(b) || !F || !T) return set_err(c, ZSPH_E_ARG, "zsph_body_reduce: bad argument");
+    if (!c->has_body) return set_err(c, ZSPH_E_STATE, "zsph_body_reduce: no body (zsph_set_body)");
+    if (c->sorted_slot < 0) return set_err(c, ZSPH_E_STATE, "zsph_body_reduce called before zsph_sort_data");
+    zs::launch_body_reduce(c->stream, c->q, c->buf[b], c->cell_start, c->acc, c->body, c->body_part);
+    int e = check_launch(c, "body reduce");
+    if (e) return e;
+    std::vector<double> part((size_t)zs::BODY_BLOCKS * zs::BODY_D);
+    HIPCHK(c, hipMemcpyAsync(part.data(), c->body_part, part.size() * sizeof(double), hipMemcpyDeviceToHost,
+                             c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    double s[zs::BODY_D] = {0, 0, 0, 0, 0, 0};
+    for (int k = 0; k < zs::BODY_BLOCKS; ++k)
+        for (int d = 0; d < zs::BODY_D; ++d) s[d] += part[(size_t)k * zs::BODY_D + d];
+    for (int d = 0; d < 3; ++d) {
+        F[d] = s[d];
+        T[d] = s[3 + d];
+    }
     return ZSPH_OK;
 }
 
