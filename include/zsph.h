@@ -378,11 +378,13 @@ int zsph_get_time(zsph_ctx *ctx, double *t);
  * FP32 terms: the body-force reduction on its own. */
 int zsph_body_reduce(zsph_ctx *ctx, int buf, double F[3], double T[3]);
 
-/* Wave-tank case (config 2): fluid of depth `depth` in [0,lx]x[0,wy], a
- * piston of `layers` dummy layers behind x = 0 over the tank height `lz`,
- * fixed floor and walls elsewhere, and a floating box [box_lo, box_hi]
- * (a `layers`-thick shell of type-2 particles on the fluid's lattice, no
- * fluid inside).  Same two-call protocol as zsph_case_dambreak. */
+/* Wave-tank case: fluid of depth `depth` in [0,lx]x[0,wy], a piston of
+ * `layers` dummy layers behind x = 0 over the tank height `lz`, fixed floor
+ * and walls elsewhere, and a floating box [box_lo, box_hi] (a `layers`-thick
+ * shell of type-2 particles on the fluid's lattice, no fluid inside).  The
+ * box is snapped to the nearest lattice faces (multiples of dx) and the
+ * snapped extents are written back to *c, so the caller's mass properties
+ * match the particle set.  Same two-call protocol as zsph_case_dambreak. */
 typedef struct zsph_wavetank {
     float dx;
     float lx, wy, lz, depth;
@@ -390,7 +392,7 @@ typedef struct zsph_wavetank {
     int layers;
     float piston_travel;         /* floor extends this far behind the piston */
 } zsph_wavetank;
-int zsph_case_wavetank(const zsph_wavetank *c, zsph_json *j, zsph_host_state *st, int *n_out);
+int zsph_case_wavetank(zsph_wavetank *c, zsph_json *j, zsph_host_state *st, int *n_out);
 
 /* Launch tuning (no counterpart in the reference, which launches 256-thread
  * blocks, deltasph/main.cu:55-56).  Results do not depend on it: both

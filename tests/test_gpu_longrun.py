@@ -69,3 +69,57 @@ def test_long_run_front_and_probes(name, steps):
         assert dg["n_killed"] == do["n_killed"] == 0
         assert dg["front_x"] == pytest.approx(do["front_x"], rel=0.01)
         np.testing.assert_allclose(pg, po, rtol=0.01, atol=0.01 * pscale)
+
+
+def test_floating_box_holds_its_equilibrium_draft():
+    # a box of half the water's density, released at its equilibrium draft in
+    # still water, stays there (Archimedes): heave within 0.1 dx, pitch/roll
+    # within 0.05 degrees over 3000 steps (0.45 s)
+    dx = 0.02
+    c = cases.wavetank3d(dx=dx)
+    say(f"[floating equilibrium] N={c.n}")
+    s = z.Solver(c)
+    s.upload(c.host)
+    body = cases.box_body(*c.box, cases.BOX_DENSITY)
+    s.set_body(body)
+    c0 = np.array(body.c[:])
+    for k in range(6):
+        s.step(500)
+        b = s.get_body()
+        dc = np.array(b.c[:]) - c0
+        angle = 2.0 * np.degrees(np.arccos(min(1.0, abs(b.q[0]))))
+        say(f"  step {500 * (k + 1)}: dc {dc} angle {angle:.4f} deg F {b.F[:]}")
+        assert abs(dc[2]) < 0.1 * dx
+        assert np.abs(dc[:2]).max() < 0.05 * dx
+        assert angle < 0.05
+    assert s.killed() == 0
+    s.close()
+
+
+def test_floating_box_with_piston_tracks_oracle():
+    # piston-driven tank with the box, 300 steps: body trajectory within 1 %
+    # of its motion and particle positions within 1e-4 dx of the FP64 oracle
+    c = cases.wavetank3d(dx=0.04, depth=0.6)
+    say(f"[floating + piston vs oracle] N={c.n}")
+    s = z.Solver(c)
+    s.upload(c.host)
+    o = oracle.Oracle(c.params)
+    o.upload(c.host)
+    body = cases.box_body(*c.box, cases.BOX_DENSITY)
+    for x in (s, o):
+        x.set_body(body)
+        x.set_piston(cases.PISTON_AMPLITUDE, cases.PISTON_FREQUENCY, 0.0, 0)
+    c0 = np.array(body.c[:])
+    for k in range(3):
+        s.step(100)
+        o.step(100)
+        bg, bo = s.get_body(), o.get_body()
+        dg, do = np.array(bg.c[:]) - c0, np.array(bo.c[:]) - c0
+        say(f"  step {100 * (k + 1)}: dc gpu {dg} oracle {do}")
+        motion = np.abs(do).max() + 1e-6
+        np.testing.assert_allclose(dg, do, rtol=0, atol=0.01 * motion)
+        np.testing.assert_allclose(bg.q[:], bo.q[:], rtol=0, atol=1e-6)
+    assert s.time() == o.time()
+    d = s.diagnostics(0)
+    assert d["n_fluid"] == int((c.host.type == z.FLUID).sum()) and d["n_killed"] == 0
+    s.close()

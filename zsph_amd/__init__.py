@@ -114,7 +114,8 @@ class Case:
     @classmethod
     def wavetank(cls, dx, lx, wy, lz, depth, box_lo=(0.0, 0.0, 0.0), box_hi=(0.0, 0.0, 0.0), layers=3,
                  piston_travel=0.0, gamma=7.0, cfl=0.2, diffusion=DIFF_ANTUONO, **overrides):
-        """Wave tank with a piston and a floating box (zsph_case_wavetank).  box_hi <= box_lo: no box."""
+        """Wave tank with a piston and a floating box (zsph_case_wavetank).  box_hi <= box_lo: no box.
+        The box is snapped to lattice faces; ``case.box`` holds the snapped (lo, hi)."""
         j = cls.defaults()
         j.gamma, j.cfl, j.diffusion = gamma, cfl, diffusion
         for k, v in overrides.items():
@@ -130,7 +131,9 @@ class Case:
         rc = lib().zsph_case_wavetank(C.byref(w), C.byref(j), C.byref(st), C.byref(n))
         if rc != abi.OK:
             raise ZsphError(rc, "zsph_case_wavetank")
-        return cls(j, host)
+        case = cls(j, host)
+        case.box = (tuple(w.box_lo[:]), tuple(w.box_hi[:]))  # snapped to the lattice
+        return case
 
     @classmethod
     def from_json(cls, path):

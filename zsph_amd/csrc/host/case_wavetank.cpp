@@ -1,7 +1,8 @@
 // Wave-tank case with a piston wavemaker and a floating box (the floating-
 // body configuration).  One lattice at (i+1/2)dx is split between fluid and
-// the box; the piston is `layers` dummy layers behind x = 0 that slide
-// between the side walls, over a floor that extends `piston_travel` behind it.
+// the box (snapped to lattice faces); the piston is `layers` dummy layers
+// behind x = 0 that slide between the side walls, over a floor that extends
+// `piston_travel` behind it.
 #include <algorithm>
 #include <cmath>
 
@@ -31,7 +32,7 @@ struct Emit {
 
 }  // namespace
 
-extern "C" int zsph_case_wavetank(const zsph_wavetank *c, zsph_json *j, zsph_host_state *st, int *n_out) {
+extern "C" int zsph_case_wavetank(zsph_wavetank *c, zsph_json *j, zsph_host_state *st, int *n_out) {
     if (!c || !j || !n_out || !(c->dx > 0.0f) || c->layers < 1) return ZSPH_E_ARG;
     if (st && !st->pos) return ZSPH_E_ARG;
     const double dx = c->dx;
@@ -40,14 +41,21 @@ extern "C" int zsph_case_wavetank(const zsph_wavetank *c, zsph_json *j, zsph_hos
     const int nd = std::min(nz, lattice_count(c->depth, dx));
     const int tc = (int)std::ceil(std::max(0.0f, c->piston_travel) / dx);
     if (nx < 1 || ny < 1 || nz < 1 || nd < 1) return ZSPH_E_ARG;
-    // box as lattice index ranges: points (i+1/2)dx inside [lo, hi]
+    // box snapped to the nearest lattice faces: cells b0..b1 along each axis,
+    // so the particle set is symmetric about the box centre; the snapped
+    // extents are written back for the body's mass properties
     int b0[3], b1[3];
     bool has_box = true;
     for (int a = 0; a < 3; ++a) {
-        b0[a] = (int)std::ceil(c->box_lo[a] / dx - 0.5);
-        b1[a] = (int)std::floor(c->box_hi[a] / dx - 0.5);
+        b0[a] = (int)std::floor(c->box_lo[a] / dx + 0.5);
+        b1[a] = (int)std::floor(c->box_hi[a] / dx + 0.5) - 1;
         if (b1[a] < b0[a]) has_box = false;
     }
+    if (has_box)
+        for (int a = 0; a < 3; ++a) {
+            c->box_lo[a] = (float)(b0[a] * dx);
+            c->box_hi[a] = (float)((b1[a] + 1) * dx);
+        }
     auto in_box = [&](int i, int k, int m) {
         return has_box && i >= b0[0] && i <= b1[0] && k >= b0[1] && k <= b1[1] && m >= b0[2] && m <= b1[2];
     };
