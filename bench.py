@@ -1,14 +1,20 @@
-"""Benchmark: particle-steps/s of the plain delta-SPH step on the 3-D dam break.
+"""Benchmark: particle-steps/s of the delta-SPH step.
 
-Workload: tank 3.22 x 1.0 x 1.0 m, water column 1.2 x 1.0 x 0.55 m, dx 0.008 m
-(1,293,750 fluid particles on a lattice + 3 dummy layers on the floor and the
-four walls, 1,854,987 particles in all), Wendland C2 with h = 1.3 dx, Tait EOS
-gamma 7, rho0 1000, c0 = 10 sqrt(2 g H), delta 0.1, alpha 0.02, CFL 0.2, FP32.
+Default workload (--case dambreak3d): the plain 3-D dam break, tank
+3.22 x 1.0 x 1.0 m, water column 1.2 x 1.0 x 0.55 m, dx 0.008 m (1,293,750
+fluid particles on a lattice + 3 dummy layers on the floor and the four walls,
+1,853,844 particles in all), Wendland C2 with h = 1.3 dx, Tait EOS gamma 7,
+rho0 1000, c0 = 10 sqrt(2 g H), delta 0.1, alpha 0.02, CFL 0.2, FP32.
 One step = one full predictor-corrector step (two half steps: hash, radix
 sort, cell table + reorder, wall pass, gradient pass, continuity+momentum,
 integrator) for every particle; particles are resident in HBM before timing.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--dx 0.008] [--no-cpu-baseline]
+--case wavetank3d: the floating-body tank, 4.0 x 1.0 x 1.2 m filled to 0.6 m
+at dx 0.006 m (11.1 M fluid, 12.5 M particles), piston 0.025 m amplitude at
+1 Hz, a 0.3 x 0.3 x 0.2 m box of density 500 coupled through its FP64 rigid
+state (force/torque reduction, body update and dummy kinematics every half step).
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--case dambreak3d|wavetank3d] [--no-cpu-baseline]
 
 Prints one JSON line.  Only one GPU is used: --gpus above 1 exits.
 """
@@ -41,19 +47,21 @@ def governing_bytes(case):
     return 72.0 * case.n + 4.0 * (case.cpu.grid_num + 1)
 
 
-def cpu_baseline(case, steps=3, warmup=1):
+def cpu_baseline(case, steps=3, warmup=1, setup=None):
     """FP32 CPU oracle (OpenMP over particles) on this host's cores, same case."""
     import oracle
     threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
     o = oracle.Oracle(case.params, fp64=False)
     o.upload(case.host)
+    if setup:
+        setup(o)
     o.step(warmup)
     t0 = time.perf_counter()
     o.step(steps)
     dt = time.perf_counter() - t0
     o.close()
     return {"value": case.n * steps / dt, "unit": "particle-steps/s", "cores": threads, "kind": "port",
-            "sample": f"{steps} steps (after {warmup} warm-up) of the same {case.n}-particle 3-D dam break, "
+            "sample": f"{steps} steps (after {warmup} warm-up) of the same {case.n}-particle case, "
                       f"FP32 oracle, OpenMP {threads} threads"}
 
 
@@ -62,7 +70,8 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=20)
-    ap.add_argument("--dx", type=float, default=0.008)
+    ap.add_argument("--case", choices=("dambreak3d", "wavetank3d"), default="dambreak3d")
+    ap.add_argument("--dx", type=float, default=None, help="particle spacing (default 0.008 / 0.006)")
     ap.add_argument("--gsf", type=float, default=2.0, help="grid_size_factor: cell edge = 2h / gsf")
     ap.add_argument("--profile-steps", type=int, default=3)
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -77,11 +86,34 @@ def main():
     import zsph_amd as z
     from zsph_amd import cases
 
-    case = cases.dambreak3d(dx=args.dx, grid_size_factor=args.gsf)
-    print(f"[bench] 3-D dam break dx={args.dx}: N={case.n} (fluid {case.json.water_ptc_num}), "
+    floating = args.case == "wavetank3d"
+    if floating:
+        args.dx = args.dx or 0.006
+        case = cases.wavetank3d(dx=args.dx, grid_size_factor=args.gsf)
+        body = cases.box_body(*case.box, cases.BOX_DENSITY)
+
+        def setup(x):  # the same body and piston on the solver and on the CPU baseline
+            x.set_body(body)
+            x.set_piston(cases.PISTON_AMPLITUDE, cases.PISTON_FREQUENCY, 0.0, 0)
+        workload = (f"3-D wave tank 4.0x1.0x1.2 m filled to 0.6 m, dx {args.dx} m, {case.n} particles "
+                    f"({case.json.water_ptc_num} fluid), piston {cases.PISTON_AMPLITUDE} m at "
+                    f"{cases.PISTON_FREQUENCY} Hz, floating box {case.box} of density {cases.BOX_DENSITY} "
+                    f"(FP64 6-DOF coupling), delta-SPH Antuono, Tait gamma 7, CFL 0.2, cell edge 2h/{args.gsf:g}")
+        metric = "particle-steps/s, 3D floating-body wave tank"
+    else:
+        args.dx = args.dx or 0.008
+        case = cases.dambreak3d(dx=args.dx, grid_size_factor=args.gsf)
+        setup = None
+        workload = (f"3-D dam break 3.22x1.0x1.0 m, column 1.2x1.0x0.55 m, dx {args.dx} m, "
+                    f"{case.n} particles ({case.json.water_ptc_num} fluid), delta-SPH Antuono, "
+                    f"Tait gamma 7, CFL 0.2, cell edge 2h/{args.gsf:g}")
+        metric = "particle-steps/s, 1.3M 3D dam-break"
+    print(f"[bench] {args.case} dx={args.dx}: N={case.n} (fluid {case.json.water_ptc_num}), "
           f"grid {case.cpu.grid_xdim}x{case.cpu.grid_ydim}x{case.cpu.grid_zdim}", file=sys.stderr, flush=True)
     s = z.Solver(case, device=0)
     s.upload(case.host)
+    if setup:
+        setup(s)
     # pick the neighbour-walk variant and sort digit width on this case (state restored after)
     if args.walk is not None:
         s.set_tuning(args.walk, s.get_tuning()["radix_bits"])
@@ -108,6 +140,10 @@ def main():
     launches = 2 * args.profile_steps
     gov_ms = prof["governing"] / launches
     killed = s.killed()
+    extra = {}
+    if floating:
+        b = s.get_body()
+        extra = {"sim_time_s": s.time(), "body_c": list(b.c[:]), "body_q": list(b.q[:])}
     s.close()
 
     ms_step = wall * 1e3 / args.steps
@@ -116,7 +152,7 @@ def main():
     gov_gbs = gbytes / (gov_ms * 1e-3) / 1e9
     b_step = step_bytes(case)
     out = {
-        "metric": "particle-steps/s, 1.3M 3D dam-break",
+        "metric": metric,
         "value": value,
         "unit": "particle-steps/s",
         "n_gpus": 1,
@@ -127,14 +163,12 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32",
-        "data": "synthetic (lattice dam break generated in-process)",
-        "config": {"workload": f"3-D dam break 3.22x1.0x1.0 m, column 1.2x1.0x0.55 m, dx {args.dx} m, "
-                               f"{case.n} particles ({case.json.water_ptc_num} fluid), delta-SPH Antuono, "
-                               f"Tait gamma 7, CFL 0.2, cell edge 2h/{args.gsf:g}",
+        "data": "synthetic (lattice case generated in-process, no input files)",
+        "config": {"workload": workload,
                    "particles": case.n, "fluid_particles": case.json.water_ptc_num,
                    "fluid_particle_steps_per_s": case.json.water_ptc_num * args.steps / wall,
                    "event_ms_per_step": ev_ms / args.steps, "removed_particles": killed,
-                   "tuning": tune},
+                   "tuning": tune, **extra},
         "roofline": {"bound": "hbm", "kernel": "k_governing", "achieved": gov_gbs, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": gov_gbs / HBM_PEAK_GBS, "traffic": None,
                      "bytes_per_launch": gbytes, "avg_launch_ms": gov_ms},
@@ -144,7 +178,7 @@ def main():
     }
     if not args.no_cpu_baseline:
         print("[bench] CPU baseline (FP32 oracle)...", file=sys.stderr, flush=True)
-        out["cpu_baseline"] = cpu_baseline(case)
+        out["cpu_baseline"] = cpu_baseline(case, steps=1 if floating else 3, setup=setup)
     print(json.dumps(out), flush=True)
 
 
