@@ -14,7 +14,12 @@ at dx 0.006 m (11.1 M fluid, 12.5 M particles), piston 0.025 m amplitude at
 1 Hz, a 0.3 x 0.3 x 0.2 m box of density 500 coupled through its FP64 rigid
 state (force/torque reduction, body update and dummy kinematics every half step).
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--case dambreak3d|wavetank3d] [--no-cpu-baseline]
+--case ensemble2d: 512 independent 2-D dam breaks (column width U(0.2,0.6),
+height U(0.3,0.8), dx U(0.004,0.01) per scene; 3.8 M particles) advanced
+together in one context, each on its own cell grid with its own dt.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--case dambreak3d|wavetank3d|ensemble2d]
+                    [--no-cpu-baseline]
 
 Prints one JSON line.  Only one GPU is used: --gpus above 1 exits.
 """
@@ -30,38 +35,40 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
 
 
-def step_bytes(case):
+def step_bytes(n, G, n_fluid):
     """Algorithmic HBM bytes per particle-step of the step's kernel schedule:
     640 + 48 G/N + 16 f_wall + 204 f_fluid (compulsory traffic, neighbour
     reads served by caches, sort at its single-pass bound)."""
-    n = case.n
-    G = case.cpu.grid_num
-    ff = case.json.water_ptc_num / n
+    ff = n_fluid / n
     return 640.0 + 48.0 * G / n + 16.0 * (1.0 - ff) + 204.0 * ff
 
 
-def governing_bytes(case):
+def governing_bytes(n, G):
     """Algorithmic bytes of one continuity+momentum launch: per particle it
     reads (x,rho) 16, (v,p) 16, (grad rho, div v) 16, type 4, hash 4 and writes
     (a, drho/dt) 16 = 72 B; plus one pass over the cell table, 4 (G+1) B."""
-    return 72.0 * case.n + 4.0 * (case.cpu.grid_num + 1)
+    return 72.0 * n + 4.0 * (G + 1)
 
 
-def cpu_baseline(case, steps=3, warmup=1, setup=None):
-    """FP32 CPU oracle (OpenMP over particles) on this host's cores, same case."""
+def cpu_baseline(scenes, steps=3, warmup=1, setup=None, what="the same case"):
+    """FP32 CPU oracle (OpenMP over particles) on this host's cores: `steps`
+    steps of each case in `scenes` (a bounded sample of the workload)."""
     import oracle
     threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
-    o = oracle.Oracle(case.params, fp64=False)
-    o.upload(case.host)
-    if setup:
-        setup(o)
-    o.step(warmup)
-    t0 = time.perf_counter()
-    o.step(steps)
-    dt = time.perf_counter() - t0
-    o.close()
-    return {"value": case.n * steps / dt, "unit": "particle-steps/s", "cores": threads, "kind": "port",
-            "sample": f"{steps} steps (after {warmup} warm-up) of the same {case.n}-particle case, "
+    work, elapsed = 0, 0.0
+    for case in scenes:
+        o = oracle.Oracle(case.params, fp64=False)
+        o.upload(case.host)
+        if setup:
+            setup(o)
+        o.step(warmup)
+        t0 = time.perf_counter()
+        o.step(steps)
+        elapsed += time.perf_counter() - t0
+        work += case.n * steps
+        o.close()
+    return {"value": work / elapsed, "unit": "particle-steps/s", "cores": threads, "kind": "port",
+            "sample": f"{steps} steps (after {warmup} warm-up) of {what} ({work // steps} particles), "
                       f"FP32 oracle, OpenMP {threads} threads"}
 
 
@@ -70,7 +77,8 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=20)
-    ap.add_argument("--case", choices=("dambreak3d", "wavetank3d"), default="dambreak3d")
+    ap.add_argument("--case", choices=("dambreak3d", "wavetank3d", "ensemble2d"), default="dambreak3d")
+    ap.add_argument("--scenes", type=int, default=512, help="ensemble2d: number of scenes")
     ap.add_argument("--dx", type=float, default=None, help="particle spacing (default 0.008 / 0.006)")
     ap.add_argument("--gsf", type=float, default=2.0, help="grid_size_factor: cell edge = 2h / gsf")
     ap.add_argument("--profile-steps", type=int, default=3)
@@ -100,6 +108,14 @@ def main():
                     f"{cases.PISTON_FREQUENCY} Hz, floating box {case.box} of density {cases.BOX_DENSITY} "
                     f"(FP64 6-DOF coupling), delta-SPH Antuono, Tait gamma 7, CFL 0.2, cell edge 2h/{args.gsf:g}")
         metric = "particle-steps/s, 3D floating-body wave tank"
+    elif args.case == "ensemble2d":
+        case = cases.ensemble2d(n_scenes=args.scenes, grid_size_factor=args.gsf)
+        setup = None
+        workload = (f"{case.nscenes} independent 2-D dam breaks in one context (tank 1.61x1.0 m; column width "
+                    f"U(0.2,0.6), height U(0.3,0.8), dx U(0.004,0.01) per scene, SplitMix64 seeds), {case.n} "
+                    f"particles, per-scene grids and dt, delta-SPH Antuono, Tait gamma 7, CFL 0.2, "
+                    f"cell edge 2h/{args.gsf:g}")
+        metric = f"particle-steps/s, {case.nscenes}-scene 2D ensemble"
     else:
         args.dx = args.dx or 0.008
         case = cases.dambreak3d(dx=args.dx, grid_size_factor=args.gsf)
@@ -108,8 +124,10 @@ def main():
                     f"{case.n} particles ({case.json.water_ptc_num} fluid), delta-SPH Antuono, "
                     f"Tait gamma 7, CFL 0.2, cell edge 2h/{args.gsf:g}")
         metric = "particle-steps/s, 1.3M 3D dam-break"
-    print(f"[bench] {args.case} dx={args.dx}: N={case.n} (fluid {case.json.water_ptc_num}), "
-          f"grid {case.cpu.grid_xdim}x{case.cpu.grid_ydim}x{case.cpu.grid_zdim}", file=sys.stderr, flush=True)
+    batch = isinstance(case, z.Batch)
+    n_fluid = int((case.host.type == z.FLUID).sum())
+    G = case.grid_num if batch else case.cpu.grid_num
+    print(f"[bench] {args.case}: N={case.n} (fluid {n_fluid}), {G} cells", file=sys.stderr, flush=True)
     s = z.Solver(case, device=0)
     s.upload(case.host)
     if setup:
@@ -148,9 +166,9 @@ def main():
 
     ms_step = wall * 1e3 / args.steps
     value = case.n * args.steps / wall
-    gbytes = governing_bytes(case)
+    gbytes = governing_bytes(case.n, G)
     gov_gbs = gbytes / (gov_ms * 1e-3) / 1e9
-    b_step = step_bytes(case)
+    b_step = step_bytes(case.n, G, n_fluid)
     out = {
         "metric": metric,
         "value": value,
@@ -165,8 +183,8 @@ def main():
         "dtype": "f32",
         "data": "synthetic (lattice case generated in-process, no input files)",
         "config": {"workload": workload,
-                   "particles": case.n, "fluid_particles": case.json.water_ptc_num,
-                   "fluid_particle_steps_per_s": case.json.water_ptc_num * args.steps / wall,
+                   "particles": case.n, "fluid_particles": n_fluid, "cells": G,
+                   "fluid_particle_steps_per_s": n_fluid * args.steps / wall,
                    "event_ms_per_step": ev_ms / args.steps, "removed_particles": killed,
                    "tuning": tune, **extra},
         "roofline": {"bound": "hbm", "kernel": "k_governing", "achieved": gov_gbs, "peak": HBM_PEAK_GBS,
@@ -178,7 +196,12 @@ def main():
     }
     if not args.no_cpu_baseline:
         print("[bench] CPU baseline (FP32 oracle)...", file=sys.stderr, flush=True)
-        out["cpu_baseline"] = cpu_baseline(case, steps=1 if floating else 3, setup=setup)
+        if batch:  # every 16th scene: a bounded, size-representative sample
+            out["cpu_baseline"] = cpu_baseline(case.cases[::16], steps=20,
+                                               what=f"every 16th scene of the {case.nscenes}-scene ensemble")
+        else:
+            out["cpu_baseline"] = cpu_baseline([case], steps=1 if floating else 3, setup=setup,
+                                               what=f"the same {case.n}-particle case")
     print(json.dumps(out), flush=True)
 
 

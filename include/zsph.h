@@ -336,6 +336,29 @@ int zsph_diagnostics(zsph_ctx *ctx, int buf, float front_zmax, zsph_diag *out);
 int zsph_probe_pressure(zsph_ctx *ctx, int buf, const float *pts, int npts, float *p_out);
 
 /* ------------------------------------------------------------------------ */
+/* Batches of independent scenes (no counterpart in the reference, which    */
+/* runs one case per process).                                              */
+/* ------------------------------------------------------------------------ */
+
+/* One context for `nscenes` independent scenes, each with its own
+ * parameters (spacing, h, c0, dt, grid; scenes[k].ptc_num particles).  The
+ * scenes share the particle buffers and every launch: the cell key space is
+ * the concatenation of the scenes' grids, so each scene keeps its own cell
+ * grid and its particles never see another scene's.  Upload the particles
+ * scene by scene (scene 0's first); every stage, zsph_step and the
+ * download then work on the whole batch, and each scene's results are
+ * bitwise those of the same scene in a context of its own.  Body, piston
+ * and probe calls are for single-scene contexts.  zsph_get_time reports
+ * scene 0's clock; scene k's is its step count times its own dt. */
+int zsph_create_batch(const zsph_params *scenes, int nscenes, int device, zsph_ctx **ctx);
+
+/* Diagnostics of each scene (out[nscenes]; one entry for a single-scene
+ * context) from the cell table of buffer `buf`, which must be in cell order
+ * (after zsph_sort_data into it, or zsph_step for buffer 0).  For a batch,
+ * zsph_diagnostics returns the sums over scenes. */
+int zsph_scene_diagnostics(zsph_ctx *ctx, int buf, float front_zmax, zsph_diag *out);
+
+/* ------------------------------------------------------------------------ */
 /* Moving dummies: floating body (6-DOF coupling) and piston wavemaker.     */
 /* The reference has the rigid-body equations only as a standalone test    */
 /* (Test_6DOF/) and zero wall-acceleration placeholders in the dummy        */

@@ -14,7 +14,7 @@ import numpy as np
 from . import _abi as abi
 from ._abi import (A_ACC, A_CELL_START, A_GD, A_INDEX, A_KEY, A_POSR, A_TABLE, A_TPOSR, A_TVELP,
                    A_TYPE, A_VELP, BODY, DIFF_ANTUONO, DIFF_MC, F_ACC, F_ALL, F_HASH, F_POS, F_RHOP,
-                   F_TABLE, F_TYPE, F_VEL, FLUID, KILLED, PISTON, WALL)
+                   F_TABLE, F_TYPE, F_VEL, FLUID, KILLED, PISTON, WALL, WALK_LIST, WALK_MASK, WALK_QUEUE)
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libzsph.so")
@@ -157,16 +157,57 @@ _ARRAY_SHAPES = {A_POSR: (4, np.float32), A_VELP: (4, np.float32), A_TPOSR: (4, 
                  A_GD: (4, np.float32)}
 
 
+class Batch:
+    """Independent scenes advanced together in one context (zsph_create_batch).
+
+    Particles are concatenated scene by scene; ``offsets[k]`` is scene k's
+    first particle in that upload order (``table + index`` of a downloaded
+    particle gives it back)."""
+
+    def __init__(self, cases):
+        self.cases = list(cases)
+        if not self.cases:
+            raise ValueError("a batch needs at least one scene")
+        self.offsets = np.cumsum([0] + [c.n for c in self.cases]).astype(np.int64)
+        self.host = HostArrays(int(self.offsets[-1]))
+        for c, o in zip(self.cases, self.offsets):
+            for k in ("pos", "vel", "rhop", "acc", "type", "table", "hash"):
+                getattr(self.host, k)[o:o + c.n] = getattr(c.host, k)
+        self.params = (abi.Params * len(self.cases))(*[c.params for c in self.cases])
+        self.grid_num = sum(c.params.grid_xdim * c.params.grid_ydim * c.params.grid_zdim for c in self.cases)
+
+    @property
+    def n(self):
+        return self.host.n
+
+    @property
+    def nscenes(self):
+        return len(self.cases)
+
+    def scene_of(self, original_index):
+        """Scene of each original particle index."""
+        return np.searchsorted(self.offsets, original_index, side="right") - 1
+
+
 class Solver:
-    """One device context (zsph_ctx): the reference's device buffers and kernels."""
+    """One device context (zsph_ctx): the reference's device buffers and kernels.
+    `case` is a Case (one scene) or a Batch (independent scenes in one context)."""
 
     def __init__(self, case, device=0):
         self.case = case
-        self.params = case.params if isinstance(case, Case) else case
-        self.n = self.params.ptc_num
-        self.grid_num = self.params.grid_xdim * self.params.grid_ydim * self.params.grid_zdim
         self._ctx = C.c_void_p()
-        rc = lib().zsph_create(C.byref(self.params), device, C.byref(self._ctx))
+        if isinstance(case, Batch):
+            self.params = case.params
+            self.n = case.n
+            self.grid_num = case.grid_num
+            self.nscenes = case.nscenes
+            rc = lib().zsph_create_batch(case.params, case.nscenes, device, C.byref(self._ctx))
+        else:
+            self.params = case.params if isinstance(case, Case) else case
+            self.n = self.params.ptc_num
+            self.grid_num = self.params.grid_xdim * self.params.grid_ydim * self.params.grid_zdim
+            self.nscenes = 1
+            rc = lib().zsph_create(C.byref(self.params), device, C.byref(self._ctx))
         if rc != abi.OK:
             raise ZsphError(rc, "zsph_create")
 
@@ -285,6 +326,12 @@ class Solver:
         d = abi.Diag()
         self._chk(lib().zsph_diagnostics(self._ctx, buf, front_zmax, C.byref(d)), "zsph_diagnostics")
         return d.as_dict()
+
+    def scene_diagnostics(self, buf=0, front_zmax=float("inf")):
+        """Per-scene diagnostics (list of dicts); buffer `buf` must be in cell order."""
+        d = (abi.Diag * self.nscenes)()
+        self._chk(lib().zsph_scene_diagnostics(self._ctx, buf, front_zmax, d), "zsph_scene_diagnostics")
+        return [x.as_dict() for x in d]
 
     def probe_pressure(self, points, buf=0):
         pts = np.ascontiguousarray(points, np.float32).reshape(-1, 3)

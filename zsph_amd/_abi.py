@@ -170,6 +170,8 @@ def load(path):
     _sig(lib, "zsph_get_time", _i, [vp, P(C.c_double)])
     _sig(lib, "zsph_body_reduce", _i, [vp, _i, P(C.c_double), P(C.c_double)])
     _sig(lib, "zsph_case_wavetank", _i, [P(WaveTank), P(Json), P(HostState), P(_i)])
+    _sig(lib, "zsph_create_batch", _i, [P(Params), _i, _i, P(vp)])
+    _sig(lib, "zsph_scene_diagnostics", _i, [vp, _i, _f, P(Diag)])
     return lib
 
 
@@ -184,6 +186,7 @@ EXPORTS = (
     "zsph_probe_pressure", "zsph_rigid_mass_properties", "zsph_rigid_acc_alpha", "zsph_rigid_euler_step",
     "zsph_set_tuning", "zsph_get_tuning", "zsph_autotune", "zsph_set_body", "zsph_get_body",
     "zsph_set_piston", "zsph_set_time", "zsph_get_time", "zsph_body_reduce", "zsph_case_wavetank",
+    "zsph_create_batch", "zsph_scene_diagnostics",
 )
 
 STAGES = ("hash", "sort", "cells", "reorder", "wall", "ldg", "governing", "integrate")

@@ -21,7 +21,8 @@ struct Prm {
     float gx0, gy0, gz0;
     int k;           // cells searched each side
     int X, Y, Z, XY;
-    int G;           // number of cells
+    int G;           // number of cells (all scenes of a batch)
+    int cell_base;   // first cell of this scene in the key space (0 for one scene)
     uint32_t flag;   // sort-key bit of non-fluid particles (1 << cell bits)
     int N;
     int dim;
@@ -84,7 +85,16 @@ struct Buf {
     float4 *tvelp;  // saved v, p
     int *type;
     int *table;
+    int *scene;     // scene of each particle (batches only; nullptr for one scene)
 };
+
+// The parameters of particle i's scene: the kernel argument for one scene,
+// the scene's entry of the parameter table for a batch.
+template <bool BATCH>
+__device__ __forceinline__ Prm scene_prm(const Prm &q0, const Prm *__restrict__ sp, const Buf &b, int i) {
+    if constexpr (BATCH) return sp[b.scene[i]];
+    else return q0;
+}
 
 #define ZS_BLOCK 256
 
@@ -151,7 +161,9 @@ __device__ __forceinline__ CellBox cell_box(const Prm &q, uint32_t key) {
 }
 
 // Sort keys put every fluid particle before every other particle: key =
-// cell | (type != fluid ? flag : 0), flag = 1 << cell bits.  The cell table
+// cell | (type != fluid ? flag : 0), flag = 1 << cell bits.  In a batch the
+// cell is global: scene s owns cells [cell_base_s, cell_base_s + G_s), so a
+// scene's fluid (and its dummies) stay contiguous after the sort.  The cell table
 // cs has 2 (G + 1) entries: fluid particles of cell c are [cs[c], cs[c+1]),
 // the others (dummies, removed) are [cs[G+1+c], cs[G+1+c+1]).  Fluid
 // particles therefore occupy [0, cs[G]) and dummies [cs[G], N).
@@ -182,7 +194,7 @@ __device__ __forceinline__ void for_each_row(const Prm &q, float4 xi, uint32_t c
             const int xa = max(b.xlo, (int)floorf(fx - w));
             const int xb = min(b.xhi, (int)floorf(fx + w));
             if (xa > xb) continue;
-            const int row = cz * q.XY + cy * q.X;
+            const int row = q.cell_base + cz * q.XY + cy * q.X;
             int fb = 0, fe = 0, db = 0, de = 0;
             if (FLUID_J) { fb = cs[row + xa]; fe = cs[row + xb + 1]; }
             if (DUMMY_J) { db = cs[q.G + 1 + row + xa]; de = cs[q.G + 1 + row + xb + 1]; }

@@ -104,6 +104,51 @@ __global__ __launch_bounds__(ZS_BLOCK) void k_probe(Prm q, Buf b, const float *_
     }
 }
 
+// One block per scene of a batch (or the single scene): the scene's fluid
+// is the contiguous range of its cells' fluid segments, its removed
+// particles sit in its dummy segments.  part[s*SCENE_D + {count, kinetic,
+// mx, my, mz, front, zmax, removed}], summed in a fixed order.
+template <bool BATCH>
+__global__ __launch_bounds__(ZS_BLOCK) void k_scene_diag(Prm q0, const Prm *__restrict__ sp, Buf b,
+                                                         const int *__restrict__ cs, float front_zmax,
+                                                         double *__restrict__ part) {
+    __shared__ double red[SCENE_D][ZS_BLOCK];
+    const Prm q = BATCH ? sp[blockIdx.x] : q0;
+    const int c0 = q.cell_base, c1 = q.cell_base + q.X * q.Y * q.Z;
+    double v[SCENE_D] = {0.0, 0.0, 0.0, 0.0, 0.0, -INFINITY, -INFINITY, 0.0};
+    for (int i = cs[c0] + threadIdx.x; i < cs[c1]; i += ZS_BLOCK) {
+        const float4 x = b.posr[i], u = b.velp[i];
+        v[0] += 1.0;
+        v[1] += 0.5 * (double)q.m * ((double)u.x * u.x + (double)u.y * u.y + (double)u.z * u.z);
+        v[2] += (double)q.m * u.x;
+        v[3] += (double)q.m * u.y;
+        v[4] += (double)q.m * u.z;
+        if (x.z < front_zmax) v[5] = fmax(v[5], (double)x.x);
+        v[6] = fmax(v[6], (double)x.z);
+    }
+    for (int i = cs[q.G + 1 + c0] + threadIdx.x; i < cs[q.G + 1 + c1]; i += ZS_BLOCK)
+        if (b.type[i] == ZSPH_KILLED) v[7] += 1.0;
+    for (int k = 0; k < SCENE_D; ++k) red[k][threadIdx.x] = v[k];
+    __syncthreads();
+    for (int w = ZS_BLOCK / 2; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w)
+            for (int k = 0; k < SCENE_D; ++k) {
+                const double a = red[k][threadIdx.x], c = red[k][threadIdx.x + w];
+                red[k][threadIdx.x] = (k == 5 || k == 6) ? fmax(a, c) : a + c;
+            }
+        __syncthreads();
+    }
+    if (threadIdx.x < SCENE_D) part[(size_t)blockIdx.x * SCENE_D + threadIdx.x] = red[threadIdx.x][0];
+}
+
+void launch_scene_diag(hipStream_t s, const Prm &q, const Prm *sp, int nscenes, const Buf &b, const int *cs,
+                       float front_zmax, double *part) {
+    if (sp)
+        k_scene_diag<true><<<nscenes, ZS_BLOCK, 0, s>>>(q, sp, b, cs, front_zmax, part);
+    else
+        k_scene_diag<false><<<1, ZS_BLOCK, 0, s>>>(q, sp, b, cs, front_zmax, part);
+}
+
 int diag_blocks(int n) { return (n + ZS_BLOCK - 1) / ZS_BLOCK; }
 
 void launch_diag(hipStream_t s, const Prm &q, const Buf &b, float front_zmax, double *part, float *pmax) {

@@ -8,7 +8,9 @@
 namespace zs {
 
 // nbr_build.hip
-void launch_hash(hipStream_t s, const Prm &q, const Buf &b, uint32_t *key, int *idx, int *killed);
+// sp: per-scene parameter table of a batch (device memory), nullptr for one
+// scene; b.scene must then hold each particle's scene.
+void launch_hash(hipStream_t s, const Prm &q, const Prm *sp, const Buf &b, uint32_t *key, int *idx, int *killed);
 int radix_blocks(int n);
 size_t radix_hist_words(int n, int bits);
 // Sorts (key[0], val[0]) over the low `keybits` bits; returns the slot (0/1)
@@ -24,21 +26,26 @@ void launch_reorder(hipStream_t s, int n, const int *perm, const Buf &a, const B
 // WALK_LIST: launch_ldg records the neighbour lists (nlist: list_words(N)
 // ints, ncount: N ints) that launch_governing then reads; a caller without a
 // current list passes another walk to launch_governing.
-void launch_wall(hipStream_t s, int walk, const Prm &q, const Buf &b, const uint32_t *skey, const int *cs);
-void launch_ldg(hipStream_t s, int walk, const Prm &q, const Buf &b, const uint32_t *skey, const int *cs,
-                float4 *gd, int *nlist, int *ncount);
-void launch_governing(hipStream_t s, int walk, const Prm &q, const Buf &b, const uint32_t *skey, const int *cs,
-                      const float4 *gd, float4 *acc, const int *nlist, const int *ncount);
-void launch_predict(hipStream_t s, const Prm &q, const Buf &b, const int *cs, const float4 *acc);
-void launch_correct(hipStream_t s, const Prm &q, const Buf &b, const int *cs, const float4 *acc);
-void launch_nbr_stats(hipStream_t s, int walk, const Prm &q, const Buf &b, const uint32_t *skey, const int *cs,
-                      int *count, uint64_t *digest);
+void launch_wall(hipStream_t s, int walk, const Prm &q, const Prm *sp, const Buf &b, const uint32_t *skey,
+                 const int *cs);
+void launch_ldg(hipStream_t s, int walk, const Prm &q, const Prm *sp, const Buf &b, const uint32_t *skey,
+                const int *cs, float4 *gd, int *nlist, int *ncount);
+void launch_governing(hipStream_t s, int walk, const Prm &q, const Prm *sp, const Buf &b, const uint32_t *skey,
+                      const int *cs, const float4 *gd, float4 *acc, const int *nlist, const int *ncount);
+void launch_predict(hipStream_t s, const Prm &q, const Prm *sp, const Buf &b, const int *cs, const float4 *acc);
+void launch_correct(hipStream_t s, const Prm &q, const Prm *sp, const Buf &b, const int *cs, const float4 *acc);
+void launch_nbr_stats(hipStream_t s, int walk, const Prm &q, const Prm *sp, const Buf &b, const uint32_t *skey,
+                      const int *cs, int *count, uint64_t *digest);
 
 // diag.hip
 constexpr int DIAG_D = 5;  // per-block FP64 partials: count, kinetic, momentum x/y/z
 int diag_blocks(int n);
 void launch_diag(hipStream_t s, const Prm &q, const Buf &b, float front_zmax, double *part, float *pmax);
 void launch_probe(hipStream_t s, const Prm &q, const Buf &b, const float *pts, int npts, float *out);
+// Per-scene diagnostics from the cell table of buffer b (one block per scene).
+constexpr int SCENE_D = 8;  // count, kinetic, momentum x/y/z, front, zmax, removed
+void launch_scene_diag(hipStream_t s, const Prm &q, const Prm *sp, int nscenes, const Buf &b, const int *cs,
+                       float front_zmax, double *part);
 
 // body.hip: floating body and piston.  Device-resident FP64 state, so a
 // captured step advances it with every replay.

@@ -16,10 +16,13 @@ namespace zs {
 // v = 0, (rho, p) = (rho0, 0) and H = 0 (hash.cu:16-23).  The sort key adds
 // the non-fluid flag above the cell bits, so fluid sorts first.
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(ZS_BLOCK) void k_hash(Prm q, Buf b, uint32_t *__restrict__ key,
-                                                   int *__restrict__ idx, int *__restrict__ killed) {
+template <bool BATCH>
+__global__ __launch_bounds__(ZS_BLOCK) void k_hash(Prm q0, const Prm *__restrict__ sp, Buf b,
+                                                   uint32_t *__restrict__ key, int *__restrict__ idx,
+                                                   int *__restrict__ killed) {
     const int i = xcd_block(gridDim.x) * ZS_BLOCK + threadIdx.x;
-    if (i >= q.N) return;
+    if (i >= q0.N) return;
+    const Prm q = scene_prm<BATCH>(q0, sp, b, i);
     const float4 x = b.posr[i];
     int cx, cy, cz;
     const bool in = cell_axis(x.x, q.gx0, q.grid_size, q.X, &cx) &&
@@ -38,7 +41,8 @@ __global__ __launch_bounds__(ZS_BLOCK) void k_hash(Prm q, Buf b, uint32_t *__res
         b.tvelp[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
         atomicAdd(killed, 1);
     }
-    key[i] = h | (t == ZSPH_FLUID ? 0u : q.flag);
+    // a removed particle stays in its own scene's cell 0 (dummy segment)
+    key[i] = ((uint32_t)q.cell_base + h) | (t == ZSPH_FLUID ? 0u : q.flag);
     idx[i] = i;
 }
 
@@ -241,6 +245,7 @@ __global__ __launch_bounds__(ZS_BLOCK) void k_reorder(int n, const int *__restri
     b.tvelp[k] = a.tvelp[p];
     b.type[k] = a.type[p];
     b.table[k] = a.table[p] + p - k;
+    if (a.scene) b.scene[k] = a.scene[p];
 }
 
 // ---------------------------------------------------------------------------
@@ -248,9 +253,12 @@ __global__ __launch_bounds__(ZS_BLOCK) void k_reorder(int n, const int *__restri
 // ---------------------------------------------------------------------------
 static inline int nblocks(int n) { return (n + ZS_BLOCK - 1) / ZS_BLOCK; }
 
-void launch_hash(hipStream_t s, const Prm &q, const Buf &b, uint32_t *key, int *idx, int *killed) {
+void launch_hash(hipStream_t s, const Prm &q, const Prm *sp, const Buf &b, uint32_t *key, int *idx, int *killed) {
     if (q.N == 0) return;
-    k_hash<<<nblocks(q.N), ZS_BLOCK, 0, s>>>(q, b, key, idx, killed);
+    if (sp)
+        k_hash<true><<<nblocks(q.N), ZS_BLOCK, 0, s>>>(q, sp, b, key, idx, killed);
+    else
+        k_hash<false><<<nblocks(q.N), ZS_BLOCK, 0, s>>>(q, sp, b, key, idx, killed);
 }
 
 int radix_blocks(int n) { return (n + RS_TILE - 1) / RS_TILE; }

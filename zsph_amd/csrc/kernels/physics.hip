@@ -22,7 +22,10 @@ static inline int nblocks(int n) { return (n + ZS_BLOCK - 1) / ZS_BLOCK; }
 
 __device__ __forceinline__ float *wptr(float4 *a, int i) { return reinterpret_cast<float *>(a + i) + 3; }
 
-__device__ __forceinline__ uint32_t cell_of(const Prm &q, uint32_t key) { return key & (q.flag - 1u); }
+// Cell of a sort key, relative to the particle's scene (cell_base = 0 for one scene).
+__device__ __forceinline__ uint32_t cell_of(const Prm &q, uint32_t key) {
+    return (key & (q.flag - 1u)) - (uint32_t)q.cell_base;
+}
 
 // Dummy particles: Shepard extrapolation of fluid pressure with the
 // hydrostatic and wall-acceleration correction,
@@ -30,13 +33,15 @@ __device__ __forceinline__ uint32_t cell_of(const Prm &q, uint32_t key) { return
 // rho_w from the inverse EOS (delta_ns.cu:81-85, :116-131; a_w = 0 there).
 // a_w is the dummy's prescribed acceleration, kept in tvelp.xyz (zero for
 // fixed walls).
-template <int WALK>
-__global__ __launch_bounds__(ZS_BLOCK) void k_wall(Prm q, Buf b, const uint32_t *__restrict__ skey,
+template <int WALK, bool BATCH>
+__global__ __launch_bounds__(ZS_BLOCK) void k_wall(Prm q0, const Prm *__restrict__ sp, Buf b,
+        const uint32_t *__restrict__ skey,
                                                    const int *__restrict__ cs) {
     __shared__ int qbuf[WalkLds<WALK>::ints];
-    const int nf = cs[q.G];
+    const int nf = cs[q0.G];
     const int i = nf + xcd_block(gridDim.x) * ZS_BLOCK + threadIdx.x;
-    if (i >= q.N) return;
+    if (i >= q0.N) return;
+    const Prm q = scene_prm<BATCH>(q0, sp, b, i);
     const float4 xi = b.posr[i];
     const float4 aw = b.tvelp[i];
     float P = 0.0f, S = 0.0f;
@@ -59,18 +64,20 @@ __global__ __launch_bounds__(ZS_BLOCK) void k_wall(Prm q, Buf b, const uint32_t 
 // G_i = L_i^-1 sum F (rho_j - rho_i) V_j x_ij (delta_ns.cu:75-79, :86-90,
 // :99-114, :133-174).  L^-1 is applied once to the accumulated sum (it does
 // not depend on j).  MC form: D only.  Output gd = (G, D); dummies get 0.
-template <int WALK>
-__global__ __launch_bounds__(ZS_BLOCK) void k_ldg(Prm q, Buf b, const uint32_t *__restrict__ skey,
+template <int WALK, bool BATCH>
+__global__ __launch_bounds__(ZS_BLOCK) void k_ldg(Prm q0, const Prm *__restrict__ sp, Buf b,
+        const uint32_t *__restrict__ skey,
                                                   const int *__restrict__ cs, float4 *__restrict__ gd,
                                                   int *__restrict__ nlist, int *__restrict__ ncount) {
     __shared__ int qbuf[WalkLds<WALK>::ints];
-    const int nf = cs[q.G];
+    const int nf = cs[q0.G];
     const int i = xcd_block(gridDim.x) * ZS_BLOCK + threadIdx.x;
-    if (i >= q.N) return;
+    if (i >= q0.N) return;
     if (i >= nf) {
         gd[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
         return;
     }
+    const Prm q = scene_prm<BATCH>(q0, sp, b, i);
     const bool delta = q.diffusion == ZSPH_DIFF_ANTUONO;
     const float4 xi = b.posr[i];
     const float4 vi = b.velp[i];
@@ -124,15 +131,17 @@ __global__ __launch_bounds__(ZS_BLOCK) void k_ldg(Prm q, Buf b, const uint32_t *
 // Continuity + momentum of fluid particles (delta_ns.cu:204-265; MC form
 // simple_ns.cu:123-177).  Pressure and continuity over all neighbours,
 // delta term and viscosity over fluid pairs, gravity added.
-template <int WALK>
-__global__ __launch_bounds__(ZS_BLOCK) void k_governing(Prm q, Buf b, const uint32_t *__restrict__ skey,
+template <int WALK, bool BATCH>
+__global__ __launch_bounds__(ZS_BLOCK) void k_governing(Prm q0, const Prm *__restrict__ sp, Buf b,
+        const uint32_t *__restrict__ skey,
                                                         const int *__restrict__ cs, const float4 *__restrict__ gd,
                                                         float4 *__restrict__ acc, const int *__restrict__ nlist,
                                                         const int *__restrict__ ncount) {
     __shared__ int qbuf[WalkLds<WALK>::ints];
-    const int nf = cs[q.G];
+    const int nf = cs[q0.G];
     const int i = xcd_block(gridDim.x) * ZS_BLOCK + threadIdx.x;
     if (i >= nf) return;
+    const Prm q = scene_prm<BATCH>(q0, sp, b, i);
     const bool delta = q.diffusion == ZSPH_DIFF_ANTUONO;
     const float4 xi = b.posr[i];
     const float4 vi = b.velp[i];
@@ -174,14 +183,16 @@ __global__ __launch_bounds__(ZS_BLOCK) void k_governing(Prm q, Buf b, const uint
 // Dummies: moving ones (body, piston) get the pressure force of their fluid
 // neighbours, without gravity (their sum is the fluid force on the body);
 // fixed walls and removed particles get 0.
-template <int WALK>
-__global__ __launch_bounds__(ZS_BLOCK) void k_governing_dummy(Prm q, Buf b, const uint32_t *__restrict__ skey,
+template <int WALK, bool BATCH>
+__global__ __launch_bounds__(ZS_BLOCK) void k_governing_dummy(Prm q0, const Prm *__restrict__ sp, Buf b,
+        const uint32_t *__restrict__ skey,
                                                               const int *__restrict__ cs, float4 *__restrict__ acc) {
     __shared__ int qbuf[WalkLds<WALK>::ints];
-    const int nf = cs[q.G];
+    const int nf = cs[q0.G];
     const int i = nf + blockIdx.x * ZS_BLOCK + threadIdx.x;
-    if (i >= q.N) return;
+    if (i >= q0.N) return;
     const int ti = b.type[i];
+    const Prm q = scene_prm<BATCH>(q0, sp, b, i);
     float ax = 0, ay = 0, az = 0, dr = 0;
     if (ti == ZSPH_BODY || ti == ZSPH_PISTON) {
         const float4 xi = b.posr[i];
@@ -205,10 +216,13 @@ __global__ __launch_bounds__(ZS_BLOCK) void k_governing_dummy(Prm q, Buf b, cons
 
 // Midpoint predictor (time_int.cu:3-48), fluid only: save state n, then
 // x += dt/2 v, v += dt/2 a, rho = max(rho + dt/2 drho, rho0), p = EOS(rho).
-__global__ __launch_bounds__(ZS_BLOCK) void k_predict(Prm q, Buf b, const int *__restrict__ cs,
+template <bool BATCH>
+__global__ __launch_bounds__(ZS_BLOCK) void k_predict(Prm q0, const Prm *__restrict__ sp, Buf b,
+        const int *__restrict__ cs,
                                                       const float4 *__restrict__ acc) {
     const int i = blockIdx.x * ZS_BLOCK + threadIdx.x;
-    if (i >= cs[q.G]) return;
+    if (i >= cs[q0.G]) return;
+    const Prm q = scene_prm<BATCH>(q0, sp, b, i);
     float4 x = b.posr[i], v = b.velp[i];
     const float4 a = acc[i];
     b.tposr[i] = x;
@@ -223,10 +237,13 @@ __global__ __launch_bounds__(ZS_BLOCK) void k_predict(Prm q, Buf b, const int *_
 
 // Midpoint corrector (time_int.cu:50-91): x = x^n + dt v*, v = v^n + dt a*,
 // rho = max(rho^n + dt drho*, rho0), p = EOS(rho).
-__global__ __launch_bounds__(ZS_BLOCK) void k_correct(Prm q, Buf b, const int *__restrict__ cs,
+template <bool BATCH>
+__global__ __launch_bounds__(ZS_BLOCK) void k_correct(Prm q0, const Prm *__restrict__ sp, Buf b,
+        const int *__restrict__ cs,
                                                       const float4 *__restrict__ acc) {
     const int i = blockIdx.x * ZS_BLOCK + threadIdx.x;
-    if (i >= cs[q.G]) return;
+    if (i >= cs[q0.G]) return;
+    const Prm q = scene_prm<BATCH>(q0, sp, b, i);
     float4 x = b.tposr[i], v = b.tvelp[i];
     const float4 vs = b.velp[i];
     const float4 a = acc[i];
@@ -247,13 +264,15 @@ __device__ __forceinline__ uint64_t mix64(uint64_t z) {
 
 // Neighbour count and set digest sum_j mix64(j) (mod 2^64) per particle,
 // through the same walk as the physics kernels.
-template <int WALK>
-__global__ __launch_bounds__(ZS_BLOCK) void k_nbr_stats(Prm q, Buf b, const uint32_t *__restrict__ skey,
+template <int WALK, bool BATCH>
+__global__ __launch_bounds__(ZS_BLOCK) void k_nbr_stats(Prm q0, const Prm *__restrict__ sp, Buf b,
+        const uint32_t *__restrict__ skey,
                                                         const int *__restrict__ cs, int *__restrict__ count,
                                                         uint64_t *__restrict__ digest) {
     __shared__ int qbuf[WalkLds<WALK>::ints];
     const int i = blockIdx.x * ZS_BLOCK + threadIdx.x;
-    if (i >= q.N) return;
+    if (i >= q0.N) return;
+    const Prm q = scene_prm<BATCH>(q0, sp, b, i);
     const float4 xi = b.posr[i];
     int c = 0;
     uint64_t h = 0;
@@ -266,44 +285,71 @@ __global__ __launch_bounds__(ZS_BLOCK) void k_nbr_stats(Prm q, Buf b, const uint
     digest[i] = h;
 }
 
-// Walks other than the queue use the mask walk where no list applies.
-#define ZS_WALK_LAUNCH(kern, grid, ...)                                          \
-    do {                                                                         \
-        if (walk == WALK_QUEUE)                                                  \
-            kern<WALK_QUEUE><<<(grid), ZS_BLOCK, 0, s>>>(__VA_ARGS__);           \
-        else                                                                     \
-            kern<WALK_MASK><<<(grid), ZS_BLOCK, 0, s>>>(__VA_ARGS__);            \
+// Walks other than the queue use the mask walk where no list applies; a
+// batch (sp != nullptr) runs the per-scene-parameter instantiation.
+#define ZS_WALK_LAUNCH(kern, grid, ...)                                                         \
+    do {                                                                                        \
+        if (sp) {                                                                               \
+            if (walk == WALK_QUEUE)                                                             \
+                kern<WALK_QUEUE, true><<<(grid), ZS_BLOCK, 0, s>>>(__VA_ARGS__);                \
+            else                                                                                \
+                kern<WALK_MASK, true><<<(grid), ZS_BLOCK, 0, s>>>(__VA_ARGS__);                 \
+        } else {                                                                                \
+            if (walk == WALK_QUEUE)                                                             \
+                kern<WALK_QUEUE, false><<<(grid), ZS_BLOCK, 0, s>>>(__VA_ARGS__);               \
+            else                                                                                \
+                kern<WALK_MASK, false><<<(grid), ZS_BLOCK, 0, s>>>(__VA_ARGS__);                \
+        }                                                                                       \
     } while (0)
 
-void launch_wall(hipStream_t s, int walk, const Prm &q, const Buf &b, const uint32_t *skey, const int *cs) {
-    if (q.N) ZS_WALK_LAUNCH(k_wall, nblocks(q.N), q, b, skey, cs);
+void launch_wall(hipStream_t s, int walk, const Prm &q, const Prm *sp, const Buf &b, const uint32_t *skey,
+                 const int *cs) {
+    if (q.N) ZS_WALK_LAUNCH(k_wall, nblocks(q.N), q, sp, b, skey, cs);
 }
-void launch_ldg(hipStream_t s, int walk, const Prm &q, const Buf &b, const uint32_t *skey, const int *cs,
-                float4 *gd, int *nlist, int *ncount) {
+void launch_ldg(hipStream_t s, int walk, const Prm &q, const Prm *sp, const Buf &b, const uint32_t *skey,
+                const int *cs, float4 *gd, int *nlist, int *ncount) {
     if (!q.N) return;
-    if (walk == WALK_LIST)
-        k_ldg<WALK_LIST><<<nblocks(q.N), ZS_BLOCK, 0, s>>>(q, b, skey, cs, gd, nlist, ncount);
-    else
-        ZS_WALK_LAUNCH(k_ldg, nblocks(q.N), q, b, skey, cs, gd, nlist, ncount);
+    if (walk == WALK_LIST) {
+        if (sp)
+            k_ldg<WALK_LIST, true><<<nblocks(q.N), ZS_BLOCK, 0, s>>>(q, sp, b, skey, cs, gd, nlist, ncount);
+        else
+            k_ldg<WALK_LIST, false><<<nblocks(q.N), ZS_BLOCK, 0, s>>>(q, sp, b, skey, cs, gd, nlist, ncount);
+    } else {
+        ZS_WALK_LAUNCH(k_ldg, nblocks(q.N), q, sp, b, skey, cs, gd, nlist, ncount);
+    }
 }
-void launch_governing(hipStream_t s, int walk, const Prm &q, const Buf &b, const uint32_t *skey, const int *cs,
-                      const float4 *gd, float4 *acc, const int *nlist, const int *ncount) {
+void launch_governing(hipStream_t s, int walk, const Prm &q, const Prm *sp, const Buf &b, const uint32_t *skey,
+                      const int *cs, const float4 *gd, float4 *acc, const int *nlist, const int *ncount) {
     if (!q.N) return;
-    if (walk == WALK_LIST)
-        k_governing<WALK_LIST><<<nblocks(q.N), ZS_BLOCK, 0, s>>>(q, b, skey, cs, gd, acc, nlist, ncount);
+    if (walk == WALK_LIST) {
+        if (sp)
+            k_governing<WALK_LIST, true><<<nblocks(q.N), ZS_BLOCK, 0, s>>>(q, sp, b, skey, cs, gd, acc, nlist,
+                                                                           ncount);
+        else
+            k_governing<WALK_LIST, false><<<nblocks(q.N), ZS_BLOCK, 0, s>>>(q, sp, b, skey, cs, gd, acc, nlist,
+                                                                            ncount);
+    } else {
+        ZS_WALK_LAUNCH(k_governing, nblocks(q.N), q, sp, b, skey, cs, gd, acc, nlist, ncount);
+    }
+    ZS_WALK_LAUNCH(k_governing_dummy, nblocks(q.N), q, sp, b, skey, cs, acc);
+}
+void launch_predict(hipStream_t s, const Prm &q, const Prm *sp, const Buf &b, const int *cs, const float4 *acc) {
+    if (!q.N) return;
+    if (sp)
+        k_predict<true><<<nblocks(q.N), ZS_BLOCK, 0, s>>>(q, sp, b, cs, acc);
     else
-        ZS_WALK_LAUNCH(k_governing, nblocks(q.N), q, b, skey, cs, gd, acc, nlist, ncount);
-    ZS_WALK_LAUNCH(k_governing_dummy, nblocks(q.N), q, b, skey, cs, acc);
+        k_predict<false><<<nblocks(q.N), ZS_BLOCK, 0, s>>>(q, sp, b, cs, acc);
 }
-void launch_predict(hipStream_t s, const Prm &q, const Buf &b, const int *cs, const float4 *acc) {
-    if (q.N) k_predict<<<nblocks(q.N), ZS_BLOCK, 0, s>>>(q, b, cs, acc);
+void launch_correct(hipStream_t s, const Prm &q, const Prm *sp, const Buf &b, const int *cs, const float4 *acc) {
+    if (!q.N) return;
+    if (sp)
+        k_correct<true><<<nblocks(q.N), ZS_BLOCK, 0, s>>>(q, sp, b, cs, acc);
+    else
+        k_correct<false><<<nblocks(q.N), ZS_BLOCK, 0, s>>>(q, sp, b, cs, acc);
 }
-void launch_correct(hipStream_t s, const Prm &q, const Buf &b, const int *cs, const float4 *acc) {
-    if (q.N) k_correct<<<nblocks(q.N), ZS_BLOCK, 0, s>>>(q, b, cs, acc);
-}
-void launch_nbr_stats(hipStream_t s, int walk, const Prm &q, const Buf &b, const uint32_t *skey, const int *cs,
-                      int *count, uint64_t *digest) {
-    if (q.N) ZS_WALK_LAUNCH(k_nbr_stats, nblocks(q.N), q, b, skey, cs, count, digest);
+void launch_nbr_stats(hipStream_t s, int walk, const Prm &q, const Prm *sp, const Buf &b, const uint32_t *skey,
+                      const int *cs, int *count, uint64_t *digest) {
+    if (q.N) ZS_WALK_LAUNCH(k_nbr_stats, nblocks(q.N), q, sp, b, skey, cs, count, digest);
 }
 
 }  // namespace zs

@@ -39,6 +39,12 @@ struct zsph_ctx {
     int *nlist = nullptr;      // WALK_LIST neighbour lists (list_words(N) ints) and counts
     int *ncount = nullptr;
     bool list_valid = false;   // the lists describe the current sorted buffer
+    int nscenes = 1;                  // > 1: a batch of independent scenes
+    zs::Prm *scene_prm = nullptr;     // per-scene parameters (device, batches only)
+    std::vector<zs::Prm> scene_host;  // the same on the host
+    std::vector<int> scene_off;       // first particle of each scene in upload order (+ N)
+    int cells_buf = -1;               // buffer the cell table describes
+    double *scene_part = nullptr;     // per-scene diagnostics (batches only)
     zs::DevBody *body = nullptr;  // moving dummies: FP64 body state, piston, device time
     double *body_part = nullptr;  // BODY_BLOCKS x BODY_D force/torque partials
     bool has_body = false, has_piston = false;
@@ -129,7 +135,7 @@ struct Scope {
 int stage_hash(zsph_ctx *c, int b, Prof *pf = nullptr) {
     {
         Scope sc(pf, ZSPH_ST_HASH);
-        zs::launch_hash(c->stream, c->q, c->buf[b], c->key[0], c->idx[0], c->killed);
+        zs::launch_hash(c->stream, c->q, c->scene_prm, c->buf[b], c->key[0], c->idx[0], c->killed);
     }
     c->hashed = 1;
     c->sorted_slot = -1;
@@ -159,6 +165,7 @@ int stage_sort_data(zsph_ctx *c, int from, int to, Prof *pf = nullptr) {
         zs::launch_reorder(c->stream, c->q.N, c->idx[c->sorted_slot], c->buf[from], c->buf[to]);
     }
     c->list_valid = false;
+    c->cells_buf = to;
     return check_launch(c, "sort_data");
 }
 int stage_boundary(zsph_ctx *c, int b, Prof *pf = nullptr) {
@@ -166,11 +173,12 @@ int stage_boundary(zsph_ctx *c, int b, Prof *pf = nullptr) {
     const uint32_t *sk = c->key[c->sorted_slot];
     {
         Scope sc(pf, ZSPH_ST_WALL);
-        zs::launch_wall(c->stream, c->walk, c->q, c->buf[b], sk, c->cell_start);
+        zs::launch_wall(c->stream, c->walk, c->q, c->scene_prm, c->buf[b], sk, c->cell_start);
     }
     {
         Scope sc(pf, ZSPH_ST_LDG);
-        zs::launch_ldg(c->stream, c->walk, c->q, c->buf[b], sk, c->cell_start, c->gd, c->nlist, c->ncount);
+        zs::launch_ldg(c->stream, c->walk, c->q, c->scene_prm, c->buf[b], sk, c->cell_start, c->gd, c->nlist,
+                       c->ncount);
     }
     c->list_valid = (c->walk == zs::WALK_LIST);
     return check_launch(c, "boundary");
@@ -181,7 +189,8 @@ int stage_governing(zsph_ctx *c, int b, Prof *pf = nullptr) {
         Scope sc(pf, ZSPH_ST_GOVERNING);
         // the list walk needs this half step's lists; otherwise walk the cells
         const int walk = (c->walk == zs::WALK_LIST && !c->list_valid) ? zs::WALK_MASK : c->walk;
-        zs::launch_governing(c->stream, walk, c->q, c->buf[b], c->key[c->sorted_slot], c->cell_start, c->gd, c->acc,
+        zs::launch_governing(c->stream, walk, c->q, c->scene_prm, c->buf[b], c->key[c->sorted_slot], c->cell_start,
+                             c->gd, c->acc,
                              c->nlist, c->ncount);
     }
     return check_launch(c, "governing");
@@ -197,9 +206,9 @@ int stage_integrate(zsph_ctx *c, int b, bool predictor, Prof *pf = nullptr) {
         zs::launch_body_reduce(c->stream, c->q, c->buf[b], c->cell_start, c->acc, c->body, c->body_part);
     if (moving) zs::launch_body_update(c->stream, c->q, c->body, c->body_part, predictor);
     if (predictor)
-        zs::launch_predict(c->stream, c->q, c->buf[b], c->cell_start, c->acc);
+        zs::launch_predict(c->stream, c->q, c->scene_prm, c->buf[b], c->cell_start, c->acc);
     else
-        zs::launch_correct(c->stream, c->q, c->buf[b], c->cell_start, c->acc);
+        zs::launch_correct(c->stream, c->q, c->scene_prm, c->buf[b], c->cell_start, c->acc);
     if (moving) zs::launch_kinematics(c->stream, c->q, c->buf[b], c->cell_start, c->body);
     return check_launch(c, predictor ? "prediction" : "correction");
 }
@@ -228,42 +237,33 @@ void drop_graph(zsph_ctx *c) {
     c->graph = nullptr;
 }
 
-}  // namespace
-
-extern "C" {
-
-int zsph_create(const zsph_params *p, int device, zsph_ctx **out) {
-    if (!p || !out) return ZSPH_E_ARG;
-    *out = nullptr;
+bool valid_params(const zsph_params *p) {
     if (p->ptc_num < 0 || p->grid_xdim < 1 || p->grid_ydim < 1 || p->grid_zdim < 1 || !(p->h > 0.0f) ||
         !(p->grid_size > 0.0f) || p->grid_factor < 1 || (p->dim != 2 && p->dim != 3) || !(p->gamma > 0.0f) ||
         !(p->rc2 > 0.0f) || !(p->rho0 > 0.0f) || !(p->cs2 > 0.0f))
-        return ZSPH_E_ARG;
+        return false;
     const long long G = (long long)p->grid_xdim * p->grid_ydim * p->grid_zdim;
-    if (G > (1ll << 30) || G - 1 != (long long)p->grid_hash_max) return ZSPH_E_ARG;
+    return G <= (1ll << 30) && G - 1 == (long long)p->grid_hash_max;
+}
+
+// Context for N particles over G cells (all scenes), c->p / c->q already set.
+int create_ctx(zsph_ctx *c, int device, size_t N, size_t G, bool batch) {
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return ZSPH_E_NODEV;
-    zsph_ctx *c = new zsph_ctx();
-    c->p = *p;
-    c->q = zs::make_prm(*p);
     if (device >= 0) {
-        if (device >= ndev) { delete c; return ZSPH_E_NODEV; }
-        if (hipSetDevice(device) != hipSuccess) { delete c; return ZSPH_E_HIP; }
+        if (device >= ndev) return ZSPH_E_NODEV;
+        if (hipSetDevice(device) != hipSuccess) return ZSPH_E_HIP;
     }
     (void)hipGetDevice(&c->device);
-    int r = ZSPH_OK;
-    auto fail = [&](int code) {
-        zsph_destroy(c);
-        return code;
-    };
-    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) return fail(ZSPH_E_HIP);
-    const size_t N = (size_t)p->ptc_num;
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) return ZSPH_E_HIP;
+    int r;
     for (int b = 0; b < 2; ++b) {
         if ((r = dalloc(c, &c->buf[b].posr, N)) || (r = dalloc(c, &c->buf[b].velp, N)) ||
             (r = dalloc(c, &c->buf[b].tposr, N)) || (r = dalloc(c, &c->buf[b].tvelp, N)) ||
             (r = dalloc(c, &c->buf[b].type, N)) || (r = dalloc(c, &c->buf[b].table, N)))
-            return fail(r);
-        if ((r = dalloc(c, &c->key[b], N)) || (r = dalloc(c, &c->idx[b], N))) return fail(r);
+            return r;
+        if (batch && (r = dalloc(c, &c->buf[b].scene, N))) return r;
+        if ((r = dalloc(c, &c->key[b], N)) || (r = dalloc(c, &c->idx[b], N))) return r;
     }
     // sort keys: cell bits + the non-fluid flag bit above them
     int kb = 1;
@@ -273,13 +273,81 @@ int zsph_create(const zsph_params *p, int device, zsph_ctx **out) {
     // fewest passes of at most 11-bit digits, spread evenly (19 bits: 2 x 10)
     const int passes = (kb + 10) / 11;
     c->radix_bits = std::max(8, (kb + passes - 1) / passes);
-    if ((r = dalloc(c, &c->cell_start, 2 * ((size_t)G + 1))) || (r = dalloc(c, &c->acc, N)) ||
-        (r = dalloc(c, &c->gd, N)) || (r = dalloc(c, &c->hist, zs::radix_hist_words((int)N, 11))) ||
-        (r = dalloc(c, &c->killed, 1)) ||
+    if ((r = dalloc(c, &c->cell_start, 2 * (G + 1))) || (r = dalloc(c, &c->acc, N)) || (r = dalloc(c, &c->gd, N)) ||
+        (r = dalloc(c, &c->hist, zs::radix_hist_words((int)N, 11))) || (r = dalloc(c, &c->killed, 1)) ||
         (r = dalloc(c, &c->diag_part, (size_t)zs::diag_blocks((int)N) * zs::DIAG_D)) ||
         (r = dalloc(c, &c->diag_max, (size_t)zs::diag_blocks((int)N) * 2)) || (r = dalloc(c, &c->body, 1)) ||
         (r = dalloc(c, &c->body_part, (size_t)zs::BODY_BLOCKS * zs::BODY_D)))
-        return fail(r);
+        return r;
+    const size_t S = (size_t)c->nscenes;
+    if ((r = dalloc(c, &c->scene_part, S * zs::SCENE_D))) return r;
+    if (batch) {
+        if ((r = dalloc(c, &c->scene_prm, S))) return r;
+        HIPCHK(c, hipMemcpy(c->scene_prm, c->scene_host.data(), S * sizeof(zs::Prm), hipMemcpyHostToDevice));
+    }
+    return ZSPH_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int zsph_create(const zsph_params *p, int device, zsph_ctx **out) {
+    if (!p || !out) return ZSPH_E_ARG;
+    *out = nullptr;
+    if (!valid_params(p)) return ZSPH_E_ARG;
+    zsph_ctx *c = new zsph_ctx();
+    c->p = *p;
+    c->q = zs::make_prm(*p);
+    c->scene_off = {0, p->ptc_num};
+    const int r = create_ctx(c, device, (size_t)p->ptc_num, (size_t)c->q.G, false);
+    if (r) {
+        zsph_destroy(c);
+        return r;
+    }
+    *out = c;
+    return ZSPH_OK;
+}
+
+int zsph_create_batch(const zsph_params *scenes, int nscenes, int device, zsph_ctx **out) {
+    if (!scenes || !out || nscenes < 1) return ZSPH_E_ARG;
+    *out = nullptr;
+    long long N = 0, G = 0;
+    for (int k = 0; k < nscenes; ++k) {
+        if (!valid_params(&scenes[k])) return ZSPH_E_ARG;
+        N += scenes[k].ptc_num;
+        G += (long long)scenes[k].grid_xdim * scenes[k].grid_ydim * scenes[k].grid_zdim;
+    }
+    if (N > (1ll << 31) - 1 || G > (1ll << 30)) return ZSPH_E_ARG;
+    zsph_ctx *c = new zsph_ctx();
+    c->nscenes = nscenes;
+    int cb = 1;
+    while (cb < 30 && (1ll << cb) < G) ++cb;
+    const uint32_t flag = 1u << cb;
+    long long base = 0, off = 0;
+    c->scene_off.push_back(0);
+    for (int k = 0; k < nscenes; ++k) {
+        zs::Prm sq = zs::make_prm(scenes[k]);
+        const long long Gk = sq.G;
+        sq.G = (int)G;
+        sq.flag = flag;
+        sq.N = (int)N;
+        sq.cell_base = (int)base;
+        base += Gk;
+        off += scenes[k].ptc_num;
+        c->scene_host.push_back(sq);
+        c->scene_off.push_back((int)off);
+    }
+    // context-wide view: scene 0's constants with the batch's sizes
+    c->p = scenes[0];
+    c->p.ptc_num = (int)N;
+    c->q = c->scene_host[0];
+    c->q.cell_base = 0;
+    const int r = create_ctx(c, device, (size_t)N, (size_t)G, true);
+    if (r) {
+        zsph_destroy(c);
+        return r;
+    }
     *out = c;
     return ZSPH_OK;
 }
@@ -327,9 +395,17 @@ int zsph_upload(zsph_ctx *c, int b, const zsph_host_state *st) {
     }
     if (st->acc_drhodt)
         HIPCHK(c, hipMemcpyAsync(c->acc, st->acc_drhodt, N * sizeof(float4), hipMemcpyHostToDevice, c->stream));
+    std::vector<int> sid;
+    if (d.scene) {  // a batch: particles arrive scene by scene
+        sid.resize(N);
+        for (int k = 0; k < c->nscenes; ++k)
+            std::fill(sid.begin() + c->scene_off[k], sid.begin() + c->scene_off[k + 1], k);
+        HIPCHK(c, hipMemcpyAsync(d.scene, sid.data(), N * sizeof(int), hipMemcpyHostToDevice, c->stream));
+    }
     HIPCHK(c, hipStreamSynchronize(c->stream));
     c->hashed = 0;
     c->sorted_slot = -1;
+    c->cells_buf = -1;
     if (b == 0 && (c->has_body || c->has_piston)) {
         // new particles in buffer 0: the moving-dummy set-up no longer applies
         zs::DevBody hb;
@@ -446,6 +522,7 @@ int zsph_step(zsph_ctx *c, int nsteps) {
     }
     // After a step the sorted tables describe buffer 0.
     c->hashed = 0;
+    c->cells_buf = 0;
     c->sorted_slot = (c->keybits + c->radix_bits - 1) / c->radix_bits % 2;
     return ZSPH_OK;
 }
@@ -482,8 +559,9 @@ int zsph_autotune(zsph_ctx *c, int steps, zsph_tuning *best, float *best_ms) {
     if (N == 0) return ZSPH_OK;
     // save buffer 0 and the removal counter; every trial starts from them
     const zs::Buf &b0 = c->buf[0];
-    void *const arrays[6] = {b0.posr, b0.velp, b0.tposr, b0.tvelp, b0.type, b0.table};
-    const size_t bytes[6] = {N * 16, N * 16, N * 16, N * 16, N * 4, N * 4};
+    const int narr = b0.scene ? 7 : 6;
+    void *const arrays[7] = {b0.posr, b0.velp, b0.tposr, b0.tvelp, b0.type, b0.table, b0.scene};
+    const size_t bytes[7] = {N * 16, N * 16, N * 16, N * 16, N * 4, N * 4, b0.scene ? N * 4 : 0};
     size_t total = 256 + sizeof(zs::DevBody);  // + removal counter (padded) + body state
     for (size_t k : bytes) total += k;
     const double t_keep = c->t, t0_keep = c->t0;
@@ -492,7 +570,7 @@ int zsph_autotune(zsph_ctx *c, int steps, zsph_tuning *best, float *best_ms) {
     auto copy_all = [&](bool to_save) -> hipError_t {
         size_t off = 0;
         hipError_t e = hipSuccess;
-        for (int k = 0; k < 6 && e == hipSuccess; ++k) {
+        for (int k = 0; k < narr && e == hipSuccess; ++k) {
             e = to_save ? hipMemcpyAsync(save + off, arrays[k], bytes[k], hipMemcpyDeviceToDevice, c->stream)
                         : hipMemcpyAsync(arrays[k], save + off, bytes[k], hipMemcpyDeviceToDevice, c->stream);
             off += bytes[k];
@@ -543,6 +621,7 @@ int zsph_autotune(zsph_ctx *c, int steps, zsph_tuning *best, float *best_ms) {
     c->t0 = t0_keep;
     c->sorted_slot = -1;  // buffer 0 is back to its unsorted starting state
     c->hashed = 0;
+    c->cells_buf = -1;
     if (r) return r;
     if (e != hipSuccess) return set_err(c, ZSPH_E_HIP, "autotune restore: %s", hipGetErrorString(e));
     if (best) *best = keep;
@@ -609,7 +688,8 @@ int zsph_neighbor_stats(zsph_ctx *c, int b, int *count, uint64_t *digest) {
         (void)hipFree(dc);
         return set_err(c, ZSPH_E_NOMEM, "hipMalloc failed");
     }
-    zs::launch_nbr_stats(c->stream, c->walk, c->q, c->buf[b], c->key[c->sorted_slot], c->cell_start, dc, dd);
+    zs::launch_nbr_stats(c->stream, c->walk, c->q, c->scene_prm, c->buf[b], c->key[c->sorted_slot], c->cell_start,
+                         dc, dd);
     e = hipGetLastError();
     if (e == hipSuccess && N) e = hipMemcpyAsync(count, dc, N * 4, hipMemcpyDeviceToHost, c->stream);
     if (e == hipSuccess && N) e = hipMemcpyAsync(digest, dd, N * 8, hipMemcpyDeviceToHost, c->stream);
@@ -665,6 +745,22 @@ int zsph_profile_steps(zsph_ctx *c, int nsteps, float *stage_ms) {
 
 int zsph_diagnostics(zsph_ctx *c, int b, float front_zmax, zsph_diag *out) {
     if (!c || !out || !valid_buf(b)) return set_err(c, ZSPH_E_ARG, "zsph_diagnostics: bad argument");
+    if (c->nscenes > 1) {  // scenes differ in particle mass: sum the per-scene figures
+        std::vector<zsph_diag> per((size_t)c->nscenes);
+        int r = zsph_scene_diagnostics(c, b, front_zmax, per.data());
+        if (r) return r;
+        std::memset(out, 0, sizeof(*out));
+        out->front_x = out->zmax_fluid = -INFINITY;
+        for (const zsph_diag &d : per) {
+            out->n_fluid += d.n_fluid;
+            out->mass_fluid += d.mass_fluid;
+            out->kinetic += d.kinetic;
+            for (int k = 0; k < 3; ++k) out->momentum[k] += d.momentum[k];
+            out->front_x = std::max(out->front_x, d.front_x);
+            out->zmax_fluid = std::max(out->zmax_fluid, d.zmax_fluid);
+        }
+        return zsph_killed(c, &out->n_killed);
+    }
     std::memset(out, 0, sizeof(*out));
     out->front_x = out->zmax_fluid = -INFINITY;
     const int nb = c->q.N ? zs::diag_blocks(c->q.N) : 0;
@@ -696,9 +792,37 @@ int zsph_diagnostics(zsph_ctx *c, int b, float front_zmax, zsph_diag *out) {
     return ZSPH_OK;
 }
 
+int zsph_scene_diagnostics(zsph_ctx *c, int b, float front_zmax, zsph_diag *out) {
+    if (!c || !out || !valid_buf(b)) return set_err(c, ZSPH_E_ARG, "zsph_scene_diagnostics: bad argument");
+    if (c->cells_buf != b)
+        return set_err(c, ZSPH_E_STATE, "zsph_scene_diagnostics: buffer %d is not in cell order (step or sort first)", b);
+    const int S = c->nscenes;
+    zs::launch_scene_diag(c->stream, c->q, c->scene_prm, S, c->buf[b], c->cell_start, front_zmax, c->scene_part);
+    int r = check_launch(c, "scene diagnostics");
+    if (r) return r;
+    std::vector<double> part((size_t)S * zs::SCENE_D);
+    if ((r = copy_sync(c, part.data(), c->scene_part, part.size() * sizeof(double), hipMemcpyDeviceToHost))) return r;
+    for (int k = 0; k < S; ++k) {
+        const double *v = &part[(size_t)k * zs::SCENE_D];
+        const float m = S > 1 ? c->scene_host[k].m : c->q.m;
+        zsph_diag &d = out[k];
+        d.n_fluid = (int)v[0];
+        d.n_killed = (int)v[7];
+        d.mass_fluid = v[0] * (double)m;
+        d.kinetic = v[1];
+        d.momentum[0] = v[2];
+        d.momentum[1] = v[3];
+        d.momentum[2] = v[4];
+        d.front_x = v[5];
+        d.zmax_fluid = v[6];
+    }
+    return ZSPH_OK;
+}
+
 int zsph_probe_pressure(zsph_ctx *c, int b, const float *pts, int npts, float *p_out) {
     if (!c || !valid_buf(b) || npts < 0 || (npts && (!pts || !p_out)))
         return set_err(c, ZSPH_E_ARG, "zsph_probe_pressure: bad argument");
+    if (c->nscenes > 1) return set_err(c, ZSPH_E_ARG, "zsph_probe_pressure: one scene only (probe a batch's scenes by their own contexts)");
     if (npts == 0) return ZSPH_OK;
     float *d = nullptr;
     HIPCHK(c, hipMalloc(&d, sizeof(float) * 4 * (size_t)npts));
@@ -771,6 +895,7 @@ int store_arrays(zsph_ctx *c, const HostArrays &h, const zs::DevBody &hb) {
 int zsph_set_body(zsph_ctx *c, const zsph_rigid *r) {
     using namespace zs::rb;
     if (!c || !r) return set_err(c, ZSPH_E_ARG, "zsph_set_body: null argument");
+    if (c->nscenes > 1) return set_err(c, ZSPH_E_ARG, "zsph_set_body: one scene only");
     const double qn = std::sqrt(r->q[0] * r->q[0] + r->q[1] * r->q[1] + r->q[2] * r->q[2] + r->q[3] * r->q[3]);
     M3 I;
     for (int k = 0; k < 9; ++k) I.m[k] = r->I[k];
@@ -819,6 +944,7 @@ int zsph_get_body(zsph_ctx *c, zsph_rigid *r) {
 
 int zsph_set_piston(zsph_ctx *c, const zsph_piston *p) {
     if (!c || !p) return set_err(c, ZSPH_E_ARG, "zsph_set_piston: null argument");
+    if (c->nscenes > 1) return set_err(c, ZSPH_E_ARG, "zsph_set_piston: one scene only");
     if (p->axis < 0 || p->axis > 2 || !(p->frequency >= 0.0) || !std::isfinite(p->frequency) ||
         !std::isfinite(p->amplitude) || !std::isfinite(p->phase))
         return set_err(c, ZSPH_E_ARG, "zsph_set_piston: axis 0..2, finite amplitude/phase, frequency >= 0");
