@@ -5,6 +5,9 @@
 //   sort      <- single_gpu_sph_src/lib/sort.cu:3-6    (thrust::sort_by_key)
 //   cells +
 //   reorder   <- single_gpu_sph_src/lib/sort.cu:8-84   (cuda_sort_data)
+#include <algorithm>
+#include <climits>
+
 #include "common.hpp"
 #include "launch.hpp"
 
@@ -214,23 +217,120 @@ __global__ __launch_bounds__(ZS_BLOCK) void k_radix_scatter(const uint32_t *__re
 // ---------------------------------------------------------------------------
 // Cell table: entry e < G+1 is the first sorted position with key >= e (the
 // fluid particles of cell c are [cs[c], cs[c+1]), so cs[G] = fluid count);
-// entry G+1+c is the first position with key >= flag | c (the dummies of
+// entry G+1+c is the first position with key >= flag + c (the dummies of
 // cell c; cs[2G+1] = N).  Empty cells have equal bounds.  Every entry is
 // rewritten on every call.
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(ZS_BLOCK) void k_cell_start(const uint32_t *__restrict__ skey, int n, int G,
-                                                         uint32_t flag, int *__restrict__ cs) {
-    const int e = blockIdx.x * ZS_BLOCK + threadIdx.x;
-    if (e > 2 * G + 1) return;
-    // flag + c, not flag | c: c runs to G, and G == flag when G is a power of two
-    const uint32_t target = (e <= G) ? (uint32_t)e : (flag + (uint32_t)(e - G - 1));
-    int lo = 0, hi = n;
-    while (lo < hi) {
-        const int mid = (lo + hi) >> 1;
-        if (skey[mid] < target) lo = mid + 1;
-        else hi = mid;
+// Built in O(N + G) rather than a binary search per entry: every entry is
+// first set to n, the first particle of each key writes its index into the
+// key's entry, and a suffix-minimum scan then gives each empty cell the
+// start of the next occupied one (which is exactly its lower bound).  Entry
+// of a key: key for fluid (key < flag), G + 1 + (key - flag) otherwise.
+constexpr int CS_ITEMS = 16;
+constexpr int CS_TILE = ZS_BLOCK * CS_ITEMS;
+constexpr int CS_PAD = CS_ITEMS + 1;  // LDS row stride: 17 is coprime with the 64 banks
+
+__global__ __launch_bounds__(ZS_BLOCK) void k_cs_fill(int E, int n, int *__restrict__ cs) {
+    for (int e = blockIdx.x * ZS_BLOCK + threadIdx.x; e < E; e += gridDim.x * ZS_BLOCK) cs[e] = n;
+}
+
+__global__ __launch_bounds__(ZS_BLOCK) void k_cs_mark(const uint32_t *__restrict__ skey, int n, int G,
+                                                      uint32_t flag, int *__restrict__ cs) {
+    const int i = blockIdx.x * ZS_BLOCK + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t k = skey[i];
+    if (i > 0 && skey[i - 1] == k) return;
+    cs[k < flag ? (int)k : G + 1 + (int)(k - flag)] = i;
+}
+
+__device__ __forceinline__ int wave_min(int v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = min(v, __shfl_xor(v, o, 64));
+    return v;
+}
+
+// Minimum of each tile of CS_TILE entries.
+__global__ __launch_bounds__(ZS_BLOCK) void k_cs_tile_min(const int *__restrict__ cs, int E,
+                                                          int *__restrict__ tmin) {
+    __shared__ int wm[ZS_BLOCK / 64];
+    const int base = blockIdx.x * CS_TILE;
+    int m = INT_MAX;
+#pragma unroll
+    for (int j = 0; j < CS_ITEMS; ++j) {
+        const int e = base + j * ZS_BLOCK + (int)threadIdx.x;
+        if (e < E) m = min(m, cs[e]);
     }
-    cs[e] = lo;
+    m = wave_min(m);
+    if ((threadIdx.x & 63) == 0) wm[threadIdx.x >> 6] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int w = 1; w < ZS_BLOCK / 64; ++w) m = min(m, wm[w]);
+        tmin[blockIdx.x] = m;
+    }
+}
+
+// Inclusive suffix minimum of v over the block (LDS Hillis-Steele from the right).
+__device__ __forceinline__ int block_suffix_min(int v, int *s) {
+    s[threadIdx.x] = v;
+    __syncthreads();
+    for (int o = 1; o < ZS_BLOCK; o <<= 1) {
+        const int t = ((int)threadIdx.x + o < ZS_BLOCK) ? s[threadIdx.x + o] : INT_MAX;
+        __syncthreads();
+        v = min(v, t);
+        s[threadIdx.x] = v;
+        __syncthreads();
+    }
+    return v;
+}
+
+// One block: carry[t] = min(n, tmin[t+1..T-1]), chunks of ZS_BLOCK from the right.
+__global__ __launch_bounds__(ZS_BLOCK) void k_cs_carry(const int *__restrict__ tmin, int T, int n,
+                                                       int *__restrict__ carry) {
+    __shared__ int s[ZS_BLOCK];
+    int right = n;  // minimum of everything right of the current chunk
+    for (int hi = T; hi > 0; hi -= ZS_BLOCK) {
+        const int lo = max(0, hi - ZS_BLOCK);
+        const int t = lo + (int)threadIdx.x;
+        block_suffix_min(t < hi ? tmin[t] : INT_MAX, s);
+        const int next = ((int)threadIdx.x + 1 < ZS_BLOCK) ? s[threadIdx.x + 1] : INT_MAX;
+        if (t < hi) carry[t] = min(right, next);
+        right = min(right, s[0]);
+        __syncthreads();
+    }
+}
+
+// Suffix minimum inside each tile, seeded with the tile's carry; in place.
+__global__ __launch_bounds__(ZS_BLOCK) void k_cs_apply(int *__restrict__ cs, int E,
+                                                       const int *__restrict__ carry) {
+    __shared__ int tile[ZS_BLOCK * CS_PAD];
+    __shared__ int s[ZS_BLOCK];
+    const int base = blockIdx.x * CS_TILE;
+    // coalesced load; tile-local entry l belongs to thread l / 16, item l % 16
+#pragma unroll
+    for (int j = 0; j < CS_ITEMS; ++j) {
+        const int l = j * ZS_BLOCK + (int)threadIdx.x;
+        const int e = base + l;
+        tile[(l / CS_ITEMS) * CS_PAD + l % CS_ITEMS] = (e < E) ? cs[e] : INT_MAX;
+    }
+    __syncthreads();
+    int *mine = tile + threadIdx.x * CS_PAD;
+    int m = INT_MAX;
+#pragma unroll
+    for (int j = 0; j < CS_ITEMS; ++j) m = min(m, mine[j]);
+    block_suffix_min(m, s);
+    int run = min(carry[blockIdx.x], ((int)threadIdx.x + 1 < ZS_BLOCK) ? s[threadIdx.x + 1] : INT_MAX);
+#pragma unroll
+    for (int j = CS_ITEMS - 1; j >= 0; --j) {
+        run = min(run, mine[j]);
+        mine[j] = run;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < CS_ITEMS; ++j) {
+        const int l = j * ZS_BLOCK + (int)threadIdx.x;
+        const int e = base + l;
+        if (e < E) cs[e] = tile[(l / CS_ITEMS) * CS_PAD + l % CS_ITEMS];
+    }
 }
 
 // Gather buffer `a` into cell order in buffer `b`; table'[k] = table[p] + p - k
@@ -294,8 +394,17 @@ int launch_sort(hipStream_t s, int n, int keybits, int bits, uint32_t *key[2], i
     }
 }
 
-void launch_cell_start(hipStream_t s, const uint32_t *skey, int n, int G, uint32_t flag, int *cs) {
-    k_cell_start<<<(2 * G + 2 + ZS_BLOCK - 1) / ZS_BLOCK, ZS_BLOCK, 0, s>>>(skey, n, G, flag, cs);
+static int cs_tiles(int G) { return (int)((2 * (size_t)G + 2 + CS_TILE - 1) / CS_TILE); }
+size_t cell_start_scratch(int G) { return 2 * (size_t)cs_tiles(G); }
+
+void launch_cell_start(hipStream_t s, const uint32_t *skey, int n, int G, uint32_t flag, int *cs, int *scratch) {
+    const int E = 2 * G + 2;
+    const int T = cs_tiles(G);
+    k_cs_fill<<<std::min((E + ZS_BLOCK - 1) / ZS_BLOCK, 4096), ZS_BLOCK, 0, s>>>(E, n, cs);
+    if (n > 0) k_cs_mark<<<nblocks(n), ZS_BLOCK, 0, s>>>(skey, n, G, flag, cs);
+    k_cs_tile_min<<<T, ZS_BLOCK, 0, s>>>(cs, E, scratch);
+    k_cs_carry<<<1, ZS_BLOCK, 0, s>>>(scratch, T, n, scratch + T);
+    k_cs_apply<<<T, ZS_BLOCK, 0, s>>>(cs, E, scratch + T);
 }
 
 void launch_reorder(hipStream_t s, int n, const int *perm, const Buf &a, const Buf &b) {
