@@ -417,6 +417,56 @@ typedef struct zsph_wavetank {
 } zsph_wavetank;
 int zsph_case_wavetank(zsph_wavetank *c, zsph_json *j, zsph_host_state *st, int *n_out);
 
+/* ------------------------------------------------------------------------ */
+/* Learned closure (no counterpart in the reference): a per-particle MLP    */
+/* (ZSPH_NF features -> 64 -> 64 -> 64, SiLU -> 2) whose outputs scale the  */
+/* density-diffusion and viscous terms, run on MFMA in BF16 with FP32       */
+/* accumulation, trained by the library's own backward kernels.             */
+/* ------------------------------------------------------------------------ */
+#define ZSPH_NF 24  /* features per fluid particle                          */
+#define ZSPH_NH 64  /* hidden width                                         */
+#define ZSPH_NT 12  /* training record: cont, diff, t_rho, pa[3], visc[3], t_a[3] */
+#define ZSPH_NPARAM (ZSPH_NF * ZSPH_NH + ZSPH_NH + 2 * (ZSPH_NH * ZSPH_NH + ZSPH_NH) + ZSPH_NH * 2 + 2)
+
+/* Network and its input statistics.  Weights are [in][out] row-major; the
+ * arrays w0 .. b3 are contiguous (ZSPH_NPARAM floats, the gradient layout).
+ * Input x is normalised as (x - mean) * inv_std; a particle with any
+ * normalised feature outside [lo, hi] (the training range) keeps the plain
+ * terms.  Outputs o give the scale factors s = 1 + out_scale tanh(o). */
+typedef struct zsph_closure {
+    float w0[ZSPH_NF * ZSPH_NH], b0[ZSPH_NH];
+    float w1[ZSPH_NH * ZSPH_NH], b1[ZSPH_NH];
+    float w2[ZSPH_NH * ZSPH_NH], b2[ZSPH_NH];
+    float w3[ZSPH_NH * 2], b3[2];
+    float mean[ZSPH_NF], inv_std[ZSPH_NF];
+    float lo[ZSPH_NF], hi[ZSPH_NF];
+    float out_scale;
+} zsph_closure;
+
+/* A device-resident network with a training set, for training and checks.
+ * The training loss over samples p is
+ *   L = mean_p [ w_rho (cont + s_d diff - t_rho)^2 + w_acc |pa + s_v visc - t_a|^2 ]
+ * with the record of sample p = (cont, diff, t_rho, pa[3], visc[3], t_a[3]):
+ * the continuity and density-diffusion parts of drho/dt, its target, the
+ * pressure + gravity and viscous parts of the acceleration, its target. */
+typedef struct zsph_mlp zsph_mlp;
+int zsph_mlp_create(int device, zsph_mlp **out);
+void zsph_mlp_destroy(zsph_mlp *m);
+const char *zsph_mlp_last_error(zsph_mlp *m);
+/* Load the network (resets the optimiser state). */
+int zsph_mlp_set(zsph_mlp *m, const zsph_closure *net);
+int zsph_mlp_get(zsph_mlp *m, zsph_closure *net);
+/* Training set: x = n x ZSPH_NF raw features, rec = n x ZSPH_NT records. */
+int zsph_mlp_data(zsph_mlp *m, const float *x, const float *rec, int n, float w_rho, float w_acc);
+/* Mean loss and its gradient (ZSPH_NPARAM floats, may be NULL) over
+ * samples [first, first + count) of the training set. */
+int zsph_mlp_eval(zsph_mlp *m, int first, int count, float *loss, float *grad);
+/* One Adam step on samples [first, first + count); *loss (may be NULL) is
+ * the loss before the step. */
+int zsph_mlp_adam(zsph_mlp *m, int first, int count, float lr, float beta1, float beta2, float eps, float *loss);
+/* Scale factors (n x 2) of n feature rows, with the range fallback. */
+int zsph_mlp_infer(zsph_mlp *m, const float *x, int n, float *s_out);
+
 /* Launch tuning (no counterpart in the reference, which launches 256-thread
  * blocks, deltasph/main.cu:55-56).  Results do not depend on it: both
  * neighbour walks visit neighbours in the same order, and the stable sort
@@ -442,7 +492,7 @@ int zsph_killed(zsph_ctx *ctx, int *killed);
 
 /* sizeof of the ABI structs, so bindings can check their mirrors:
  * 0 zsph_json, 1 zsph_cpu_param, 2 zsph_params, 3 zsph_host_state, 4 zsph_dambreak,
- * 5 zsph_rigid, 6 zsph_diag, 7 zsph_tuning, 8 zsph_piston, 9 zsph_wavetank. */
+ * 5 zsph_rigid, 6 zsph_diag, 7 zsph_tuning, 8 zsph_piston, 9 zsph_wavetank, 10 zsph_closure. */
 size_t zsph_sizeof(int which);
 
 #ifdef __cplusplus

@@ -56,6 +56,10 @@ def lib():
         sig("orc_get_time", C.c_int, [vp, P(C.c_double)])
         sig("orc_body_reduce", C.c_int, [vp, C.c_int, P(C.c_double), P(C.c_double)])
         sig("orc_rigid_acc_alpha", None, [P(abi.Rigid), C.c_double])
+        sig("orc_mlp_loss_grad", C.c_double, [P(abi.Closure), C.c_int, P(C.c_float), P(C.c_float), C.c_int,
+                                              C.c_double, C.c_double, P(C.c_double)])
+        sig("orc_mlp_infer", None, [P(abi.Closure), C.c_int, P(C.c_float), C.c_int, P(C.c_double)])
+        sig("orc_bf16_round", C.c_double, [C.c_double])
         sig("orc_rigid_advance", None, [P(abi.Rigid), P(abi.Rigid), C.c_double, P(abi.Rigid)])
         _LIB = L
     return _LIB
@@ -200,3 +204,31 @@ def rigid_advance(s0, rates, dt):
     out = abi.Rigid()
     lib().orc_rigid_advance(C.byref(s0), C.byref(rates), dt, C.byref(out))
     return out
+
+
+def mlp_loss_grad(net, x, rec, w_rho=1.0, w_acc=1.0, emulate_bf16=False, grad=True):
+    """(mean loss, gradient or None) of the closure network `net` (abi.Closure) in FP64."""
+    x = np.ascontiguousarray(x, np.float32)
+    rec = np.ascontiguousarray(rec, np.float32)
+    g = np.zeros(net_nparam(), np.float64) if grad else None
+    loss = lib().orc_mlp_loss_grad(C.byref(net), 1 if emulate_bf16 else 0, x.ctypes.data_as(C.POINTER(C.c_float)),
+                                   rec.ctypes.data_as(C.POINTER(C.c_float)), len(x), w_rho, w_acc,
+                                   g.ctypes.data_as(C.POINTER(C.c_double)) if grad else None)
+    return loss, g
+
+
+def mlp_infer(net, x, emulate_bf16=False):
+    x = np.ascontiguousarray(x, np.float32)
+    out = np.zeros((len(x), 2), np.float64)
+    lib().orc_mlp_infer(C.byref(net), 1 if emulate_bf16 else 0, x.ctypes.data_as(C.POINTER(C.c_float)), len(x),
+                        out.ctypes.data_as(C.POINTER(C.c_double)))
+    return out
+
+
+def bf16_round(v):
+    return lib().orc_bf16_round(float(v))
+
+
+def net_nparam():
+    from zsph_amd import _abi as abi
+    return abi.NPARAM

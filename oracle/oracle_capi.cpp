@@ -6,6 +6,7 @@
 #include <vector>
 
 #include "zsph.h"
+#include "mlp_oracle.hpp"
 #include "zsph_oracle.hpp"
 
 namespace {
@@ -177,6 +178,17 @@ void orc_rigid_acc_alpha(zsph_rigid *r, double g) { orc::Oracle<double>::body_ac
 void orc_rigid_advance(const zsph_rigid *s0, const zsph_rigid *rates, double dt, zsph_rigid *out) {
     orc::Oracle<double>::body_advance(*s0, *rates, dt, *out);
 }
+
+// Closure network (emulate_bf16: the device's BF16 roundings).
+double orc_mlp_loss_grad(const zsph_closure *net, int emulate_bf16, const float *x, const float *rec, int n,
+                         double w_rho, double w_acc, double *grad) {
+    return orc::MlpOracle(*net, emulate_bf16 != 0).loss_grad(x, rec, n, w_rho, w_acc, grad);
+}
+void orc_mlp_infer(const zsph_closure *net, int emulate_bf16, const float *x, int n, double *s) {
+    const orc::MlpOracle m(*net, emulate_bf16 != 0);
+    for (int p = 0; p < n; ++p) m.infer(x + (size_t)p * ZSPH_NF, s + 2 * (size_t)p);
+}
+double orc_bf16_round(double x) { return orc::bf16_round(x); }
 
 // Scalar kernels of the oracle, exposed for closed-form tests (a particle-
 // free oracle carries only the constants).

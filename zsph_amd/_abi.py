@@ -94,6 +94,20 @@ class Tuning(C.Structure):
 
 WALK_MASK, WALK_QUEUE, WALK_LIST = 0, 1, 2
 
+NF, NH, NT = 24, 64, 12
+NPARAM = NF * NH + NH + 2 * (NH * NH + NH) + NH * 2 + 2
+
+
+class Closure(C.Structure):
+    _fields_ = [("w0", _f * (NF * NH)), ("b0", _f * NH), ("w1", _f * (NH * NH)), ("b1", _f * NH),
+                ("w2", _f * (NH * NH)), ("b2", _f * NH), ("w3", _f * (NH * 2)), ("b3", _f * 2),
+                ("mean", _f * NF), ("inv_std", _f * NF), ("lo", _f * NF), ("hi", _f * NF), ("out_scale", _f)]
+
+    def params(self):
+        """The ZSPH_NPARAM parameters (w0 .. b3) as a writable numpy view."""
+        import numpy as np
+        return np.frombuffer(self, dtype=np.float32, count=NPARAM)
+
 
 class DamBreak(C.Structure):
     _fields_ = [("dim", _i), ("dx", _f), ("lx", _f), ("wy", _f), ("lz", _f), ("cw", _f), ("ch", _f),
@@ -126,7 +140,7 @@ def load(path):
     vp = C.c_void_p
     _sig(lib, "zsph_sizeof", C.c_size_t, [_i])
     for which, cls in enumerate((Json, CpuParam, Params, HostState, DamBreak, Rigid, Diag, Tuning, Piston,
-                                 WaveTank)):
+                                 WaveTank, Closure)):
         got = lib.zsph_sizeof(which)
         if got != C.sizeof(cls):
             raise RuntimeError(f"ABI mismatch: sizeof({cls.__name__}) = {C.sizeof(cls)} here, {got} in {path}")
@@ -172,6 +186,16 @@ def load(path):
     _sig(lib, "zsph_case_wavetank", _i, [P(WaveTank), P(Json), P(HostState), P(_i)])
     _sig(lib, "zsph_create_batch", _i, [P(Params), _i, _i, P(vp)])
     _sig(lib, "zsph_scene_diagnostics", _i, [vp, _i, _f, P(Diag)])
+    Pf = P(_f)
+    _sig(lib, "zsph_mlp_create", _i, [_i, P(vp)])
+    _sig(lib, "zsph_mlp_destroy", None, [vp])
+    _sig(lib, "zsph_mlp_last_error", C.c_char_p, [vp])
+    _sig(lib, "zsph_mlp_set", _i, [vp, P(Closure)])
+    _sig(lib, "zsph_mlp_get", _i, [vp, P(Closure)])
+    _sig(lib, "zsph_mlp_data", _i, [vp, Pf, Pf, _i, _f, _f])
+    _sig(lib, "zsph_mlp_eval", _i, [vp, _i, _i, Pf, Pf])
+    _sig(lib, "zsph_mlp_adam", _i, [vp, _i, _i, _f, _f, _f, _f, Pf])
+    _sig(lib, "zsph_mlp_infer", _i, [vp, Pf, _i, Pf])
     return lib
 
 
@@ -186,7 +210,8 @@ EXPORTS = (
     "zsph_probe_pressure", "zsph_rigid_mass_properties", "zsph_rigid_acc_alpha", "zsph_rigid_euler_step",
     "zsph_set_tuning", "zsph_get_tuning", "zsph_autotune", "zsph_set_body", "zsph_get_body",
     "zsph_set_piston", "zsph_set_time", "zsph_get_time", "zsph_body_reduce", "zsph_case_wavetank",
-    "zsph_create_batch", "zsph_scene_diagnostics",
+    "zsph_create_batch", "zsph_scene_diagnostics", "zsph_mlp_create", "zsph_mlp_destroy", "zsph_mlp_last_error",
+    "zsph_mlp_set", "zsph_mlp_get", "zsph_mlp_data", "zsph_mlp_eval", "zsph_mlp_adam", "zsph_mlp_infer",
 )
 
 STAGES = ("hash", "sort", "cells", "reorder", "wall", "ldg", "governing", "integrate")

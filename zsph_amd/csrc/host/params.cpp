@@ -178,6 +178,7 @@ extern "C" size_t zsph_sizeof(int which) {
     case 7: return sizeof(zsph_tuning);
     case 8: return sizeof(zsph_piston);
     case 9: return sizeof(zsph_wavetank);
+    case 10: return sizeof(zsph_closure);
     default: return 0;
     }
 }

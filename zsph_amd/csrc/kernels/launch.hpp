@@ -65,4 +65,28 @@ void launch_body_reduce(hipStream_t s, const Prm &q, const Buf &b, const int *cs
 void launch_body_update(hipStream_t s, const Prm &q, DevBody *body, const double *part, bool predictor);
 void launch_kinematics(hipStream_t s, const Prm &q, const Buf &b, const int *cs, const DevBody *body);
 
+// mlp.hip: closure network 24 -> 64 -> 64 -> 64 -> 2 on MFMA (BF16, FP32 accumulate).
+constexpr int MLP_NF = ZSPH_NF;          // features
+constexpr int MLP_NH = ZSPH_NH;          // hidden width
+constexpr int MLP_NO = 2;                // outputs: scale of the density diffusion, of the viscous term
+constexpr int MLP_NT = ZSPH_NT;          // training record per sample
+constexpr int MLP_NPARAM = MLP_NF * MLP_NH + MLP_NH + 2 * (MLP_NH * MLP_NH + MLP_NH) + MLP_NH * MLP_NO + MLP_NO;
+struct MlpAux {
+    float mean[MLP_NF], inv_std[MLP_NF], lo[MLP_NF], hi[MLP_NF];
+    float out_scale, w_rho, w_acc;
+};
+size_t mlp_pack_elems();          // BF16 elements of the packed weight image
+int mlp_blocks(int n);            // blocks of a training launch over n samples (partials buffer size)
+hipError_t mlp_prepare();         // allows the kernels' dynamic LDS (call once per device)
+void launch_mlp_pack(hipStream_t s, const float *P, void *pack);
+// n samples, or *n_dev when n_dev is given; out: n x 2 scale factors.
+void launch_mlp_infer(hipStream_t s, const float *P, const void *pack, const MlpAux *aux, const float *feat, int n,
+                      const int *n_dev, float2 *out, int nblk);
+// grad[0..MLP_NPARAM) = mean gradient over the n samples, grad[MLP_NPARAM] = mean loss;
+// part: mlp_blocks(n) x (MLP_NPARAM + 1) floats.
+void launch_mlp_grad(hipStream_t s, const float *P, const void *pack, const MlpAux *aux, const float *feat,
+                     const float *rec, int n, float *part, float *grad);
+void launch_mlp_adam(hipStream_t s, float *P, float *m, float *v, const float *g, float lr, float b1, float b2,
+                     float eps, int t);
+
 }  // namespace zs
