@@ -467,6 +467,21 @@ int zsph_mlp_adam(zsph_mlp *m, int first, int count, float lr, float beta1, floa
 /* Scale factors (n x 2) of n feature rows, with the range fallback. */
 int zsph_mlp_infer(zsph_mlp *m, const float *x, int n, float *s_out);
 
+/* Use the network in the step of a single-scene context (NULL: plain
+ * terms again).  Each half step then computes the features of every fluid
+ * particle in the gradient pass, runs the network on them, and scales the
+ * density-diffusion part of drho/dt by s_d and the viscous part of the
+ * acceleration (bulk and artificial viscosity) by s_v, particle by
+ * particle; out-of-range particles keep the plain terms. */
+int zsph_set_closure(zsph_ctx *ctx, const zsph_closure *net);
+
+/* Training data of the state in buffer `buf` (in cell order): runs the wall,
+ * gradient and continuity+momentum passes (plain terms) and returns, per
+ * particle in the buffer's order, the ZSPH_NF features (features[N][NF]) and
+ * the term decomposition terms[N][8] = (continuity, density diffusion,
+ * pressure + gravity (3), viscous (3)); dummy rows are zero. */
+int zsph_closure_sample(zsph_ctx *ctx, int buf, float *features, float *terms);
+
 /* Launch tuning (no counterpart in the reference, which launches 256-thread
  * blocks, deltasph/main.cu:55-56).  Results do not depend on it: both
  * neighbour walks visit neighbours in the same order, and the stable sort

@@ -15,6 +15,9 @@ _LIB = None
 
 A_ACC_SCALE = 100   # per particle (sum |acc terms|, sum |drho terms|)
 A_GD_SCALE = 101    # per particle (sum |grad rho terms|, sum |div v terms|)
+A_FEATURES = 102    # per particle, closure features (after features() or a closure governing pass)
+A_TERMS = 103       # per particle: cont, diff, pressure + g (3), viscous (3) of the last governing pass
+A_SCALE = 104       # per particle closure scale factors (empty without the closure)
 
 
 def lib():
@@ -56,6 +59,8 @@ def lib():
         sig("orc_get_time", C.c_int, [vp, P(C.c_double)])
         sig("orc_body_reduce", C.c_int, [vp, C.c_int, P(C.c_double), P(C.c_double)])
         sig("orc_rigid_acc_alpha", None, [P(abi.Rigid), C.c_double])
+        sig("orc_set_closure", C.c_int, [vp, P(abi.Closure), C.c_int])
+        sig("orc_features", C.c_int, [vp, C.c_int])
         sig("orc_mlp_loss_grad", C.c_double, [P(abi.Closure), C.c_int, P(C.c_float), P(C.c_float), C.c_int,
                                               C.c_double, C.c_double, P(C.c_double)])
         sig("orc_mlp_infer", None, [P(abi.Closure), C.c_int, P(C.c_float), C.c_int, P(C.c_double)])
@@ -121,8 +126,12 @@ class Oracle:
         if which == A_CELL_START:
             count = 2 * (self.grid_num + 1)
             shape = (count,)
-        elif which in (A_ACC_SCALE, A_GD_SCALE):
+        elif which in (A_ACC_SCALE, A_GD_SCALE, A_SCALE):
             count, shape = 2 * self.n, (self.n, 2)
+        elif which == A_FEATURES:
+            count, shape = 24 * self.n, (self.n, 24)
+        elif which == A_TERMS:
+            count, shape = 8 * self.n, (self.n, 8)
         elif which in (0, 1, 2, 3, 9, 10):
             count, shape = 4 * self.n, (self.n, 4)
         else:
@@ -156,6 +165,15 @@ class Oracle:
         self._ok(lib().orc_neighbor_stats(self._h, buf, 1 if brute else 0, cnt.ctypes.data_as(C.POINTER(C.c_int)),
                                           dig.ctypes.data_as(C.POINTER(C.c_uint64))), "neighbor_stats")
         return cnt, dig
+
+    # ---- closure ---------------------------------------------------------------
+    def set_closure(self, net, emulate_bf16=True):
+        self._ok(lib().orc_set_closure(self._h, C.byref(net) if net is not None else None,
+                                       1 if emulate_bf16 else 0), "set_closure")
+
+    def features(self, buf=1):
+        self._ok(lib().orc_features(self._h, buf), "features")
+        return self.get(A_FEATURES, buf)
 
     # ---- moving dummies (buffer 0) -------------------------------------------
     def set_body(self, rigid):

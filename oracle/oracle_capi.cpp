@@ -87,6 +87,9 @@ int array_get(orc::Oracle<R> &o, int which, int bi, double *out, size_t count) {
     case ZSPH_A_GD: return copy(o.gd);
     case 100: return copy(o.acc_scale);
     case 101: return copy(o.gd_scale);
+    case 102: return copy(o.feat);
+    case 103: return copy(o.terms);
+    case 104: return copy(o.scale);
     default: return ZSPH_E_ARG;
     }
 }
@@ -178,6 +181,12 @@ void orc_rigid_acc_alpha(zsph_rigid *r, double g) { orc::Oracle<double>::body_ac
 void orc_rigid_advance(const zsph_rigid *s0, const zsph_rigid *rates, double dt, zsph_rigid *out) {
     orc::Oracle<double>::body_advance(*s0, *rates, dt, *out);
 }
+
+// Closure in the oracle's step (NULL: plain); features of a buffer after ldg.
+int orc_set_closure(void *h, const zsph_closure *net, int emulate_bf16) {
+    return ORC_CALL(o.closure_on = net != nullptr; if (net) o.closure = *net; o.closure_bf16 = emulate_bf16 != 0);
+}
+int orc_features(void *h, int b) { return ORC_CALL(o.features(b)); }
 
 // Closure network (emulate_bf16: the device's BF16 roundings).
 double orc_mlp_loss_grad(const zsph_closure *net, int emulate_bf16, const float *x, const float *rec, int n,

@@ -31,6 +31,7 @@
 #include <numeric>
 #include <vector>
 
+#include "mlp_oracle.hpp"
 #include "zsph.h"
 
 namespace orc {
@@ -69,6 +70,12 @@ struct Oracle {
     std::vector<int> cs;        // G+1
     std::vector<R> acc, gd;     // 4 per particle
     std::vector<R> acc_scale, gd_scale;  // sum |term| per particle: (acc, drho), (G, D)
+    std::vector<double> terms;           // 8 per particle: cont, diff, pressure + g (3), viscous (3)
+    std::vector<double> feat;            // ZSPH_NF per particle (closure features)
+    std::vector<double> scale;           // 2 per particle: closure scale factors (empty: plain)
+    bool closure_on = false;
+    bool closure_bf16 = true;
+    zsph_closure closure{};
     long long killed = 0;
     bool hashed = false, sorted = false;
 
@@ -82,6 +89,8 @@ struct Oracle {
         key.assign(N, 0); idx.assign(N, 0); cs.assign(2 * ((size_t)G + 1), 0);
         acc.assign(4 * (size_t)N, R(0)); gd.assign(4 * (size_t)N, R(0));
         acc_scale.assign(2 * (size_t)N, R(0)); gd_scale.assign(2 * (size_t)N, R(0));
+        terms.assign(8 * (size_t)N, 0.0);
+        feat.assign((size_t)ZSPH_NF * N, 0.0);
     }
 
     // ---- constants in the working precision --------------------------------
@@ -307,8 +316,101 @@ struct Oracle {
         }
     }
 
-    // delta_ns.cu:179-267 / simple_ns.cu:99-180.
+    // Closure features of every fluid particle of buffer bi (after ldg): the
+    // same definitions as the device's gradient pass, in the working precision.
+    void features(int bi) {
+        const Buffer<R> &b = buf[bi];
+        const R m = R(p.m), h = R(p.h), rho0 = R(p.rho0), ic0 = R(1) / std::sqrt(R(p.cs2));
+        const int dim = p.dim;
+#pragma omp parallel for schedule(static)
+        for (int i = 0; i < N; ++i) {
+            double *f = &feat[(size_t)ZSPH_NF * i];
+            for (int k = 0; k < ZSPH_NF; ++k) f[k] = 0.0;
+            if (b.type[i] != ZSPH_FLUID) continue;
+            R L[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}}, K[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+            R ws = 0, wv2 = 0, wdr = 0;
+            int nnb = 0, ndum = 0;
+            const R rhoi = b.posr[4 * i + 3];
+            walk(b, i, [&](int j) {
+                if (j == i || !is_nbr(b, i, j)) return;
+                const R x[3] = {b.posr[4 * i] - b.posr[4 * j], b.posr[4 * i + 1] - b.posr[4 * j + 1],
+                                b.posr[4 * i + 2] - b.posr[4 * j + 2]};
+                const R r = std::sqrt(x[0] * x[0] + x[1] * x[1] + x[2] * x[2]);
+                const R rhoj = b.posr[4 * j + 3];
+                const R FV = kernel_F(r) * m / rhoj, WV = kernel_W(r) * m / rhoj;
+                for (int a = 0; a < 3; ++a)
+                    for (int c = 0; c < 3; ++c) L[a][c] -= FV * x[a] * x[c];
+                ws += WV;
+                wdr += WV * (rhoj - rhoi);
+                ++nnb;
+                if (b.type[j] == ZSPH_FLUID) {
+                    R u[3], u2 = 0;
+                    for (int a = 0; a < 3; ++a) { u[a] = b.velp[4 * j + a] - b.velp[4 * i + a]; u2 += u[a] * u[a]; }
+                    for (int a = 0; a < 3; ++a)
+                        for (int c = 0; c < 3; ++c) K[a][c] += FV * u[a] * x[c];
+                    wv2 += WV * u2;
+                } else {
+                    ++ndum;
+                }
+            });
+            const R trL = L[0][0] + L[1][1] + L[2][2];
+            if (dim == 2) L[1][1] += R(1);
+            const R det = L[0][0] * (L[1][1] * L[2][2] - L[1][2] * L[2][1]) -
+                          L[0][1] * (L[1][0] * L[2][2] - L[1][2] * L[2][0]) +
+                          L[0][2] * (L[1][0] * L[2][1] - L[1][1] * L[2][0]);
+            const R *G = &gd[4 * i];
+            const R D = gd[4 * i + 3];
+            const R hc = h * ic0, hr = h / rho0;
+            const R w[3] = {K[2][1] - K[1][2], K[0][2] - K[2][0], K[1][0] - K[0][1]};
+            const R third = (K[0][0] + K[1][1] + K[2][2]) / R(dim);
+            const R S00 = K[0][0] - third, S11 = dim == 3 ? K[1][1] - third : R(0), S22 = K[2][2] - third;
+            const R S01 = R(0.5) * (K[0][1] + K[1][0]), S02 = R(0.5) * (K[0][2] + K[2][0]),
+                    S12 = R(0.5) * (K[1][2] + K[2][1]);
+            const R *v = &b.velp[4 * i];
+            const R v2 = v[0] * v[0] + v[1] * v[1] + v[2] * v[2];
+            f[0] = double(R(1) - (ws + R(p.adh) * m / rhoi));
+            f[1] = double(trL / R(dim));
+            f[2] = double(det);
+            f[3] = double(std::sqrt(L[0][1] * L[0][1] + L[0][2] * L[0][2] + L[1][2] * L[1][2]));
+            for (int a = 0; a < 3; ++a) f[4 + a] = double(G[a] * hr);
+            f[7] = double(std::sqrt(G[0] * G[0] + G[1] * G[1] + G[2] * G[2]) * hr);
+            f[8] = double(D * hc);
+            for (int a = 0; a < 3; ++a) f[9 + a] = double(w[a] * hc);
+            f[12] = double(std::sqrt(w[0] * w[0] + w[1] * w[1] + w[2] * w[2]) * hc);
+            f[13] = double(std::sqrt(S00 * S00 + S11 * S11 + S22 * S22 + R(2) * (S01 * S01 + S02 * S02 + S12 * S12)) * hc);
+            for (int a = 0; a < 3; ++a) f[14 + a] = double(v[a] * ic0);
+            f[17] = double(std::sqrt(v2) * ic0);
+            f[18] = double((rhoi - rho0) / rho0);
+            f[19] = double(v[3] / (rho0 * R(p.cs2)));
+            f[20] = nnb;
+            f[21] = ws > 0 ? double(std::sqrt(wv2 / ws) * ic0) : 0.0;
+            f[22] = ws > 0 ? double(wdr / (ws * rho0)) : 0.0;
+            f[23] = nnb ? double(ndum) / nnb : 0.0;
+        }
+    }
+
+    // Closure scale factors of buffer bi's fluid particles (features, then the network).
+    void closure_scales(int bi) {
+        features(bi);
+        const MlpOracle net(closure, closure_bf16);
+        scale.assign(2 * (size_t)N, 1.0);
+        const Buffer<R> &b = buf[bi];
+#pragma omp parallel for schedule(static)
+        for (int i = 0; i < N; ++i) {
+            if (b.type[i] != ZSPH_FLUID) continue;
+            float x[ZSPH_NF];
+            for (int k = 0; k < ZSPH_NF; ++k) x[k] = (float)feat[(size_t)ZSPH_NF * i + k];
+            net.infer(x, &scale[2 * (size_t)i]);
+        }
+    }
+
+    // delta_ns.cu:179-267 / simple_ns.cu:99-180.  The continuity, density-
+    // diffusion, pressure (+ gravity) and viscous parts are kept apart in
+    // `terms`; with the closure on, the diffusion and viscous parts are scaled
+    // by the network's factors.
     void governing(int bi) {
+        if (closure_on) closure_scales(bi);
+        else scale.clear();
         const Buffer<R> &b = buf[bi];
         const bool delta = p.diffusion == ZSPH_DIFF_ANTUONO;
         const R m = R(p.m), eta2 = R(p.eta2), dhc = R(p.delta_h_cs), hcrm = R(p.h_cs_rho_m), al = R(p.alpha);
@@ -316,11 +418,12 @@ struct Oracle {
         for (int i = 0; i < N; ++i) {
             const int ti = b.type[i];
             for (int c = 0; c < 4; ++c) acc[4 * i + c] = 0;
+            for (int c = 0; c < 8; ++c) terms[8 * (size_t)i + c] = 0.0;
             acc_scale[2 * i] = acc_scale[2 * i + 1] = 0;
             if (ti != ZSPH_FLUID && ti != ZSPH_BODY && ti != ZSPH_PISTON) continue;
             const bool fi = ti == ZSPH_FLUID;
             const R rhoi = b.posr[4 * i + 3], pi = b.velp[4 * i + 3];
-            R a[3] = {0, 0, 0}, dr = 0, as = 0, drs = 0;
+            R ap[3] = {0, 0, 0}, av[3] = {0, 0, 0}, cont = 0, diff = 0, as = 0, drs = 0;
             walk(b, i, [&](int j) {
                 if (j == i) return;
                 const bool fj = b.type[j] == ZSPH_FLUID;
@@ -334,10 +437,11 @@ struct Oracle {
                 const R rhoj = b.posr[4 * j + 3], pj = b.velp[4 * j + 3];
                 R dvdx = 0;
                 for (int c = 0; c < 3; ++c) dvdx += (b.velp[4 * i + c] - b.velp[4 * j + c]) * x[c];
-                const R cont = rhoi * dvdx * F * m / rhoj;
-                dr += cont;
-                drs += std::fabs(cont);
-                R pt = -(pi + pj) * m * F / (rhoi * rhoj);
+                const R cterm = rhoi * dvdx * F * m / rhoj;
+                cont += cterm;
+                drs += std::fabs(cterm);
+                const R pt = -(pi + pj) * m * F / (rhoi * rhoj);
+                R pv = 0;
                 R ptabs = std::fabs(pt);
                 if (fi && fj) {
                     if (delta) {
@@ -345,23 +449,28 @@ struct Oracle {
                         for (int c = 0; c < 3; ++c) gx += (gd[4 * i + c] + gd[4 * j + c]) * x[c];
                         const R d1 = dhc * F * R(2) * (rhoi - rhoj) * r2 / (r2 + eta2) * m / rhoj;
                         const R d2 = -dhc * F * gx * m / rhoj;
-                        dr += d1 + d2;
+                        diff += d1 + d2;
                         drs += std::fabs(d1) + std::fabs(d2);
                     }
-                    const R v = hcrm * F * (gd[4 * i + 3] + gd[4 * j + 3] + al * dvdx / (r2 + eta2)) / (rhoi * rhoj);
-                    pt += v;
-                    ptabs += std::fabs(v);
+                    pv = hcrm * F * (gd[4 * i + 3] + gd[4 * j + 3] + al * dvdx / (r2 + eta2)) / (rhoi * rhoj);
+                    ptabs += std::fabs(pv);
                 }
                 if (!delta) {
                     const R d = (rhoi - rhoj) * r2 * F * dhc / (r2 + eta2) * m / rhoj;
-                    dr += d;
+                    diff += d;
                     drs += std::fabs(d);
                 }
-                for (int c = 0; c < 3; ++c) a[c] += pt * x[c];
+                for (int c = 0; c < 3; ++c) { ap[c] += pt * x[c]; av[c] += pv * x[c]; }
                 as += ptabs * r;
             });
-            if (fi) a[2] += R(p.g);
-            acc[4 * i] = a[0]; acc[4 * i + 1] = a[1]; acc[4 * i + 2] = a[2]; acc[4 * i + 3] = dr;
+            if (fi) ap[2] += R(p.g);
+            const R sd = scale.empty() ? R(1) : R(scale[2 * (size_t)i]);
+            const R sv = scale.empty() ? R(1) : R(scale[2 * (size_t)i + 1]);
+            for (int c = 0; c < 3; ++c) acc[4 * i + c] = ap[c] + sv * av[c];
+            acc[4 * i + 3] = cont + sd * diff;
+            double *t = &terms[8 * (size_t)i];
+            t[0] = double(cont); t[1] = double(diff);
+            for (int c = 0; c < 3; ++c) { t[2 + c] = double(ap[c]); t[5 + c] = double(av[c]); }
             acc_scale[2 * i] = as + (fi ? std::fabs(R(p.g)) : R(0));
             acc_scale[2 * i + 1] = drs;
         }

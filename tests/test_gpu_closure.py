@@ -98,3 +98,105 @@ def test_adam_fits_a_known_scale_field():
     ref = oracle.mlp_infer(trained, x[:2000], emulate_bf16=True)
     np.testing.assert_allclose(s, ref, rtol=0, atol=2e-3)
     m.close()
+
+
+# ---- the closure inside the SPH step ----------------------------------------------
+
+def _state(name="3d"):
+    from zsph_amd import cases
+    from tests.helpers import jitter
+    if name == "3d":
+        return jitter(cases.dambreak3d(dx=0.04, lx=0.8, wy=0.4, lz=0.5, cw=0.3, ch=0.3), seed=12345)
+    return jitter(cases.dambreak2d(dx=0.01), seed=7)
+
+
+def _sorted_pair(c, net=None):
+    import zsph_amd as z
+    s = z.Solver(c)
+    s.upload(c.host)
+    o = oracle.Oracle(c.params)
+    o.upload(c.host)
+    if net is not None:
+        s.set_closure(net)
+        o.set_closure(net, emulate_bf16=True)
+    for x in (s, o):
+        x.hash(0); x.sort_index(); x.sort_data(0, 1)
+    return s, o
+
+
+@pytest.mark.parametrize("name", ["2d", "3d"])
+def test_features_and_terms_match_oracle(name):
+    from zsph_amd import FLUID
+    from oracle import A_ACC_SCALE, A_TERMS
+    c = _state(name)
+    say(f"[closure sample {name}] N={c.n}")
+    s, o = _sorted_pair(c)
+    fg, tg = s.closure_sample(1)
+    o.boundary(1)
+    fo = o.features(1)
+    o.governing(1)
+    to = o.get(A_TERMS, 1)
+    fl = o.get(4, 1) == FLUID
+    # features: FP32 neighbour sums against FP64, 1e-4 of each feature's spread (+ absolute floor)
+    spread = np.abs(fo[fl]).max(0) + 1e-6
+    err = np.abs(fg[fl] - fo[fl]).max(0) / spread
+    say("  worst feature errors / scale:", np.array2string(err, precision=1))
+    assert (err < 2e-4).all(), np.argmax(err)
+    assert np.all(fg[~fl] == 0.0)
+    # term decomposition: 1e-5 of the summed absolute terms
+    sc = o.get(A_ACC_SCALE, 1)
+    for k, col in ((0, 1), (1, 1), (2, 0), (3, 0), (4, 0), (5, 0), (6, 0), (7, 0)):
+        assert np.all(np.abs(tg[fl, k] - to[fl, k]) <= 1e-5 * sc[fl, col] + 1e-30), f"term {k}"
+    s.close()
+
+
+def test_learned_step_matches_oracle():
+    from zsph_amd import FLUID
+    from tests.helpers import assert_close_scaled, p_scale
+    from tests.test_gpu_parity import by_original_id
+    c = _state("3d")
+    net = random_net(6)
+    P = net.params()
+    P[-130:] *= 0.3  # moderate output layer: scale factors spread around 1
+    # normalise with the state's own features, with a margin: the range test is a
+    # discontinuity, so no particle may sit at its edge (where FP32 and FP64 features
+    # could fall on different sides)
+    s0, o0 = _sorted_pair(c)
+    f0, _ = s0.closure_sample(1)
+    s0.close()
+    fl0 = f0.any(1)
+    closure.set_normalisation(net, f0[fl0], margin=0.5)
+    say(f"[learned step] N={c.n}")
+    s, o = _sorted_pair(c, net)
+    s.boundary(1); s.governing(1)
+    o.boundary(1); o.governing(1)
+    from oracle import A_ACC_SCALE, A_SCALE
+    sg = oracle.mlp_infer(net, o.get(oracle.A_FEATURES, 1).astype(np.float32), emulate_bf16=True)
+    so = o.get(A_SCALE, 1)
+    fl = o.get(4, 1) == FLUID
+    np.testing.assert_allclose(so[fl], sg[fl], atol=0)  # the oracle applies its own network
+    say(f"  scale factors: s_d {so[fl, 0].min():.3f}..{so[fl, 0].max():.3f}, "
+        f"s_v {so[fl, 1].min():.3f}..{so[fl, 1].max():.3f}; plain fallback {np.mean(so[fl, 0] == 1.0):.2%}")
+    assert np.ptp(so[fl, 0]) > 0.05 and np.all(so[fl, 0] != 1.0)
+    # Tolerance 1e-3 of the summed terms: the network reads BF16 inputs, and a feature
+    # whose FP32 (device) and FP64 (oracle) values straddle a BF16 rounding boundary
+    # gives scale factors that differ by about one BF16 step propagated through the net.
+    ag, ao = s.get(9, 1), o.get(9, 1)
+    sc = o.get(A_ACC_SCALE, 1)
+    assert_close_scaled(ag[fl, 3], ao[fl, 3], sc[fl, 1], "learned drho/dt", rel=1e-3)
+    assert_close_scaled(ag[fl, :3], ao[fl, :3], np.broadcast_to(sc[fl, 0:1], (fl.sum(), 3)), "learned a", rel=1e-3)
+    worst = np.abs(ag[fl, :3] - ao[fl, :3]).max(1) / sc[fl, 0]
+    say(f"  learned a: median error {np.median(worst):.2e}, worst {worst.max():.2e} of the summed terms")
+    assert np.median(worst) < 1e-5
+    # a full step with the closure (graph-captured)
+    import zsph_amd as z
+    s3 = z.Solver(c); s3.upload(c.host); s3.set_closure(net); s3.step(1); s3.sync()
+    o3 = oracle.Oracle(c.params); o3.upload(c.host); o3.set_closure(net); o3.step(1)
+    g, r = s3.download(0), o3.download(0)
+    og, orr = by_original_id(g), by_original_id(r)
+    f = r.type[orr] == FLUID
+    assert_close_scaled(g.pos[og][f], r.pos[orr][f], np.abs(r.pos[orr][f]) + c.json.dx, "x after a learned step")
+    assert_close_scaled(g.rhop[og, 1][f], r.rhop[orr, 1][f], np.maximum(np.abs(r.rhop[orr, 1][f]), p_scale(c)),
+                        "p after a learned step", rel=1e-4)
+    for x in (s, s3):
+        x.close()

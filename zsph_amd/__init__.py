@@ -345,6 +345,19 @@ class Solver:
         self._chk(lib().zsph_killed(self._ctx, C.byref(k)), "zsph_killed")
         return k.value
 
+    # ---- learned closure ------------------------------------------------------
+    def set_closure(self, net):
+        """Run the step with the closure network `net` (abi.Closure); None: plain terms."""
+        self._chk(lib().zsph_set_closure(self._ctx, C.byref(net) if net is not None else None), "zsph_set_closure")
+
+    def closure_sample(self, buf=1):
+        """(features [N, NF], terms [N, 8]) of buffer `buf` (in cell order), rows in that buffer's order."""
+        f = np.zeros((self.n, abi.NF), np.float32)
+        t = np.zeros((self.n, 8), np.float32)
+        self._chk(lib().zsph_closure_sample(self._ctx, buf, _ptr(f, C.c_float), _ptr(t, C.c_float)),
+                  "zsph_closure_sample")
+        return f, t
+
     # ---- moving dummies -------------------------------------------------------
     def set_body(self, rigid):
         """Make the type-2 particles of buffer 0 one rigid body with state `rigid` (abi.Rigid)."""

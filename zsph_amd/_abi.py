@@ -196,6 +196,8 @@ def load(path):
     _sig(lib, "zsph_mlp_eval", _i, [vp, _i, _i, Pf, Pf])
     _sig(lib, "zsph_mlp_adam", _i, [vp, _i, _i, _f, _f, _f, _f, Pf])
     _sig(lib, "zsph_mlp_infer", _i, [vp, Pf, _i, Pf])
+    _sig(lib, "zsph_set_closure", _i, [vp, P(Closure)])
+    _sig(lib, "zsph_closure_sample", _i, [vp, _i, Pf, Pf])
     return lib
 
 
@@ -212,6 +214,7 @@ EXPORTS = (
     "zsph_set_piston", "zsph_set_time", "zsph_get_time", "zsph_body_reduce", "zsph_case_wavetank",
     "zsph_create_batch", "zsph_scene_diagnostics", "zsph_mlp_create", "zsph_mlp_destroy", "zsph_mlp_last_error",
     "zsph_mlp_set", "zsph_mlp_get", "zsph_mlp_data", "zsph_mlp_eval", "zsph_mlp_adam", "zsph_mlp_infer",
+    "zsph_set_closure", "zsph_closure_sample",
 )
 
 STAGES = ("hash", "sort", "cells", "reorder", "wall", "ldg", "governing", "integrate")

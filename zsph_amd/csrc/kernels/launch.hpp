@@ -30,10 +30,14 @@ void launch_reorder(hipStream_t s, int n, const int *perm, const Buf &a, const B
 // current list passes another walk to launch_governing.
 void launch_wall(hipStream_t s, int walk, const Prm &q, const Prm *sp, const Buf &b, const uint32_t *skey,
                  const int *cs);
+// feat (one scene only; may be nullptr): the closure features, ZSPH_NF floats per particle.
 void launch_ldg(hipStream_t s, int walk, const Prm &q, const Prm *sp, const Buf &b, const uint32_t *skey,
-                const int *cs, float4 *gd, int *nlist, int *ncount);
+                const int *cs, float4 *gd, int *nlist, int *ncount, float *feat);
+// mode (one scene only): 0 plain, 1 closure-scaled (scale: 2 floats per
+// particle), 2 plain + term decomposition (terms: 8 floats per particle).
 void launch_governing(hipStream_t s, int walk, const Prm &q, const Prm *sp, const Buf &b, const uint32_t *skey,
-                      const int *cs, const float4 *gd, float4 *acc, const int *nlist, const int *ncount);
+                      const int *cs, const float4 *gd, float4 *acc, const int *nlist, const int *ncount, int mode,
+                      const float2 *scale, float *terms);
 void launch_predict(hipStream_t s, const Prm &q, const Prm *sp, const Buf &b, const int *cs, const float4 *acc);
 void launch_correct(hipStream_t s, const Prm &q, const Prm *sp, const Buf &b, const int *cs, const float4 *acc);
 void launch_nbr_stats(hipStream_t s, int walk, const Prm &q, const Prm *sp, const Buf &b, const uint32_t *skey,

@@ -64,11 +64,13 @@ __global__ __launch_bounds__(ZS_BLOCK) void k_wall(Prm q0, const Prm *__restrict
 // G_i = L_i^-1 sum F (rho_j - rho_i) V_j x_ij (delta_ns.cu:75-79, :86-90,
 // :99-114, :133-174).  L^-1 is applied once to the accumulated sum (it does
 // not depend on j).  MC form: D only.  Output gd = (G, D); dummies get 0.
-template <int WALK, bool BATCH>
+// FEAT: also the closure network's ZSPH_NF features of each fluid particle
+// (feat[i * ZSPH_NF ...], see closure_features) from the same neighbour walk.
+template <int WALK, bool BATCH, bool FEAT>
 __global__ __launch_bounds__(ZS_BLOCK) void k_ldg(Prm q0, const Prm *__restrict__ sp, Buf b,
-        const uint32_t *__restrict__ skey,
-                                                  const int *__restrict__ cs, float4 *__restrict__ gd,
-                                                  int *__restrict__ nlist, int *__restrict__ ncount) {
+                                                  const uint32_t *__restrict__ skey, const int *__restrict__ cs,
+                                                  float4 *__restrict__ gd, int *__restrict__ nlist,
+                                                  int *__restrict__ ncount, float *__restrict__ feat) {
     __shared__ int qbuf[WalkLds<WALK>::ints];
     const int nf = cs[q0.G];
     const int i = xcd_block(gridDim.x) * ZS_BLOCK + threadIdx.x;
@@ -79,23 +81,47 @@ __global__ __launch_bounds__(ZS_BLOCK) void k_ldg(Prm q0, const Prm *__restrict_
     }
     const Prm q = scene_prm<BATCH>(q0, sp, b, i);
     const bool delta = q.diffusion == ZSPH_DIFF_ANTUONO;
+    const bool ell = delta || FEAT;  // L is needed
     const float4 xi = b.posr[i];
     const float4 vi = b.velp[i];
     float lxx = 0, lxy = 0, lxz = 0, lyy = 0, lyz = 0, lzz = 0;
     float gx = 0, gy = 0, gz = 0, D = 0;
+    // FEAT only: velocity-gradient tensor K_ab = sum F V (v_j - v_i)_a x_b (fluid
+    // neighbours), kernel sum, neighbour counts, velocity spread, density contrast
+    float kxx = 0, kxy = 0, kxz = 0, kyx = 0, kyy = 0, kyz = 0, kzx = 0, kzy = 0, kzz = 0;
+    float ws = 0, wv2 = 0, wdr = 0;
+    int nnb = 0, ndum = 0;
     auto body = [&](int j, float dx, float dy, float dz, float r2, float4 xj) {
         const float qq = sqrtf(r2) * q.inv_h;
         const float t = 1.0f - 0.5f * qq;
         const float FV = q.fk * t * t * t * q.m / xj.w;
-        if (delta) {
+        if (ell) {
             lxx -= FV * dx * dx; lxy -= FV * dx * dy; lxz -= FV * dx * dz;
             lyy -= FV * dy * dy; lyz -= FV * dy * dz; lzz -= FV * dz * dz;
+        }
+        if (delta) {
             const float c = FV * (xj.w - xi.w);
             gx += c * dx; gy += c * dy; gz += c * dz;
         }
+        float WV = 0.0f;
+        if (FEAT) {
+            WV = q.adh * t * t * t * t * (2.0f * qq + 1.0f) * q.m / xj.w;
+            ws += WV;
+            wdr += WV * (xj.w - xi.w);
+            ++nnb;
+        }
         if (j < nf) {
             const float4 vj = b.velp[j];
+            const float ux = vj.x - vi.x, uy = vj.y - vi.y, uz = vj.z - vi.z;
             D -= FV * ((vi.x - vj.x) * dx + (vi.y - vj.y) * dy + (vi.z - vj.z) * dz);
+            if (FEAT) {
+                kxx += FV * ux * dx; kxy += FV * ux * dy; kxz += FV * ux * dz;
+                kyx += FV * uy * dx; kyy += FV * uy * dy; kyz += FV * uy * dz;
+                kzx += FV * uz * dx; kzy += FV * uz * dy; kzz += FV * uz * dz;
+                wv2 += WV * (ux * ux + uy * uy + uz * uz);
+            }
+        } else if (FEAT) {
+            ++ndum;
         }
     };
     const uint32_t cell = cell_of(q, skey[i]);
@@ -106,12 +132,13 @@ __global__ __launch_bounds__(ZS_BLOCK) void k_ldg(Prm q0, const Prm *__restrict_
     } else {
         for_each_neighbor<WALK, true, true>(q, i, xi, cell, cs, b.posr, qbuf, body);
     }
+    const float trL = lxx + lyy + lzz;
+    if (ell && q.dim == 2) lyy += 1.0f;  // x-z plane: unit yy entry keeps L invertible
+    const float c00 = lyy * lzz - lyz * lyz;
+    const float c01 = lxz * lyz - lxy * lzz;
+    const float c02 = lxy * lyz - lxz * lyy;
+    const float det = lxx * c00 + lxy * c01 + lxz * c02;
     if (delta) {
-        if (q.dim == 2) lyy += 1.0f;  // x-z plane: unit yy entry keeps L invertible
-        const float c00 = lyy * lzz - lyz * lyz;
-        const float c01 = lxz * lyz - lxy * lzz;
-        const float c02 = lxy * lyz - lxz * lyy;
-        const float det = lxx * c00 + lxy * c01 + lxz * c02;
         if (fabsf(det) > 1.8e-8f) {
             const float c11 = lxx * lzz - lxz * lxz;
             const float c12 = lxy * lxz - lxx * lyz;
@@ -126,17 +153,54 @@ __global__ __launch_bounds__(ZS_BLOCK) void k_ldg(Prm q0, const Prm *__restrict_
         gx = gy = gz = 0.0f;
     }
     gd[i] = make_float4(gx, gy, gz, D);
+    if constexpr (FEAT) {
+        const float ic0 = rsqrtf(q.cs2), hc = q.h * ic0, hr = q.h / q.rho0;
+        const float self = q.adh * q.m / xi.w;  // W(0) V_i
+        const float wx = kzy - kyz, wy = kxz - kzx, wz = kyx - kxy;
+        const float dv = kxx + kyy + kzz, third = dv / (float)q.dim;
+        const float sxx = kxx - third, syy = (q.dim == 3) ? kyy - third : 0.0f, szz = kzz - third;
+        const float sxy = 0.5f * (kxy + kyx), sxz = 0.5f * (kxz + kzx), syz = 0.5f * (kyz + kzy);
+        const float g2 = gx * gx + gy * gy + gz * gz;
+        const float v2 = vi.x * vi.x + vi.y * vi.y + vi.z * vi.z;
+        float f[ZSPH_NF];
+        f[0] = 1.0f - (ws + self);
+        f[1] = trL / (float)q.dim;
+        f[2] = det;
+        f[3] = sqrtf(lxy * lxy + lxz * lxz + lyz * lyz);
+        f[4] = gx * hr; f[5] = gy * hr; f[6] = gz * hr;
+        f[7] = sqrtf(g2) * hr;
+        f[8] = D * hc;
+        f[9] = wx * hc; f[10] = wy * hc; f[11] = wz * hc;
+        f[12] = sqrtf(wx * wx + wy * wy + wz * wz) * hc;
+        f[13] = sqrtf(sxx * sxx + syy * syy + szz * szz + 2.0f * (sxy * sxy + sxz * sxz + syz * syz)) * hc;
+        f[14] = vi.x * ic0; f[15] = vi.y * ic0; f[16] = vi.z * ic0;
+        f[17] = sqrtf(v2) * ic0;
+        f[18] = (xi.w - q.rho0) / q.rho0;
+        f[19] = vi.w / (q.rho0 * q.cs2);
+        f[20] = (float)nnb;
+        f[21] = (ws > 0.0f) ? sqrtf(wv2 / ws) * ic0 : 0.0f;
+        f[22] = (ws > 0.0f) ? wdr / (ws * q.rho0) : 0.0f;
+        f[23] = nnb ? (float)ndum / (float)nnb : 0.0f;
+        float *o = feat + (size_t)i * ZSPH_NF;
+#pragma unroll
+        for (int k = 0; k < ZSPH_NF; k += 4) *reinterpret_cast<float4 *>(o + k) = make_float4(f[k], f[k + 1], f[k + 2], f[k + 3]);
+    }
 }
 
 // Continuity + momentum of fluid particles (delta_ns.cu:204-265; MC form
 // simple_ns.cu:123-177).  Pressure and continuity over all neighbours,
 // delta term and viscosity over fluid pairs, gravity added.
-template <int WALK, bool BATCH>
+// MODE 0: plain.  MODE 1 (learned closure): the density-diffusion part of
+// drho/dt is scaled by scale[i].x and the viscous part of the acceleration
+// (bulk + artificial viscosity) by scale[i].y.  MODE 2: plain, and the parts
+// are written to terms[i * 8 ...] = (continuity, diffusion, pressure + g (3),
+// viscous (3)) for training.
+template <int WALK, bool BATCH, int MODE>
 __global__ __launch_bounds__(ZS_BLOCK) void k_governing(Prm q0, const Prm *__restrict__ sp, Buf b,
-        const uint32_t *__restrict__ skey,
-                                                        const int *__restrict__ cs, const float4 *__restrict__ gd,
-                                                        float4 *__restrict__ acc, const int *__restrict__ nlist,
-                                                        const int *__restrict__ ncount) {
+                                                        const uint32_t *__restrict__ skey, const int *__restrict__ cs,
+                                                        const float4 *__restrict__ gd, float4 *__restrict__ acc,
+                                                        const int *__restrict__ nlist, const int *__restrict__ ncount,
+                                                        const float2 *__restrict__ scale, float *__restrict__ terms) {
     __shared__ int qbuf[WalkLds<WALK>::ints];
     const int nf = cs[q0.G];
     const int i = xcd_block(gridDim.x) * ZS_BLOCK + threadIdx.x;
@@ -148,6 +212,7 @@ __global__ __launch_bounds__(ZS_BLOCK) void k_governing(Prm q0, const Prm *__res
     const float4 gi = gd[i];
     const float rhoi = xi.w, pi = vi.w;
     float ax = 0, ay = 0, az = 0, dr = 0;
+    float dd = 0, vx = 0, vy = 0, vz = 0;  // diffusion and viscous parts (MODE != 0)
     auto body = [&](int j, float dx, float dy, float dz, float r2, float4 xj) {
         const float qq = sqrtf(r2) * q.inv_h;
         const float t = 1.0f - 0.5f * qq;
@@ -159,25 +224,42 @@ __global__ __launch_bounds__(ZS_BLOCK) void k_governing(Prm q0, const Prm *__res
         dr += rhoi * dvdx * F * Vj;
         const float irr = 1.0f / (rhoi * rhoj);
         float pt = -(pi + pj) * q.m * F * irr;
+        float pv = 0.0f;
         if (j < nf) {
             const float ire = 1.0f / (r2 + q.eta2);
             const float4 gj = gd[j];
             if (delta) {
                 const float psi = 2.0f * (rhoi - rhoj) * r2 * ire -
                                   ((gi.x + gj.x) * dx + (gi.y + gj.y) * dy + (gi.z + gj.z) * dz);
-                dr += q.delta_h_cs * F * psi * Vj;
+                if (MODE) dd += q.delta_h_cs * F * psi * Vj;
+                else dr += q.delta_h_cs * F * psi * Vj;
             }
-            pt += q.h_cs_rho_m * F * (gi.w + gj.w + q.alpha * dvdx * ire) * irr;
+            if (MODE) pv = q.h_cs_rho_m * F * (gi.w + gj.w + q.alpha * dvdx * ire) * irr;
+            else pt += q.h_cs_rho_m * F * (gi.w + gj.w + q.alpha * dvdx * ire) * irr;
         }
-        if (!delta) dr += (rhoi - rhoj) * r2 * F * q.delta_h_cs / (r2 + q.eta2) * Vj;
+        if (!delta) {
+            if (MODE) dd += (rhoi - rhoj) * r2 * F * q.delta_h_cs / (r2 + q.eta2) * Vj;
+            else dr += (rhoi - rhoj) * r2 * F * q.delta_h_cs / (r2 + q.eta2) * Vj;
+        }
         ax += pt * dx; ay += pt * dy; az += pt * dz;
+        if (MODE) { vx += pv * dx; vy += pv * dy; vz += pv * dz; }
     };
     const uint32_t cell = cell_of(q, skey[i]);
     if (WALK == WALK_LIST)
         visit_list<true, true>(q, i, xi, cell, cs, b.posr, nlist, ncount[i], body);
     else
         for_each_neighbor<WALK, true, true>(q, i, xi, cell, cs, b.posr, qbuf, body);
-    acc[i] = make_float4(ax, ay, az + q.g, dr);
+    if constexpr (MODE == 0) {
+        acc[i] = make_float4(ax, ay, az + q.g, dr);
+    } else if constexpr (MODE == 1) {
+        const float2 sc = scale[i];
+        acc[i] = make_float4(ax + sc.y * vx, ay + sc.y * vy, az + q.g + sc.y * vz, dr + sc.x * dd);
+    } else {
+        float4 *o = reinterpret_cast<float4 *>(terms + (size_t)i * 8);
+        o[0] = make_float4(dr, dd, ax, ay);
+        o[1] = make_float4(az + q.g, vx, vy, vz);
+        acc[i] = make_float4(ax + vx, ay + vy, az + q.g + vz, dr + dd);
+    }
 }
 
 // Dummies: moving ones (body, piston) get the pressure force of their fluid
@@ -306,30 +388,68 @@ void launch_wall(hipStream_t s, int walk, const Prm &q, const Prm *sp, const Buf
                  const int *cs) {
     if (q.N) ZS_WALK_LAUNCH(k_wall, nblocks(q.N), q, sp, b, skey, cs);
 }
+template <int WALK, bool BATCH>
+static void ldg_walk(hipStream_t s, bool feat, int grid, const Prm &q, const Prm *sp, const Buf &b,
+                     const uint32_t *skey, const int *cs, float4 *gd, int *nlist, int *ncount, float *f) {
+    if constexpr (!BATCH) {
+        if (feat) {
+            k_ldg<WALK, false, true><<<grid, ZS_BLOCK, 0, s>>>(q, sp, b, skey, cs, gd, nlist, ncount, f);
+            return;
+        }
+    }
+    k_ldg<WALK, BATCH, false><<<grid, ZS_BLOCK, 0, s>>>(q, sp, b, skey, cs, gd, nlist, ncount, f);
+}
+
 void launch_ldg(hipStream_t s, int walk, const Prm &q, const Prm *sp, const Buf &b, const uint32_t *skey,
-                const int *cs, float4 *gd, int *nlist, int *ncount) {
+                const int *cs, float4 *gd, int *nlist, int *ncount, float *feat) {
     if (!q.N) return;
+    const int g = nblocks(q.N);
+    const bool f = feat != nullptr;
     if (walk == WALK_LIST) {
-        if (sp)
-            k_ldg<WALK_LIST, true><<<nblocks(q.N), ZS_BLOCK, 0, s>>>(q, sp, b, skey, cs, gd, nlist, ncount);
-        else
-            k_ldg<WALK_LIST, false><<<nblocks(q.N), ZS_BLOCK, 0, s>>>(q, sp, b, skey, cs, gd, nlist, ncount);
+        if (sp) ldg_walk<WALK_LIST, true>(s, f, g, q, sp, b, skey, cs, gd, nlist, ncount, feat);
+        else ldg_walk<WALK_LIST, false>(s, f, g, q, sp, b, skey, cs, gd, nlist, ncount, feat);
+    } else if (walk == WALK_QUEUE) {
+        if (sp) ldg_walk<WALK_QUEUE, true>(s, f, g, q, sp, b, skey, cs, gd, nlist, ncount, feat);
+        else ldg_walk<WALK_QUEUE, false>(s, f, g, q, sp, b, skey, cs, gd, nlist, ncount, feat);
     } else {
-        ZS_WALK_LAUNCH(k_ldg, nblocks(q.N), q, sp, b, skey, cs, gd, nlist, ncount);
+        if (sp) ldg_walk<WALK_MASK, true>(s, f, g, q, sp, b, skey, cs, gd, nlist, ncount, feat);
+        else ldg_walk<WALK_MASK, false>(s, f, g, q, sp, b, skey, cs, gd, nlist, ncount, feat);
     }
 }
+
+template <int WALK, bool BATCH>
+static void gov_walk(hipStream_t s, int mode, int grid, const Prm &q, const Prm *sp, const Buf &b,
+                     const uint32_t *skey, const int *cs, const float4 *gd, float4 *acc, const int *nlist,
+                     const int *ncount, const float2 *scale, float *terms) {
+    if constexpr (!BATCH) {
+        if (mode == 1) {
+            k_governing<WALK, false, 1><<<grid, ZS_BLOCK, 0, s>>>(q, sp, b, skey, cs, gd, acc, nlist, ncount, scale,
+                                                                 terms);
+            return;
+        }
+        if (mode == 2) {
+            k_governing<WALK, false, 2><<<grid, ZS_BLOCK, 0, s>>>(q, sp, b, skey, cs, gd, acc, nlist, ncount, scale,
+                                                                 terms);
+            return;
+        }
+    }
+    k_governing<WALK, BATCH, 0><<<grid, ZS_BLOCK, 0, s>>>(q, sp, b, skey, cs, gd, acc, nlist, ncount, scale, terms);
+}
+
 void launch_governing(hipStream_t s, int walk, const Prm &q, const Prm *sp, const Buf &b, const uint32_t *skey,
-                      const int *cs, const float4 *gd, float4 *acc, const int *nlist, const int *ncount) {
+                      const int *cs, const float4 *gd, float4 *acc, const int *nlist, const int *ncount, int mode,
+                      const float2 *scale, float *terms) {
     if (!q.N) return;
+    const int g = nblocks(q.N);
     if (walk == WALK_LIST) {
-        if (sp)
-            k_governing<WALK_LIST, true><<<nblocks(q.N), ZS_BLOCK, 0, s>>>(q, sp, b, skey, cs, gd, acc, nlist,
-                                                                           ncount);
-        else
-            k_governing<WALK_LIST, false><<<nblocks(q.N), ZS_BLOCK, 0, s>>>(q, sp, b, skey, cs, gd, acc, nlist,
-                                                                            ncount);
+        if (sp) gov_walk<WALK_LIST, true>(s, mode, g, q, sp, b, skey, cs, gd, acc, nlist, ncount, scale, terms);
+        else gov_walk<WALK_LIST, false>(s, mode, g, q, sp, b, skey, cs, gd, acc, nlist, ncount, scale, terms);
+    } else if (walk == WALK_QUEUE) {
+        if (sp) gov_walk<WALK_QUEUE, true>(s, mode, g, q, sp, b, skey, cs, gd, acc, nlist, ncount, scale, terms);
+        else gov_walk<WALK_QUEUE, false>(s, mode, g, q, sp, b, skey, cs, gd, acc, nlist, ncount, scale, terms);
     } else {
-        ZS_WALK_LAUNCH(k_governing, nblocks(q.N), q, sp, b, skey, cs, gd, acc, nlist, ncount);
+        if (sp) gov_walk<WALK_MASK, true>(s, mode, g, q, sp, b, skey, cs, gd, acc, nlist, ncount, scale, terms);
+        else gov_walk<WALK_MASK, false>(s, mode, g, q, sp, b, skey, cs, gd, acc, nlist, ncount, scale, terms);
     }
     ZS_WALK_LAUNCH(k_governing_dummy, nblocks(q.N), q, sp, b, skey, cs, acc);
 }

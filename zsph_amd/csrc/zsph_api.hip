@@ -46,6 +46,13 @@ struct zsph_ctx {
     std::vector<int> scene_off;       // first particle of each scene in upload order (+ N)
     int cells_buf = -1;               // buffer the cell table describes
     double *scene_part = nullptr;     // per-scene diagnostics (batches only)
+    bool closure_on = false;      // learned closure: features, network, scaled terms
+    float *feat = nullptr;        // N x ZSPH_NF closure features
+    float2 *scale = nullptr;      // N scale factors (diffusion, viscous)
+    float *terms = nullptr;       // N x 8 term decomposition (zsph_closure_sample)
+    float *mlp_P = nullptr;       // network parameters (FP32)
+    void *mlp_pack = nullptr;     // their BF16 image
+    zs::MlpAux *mlp_aux = nullptr;
     zs::DevBody *body = nullptr;  // moving dummies: FP64 body state, piston, device time
     double *body_part = nullptr;  // BODY_BLOCKS x BODY_D force/torque partials
     bool has_body = false, has_piston = false;
@@ -180,7 +187,7 @@ int stage_boundary(zsph_ctx *c, int b, Prof *pf = nullptr) {
     {
         Scope sc(pf, ZSPH_ST_LDG);
         zs::launch_ldg(c->stream, c->walk, c->q, c->scene_prm, c->buf[b], sk, c->cell_start, c->gd, c->nlist,
-                       c->ncount);
+                       c->ncount, c->closure_on ? c->feat : nullptr);
     }
     c->list_valid = (c->walk == zs::WALK_LIST);
     return check_launch(c, "boundary");
@@ -191,9 +198,11 @@ int stage_governing(zsph_ctx *c, int b, Prof *pf = nullptr) {
         Scope sc(pf, ZSPH_ST_GOVERNING);
         // the list walk needs this half step's lists; otherwise walk the cells
         const int walk = (c->walk == zs::WALK_LIST && !c->list_valid) ? zs::WALK_MASK : c->walk;
+        if (c->closure_on)  // scale factors of the fluid particles [0, cs[G])
+            zs::launch_mlp_infer(c->stream, c->mlp_P, c->mlp_pack, c->mlp_aux, c->feat, 0, c->cell_start + c->q.G,
+                                 c->scale, zs::mlp_blocks(c->q.N));
         zs::launch_governing(c->stream, walk, c->q, c->scene_prm, c->buf[b], c->key[c->sorted_slot], c->cell_start,
-                             c->gd, c->acc,
-                             c->nlist, c->ncount);
+                             c->gd, c->acc, c->nlist, c->ncount, c->closure_on ? 1 : 0, c->scale, c->terms);
     }
     return check_launch(c, "governing");
 }
@@ -839,6 +848,65 @@ int zsph_probe_pressure(zsph_ctx *c, int b, const float *pts, int npts, float *p
     (void)hipFree(d);
     if (e != hipSuccess) return set_err(c, ZSPH_E_HIP, "probe pressure failed: %s", hipGetErrorString(e));
     return ZSPH_OK;
+}
+
+// ---- learned closure ------------------------------------------------------
+
+int zsph_set_closure(zsph_ctx *c, const zsph_closure *net) {
+    if (!c) return ZSPH_E_ARG;
+    if (c->nscenes > 1) return set_err(c, ZSPH_E_ARG, "zsph_set_closure: one scene only");
+    drop_graph(c);  // the captured step has the other governing variant
+    if (!net) {
+        c->closure_on = false;
+        return ZSPH_OK;
+    }
+    const size_t N = (size_t)c->q.N;
+    int r;
+    if (!c->mlp_P) {
+        uint16_t *pack = nullptr;
+        if ((r = dalloc(c, &c->feat, N * ZSPH_NF)) || (r = dalloc(c, &c->scale, N)) ||
+            (r = dalloc(c, &c->mlp_P, (size_t)ZSPH_NPARAM)) || (r = dalloc(c, &pack, zs::mlp_pack_elems())) ||
+            (r = dalloc(c, &c->mlp_aux, 1)))
+            return r;
+        c->mlp_pack = pack;
+        HIPCHK(c, zs::mlp_prepare());
+    }
+    zs::MlpAux aux{};
+    std::memcpy(aux.mean, net->mean, sizeof(aux.mean));
+    std::memcpy(aux.inv_std, net->inv_std, sizeof(aux.inv_std));
+    std::memcpy(aux.lo, net->lo, sizeof(aux.lo));
+    std::memcpy(aux.hi, net->hi, sizeof(aux.hi));
+    aux.out_scale = net->out_scale;
+    if ((r = copy_sync(c, c->mlp_P, net->w0, ZSPH_NPARAM * sizeof(float), hipMemcpyHostToDevice))) return r;
+    if ((r = copy_sync(c, c->mlp_aux, &aux, sizeof(aux), hipMemcpyHostToDevice))) return r;
+    zs::launch_mlp_pack(c->stream, c->mlp_P, c->mlp_pack);
+    if ((r = check_launch(c, "closure pack"))) return r;
+    c->closure_on = true;
+    return ZSPH_OK;
+}
+
+int zsph_closure_sample(zsph_ctx *c, int b, float *features, float *terms) {
+    if (!c || !valid_buf(b) || !features || !terms) return set_err(c, ZSPH_E_ARG, "zsph_closure_sample: bad argument");
+    if (c->nscenes > 1) return set_err(c, ZSPH_E_ARG, "zsph_closure_sample: one scene only");
+    if (c->cells_buf != b)
+        return set_err(c, ZSPH_E_STATE, "zsph_closure_sample: buffer %d is not in cell order (sort into it first)", b);
+    const size_t N = (size_t)c->q.N;
+    int r;
+    if (!c->feat && (r = dalloc(c, &c->feat, N * ZSPH_NF))) return r;
+    if (!c->terms && (r = dalloc(c, &c->terms, N * 8))) return r;
+    const uint32_t *sk = c->key[c->sorted_slot];
+    // zero rows for the dummies, which the kernels do not write
+    HIPCHK(c, hipMemsetAsync(c->feat, 0, N * ZSPH_NF * sizeof(float), c->stream));
+    HIPCHK(c, hipMemsetAsync(c->terms, 0, N * 8 * sizeof(float), c->stream));
+    const int walk = c->walk == ZSPH_WALK_LIST ? ZSPH_WALK_MASK : c->walk;
+    zs::launch_wall(c->stream, walk, c->q, nullptr, c->buf[b], sk, c->cell_start);
+    zs::launch_ldg(c->stream, walk, c->q, nullptr, c->buf[b], sk, c->cell_start, c->gd, c->nlist, c->ncount, c->feat);
+    zs::launch_governing(c->stream, walk, c->q, nullptr, c->buf[b], sk, c->cell_start, c->gd, c->acc, c->nlist,
+                         c->ncount, 2, c->scale, c->terms);
+    if ((r = check_launch(c, "closure sample"))) return r;
+    c->list_valid = false;
+    if ((r = copy_sync(c, features, c->feat, N * ZSPH_NF * sizeof(float), hipMemcpyDeviceToHost))) return r;
+    return copy_sync(c, terms, c->terms, N * 8 * sizeof(float), hipMemcpyDeviceToHost);
 }
 
 int zsph_killed(zsph_ctx *c, int *killed) {
