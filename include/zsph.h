@@ -482,6 +482,13 @@ int zsph_set_closure(zsph_ctx *ctx, const zsph_closure *net);
  * pressure + gravity (3), viscous (3)); dummy rows are zero. */
 int zsph_closure_sample(zsph_ctx *ctx, int buf, float *features, float *terms);
 
+/* Coarse-graining filter (training data of the closure): Shepard averages
+ * of the fluid fields around particles rows[0..nq) of buffer `buf` (in cell
+ * order, accelerations from the last zsph_governing on it), with weights
+ * V_j W(r; hc) over fluid particles within `radius` (<= 2 hc):
+ * out[t * 9 ...] = (sum of weights, rho, v (3), a (3), drho/dt). */
+int zsph_filter_fields(zsph_ctx *ctx, int buf, float radius, float hc, const int *rows, int nq, float *out);
+
 /* Launch tuning (no counterpart in the reference, which launches 256-thread
  * blocks, deltasph/main.cu:55-56).  Results do not depend on it: both
  * neighbour walks visit neighbours in the same order, and the stable sort

@@ -358,6 +358,14 @@ class Solver:
                   "zsph_closure_sample")
         return f, t
 
+    def filter_fields(self, rows, radius, hc, buf=1):
+        """Shepard-filtered (weight sum, rho, v, a, drho/dt) around particles `rows` of buffer `buf`."""
+        rows = np.ascontiguousarray(rows, np.int32)
+        out = np.zeros((len(rows), 9), np.float32)
+        self._chk(lib().zsph_filter_fields(self._ctx, buf, radius, hc, _ptr(rows, C.c_int), len(rows),
+                                           _ptr(out, C.c_float)), "zsph_filter_fields")
+        return out
+
     # ---- moving dummies -------------------------------------------------------
     def set_body(self, rigid):
         """Make the type-2 particles of buffer 0 one rigid body with state `rigid` (abi.Rigid)."""

@@ -149,6 +149,53 @@ void launch_scene_diag(hipStream_t s, const Prm &q, const Prm *sp, int nscenes, 
         k_scene_diag<false><<<1, ZS_BLOCK, 0, s>>>(q, sp, b, cs, front_zmax, part);
 }
 
+// Shepard filter of the fluid fields at particles rows[t] of a buffer in cell
+// order: with w_j = V_j W(r_ij; hc) over fluid j within r <= radius (the
+// walk's box is qf.k cells, trimmed at qf.trim2),
+//   out[t] = (sum w, <rho>, <v> (3), <a> (3), <drho/dt>).
+__global__ __launch_bounds__(ZS_BLOCK) void k_filter(Prm qf, Buf b, const uint32_t *__restrict__ skey,
+                                                     const int *__restrict__ cs, const float4 *__restrict__ acc,
+                                                     const int *__restrict__ rows, int nq, float hc, float r2max,
+                                                     float *__restrict__ out) {
+    const int t = blockIdx.x * ZS_BLOCK + threadIdx.x;
+    if (t >= nq) return;
+    const int i = rows[t];
+    const float4 xi = b.posr[i];
+    const float ih = 1.0f / hc;
+    float sw = 0, sr = 0, svx = 0, svy = 0, svz = 0, sax = 0, say = 0, saz = 0, sd = 0;
+    const uint32_t cell = (skey[i] & (qf.flag - 1u)) - (uint32_t)qf.cell_base;
+    for_each_row<true, false>(qf, xi, cell, cs, [&](int fb, int fe, int, int) {
+        for (int j = fb; j < fe; ++j) {
+            const float4 xj = b.posr[j];
+            const float r2 = dist2(xi.x - xj.x, xi.y - xj.y, xi.z - xj.z);
+            if (r2 > r2max) continue;
+            const float q = sqrtf(r2) * ih;
+            const float u = fmaxf(1.0f - 0.5f * q, 0.0f);
+            const float w = u * u * u * u * (2.0f * q + 1.0f) * qf.m / xj.w;
+            const float4 vj = b.velp[j], aj = acc[j];
+            sw += w;
+            sr += w * xj.w;
+            svx += w * vj.x; svy += w * vj.y; svz += w * vj.z;
+            sax += w * aj.x; say += w * aj.y; saz += w * aj.z;
+            sd += w * aj.w;
+        }
+    });
+    const float inv = sw > 0.0f ? 1.0f / sw : 0.0f;
+    float *o = out + (size_t)t * 9;
+    o[0] = sw;
+    o[1] = sr * inv;
+    o[2] = svx * inv; o[3] = svy * inv; o[4] = svz * inv;
+    o[5] = sax * inv; o[6] = say * inv; o[7] = saz * inv;
+    o[8] = sd * inv;
+}
+
+void launch_filter(hipStream_t s, const Prm &qf, const Buf &b, const uint32_t *skey, const int *cs, const float4 *acc,
+                   const int *rows, int nq, float hc, float radius, float *out) {
+    if (nq > 0)
+        k_filter<<<(nq + ZS_BLOCK - 1) / ZS_BLOCK, ZS_BLOCK, 0, s>>>(qf, b, skey, cs, acc, rows, nq, hc,
+                                                                     radius * radius, out);
+}
+
 int diag_blocks(int n) { return (n + ZS_BLOCK - 1) / ZS_BLOCK; }
 
 void launch_diag(hipStream_t s, const Prm &q, const Buf &b, float front_zmax, double *part, float *pmax) {

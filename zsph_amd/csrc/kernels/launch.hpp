@@ -48,6 +48,10 @@ constexpr int DIAG_D = 5;  // per-block FP64 partials: count, kinetic, momentum 
 int diag_blocks(int n);
 void launch_diag(hipStream_t s, const Prm &q, const Buf &b, float front_zmax, double *part, float *pmax);
 void launch_probe(hipStream_t s, const Prm &q, const Buf &b, const float *pts, int npts, float *out);
+// Shepard-filtered fluid fields at particles `rows` (cell order; qf: the
+// buffer's grid with k and trim2 covering `radius`); out: nq x 9 floats.
+void launch_filter(hipStream_t s, const Prm &qf, const Buf &b, const uint32_t *skey, const int *cs, const float4 *acc,
+                   const int *rows, int nq, float hc, float radius, float *out);
 // Per-scene diagnostics from the cell table of buffer b (one block per scene).
 constexpr int SCENE_D = 8;  // count, kinetic, momentum x/y/z, front, zmax, removed
 void launch_scene_diag(hipStream_t s, const Prm &q, const Prm *sp, int nscenes, const Buf &b, const int *cs,

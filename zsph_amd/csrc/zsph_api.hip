@@ -909,6 +909,38 @@ int zsph_closure_sample(zsph_ctx *c, int b, float *features, float *terms) {
     return copy_sync(c, terms, c->terms, N * 8 * sizeof(float), hipMemcpyDeviceToHost);
 }
 
+int zsph_filter_fields(zsph_ctx *c, int b, float radius, float hc, const int *rows, int nq, float *out) {
+    if (!c || !valid_buf(b) || nq < 0 || (nq && (!rows || !out)) || !(radius > 0.0f) || !(hc > 0.0f))
+        return set_err(c, ZSPH_E_ARG, "zsph_filter_fields: bad argument");
+    if (c->nscenes > 1) return set_err(c, ZSPH_E_ARG, "zsph_filter_fields: one scene only");
+    if (c->cells_buf != b)
+        return set_err(c, ZSPH_E_STATE, "zsph_filter_fields: buffer %d is not in cell order", b);
+    for (int t = 0; t < nq; ++t)
+        if (rows[t] < 0 || rows[t] >= c->q.N) return set_err(c, ZSPH_E_ARG, "zsph_filter_fields: row out of range");
+    if (nq == 0) return ZSPH_OK;
+    zs::Prm qf = c->q;
+    const float rc = radius * qf.inv_cell;
+    qf.k = (int)std::ceil(rc);
+    qf.trim2 = rc * rc * 1.002f;
+    int *drows = nullptr;
+    float *dout = nullptr;
+    HIPCHK(c, hipMalloc(&drows, sizeof(int) * (size_t)nq));
+    hipError_t e = hipMalloc(&dout, sizeof(float) * 9 * (size_t)nq);
+    if (e == hipSuccess) e = hipMemcpyAsync(drows, rows, sizeof(int) * (size_t)nq, hipMemcpyHostToDevice, c->stream);
+    if (e == hipSuccess) {
+        zs::launch_filter(c->stream, qf, c->buf[b], c->key[c->sorted_slot], c->cell_start, c->acc, drows, nq, hc,
+                          radius, dout);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(out, dout, sizeof(float) * 9 * (size_t)nq, hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    (void)hipFree(drows);
+    if (dout) (void)hipFree(dout);
+    if (e != hipSuccess) return set_err(c, ZSPH_E_HIP, "zsph_filter_fields: %s", hipGetErrorString(e));
+    return ZSPH_OK;
+}
+
 int zsph_killed(zsph_ctx *c, int *killed) {
     if (!c || !killed) return ZSPH_E_ARG;
     HIPCHK(c, hipMemcpyAsync(killed, c->killed, sizeof(int), hipMemcpyDeviceToHost, c->stream));
