@@ -215,3 +215,29 @@ def test_pipeline_small_scale_learns_something():
     say(f"[pipeline] {rep}")
     assert rep["validation_loss_plain"] == pytest.approx(rep["validation_loss_plain_check"], rel=1e-3)
     assert rep["validation_loss_trained"] < 0.97 * rep["validation_loss_plain"]
+
+
+def test_trained_closure_gains_over_plain_coarse():
+    # The committed network (trained on seeded columns other than this one) on the
+    # held-out 3-D dam break: coarse (dx 0.016) runs against the fine plain run
+    # (dx 0.008) up to t = 0.45 s.  The learned closure must reduce the front
+    # error of the plain coarse method and keep it within 2 %, at >= 5x less
+    # stepping time than the fine run.
+    import os
+    from zsph_amd import cases
+    path = os.path.join(os.path.dirname(closure.__file__), "data", "closure_dambreak3d.npz")
+    net = closure.load_network(path)
+    fine = cases.dambreak3d(dx=0.008, grid_size_factor=2.0)
+    coarse = cases.dambreak3d(dx=0.016, grid_size_factor=2.0)
+    xw = closure.far_wall(fine)
+    say(f"[closure gain] fine N={fine.n}, coarse N={coarse.n}")
+    hf = closure.run_history(fine, 0.45, sample_dt=2e-3)
+    say(f"  fine: {hf['steps']} steps, {hf['ms_per_step']:.3f} ms/step")
+    hp = closure.run_history(coarse, 0.45, sample_dt=2e-3)
+    hl = closure.run_history(coarse, 0.45, net=net, sample_dt=2e-3)
+    plain, learned = closure.compare(hf, hp, xw), closure.compare(hf, hl, xw)
+    say(f"  front error: plain {plain['front_err_max']:.4f}, learned {learned['front_err_max']:.4f}; "
+        f"stepping time ratio: plain {plain['speedup']:.1f}x, learned {learned['speedup']:.1f}x")
+    assert learned["front_err_max"] < plain["front_err_max"]
+    assert learned["front_err_max"] <= 0.02
+    assert learned["speedup"] >= 5.0

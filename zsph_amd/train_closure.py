@@ -67,13 +67,16 @@ def main():
         hl = closure.run_history(coarse, args.t_eval, net=net, log=log)
         plain = closure.compare(hf, hp, xw)
         learned = closure.compare(hf, hl, xw)
-        log(f"[closure] coarse plain:   front err {plain['front_err_max']:.4f}, peak p err {plain['peak_p_err']}, "
-            f"{plain['speedup']:.1f}x faster")
-        log(f"[closure] coarse learned: front err {learned['front_err_max']:.4f}, peak p err "
-            f"{learned['peak_p_err']}, {learned['speedup']:.1f}x faster")
+        for name, r in (("plain", plain), ("learned", learned)):
+            log(f"[closure] coarse {name:8s} front err {r['front_err_max']:.4f}, smoothed peak p err "
+                f"{np.round(r['smoothed_peak_p_err'], 3).tolist()}, raw peak p err "
+                f"{np.round(r['peak_p_err'], 3).tolist()}, {r['speedup']:.1f}x less stepping time")
         report["evaluation"] = {"column": [1.2, 0.55], "t_end": args.t_eval, "far_wall_x": xw,
-                                "probe_z": list(closure.FAR_WALL_PROBES_Z), "fine_wall_s": hf["wall_s"],
-                                "fine_steps": hf["steps"], "coarse_plain": plain, "coarse_learned": learned}
+                                "probe_z": list(closure.FAR_WALL_PROBES_Z), "fine_step_s": hf["step_s"],
+                                "fine_ms_per_step": hf["ms_per_step"], "fine_steps": hf["steps"],
+                                "coarse_plain_ms_per_step": hp["ms_per_step"],
+                                "coarse_learned_ms_per_step": hl["ms_per_step"],
+                                "coarse_plain": plain, "coarse_learned": learned}
         np.savez(os.path.splitext(args.report)[0] + "_histories.npz",
                  t_f=hf["t"], front_f=hf["front"], probes_f=hf["probes"],
                  t_c=hp["t"], front_p=hp["front"], probes_p=hp["probes"],
