@@ -8,6 +8,9 @@ rho0 1000, c0 = 10 sqrt(2 g H), delta 0.1, alpha 0.02, CFL 0.2, FP32.
 One step = one full predictor-corrector step (two half steps: hash, radix
 sort, cell table + reorder, wall pass, gradient pass, continuity+momentum,
 integrator) for every particle; particles are resident in HBM before timing.
+The default timed run is the configured 10,000 steps (t = 0 to 0.63 s, through
+the column's collapse and the run-up to the far wall), so the figure is the
+whole run's throughput, not the compact initial state's.
 
 --case wavetank3d: the floating-body tank, 4.0 x 1.0 x 1.2 m filled to 0.6 m
 at dx 0.006 m (11.1 M fluid, 12.5 M particles), piston 0.025 m amplitude at
@@ -33,6 +36,12 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
+
+# Measured HBM bytes of one continuity+momentum launch on the default 3-D dam
+# break (dx 0.008, cell edge h): rocprofv3 FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0
+# (KiB per dispatch; the factors make a 512 MiB device copy read 512 MiB and
+# write 512 MiB).  Other workloads report no measured traffic.
+GOVERNING_TRAFFIC_BYTES = {("dambreak3d", 0.008, 2.0): (94006.8 * 2.0 + 20214.8) * 1024.0}
 
 
 def step_bytes(n, G, n_fluid):
@@ -75,8 +84,13 @@ def cpu_baseline(scenes, steps=3, warmup=1, setup=None, what="the same case"):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--steps", type=int, default=None,
+                    help="timed steps (default: the configured run, 10000 for the 3-D dam break)")
     ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--event-steps", type=int, default=200, help="steps timed again between HIP events")
+    ap.add_argument("--chunk", type=int, default=100, help="graph replays between host syncs in the timed region")
+    ap.add_argument("--no-graph", dest="graph", action="store_false",
+                    help="launch the step's kernels directly (for kernel tracers that cannot follow graph launches)")
     ap.add_argument("--case", choices=("dambreak3d", "wavetank3d", "ensemble2d"), default="dambreak3d")
     ap.add_argument("--scenes", type=int, default=512, help="ensemble2d: number of scenes")
     ap.add_argument("--dx", type=float, default=None, help="particle spacing (default 0.008 / 0.006)")
@@ -124,6 +138,8 @@ def main():
                     f"{case.n} particles ({case.json.water_ptc_num} fluid), delta-SPH Antuono, "
                     f"Tait gamma 7, CFL 0.2, cell edge 2h/{args.gsf:g}")
         metric = "particle-steps/s, 1.3M 3D dam-break"
+    if args.steps is None:  # the configured run length (the ensemble and the tank: a bounded sample)
+        args.steps = {"dambreak3d": 10000, "wavetank3d": 1000, "ensemble2d": 3000}[args.case]
     batch = isinstance(case, z.Batch)
     n_fluid = int((case.host.type == z.FLUID).sum())
     G = case.grid_num if batch else case.cpu.grid_num
@@ -138,30 +154,39 @@ def main():
         args.autotune = False
     tune = s.autotune(steps=3) if args.autotune else s.get_tuning()
     print(f"[bench] tuning: {tune}", file=sys.stderr, flush=True)
+    s.set_graph(args.graph)
+    # device clock around every continuity+momentum launch, inside the captured step
+    s.clock_governing(True)
     s.step(args.warmup)
     s.sync()
     print(f"[bench] warm-up {args.warmup} steps done", file=sys.stderr, flush=True)
 
     # timed region: K graph-replayed steps, synchronised on both sides
     s.sync()
+    s.clock_read()  # reset
     t0 = time.perf_counter()
-    s.step(args.steps)
-    s.sync()
+    # replayed in chunks with a host sync between them: bounds the launches in
+    # flight (profilers that trace every dispatch keep a record per launch)
+    for k0 in range(0, args.steps, args.chunk):
+        s.step(min(args.chunk, args.steps - k0))
+        s.sync()
     wall = time.perf_counter() - t0
-    # the same K steps again between HIP events on the solver's stream
-    ev_ms = s.time_steps(args.steps)
-    print(f"[bench] {args.steps} steps: host {wall * 1e3 / args.steps:.3f} ms/step, "
-          f"events {ev_ms / args.steps:.3f} ms/step", file=sys.stderr, flush=True)
+    gov_total_ms, gov_launches = s.clock_read()
+    s.clock_governing(False)
+    # a sample of further steps between HIP events on the solver's stream (cross-check)
+    ev_ms = s.time_steps(args.event_steps)
+    print(f"[bench] {args.steps} steps: host {wall * 1e3 / args.steps:.3f} ms/step; next {args.event_steps} steps "
+          f"between events {ev_ms / args.event_steps:.3f} ms/step", file=sys.stderr, flush=True)
 
     # per-kernel times (HIP events around every stage on the same stream)
     prof = s.profile_steps(args.profile_steps)
-    launches = 2 * args.profile_steps
-    gov_ms = prof["governing"] / launches
+    gov_ms = gov_total_ms / max(gov_launches, 1)  # average over the timed region's launches
     killed = s.killed()
+    sim_t = s.time()
     extra = {}
     if floating:
         b = s.get_body()
-        extra = {"sim_time_s": s.time(), "body_c": list(b.c[:]), "body_q": list(b.q[:])}
+        extra = {"body_c": list(b.c[:]), "body_q": list(b.q[:])}
     s.close()
 
     ms_step = wall * 1e3 / args.steps
@@ -185,11 +210,14 @@ def main():
         "config": {"workload": workload,
                    "particles": case.n, "fluid_particles": n_fluid, "cells": G,
                    "fluid_particle_steps_per_s": n_fluid * args.steps / wall,
-                   "event_ms_per_step": ev_ms / args.steps, "removed_particles": killed,
-                   "tuning": tune, **extra},
+                   "event_ms_per_step": ev_ms / args.event_steps, "event_steps": args.event_steps,
+                   "sim_time_s": sim_t, "removed_particles": killed,
+                   "tuning": tune, "graph": args.graph, **extra},
         "roofline": {"bound": "hbm", "kernel": "k_governing", "achieved": gov_gbs, "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": gov_gbs / HBM_PEAK_GBS, "traffic": None,
-                     "bytes_per_launch": gbytes, "avg_launch_ms": gov_ms},
+                     "unit": "GB/s", "frac": gov_gbs / HBM_PEAK_GBS,
+                     "traffic": GOVERNING_TRAFFIC_BYTES.get((args.case, args.dx, args.gsf)),
+                     "bytes_per_launch": gbytes, "avg_launch_ms": gov_ms, "launches": gov_launches,
+                     "timing": "device wall-clock stamps around every launch of the timed region"},
         "step_roofline": {"bytes_per_particle_step": b_step, "achieved": b_step * value / 1e9,
                           "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": b_step * value / 1e9 / HBM_PEAK_GBS},
         "stage_ms_per_step": {k: v / args.profile_steps for k, v in prof.items()},

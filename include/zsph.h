@@ -509,6 +509,22 @@ int zsph_get_tuning(zsph_ctx *ctx, zsph_tuning *t);
  * fastest; *best and *best_ms (may be NULL) receive it. */
 int zsph_autotune(zsph_ctx *ctx, int steps, zsph_tuning *best, float *best_ms);
 
+/* zsph_step launches each step's kernels directly instead of replaying the
+ * captured graph (on = 0); the results are the same.  For tools that cannot
+ * follow graph launches (kernel tracers), and for comparison. */
+int zsph_set_graph(zsph_ctx *ctx, int on);
+
+/* Kernel clock of the continuity+momentum kernel (no counterpart in the
+ * reference): with it on, every launch of that kernel -- in zsph_step's
+ * captured graph too -- is bracketed by two one-thread kernels that read the
+ * device's constant-rate wall clock, so its average duration over a long
+ * graph-replayed run can be measured without a profiler.  Each stamp pair
+ * adds about two microseconds per half step.  zsph_clock_read returns the
+ * accumulated milliseconds and launch count and resets them (either pointer
+ * may be NULL); zsph_clock_governing also resets. */
+int zsph_clock_governing(zsph_ctx *ctx, int on);
+int zsph_clock_read(zsph_ctx *ctx, double *ms, long long *launches);
+
 /* Number of particles removed by zsph_hash since zsph_create. */
 int zsph_killed(zsph_ctx *ctx, int *killed);
 

@@ -358,6 +358,21 @@ class Solver:
                   "zsph_closure_sample")
         return f, t
 
+    def set_graph(self, on=True):
+        """Replay the captured step graph (default) or launch each step's kernels directly."""
+        self._chk(lib().zsph_set_graph(self._ctx, 1 if on else 0), "zsph_set_graph")
+
+    def clock_governing(self, on=True):
+        """Device clock around every continuity+momentum launch (also inside the captured step)."""
+        self._chk(lib().zsph_clock_governing(self._ctx, 1 if on else 0), "zsph_clock_governing")
+
+    def clock_read(self):
+        """(milliseconds, launches) of the clocked kernel since the last read; resets."""
+        ms = C.c_double(0.0)
+        n = C.c_longlong(0)
+        self._chk(lib().zsph_clock_read(self._ctx, C.byref(ms), C.byref(n)), "zsph_clock_read")
+        return ms.value, n.value
+
     def filter_fields(self, rows, radius, hc, buf=1):
         """Shepard-filtered (weight sum, rho, v, a, drho/dt) around particles `rows` of buffer `buf`."""
         rows = np.ascontiguousarray(rows, np.int32)

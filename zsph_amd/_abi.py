@@ -199,6 +199,9 @@ def load(path):
     _sig(lib, "zsph_set_closure", _i, [vp, P(Closure)])
     _sig(lib, "zsph_closure_sample", _i, [vp, _i, Pf, Pf])
     _sig(lib, "zsph_filter_fields", _i, [vp, _i, _f, _f, P(_i), _i, Pf])
+    _sig(lib, "zsph_clock_governing", _i, [vp, _i])
+    _sig(lib, "zsph_set_graph", _i, [vp, _i])
+    _sig(lib, "zsph_clock_read", _i, [vp, P(C.c_double), P(C.c_longlong)])
     return lib
 
 
@@ -215,7 +218,8 @@ EXPORTS = (
     "zsph_set_piston", "zsph_set_time", "zsph_get_time", "zsph_body_reduce", "zsph_case_wavetank",
     "zsph_create_batch", "zsph_scene_diagnostics", "zsph_mlp_create", "zsph_mlp_destroy", "zsph_mlp_last_error",
     "zsph_mlp_set", "zsph_mlp_get", "zsph_mlp_data", "zsph_mlp_eval", "zsph_mlp_adam", "zsph_mlp_infer",
-    "zsph_set_closure", "zsph_closure_sample", "zsph_filter_fields",
+    "zsph_set_closure", "zsph_closure_sample", "zsph_filter_fields", "zsph_clock_governing", "zsph_clock_read",
+    "zsph_set_graph",
 )
 
 STAGES = ("hash", "sort", "cells", "reorder", "wall", "ldg", "governing", "integrate")

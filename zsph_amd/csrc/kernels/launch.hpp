@@ -35,9 +35,10 @@ void launch_ldg(hipStream_t s, int walk, const Prm &q, const Prm *sp, const Buf 
                 const int *cs, float4 *gd, int *nlist, int *ncount, float *feat);
 // mode (one scene only): 0 plain, 1 closure-scaled (scale: 2 floats per
 // particle), 2 plain + term decomposition (terms: 8 floats per particle).
+// clock (may be nullptr): device stamps around the continuity+momentum kernel (launch_clock).
 void launch_governing(hipStream_t s, int walk, const Prm &q, const Prm *sp, const Buf &b, const uint32_t *skey,
                       const int *cs, const float4 *gd, float4 *acc, const int *nlist, const int *ncount, int mode,
-                      const float2 *scale, float *terms);
+                      const float2 *scale, float *terms, unsigned long long *clock = nullptr);
 void launch_predict(hipStream_t s, const Prm &q, const Prm *sp, const Buf &b, const int *cs, const float4 *acc);
 void launch_correct(hipStream_t s, const Prm &q, const Prm *sp, const Buf &b, const int *cs, const float4 *acc);
 void launch_nbr_stats(hipStream_t s, int walk, const Prm &q, const Prm *sp, const Buf &b, const uint32_t *skey,
@@ -52,6 +53,8 @@ void launch_probe(hipStream_t s, const Prm &q, const Buf &b, const float *pts, i
 // buffer's grid with k and trim2 covering `radius`); out: nq x 9 floats.
 void launch_filter(hipStream_t s, const Prm &qf, const Buf &b, const uint32_t *skey, const int *cs, const float4 *acc,
                    const int *rows, int nq, float hc, float radius, float *out);
+// Device wall-clock stamp (start) or accumulation (stop) on stream s.
+void launch_clock(hipStream_t s, unsigned long long *c, bool start);
 // Per-scene diagnostics from the cell table of buffer b (one block per scene).
 constexpr int SCENE_D = 8;  // count, kinetic, momentum x/y/z, front, zmax, removed
 void launch_scene_diag(hipStream_t s, const Prm &q, const Prm *sp, int nscenes, const Buf &b, const int *cs,

@@ -438,9 +438,10 @@ static void gov_walk(hipStream_t s, int mode, int grid, const Prm &q, const Prm 
 
 void launch_governing(hipStream_t s, int walk, const Prm &q, const Prm *sp, const Buf &b, const uint32_t *skey,
                       const int *cs, const float4 *gd, float4 *acc, const int *nlist, const int *ncount, int mode,
-                      const float2 *scale, float *terms) {
+                      const float2 *scale, float *terms, unsigned long long *clock) {
     if (!q.N) return;
     const int g = nblocks(q.N);
+    if (clock) launch_clock(s, clock, true);
     if (walk == WALK_LIST) {
         if (sp) gov_walk<WALK_LIST, true>(s, mode, g, q, sp, b, skey, cs, gd, acc, nlist, ncount, scale, terms);
         else gov_walk<WALK_LIST, false>(s, mode, g, q, sp, b, skey, cs, gd, acc, nlist, ncount, scale, terms);
@@ -451,6 +452,7 @@ void launch_governing(hipStream_t s, int walk, const Prm &q, const Prm *sp, cons
         if (sp) gov_walk<WALK_MASK, true>(s, mode, g, q, sp, b, skey, cs, gd, acc, nlist, ncount, scale, terms);
         else gov_walk<WALK_MASK, false>(s, mode, g, q, sp, b, skey, cs, gd, acc, nlist, ncount, scale, terms);
     }
+    if (clock) launch_clock(s, clock, false);
     ZS_WALK_LAUNCH(k_governing_dummy, nblocks(q.N), q, sp, b, skey, cs, acc);
 }
 void launch_predict(hipStream_t s, const Prm &q, const Prm *sp, const Buf &b, const int *cs, const float4 *acc) {

@@ -46,6 +46,8 @@ struct zsph_ctx {
     std::vector<int> scene_off;       // first particle of each scene in upload order (+ N)
     int cells_buf = -1;               // buffer the cell table describes
     double *scene_part = nullptr;     // per-scene diagnostics (batches only)
+    unsigned long long *clock = nullptr;  // continuity+momentum kernel clock {start, ticks, launches}
+    bool clock_on = false;
     bool closure_on = false;      // learned closure: features, network, scaled terms
     float *feat = nullptr;        // N x ZSPH_NF closure features
     float2 *scale = nullptr;      // N scale factors (diffusion, viscous)
@@ -59,6 +61,7 @@ struct zsph_ctx {
     double t = 0.0, t0 = 0.0;     // simulated time while no dummies move (else on the device)
     hipGraph_t graph = nullptr;
     hipGraphExec_t graph_exec = nullptr;
+    bool use_graph = true;        // zsph_step replays a captured graph (else launches directly)
     std::string err;
     std::vector<void *> allocs;
 };
@@ -202,7 +205,8 @@ int stage_governing(zsph_ctx *c, int b, Prof *pf = nullptr) {
             zs::launch_mlp_infer(c->stream, c->mlp_P, c->mlp_pack, c->mlp_aux, c->feat, 0, c->cell_start + c->q.G,
                                  c->scale, zs::mlp_blocks(c->q.N));
         zs::launch_governing(c->stream, walk, c->q, c->scene_prm, c->buf[b], c->key[c->sorted_slot], c->cell_start,
-                             c->gd, c->acc, c->nlist, c->ncount, c->closure_on ? 1 : 0, c->scale, c->terms);
+                             c->gd, c->acc, c->nlist, c->ncount, c->closure_on ? 1 : 0, c->scale, c->terms,
+                             c->clock_on ? c->clock : nullptr);
     }
     return check_launch(c, "governing");
 }
@@ -511,6 +515,16 @@ int zsph_correction(zsph_ctx *c, int b) {
 int zsph_step(zsph_ctx *c, int nsteps) {
     if (!c || nsteps < 0) return set_err(c, ZSPH_E_ARG, "zsph_step: bad argument");
     if (nsteps == 0 || c->q.N == 0) return ZSPH_OK;
+    if (!c->use_graph) {  // the same kernels, launched one by one
+        for (int s = 0; s < nsteps; ++s) {
+            host_time(c, true);
+            int r = half_step(c, 0, 1, true);
+            host_time(c, false);
+            if (!r) r = half_step(c, 1, 0, false);
+            if (r) return r;
+        }
+        return ZSPH_OK;
+    }
     if (!c->graph_exec) {
         HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
         int r = half_step(c, 0, 1, true);
@@ -939,6 +953,36 @@ int zsph_filter_fields(zsph_ctx *c, int b, float radius, float hc, const int *ro
     if (dout) (void)hipFree(dout);
     if (e != hipSuccess) return set_err(c, ZSPH_E_HIP, "zsph_filter_fields: %s", hipGetErrorString(e));
     return ZSPH_OK;
+}
+
+int zsph_set_graph(zsph_ctx *c, int on) {
+    if (!c) return ZSPH_E_ARG;
+    c->use_graph = on != 0;
+    drop_graph(c);
+    return ZSPH_OK;
+}
+
+int zsph_clock_governing(zsph_ctx *c, int on) {
+    if (!c) return ZSPH_E_ARG;
+    int r;
+    if (!c->clock && (r = dalloc(c, &c->clock, 3))) return r;
+    if ((on != 0) != c->clock_on) drop_graph(c);  // the captured step gains / loses the stamps
+    c->clock_on = on != 0;
+    return zsph_clock_read(c, nullptr, nullptr);
+}
+
+int zsph_clock_read(zsph_ctx *c, double *ms, long long *launches) {
+    if (!c) return ZSPH_E_ARG;
+    if (!c->clock) return set_err(c, ZSPH_E_STATE, "zsph_clock_read: clock not enabled");
+    unsigned long long v[3] = {0, 0, 0};
+    int r = copy_sync(c, v, c->clock, sizeof(v), hipMemcpyDeviceToHost);
+    if (r) return r;
+    int khz = 0;
+    HIPCHK(c, hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, c->device));
+    if (ms) *ms = khz > 0 ? (double)v[1] / (double)khz : 0.0;
+    if (launches) *launches = (long long)v[2];
+    const unsigned long long zero[3] = {0, 0, 0};
+    return copy_sync(c, c->clock, zero, sizeof(zero), hipMemcpyHostToDevice);
 }
 
 int zsph_killed(zsph_ctx *c, int *killed) {
