@@ -528,9 +528,54 @@ int zsph_clock_read(zsph_ctx *ctx, double *ms, long long *launches);
 /* Number of particles removed by zsph_hash since zsph_create. */
 int zsph_killed(zsph_ctx *ctx, int *killed);
 
+/* ---- Checkpoint / restart ------------------------------------------------
+ * Replaces the restart the reference names but does not implement: its case
+ * file carries start_step / current_step (io/read_json.cu:32-33) and its loop
+ * starts at start_step (deltasph/main.cu:58), but write_json
+ * (io/write_json.cu:3-12) is never called and reading its own VTK output back
+ * loses every field (io/read_vkt.cu:26-40 against io/write_vtk.cu:66-127).
+ *
+ * A checkpoint holds the complete state between steps: buffer 0's particle
+ * arrays in their current order (position, velocity, density/pressure, the
+ * saved and moving-dummy arrays, type, table, scene), the removal count, the
+ * simulated time, the FP64 body / piston state and the closure network.  A
+ * context created with the same parameters and restored from it continues
+ * bit for bit as the run that wrote it.
+ *
+ * File: a JSON object of top-level scalars ("format": "zsph-checkpoint",
+ * "version": 1, "step", "time", sizes, flags, "params" = FNV-1a 64 of the
+ * parameter blocks, "sections" = comma-separated names, "<name>_bytes",
+ * "payload_bytes", "checksum" = FNV-1a 64 of the payload, both as 16 hex
+ * digits) padded with spaces to 4096 bytes, the last one '\n'; then the
+ * sections' raw little-endian bytes in the listed order.  Written to
+ * <path>.tmp and renamed into place.
+ *
+ * zsph_checkpoint_write: `step` is the caller's step count (the loop index to
+ * continue from); call it between steps (after zsph_step or zsph_upload).
+ * zsph_checkpoint_read: the file must match the context (particle and cell
+ * counts, scenes, dimension and the parameter hash; else ZSPH_E_ARG and the
+ * context is unchanged); a truncated or corrupt file is ZSPH_E_IO.  Restores
+ * the body / piston / closure set-up with the state; *step (may be NULL)
+ * receives the stored step.  zsph_checkpoint_inspect reads the header
+ * without a context (host only) and, with verify != 0, checks the payload
+ * against its checksum. */
+typedef struct zsph_checkpoint_info {
+    long long step;
+    double time;
+    int ptc_num, cells, scenes, dim;
+    int grid[3];
+    int killed;
+    int has_body, has_piston, closure;
+    long long payload_bytes;
+} zsph_checkpoint_info;
+int zsph_checkpoint_write(zsph_ctx *ctx, const char *path, long long step);
+int zsph_checkpoint_read(zsph_ctx *ctx, const char *path, long long *step);
+int zsph_checkpoint_inspect(const char *path, int verify, zsph_checkpoint_info *info);
+
 /* sizeof of the ABI structs, so bindings can check their mirrors:
  * 0 zsph_json, 1 zsph_cpu_param, 2 zsph_params, 3 zsph_host_state, 4 zsph_dambreak,
- * 5 zsph_rigid, 6 zsph_diag, 7 zsph_tuning, 8 zsph_piston, 9 zsph_wavetank, 10 zsph_closure. */
+ * 5 zsph_rigid, 6 zsph_diag, 7 zsph_tuning, 8 zsph_piston, 9 zsph_wavetank, 10 zsph_closure,
+ * 11 zsph_checkpoint_info. */
 size_t zsph_sizeof(int which);
 
 #ifdef __cplusplus

@@ -189,6 +189,17 @@ class Batch:
         return np.searchsorted(self.offsets, original_index, side="right") - 1
 
 
+def checkpoint_info(path, verify=True):
+    """Header of checkpoint `path` as a dict (host only; verify: check the payload checksum)."""
+    info = abi.CheckpointInfo()
+    rc = lib().zsph_checkpoint_inspect(os.fsencode(path), 1 if verify else 0, C.byref(info))
+    if rc != abi.OK:
+        raise ZsphError(rc, f"zsph_checkpoint_inspect({path})")
+    d = {k: getattr(info, k) for k, _ in abi.CheckpointInfo._fields_}
+    d["grid"] = tuple(info.grid)
+    return d
+
+
 class Solver:
     """One device context (zsph_ctx): the reference's device buffers and kernels.
     `case` is a Case (one scene) or a Batch (independent scenes in one context)."""
@@ -344,6 +355,17 @@ class Solver:
         k = C.c_int(0)
         self._chk(lib().zsph_killed(self._ctx, C.byref(k)), "zsph_killed")
         return k.value
+
+    # ---- checkpoint / restart (io/read_json.cu:32-33 start_step/current_step) --
+    def checkpoint(self, path, step):
+        """Write the state between steps to `path` (step = the loop index to continue from)."""
+        self._chk(lib().zsph_checkpoint_write(self._ctx, os.fsencode(path), int(step)), "zsph_checkpoint_write")
+
+    def restore(self, path):
+        """Take the state of checkpoint `path` (same parameters); returns its step."""
+        step = C.c_longlong(0)
+        self._chk(lib().zsph_checkpoint_read(self._ctx, os.fsencode(path), C.byref(step)), "zsph_checkpoint_read")
+        return step.value
 
     # ---- learned closure ------------------------------------------------------
     def set_closure(self, net):

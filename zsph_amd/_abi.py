@@ -123,6 +123,12 @@ class WaveTank(C.Structure):
                 ("box_hi", _f * 3), ("layers", _i), ("piston_travel", _f)]
 
 
+class CheckpointInfo(C.Structure):
+    _fields_ = [("step", C.c_longlong), ("time", C.c_double), ("ptc_num", _i), ("cells", _i), ("scenes", _i),
+                ("dim", _i), ("grid", _i * 3), ("killed", _i), ("has_body", _i), ("has_piston", _i),
+                ("closure", _i), ("payload_bytes", C.c_longlong)]
+
+
 def _sig(lib, name, res, args):
     fn = getattr(lib, name)
     fn.restype = res
@@ -140,7 +146,7 @@ def load(path):
     vp = C.c_void_p
     _sig(lib, "zsph_sizeof", C.c_size_t, [_i])
     for which, cls in enumerate((Json, CpuParam, Params, HostState, DamBreak, Rigid, Diag, Tuning, Piston,
-                                 WaveTank, Closure)):
+                                 WaveTank, Closure, CheckpointInfo)):
         got = lib.zsph_sizeof(which)
         if got != C.sizeof(cls):
             raise RuntimeError(f"ABI mismatch: sizeof({cls.__name__}) = {C.sizeof(cls)} here, {got} in {path}")
@@ -202,6 +208,9 @@ def load(path):
     _sig(lib, "zsph_clock_governing", _i, [vp, _i])
     _sig(lib, "zsph_set_graph", _i, [vp, _i])
     _sig(lib, "zsph_clock_read", _i, [vp, P(C.c_double), P(C.c_longlong)])
+    _sig(lib, "zsph_checkpoint_write", _i, [vp, C.c_char_p, C.c_longlong])
+    _sig(lib, "zsph_checkpoint_read", _i, [vp, C.c_char_p, P(C.c_longlong)])
+    _sig(lib, "zsph_checkpoint_inspect", _i, [C.c_char_p, _i, P(CheckpointInfo)])
     return lib
 
 
@@ -219,7 +228,7 @@ EXPORTS = (
     "zsph_create_batch", "zsph_scene_diagnostics", "zsph_mlp_create", "zsph_mlp_destroy", "zsph_mlp_last_error",
     "zsph_mlp_set", "zsph_mlp_get", "zsph_mlp_data", "zsph_mlp_eval", "zsph_mlp_adam", "zsph_mlp_infer",
     "zsph_set_closure", "zsph_closure_sample", "zsph_filter_fields", "zsph_clock_governing", "zsph_clock_read",
-    "zsph_set_graph",
+    "zsph_set_graph", "zsph_checkpoint_write", "zsph_checkpoint_read", "zsph_checkpoint_inspect",
 )
 
 STAGES = ("hash", "sort", "cells", "reorder", "wall", "ldg", "governing", "integrate")

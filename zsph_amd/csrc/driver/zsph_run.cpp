@@ -3,9 +3,14 @@
 // <input_path>/input.vtk, or generates a dam break; steps on the GPU and
 // hands output frames to a pool of writer threads (io/mul_thread.cu:3-80),
 // here a blocking queue that is drained and joined before exit.
+// Checkpoints: --checkpoint-step K writes <output_path>/zsph_ckpt<s>.bin
+// after every K-th step (s = steps done); --restart FILE takes the state of
+// such a file and continues the loop at its step (the start_step /
+// current_step the reference's case file carries, io/read_json.cu:32-33).
 //
-//   zsph_run case.json
+//   zsph_run case.json [--checkpoint-step K] [--restart FILE]
 //   zsph_run --dambreak {2,3} [--dx DX] [--steps N] [--output-step K] [--out DIR] [--gamma G]
+//            [--checkpoint-step K] [--restart FILE]
 #include <chrono>
 #include <condition_variable>
 #include <cstdio>
@@ -85,8 +90,9 @@ class Writers {
 
 int usage() {
     std::fprintf(stderr,
-                 "usage: zsph_run case.json\n"
-                 "       zsph_run --dambreak {2,3} [--dx DX] [--steps N] [--output-step K] [--out DIR] [--gamma G]\n");
+                 "usage: zsph_run case.json [--checkpoint-step K] [--restart FILE]\n"
+                 "       zsph_run --dambreak {2,3} [--dx DX] [--steps N] [--output-step K] [--out DIR] [--gamma G]\n"
+                 "                [--checkpoint-step K] [--restart FILE]\n");
     return 2;
 }
 
@@ -99,6 +105,15 @@ int main(int argc, char **argv) {
     std::vector<float> pos, vel, rhop, acc;
     std::vector<int> type, table;
     int n = 0;
+    int ckpt_every = 0;
+    std::string restart;
+    // options common to both forms; true if argv[a] was one
+    auto common = [&](const std::string &k, const char *v) {
+        if (k == "--checkpoint-step") ckpt_every = std::atoi(v);
+        else if (k == "--restart") restart = v;
+        else return false;
+        return true;
+    };
     if (std::strcmp(argv[1], "--dambreak") == 0) {
         if (argc < 3) return usage();
         zsph_dambreak c{};
@@ -121,8 +136,9 @@ int main(int argc, char **argv) {
             else if (k == "--output-step") j.output_step = std::atoi(v);
             else if (k == "--out") std::snprintf(j.output_path, sizeof(j.output_path), "%s", v);
             else if (k == "--gamma") j.gamma = (float)std::atof(v);
-            else return usage();
+            else if (!common(k, v)) return usage();
         }
+        if ((argc - 3) % 2) return usage();
         if (zsph_case_dambreak(&c, &j, nullptr, &n) != ZSPH_OK) return 1;
         pos.resize(3 * (size_t)n); vel.resize(3 * (size_t)n); rhop.resize(2 * (size_t)n);
         acc.resize(4 * (size_t)n); type.resize(n); table.resize(n);
@@ -133,12 +149,15 @@ int main(int argc, char **argv) {
             std::fprintf(stderr, "cannot read %s\n", argv[1]);
             return 1;
         }
+        if (argc % 2) return usage();
+        for (int a = 2; a + 1 < argc; a += 2)
+            if (!common(argv[a], argv[a + 1])) return usage();
         n = j.ptc_num;
         pos.resize(3 * (size_t)n); vel.resize(3 * (size_t)n); rhop.resize(2 * (size_t)n);
         acc.resize(4 * (size_t)n); type.resize(n); table.resize(n);
         zsph_host_state st{pos.data(), vel.data(), rhop.data(), acc.data(), type.data(), table.data(), nullptr};
         const std::string vtk = std::string(j.input_path) + "/input.vtk";
-        if (zsph_read_vtk(vtk.c_str(), &st, n, j.rho0) != ZSPH_OK) {
+        if (restart.empty() && zsph_read_vtk(vtk.c_str(), &st, n, j.rho0) != ZSPH_OK) {
             std::fprintf(stderr, "cannot read %s (ptc_num %d)\n", vtk.c_str(), n);
             return 1;
         }
@@ -163,16 +182,26 @@ int main(int argc, char **argv) {
         return 1;
     }
     zsph_host_state st{pos.data(), vel.data(), rhop.data(), acc.data(), type.data(), table.data(), nullptr};
-    if (zsph_upload(ctx, 0, &st) != ZSPH_OK) {
+    int first = cp.start_step;
+    if (restart.empty() ? zsph_upload(ctx, 0, &st) != ZSPH_OK : zsph_checkpoint_read(ctx, restart.c_str(), nullptr) != ZSPH_OK) {
         std::fprintf(stderr, "%s\n", zsph_last_error(ctx));
         zsph_destroy(ctx);
         return 1;
+    }
+    if (!restart.empty()) {
+        zsph_checkpoint_info info;
+        if (zsph_checkpoint_inspect(restart.c_str(), 0, &info) != ZSPH_OK) {
+            zsph_destroy(ctx);
+            return 1;
+        }
+        first = (int)info.step;
+        std::printf("* restart from %s at step %d (t = %.6g s)\n", restart.c_str(), first, info.time);
     }
     int status = 0;
     {
         Writers w(cp.thread_num, cp.output_path, cp.output_fields | ZSPH_F_POS, cp.binary);
         auto t0 = std::chrono::steady_clock::now();
-        for (int step = cp.start_step; step < cp.end_step; ++step) {
+        for (int step = first; step < cp.end_step; ++step) {
             if (zsph_step(ctx, 1) != ZSPH_OK) {
                 std::fprintf(stderr, "step %d: %s\n", step, zsph_last_error(ctx));
                 status = 1;
@@ -195,6 +224,15 @@ int main(int argc, char **argv) {
                 std::printf("step %d  t=%.5f s  wall %.2f s\n", step, (step + 1) * cp.dt, el);
                 std::fflush(stdout);
                 w.push(std::move(f));
+            }
+            if (ckpt_every > 0 && (step + 1) % ckpt_every == 0) {
+                const std::string path =
+                    std::string(cp.output_path) + "/zsph_ckpt" + std::to_string(step + 1) + ".bin";
+                if (zsph_checkpoint_write(ctx, path.c_str(), step + 1) != ZSPH_OK) {
+                    std::fprintf(stderr, "checkpoint: %s\n", zsph_last_error(ctx));
+                    status = 1;
+                    break;
+                }
             }
         }
         if (zsph_sync(ctx) != ZSPH_OK) {

@@ -123,6 +123,18 @@ struct Parser {
 
 }  // namespace
 
+bool zsph_json_scalars(const std::string &text, std::map<std::string, double> *nums,
+                       std::map<std::string, std::string> *strs) {
+    std::map<std::string, Val> kv;
+    Parser ps(text);
+    if (!ps.object(&kv)) return false;
+    for (const auto &e : kv) {
+        if (e.second.kind == Val::NUM || e.second.kind == Val::BOOL) (*nums)[e.first] = e.second.num;
+        else if (e.second.kind == Val::STR) (*strs)[e.first] = e.second.str;
+    }
+    return true;
+}
+
 extern "C" int zsph_read_json(const char *path, zsph_json *j) {
     if (!path || !j) return ZSPH_E_ARG;
     std::ifstream f(path);

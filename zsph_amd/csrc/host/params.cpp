@@ -179,6 +179,7 @@ extern "C" size_t zsph_sizeof(int which) {
     case 8: return sizeof(zsph_piston);
     case 9: return sizeof(zsph_wavetank);
     case 10: return sizeof(zsph_closure);
+    case 11: return sizeof(zsph_checkpoint_info);
     default: return 0;
     }
 }

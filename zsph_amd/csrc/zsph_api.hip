@@ -11,6 +11,7 @@
 #include <string>
 #include <vector>
 
+#include "host/host_internal.hpp"
 #include "host/rigid.hpp"
 #include "kernels/common.hpp"
 #include "kernels/launch.hpp"
@@ -62,6 +63,7 @@ struct zsph_ctx {
     hipGraph_t graph = nullptr;
     hipGraphExec_t graph_exec = nullptr;
     bool use_graph = true;        // zsph_step replays a captured graph (else launches directly)
+    std::vector<unsigned char> closure_net;  // the zsph_closure last set (checkpoints carry it)
     std::string err;
     std::vector<void *> allocs;
 };
@@ -872,6 +874,7 @@ int zsph_set_closure(zsph_ctx *c, const zsph_closure *net) {
     drop_graph(c);  // the captured step has the other governing variant
     if (!net) {
         c->closure_on = false;
+        c->closure_net.clear();
         return ZSPH_OK;
     }
     const size_t N = (size_t)c->q.N;
@@ -895,6 +898,8 @@ int zsph_set_closure(zsph_ctx *c, const zsph_closure *net) {
     if ((r = copy_sync(c, c->mlp_aux, &aux, sizeof(aux), hipMemcpyHostToDevice))) return r;
     zs::launch_mlp_pack(c->stream, c->mlp_P, c->mlp_pack);
     if ((r = check_launch(c, "closure pack"))) return r;
+    const unsigned char *raw = reinterpret_cast<const unsigned char *>(net);
+    c->closure_net.assign(raw, raw + sizeof(zsph_closure));
     c->closure_on = true;
     return ZSPH_OK;
 }
@@ -1165,6 +1170,165 @@ int zsph_body_reduce(zsph_ctx *c, int b, double F[3], double T[3]) {
         F[d] = s[d];
         T[d] = s[3 + d];
     }
+    return ZSPH_OK;
+}
+
+// ---- checkpoint / restart ------------------------------------------------
+
+namespace {
+
+// FNV-1a 64 of the parameter blocks (all 4-byte fields, no padding): a
+// restart must use the parameters the checkpoint was written with.
+static_assert(sizeof(zs::Prm) % 4 == 0, "Prm is a block of 4-byte fields");
+uint64_t params_hash(const zsph_ctx *c) {
+    uint64_t h = zsph_fnv1a64(&c->q, sizeof(c->q));
+    for (const zs::Prm &p : c->scene_host) h = zsph_fnv1a64(&p, sizeof(p), h);
+    return h;
+}
+
+// The sections of this context's checkpoint, in file order (sizes only).
+// "body" is the FP64 rigid-body / piston block in the library's own layout,
+// tied to the format version.
+std::vector<CkptSection> ckpt_layout(const zsph_ctx *c, bool closure) {
+    const size_t N = (size_t)c->q.N;
+    std::vector<CkptSection> v;
+    auto add = [&](const char *name, size_t bytes) {
+        CkptSection s;
+        s.name = name;
+        s.bytes = bytes;
+        v.push_back(s);
+    };
+    add("posr", N * sizeof(float4));
+    add("velp", N * sizeof(float4));
+    add("tposr", N * sizeof(float4));
+    add("tvelp", N * sizeof(float4));
+    add("type", N * sizeof(int));
+    add("table", N * sizeof(int));
+    if (c->buf[0].scene) add("scene", N * sizeof(int));
+    add("body", sizeof(zs::DevBody));
+    if (closure) add("closure", sizeof(zsph_closure));
+    return v;
+}
+
+// Device address of a particle-array section of buffer 0 (null for the others).
+void *ckpt_device(zsph_ctx *c, const std::string &name) {
+    const zs::Buf &d = c->buf[0];
+    if (name == "posr") return d.posr;
+    if (name == "velp") return d.velp;
+    if (name == "tposr") return d.tposr;
+    if (name == "tvelp") return d.tvelp;
+    if (name == "type") return d.type;
+    if (name == "table") return d.table;
+    if (name == "scene") return d.scene;
+    return nullptr;
+}
+
+}  // namespace
+
+int zsph_checkpoint_write(zsph_ctx *c, const char *path, long long step) {
+    if (!c || !path || !*path || step < 0) return set_err(c, ZSPH_E_ARG, "zsph_checkpoint_write: bad argument");
+    std::vector<CkptSection> secs = ckpt_layout(c, c->closure_on);
+    size_t total = 0;
+    for (const CkptSection &s : secs) total += s.bytes;
+    std::vector<char> host(total);
+    zs::DevBody hb;
+    int killed = 0;
+    int e;
+    if ((e = body_fetch(c, &hb))) return e;  // waits for the stream
+    if ((e = copy_sync(c, &killed, c->killed, sizeof(int), hipMemcpyDeviceToHost))) return e;
+    size_t off = 0;
+    for (CkptSection &s : secs) {
+        char *dst = host.data() + off;
+        if (void *src = ckpt_device(c, s.name)) {
+            HIPCHK(c, hipMemcpyAsync(dst, src, s.bytes, hipMemcpyDeviceToHost, c->stream));
+        } else if (s.name == "body") {
+            std::memcpy(dst, &hb, sizeof(hb));
+        } else {  // closure
+            std::memcpy(dst, c->closure_net.data(), sizeof(zsph_closure));
+        }
+        s.data = dst;
+        off += s.bytes;
+    }
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    CkptMeta m;
+    m.step = step;
+    m.time = hb.t;
+    m.t0 = hb.t0;
+    m.ptc_num = c->q.N;
+    m.cells = c->q.G;
+    m.scenes = c->nscenes;
+    m.dim = c->q.dim;
+    m.grid[0] = c->q.X;
+    m.grid[1] = c->q.Y;
+    m.grid[2] = c->q.Z;
+    m.killed = killed;
+    m.has_body = c->has_body ? 1 : 0;
+    m.has_piston = c->has_piston ? 1 : 0;
+    m.closure = c->closure_on ? 1 : 0;
+    m.params = params_hash(c);
+    std::string err;
+    if ((e = zsph_ckpt_write(path, m, secs, &err))) return set_err(c, e, "%s", err.c_str());
+    return ZSPH_OK;
+}
+
+int zsph_checkpoint_read(zsph_ctx *c, const char *path, long long *step) {
+    if (!c || !path || !*path) return set_err(c, ZSPH_E_ARG, "zsph_checkpoint_read: bad argument");
+    CkptMeta m;
+    std::vector<CkptSection> secs;
+    std::vector<char> payload;
+    std::string err;
+    int e = zsph_ckpt_read(path, &m, &secs, &payload, &err);
+    if (e) return set_err(c, e, "%s", err.c_str());
+    if (m.ptc_num != c->q.N || m.cells != c->q.G || m.scenes != c->nscenes || m.dim != c->q.dim ||
+        m.grid[0] != c->q.X || m.grid[1] != c->q.Y || m.grid[2] != c->q.Z)
+        return set_err(c, ZSPH_E_ARG,
+                       "%s: checkpoint of %d particles, %d cells, %d scene(s), %d-D; this context has %d, %d, %d, %d-D",
+                       path, m.ptc_num, m.cells, m.scenes, m.dim, c->q.N, c->q.G, c->nscenes, c->q.dim);
+    if (m.params != params_hash(c))
+        return set_err(c, ZSPH_E_ARG, "%s: written with other parameters (parameter hash differs)", path);
+    const std::vector<CkptSection> want = ckpt_layout(c, m.closure != 0);
+    bool same = want.size() == secs.size();
+    for (size_t k = 0; same && k < want.size(); ++k) same = want[k].name == secs[k].name && want[k].bytes == secs[k].bytes;
+    if (!same) return set_err(c, ZSPH_E_ARG, "%s: sections differ from this context's layout", path);
+    zs::DevBody hb;
+    size_t off = 0, body_off = 0, closure_off = 0;
+    for (const CkptSection &s : secs) {
+        if (s.name == "body") body_off = off;
+        if (s.name == "closure") closure_off = off;
+        off += s.bytes;
+    }
+    std::memcpy(&hb, payload.data() + body_off, sizeof(hb));
+    if ((hb.has_body != 0) != (m.has_body != 0) || (hb.has_piston != 0) != (m.has_piston != 0) ||
+        ((m.has_body || m.has_piston) && c->nscenes > 1))
+        return set_err(c, ZSPH_E_ARG, "%s: inconsistent moving-dummy flags", path);
+    // validated: from here on the context takes the checkpoint's state
+    if (m.closure) {
+        zsph_closure net;
+        std::memcpy(&net, payload.data() + closure_off, sizeof(net));
+        if ((e = zsph_set_closure(c, &net))) return e;
+    } else if (c->closure_on && (e = zsph_set_closure(c, nullptr))) {
+        return e;
+    }
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    off = 0;
+    for (const CkptSection &s : secs) {
+        if (void *dst = ckpt_device(c, s.name))
+            HIPCHK(c, hipMemcpyAsync(dst, payload.data() + off, s.bytes, hipMemcpyHostToDevice, c->stream));
+        off += s.bytes;
+    }
+    HIPCHK(c, hipMemcpyAsync(c->body, &hb, sizeof(hb), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipMemcpyAsync(c->killed, &m.killed, sizeof(int), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    c->has_body = m.has_body != 0;
+    c->has_piston = m.has_piston != 0;
+    c->t = m.time;
+    c->t0 = m.t0;
+    c->sorted_slot = -1;  // buffer 0 holds a state between steps, not yet hashed
+    c->hashed = 0;
+    c->cells_buf = -1;
+    c->list_valid = false;
+    drop_graph(c);  // the captured step may lack (or hold) the moving-dummy kernels
+    if (step) *step = m.step;
     return ZSPH_OK;
 }
 
