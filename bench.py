@@ -173,10 +173,11 @@ def main():
     wall = time.perf_counter() - t0
     gov_total_ms, gov_launches = s.clock_read()
     s.clock_governing(False)
-    # a sample of further steps between HIP events on the solver's stream (cross-check)
-    ev_ms = s.time_steps(args.event_steps)
-    print(f"[bench] {args.steps} steps: host {wall * 1e3 / args.steps:.3f} ms/step; next {args.event_steps} steps "
-          f"between events {ev_ms / args.event_steps:.3f} ms/step", file=sys.stderr, flush=True)
+    # a sample of further steps between HIP events on the solver's stream (cross-check; 0: none)
+    ev_ms = s.time_steps(args.event_steps) if args.event_steps > 0 else None
+    if ev_ms is not None:
+        print(f"[bench] {args.steps} steps: host {wall * 1e3 / args.steps:.3f} ms/step; next {args.event_steps} "
+              f"steps between events {ev_ms / args.event_steps:.3f} ms/step", file=sys.stderr, flush=True)
 
     # per-kernel times (HIP events around every stage on the same stream)
     prof = s.profile_steps(args.profile_steps)
@@ -210,7 +211,7 @@ def main():
         "config": {"workload": workload,
                    "particles": case.n, "fluid_particles": n_fluid, "cells": G,
                    "fluid_particle_steps_per_s": n_fluid * args.steps / wall,
-                   "event_ms_per_step": ev_ms / args.event_steps, "event_steps": args.event_steps,
+                   "event_ms_per_step": ev_ms / args.event_steps if ev_ms is not None else None, "event_steps": args.event_steps,
                    "sim_time_s": sim_t, "removed_particles": killed,
                    "tuning": tune, "graph": args.graph, **extra},
         "roofline": {"bound": "hbm", "kernel": "k_governing", "achieved": gov_gbs, "peak": HBM_PEAK_GBS,

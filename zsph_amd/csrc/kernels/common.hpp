@@ -139,6 +139,27 @@ __host__ __device__ inline float eos_rho(const Prm &q, float p) {
     return q.rho0 + q.rho0 * expm1f(q.inv_gamma * log1pf(p / q.B));
 }
 
+// Element j of a particle array, addressed by a 32-bit byte offset from the
+// wave-uniform base: a scalar-base vector load with no 64-bit address
+// arithmetic.  Needs j * sizeof(T) < 2^32, i.e. N < 2^28 for 16-byte
+// elements (zsph_create refuses larger N).
+template <class T>
+__device__ __forceinline__ T ld(const T *__restrict__ base, int j) {
+    return *reinterpret_cast<const T *>(reinterpret_cast<const char *>(base) + (uint32_t)j * (uint32_t)sizeof(T));
+}
+// x, y, z of a float4 particle array (12-byte load).
+__device__ __forceinline__ float3 ld_xyz(const float4 *__restrict__ base, int j) {
+    return *reinterpret_cast<const float3 *>(reinterpret_cast<const char *>(base) + (uint32_t)j * 16u);
+}
+constexpr int MAX_PARTICLES = 1 << 28;
+
+// Hardware reciprocal and square root (v_rcp_f32, v_sqrt_f32: about 1 ulp),
+// without the denormal range scaling of the library forms; the operands in
+// the particle-pair bodies (densities, r^2 + eta^2, r^2 of distinct
+// particles) are never denormal.
+__device__ __forceinline__ float hw_rcp(float x) { return __builtin_amdgcn_rcpf(x); }
+__device__ __forceinline__ float hw_sqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
+
 // r^2 in one fixed FP32 form (shared with the oracle): fma(dz,dz,fma(dy,dy,dx*dx)).
 __host__ __device__ inline float dist2(float dx, float dy, float dz) {
     return __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx));
@@ -211,22 +232,20 @@ __device__ __forceinline__ void for_each_row(const Prm &q, float4 xi, uint32_t c
 //  WALK_QUEUE: hits go to a per-lane LDS queue; whenever every active lane
 //              holds >= 4 queued hits the wave processes 4 of each with all
 //              lanes busy, the rest at the end.
-//  WALK_LIST:  the gradient pass records each fluid particle's neighbours
-//              (in walk order) in a global list, then runs its interactions
-//              from the list with every lane busy; the continuity+momentum
-//              pass of the same half step reads the list and tests no
-//              candidates.  Particles with more than LIST_MAX neighbours
-//              fall back to the walk (same order, same results).
+//  WALK_LIST:  the gradient pass runs the mask walk and records each fluid
+//              particle's neighbours, in visiting order, in its row of a
+//              global list; the continuity+momentum pass of the same half
+//              step runs its interactions from the list with every lane
+//              busy and tests no candidates.  Particles with more than
+//              LIST_MAX neighbours take the mask walk there (same order,
+//              same results).
 constexpr int WALK_MASK = 0;
 constexpr int WALK_QUEUE = 1;
 constexpr int WALK_LIST = 2;
 constexpr int LIST_MAX = 128;
-// List layout: 64-particle blocks (one wave), slot-major inside a block, so
-// the lanes of a wave reading their k-th neighbours touch 256 contiguous bytes.
-__device__ __forceinline__ size_t list_at(int i, int k) {
-    return ((size_t)(i >> 6) * LIST_MAX + (size_t)k) * 64 + (size_t)(i & 63);
-}
-inline size_t list_words(int n) { return ((size_t)(n + 63) >> 6) * LIST_MAX * 64; }
+// List layout: particle-major, LIST_MAX ints per particle (512 B rows, so
+// every 4-entry piece is one aligned 16-byte vector access).
+inline size_t list_words(int n) { return (size_t)n * LIST_MAX; }
 constexpr int QCAP = 32;   // queue entries per thread, stored [slot][thread]
 constexpr int QCHUNK = 8;  // candidates tested per lane between queue checks
 template <int WALK>
@@ -243,7 +262,7 @@ __device__ __forceinline__ void for_each_neighbor(const Prm &q, int i, float4 xi
                                                   const int *__restrict__ cs, const float4 *__restrict__ posr,
                                                   int *__restrict__ qbuf, F &&f) {
     auto visit = [&](int j) {
-        const float4 xj = posr[j];
+        const float4 xj = ld(posr, j);
         const float dx = xi.x - xj.x, dy = xi.y - xj.y, dz = xi.z - xj.z;
         f(j, dx, dy, dz, dist2(dx, dy, dz), xj);
     };
@@ -258,10 +277,24 @@ __device__ __forceinline__ void for_each_neighbor(const Prm &q, int i, float4 xi
             for (int base = 0; base < total; base += 32) {
                 const int cnt = min(32, total - base);
                 uint32_t m = 0;
-                for (int t = 0; t < cnt; ++t) {
-                    const int u = base + t;
-                    const int j = (u < nfl) ? fb + u : db + (u - nfl);
-                    m |= (hit(j) ? 1u : 0u) << t;
+                // four candidates per round, their loads issued together
+                // (past the chunk's end the last candidate is loaded again
+                // and its bit dropped); t + e <= 31 since t < cnt <= 32
+                for (int t = 0; t < cnt; t += 4) {
+                    int js[4];
+                    float3 xs[4];
+#pragma unroll
+                    for (int e = 0; e < 4; ++e) {
+                        const int u = base + min(t + e, cnt - 1);
+                        js[e] = (u < nfl) ? fb + u : db + (u - nfl);
+                        xs[e] = ld_xyz(posr, js[e]);
+                    }
+#pragma unroll
+                    for (int e = 0; e < 4; ++e) {
+                        const bool h = dist2(xi.x - xs[e].x, xi.y - xs[e].y, xi.z - xs[e].z) <= q.rc2 &&
+                                       js[e] != i && t + e < cnt;
+                        m |= h ? (1u << (t + e)) : 0u;
+                    }
                 }
                 while (m) {
                     const int u = base + __builtin_ctz(m);
@@ -307,57 +340,32 @@ __device__ __forceinline__ void for_each_neighbor(const Prm &q, int i, float4 xi
     }
 }
 
-// WALK_LIST, recording side: stores the neighbours of i in walk order (the
-// first LIST_MAX of them) and returns how many there are.
-template <bool FLUID_J, bool DUMMY_J>
-__device__ __forceinline__ int build_list(const Prm &q, int i, float4 xi, uint32_t cell, const int *__restrict__ cs,
-                                          const float4 *__restrict__ posr, int *__restrict__ list) {
+// WALK_LIST, recording side: the neighbours of one particle, in the order
+// put() receives them, into its list row.  Four entries are staged in
+// registers and written as one 16-byte store; entries past LIST_MAX are
+// counted but not stored.
+struct ListWriter {
+    int *row;
     int cnt = 0;
-    auto seg = [&](int jb, int je) {
-        for (int j = jb; j < je; ++j) {
-            const float4 xj = posr[j];
-            if (dist2(xi.x - xj.x, xi.y - xj.y, xi.z - xj.z) <= q.rc2 && j != i) {
-                if (cnt < LIST_MAX) list[list_at(i, cnt)] = j;
-                ++cnt;
-            }
-        }
-    };
-    for_each_row<FLUID_J, DUMMY_J>(q, xi, cell, cs, [&](int fb, int fe, int db, int de) {
-        if (FLUID_J) seg(fb, fe);
-        if (DUMMY_J) seg(db, de);
-    });
-    return cnt;
-}
-
-// WALK_LIST, visiting side: f for each recorded neighbour of i, in walk
-// order; past LIST_MAX the walk is repeated and the recorded ones skipped.
-template <bool FLUID_J, bool DUMMY_J, class F>
-__device__ __forceinline__ void visit_list(const Prm &q, int i, float4 xi, uint32_t cell, const int *__restrict__ cs,
-                                           const float4 *__restrict__ posr, const int *__restrict__ list, int cnt,
-                                           F &&f) {
-    auto visit = [&](int j) {
-        const float4 xj = posr[j];
-        const float dx = xi.x - xj.x, dy = xi.y - xj.y, dz = xi.z - xj.z;
-        f(j, dx, dy, dz, dist2(dx, dy, dz), xj);
-    };
-    const int n = min(cnt, LIST_MAX);
-    for (int k = 0; k < n; ++k) visit(list[list_at(i, k)]);
-    if (cnt > LIST_MAX) {
-        int seen = 0;
-        auto seg = [&](int jb, int je) {
-            for (int j = jb; j < je; ++j) {
-                const float4 xj = posr[j];
-                if (dist2(xi.x - xj.x, xi.y - xj.y, xi.z - xj.z) <= q.rc2 && j != i) {
-                    if (seen >= LIST_MAX) visit(j);
-                    ++seen;
-                }
-            }
-        };
-        for_each_row<FLUID_J, DUMMY_J>(q, xi, cell, cs, [&](int fb, int fe, int db, int de) {
-            if (FLUID_J) seg(fb, fe);
-            if (DUMMY_J) seg(db, de);
-        });
+    int s0 = 0, s1 = 0, s2 = 0, s3 = 0;
+    __device__ explicit ListWriter(int *r) : row(r) {}
+    __device__ __forceinline__ void put(int j) {
+        s0 = s1; s1 = s2; s2 = s3; s3 = j;
+        ++cnt;
+        if ((cnt & 3) == 0 && cnt <= LIST_MAX) *reinterpret_cast<int4 *>(row + cnt - 4) = make_int4(s0, s1, s2, s3);
     }
-}
+    // Stores the last cnt % 4 entries (the rest of their piece is never
+    // read) and returns the neighbour count.
+    __device__ __forceinline__ int finish() {
+        const int r = cnt & 3;
+        if (r && cnt <= LIST_MAX) {
+            const int4 v = (r == 1) ? make_int4(s3, 0, 0, 0)
+                         : (r == 2) ? make_int4(s2, s3, 0, 0)
+                                    : make_int4(s1, s2, s3, 0);
+            *reinterpret_cast<int4 *>(row + (cnt - r)) = v;
+        }
+        return cnt;
+    }
+};
 
 }  // namespace zs

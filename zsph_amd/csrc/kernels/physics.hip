@@ -47,11 +47,11 @@ __global__ __launch_bounds__(ZS_BLOCK) void k_wall(Prm q0, const Prm *__restrict
     float P = 0.0f, S = 0.0f;
     for_each_neighbor<WALK, true, false>(q, i, xi, cell_of(q, skey[i]), cs, b.posr, qbuf,
                                    [&](int j, float dx, float dy, float dz, float r2, float4 xj) {
-                                       const float qq = sqrtf(r2) * q.inv_h;
+                                       const float qq = hw_sqrt(r2) * q.inv_h;
                                        const float t = 1.0f - 0.5f * qq;
                                        const float W = q.adh * t * t * t * t * (2.0f * qq + 1.0f);
                                        const float ax = aw.x * dx + aw.y * dy + (aw.z - q.g) * dz;
-                                       P += (b.velp[j].w - xj.w * ax) * W;
+                                       P += (ld(b.velp, j).w - xj.w * ax) * W;
                                        S += W;
                                    });
     float p = (fabsf(S) > 1e-8f) ? P / S : 0.0f;
@@ -92,9 +92,10 @@ __global__ __launch_bounds__(ZS_BLOCK) void k_ldg(Prm q0, const Prm *__restrict_
     float ws = 0, wv2 = 0, wdr = 0;
     int nnb = 0, ndum = 0;
     auto body = [&](int j, float dx, float dy, float dz, float r2, float4 xj) {
-        const float qq = sqrtf(r2) * q.inv_h;
+        const float qq = hw_sqrt(r2) * q.inv_h;
         const float t = 1.0f - 0.5f * qq;
-        const float FV = q.fk * t * t * t * q.m / xj.w;
+        const float Vj = q.m * hw_rcp(xj.w);
+        const float FV = q.fk * t * t * t * Vj;
         if (ell) {
             lxx -= FV * dx * dx; lxy -= FV * dx * dy; lxz -= FV * dx * dz;
             lyy -= FV * dy * dy; lyz -= FV * dy * dz; lzz -= FV * dz * dz;
@@ -105,13 +106,13 @@ __global__ __launch_bounds__(ZS_BLOCK) void k_ldg(Prm q0, const Prm *__restrict_
         }
         float WV = 0.0f;
         if (FEAT) {
-            WV = q.adh * t * t * t * t * (2.0f * qq + 1.0f) * q.m / xj.w;
+            WV = q.adh * t * t * t * t * (2.0f * qq + 1.0f) * Vj;
             ws += WV;
             wdr += WV * (xj.w - xi.w);
             ++nnb;
         }
         if (j < nf) {
-            const float4 vj = b.velp[j];
+            const float4 vj = ld(b.velp, j);
             const float ux = vj.x - vi.x, uy = vj.y - vi.y, uz = vj.z - vi.z;
             D -= FV * ((vi.x - vj.x) * dx + (vi.y - vj.y) * dy + (vi.z - vj.z) * dz);
             if (FEAT) {
@@ -125,10 +126,14 @@ __global__ __launch_bounds__(ZS_BLOCK) void k_ldg(Prm q0, const Prm *__restrict_
         }
     };
     const uint32_t cell = cell_of(q, skey[i]);
-    if (WALK == WALK_LIST) {
-        const int cnt = build_list<true, true>(q, i, xi, cell, cs, b.posr, nlist);
-        ncount[i] = cnt;
-        visit_list<true, true>(q, i, xi, cell, cs, b.posr, nlist, cnt, body);
+    if (WALK == WALK_LIST) {  // the mask walk, recording what it visits
+        ListWriter lw(nlist + (size_t)i * LIST_MAX);
+        for_each_neighbor<WALK_MASK, true, true>(q, i, xi, cell, cs, b.posr, qbuf,
+                                                 [&](int j, float dx, float dy, float dz, float r2, float4 xj) {
+                                                     lw.put(j);
+                                                     body(j, dx, dy, dz, r2, xj);
+                                                 });
+        ncount[i] = lw.finish();
     } else {
         for_each_neighbor<WALK, true, true>(q, i, xi, cell, cs, b.posr, qbuf, body);
     }
@@ -213,21 +218,20 @@ __global__ __launch_bounds__(ZS_BLOCK) void k_governing(Prm q0, const Prm *__res
     const float rhoi = xi.w, pi = vi.w;
     float ax = 0, ay = 0, az = 0, dr = 0;
     float dd = 0, vx = 0, vy = 0, vz = 0;  // diffusion and viscous parts (MODE != 0)
-    auto body = [&](int j, float dx, float dy, float dz, float r2, float4 xj) {
-        const float qq = sqrtf(r2) * q.inv_h;
+    // gj: gd[j] (read only for fluid j)
+    auto body = [&](int j, float dx, float dy, float dz, float r2, float4 xj, float4 vj, float4 gj) {
+        const float qq = hw_sqrt(r2) * q.inv_h;
         const float t = 1.0f - 0.5f * qq;
         const float F = q.fk * t * t * t;
-        const float4 vj = b.velp[j];
         const float rhoj = xj.w, pj = vj.w;
-        const float Vj = q.m / rhoj;
+        const float Vj = q.m * hw_rcp(rhoj);
         const float dvdx = (vi.x - vj.x) * dx + (vi.y - vj.y) * dy + (vi.z - vj.z) * dz;
         dr += rhoi * dvdx * F * Vj;
-        const float irr = 1.0f / (rhoi * rhoj);
+        const float irr = hw_rcp(rhoi * rhoj);
         float pt = -(pi + pj) * q.m * F * irr;
         float pv = 0.0f;
         if (j < nf) {
-            const float ire = 1.0f / (r2 + q.eta2);
-            const float4 gj = gd[j];
+            const float ire = hw_rcp(r2 + q.eta2);
             if (delta) {
                 const float psi = 2.0f * (rhoi - rhoj) * r2 * ire -
                                   ((gi.x + gj.x) * dx + (gi.y + gj.y) * dy + (gi.z + gj.z) * dz);
@@ -238,17 +242,49 @@ __global__ __launch_bounds__(ZS_BLOCK) void k_governing(Prm q0, const Prm *__res
             else pt += q.h_cs_rho_m * F * (gi.w + gj.w + q.alpha * dvdx * ire) * irr;
         }
         if (!delta) {
-            if (MODE) dd += (rhoi - rhoj) * r2 * F * q.delta_h_cs / (r2 + q.eta2) * Vj;
-            else dr += (rhoi - rhoj) * r2 * F * q.delta_h_cs / (r2 + q.eta2) * Vj;
+            const float ire = hw_rcp(r2 + q.eta2);
+            if (MODE) dd += (rhoi - rhoj) * r2 * F * q.delta_h_cs * ire * Vj;
+            else dr += (rhoi - rhoj) * r2 * F * q.delta_h_cs * ire * Vj;
         }
         ax += pt * dx; ay += pt * dy; az += pt * dz;
         if (MODE) { vx += pv * dx; vy += pv * dy; vz += pv * dz; }
     };
+    auto walk_body = [&](int j, float dx, float dy, float dz, float r2, float4 xj) {
+        const float4 vj = ld(b.velp, j);
+        const float4 gj = (j < nf) ? ld(gd, j) : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        body(j, dx, dy, dz, r2, xj, vj, gj);
+    };
     const uint32_t cell = cell_of(q, skey[i]);
-    if (WALK == WALK_LIST)
-        visit_list<true, true>(q, i, xi, cell, cs, b.posr, nlist, ncount[i], body);
-    else
-        for_each_neighbor<WALK, true, true>(q, i, xi, cell, cs, b.posr, qbuf, body);
+    const int n = (WALK == WALK_LIST) ? ncount[i] : 0;
+    if (WALK == WALK_LIST && n <= LIST_MAX) {
+        // four recorded neighbours per round: the list entries and all twelve
+        // gathers are issued before the first interaction (gd of a dummy is
+        // zero, so it is read unconditionally here)
+        const int4 *row = reinterpret_cast<const int4 *>(nlist + (size_t)i * LIST_MAX);
+        auto run = [&](int j, float4 xj, float4 vj, float4 gj) {
+            const float dx = xi.x - xj.x, dy = xi.y - xj.y, dz = xi.z - xj.z;
+            body(j, dx, dy, dz, dist2(dx, dy, dz), xj, vj, gj);
+        };
+        int4 next = (n > 0) ? row[0] : make_int4(0, 0, 0, 0);
+        for (int k = 0; k < n; k += 4) {
+            const int4 jj = next;
+            if (k + 4 < n) next = row[(k >> 2) + 1];
+            const int j0 = jj.x;
+            const int j1 = (k + 1 < n) ? jj.y : j0;
+            const int j2 = (k + 2 < n) ? jj.z : j0;
+            const int j3 = (k + 3 < n) ? jj.w : j0;
+            const float4 x0 = ld(b.posr, j0), x1 = ld(b.posr, j1), x2 = ld(b.posr, j2), x3 = ld(b.posr, j3);
+            const float4 v0 = ld(b.velp, j0), v1 = ld(b.velp, j1), v2 = ld(b.velp, j2), v3 = ld(b.velp, j3);
+            const float4 g0 = ld(gd, j0), g1 = ld(gd, j1), g2 = ld(gd, j2), g3 = ld(gd, j3);
+            run(j0, x0, v0, g0);
+            if (k + 1 < n) run(j1, x1, v1, g1);
+            if (k + 2 < n) run(j2, x2, v2, g2);
+            if (k + 3 < n) run(j3, x3, v3, g3);
+        }
+    } else {
+        for_each_neighbor<WALK == WALK_LIST ? WALK_MASK : WALK, true, true>(q, i, xi, cell, cs, b.posr, qbuf,
+                                                                          walk_body);
+    }
     if constexpr (MODE == 0) {
         acc[i] = make_float4(ax, ay, az + q.g, dr);
     } else if constexpr (MODE == 1) {
@@ -282,14 +318,14 @@ __global__ __launch_bounds__(ZS_BLOCK) void k_governing_dummy(Prm q0, const Prm 
         const float rhoi = xi.w, pi = vi.w;
         for_each_neighbor<WALK, true, false>(q, i, xi, cell_of(q, skey[i]), cs, b.posr, qbuf,
                                        [&](int j, float dx, float dy, float dz, float r2, float4 xj) {
-            const float qq = sqrtf(r2) * q.inv_h;
+            const float qq = hw_sqrt(r2) * q.inv_h;
             const float t = 1.0f - 0.5f * qq;
             const float F = q.fk * t * t * t;
-            const float4 vj = b.velp[j];
+            const float4 vj = ld(b.velp, j);
             const float rhoj = xj.w;
             const float dvdx = (vi.x - vj.x) * dx + (vi.y - vj.y) * dy + (vi.z - vj.z) * dz;
-            dr += rhoi * dvdx * F * q.m / rhoj;
-            const float pt = -(pi + vj.w) * q.m * F / (rhoi * rhoj);
+            dr += rhoi * dvdx * F * (q.m * hw_rcp(rhoj));
+            const float pt = -(pi + vj.w) * q.m * F * hw_rcp(rhoi * rhoj);
             ax += pt * dx; ay += pt * dy; az += pt * dz;
         });
     }

@@ -255,7 +255,7 @@ void drop_graph(zsph_ctx *c) {
 }
 
 bool valid_params(const zsph_params *p) {
-    if (p->ptc_num < 0 || p->grid_xdim < 1 || p->grid_ydim < 1 || p->grid_zdim < 1 || !(p->h > 0.0f) ||
+    if (p->ptc_num < 0 || p->ptc_num >= zs::MAX_PARTICLES || p->grid_xdim < 1 || p->grid_ydim < 1 || p->grid_zdim < 1 || !(p->h > 0.0f) ||
         !(p->grid_size > 0.0f) || p->grid_factor < 1 || (p->dim != 2 && p->dim != 3) || !(p->gamma > 0.0f) ||
         !(p->rc2 > 0.0f) || !(p->rho0 > 0.0f) || !(p->cs2 > 0.0f))
         return false;
@@ -335,7 +335,7 @@ int zsph_create_batch(const zsph_params *scenes, int nscenes, int device, zsph_c
         N += scenes[k].ptc_num;
         G += (long long)scenes[k].grid_xdim * scenes[k].grid_ydim * scenes[k].grid_zdim;
     }
-    if (N > (1ll << 31) - 1 || G > (1ll << 30)) return ZSPH_E_ARG;
+    if (N >= zs::MAX_PARTICLES || G > (1ll << 30)) return ZSPH_E_ARG;
     zsph_ctx *c = new zsph_ctx();
     c->nscenes = nscenes;
     int cb = 1;
