@@ -229,9 +229,10 @@ __device__ __forceinline__ void for_each_row(const Prm &q, float4 xi, uint32_t c
 //              tested 32 at a time into a per-lane hit mask, then the hits
 //              are visited; the wave runs the interaction body once per hit
 //              of its busiest lane in that chunk.
-//  WALK_QUEUE: hits go to a per-lane LDS queue; whenever every active lane
-//              holds >= 4 queued hits the wave processes 4 of each with all
-//              lanes busy, the rest at the end.
+//  WALK_QUEUE: the same tests; each lane's hits go to a ring in LDS and
+//              the interaction body runs in rounds in which every active
+//              lane has a queued hit (fewer idle lanes than per-chunk
+//              visits), at the cost of QCAP ints of LDS per thread.
 //  WALK_LIST:  the gradient pass runs the mask walk and records each fluid
 //              particle's neighbours, in visiting order, in its row of a
 //              global list; the continuity+momentum pass of the same half
@@ -246,8 +247,7 @@ constexpr int LIST_MAX = 128;
 // List layout: particle-major, LIST_MAX ints per particle (512 B rows, so
 // every 4-entry piece is one aligned 16-byte vector access).
 inline size_t list_words(int n) { return (size_t)n * LIST_MAX; }
-constexpr int QCAP = 32;   // queue entries per thread, stored [slot][thread]
-constexpr int QCHUNK = 8;  // candidates tested per lane between queue checks
+constexpr int QCAP = 16;  // WALK_QUEUE ring entries per thread (a power of two, <= 32; also its chunk size)
 template <int WALK>
 struct WalkLds {
     static constexpr int ints = (WALK == WALK_QUEUE) ? QCAP * ZS_BLOCK : 1;
@@ -266,36 +266,37 @@ __device__ __forceinline__ void for_each_neighbor(const Prm &q, int i, float4 xi
         const float dx = xi.x - xj.x, dy = xi.y - xj.y, dz = xi.z - xj.z;
         f(j, dx, dy, dz, dist2(dx, dy, dz), xj);
     };
-    auto hit = [&](int j) {
-        const float4 xj = posr[j];
-        return dist2(xi.x - xj.x, xi.y - xj.y, xi.z - xj.z) <= q.rc2 && j != i;
+    // Hit mask of candidates [base, base + cnt) (cnt <= 32) of a row whose
+    // candidate u is fb + u for u < nfl, else db + (u - nfl).  Four
+    // candidates per round, their loads issued together (past the chunk's
+    // end the last candidate is loaded again and its bit dropped; t + e <= 31
+    // since t < cnt <= 32).
+    auto test_chunk = [&](int base, int cnt, int fb, int nfl, int db) {
+        uint32_t m = 0;
+        for (int t = 0; t < cnt; t += 4) {
+            int js[4];
+            float3 xs[4];
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                const int u = base + min(t + e, cnt - 1);
+                js[e] = (u < nfl) ? fb + u : db + (u - nfl);
+                xs[e] = ld_xyz(posr, js[e]);
+            }
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                const bool h =
+                    dist2(xi.x - xs[e].x, xi.y - xs[e].y, xi.z - xs[e].z) <= q.rc2 && js[e] != i && t + e < cnt;
+                m |= h ? (1u << (t + e)) : 0u;
+            }
+        }
+        return m;
     };
     if (WALK != WALK_QUEUE) {  // WALK_MASK (also the walk WALK_LIST falls back to)
         for_each_row<FLUID_J, DUMMY_J>(q, xi, cell, cs, [&](int fb, int fe, int db, int de) {
             const int nfl = fe - fb;
             const int total = nfl + (de - db);
             for (int base = 0; base < total; base += 32) {
-                const int cnt = min(32, total - base);
-                uint32_t m = 0;
-                // four candidates per round, their loads issued together
-                // (past the chunk's end the last candidate is loaded again
-                // and its bit dropped); t + e <= 31 since t < cnt <= 32
-                for (int t = 0; t < cnt; t += 4) {
-                    int js[4];
-                    float3 xs[4];
-#pragma unroll
-                    for (int e = 0; e < 4; ++e) {
-                        const int u = base + min(t + e, cnt - 1);
-                        js[e] = (u < nfl) ? fb + u : db + (u - nfl);
-                        xs[e] = ld_xyz(posr, js[e]);
-                    }
-#pragma unroll
-                    for (int e = 0; e < 4; ++e) {
-                        const bool h = dist2(xi.x - xs[e].x, xi.y - xs[e].y, xi.z - xs[e].z) <= q.rc2 &&
-                                       js[e] != i && t + e < cnt;
-                        m |= h ? (1u << (t + e)) : 0u;
-                    }
-                }
+                uint32_t m = test_chunk(base, min(32, total - base), fb, nfl, db);
                 while (m) {
                     const int u = base + __builtin_ctz(m);
                     m &= m - 1u;
@@ -304,39 +305,41 @@ __device__ __forceinline__ void for_each_neighbor(const Prm &q, int i, float4 xi
             }
         });
     } else {
+        // Each lane's hits go, chunk by chunk, into its ring of QCAP entries
+        // in LDS ([slot][thread]: conflict-free); the interaction body runs
+        // while every active lane has a queued hit, so those rounds keep all
+        // lanes busy.  A lane that could not take its next chunk's hits
+        // first runs rounds of its own (the others that still have queued
+        // hits join them); what is left runs at the end.  Each lane visits
+        // its hits in the order the mask walk does.
         int *myq = qbuf + threadIdx.x;
         uint32_t head = 0, tail = 0;
-        auto drain_all = [&]() {
-            while (head != tail) {
-                visit(myq[(head & (QCAP - 1)) * ZS_BLOCK]);
-                ++head;
-            }
-        };
-        auto scan = [&](int jb, int je) {
-            for (int base = jb; base < je; base += QCHUNK) {
-                if (__any(tail - head > (uint32_t)(QCAP - QCHUNK))) drain_all();
-#pragma unroll
-                for (int t = 0; t < QCHUNK; ++t) {
-                    const int j = base + t;
-                    if (j < je && hit(j)) {
-                        myq[(tail & (QCAP - 1)) * ZS_BLOCK] = j;
-                        ++tail;
-                    }
-                }
-                while (__all(tail - head >= 4u)) {
-#pragma unroll
-                    for (int u = 0; u < 4; ++u) {
-                        visit(myq[(head & (QCAP - 1)) * ZS_BLOCK]);
-                        ++head;
-                    }
-                }
-            }
+        auto pop_visit = [&]() {
+            const int j = myq[(head & (QCAP - 1)) * ZS_BLOCK];
+            ++head;
+            visit(j);
         };
         for_each_row<FLUID_J, DUMMY_J>(q, xi, cell, cs, [&](int fb, int fe, int db, int de) {
-            if (FLUID_J) scan(fb, fe);
-            if (DUMMY_J) scan(db, de);
+            const int nfl = fe - fb;
+            const int total = nfl + (de - db);
+            for (int base = 0; base < total; base += QCAP) {
+                uint32_t m = test_chunk(base, min(QCAP, total - base), fb, nfl, db);
+                const uint32_t need = (uint32_t)__popc(m);  // <= QCAP: an empty ring always has room
+                while (__any(tail - head + need > (uint32_t)QCAP)) {
+                    if (head != tail) pop_visit();
+                }
+                while (m) {
+                    const int u = base + __builtin_ctz(m);
+                    m &= m - 1u;
+                    myq[(tail & (QCAP - 1)) * ZS_BLOCK] = (u < nfl) ? fb + u : db + (u - nfl);
+                    ++tail;
+                }
+                while (__all(head != tail)) pop_visit();
+            }
         });
-        drain_all();
+        while (__any(head != tail)) {
+            if (head != tail) pop_visit();
+        }
     }
 }
 
