@@ -41,7 +41,7 @@ HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
 # break (dx 0.008, cell edge h): rocprofv3 FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0
 # (KiB per dispatch; the factors make a 512 MiB device copy read 512 MiB and
 # write 512 MiB).  Other workloads report no measured traffic.
-GOVERNING_TRAFFIC_BYTES = {("dambreak3d", 0.008, 2.0): (94006.8 * 2.0 + 20214.8) * 1024.0}
+GOVERNING_TRAFFIC_BYTES = {("dambreak3d", 0.008, 2.0): (80486.6 * 2.0 + 20214.8) * 1024.0}
 
 
 def step_bytes(n, G, n_fluid):

@@ -234,8 +234,7 @@ __device__ __forceinline__ void for_each_row(const Prm &q, float4 xi, uint32_t c
 //              lane has a queued hit (fewer idle lanes than per-chunk
 //              visits), at the cost of QCAP ints of LDS per thread.
 //  WALK_LIST:  the gradient pass runs the mask walk and records each fluid
-//              particle's neighbours, in visiting order, in its row of a
-//              global list; the continuity+momentum pass of the same half
+//              particle's neighbours, in visiting order, in a global list; the continuity+momentum pass of the same half
 //              step runs its interactions from the list with every lane
 //              busy and tests no candidates.  Particles with more than
 //              LIST_MAX neighbours take the mask walk there (same order,
@@ -244,9 +243,14 @@ constexpr int WALK_MASK = 0;
 constexpr int WALK_QUEUE = 1;
 constexpr int WALK_LIST = 2;
 constexpr int LIST_MAX = 128;
-// List layout: particle-major, LIST_MAX ints per particle (512 B rows, so
-// every 4-entry piece is one aligned 16-byte vector access).
-inline size_t list_words(int n) { return (size_t)n * LIST_MAX; }
+// List layout: 4-entry pieces (16 bytes), interleaved per wave: quad q
+// (entries 4q .. 4q+3) of particle i is at int4 index
+// ((i / 64) * LIST_MAX / 4 + q) * 64 + i % 64, so the 64 lanes of a wave
+// reading their q-th quads read 1 KB contiguously.
+__host__ __device__ inline size_t list_quad(int i, int q) {
+    return ((size_t)(i >> 6) * (LIST_MAX / 4) + (size_t)q) * 64 + (size_t)(i & 63);
+}
+inline size_t list_words(int n) { return ((size_t)(n + 63) >> 6) * 64 * LIST_MAX; }
 constexpr int QCAP = 16;  // WALK_QUEUE ring entries per thread (a power of two, <= 32; also its chunk size)
 template <int WALK>
 struct WalkLds {
@@ -343,21 +347,35 @@ __device__ __forceinline__ void for_each_neighbor(const Prm &q, int i, float4 xi
     }
 }
 
-// WALK_LIST, recording side: the neighbours of one particle, in the order
-// put() receives them, into its list row.  Four entries are staged in
+// Nontemporal (streaming) 16-byte store and load.
+typedef int i32x4 __attribute__((ext_vector_type(4)));
+__device__ __forceinline__ void nt_store(int4 *p, int4 v) {
+    const i32x4 x = {v.x, v.y, v.z, v.w};
+    __builtin_nontemporal_store(x, reinterpret_cast<i32x4 *>(p));
+}
+__device__ __forceinline__ int4 nt_load(const int4 *p) {
+    const i32x4 x = __builtin_nontemporal_load(reinterpret_cast<const i32x4 *>(p));
+    return make_int4(x.x, x.y, x.z, x.w);
+}
+
+// WALK_LIST, recording side: the neighbours of particle i, in the order
+// put() receives them, into its list quads.  Four entries are staged in
 // registers and written as one 16-byte store; entries past LIST_MAX are
-// counted but not stored.
+// counted but not stored.  The list is written once and read once, by the
+// next kernel: the stores (and its loads) are nontemporal, keeping the
+// neighbour gathers' lines in L2.
 struct ListWriter {
-    int *row;
+    int4 *list;
+    int i;
     int cnt = 0;
     int s0 = 0, s1 = 0, s2 = 0, s3 = 0;
-    __device__ explicit ListWriter(int *r) : row(r) {}
+    __device__ ListWriter(int *l, int particle) : list(reinterpret_cast<int4 *>(l)), i(particle) {}
     __device__ __forceinline__ void put(int j) {
         s0 = s1; s1 = s2; s2 = s3; s3 = j;
         ++cnt;
-        if ((cnt & 3) == 0 && cnt <= LIST_MAX) *reinterpret_cast<int4 *>(row + cnt - 4) = make_int4(s0, s1, s2, s3);
+        if ((cnt & 3) == 0 && cnt <= LIST_MAX) nt_store(&list[list_quad(i, (cnt >> 2) - 1)], make_int4(s0, s1, s2, s3));
     }
-    // Stores the last cnt % 4 entries (the rest of their piece is never
+    // Stores the last cnt % 4 entries (the rest of their quad is never
     // read) and returns the neighbour count.
     __device__ __forceinline__ int finish() {
         const int r = cnt & 3;
@@ -365,7 +383,7 @@ struct ListWriter {
             const int4 v = (r == 1) ? make_int4(s3, 0, 0, 0)
                          : (r == 2) ? make_int4(s2, s3, 0, 0)
                                     : make_int4(s1, s2, s3, 0);
-            *reinterpret_cast<int4 *>(row + (cnt - r)) = v;
+            nt_store(&list[list_quad(i, cnt >> 2)], v);
         }
         return cnt;
     }

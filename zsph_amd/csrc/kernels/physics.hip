@@ -91,7 +91,8 @@ __global__ __launch_bounds__(ZS_BLOCK) void k_ldg(Prm q0, const Prm *__restrict_
     float kxx = 0, kxy = 0, kxz = 0, kyx = 0, kyy = 0, kyz = 0, kzx = 0, kzy = 0, kzz = 0;
     float ws = 0, wv2 = 0, wdr = 0;
     int nnb = 0, ndum = 0;
-    auto body = [&](int j, float dx, float dy, float dz, float r2, float4 xj) {
+    // vj: velp[j] (read only for fluid j)
+    auto body = [&](int j, float dx, float dy, float dz, float r2, float4 xj, float4 vj) {
         const float qq = hw_sqrt(r2) * q.inv_h;
         const float t = 1.0f - 0.5f * qq;
         const float Vj = q.m * hw_rcp(xj.w);
@@ -112,7 +113,6 @@ __global__ __launch_bounds__(ZS_BLOCK) void k_ldg(Prm q0, const Prm *__restrict_
             ++nnb;
         }
         if (j < nf) {
-            const float4 vj = ld(b.velp, j);
             const float ux = vj.x - vi.x, uy = vj.y - vi.y, uz = vj.z - vi.z;
             D -= FV * ((vi.x - vj.x) * dx + (vi.y - vj.y) * dy + (vi.z - vj.z) * dz);
             if (FEAT) {
@@ -125,17 +125,20 @@ __global__ __launch_bounds__(ZS_BLOCK) void k_ldg(Prm q0, const Prm *__restrict_
             ++ndum;
         }
     };
+    auto walk_body = [&](int j, float dx, float dy, float dz, float r2, float4 xj) {
+        body(j, dx, dy, dz, r2, xj, (j < nf) ? ld(b.velp, j) : make_float4(0.0f, 0.0f, 0.0f, 0.0f));
+    };
     const uint32_t cell = cell_of(q, skey[i]);
     if (WALK == WALK_LIST) {  // the mask walk, recording what it visits
-        ListWriter lw(nlist + (size_t)i * LIST_MAX);
+        ListWriter lw(nlist, i);
         for_each_neighbor<WALK_MASK, true, true>(q, i, xi, cell, cs, b.posr, qbuf,
                                                  [&](int j, float dx, float dy, float dz, float r2, float4 xj) {
                                                      lw.put(j);
-                                                     body(j, dx, dy, dz, r2, xj);
+                                                     walk_body(j, dx, dy, dz, r2, xj);
                                                  });
         ncount[i] = lw.finish();
     } else {
-        for_each_neighbor<WALK, true, true>(q, i, xi, cell, cs, b.posr, qbuf, body);
+        for_each_neighbor<WALK, true, true>(q, i, xi, cell, cs, b.posr, qbuf, walk_body);
     }
     const float trL = lxx + lyy + lzz;
     if (ell && q.dim == 2) lyy += 1.0f;  // x-z plane: unit yy entry keeps L invertible
@@ -257,29 +260,36 @@ __global__ __launch_bounds__(ZS_BLOCK) void k_governing(Prm q0, const Prm *__res
     const uint32_t cell = cell_of(q, skey[i]);
     const int n = (WALK == WALK_LIST) ? ncount[i] : 0;
     if (WALK == WALK_LIST && n <= LIST_MAX) {
-        // four recorded neighbours per round: the list entries and all twelve
-        // gathers are issued before the first interaction (gd of a dummy is
-        // zero, so it is read unconditionally here)
-        const int4 *row = reinterpret_cast<const int4 *>(nlist + (size_t)i * LIST_MAX);
-        auto run = [&](int j, float4 xj, float4 vj, float4 gj) {
-            const float dx = xi.x - xj.x, dy = xi.y - xj.y, dz = xi.z - xj.z;
-            body(j, dx, dy, dz, dist2(dx, dy, dz), xj, vj, gj);
-        };
-        int4 next = (n > 0) ? row[0] : make_int4(0, 0, 0, 0);
-        for (int k = 0; k < n; k += 4) {
-            const int4 jj = next;
-            if (k + 4 < n) next = row[(k >> 2) + 1];
-            const int j0 = jj.x;
-            const int j1 = (k + 1 < n) ? jj.y : j0;
-            const int j2 = (k + 2 < n) ? jj.z : j0;
-            const int j3 = (k + 3 < n) ? jj.w : j0;
-            const float4 x0 = ld(b.posr, j0), x1 = ld(b.posr, j1), x2 = ld(b.posr, j2), x3 = ld(b.posr, j3);
-            const float4 v0 = ld(b.velp, j0), v1 = ld(b.velp, j1), v2 = ld(b.velp, j2), v3 = ld(b.velp, j3);
-            const float4 g0 = ld(gd, j0), g1 = ld(gd, j1), g2 = ld(gd, j2), g3 = ld(gd, j3);
-            run(j0, x0, v0, g0);
-            if (k + 1 < n) run(j1, x1, v1, g1);
-            if (k + 2 < n) run(j2, x2, v2, g2);
-            if (k + 3 < n) run(j3, x3, v3, g3);
+        // R quads (4R recorded neighbours) per round: the round's list reads
+        // and all its gathers are issued before its first interaction, so
+        // they are in flight together (this pass is latency-bound; 8 per
+        // round measured best).  gd of a dummy is zero (k_ldg writes it), so
+        // every gather is unconditional.
+        constexpr int R = 2;
+        const int4 *list = reinterpret_cast<const int4 *>(nlist);
+        for (int k = 0; k < n; k += 4 * R) {
+            int js[4 * R];
+#pragma unroll
+            for (int r = 0; r < R; ++r) {
+                const int4 jj = (k + 4 * r < n) ? nt_load(&list[list_quad(i, (k >> 2) + r)]) : make_int4(0, 0, 0, 0);
+                js[4 * r] = jj.x; js[4 * r + 1] = jj.y; js[4 * r + 2] = jj.z; js[4 * r + 3] = jj.w;
+            }
+#pragma unroll
+            for (int e = 1; e < 4 * R; ++e) js[e] = (k + e < n) ? js[e] : js[0];  // past the end: a valid index
+            float4 xs[4 * R], vs[4 * R], gs[4 * R];
+#pragma unroll
+            for (int e = 0; e < 4 * R; ++e) {
+                xs[e] = ld(b.posr, js[e]);
+                vs[e] = ld(b.velp, js[e]);
+                gs[e] = ld(gd, js[e]);
+            }
+#pragma unroll
+            for (int e = 0; e < 4 * R; ++e) {
+                if (k + e < n) {
+                    const float dx = xi.x - xs[e].x, dy = xi.y - xs[e].y, dz = xi.z - xs[e].z;
+                    body(js[e], dx, dy, dz, dist2(dx, dy, dz), xs[e], vs[e], gs[e]);
+                }
+            }
         }
     } else {
         for_each_neighbor<WALK == WALK_LIST ? WALK_MASK : WALK, true, true>(q, i, xi, cell, cs, b.posr, qbuf,
