@@ -38,10 +38,11 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
 
 # Measured HBM bytes of one continuity+momentum launch on the default 3-D dam
-# break (dx 0.008, cell edge h): rocprofv3 FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0
-# (KiB per dispatch; the factors make a 512 MiB device copy read 512 MiB and
-# write 512 MiB).  Other workloads report no measured traffic.
-GOVERNING_TRAFFIC_BYTES = {("dambreak3d", 0.008, 2.0): (80486.6 * 2.0 + 20214.8) * 1024.0}
+# break (dx 0.008, cell edge h), per neighbour walk (0 mask, 2 list):
+# rocprofv3 FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0 (KiB per dispatch; the factors
+# make a 512 MiB device copy read 512 MiB and write 512 MiB).  Other workloads
+# report no measured traffic.
+GOVERNING_TRAFFIC_BYTES = {("dambreak3d", 0.008, 2.0, 0): (80486.6 * 2.0 + 20214.8) * 1024.0}
 
 
 def step_bytes(n, G, n_fluid):
@@ -99,6 +100,9 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-autotune", dest="autotune", action="store_false")
     ap.add_argument("--walk", type=int, default=None, help="force a neighbour walk (0 mask, 1 queue, 2 list)")
+    ap.add_argument("--tune-at", type=int, default=None,
+                    help="autotune on the state after this many steps, then restart from the initial state "
+                         "(default: 30%% of the timed steps)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); the step is single-GPU.",
@@ -152,7 +156,23 @@ def main():
     if args.walk is not None:
         s.set_tuning(args.walk, s.get_tuning()["radix_bits"])
         args.autotune = False
-    tune = s.autotune(steps=3) if args.autotune else s.get_tuning()
+    if args.autotune:
+        # The walks' relative speed changes as the flow develops, so the
+        # candidates are timed on a developed state: checkpoint the initial
+        # state, step, autotune there, then restore the initial state bit for
+        # bit (all outside the timed region).
+        import tempfile
+        tune_at = args.tune_at if args.tune_at is not None else (3 * args.steps) // 10
+        with tempfile.TemporaryDirectory() as tmp:
+            ckpt = os.path.join(tmp, "initial.bin")
+            s.checkpoint(ckpt, 0)
+            s.step(tune_at)
+            s.sync()
+            tune = s.autotune(steps=3)
+            s.restore(ckpt)
+        tune["tuned_at_step"] = tune_at
+    else:
+        tune = s.get_tuning()
     print(f"[bench] tuning: {tune}", file=sys.stderr, flush=True)
     s.set_graph(args.graph)
     # device clock around every continuity+momentum launch, inside the captured step
@@ -216,7 +236,7 @@ def main():
                    "tuning": tune, "graph": args.graph, **extra},
         "roofline": {"bound": "hbm", "kernel": "k_governing", "achieved": gov_gbs, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": gov_gbs / HBM_PEAK_GBS,
-                     "traffic": GOVERNING_TRAFFIC_BYTES.get((args.case, args.dx, args.gsf)),
+                     "traffic": GOVERNING_TRAFFIC_BYTES.get((args.case, args.dx, args.gsf, tune["walk"])),
                      "bytes_per_launch": gbytes, "avg_launch_ms": gov_ms, "launches": gov_launches,
                      "timing": "device wall-clock stamps around every launch of the timed region"},
         "step_roofline": {"bytes_per_particle_step": b_step, "achieved": b_step * value / 1e9,

@@ -492,11 +492,15 @@ int zsph_closure_sample(zsph_ctx *ctx, int buf, float *features, float *terms);
 int zsph_filter_fields(zsph_ctx *ctx, int buf, float radius, float hc, const int *rows, int nq, float *out);
 
 /* Launch tuning (no counterpart in the reference, which launches 256-thread
- * blocks, deltasph/main.cu:55-56).  Results do not depend on it: both
+ * blocks, deltasph/main.cu:55-56).  Results do not depend on it: all
  * neighbour walks visit neighbours in the same order, and the stable sort
  * defines one order whatever its digit width. */
 #define ZSPH_WALK_MASK 0  /* per-row hit bitmask, then the hits            */
-#define ZSPH_WALK_QUEUE 1 /* per-lane LDS queue of hits, drained wave-wide */
+#define ZSPH_WALK_HITS 1  /* gradient pass records every row chunk's hit
+                           * mask; the continuity+momentum pass visits the
+                           * recorded hits without testing (4 B per row
+                           * chunk: 200 B of device memory per particle at
+                           * grid_factor 2)                                */
 #define ZSPH_WALK_LIST 2  /* gradient pass records neighbour lists (in walk
                            * order) that the governing pass reuses; takes
                            * 512 B of device memory per particle          */

@@ -81,7 +81,7 @@ def test_batch_sort_matches_standalone_scenes():
     sb.close()
 
 
-@pytest.mark.parametrize("walk", [z.WALK_MASK, z.WALK_LIST])
+@pytest.mark.parametrize("walk", [z.WALK_MASK, z.WALK_HITS, z.WALK_LIST])
 def test_batch_steps_match_standalone_scenes(walk):
     b = uneven_batch()
     steps = 12
@@ -106,7 +106,7 @@ def test_batch_steps_match_standalone_scenes(walk):
 def test_batch_graph_replay_is_deterministic_and_tuning_independent():
     b = uneven_batch(6, seed=3)
     runs = []
-    for walk in (z.WALK_MASK, z.WALK_MASK, z.WALK_QUEUE, z.WALK_LIST):
+    for walk in (z.WALK_MASK, z.WALK_MASK, z.WALK_HITS, z.WALK_LIST):
         s = z.Solver(b)
         s.upload(b.host)
         s.set_tuning(walk, 9)

@@ -221,7 +221,7 @@ def test_results_do_not_depend_on_tuning(name):
     c = CASES[name]()
     say(f"[case tuning {name}] N={c.n}")
     ref = None
-    for walk in (z._abi.WALK_MASK, z._abi.WALK_QUEUE, z._abi.WALK_LIST):
+    for walk in (z._abi.WALK_MASK, z._abi.WALK_HITS, z._abi.WALK_LIST):
         for bits in (8, 11):
             s = z.Solver(c)
             s.set_tuning(walk, bits)
@@ -240,7 +240,7 @@ def test_results_do_not_depend_on_tuning(name):
     s.upload(c.host)
     before = s.get(A_POSR, 0)
     t = s.autotune(steps=2)
-    assert t["walk"] in (z._abi.WALK_MASK, z._abi.WALK_QUEUE, z._abi.WALK_LIST) and 8 <= t["radix_bits"] <= 11
+    assert t["walk"] in (z._abi.WALK_MASK, z._abi.WALK_HITS, z._abi.WALK_LIST) and 8 <= t["radix_bits"] <= 11
     assert np.array_equal(s.get(A_POSR, 0), before) and s.killed() == 0
 
 

@@ -92,7 +92,7 @@ class Tuning(C.Structure):
     _fields_ = [("walk", _i), ("radix_bits", _i)]
 
 
-WALK_MASK, WALK_QUEUE, WALK_LIST = 0, 1, 2
+WALK_MASK, WALK_HITS, WALK_LIST = 0, 1, 2
 
 NF, NH, NT = 24, 64, 12
 NPARAM = NF * NH + NH + 2 * (NH * NH + NH) + NH * 2 + 2

@@ -168,6 +168,7 @@ __host__ __device__ inline float dist2(float dx, float dy, float dz) {
 // Decode a linear cell index into the clamped search box.
 struct CellBox {
     int xlo, xhi, ylo, yhi, zlo, zhi;
+    int cy, cz;  // the particle's own cell row
 };
 __device__ __forceinline__ CellBox cell_box(const Prm &q, uint32_t key) {
     const int cz = (int)(key / (uint32_t)q.XY);
@@ -178,6 +179,8 @@ __device__ __forceinline__ CellBox cell_box(const Prm &q, uint32_t key) {
     b.xlo = max(cx - q.k, 0); b.xhi = min(cx + q.k, q.X - 1);
     b.ylo = max(cy - q.k, 0); b.yhi = min(cy + q.k, q.Y - 1);
     b.zlo = max(cz - q.k, 0); b.zhi = min(cz + q.k, q.Z - 1);
+    b.cy = cy;
+    b.cz = cz;
     return b;
 }
 
@@ -189,11 +192,12 @@ __device__ __forceinline__ CellBox cell_box(const Prm &q, uint32_t key) {
 // the others (dummies, removed) are [cs[G+1+c], cs[G+1+c+1]).  Fluid
 // particles therefore occupy [0, cs[G]) and dummies [cs[G], N).
 
-// Calls g(fb, fe, db, de) for every (y,z) cell row of the (2k+1)^3 box
+// Calls g(fb, fe, db, de, r) for every (y,z) cell row of the (2k+1)^3 box
 // around xi's cell, trimmed to the x-cells that can hold a point within the
 // cut-off of xi (rows beyond the cut-off are skipped): [fb, fe) are the
 // row's fluid candidates, [db, de) its other particles (empty when the
-// segment is not requested).  The trimming works in cell units with a
+// segment is not requested), r = (dz + k) (2k + 1) + dy + k the row's place
+// in the (2k+1)^2 rows of the box (dy, dz: offsets from xi's cell).  The trimming works in cell units with a
 // margin, so it only ever keeps extra candidates; the r^2 predicate alone
 // decides the neighbour set.
 template <bool FLUID_J, bool DUMMY_J, class G>
@@ -219,7 +223,7 @@ __device__ __forceinline__ void for_each_row(const Prm &q, float4 xi, uint32_t c
             int fb = 0, fe = 0, db = 0, de = 0;
             if (FLUID_J) { fb = cs[row + xa]; fe = cs[row + xb + 1]; }
             if (DUMMY_J) { db = cs[q.G + 1 + row + xa]; de = cs[q.G + 1 + row + xb + 1]; }
-            g(fb, fe, db, de);
+            g(fb, fe, db, de, (cz - b.cz + q.k) * (2 * q.k + 1) + (cy - b.cy + q.k));
         }
     }
 }
@@ -229,10 +233,11 @@ __device__ __forceinline__ void for_each_row(const Prm &q, float4 xi, uint32_t c
 //              tested 32 at a time into a per-lane hit mask, then the hits
 //              are visited; the wave runs the interaction body once per hit
 //              of its busiest lane in that chunk.
-//  WALK_QUEUE: the same tests; each lane's hits go to a ring in LDS and
-//              the interaction body runs in rounds in which every active
-//              lane has a queued hit (fewer idle lanes than per-chunk
-//              visits), at the cost of QCAP ints of LDS per thread.
+//  WALK_HITS:  the gradient pass runs the mask walk and records every row
+//              chunk's hit mask; the continuity+momentum pass of the same
+//              half step walks the same rows and visits the recorded hits
+//              without testing candidates.  Particles with a row of more
+//              than 64 candidates take the mask walk there.
 //  WALK_LIST:  the gradient pass runs the mask walk and records each fluid
 //              particle's neighbours, in visiting order, in a global list; the continuity+momentum pass of the same half
 //              step runs its interactions from the list with every lane
@@ -240,7 +245,7 @@ __device__ __forceinline__ void for_each_row(const Prm &q, float4 xi, uint32_t c
 //              LIST_MAX neighbours take the mask walk there (same order,
 //              same results).
 constexpr int WALK_MASK = 0;
-constexpr int WALK_QUEUE = 1;
+constexpr int WALK_HITS = 1;
 constexpr int WALK_LIST = 2;
 constexpr int LIST_MAX = 128;
 // List layout: 4-entry pieces (16 bytes), interleaved per wave: quad q
@@ -251,20 +256,35 @@ __host__ __device__ inline size_t list_quad(int i, int q) {
     return ((size_t)(i >> 6) * (LIST_MAX / 4) + (size_t)q) * 64 + (size_t)(i & 63);
 }
 inline size_t list_words(int n) { return ((size_t)(n + 63) >> 6) * 64 * LIST_MAX; }
-constexpr int QCAP = 16;  // WALK_QUEUE ring entries per thread (a power of two, <= 32; also its chunk size)
+// WALK_HITS record: 32-bit hit masks, two chunk slots per row of the box,
+// slot-major over the particles (hits[slot * N + i]), so a wave's lanes
+// storing or loading the same slot touch 256 contiguous bytes.
+inline int hits_slots(int k) { return 2 * (2 * k + 1) * (2 * k + 1); }
+// LDS a walk needs per block (none; kept so kernels declare one shape).
 template <int WALK>
 struct WalkLds {
-    static constexpr int ints = (WALK == WALK_QUEUE) ? QCAP * ZS_BLOCK : 1;
+    static constexpr int ints = 1;
+};
+
+// No-op recorder of the mask walk's chunk masks.
+struct NoRec {
+    __device__ __forceinline__ void operator()(int, int, uint32_t) const {}
 };
 
 // Visit every neighbour j != i (FP32 r^2 <= rc2) of particle i at xi whose
 // segment is enabled (FLUID_J: j < cs[G]; DUMMY_J: j >= cs[G]).
-// f(j, dx, dy, dz, r2, xj) with x_ij = xi - xj.  qbuf: WalkLds<WALK>::ints
-// ints of LDS per block (unused by WALK_MASK).
-template <int WALK, bool FLUID_J, bool DUMMY_J, class F>
+// f(j, dx, dy, dz, r2, xj) with x_ij = xi - xj.  Every walk variant visits
+// the neighbours in this order: rows in for_each_row order, each row's
+// candidates (fluid range, then the others) tested 32 at a time into a
+// per-lane hit mask, then the hits in ascending order; the wave runs f once
+// per hit of its busiest lane in each chunk.  rec(r, c, m) receives the hit
+// mask m of chunk c of box row r.  qbuf: WalkLds<WALK>::ints ints of LDS per
+// block (unused).
+template <int WALK, bool FLUID_J, bool DUMMY_J, class F, class Rec = NoRec>
 __device__ __forceinline__ void for_each_neighbor(const Prm &q, int i, float4 xi, uint32_t cell,
                                                   const int *__restrict__ cs, const float4 *__restrict__ posr,
-                                                  int *__restrict__ qbuf, F &&f) {
+                                                  int *__restrict__ qbuf, F &&f, Rec &&rec = Rec{}) {
+    (void)qbuf;
     auto visit = [&](int j) {
         const float4 xj = ld(posr, j);
         const float dx = xi.x - xj.x, dy = xi.y - xj.y, dz = xi.z - xj.z;
@@ -295,56 +315,45 @@ __device__ __forceinline__ void for_each_neighbor(const Prm &q, int i, float4 xi
         }
         return m;
     };
-    if (WALK != WALK_QUEUE) {  // WALK_MASK (also the walk WALK_LIST falls back to)
-        for_each_row<FLUID_J, DUMMY_J>(q, xi, cell, cs, [&](int fb, int fe, int db, int de) {
-            const int nfl = fe - fb;
-            const int total = nfl + (de - db);
-            for (int base = 0; base < total; base += 32) {
-                uint32_t m = test_chunk(base, min(32, total - base), fb, nfl, db);
-                while (m) {
-                    const int u = base + __builtin_ctz(m);
-                    m &= m - 1u;
-                    visit((u < nfl) ? fb + u : db + (u - nfl));
-                }
+    for_each_row<FLUID_J, DUMMY_J>(q, xi, cell, cs, [&](int fb, int fe, int db, int de, int r) {
+        const int nfl = fe - fb;
+        const int total = nfl + (de - db);
+        for (int base = 0, c = 0; base < total; base += 32, ++c) {
+            uint32_t m = test_chunk(base, min(32, total - base), fb, nfl, db);
+            rec(r, c, m);
+            while (m) {
+                const int u = base + __builtin_ctz(m);
+                m &= m - 1u;
+                visit((u < nfl) ? fb + u : db + (u - nfl));
             }
-        });
-    } else {
-        // Each lane's hits go, chunk by chunk, into its ring of QCAP entries
-        // in LDS ([slot][thread]: conflict-free); the interaction body runs
-        // while every active lane has a queued hit, so those rounds keep all
-        // lanes busy.  A lane that could not take its next chunk's hits
-        // first runs rounds of its own (the others that still have queued
-        // hits join them); what is left runs at the end.  Each lane visits
-        // its hits in the order the mask walk does.
-        int *myq = qbuf + threadIdx.x;
-        uint32_t head = 0, tail = 0;
-        auto pop_visit = [&]() {
-            const int j = myq[(head & (QCAP - 1)) * ZS_BLOCK];
-            ++head;
-            visit(j);
-        };
-        for_each_row<FLUID_J, DUMMY_J>(q, xi, cell, cs, [&](int fb, int fe, int db, int de) {
-            const int nfl = fe - fb;
-            const int total = nfl + (de - db);
-            for (int base = 0; base < total; base += QCAP) {
-                uint32_t m = test_chunk(base, min(QCAP, total - base), fb, nfl, db);
-                const uint32_t need = (uint32_t)__popc(m);  // <= QCAP: an empty ring always has room
-                while (__any(tail - head + need > (uint32_t)QCAP)) {
-                    if (head != tail) pop_visit();
-                }
-                while (m) {
-                    const int u = base + __builtin_ctz(m);
-                    m &= m - 1u;
-                    myq[(tail & (QCAP - 1)) * ZS_BLOCK] = (u < nfl) ? fb + u : db + (u - nfl);
-                    ++tail;
-                }
-                while (__all(head != tail)) pop_visit();
-            }
-        });
-        while (__any(head != tail)) {
-            if (head != tail) pop_visit();
         }
-    }
+    });
+}
+
+// WALK_HITS, replay side: the walk of for_each_neighbor with each chunk's
+// hit mask read from the record (hits[(2 r + c) n + i], written by the
+// gradient pass of the same half step on the same buffer, so the rows and
+// their candidate ranges are the same) instead of tested.  Every row of the
+// particle must have at most 64 candidates.
+template <bool FLUID_J, bool DUMMY_J, class F>
+__device__ __forceinline__ void for_each_recorded(const Prm &q, int i, float4 xi, uint32_t cell,
+                                                  const int *__restrict__ cs, const float4 *__restrict__ posr,
+                                                  const uint32_t *__restrict__ hits, size_t n, F &&f) {
+    for_each_row<FLUID_J, DUMMY_J>(q, xi, cell, cs, [&](int fb, int fe, int db, int de, int r) {
+        const int nfl = fe - fb;
+        const int total = nfl + (de - db);
+        for (int base = 0, c = 0; base < total; base += 32, ++c) {
+            uint32_t m = hits[(size_t)(2 * r + c) * n + (size_t)i];
+            while (m) {
+                const int u = base + __builtin_ctz(m);
+                m &= m - 1u;
+                const int j = (u < nfl) ? fb + u : db + (u - nfl);
+                const float4 xj = ld(posr, j);
+                const float dx = xi.x - xj.x, dy = xi.y - xj.y, dz = xi.z - xj.z;
+                f(j, dx, dy, dz, dist2(dx, dy, dz), xj);
+            }
+        }
+    });
 }
 
 // Nontemporal (streaming) 16-byte store and load.

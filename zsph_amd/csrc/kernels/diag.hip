@@ -164,7 +164,7 @@ __global__ __launch_bounds__(ZS_BLOCK) void k_filter(Prm qf, Buf b, const uint32
     const float ih = 1.0f / hc;
     float sw = 0, sr = 0, svx = 0, svy = 0, svz = 0, sax = 0, say = 0, saz = 0, sd = 0;
     const uint32_t cell = (skey[i] & (qf.flag - 1u)) - (uint32_t)qf.cell_base;
-    for_each_row<true, false>(qf, xi, cell, cs, [&](int fb, int fe, int, int) {
+    for_each_row<true, false>(qf, xi, cell, cs, [&](int fb, int fe, int, int, int) {
         for (int j = fb; j < fe; ++j) {
             const float4 xj = b.posr[j];
             const float r2 = dist2(xi.x - xj.x, xi.y - xj.y, xi.z - xj.z);

@@ -24,10 +24,12 @@ size_t cell_start_scratch(int G);
 void launch_cell_start(hipStream_t s, const uint32_t *skey, int n, int G, uint32_t flag, int *cs, int *scratch);
 void launch_reorder(hipStream_t s, int n, const int *perm, const Buf &a, const Buf &b);
 
-// physics.hip; walk = WALK_MASK, WALK_QUEUE or WALK_LIST (common.hpp).
+// physics.hip; walk = WALK_MASK, WALK_HITS or WALK_LIST (common.hpp).
 // WALK_LIST: launch_ldg records the neighbour lists (nlist: list_words(N)
-// ints, ncount: N ints) that launch_governing then reads; a caller without a
-// current list passes another walk to launch_governing.
+// ints, ncount: N ints) that launch_governing then reads.  WALK_HITS: the
+// same with the chunk hit masks (nlist: hits_slots(k) x N words; ncount:
+// overflow flags).  A caller without a current record passes WALK_MASK to
+// launch_governing.
 void launch_wall(hipStream_t s, int walk, const Prm &q, const Prm *sp, const Buf &b, const uint32_t *skey,
                  const int *cs);
 // feat (one scene only; may be nullptr): the closure features, ZSPH_NF floats per particle.

@@ -137,6 +137,15 @@ __global__ __launch_bounds__(ZS_BLOCK) void k_ldg(Prm q0, const Prm *__restrict_
                                                      walk_body(j, dx, dy, dz, r2, xj);
                                                  });
         ncount[i] = lw.finish();
+    } else if (WALK == WALK_HITS) {  // the mask walk, recording each chunk's hit mask
+        uint32_t *hits = reinterpret_cast<uint32_t *>(nlist);
+        bool over = false;  // a row with more than 64 candidates: not recorded
+        for_each_neighbor<WALK_MASK, true, true>(q, i, xi, cell, cs, b.posr, qbuf, walk_body,
+                                                 [&](int r, int c, uint32_t m) {
+                                                     if (c < 2) hits[(size_t)(2 * r + c) * (size_t)q0.N + (size_t)i] = m;
+                                                     else over = true;
+                                                 });
+        ncount[i] = over ? 1 : 0;
     } else {
         for_each_neighbor<WALK, true, true>(q, i, xi, cell, cs, b.posr, qbuf, walk_body);
     }
@@ -258,8 +267,11 @@ __global__ __launch_bounds__(ZS_BLOCK) void k_governing(Prm q0, const Prm *__res
         body(j, dx, dy, dz, r2, xj, vj, gj);
     };
     const uint32_t cell = cell_of(q, skey[i]);
-    const int n = (WALK == WALK_LIST) ? ncount[i] : 0;
-    if (WALK == WALK_LIST && n <= LIST_MAX) {
+    const int n = (WALK == WALK_LIST || WALK == WALK_HITS) ? ncount[i] : 0;
+    if (WALK == WALK_HITS && n == 0) {
+        for_each_recorded<true, true>(q, i, xi, cell, cs, b.posr, reinterpret_cast<const uint32_t *>(nlist),
+                                      (size_t)q0.N, walk_body);
+    } else if (WALK == WALK_LIST && n <= LIST_MAX) {
         // R quads (4R recorded neighbours) per round: the round's list reads
         // and all its gathers are issued before its first interaction, so
         // they are in flight together (this pass is latency-bound; 8 per
@@ -292,8 +304,7 @@ __global__ __launch_bounds__(ZS_BLOCK) void k_governing(Prm q0, const Prm *__res
             }
         }
     } else {
-        for_each_neighbor<WALK == WALK_LIST ? WALK_MASK : WALK, true, true>(q, i, xi, cell, cs, b.posr, qbuf,
-                                                                          walk_body);
+        for_each_neighbor<WALK_MASK, true, true>(q, i, xi, cell, cs, b.posr, qbuf, walk_body);
     }
     if constexpr (MODE == 0) {
         acc[i] = make_float4(ax, ay, az + q.g, dr);
@@ -415,19 +426,15 @@ __global__ __launch_bounds__(ZS_BLOCK) void k_nbr_stats(Prm q0, const Prm *__res
 
 // Walks other than the queue use the mask walk where no list applies; a
 // batch (sp != nullptr) runs the per-scene-parameter instantiation.
+// Kernels that only ever take the mask walk (the walk variants differ in the
+// gradient and continuity+momentum passes alone).
 #define ZS_WALK_LAUNCH(kern, grid, ...)                                                         \
     do {                                                                                        \
-        if (sp) {                                                                               \
-            if (walk == WALK_QUEUE)                                                             \
-                kern<WALK_QUEUE, true><<<(grid), ZS_BLOCK, 0, s>>>(__VA_ARGS__);                \
-            else                                                                                \
-                kern<WALK_MASK, true><<<(grid), ZS_BLOCK, 0, s>>>(__VA_ARGS__);                 \
-        } else {                                                                                \
-            if (walk == WALK_QUEUE)                                                             \
-                kern<WALK_QUEUE, false><<<(grid), ZS_BLOCK, 0, s>>>(__VA_ARGS__);               \
-            else                                                                                \
-                kern<WALK_MASK, false><<<(grid), ZS_BLOCK, 0, s>>>(__VA_ARGS__);                \
-        }                                                                                       \
+        (void)walk;                                                                             \
+        if (sp)                                                                                 \
+            kern<WALK_MASK, true><<<(grid), ZS_BLOCK, 0, s>>>(__VA_ARGS__);                     \
+        else                                                                                    \
+            kern<WALK_MASK, false><<<(grid), ZS_BLOCK, 0, s>>>(__VA_ARGS__);                    \
     } while (0)
 
 void launch_wall(hipStream_t s, int walk, const Prm &q, const Prm *sp, const Buf &b, const uint32_t *skey,
@@ -454,9 +461,9 @@ void launch_ldg(hipStream_t s, int walk, const Prm &q, const Prm *sp, const Buf 
     if (walk == WALK_LIST) {
         if (sp) ldg_walk<WALK_LIST, true>(s, f, g, q, sp, b, skey, cs, gd, nlist, ncount, feat);
         else ldg_walk<WALK_LIST, false>(s, f, g, q, sp, b, skey, cs, gd, nlist, ncount, feat);
-    } else if (walk == WALK_QUEUE) {
-        if (sp) ldg_walk<WALK_QUEUE, true>(s, f, g, q, sp, b, skey, cs, gd, nlist, ncount, feat);
-        else ldg_walk<WALK_QUEUE, false>(s, f, g, q, sp, b, skey, cs, gd, nlist, ncount, feat);
+    } else if (walk == WALK_HITS) {
+        if (sp) ldg_walk<WALK_HITS, true>(s, f, g, q, sp, b, skey, cs, gd, nlist, ncount, feat);
+        else ldg_walk<WALK_HITS, false>(s, f, g, q, sp, b, skey, cs, gd, nlist, ncount, feat);
     } else {
         if (sp) ldg_walk<WALK_MASK, true>(s, f, g, q, sp, b, skey, cs, gd, nlist, ncount, feat);
         else ldg_walk<WALK_MASK, false>(s, f, g, q, sp, b, skey, cs, gd, nlist, ncount, feat);
@@ -491,9 +498,9 @@ void launch_governing(hipStream_t s, int walk, const Prm &q, const Prm *sp, cons
     if (walk == WALK_LIST) {
         if (sp) gov_walk<WALK_LIST, true>(s, mode, g, q, sp, b, skey, cs, gd, acc, nlist, ncount, scale, terms);
         else gov_walk<WALK_LIST, false>(s, mode, g, q, sp, b, skey, cs, gd, acc, nlist, ncount, scale, terms);
-    } else if (walk == WALK_QUEUE) {
-        if (sp) gov_walk<WALK_QUEUE, true>(s, mode, g, q, sp, b, skey, cs, gd, acc, nlist, ncount, scale, terms);
-        else gov_walk<WALK_QUEUE, false>(s, mode, g, q, sp, b, skey, cs, gd, acc, nlist, ncount, scale, terms);
+    } else if (walk == WALK_HITS) {
+        if (sp) gov_walk<WALK_HITS, true>(s, mode, g, q, sp, b, skey, cs, gd, acc, nlist, ncount, scale, terms);
+        else gov_walk<WALK_HITS, false>(s, mode, g, q, sp, b, skey, cs, gd, acc, nlist, ncount, scale, terms);
     } else {
         if (sp) gov_walk<WALK_MASK, true>(s, mode, g, q, sp, b, skey, cs, gd, acc, nlist, ncount, scale, terms);
         else gov_walk<WALK_MASK, false>(s, mode, g, q, sp, b, skey, cs, gd, acc, nlist, ncount, scale, terms);

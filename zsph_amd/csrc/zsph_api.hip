@@ -39,8 +39,9 @@ struct zsph_ctx {
     int radix_bits = 8;
     int walk = zs::WALK_MASK;  // neighbour-walk variant (tuning)
     int *nlist = nullptr;      // WALK_LIST neighbour lists (list_words(N) ints) and counts
-    int *ncount = nullptr;
-    bool list_valid = false;   // the lists describe the current sorted buffer
+    int *ncount = nullptr;     // WALK_LIST counts / WALK_HITS overflow flags
+    int *hits = nullptr;       // WALK_HITS hit masks (hits_slots(k) x N words)
+    bool list_valid = false;   // the lists / hit masks describe the current sorted buffer
     int nscenes = 1;                  // > 1: a batch of independent scenes
     zs::Prm *scene_prm = nullptr;     // per-scene parameters (device, batches only)
     std::vector<zs::Prm> scene_host;  // the same on the host
@@ -182,6 +183,10 @@ int stage_sort_data(zsph_ctx *c, int from, int to, Prof *pf = nullptr) {
     c->cells_buf = to;
     return check_launch(c, "sort_data");
 }
+// The record a walk's gradient pass writes and its continuity+momentum pass reads.
+int *walk_record(zsph_ctx *c) { return c->walk == zs::WALK_HITS ? c->hits : c->nlist; }
+bool walk_records(int walk) { return walk == zs::WALK_LIST || walk == zs::WALK_HITS; }
+
 int stage_boundary(zsph_ctx *c, int b, Prof *pf = nullptr) {
     if (c->sorted_slot < 0) return set_err(c, ZSPH_E_STATE, "zsph_boundary called before zsph_sort_data");
     const uint32_t *sk = c->key[c->sorted_slot];
@@ -191,23 +196,23 @@ int stage_boundary(zsph_ctx *c, int b, Prof *pf = nullptr) {
     }
     {
         Scope sc(pf, ZSPH_ST_LDG);
-        zs::launch_ldg(c->stream, c->walk, c->q, c->scene_prm, c->buf[b], sk, c->cell_start, c->gd, c->nlist,
-                       c->ncount, c->closure_on ? c->feat : nullptr);
+        zs::launch_ldg(c->stream, c->walk, c->q, c->scene_prm, c->buf[b], sk, c->cell_start, c->gd,
+                       walk_record(c), c->ncount, c->closure_on ? c->feat : nullptr);
     }
-    c->list_valid = (c->walk == zs::WALK_LIST);
+    c->list_valid = walk_records(c->walk);
     return check_launch(c, "boundary");
 }
 int stage_governing(zsph_ctx *c, int b, Prof *pf = nullptr) {
     if (c->sorted_slot < 0) return set_err(c, ZSPH_E_STATE, "zsph_governing called before zsph_sort_data");
     {
         Scope sc(pf, ZSPH_ST_GOVERNING);
-        // the list walk needs this half step's lists; otherwise walk the cells
-        const int walk = (c->walk == zs::WALK_LIST && !c->list_valid) ? zs::WALK_MASK : c->walk;
+        // the recording walks need this half step's record; otherwise walk the cells
+        const int walk = (walk_records(c->walk) && !c->list_valid) ? zs::WALK_MASK : c->walk;
         if (c->closure_on)  // scale factors of the fluid particles [0, cs[G])
             zs::launch_mlp_infer(c->stream, c->mlp_P, c->mlp_pack, c->mlp_aux, c->feat, 0, c->cell_start + c->q.G,
                                  c->scale, zs::mlp_blocks(c->q.N));
         zs::launch_governing(c->stream, walk, c->q, c->scene_prm, c->buf[b], c->key[c->sorted_slot], c->cell_start,
-                             c->gd, c->acc, c->nlist, c->ncount, c->closure_on ? 1 : 0, c->scale, c->terms,
+                             c->gd, c->acc, walk_record(c), c->ncount, c->closure_on ? 1 : 0, c->scale, c->terms,
                              c->clock_on ? c->clock : nullptr);
     }
     return check_launch(c, "governing");
@@ -559,10 +564,13 @@ int zsph_set_tuning(zsph_ctx *c, const zsph_tuning *t) {
     if (t->walk < ZSPH_WALK_MASK || t->walk > ZSPH_WALK_LIST || t->radix_bits < 8 || t->radix_bits > 11)
         return set_err(c, ZSPH_E_ARG, "zsph_set_tuning: walk %d / radix_bits %d out of range", t->walk, t->radix_bits);
     if (t->walk == c->walk && t->radix_bits == c->radix_bits) return ZSPH_OK;
-    if (t->walk == ZSPH_WALK_LIST && !c->nlist) {
-        int r;
-        if ((r = dalloc(c, &c->nlist, zs::list_words(c->q.N))) || (r = dalloc(c, &c->ncount, (size_t)c->q.N)))
-            return r;
+    int r;
+    if (walk_records(t->walk) && !c->ncount && (r = dalloc(c, &c->ncount, (size_t)c->q.N))) return r;
+    if (t->walk == ZSPH_WALK_LIST && !c->nlist && (r = dalloc(c, &c->nlist, zs::list_words(c->q.N)))) return r;
+    if (t->walk == ZSPH_WALK_HITS && !c->hits) {
+        int k = c->q.k;  // the widest search of any scene
+        for (const zs::Prm &p : c->scene_host) k = std::max(k, p.k);
+        if ((r = dalloc(c, &c->hits, (size_t)zs::hits_slots(k) * (size_t)c->q.N))) return r;
     }
     c->list_valid = false;
     c->walk = t->walk;
@@ -917,7 +925,7 @@ int zsph_closure_sample(zsph_ctx *c, int b, float *features, float *terms) {
     // zero rows for the dummies, which the kernels do not write
     HIPCHK(c, hipMemsetAsync(c->feat, 0, N * ZSPH_NF * sizeof(float), c->stream));
     HIPCHK(c, hipMemsetAsync(c->terms, 0, N * 8 * sizeof(float), c->stream));
-    const int walk = c->walk == ZSPH_WALK_LIST ? ZSPH_WALK_MASK : c->walk;
+    const int walk = ZSPH_WALK_MASK;  // every walk visits the same neighbours in the same order
     zs::launch_wall(c->stream, walk, c->q, nullptr, c->buf[b], sk, c->cell_start);
     zs::launch_ldg(c->stream, walk, c->q, nullptr, c->buf[b], sk, c->cell_start, c->gd, c->nlist, c->ncount, c->feat);
     zs::launch_governing(c->stream, walk, c->q, nullptr, c->buf[b], sk, c->cell_start, c->gd, c->acc, c->nlist,
