@@ -38,11 +38,15 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
 
 # Measured HBM bytes of one continuity+momentum launch on the default 3-D dam
-# break (dx 0.008, cell edge h), per neighbour walk (0 mask, 2 list):
+# break (dx 0.008, cell edge h), per neighbour walk (0 mask, 1 hit masks, 2 list):
 # rocprofv3 FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0 (KiB per dispatch; the factors
 # make a 512 MiB device copy read 512 MiB and write 512 MiB).  Other workloads
 # report no measured traffic.
-GOVERNING_TRAFFIC_BYTES = {("dambreak3d", 0.008, 2.0, 0): (80486.6 * 2.0 + 20214.8) * 1024.0}
+GOVERNING_TRAFFIC_BYTES = {
+    ("dambreak3d", 0.008, 2.0, 0): (80419.6 * 2.0 + 20214.8) * 1024.0,
+    ("dambreak3d", 0.008, 2.0, 1): (162844.2 * 2.0 + 20214.8) * 1024.0,
+    ("dambreak3d", 0.008, 2.0, 2): (299856.1 * 2.0 + 20214.8) * 1024.0,
+}
 
 
 def step_bytes(n, G, n_fluid):
