@@ -216,10 +216,27 @@ def test_graph_replay_is_deterministic_and_equals_stages():
         assert np.array_equal(a, b)
 
 
-@pytest.mark.parametrize("name", ["2d", "3d-k2"])
+def max_row_window(c):
+    """Largest particle count (all types) in 3 adjacent x-cells of one cell row: a particle in
+    the middle cell of such a window has a row of that many candidates when k = 1."""
+    p = c.params
+    x = c.host.pos
+    cx = ((x[:, 0] - p.grid_xmin) / p.grid_size).astype(int)
+    cy = ((x[:, 1] - p.grid_ymin) / p.grid_size).astype(int)
+    cz = ((x[:, 2] - p.grid_zmin) / p.grid_size).astype(int)
+    cnt = np.zeros((p.grid_zdim, p.grid_ydim, p.grid_xdim + 2), int)
+    np.add.at(cnt, (cz, cy, cx + 1), 1)
+    return int((cnt[:, :, :-2] + cnt[:, :, 1:-1] + cnt[:, :, 2:]).max())
+
+
+@pytest.mark.parametrize("name", ["2d", "3d-k2", "3d-small"])
 def test_results_do_not_depend_on_tuning(name):
     c = CASES[name]()
     say(f"[case tuning {name}] N={c.n}")
+    if name == "3d-small":
+        # cells of edge 2h (k = 1) hold ~18 particles: rows near the walls exceed the 64
+        # candidates the hit-mask record covers, so its fallback to testing runs here too
+        assert c.params.grid_factor == 1 and max_row_window(c) > 64
     ref = None
     for walk in (z._abi.WALK_MASK, z._abi.WALK_HITS, z._abi.WALK_LIST):
         for bits in (8, 11):
