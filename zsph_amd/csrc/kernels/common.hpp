@@ -306,6 +306,9 @@ __device__ __forceinline__ void for_each_neighbor(const Prm &q, int i, float4 xi
                 js[e] = (u < nfl) ? fb + u : db + (u - nfl);
                 xs[e] = ld_xyz(posr, js[e]);
             }
+            // keep all four loads ahead of the tests (the scheduler would
+            // otherwise interleave them and wait after two)
+            __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
             for (int e = 0; e < 4; ++e) {
                 const bool h =
