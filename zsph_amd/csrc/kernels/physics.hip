@@ -228,6 +228,7 @@ __global__ __launch_bounds__(ZS_BLOCK) void k_governing(Prm q0, const Prm *__res
     const float4 vi = b.velp[i];
     const float4 gi = gd[i];
     const float rhoi = xi.w, pi = vi.w;
+    const float irhoi = hw_rcp(rhoi);  // 1/(rho_i rho_j) = irhoi irhoj: one reciprocal per pair
     float ax = 0, ay = 0, az = 0, dr = 0;
     float dd = 0, vx = 0, vy = 0, vz = 0;  // diffusion and viscous parts (MODE != 0)
     // gj: gd[j] (read only for fluid j)
@@ -236,10 +237,11 @@ __global__ __launch_bounds__(ZS_BLOCK) void k_governing(Prm q0, const Prm *__res
         const float t = 1.0f - 0.5f * qq;
         const float F = q.fk * t * t * t;
         const float rhoj = xj.w, pj = vj.w;
-        const float Vj = q.m * hw_rcp(rhoj);
+        const float irhoj = hw_rcp(rhoj);
+        const float Vj = q.m * irhoj;
         const float dvdx = (vi.x - vj.x) * dx + (vi.y - vj.y) * dy + (vi.z - vj.z) * dz;
         dr += rhoi * dvdx * F * Vj;
-        const float irr = hw_rcp(rhoi * rhoj);
+        const float irr = irhoi * irhoj;
         float pt = -(pi + pj) * q.m * F * irr;
         float pv = 0.0f;
         if (j < nf) {
