@@ -89,7 +89,9 @@ struct MlpAux {
     float out_scale, w_rho, w_acc;
 };
 size_t mlp_pack_elems();          // BF16 elements of the packed weight image
-int mlp_blocks(int n);            // blocks of a training launch over n samples (partials buffer size)
+int mlp_blocks(int n);
+int mlp_infer_blocks(int n);  // blocks for launch_mlp_infer over n samples
+           // blocks of a training launch over n samples (partials buffer size)
 hipError_t mlp_prepare();         // allows the kernels' dynamic LDS (call once per device)
 void launch_mlp_pack(hipStream_t s, const float *P, void *pack);
 // n samples, or *n_dev when n_dev is given; out: n x 2 scale factors.

@@ -61,7 +61,11 @@ constexpr int A_X = 0;                  // [32][LDI] normalised features
 constexpr int A_H1 = A_X + MT * LDI;    // [32][LDH]
 constexpr int A_H2 = A_H1 + MT * LDH;
 constexpr int A_H3 = A_H2 + MT * LDH;
-constexpr int A_FWD = A_H3 + MT * LDH;  // inference needs only these
+constexpr int A_FWD = A_H3 + MT * LDH;  // the forward buffers training keeps
+// Inference keeps no activations for a backward pass: the third hidden layer
+// reuses the first's buffer (read for the last time by the second layer's
+// product), so a block needs only X, H1 and H2.
+constexpr int A_INF = A_H2 + MT * LDH;
 constexpr int A_XT = A_FWD;             // [32 in][LDI] (32 samples)
 constexpr int A_H1T = A_XT + 32 * LDI;  // [64][LDI]
 constexpr int A_H2T = A_H1T + 64 * LDI;
@@ -75,7 +79,7 @@ constexpr int A_ALL = A_DZT + 64 * LDI;
 template <bool TRAIN>
 constexpr size_t lds_bytes() {
     return TRAIN ? (size_t)(PACK_ALL + A_ALL) * 2 + (size_t)(MLP_NPARAM + MT + 8) * 4
-                 : (size_t)(PACK_FWD + A_FWD) * 2 + (size_t)(MT + 8) * 4;
+                 : (size_t)(PACK_FWD + A_INF) * 2 + (size_t)(MT + 8) * 4;
 }
 
 __device__ __forceinline__ int lane() { return (int)(threadIdx.x & 63); }
@@ -206,7 +210,7 @@ __global__ __launch_bounds__(64) void k_mlp(const float *__restrict__ P, const b
     constexpr int PACK = TRAIN ? PACK_ALL : PACK_FWD;
     bf16 *W = reinterpret_cast<bf16 *>(smem);
     bf16 *act = W + PACK;
-    float *dW = reinterpret_cast<float *>(act + (TRAIN ? A_ALL : A_FWD));
+    float *dW = reinterpret_cast<float *>(act + (TRAIN ? A_ALL : A_INF));
     int *keep = reinterpret_cast<int *>(dW + (TRAIN ? MLP_NPARAM : 0));  // per sample: inside the range
     float *lsum = reinterpret_cast<float *>(keep + MT);
     const int l = lane();
@@ -215,13 +219,13 @@ __global__ __launch_bounds__(64) void k_mlp(const float *__restrict__ P, const b
     // weights (16-byte copies), zeroed activations and gradient accumulators
     for (int e = 8 * l; e < PACK; e += 8 * 64)
         *reinterpret_cast<bf16x8 *>(W + e) = *reinterpret_cast<const bf16x8 *>(pack + e);
-    for (int e = l; e < (TRAIN ? A_ALL : A_FWD); e += 64) act[e] = (bf16)0.0f;
+    for (int e = l; e < (TRAIN ? A_ALL : A_INF); e += 64) act[e] = (bf16)0.0f;
     if (TRAIN)
         for (int e = l; e < MLP_NPARAM; e += 64) dW[e] = 0.0f;
     if (l == 0) lsum[0] = 0.0f;
     __syncthreads();
 
-    bf16 *X = act + A_X, *H1 = act + A_H1, *H2 = act + A_H2, *H3 = act + A_H3;
+    bf16 *X = act + A_X, *H1 = act + A_H1, *H2 = act + A_H2, *H3 = act + (TRAIN ? A_H3 : A_H1);
     bf16 *XT = TRAIN ? act + A_XT : nullptr, *H1T = TRAIN ? act + A_H1T : nullptr;
     bf16 *H2T = TRAIN ? act + A_H2T : nullptr, *H3T = TRAIN ? act + A_H3T : nullptr;
     const float os = aux->out_scale;
@@ -409,6 +413,9 @@ __global__ __launch_bounds__(ZS_BLOCK) void k_mlp_pack(const float *__restrict__
 size_t mlp_pack_elems() { return mlp::PACK_ALL; }
 
 int mlp_blocks(int n) { return std::max(1, std::min(256, (n + mlp::MT - 1) / mlp::MT)); }
+// Inference has no cross-block reduction: enough one-wave blocks for several
+// per CU (each sample's result does not depend on which block computes it).
+int mlp_infer_blocks(int n) { return std::max(1, std::min(2048, (n + mlp::MT - 1) / mlp::MT)); }
 
 hipError_t mlp_prepare() {
     hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(&k_mlp<true>),

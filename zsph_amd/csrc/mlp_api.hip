@@ -218,7 +218,7 @@ int zsph_mlp_infer(zsph_mlp *m, const float *x, int n, float *s_out) {
     int r = upload(m, dx, x, (size_t)n * ZSPH_NF * sizeof(float));
     if (!r) {
         zs::launch_mlp_infer(m->stream, m->P, m->pack, m->aux, dx, n, nullptr, reinterpret_cast<float2 *>(ds),
-                             zs::mlp_blocks(n));
+                             zs::mlp_infer_blocks(n));
         if (hipGetLastError() != hipSuccess) r = fail(m, ZSPH_E_HIP, "zsph_mlp_infer: launch failed");
     }
     if (!r) r = download(m, s_out, ds, (size_t)n * 2 * sizeof(float));

@@ -210,7 +210,7 @@ int stage_governing(zsph_ctx *c, int b, Prof *pf = nullptr) {
         const int walk = (walk_records(c->walk) && !c->list_valid) ? zs::WALK_MASK : c->walk;
         if (c->closure_on)  // scale factors of the fluid particles [0, cs[G])
             zs::launch_mlp_infer(c->stream, c->mlp_P, c->mlp_pack, c->mlp_aux, c->feat, 0, c->cell_start + c->q.G,
-                                 c->scale, zs::mlp_blocks(c->q.N));
+                                 c->scale, zs::mlp_infer_blocks(c->q.N));
         zs::launch_governing(c->stream, walk, c->q, c->scene_prm, c->buf[b], c->key[c->sorted_slot], c->cell_start,
                              c->gd, c->acc, walk_record(c), c->ncount, c->closure_on ? 1 : 0, c->scale, c->terms,
                              c->clock_on ? c->clock : nullptr);
