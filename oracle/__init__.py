@@ -10,7 +10,9 @@ import os
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_HERE, "liboracle.so")
+# ZSPH_ORACLE_LIB: another build of the same library (the sanitizer build of
+# tests/test_sanitizers.py)
+LIB_PATH = os.environ.get("ZSPH_ORACLE_LIB") or os.path.join(_HERE, "liboracle.so")
 _LIB = None
 
 A_ACC_SCALE = 100   # per particle (sum |acc terms|, sum |drho terms|)

@@ -9,6 +9,7 @@
 // (host code), the device side gathers and scatters the arrays.
 #include <cerrno>
 #include <cinttypes>
+#include <cmath>
 #include <cstdio>
 #include <cstring>
 #include <string>
@@ -140,53 +141,70 @@ int zsph_ckpt_read(const char *path, CkptMeta *m, std::vector<CkptSection> *secs
         std::fclose(f);
         return io_fail(err, std::string(path) + ": unsupported checkpoint version or byte order");
     }
-    auto geti = [&](const char *k, bool *ok) {
+    // every field must be present; integers must be finite, integral and in
+    // range before they are converted (a damaged header is refused, never cast)
+    bool ok = true;
+    auto getd = [&](const char *k) {
         auto it = num.find(k);
-        if (it == num.end()) {
-            *ok = false;
+        if (it == num.end() || !std::isfinite(it->second)) {
+            ok = false;
             return 0.0;
         }
         return it->second;
     };
-    bool ok = true;
-    m->step = (long long)geti("step", &ok);
-    m->time = geti("time", &ok);
-    m->t0 = geti("t0", &ok);
-    m->ptc_num = (int)geti("ptc_num", &ok);
-    m->cells = (int)geti("cells", &ok);
-    m->scenes = (int)geti("scenes", &ok);
-    m->dim = (int)geti("dim", &ok);
-    m->grid[0] = (int)geti("grid_x", &ok);
-    m->grid[1] = (int)geti("grid_y", &ok);
-    m->grid[2] = (int)geti("grid_z", &ok);
-    m->killed = (int)geti("killed", &ok);
-    m->has_body = (int)geti("has_body", &ok);
-    m->has_piston = (int)geti("has_piston", &ok);
-    m->closure = (int)geti("closure", &ok);
-    const double total = geti("payload_bytes", &ok);
+    auto geti = [&](const char *k, double lo, double hi) -> long long {
+        const double v = getd(k);
+        if (!(v >= lo && v <= hi && v == std::floor(v))) {
+            ok = false;
+            return 0;
+        }
+        return (long long)v;
+    };
+    constexpr double IMAX = 2147483647.0, BYTES_MAX = 1125899906842624.0;  // 2^31 - 1, 2^50
+    m->step = geti("step", 0.0, 9007199254740992.0);
+    m->time = getd("time");
+    m->t0 = getd("t0");
+    m->ptc_num = (int)geti("ptc_num", 0.0, IMAX);
+    m->cells = (int)geti("cells", 0.0, IMAX);
+    m->scenes = (int)geti("scenes", 1.0, IMAX);
+    m->dim = (int)geti("dim", 2.0, 3.0);
+    m->grid[0] = (int)geti("grid_x", 0.0, IMAX);
+    m->grid[1] = (int)geti("grid_y", 0.0, IMAX);
+    m->grid[2] = (int)geti("grid_z", 0.0, IMAX);
+    m->killed = (int)geti("killed", 0.0, IMAX);
+    m->has_body = (int)geti("has_body", 0.0, 1.0);
+    m->has_piston = (int)geti("has_piston", 0.0, 1.0);
+    m->closure = (int)geti("closure", 0.0, 1.0);
+    const size_t total = (size_t)geti("payload_bytes", 0.0, BYTES_MAX);
     uint64_t checksum = 0;
     ok = ok && parse_hex64(str["params"], &m->params) && parse_hex64(str["checksum"], &checksum);
     secs->clear();
-    double sum = 0.0;
+    size_t sum = 0;
     const std::string list = str["sections"];
-    for (size_t a = 0; ok && a < list.size();) {
+    for (size_t a = 0; ok && a < list.size() && secs->size() < 64;) {
         size_t b = list.find(',', a);
         if (b == std::string::npos) b = list.size();
         CkptSection c;
         c.name = list.substr(a, b - a);
-        const double nb = geti((c.name + "_bytes").c_str(), &ok);
-        ok = ok && !c.name.empty() && nb >= 0.0;
-        c.bytes = (size_t)nb;
-        sum += nb;
+        ok = !c.name.empty();
+        c.bytes = (size_t)geti((c.name + "_bytes").c_str(), 0.0, BYTES_MAX);
+        sum += c.bytes;  // at most 64 sections of < 2^50 bytes: no overflow
         secs->push_back(c);
         a = b + 1;
     }
-    if (!ok || sum != total) {
+    if (!ok || list.empty() || sum != total) {
         std::fclose(f);
         return io_fail(err, std::string(path) + ": incomplete checkpoint header");
     }
     if (payload) {
-        payload->resize((size_t)total);
+        // the payload must be there before memory is set aside for it
+        const long here = std::ftell(f);
+        if (std::fseek(f, 0, SEEK_END) != 0 || std::ftell(f) - here != (long)total ||
+            std::fseek(f, here, SEEK_SET) != 0) {
+            std::fclose(f);
+            return io_fail(err, std::string(path) + ": payload size differs from its header");
+        }
+        payload->resize(total);
         const bool got = std::fread(payload->data(), 1, payload->size(), f) == payload->size();
         const bool at_end = got && std::fgetc(f) == EOF;
         std::fclose(f);

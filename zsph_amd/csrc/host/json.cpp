@@ -4,6 +4,7 @@
 // parser: top-level object members with number / string / bool values are
 // kept, nested values are parsed and skipped.
 #include <cctype>
+#include <cmath>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -24,8 +25,10 @@ struct Val {
 };
 
 struct Parser {
+    static constexpr int MAX_DEPTH = 64;  // nesting limit: the recursion stays bounded on any input
     const std::string &s;
     size_t i = 0;
+    int depth = 0;
     bool ok = true;
     explicit Parser(const std::string &src) : s(src) {}
 
@@ -68,6 +71,7 @@ struct Parser {
         ws();
         if (i >= s.size()) return ok = false;
         char c = s[i];
+        if ((c == '{' || c == '[') && depth >= MAX_DEPTH) return ok = false;
         if (c == '"') {
             v->kind = Val::STR;
             return string(&v->str);
@@ -76,22 +80,26 @@ struct Parser {
             ++i;
             v->kind = Val::OTHER;
             if (eat('}')) return true;
+            ++depth;
             do {
                 if (!string(nullptr)) return false;
                 if (!eat(':')) return ok = false;
                 Val sub;
                 if (!value(&sub)) return false;
             } while (eat(','));
+            --depth;
             return eat('}') ? true : (ok = false);
         }
         if (c == '[') {
             ++i;
             v->kind = Val::OTHER;
             if (eat(']')) return true;
+            ++depth;
             do {
                 Val sub;
                 if (!value(&sub)) return false;
             } while (eat(','));
+            --depth;
             return eat(']') ? true : (ok = false);
         }
         if (s.compare(i, 4, "true") == 0) { i += 4; v->kind = Val::BOOL; v->num = 1; return true; }
@@ -147,15 +155,23 @@ extern "C" int zsph_read_json(const char *path, zsph_json *j) {
     if (!ps.object(&kv)) return ZSPH_E_IO;
 
     zsph_json_defaults(j);
+    // numbers outside the field's type are refused (their conversion is undefined)
+    bool bad = false;
     auto num = [&](const char *k, float *dst) {
         auto it = kv.find(k);
-        if (it != kv.end() && (it->second.kind == Val::NUM || it->second.kind == Val::BOOL))
-            *dst = (float)it->second.num;
+        if (it != kv.end() && (it->second.kind == Val::NUM || it->second.kind == Val::BOOL)) {
+            const double v = it->second.num;
+            if (std::isfinite(v) && std::fabs(v) > 3.4028234663852886e38) bad = true;
+            else *dst = (float)v;
+        }
     };
     auto inum = [&](const char *k, int *dst) {
         auto it = kv.find(k);
-        if (it != kv.end() && (it->second.kind == Val::NUM || it->second.kind == Val::BOOL))
-            *dst = (int)it->second.num;
+        if (it != kv.end() && (it->second.kind == Val::NUM || it->second.kind == Val::BOOL)) {
+            const double v = it->second.num;
+            if (!(v >= -2147483648.0 && v <= 2147483647.0)) bad = true;
+            else *dst = (int)v;
+        }
     };
     auto str = [&](const char *k, char *dst, size_t cap) {
         auto it = kv.find(k);
@@ -205,9 +221,9 @@ extern "C" int zsph_read_json(const char *path, zsph_json *j) {
             j->diffusion = (it->second.str == "MC" || it->second.str == "mc") ? ZSPH_DIFF_MC
                                                                                : ZSPH_DIFF_ANTUONO;
         else if (it->second.kind == Val::NUM)
-            j->diffusion = (int)it->second.num;
+            inum("diffusion", &j->diffusion);
     }
     inum("output_fields", &j->output_fields);
     inum("binary", &j->binary);
-    return ZSPH_OK;
+    return bad ? ZSPH_E_IO : ZSPH_OK;
 }

@@ -5,6 +5,7 @@
 // reader of io/read_vkt.cu:3-92 (fields p, rho, vel, acc, type, table;
 // missing p/rho/vel/acc default to 0 / rho0 / 0 / 0) and accepts float or
 // double arrays, so files written here restart without losing fields.
+#include <cmath>
 #include <cstdint>
 #include <cstdio>
 #include <cstdlib>
@@ -86,14 +87,15 @@ struct In {
         while (i < s.size() && s[i] != '\n') ++i;
         if (i < s.size()) ++i;
     }
-    // Read `count` values of VTK type `ty` as doubles.
+    // Read `count` values of VTK type `ty` as doubles.  The count is checked
+    // against what is left of the file before any memory is set aside.
     bool values(bool bin, const std::string &ty, size_t count, std::vector<double> *out) {
-        out->resize(count);
         if (bin) {
             size_t w = (ty == "double" || ty == "long" || ty == "unsigned_long") ? 8 : 4;
             if (ty == "short" || ty == "unsigned_short") w = 2;
             if (ty == "char" || ty == "unsigned_char" || ty == "bit") return false;
-            if (i + w * count > s.size()) return false;
+            if (count > (s.size() - i) / w) return false;
+            out->resize(count);
             for (size_t k = 0; k < count; ++k) {
                 const char *p = s.data() + i + w * k;
                 double v;
@@ -115,6 +117,8 @@ struct In {
             }
             i += w * count;
         } else {
+            if (count > (s.size() - i + 1) / 2) return false;  // each value takes a digit and a separator
+            out->resize(count);
             for (size_t k = 0; k < count; ++k) {
                 std::string t = tok();
                 if (t.empty()) return false;
@@ -126,6 +130,25 @@ struct In {
         return true;
     }
 };
+
+// A count token: a non-negative int, else -1.
+int count_of(const std::string &t) {
+    char *e = nullptr;
+    const long v = std::strtol(t.c_str(), &e, 10);
+    return (e == t.c_str() || *e || v < 0 || v > 2147483647L) ? -1 : (int)v;
+}
+// Conversions of file values that are undefined for out-of-range numbers
+// (float: finite values beyond FLT_MAX; int: anything outside int).
+bool as_float(double v, float *out) {
+    if (std::isfinite(v) && std::fabs(v) > 3.4028234663852886e38) return false;
+    *out = (float)v;
+    return true;
+}
+bool as_int(double v, int *out) {
+    if (!(v >= -2147483648.0 && v <= 2147483647.0)) return false;
+    *out = (int)v;
+    return true;
+}
 
 }  // namespace
 
@@ -202,26 +225,27 @@ int read_vtk_impl(const char *path, zsph_host_state *st, int n, float rho0, int 
         if (kw == "DATASET") {
             in.tok();
         } else if (kw == "POINTS") {
-            npts = std::atoi(in.tok().c_str());
+            npts = count_of(in.tok());
             std::string ty = in.tok();
             in.rest_of_line();
+            if (npts < 0) return ZSPH_E_IO;
             if (n_out) { *n_out = npts; return ZSPH_OK; }
             if (npts != n) return ZSPH_E_ARG;  // read_vkt.cu:14 assert
             if (!in.values(bin, ty, (size_t)npts * 3, &buf)) return ZSPH_E_IO;
             if (st->pos)
-                for (size_t k = 0; k < buf.size(); ++k) st->pos[k] = (float)buf[k];
+                for (size_t k = 0; k < buf.size(); ++k)
+                    if (!as_float(buf[k], &st->pos[k])) return ZSPH_E_IO;
         } else if (kw == "POINT_DATA" || kw == "CELL_DATA") {
             in.tok();
         } else if (kw == "CELLS") {
-            int nc = std::atoi(in.tok().c_str());
-            int sz = std::atoi(in.tok().c_str());
-            (void)nc;
+            const int nc = count_of(in.tok());
+            const int sz = count_of(in.tok());
             in.rest_of_line();
-            if (!in.values(bin, "int", (size_t)sz, &buf)) return ZSPH_E_IO;
+            if (nc < 0 || sz < 0 || !in.values(bin, "int", (size_t)sz, &buf)) return ZSPH_E_IO;
         } else if (kw == "CELL_TYPES") {
-            int nc = std::atoi(in.tok().c_str());
+            const int nc = count_of(in.tok());
             in.rest_of_line();
-            if (!in.values(bin, "int", (size_t)nc, &buf)) return ZSPH_E_IO;
+            if (nc < 0 || !in.values(bin, "int", (size_t)nc, &buf)) return ZSPH_E_IO;
         } else if (kw == "SCALARS" || kw == "VECTORS") {
             // "SCALARS name type [ncomp]" / "VECTORS name type" on one line
             std::istringstream hdr(in.line());
@@ -238,23 +262,29 @@ int read_vtk_impl(const char *path, zsph_host_state *st, int n, float rho0, int 
             if (npts < 0) return ZSPH_E_IO;
             if (!in.values(bin, ty, (size_t)npts * ncomp, &buf)) return ZSPH_E_IO;
             if (name == "p" && st->rhop) {
-                for (int k = 0; k < npts; ++k) st->rhop[2 * k + 1] = (float)buf[(size_t)k * ncomp];
+                for (int k = 0; k < npts; ++k)
+                    if (!as_float(buf[(size_t)k * ncomp], &st->rhop[2 * k + 1])) return ZSPH_E_IO;
                 have_p = true;
             } else if (name == "rho" && st->rhop) {
-                for (int k = 0; k < npts; ++k) st->rhop[2 * k] = (float)buf[(size_t)k * ncomp];
+                for (int k = 0; k < npts; ++k)
+                    if (!as_float(buf[(size_t)k * ncomp], &st->rhop[2 * k])) return ZSPH_E_IO;
                 have_rho = true;
             } else if (name == "vel" && st->vel && ncomp == 3) {
-                for (size_t k = 0; k < buf.size(); ++k) st->vel[k] = (float)buf[k];
+                for (size_t k = 0; k < buf.size(); ++k)
+                    if (!as_float(buf[k], &st->vel[k])) return ZSPH_E_IO;
                 have_vel = true;
             } else if (name == "acc" && st->acc_drhodt && ncomp == 3) {
                 for (int k = 0; k < npts; ++k)
-                    for (int c = 0; c < 3; ++c) st->acc_drhodt[4 * k + c] = (float)buf[3 * (size_t)k + c];
+                    for (int c = 0; c < 3; ++c)
+                        if (!as_float(buf[3 * (size_t)k + c], &st->acc_drhodt[4 * k + c])) return ZSPH_E_IO;
                 have_acc = true;
             } else if (name == "type" && st->type) {
-                for (int k = 0; k < npts; ++k) st->type[k] = (int)buf[(size_t)k * ncomp];
+                for (int k = 0; k < npts; ++k)
+                    if (!as_int(buf[(size_t)k * ncomp], &st->type[k])) return ZSPH_E_IO;
                 have_type = true;
             } else if (name == "table" && st->table) {
-                for (int k = 0; k < npts; ++k) st->table[k] = (int)buf[(size_t)k * ncomp];
+                for (int k = 0; k < npts; ++k)
+                    if (!as_int(buf[(size_t)k * ncomp], &st->table[k])) return ZSPH_E_IO;
                 have_table = true;
             }
         } else if (kw == "FIELD") {
