@@ -1,0 +1,193 @@
+"""GPU parity at the headline size: the 3-D dam break of 1,853,844 particles
+(dx 0.008 m, 3.22 x 1.0 x 1.0 m tank, 1.2 x 1.0 x 0.55 m column, cells of
+edge h), the workload the throughput is measured on.
+
+* One jittered state: the integer stages (cell hash, sort order, cell table,
+  gathers, neighbour counts) are bit-exact against the FP64 oracle, and the
+  wall, gradient and continuity+momentum passes are within the tolerances of
+  tests/helpers.py; every neighbour walk gives the same bits as the mask walk.
+* One whole step of that state: density, positions and accelerations within
+  the one-step tolerances of the oracle's; pressure within 1e-5 of the FP64
+  EOS of the stepped density (at this size a few of the 1.3 million
+  densities differ from the oracle's by 4-5 float32 ulps, which the stiff EOS
+  turns into 1.5e-5 of rho0 g H).
+* The unperturbed case is symmetric about the tank's mid-plane y = wy / 2:
+  after 1,000 steps every particle sits at its mirror partner's mirrored
+  position within 0.02 dx (the float32 rounding of the two halves differs, so
+  the error grows slowly: 0.003 dx measured at step 1,000), no particle has
+  been removed and the fluid count is unchanged.  A pair term with a wrong
+  sign or index in one direction breaks the symmetry at the first step.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+import oracle
+import zsph_amd as z
+from oracle import A_ACC_SCALE, A_GD_SCALE
+from zsph_amd import A_ACC, A_CELL_START, A_GD, A_INDEX, A_KEY, A_POSR, A_TABLE, A_TYPE, A_VELP, FLUID, cases
+from tests.helpers import assert_close_scaled, jitter, p_scale
+from tests.test_gpu_parity import by_original_id
+
+pytestmark = pytest.mark.gpu
+
+DX = 0.008
+WY = 1.0
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def headline(perturb):
+    c = cases.dambreak3d(dx=DX, wy=WY, grid_size_factor=2.0)
+    assert c.n == 1853844
+    return jitter(c, seed=2024) if perturb else c
+
+
+def stages(x):
+    x.hash(0)
+    x.sort_index()
+    x.sort_data(0, 1)
+
+
+def test_stages_at_full_size_every_walk():
+    c = headline(True)
+    say(f"[full size stages] N={c.n} grid={c.cpu.grid_xdim}x{c.cpu.grid_ydim}x{c.cpu.grid_zdim}")
+    s = z.Solver(c)
+    s.upload(c.host)
+    o = oracle.Oracle(c.params)
+    o.upload(c.host)
+    s.set_tuning(z.WALK_MASK, 8)
+    for x in (s, o):
+        stages(x)
+    assert np.array_equal(s.get(A_KEY), o.get(A_KEY).astype(np.uint32))
+    assert np.array_equal(s.get(A_INDEX), o.get(A_INDEX).astype(np.int32))
+    assert np.array_equal(s.get(A_CELL_START), o.get(A_CELL_START).astype(np.int32))
+    for a in (A_POSR, A_VELP):
+        assert np.array_equal(s.get(a, 1), o.get(a, 1).astype(np.float32)), a
+    for a in (A_TYPE, A_TABLE):
+        assert np.array_equal(s.get(a, 1), o.get(a, 1).astype(np.int32)), a
+    cg, dg = s.neighbor_stats(1)
+    co, do = o.neighbor_stats(1)
+    assert np.array_equal(cg, co) and np.array_equal(dg, do)
+    say(f"  neighbours: mean {cg.mean():.1f} max {cg.max()}")
+
+    # wall pass + gradient pass, then continuity+momentum on the GPU's inputs
+    s.boundary(1)
+    o.wall(1)
+    typ = o.get(A_TYPE, 1)
+    wall = typ != FLUID
+    f = ~wall
+    gp, op = s.get(A_POSR, 1), o.get(A_POSR, 1)
+    gv, ov = s.get(A_VELP, 1), o.get(A_VELP, 1)
+    ps = p_scale(c)
+    assert_close_scaled(gv[wall, 3], ov[wall, 3], np.maximum(np.abs(ov[wall, 3]), ps), "wall p")
+    assert_close_scaled(gp[wall, 3], op[wall, 3], np.abs(op[wall, 3]), "wall rho")
+    o.set(A_POSR, gp, 1)
+    o.set(A_VELP, gv, 1)
+    o.ldg(1)
+    gg, og = s.get(A_GD), o.get(A_GD)
+    sc = o.get(A_GD_SCALE)
+    gsc = np.maximum(sc[:, 0:1], np.abs(og[:, :3]).max(axis=1, keepdims=True))
+    assert_close_scaled(gg[f, :3], og[f, :3], np.broadcast_to(gsc[f], gg[f, :3].shape), "grad rho^L", rel=2e-5)
+    assert_close_scaled(gg[f, 3], og[f, 3], sc[f, 1], "div v")
+    o.set(A_GD, gg)
+    s.governing(1)
+    o.governing(1)
+    ga, oa = s.get(A_ACC), o.get(A_ACC)
+    asc = o.get(A_ACC_SCALE)
+    err_a = assert_close_scaled(ga[f, :3], oa[f, :3], np.broadcast_to(asc[f, 0:1], ga[f, :3].shape), "acc")
+    err_d = assert_close_scaled(ga[f, 3], oa[f, 3], asc[f, 1], "drho/dt")
+    say(f"  mask walk: max scaled err acc {err_a:.2e} drho {err_d:.2e}")
+    o.close()
+
+    # the recorded walks (hit masks, neighbour lists) from the same state
+    ref = (gp, gv, gg, ga)
+    for walk in (z.WALK_HITS, z.WALK_LIST):
+        s.set_tuning(walk, 8)
+        stages(s)
+        s.boundary(1)
+        s.governing(1)
+        got = (s.get(A_POSR, 1), s.get(A_VELP, 1), s.get(A_GD), s.get(A_ACC))
+        for a, b, what in zip(ref, got, ("posr", "velp", "gd", "acc")):
+            assert np.array_equal(a, b), f"walk {walk}: {what} differs from the mask walk"
+        say(f"  walk {walk}: bit-identical to the mask walk")
+    s.close()
+
+
+def test_one_step_at_full_size():
+    c = headline(True)
+    say(f"[full size step] N={c.n}")
+    s = z.Solver(c)
+    s.upload(c.host)
+    s.step(1)
+    s.sync()
+    o = oracle.Oracle(c.params)
+    o.upload(c.host)
+    o.step(1)
+    g, r = s.download(0), o.download(0)
+    og, orr = by_original_id(g), by_original_id(r)
+    f = r.type[orr] == FLUID
+    assert np.array_equal(g.type[og], r.type[orr])
+    rho_g, rho_r = g.rhop[og, 0][f], r.rhop[orr, 0][f]
+    assert_close_scaled(rho_g, rho_r, np.abs(rho_r), "rho after one step")
+    # p is the EOS of the state's density: one ulp of rho (6e-5 kg/m^3) moves
+    # p by c0^2 ulp ~ 0.03 Pa, so p is held to the FP64 EOS of the GPU's own
+    # density (the density itself to the oracle's, above)
+    p_of_rho = np.fromiter((oracle.eos_p(c.params, float(x)) for x in rho_g), np.float64, rho_g.size)
+    assert_close_scaled(g.rhop[og, 1][f], p_of_rho, np.maximum(np.abs(p_of_rho), p_scale(c)), "p = EOS(rho)")
+    assert_close_scaled(g.pos[og][f], r.pos[orr][f], np.abs(r.pos[orr][f]) + DX, "x after one step")
+    sc = o.get(A_ACC_SCALE)[orr, 0][f]
+    assert_close_scaled(g.acc[og, :3][f], r.acc[orr, :3][f], np.broadcast_to(sc[:, None], (f.sum(), 3)),
+                        "acc after one step", rel=3e-5)
+    assert s.killed() == o.killed() == 0
+    o.close()
+    s.close()
+
+
+def lattice_key(p):
+    """One int64 per particle from its position rounded to half the spacing
+    (fluid and dummy particles sit at odd multiples of dx/2)."""
+    k = np.round(np.asarray(p, np.float64) / (DX / 2)).astype(np.int64) + (1 << 20)
+    assert k.min() >= 0 and k.max() < (1 << 21)
+    return (k[:, 0] << 42) | (k[:, 1] << 21) | k[:, 2]
+
+
+def mirror(p):
+    return np.asarray(p, np.float64) * [1.0, -1.0, 1.0] + [0.0, WY, 0.0]
+
+
+def test_mirror_symmetry_over_1000_steps():
+    c = headline(False)
+    p0 = c.host.pos
+    k0 = lattice_key(p0)
+    order = np.argsort(k0)
+    km = lattice_key(mirror(p0))
+    at = np.minimum(np.searchsorted(k0, km, sorter=order), c.n - 1)
+    partner = order[at]
+    assert np.array_equal(k0[partner], km), "initial lattice is not symmetric about y = wy / 2"
+    assert np.array_equal(c.host.type[partner], c.host.type)
+    n_fluid = int((c.host.type == FLUID).sum())
+    say(f"[full size mirror] N={c.n} fluid={n_fluid}")
+    s = z.Solver(c)
+    s.upload(c.host)
+    done = 0
+    for target in (200, 1000):
+        s.step(target - done)
+        s.sync()
+        done = target
+        h = s.download(0)
+        orig = np.arange(c.n) + h.table.astype(np.int64)
+        pos = np.empty((c.n, 3), np.float64)
+        pos[orig] = h.pos
+        typ = np.empty(c.n, h.type.dtype)
+        typ[orig] = h.type
+        err = np.abs(pos - mirror(pos[partner])).max(axis=0)
+        say(f"  step {target}: max mirror error / dx {err / DX}")
+        assert s.killed() == 0
+        assert int((typ == FLUID).sum()) == n_fluid
+        assert err.max() < 0.02 * DX
+    s.close()
