@@ -1,0 +1,256 @@
+"""GPU edge cases of the step against the FP64 oracle, for every neighbour walk.
+
+* No fluid left: every fluid particle lies outside the grid, so the first
+  cell hash removes all of them. The removed count is the fluid count; the
+  dummies keep their places, with p = 0 and rho = rho0 (no fluid neighbours);
+  the state equals the oracle's bit for bit.
+* One fluid particle, in free fall far from the walls. Every wave holds at
+  most one active lane. Position and velocity follow the oracle's and
+  x0 + v0 t + g t^2 / 2.
+* A dense cluster: 400 fluid particles inside one cube of edge h/2. Rows then
+  hold far more than 64 candidates, so the hit-mask record overflows, and
+  particles have far more than 128 neighbours, so the list overflows: every
+  walk takes its fallback. Hash, sort and neighbour sets are bit-exact; the
+  passes are within the tolerances of tests/helpers.py; the walks give the
+  same bits over one step (both half steps).
+* Particles on and beside every cell face: the cell hash equals the
+  oracle's, i.e. the floor of the correctly rounded FP32 quotient
+  (x - o) / size, also where that quotient rounds up to an integer from an
+  exact value below it.
+The particle counts (5,730, 891 fluid) are not multiples of the wave (64) or
+block (256) size.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+import oracle
+import zsph_amd as z
+from oracle import A_ACC_SCALE, A_GD_SCALE
+from zsph_amd import A_ACC, A_GD, A_INDEX, A_KEY, A_POSR, A_TYPE, A_VELP, FLUID, cases
+from tests.helpers import assert_close_scaled, jitter
+
+pytestmark = pytest.mark.gpu
+
+WALKS = (z.WALK_MASK, z.WALK_HITS, z.WALK_LIST)
+AIR = (0.6, 0.2, 0.35)  # a point of the tank above the water and away from the walls
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def small(grid_size_factor=1.0):
+    c = cases.dambreak3d(dx=0.035, lx=0.8, wy=0.4, lz=0.5, cw=0.3, ch=0.3, grid_size_factor=grid_size_factor)
+    assert c.n % 64 and int((c.host.type == FLUID).sum()) % 64
+    return c
+
+
+def fluid_ids(c):
+    return np.flatnonzero(c.host.type == FLUID)
+
+
+def by_original_id(h):
+    oid = h.table.astype(np.int64) + np.arange(h.n)
+    order = np.argsort(oid)
+    return order
+
+
+def run(c, walk, steps):
+    s = z.Solver(c)
+    s.upload(c.host)
+    s.set_tuning(walk, 8)
+    s.step(steps)
+    s.sync()
+    h = s.download(0)
+    acc = s.get(A_ACC)
+    killed = s.killed()
+    s.close()
+    return h, acc, killed
+
+
+def test_no_fluid_left():
+    c = small()
+    f = fluid_ids(c)
+    c.host.pos[f] = (50.0, 0.1, 0.1)
+    say(f"[no fluid] N={c.n}, {f.size} fluid particles outside the grid")
+    o = oracle.Oracle(c.params)
+    o.upload(c.host)
+    o.step(3)
+    r = o.download(0)
+    assert o.killed() == f.size
+    for walk in WALKS:
+        g, _, killed = run(c, walk, 3)
+        assert killed == f.size
+        assert np.array_equal(g.type, r.type)
+        assert np.array_equal(g.table, r.table)
+        assert np.array_equal(g.pos, r.pos.astype(np.float32))
+        assert np.array_equal(g.vel, r.vel.astype(np.float32))
+        assert np.array_equal(g.rhop, r.rhop.astype(np.float32))
+        wall = g.type == z.WALL
+        assert np.all(g.rhop[wall, 0] == c.json.rho0) and np.all(g.rhop[wall, 1] == 0.0)
+        say(f"  walk {walk}: {killed} removed, state equals the oracle's")
+    o.close()
+
+
+def test_one_fluid_particle_in_free_fall():
+    c = small()
+    f = fluid_ids(c)
+    c.host.pos[f] = (50.0, 0.1, 0.1)
+    k = f[0]
+    x0 = np.array(AIR, np.float32)
+    v0 = np.array((0.1, -0.05, 0.0), np.float32)
+    c.host.pos[k] = x0
+    c.host.vel[k] = v0
+    c.host.rhop[k] = (c.json.rho0, 0.0)
+    steps = 20
+    say(f"[one fluid particle] N={c.n}, {steps} steps")
+    o = oracle.Oracle(c.params)
+    o.upload(c.host)
+    o.step(steps)
+    r = o.download(0)
+    t = steps * 2.0 * c.params.half_dt
+    exact = x0.astype(np.float64) + v0 * t + 0.5 * np.array((0.0, 0.0, c.params.g)) * t * t
+    ro = by_original_id(r)
+    # the oracle's FP64 integrator is exact for constant acceleration (to the FP32 download's rounding)
+    assert np.all(np.abs(r.pos[ro][k] - exact) <= np.spacing(np.abs(exact).astype(np.float32)))
+    for walk in WALKS:
+        g, acc, killed = run(c, walk, steps)
+        assert killed == f.size - 1
+        go = by_original_id(g)
+        assert g.type[go][k] == FLUID
+        assert_close_scaled(g.pos[go][k], r.pos[ro][k], np.abs(exact) + c.json.dx, "x")
+        assert_close_scaled(g.vel[go][k], r.vel[ro][k], np.abs(r.vel[ro][k]) + 1e-3, "v")
+        assert g.rhop[go][k, 0] == np.float32(c.json.rho0) and g.rhop[go][k, 1] == 0.0
+        say(f"  walk {walk}: x {g.pos[go][k]} (exact {exact})")
+
+
+def cluster(grid_size_factor):
+    # small velocity and density perturbations give every term of the passes a
+    # value; the cluster is no physical state (its density triples in one step
+    # and diverges after that), it only drives the walks' overflow paths
+    c = jitter(small(grid_size_factor), seed=77, vmax=0.01, drho=1e-4)
+    f = fluid_ids(c)
+    rng = np.random.default_rng(5)
+    h = c.params.h
+    pick = f[:400]
+    c.host.pos[pick] = (np.array(AIR) + rng.uniform(-0.25 * h, 0.25 * h, (pick.size, 3))).astype(np.float32)
+    return c
+
+
+@pytest.mark.parametrize("gsf", [1.0, 2.0])
+def test_dense_cluster_stages(gsf):
+    c = cluster(gsf)
+    say(f"[dense cluster, grid_size_factor {gsf}] N={c.n}")
+    s = z.Solver(c)
+    s.upload(c.host)
+    s.set_tuning(z.WALK_MASK, 8)
+    o = oracle.Oracle(c.params)
+    o.upload(c.host)
+    for x in (s, o):
+        x.hash(0); x.sort_index(); x.sort_data(0, 1)
+    assert np.array_equal(s.get(A_KEY), o.get(A_KEY).astype(np.uint32))
+    assert np.array_equal(s.get(A_INDEX), o.get(A_INDEX).astype(np.int32))
+    for a in (A_POSR, A_VELP):
+        assert np.array_equal(s.get(a, 1), o.get(a, 1).astype(np.float32)), a
+    cg, dg = s.neighbor_stats(1)
+    co, do = o.neighbor_stats(1)
+    cb, db = o.neighbor_stats(1, brute=True)
+    assert np.array_equal(cg, co) and np.array_equal(dg, do)
+    assert np.array_equal(cg, cb) and np.array_equal(dg, db)
+    assert cg.max() >= 399
+    say(f"  neighbours: max {cg.max()}")
+    s.boundary(1)
+    o.wall(1)
+    typ = o.get(A_TYPE, 1)
+    f = typ == FLUID
+    gp, gv = s.get(A_POSR, 1), s.get(A_VELP, 1)
+    o.set(A_POSR, gp, 1)
+    o.set(A_VELP, gv, 1)
+    o.ldg(1)
+    gg, og = s.get(A_GD), o.get(A_GD)
+    sc = o.get(A_GD_SCALE)
+    gsc = np.maximum(sc[:, 0:1], np.abs(og[:, :3]).max(axis=1, keepdims=True))
+    assert_close_scaled(gg[f, :3], og[f, :3], np.broadcast_to(gsc[f], gg[f, :3].shape), "grad rho^L", rel=2e-5)
+    assert_close_scaled(gg[f, 3], og[f, 3], sc[f, 1], "div v")
+    o.set(A_GD, gg)
+    s.governing(1)
+    o.governing(1)
+    ga, oa = s.get(A_ACC), o.get(A_ACC)
+    asc = o.get(A_ACC_SCALE)
+    assert_close_scaled(ga[f, :3], oa[f, :3], np.broadcast_to(asc[f, 0:1], ga[f, :3].shape), "acc")
+    assert_close_scaled(ga[f, 3], oa[f, 3], asc[f, 1], "drho/dt")
+    o.close()
+    s.close()
+
+
+@pytest.mark.parametrize("gsf", [1.0, 2.0])
+def test_dense_cluster_walks_agree(gsf):
+    c = cluster(gsf)
+    say(f"[dense cluster walks, grid_size_factor {gsf}] N={c.n}")
+    ref = None
+    for walk in WALKS:
+        g, acc, killed = run(c, walk, 1)
+        assert killed == 0 and np.isfinite(acc).all()
+        got = (g.pos, g.vel, g.rhop, g.table, acc)
+        if ref is None:
+            ref = got
+            continue
+        for a, b in zip(ref, got):
+            assert np.array_equal(np.asarray(a), np.asarray(b)), f"walk {walk} differs from the mask walk"
+        say(f"  walk {walk}: bit-identical to the mask walk")
+
+
+@pytest.mark.parametrize("gsf", [1.0, 2.0])
+def test_hash_on_cell_faces(gsf):
+    c = small(gsf)
+    p = c.params
+    o3 = np.array([p.grid_xmin, p.grid_ymin, p.grid_zmin], np.float32)
+    dims = (p.grid_xdim, p.grid_ydim, p.grid_zdim)
+    size = np.float32(p.grid_size)
+    mid = o3 + size * (np.array(dims, np.float32) / np.float32(2.0))
+    pts = []
+    for a in range(3):
+        for k in range(1, dims[a]):
+            face = np.float32(o3[a] + np.float32(k) * size)
+            for u in range(-3, 4):  # the face and three FP32 neighbours on each side
+                x = face
+                for _ in range(abs(u)):
+                    x = np.nextafter(x, np.float32(np.inf if u > 0 else -np.inf))
+                q = mid.copy()
+                q[a] = x
+                pts.append(q)
+    pts = np.array(pts, np.float32)
+    assert pts.shape[0] <= c.n
+    num = pts - o3
+    quot = num / size  # IEEE FP32
+    up = (quot == np.floor(quot)) & (num.astype(np.float64) / np.float64(size) < quot)
+    assert up.any(), "no point whose quotient rounds up to an integer"
+    say(f"[hash on cell faces, grid_size_factor {gsf}] {pts.shape[0]} points, {int(up.sum())} rounding up to a face")
+    c.host.pos[:pts.shape[0]] = pts
+    s = z.Solver(c)
+    s.upload(c.host)
+    o = oracle.Oracle(c.params)
+    o.upload(c.host)
+    s.hash(0)
+    o.hash(0)
+    kg, ko = s.get(A_KEY), o.get(A_KEY).astype(np.uint32)
+    t = np.floor(quot).astype(np.int64)
+    cell = (t[:, 2] * dims[1] + t[:, 1]) * dims[0] + t[:, 0]
+    assert np.array_equal(ko[:pts.shape[0]] & ((1 << 30) - 1) & ~np.uint32(p_flag(c)), cell.astype(np.uint32))
+    bad = np.flatnonzero(kg != ko)
+    assert bad.size == 0, f"{bad.size} keys differ, first at {pts[bad[0]] if bad[0] < len(pts) else bad[0]}"
+    assert s.killed() == o.killed() == 0
+    s.close()
+    o.close()
+
+
+def p_flag(c):
+    """The non-fluid flag bit of the sort key: the first power of two >= the cell count."""
+    g = c.params.grid_xdim * c.params.grid_ydim * c.params.grid_zdim
+    b = 1
+    while b < 30 and (1 << b) < g:
+        b += 1
+    return 1 << b

@@ -107,14 +107,15 @@ __device__ __forceinline__ int xcd_block(int nblocks) {
     return x * per + (x < rem ? x : rem) + local;
 }
 
-// Cell coordinate along one axis: floor((x - o) / size) with IEEE FP32
-// division; returns false outside [0, dim).  NaN positions fail.
+// Cell coordinate along one axis: floor of the FP32 quotient (x - o) / size
+// rounded to nearest; returns false outside [0, dim).  NaN positions fail.
+// The quotient is the FP64 quotient rounded to FP32: for FP32 operands that
+// double rounding equals the correctly rounded FP32 quotient (53 >= 2*24 + 2
+// bits, Figueroa 1995), and it does not depend on the build's FP32 division
+// mode (this library is built with approximate FP32 division, which puts a
+// point whose quotient rounds to an integer k in cell k - 1).
 __host__ __device__ inline bool cell_axis(float x, float o, float size, int dim, int *c) {
-#ifdef __HIP_DEVICE_COMPILE__
-    const float t = floorf(__fdiv_rn(x - o, size));
-#else
-    const float t = floorf((x - o) / size);
-#endif
+    const float t = floorf((float)((double)(x - o) / (double)size));
     if (!(t >= 0.0f && t < (float)dim)) return false;
     *c = (int)t;
     return true;

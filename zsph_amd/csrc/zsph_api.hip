@@ -260,7 +260,7 @@ void drop_graph(zsph_ctx *c) {
 }
 
 bool valid_params(const zsph_params *p) {
-    if (p->ptc_num < 0 || p->ptc_num >= zs::MAX_PARTICLES || p->grid_xdim < 1 || p->grid_ydim < 1 || p->grid_zdim < 1 || !(p->h > 0.0f) ||
+    if (p->ptc_num < 1 || p->ptc_num >= zs::MAX_PARTICLES || p->grid_xdim < 1 || p->grid_ydim < 1 || p->grid_zdim < 1 || !(p->h > 0.0f) ||
         !(p->grid_size > 0.0f) || p->grid_factor < 1 || (p->dim != 2 && p->dim != 3) || !(p->gamma > 0.0f) ||
         !(p->rc2 > 0.0f) || !(p->rho0 > 0.0f) || !(p->cs2 > 0.0f))
         return false;
@@ -1183,6 +1183,8 @@ int zsph_body_reduce(zsph_ctx *c, int b, double F[3], double T[3]) {
 
 // ---- checkpoint / restart ------------------------------------------------
 
+}  // extern "C"
+
 namespace {
 
 // FNV-1a 64 of the parameter blocks (all 4-byte fields, no padding): a
@@ -1232,6 +1234,8 @@ void *ckpt_device(zsph_ctx *c, const std::string &name) {
 }
 
 }  // namespace
+
+extern "C" {
 
 int zsph_checkpoint_write(zsph_ctx *c, const char *path, long long step) {
     if (!c || !path || !*path || step < 0) return set_err(c, ZSPH_E_ARG, "zsph_checkpoint_write: bad argument");
