@@ -91,7 +91,7 @@ __global__ __launch_bounds__(64) void k_body_update(Prm q, DevBody *__restrict__
 // Piston dummies: x = x0 + d(t) e, v = d'(t) e, a_w = d''(t) e along the axis e.
 __global__ __launch_bounds__(ZS_BLOCK) void k_kinematics(Prm q, Buf b, const int *__restrict__ cs,
                                                          const DevBody *__restrict__ body) {
-    const int i = cs[q.G] + xcd_block(gridDim.x) * ZS_BLOCK + threadIdx.x;
+    const int i = cs[q.G] + xcd_block(blocks_of(q.N - cs[q.G])) * ZS_BLOCK + threadIdx.x;
     if (i >= q.N) return;
     const int ti = b.type[i];
     if (ti != ZSPH_BODY && ti != ZSPH_PISTON) return;

@@ -98,14 +98,23 @@ __device__ __forceinline__ Prm scene_prm(const Prm &q0, const Prm *__restrict__ 
 
 #define ZS_BLOCK 256
 
-// Blocks are dispatched round-robin over the 8 XCDs; remap so each XCD
-// (own L2) works on one contiguous range of particles.
+// Blocks are dispatched round-robin over the 8 XCDs; remap blocks
+// [0, nblocks) among themselves so each XCD (own L2) works on one contiguous
+// range of them, and leave blocks at or beyond nblocks where they are.
+// nblocks counts the blocks that hold the launch's work: a pass over the
+// fluid alone (the first cs[G] particles) or the dummies alone gives the
+// remap only their blocks, so that all eight XCDs share that work (remapping
+// the whole grid would leave the XCDs whose ranges hold no such particle
+// idle).
 __device__ __forceinline__ int xcd_block(int nblocks) {
     const int b = blockIdx.x;
+    if (b >= nblocks) return b;
     const int per = nblocks >> 3, rem = nblocks & 7;
     const int x = b & 7, local = b >> 3;
     return x * per + (x < rem ? x : rem) + local;
 }
+// Blocks of ZS_BLOCK particles that cover n particles.
+__device__ __forceinline__ int blocks_of(int n) { return (n + ZS_BLOCK - 1) / ZS_BLOCK; }
 
 // Cell coordinate along one axis: floor of the FP32 quotient (x - o) / size
 // rounded to nearest; returns false outside [0, dim).  NaN positions fail.

@@ -39,7 +39,7 @@ __global__ __launch_bounds__(ZS_BLOCK) void k_wall(Prm q0, const Prm *__restrict
                                                    const int *__restrict__ cs) {
     __shared__ int qbuf[WalkLds<WALK>::ints];
     const int nf = cs[q0.G];
-    const int i = nf + xcd_block(gridDim.x) * ZS_BLOCK + threadIdx.x;
+    const int i = nf + xcd_block(blocks_of(q0.N - nf)) * ZS_BLOCK + threadIdx.x;
     if (i >= q0.N) return;
     const Prm q = scene_prm<BATCH>(q0, sp, b, i);
     const float4 xi = b.posr[i];
@@ -73,7 +73,7 @@ __global__ __launch_bounds__(ZS_BLOCK) void k_ldg(Prm q0, const Prm *__restrict_
                                                   int *__restrict__ ncount, float *__restrict__ feat) {
     __shared__ int qbuf[WalkLds<WALK>::ints];
     const int nf = cs[q0.G];
-    const int i = xcd_block(gridDim.x) * ZS_BLOCK + threadIdx.x;
+    const int i = xcd_block(blocks_of(nf)) * ZS_BLOCK + threadIdx.x;
     if (i >= q0.N) return;
     if (i >= nf) {
         gd[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
@@ -220,7 +220,7 @@ __global__ __launch_bounds__(ZS_BLOCK) void k_governing(Prm q0, const Prm *__res
                                                         const float2 *__restrict__ scale, float *__restrict__ terms) {
     __shared__ int qbuf[WalkLds<WALK>::ints];
     const int nf = cs[q0.G];
-    const int i = xcd_block(gridDim.x) * ZS_BLOCK + threadIdx.x;
+    const int i = xcd_block(blocks_of(nf)) * ZS_BLOCK + threadIdx.x;
     if (i >= nf) return;
     const Prm q = scene_prm<BATCH>(q0, sp, b, i);
     const bool delta = q.diffusion == ZSPH_DIFF_ANTUONO;
