@@ -39,7 +39,9 @@ __global__ __launch_bounds__(ZS_BLOCK) void k_wall(Prm q0, const Prm *__restrict
                                                    const int *__restrict__ cs) {
     __shared__ int qbuf[WalkLds<WALK>::ints];
     const int nf = cs[q0.G];
-    const int i = nf + xcd_block(blocks_of(q0.N - nf)) * ZS_BLOCK + threadIdx.x;
+    // dummy blocks round-robin over the XCDs: dummies far from the fluid
+    // cost little, so contiguous ranges of them would load the XCDs unevenly
+    const int i = nf + blockIdx.x * ZS_BLOCK + threadIdx.x;
     if (i >= q0.N) return;
     const Prm q = scene_prm<BATCH>(q0, sp, b, i);
     const float4 xi = b.posr[i];
