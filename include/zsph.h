@@ -223,10 +223,10 @@ void zsph_rigid_euler_step(zsph_rigid *r, float *pts, int n, double dt);
 /* lib/init.cu:38-115 alloc_gpu_ptc_data x2 + alloc_gpu_tmp_data, plus the
  * cudaMemcpyToSymbol(par/tmp_data) of deltasph/main.cu:47-48 and the device
  * checks of lib/check.cu:19-59.  device < 0 keeps the current device.
- * ptc_num must be at least 1 and below 2^28 (particle arrays are addressed by
- * 32-bit byte offsets), as must the total of a batch; every scene of a batch
- * holds at least one particle.  Other sizes return ZSPH_E_ARG before any
- * device call. */
+ * ptc_num must be at least 1 and below 2^28 - 64 (particle arrays, with 64
+ * spare elements, are addressed by 32-bit byte offsets), as must the total of
+ * a batch; every scene of a batch holds at least one particle.  Other sizes
+ * return ZSPH_E_ARG before any device call. */
 int zsph_create(const zsph_params *p, int device, zsph_ctx **out);
 
 /* lib/init.cu:117-134 delete_gpu_*_data. */

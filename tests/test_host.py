@@ -155,10 +155,10 @@ def test_invalid_params_rejected_without_gpu():
     p = abi.Params()
     ctx = C.c_void_p()
     assert z.lib().zsph_create(C.byref(p), 0, C.byref(ctx)) == abi.E_ARG
-    # sizes: an empty context and one at the 2^28 addressing limit are refused
-    # before any device call, alone and in a batch
+    # sizes: an empty context and one at the addressing limit (2^28 - 64
+    # particles) are refused before any device call, alone and in a batch
     good = cases.dambreak2d(dx=0.02).params
-    for n in (0, -1, 1 << 28, (1 << 31) - 1):
+    for n in (0, -1, (1 << 28) - 64, 1 << 28, (1 << 31) - 1):
         q = abi.Params.from_buffer_copy(good)
         q.ptc_num = n
         assert z.lib().zsph_create(C.byref(q), 0, C.byref(ctx)) == abi.E_ARG, n
@@ -166,5 +166,5 @@ def test_invalid_params_rejected_without_gpu():
         assert z.lib().zsph_create_batch(scenes, 2, 0, C.byref(ctx)) == abi.E_ARG, n
         assert not ctx.value
     big = (abi.Params * 2)(abi.Params.from_buffer_copy(good), abi.Params.from_buffer_copy(good))
-    big[0].ptc_num = big[1].ptc_num = 1 << 27
+    big[0].ptc_num = big[1].ptc_num = (1 << 27) - 32
     assert z.lib().zsph_create_batch(big, 2, 0, C.byref(ctx)) == abi.E_ARG

@@ -162,7 +162,11 @@ __device__ __forceinline__ T ld(const T *__restrict__ base, int j) {
 __device__ __forceinline__ float3 ld_xyz(const float4 *__restrict__ base, int j) {
     return *reinterpret_cast<const float3 *>(reinterpret_cast<const char *>(base) + (uint32_t)j * 16u);
 }
-constexpr int MAX_PARTICLES = 1 << 28;
+// Spare elements after the N particles of every position array: the mask
+// walk's candidate rounds read up to three past a row's end.
+constexpr int POS_PAD = 64;
+// N + POS_PAD elements of 16 bytes stay below 2^32 bytes.
+constexpr int MAX_PARTICLES = (1 << 28) - POS_PAD;
 
 // Hardware reciprocal and square root (v_rcp_f32, v_sqrt_f32: about 1 ulp),
 // without the denormal range scaling of the library forms; the operands in
@@ -302,43 +306,45 @@ __device__ __forceinline__ void for_each_neighbor(const Prm &q, int i, float4 xi
         f(j, dx, dy, dz, dist2(dx, dy, dz), xj);
     };
     // Hit mask of candidates [base, base + cnt) (cnt <= 32) of a row whose
-    // candidate u is fb + u for u < nfl, else db + (u - nfl).  Four
-    // candidates per round, their loads issued together (past the chunk's
-    // end the last candidate is loaded again and its bit dropped; t + e <= 31
-    // since t < cnt <= 32).
-    auto test_chunk = [&](int base, int cnt, int fb, int nfl, int db) {
+    // candidate u is off(u) + u, off(u) = fb for u < nfl, else dbo = db - nfl.
+    // Four candidates per round, their loads issued together.  Past the
+    // chunk's end a round reads the particles that follow the row's range
+    // (at most index N + 2: position arrays carry POS_PAD spare elements) and
+    // the final mask clears those bits; i itself is cleared by the caller.
+    auto test_chunk = [&](int base, int cnt, int fb, int nfl, int dbo) {
         uint32_t m = 0;
         for (int t = 0; t < cnt; t += 4) {
-            int js[4];
             float3 xs[4];
 #pragma unroll
             for (int e = 0; e < 4; ++e) {
-                const int u = base + min(t + e, cnt - 1);
-                js[e] = (u < nfl) ? fb + u : db + (u - nfl);
-                xs[e] = ld_xyz(posr, js[e]);
+                const int u = base + t + e;
+                xs[e] = ld_xyz(posr, ((u < nfl) ? fb : dbo) + u);
             }
             // keep all four loads ahead of the tests (the scheduler would
             // otherwise interleave them and wait after two)
             __builtin_amdgcn_sched_barrier(0);
+            uint32_t nib = 0;
 #pragma unroll
-            for (int e = 0; e < 4; ++e) {
-                const bool h =
-                    dist2(xi.x - xs[e].x, xi.y - xs[e].y, xi.z - xs[e].z) <= q.rc2 && js[e] != i && t + e < cnt;
-                m |= h ? (1u << (t + e)) : 0u;
-            }
+            for (int e = 0; e < 4; ++e)
+                nib |= (dist2(xi.x - xs[e].x, xi.y - xs[e].y, xi.z - xs[e].z) <= q.rc2) ? (1u << e) : 0u;
+            m |= nib << t;  // t <= 28
         }
-        return m;
+        return (cnt < 32) ? m & ((1u << cnt) - 1u) : m;
     };
     for_each_row<FLUID_J, DUMMY_J>(q, xi, cell, cs, [&](int fb, int fe, int db, int de, int r) {
         const int nfl = fe - fb;
         const int total = nfl + (de - db);
+        const int dbo = db - nfl;
+        // i's own place among the row's candidates (-1: not in this row)
+        const int us = (FLUID_J && i >= fb && i < fe) ? i - fb : (DUMMY_J && i >= db && i < de) ? nfl + (i - db) : -1;
         for (int base = 0, c = 0; base < total; base += 32, ++c) {
-            uint32_t m = test_chunk(base, min(32, total - base), fb, nfl, db);
+            uint32_t m = test_chunk(base, min(32, total - base), fb, nfl, dbo);
+            if ((uint32_t)(us - base) < 32u) m &= ~(1u << (us - base));
             rec(r, c, m);
             while (m) {
                 const int u = base + __builtin_ctz(m);
                 m &= m - 1u;
-                visit((u < nfl) ? fb + u : db + (u - nfl));
+                visit(((u < nfl) ? fb : dbo) + u);
             }
         }
     });
@@ -361,7 +367,7 @@ __device__ __forceinline__ void for_each_recorded(const Prm &q, int i, float4 xi
             while (m) {
                 const int u = base + __builtin_ctz(m);
                 m &= m - 1u;
-                const int j = (u < nfl) ? fb + u : db + (u - nfl);
+                const int j = ((u < nfl) ? fb : db - nfl) + u;
                 const float4 xj = ld(posr, j);
                 const float dx = xi.x - xj.x, dy = xi.y - xj.y, dz = xi.z - xj.z;
                 f(j, dx, dy, dz, dist2(dx, dy, dz), xj);

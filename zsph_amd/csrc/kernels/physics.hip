@@ -27,6 +27,17 @@ __device__ __forceinline__ uint32_t cell_of(const Prm &q, uint32_t key) {
     return (key & (q.flag - 1u)) - (uint32_t)q.cell_base;
 }
 
+// Diffusion form of a launch (template argument DF): ZSPH_DIFF_ANTUONO or
+// ZSPH_DIFF_MC fixed at compile time for one scene, so the pair bodies carry
+// no per-pair test of it; DIFF_PER_SCENE reads it from each particle's scene
+// parameters (a batch's scenes may differ).
+constexpr int DIFF_PER_SCENE = -1;
+template <int DF>
+__device__ __forceinline__ bool is_antuono(const Prm &q) {
+    if constexpr (DF == DIFF_PER_SCENE) return q.diffusion == ZSPH_DIFF_ANTUONO;
+    else return DF == ZSPH_DIFF_ANTUONO;
+}
+
 // Dummy particles: Shepard extrapolation of fluid pressure with the
 // hydrostatic and wall-acceleration correction,
 //   p_w = max(sum (p_j - rho_j (a_w - g) . x_wj) W / sum W, 0),
@@ -68,7 +79,7 @@ __global__ __launch_bounds__(ZS_BLOCK) void k_wall(Prm q0, const Prm *__restrict
 // not depend on j).  MC form: D only.  Output gd = (G, D); dummies get 0.
 // FEAT: also the closure network's ZSPH_NF features of each fluid particle
 // (feat[i * ZSPH_NF ...], see closure_features) from the same neighbour walk.
-template <int WALK, bool BATCH, bool FEAT>
+template <int WALK, bool BATCH, bool FEAT, int DF>
 __global__ __launch_bounds__(ZS_BLOCK) void k_ldg(Prm q0, const Prm *__restrict__ sp, Buf b,
                                                   const uint32_t *__restrict__ skey, const int *__restrict__ cs,
                                                   float4 *__restrict__ gd, int *__restrict__ nlist,
@@ -82,7 +93,7 @@ __global__ __launch_bounds__(ZS_BLOCK) void k_ldg(Prm q0, const Prm *__restrict_
         return;
     }
     const Prm q = scene_prm<BATCH>(q0, sp, b, i);
-    const bool delta = q.diffusion == ZSPH_DIFF_ANTUONO;
+    const bool delta = is_antuono<DF>(q);
     const bool ell = delta || FEAT;  // L is needed
     const float4 xi = b.posr[i];
     const float4 vi = b.velp[i];
@@ -214,7 +225,7 @@ __global__ __launch_bounds__(ZS_BLOCK) void k_ldg(Prm q0, const Prm *__restrict_
 // (bulk + artificial viscosity) by scale[i].y.  MODE 2: plain, and the parts
 // are written to terms[i * 8 ...] = (continuity, diffusion, pressure + g (3),
 // viscous (3)) for training.
-template <int WALK, bool BATCH, int MODE>
+template <int WALK, bool BATCH, int MODE, int DF>
 __global__ __launch_bounds__(ZS_BLOCK) void k_governing(Prm q0, const Prm *__restrict__ sp, Buf b,
                                                         const uint32_t *__restrict__ skey, const int *__restrict__ cs,
                                                         const float4 *__restrict__ gd, float4 *__restrict__ acc,
@@ -225,7 +236,7 @@ __global__ __launch_bounds__(ZS_BLOCK) void k_governing(Prm q0, const Prm *__res
     const int i = xcd_block(blocks_of(nf)) * ZS_BLOCK + threadIdx.x;
     if (i >= nf) return;
     const Prm q = scene_prm<BATCH>(q0, sp, b, i);
-    const bool delta = q.diffusion == ZSPH_DIFF_ANTUONO;
+    const bool delta = is_antuono<DF>(q);
     const float4 xi = b.posr[i];
     const float4 vi = b.velp[i];
     const float4 gi = gd[i];
@@ -449,12 +460,18 @@ template <int WALK, bool BATCH>
 static void ldg_walk(hipStream_t s, bool feat, int grid, const Prm &q, const Prm *sp, const Buf &b,
                      const uint32_t *skey, const int *cs, float4 *gd, int *nlist, int *ncount, float *f) {
     if constexpr (!BATCH) {
-        if (feat) {
-            k_ldg<WALK, false, true><<<grid, ZS_BLOCK, 0, s>>>(q, sp, b, skey, cs, gd, nlist, ncount, f);
-            return;
-        }
+        const bool a = q.diffusion == ZSPH_DIFF_ANTUONO;
+        if (feat && a)
+            k_ldg<WALK, false, true, ZSPH_DIFF_ANTUONO><<<grid, ZS_BLOCK, 0, s>>>(q, sp, b, skey, cs, gd, nlist, ncount, f);
+        else if (feat)
+            k_ldg<WALK, false, true, ZSPH_DIFF_MC><<<grid, ZS_BLOCK, 0, s>>>(q, sp, b, skey, cs, gd, nlist, ncount, f);
+        else if (a)
+            k_ldg<WALK, false, false, ZSPH_DIFF_ANTUONO><<<grid, ZS_BLOCK, 0, s>>>(q, sp, b, skey, cs, gd, nlist, ncount, f);
+        else
+            k_ldg<WALK, false, false, ZSPH_DIFF_MC><<<grid, ZS_BLOCK, 0, s>>>(q, sp, b, skey, cs, gd, nlist, ncount, f);
+    } else {
+        k_ldg<WALK, true, false, DIFF_PER_SCENE><<<grid, ZS_BLOCK, 0, s>>>(q, sp, b, skey, cs, gd, nlist, ncount, f);
     }
-    k_ldg<WALK, BATCH, false><<<grid, ZS_BLOCK, 0, s>>>(q, sp, b, skey, cs, gd, nlist, ncount, f);
 }
 
 void launch_ldg(hipStream_t s, int walk, const Prm &q, const Prm *sp, const Buf &b, const uint32_t *skey,
@@ -478,19 +495,24 @@ template <int WALK, bool BATCH>
 static void gov_walk(hipStream_t s, int mode, int grid, const Prm &q, const Prm *sp, const Buf &b,
                      const uint32_t *skey, const int *cs, const float4 *gd, float4 *acc, const int *nlist,
                      const int *ncount, const float2 *scale, float *terms) {
+#define ZS_GOV(MODE_, DF_) \
+    k_governing<WALK, BATCH, MODE_, DF_><<<grid, ZS_BLOCK, 0, s>>>(q, sp, b, skey, cs, gd, acc, nlist, ncount, scale, terms)
     if constexpr (!BATCH) {
+        const bool a = q.diffusion == ZSPH_DIFF_ANTUONO;
         if (mode == 1) {
-            k_governing<WALK, false, 1><<<grid, ZS_BLOCK, 0, s>>>(q, sp, b, skey, cs, gd, acc, nlist, ncount, scale,
-                                                                 terms);
-            return;
+            if (a) ZS_GOV(1, ZSPH_DIFF_ANTUONO);
+            else ZS_GOV(1, ZSPH_DIFF_MC);
+        } else if (mode == 2) {
+            if (a) ZS_GOV(2, ZSPH_DIFF_ANTUONO);
+            else ZS_GOV(2, ZSPH_DIFF_MC);
+        } else {
+            if (a) ZS_GOV(0, ZSPH_DIFF_ANTUONO);
+            else ZS_GOV(0, ZSPH_DIFF_MC);
         }
-        if (mode == 2) {
-            k_governing<WALK, false, 2><<<grid, ZS_BLOCK, 0, s>>>(q, sp, b, skey, cs, gd, acc, nlist, ncount, scale,
-                                                                 terms);
-            return;
-        }
+    } else {
+        ZS_GOV(0, DIFF_PER_SCENE);
     }
-    k_governing<WALK, BATCH, 0><<<grid, ZS_BLOCK, 0, s>>>(q, sp, b, skey, cs, gd, acc, nlist, ncount, scale, terms);
+#undef ZS_GOV
 }
 
 void launch_governing(hipStream_t s, int walk, const Prm &q, const Prm *sp, const Buf &b, const uint32_t *skey,

@@ -280,7 +280,8 @@ int create_ctx(zsph_ctx *c, int device, size_t N, size_t G, bool batch) {
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) return ZSPH_E_HIP;
     int r;
     for (int b = 0; b < 2; ++b) {
-        if ((r = dalloc(c, &c->buf[b].posr, N)) || (r = dalloc(c, &c->buf[b].velp, N)) ||
+        // posr carries zs::POS_PAD zeroed spare elements read by the neighbour walks' candidate rounds
+        if ((r = dalloc(c, &c->buf[b].posr, N + zs::POS_PAD)) || (r = dalloc(c, &c->buf[b].velp, N)) ||
             (r = dalloc(c, &c->buf[b].tposr, N)) || (r = dalloc(c, &c->buf[b].tvelp, N)) ||
             (r = dalloc(c, &c->buf[b].type, N)) || (r = dalloc(c, &c->buf[b].table, N)))
             return r;
