@@ -214,10 +214,15 @@ __device__ __forceinline__ CellBox cell_box(const Prm &q, uint32_t key) {
 // segment is not requested), r = (dz + k) (2k + 1) + dy + k the row's place
 // in the (2k+1)^2 rows of the box (dy, dz: offsets from xi's cell).  The trimming works in cell units with a
 // margin, so it only ever keeps extra candidates; the r^2 predicate alone
-// decides the neighbour set.
-template <bool FLUID_J, bool DUMMY_J, class G>
+// decides the neighbour set.  pre(r) runs for each row within the cut-off
+// distance before the row's cell-table loads are issued (a caller's own
+// loads for the row then travel together with them).
+struct NoPre {
+    __device__ __forceinline__ void operator()(int) const {}
+};
+template <bool FLUID_J, bool DUMMY_J, class G, class P = NoPre>
 __device__ __forceinline__ void for_each_row(const Prm &q, float4 xi, uint32_t cell, const int *__restrict__ cs,
-                                             G &&g) {
+                                             G &&g, P &&pre = P{}) {
     const CellBox b = cell_box(q, cell);
     const float fx = (xi.x - q.gx0) * q.inv_cell;
     const float fy = (xi.y - q.gy0) * q.inv_cell;
@@ -230,6 +235,8 @@ __device__ __forceinline__ void for_each_row(const Prm &q, float4 xi, uint32_t c
             const float dyc = fmaxf(0.0f, fmaxf((float)cy - fy, fy - (float)(cy + 1)));
             const float rem = remz - dyc * dyc;
             if (rem < 0.0f) continue;
+            const int r = (cz - b.cz + q.k) * (2 * q.k + 1) + (cy - b.cy + q.k);
+            pre(r);
             const float w = sqrtf(rem) + q.trim_pad;
             const int xa = max(b.xlo, (int)floorf(fx - w));
             const int xb = min(b.xhi, (int)floorf(fx + w));
@@ -238,7 +245,7 @@ __device__ __forceinline__ void for_each_row(const Prm &q, float4 xi, uint32_t c
             int fb = 0, fe = 0, db = 0, de = 0;
             if (FLUID_J) { fb = cs[row + xa]; fe = cs[row + xb + 1]; }
             if (DUMMY_J) { db = cs[q.G + 1 + row + xa]; de = cs[q.G + 1 + row + xb + 1]; }
-            g(fb, fe, db, de, (cz - b.cz + q.k) * (2 * q.k + 1) + (cy - b.cy + q.k));
+            g(fb, fe, db, de, r);
         }
     }
 }
@@ -359,11 +366,13 @@ template <bool FLUID_J, bool DUMMY_J, class F>
 __device__ __forceinline__ void for_each_recorded(const Prm &q, int i, float4 xi, uint32_t cell,
                                                   const int *__restrict__ cs, const float4 *__restrict__ posr,
                                                   const uint32_t *__restrict__ hits, size_t n, F &&f) {
+    uint32_t m0 = 0;  // the row's first mask, loaded with its cell-table entries
     for_each_row<FLUID_J, DUMMY_J>(q, xi, cell, cs, [&](int fb, int fe, int db, int de, int r) {
         const int nfl = fe - fb;
         const int total = nfl + (de - db);
+        if (m0 == 0 && total <= 32) return;  // no neighbour in this row
         for (int base = 0, c = 0; base < total; base += 32, ++c) {
-            uint32_t m = hits[(size_t)(2 * r + c) * n + (size_t)i];
+            uint32_t m = c ? hits[(size_t)(2 * r + c) * n + (size_t)i] : m0;
             while (m) {
                 const int u = base + __builtin_ctz(m);
                 m &= m - 1u;
@@ -373,7 +382,7 @@ __device__ __forceinline__ void for_each_recorded(const Prm &q, int i, float4 xi
                 f(j, dx, dy, dz, dist2(dx, dy, dz), xj);
             }
         }
-    });
+    }, [&](int r) { m0 = hits[(size_t)(2 * r) * n + (size_t)i]; });
 }
 
 // Nontemporal (streaming) 16-byte store and load.
