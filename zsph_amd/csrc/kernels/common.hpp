@@ -207,47 +207,85 @@ __device__ __forceinline__ CellBox cell_box(const Prm &q, uint32_t key) {
 // the others (dummies, removed) are [cs[G+1+c], cs[G+1+c+1]).  Fluid
 // particles therefore occupy [0, cs[G]) and dummies [cs[G], N).
 
-// Calls g(fb, fe, db, de, r) for every (y,z) cell row of the (2k+1)^3 box
-// around xi's cell, trimmed to the x-cells that can hold a point within the
-// cut-off of xi (rows beyond the cut-off are skipped): [fb, fe) are the
-// row's fluid candidates, [db, de) its other particles (empty when the
-// segment is not requested), r = (dz + k) (2k + 1) + dy + k the row's place
-// in the (2k+1)^2 rows of the box (dy, dz: offsets from xi's cell).  The trimming works in cell units with a
-// margin, so it only ever keeps extra candidates; the r^2 predicate alone
-// decides the neighbour set.  pre(r) runs for each row within the cut-off
-// distance before the row's cell-table loads are issued (a caller's own
-// loads for the row then travel together with them).
-struct NoPre {
-    __device__ __forceinline__ void operator()(int) const {}
+// Candidate ranges of one row of the search box: fluid particles [fb, fe),
+// the others (dummies, removed) [db, de); empty when the row lies outside
+// the grid or beyond the cut-off.
+struct RowRange {
+    int fb = 0, fe = 0, db = 0, de = 0;
 };
+struct NoPre {
+    __device__ __forceinline__ void operator()(int, int) const {}
+};
+
+// Row (dz, dy) of the (2k+1)^3 cell box around xi's cell (offsets from that
+// cell), trimmed to the x-cells that can hold a point within the cut-off of
+// xi.  The trimming works in cell units with a margin, so it only ever keeps
+// extra candidates; the r^2 predicate alone decides the neighbour set.
+// pre(r, s) runs when the row lies within the cut-off distance, before its
+// cell-table loads are issued (a caller's own loads for the row then travel
+// together with them).
+template <bool FLUID_J, bool DUMMY_J, class P>
+__device__ __forceinline__ RowRange box_row(const Prm &q, const CellBox &b, float fx, float fy, float fz,
+                                            const int *__restrict__ cs, int dz, int dy, int r, int s, P &pre) {
+    RowRange w;
+    const int cz = b.cz + dz, cy = b.cy + dy;
+    if (cz < b.zlo || cz > b.zhi || cy < b.ylo || cy > b.yhi) return w;
+    const float dzc = fmaxf(0.0f, fmaxf((float)cz - fz, fz - (float)(cz + 1)));
+    const float dyc = fmaxf(0.0f, fmaxf((float)cy - fy, fy - (float)(cy + 1)));
+    const float remz = q.trim2 - dzc * dzc;
+    const float rem = remz - dyc * dyc;
+    if (rem < 0.0f) return w;
+    pre(r, s);
+    const float wd = sqrtf(rem) + q.trim_pad;
+    const int xa = max(b.xlo, (int)floorf(fx - wd));
+    const int xb = min(b.xhi, (int)floorf(fx + wd));
+    if (xa > xb) return w;
+    const int row = q.cell_base + cz * q.XY + cy * q.X;
+    if (FLUID_J) { w.fb = cs[row + xa]; w.fe = cs[row + xb + 1]; }
+    if (DUMMY_J) { w.db = cs[q.G + 1 + row + xa]; w.de = cs[q.G + 1 + row + xb + 1]; }
+    return w;
+}
+
+// Calls g(A, rA, B, rB) for the rows of the box in point-mirror pairs: row
+// (dz, dy) with row (-dz, -dy), for dz < 0 (every dy) and for dz = 0, dy < 0,
+// then the centre row (0, 0) alone (B empty, rB = -1).  r = (dz + k)(2k + 1)
+// + dy + k is a row's place in the (2k+1)^2 rows of the box; the mirror of r
+// is (2k+1)^2 - 1 - r.  A particle near the top of its cell has more
+// neighbours in the rows above and fewer in the rows below, so the hit counts
+// of a mirror pair vary less across the lanes of a wave than those of either
+// row: the walks visit a pair's hits as one sequence.  pre(r, s): see box_row
+// (s = 0 for A, 1 for B).
 template <bool FLUID_J, bool DUMMY_J, class G, class P = NoPre>
-__device__ __forceinline__ void for_each_row(const Prm &q, float4 xi, uint32_t cell, const int *__restrict__ cs,
-                                             G &&g, P &&pre = P{}) {
+__device__ __forceinline__ void for_each_row_pair(const Prm &q, float4 xi, uint32_t cell, const int *__restrict__ cs,
+                                                  G &&g, P &&pre = P{}) {
     const CellBox b = cell_box(q, cell);
     const float fx = (xi.x - q.gx0) * q.inv_cell;
     const float fy = (xi.y - q.gy0) * q.inv_cell;
     const float fz = (xi.z - q.gz0) * q.inv_cell;
-    for (int cz = b.zlo; cz <= b.zhi; ++cz) {
-        const float dzc = fmaxf(0.0f, fmaxf((float)cz - fz, fz - (float)(cz + 1)));
-        const float remz = q.trim2 - dzc * dzc;
-        if (remz < 0.0f) continue;
-        for (int cy = b.ylo; cy <= b.yhi; ++cy) {
-            const float dyc = fmaxf(0.0f, fmaxf((float)cy - fy, fy - (float)(cy + 1)));
-            const float rem = remz - dyc * dyc;
-            if (rem < 0.0f) continue;
-            const int r = (cz - b.cz + q.k) * (2 * q.k + 1) + (cy - b.cy + q.k);
-            pre(r);
-            const float w = sqrtf(rem) + q.trim_pad;
-            const int xa = max(b.xlo, (int)floorf(fx - w));
-            const int xb = min(b.xhi, (int)floorf(fx + w));
-            if (xa > xb) continue;
-            const int row = q.cell_base + cz * q.XY + cy * q.X;
-            int fb = 0, fe = 0, db = 0, de = 0;
-            if (FLUID_J) { fb = cs[row + xa]; fe = cs[row + xb + 1]; }
-            if (DUMMY_J) { db = cs[q.G + 1 + row + xa]; de = cs[q.G + 1 + row + xb + 1]; }
-            g(fb, fe, db, de, r);
+    const int K = 2 * q.k + 1, last = K * K - 1;
+    for (int dz = -q.k; dz <= 0; ++dz) {
+        const int dyend = (dz < 0) ? q.k : -1;
+        for (int dy = -q.k; dy <= dyend; ++dy) {
+            const int r = (dz + q.k) * K + dy + q.k;
+            const RowRange A = box_row<FLUID_J, DUMMY_J>(q, b, fx, fy, fz, cs, dz, dy, r, 0, pre);
+            const RowRange B = box_row<FLUID_J, DUMMY_J>(q, b, fx, fy, fz, cs, -dz, -dy, last - r, 1, pre);
+            g(A, r, B, last - r);
         }
     }
+    const RowRange C = box_row<FLUID_J, DUMMY_J>(q, b, fx, fy, fz, cs, 0, 0, last / 2, 0, pre);
+    g(C, last / 2, RowRange{}, -1);
+}
+
+// Calls g(fb, fe, db, de, r) for every row of the box, in the order of
+// for_each_row_pair (A then B of each pair); rows beyond the cut-off come
+// with empty ranges.
+template <bool FLUID_J, bool DUMMY_J, class G>
+__device__ __forceinline__ void for_each_row(const Prm &q, float4 xi, uint32_t cell, const int *__restrict__ cs,
+                                             G &&g) {
+    for_each_row_pair<FLUID_J, DUMMY_J>(q, xi, cell, cs, [&](const RowRange &A, int rA, const RowRange &B, int rB) {
+        g(A.fb, A.fe, A.db, A.de, rA);
+        if (rB >= 0) g(B.fb, B.fe, B.db, B.de, rB);
+    });
 }
 
 // Neighbour-walk variants (chosen per context by measurement):
@@ -287,6 +325,23 @@ template <int WALK>
 struct WalkLds {
     static constexpr int ints = 1;
 };
+
+// Visits the hits of two one-chunk rows as one sequence: bit u of mA is
+// candidate u of row A (fb_a + u for u < nfl_a, else dbo_a + u), bit u of mB
+// the same in row B; A's hits first, each row's in ascending order.
+template <class V>
+__device__ __forceinline__ void visit_pair(uint32_t mA, uint32_t mB, int fbA, int nA, int dA, int fbB, int nB, int dB,
+                                           V &visit) {
+    uint64_t m = (uint64_t)mA | ((uint64_t)mB << 32);
+    while (m) {
+        const int bit = __builtin_ctzll(m);
+        m &= m - 1u;
+        const bool hi = bit >= 32;
+        const int u = bit & 31;
+        const int nfl = hi ? nB : nA;
+        visit(((u < nfl) ? (hi ? fbB : fbA) : (hi ? dB : dA)) + u);
+    }
+}
 
 // No-op recorder of the mask walk's chunk masks.
 struct NoRec {
@@ -338,21 +393,52 @@ __device__ __forceinline__ void for_each_neighbor(const Prm &q, int i, float4 xi
         }
         return (cnt < 32) ? m & ((1u << cnt) - 1u) : m;
     };
-    for_each_row<FLUID_J, DUMMY_J>(q, xi, cell, cs, [&](int fb, int fe, int db, int de, int r) {
-        const int nfl = fe - fb;
-        const int total = nfl + (de - db);
-        const int dbo = db - nfl;
-        // i's own place among the row's candidates (-1: not in this row)
-        const int us = (FLUID_J && i >= fb && i < fe) ? i - fb : (DUMMY_J && i >= db && i < de) ? nfl + (i - db) : -1;
+    // i's place among a row's candidates (-1: not in the row)
+    auto self_place = [&](const RowRange &w) {
+        return (FLUID_J && i >= w.fb && i < w.fe)   ? i - w.fb
+               : (DUMMY_J && i >= w.db && i < w.de) ? (w.fe - w.fb) + (i - w.db)
+                                                    : -1;
+    };
+    // one row, 32 candidates at a time
+    auto row_chunks = [&](const RowRange &w, int r) {
+        const int nfl = w.fe - w.fb;
+        const int total = nfl + (w.de - w.db);
+        const int dbo = w.db - nfl;
+        const int us = self_place(w);
         for (int base = 0, c = 0; base < total; base += 32, ++c) {
-            uint32_t m = test_chunk(base, min(32, total - base), fb, nfl, dbo);
+            uint32_t m = test_chunk(base, min(32, total - base), w.fb, nfl, dbo);
             if ((uint32_t)(us - base) < 32u) m &= ~(1u << (us - base));
             rec(r, c, m);
             while (m) {
                 const int u = base + __builtin_ctz(m);
                 m &= m - 1u;
-                visit(((u < nfl) ? fb : dbo) + u);
+                visit(((u < nfl) ? w.fb : dbo) + u);
             }
+        }
+    };
+    for_each_row_pair<FLUID_J, DUMMY_J>(q, xi, cell, cs, [&](const RowRange &A, int rA, const RowRange &B, int rB) {
+        const int nA = A.fe - A.fb, tA = nA + (A.de - A.db);
+        const int nB = B.fe - B.fb, tB = nB + (B.de - B.db);
+        if (tA <= 32 && tB <= 32) {
+            // both rows in one chunk: one 64-bit hit mask, A's hits first
+            const int dA = A.db - nA, dB = B.db - nB;
+            uint32_t mA = 0, mB = 0;
+            if (tA) {
+                mA = test_chunk(0, tA, A.fb, nA, dA);
+                const int us = self_place(A);
+                if ((uint32_t)us < 32u) mA &= ~(1u << us);
+                rec(rA, 0, mA);
+            }
+            if (tB) {
+                mB = test_chunk(0, tB, B.fb, nB, dB);
+                const int us = self_place(B);
+                if ((uint32_t)us < 32u) mB &= ~(1u << us);
+                rec(rB, 0, mB);
+            }
+            visit_pair(mA, mB, A.fb, nA, dA, B.fb, nB, dB, visit);
+        } else {
+            row_chunks(A, rA);
+            if (rB >= 0) row_chunks(B, rB);
         }
     });
 }
@@ -360,29 +446,53 @@ __device__ __forceinline__ void for_each_neighbor(const Prm &q, int i, float4 xi
 // WALK_HITS, replay side: the walk of for_each_neighbor with each chunk's
 // hit mask read from the record (hits[(2 r + c) n + i], written by the
 // gradient pass of the same half step on the same buffer, so the rows and
-// their candidate ranges are the same) instead of tested.  Every row of the
-// particle must have at most 64 candidates.
+// their candidate ranges are the same) instead of tested, in the same order
+// (row pairs merged the same way).  Every row of the particle must have at
+// most 64 candidates.
 template <bool FLUID_J, bool DUMMY_J, class F>
 __device__ __forceinline__ void for_each_recorded(const Prm &q, int i, float4 xi, uint32_t cell,
                                                   const int *__restrict__ cs, const float4 *__restrict__ posr,
                                                   const uint32_t *__restrict__ hits, size_t n, F &&f) {
-    uint32_t m0 = 0;  // the row's first mask, loaded with its cell-table entries
-    for_each_row<FLUID_J, DUMMY_J>(q, xi, cell, cs, [&](int fb, int fe, int db, int de, int r) {
-        const int nfl = fe - fb;
-        const int total = nfl + (de - db);
-        if (m0 == 0 && total <= 32) return;  // no neighbour in this row
+    auto visit = [&](int j) {
+        const float4 xj = ld(posr, j);
+        const float dx = xi.x - xj.x, dy = xi.y - xj.y, dz = xi.z - xj.z;
+        f(j, dx, dy, dz, dist2(dx, dy, dz), xj);
+    };
+    // one row, chunk by chunk (m0: its first chunk's mask)
+    auto row_chunks = [&](const RowRange &w, int r, uint32_t m0) {
+        const int nfl = w.fe - w.fb;
+        const int total = nfl + (w.de - w.db);
         for (int base = 0, c = 0; base < total; base += 32, ++c) {
             uint32_t m = c ? hits[(size_t)(2 * r + c) * n + (size_t)i] : m0;
             while (m) {
                 const int u = base + __builtin_ctz(m);
                 m &= m - 1u;
-                const int j = ((u < nfl) ? fb : db - nfl) + u;
-                const float4 xj = ld(posr, j);
-                const float dx = xi.x - xj.x, dy = xi.y - xj.y, dz = xi.z - xj.z;
-                f(j, dx, dy, dz, dist2(dx, dy, dz), xj);
+                visit(((u < nfl) ? w.fb : w.db - nfl) + u);
             }
         }
-    }, [&](int r) { m0 = hits[(size_t)(2 * r) * n + (size_t)i]; });
+    };
+    // the first masks of the pair's rows, loaded with their cell-table entries
+    uint32_t mA = 0, mB = 0;
+    for_each_row_pair<FLUID_J, DUMMY_J>(
+        q, xi, cell, cs,
+        [&](const RowRange &A, int rA, const RowRange &B, int rB) {
+            const int nA = A.fe - A.fb, tA = nA + (A.de - A.db);
+            const int nB = B.fe - B.fb, tB = nB + (B.de - B.db);
+            // a row without candidates has no record (its load, if any, is stale)
+            const uint32_t a = tA ? mA : 0u, b = tB ? mB : 0u;
+            if (tA <= 32 && tB <= 32) {
+                if ((a | b) == 0u) return;  // no neighbour in either row
+                visit_pair(a, b, A.fb, nA, A.db - nA, B.fb, nB, B.db - nB, visit);
+            } else {
+                row_chunks(A, rA, a);
+                if (rB >= 0) row_chunks(B, rB, b);
+            }
+        },
+        [&](int r, int side) {
+            const uint32_t m = hits[(size_t)(2 * r) * n + (size_t)i];
+            if (side) mB = m;
+            else mA = m;
+        });
 }
 
 // Nontemporal (streaming) 16-byte store and load.

@@ -79,8 +79,11 @@ __global__ __launch_bounds__(ZS_BLOCK) void k_wall(Prm q0, const Prm *__restrict
 // not depend on j).  MC form: D only.  Output gd = (G, D); dummies get 0.
 // FEAT: also the closure network's ZSPH_NF features of each fluid particle
 // (feat[i * ZSPH_NF ...], see closure_features) from the same neighbour walk.
+// Compiled for at least 7 waves per SIMD (at most 72 VGPRs; the compiler's
+// own choice, 77, allows 6) in the single-scene mask and hit-mask walks:
+// measured 5 % faster.  8 waves (64 VGPRs) spill and measured slower.
 template <int WALK, bool BATCH, bool FEAT, int DF>
-__global__ __launch_bounds__(ZS_BLOCK) void k_ldg(Prm q0, const Prm *__restrict__ sp, Buf b,
+__global__ __launch_bounds__(ZS_BLOCK, ((WALK != WALK_LIST && !BATCH && !FEAT) ? 7 : 1)) void k_ldg(Prm q0, const Prm *__restrict__ sp, Buf b,
                                                   const uint32_t *__restrict__ skey, const int *__restrict__ cs,
                                                   float4 *__restrict__ gd, int *__restrict__ nlist,
                                                   int *__restrict__ ncount, float *__restrict__ feat) {
@@ -225,8 +228,9 @@ __global__ __launch_bounds__(ZS_BLOCK) void k_ldg(Prm q0, const Prm *__restrict_
 // (bulk + artificial viscosity) by scale[i].y.  MODE 2: plain, and the parts
 // are written to terms[i * 8 ...] = (continuity, diffusion, pressure + g (3),
 // viscous (3)) for training.
+// At least 7 waves per SIMD in the single-scene plain walks (as k_ldg).
 template <int WALK, bool BATCH, int MODE, int DF>
-__global__ __launch_bounds__(ZS_BLOCK) void k_governing(Prm q0, const Prm *__restrict__ sp, Buf b,
+__global__ __launch_bounds__(ZS_BLOCK, ((WALK != WALK_LIST && !BATCH && MODE == 0) ? 7 : 1)) void k_governing(Prm q0, const Prm *__restrict__ sp, Buf b,
                                                         const uint32_t *__restrict__ skey, const int *__restrict__ cs,
                                                         const float4 *__restrict__ gd, float4 *__restrict__ acc,
                                                         const int *__restrict__ nlist, const int *__restrict__ ncount,
