@@ -103,7 +103,8 @@ def main():
     ap.add_argument("--profile-steps", type=int, default=3)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-autotune", dest="autotune", action="store_false")
-    ap.add_argument("--walk", type=int, default=None, help="force a neighbour walk (0 mask, 1 queue, 2 list)")
+    ap.add_argument("--walk", type=int, default=None, help="force a neighbour walk (0 mask, 1 hit masks, 2 lists)")
+    ap.add_argument("--radix-bits", type=int, default=None, help="force the sort's digit width (8 to 11 bits)")
     ap.add_argument("--tune-at", type=int, default=None,
                     help="autotune on the state after this many steps, then restart from the initial state "
                          "(default: 30%% of the timed steps)")
@@ -157,8 +158,10 @@ def main():
     if setup:
         setup(s)
     # pick the neighbour-walk variant and sort digit width on this case (state restored after)
-    if args.walk is not None:
-        s.set_tuning(args.walk, s.get_tuning()["radix_bits"])
+    if args.walk is not None or args.radix_bits is not None:
+        t = s.get_tuning()
+        s.set_tuning(t["walk"] if args.walk is None else args.walk,
+                     t["radix_bits"] if args.radix_bits is None else args.radix_bits)
         args.autotune = False
     if args.autotune:
         # The walks' relative speed changes as the flow develops, so the

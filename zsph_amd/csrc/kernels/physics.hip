@@ -142,7 +142,9 @@ __global__ __launch_bounds__(ZS_BLOCK, ((WALK != WALK_LIST && !BATCH && !FEAT) ?
         }
     };
     auto walk_body = [&](int j, float dx, float dy, float dz, float r2, float4 xj) {
-        body(j, dx, dy, dz, r2, xj, (j < nf) ? ld(b.velp, j) : make_float4(0.0f, 0.0f, 0.0f, 0.0f));
+        float4 vj;  // loaded, and read by the body, for fluid neighbours only
+        if (j < nf) vj = ld(b.velp, j);
+        body(j, dx, dy, dz, r2, xj, vj);
     };
     const uint32_t cell = cell_of(q, skey[i]);
     if (WALK == WALK_LIST) {  // the mask walk, recording what it visits
@@ -282,7 +284,8 @@ __global__ __launch_bounds__(ZS_BLOCK, ((WALK != WALK_LIST && !BATCH && MODE == 
     };
     auto walk_body = [&](int j, float dx, float dy, float dz, float r2, float4 xj) {
         const float4 vj = ld(b.velp, j);
-        const float4 gj = (j < nf) ? ld(gd, j) : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        float4 gj;  // loaded, and read by the body, for fluid neighbours only
+        if (j < nf) gj = ld(gd, j);
         body(j, dx, dy, dz, r2, xj, vj, gj);
     };
     const uint32_t cell = cell_of(q, skey[i]);
