@@ -43,9 +43,9 @@ HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
 # make a 512 MiB device copy read 512 MiB and write 512 MiB).  Other workloads
 # report no measured traffic.
 GOVERNING_TRAFFIC_BYTES = {
-    ("dambreak3d", 0.008, 2.0, 0): (77446.5 * 2.0 + 20214.8) * 1024.0,
-    ("dambreak3d", 0.008, 2.0, 1): (171708.0 * 2.0 + 20214.8) * 1024.0,
-    ("dambreak3d", 0.008, 2.0, 2): (304137.4 * 2.0 + 20214.8) * 1024.0,
+    ("dambreak3d", 0.008, 2.0, 0): (252352.5 * 2.0 + 20214.8) * 1024.0,
+    ("dambreak3d", 0.008, 2.0, 1): (329708.6 * 2.0 + 20214.8) * 1024.0,
+    ("dambreak3d", 0.008, 2.0, 2): (337566.0 * 2.0 + 20214.8) * 1024.0,
 }
 
 
