@@ -291,8 +291,9 @@ __device__ __forceinline__ void for_each_row(const Prm &q, float4 xi, uint32_t c
 // Neighbour-walk variants (chosen per context by measurement):
 //  WALK_MASK:  each row's candidates (fluid range, then the others) are
 //              tested 32 at a time into a per-lane hit mask, then the hits
-//              are visited; the wave runs the interaction body once per hit
-//              of its busiest lane in that chunk.
+//              are visited (a point-mirror row pair's as one sequence when
+//              both rows fit one chunk); the wave runs the interaction body
+//              once per hit of its busiest lane in that sequence.
 //  WALK_HITS:  the gradient pass runs the mask walk and records every row
 //              chunk's hit mask; the continuity+momentum pass of the same
 //              half step walks the same rows and visits the recorded hits
@@ -320,12 +321,6 @@ inline size_t list_words(int n) { return ((size_t)(n + 63) >> 6) * 64 * LIST_MAX
 // slot-major over the particles (hits[slot * N + i]), so a wave's lanes
 // storing or loading the same slot touch 256 contiguous bytes.
 inline int hits_slots(int k) { return 2 * (2 * k + 1) * (2 * k + 1); }
-// LDS a walk needs per block (none; kept so kernels declare one shape).
-template <int WALK>
-struct WalkLds {
-    static constexpr int ints = 1;
-};
-
 // Visits the hits of two one-chunk rows as one sequence: bit u of mA is
 // candidate u of row A (fb_a + u for u < nfl_a, else dbo_a + u), bit u of mB
 // the same in row B; A's hits first, each row's in ascending order.
@@ -351,17 +346,17 @@ struct NoRec {
 // Visit every neighbour j != i (FP32 r^2 <= rc2) of particle i at xi whose
 // segment is enabled (FLUID_J: j < cs[G]; DUMMY_J: j >= cs[G]).
 // f(j, dx, dy, dz, r2, xj) with x_ij = xi - xj.  Every walk variant visits
-// the neighbours in this order: rows in for_each_row order, each row's
-// candidates (fluid range, then the others) tested 32 at a time into a
-// per-lane hit mask, then the hits in ascending order; the wave runs f once
-// per hit of its busiest lane in each chunk.  rec(r, c, m) receives the hit
-// mask m of chunk c of box row r.  qbuf: WalkLds<WALK>::ints ints of LDS per
-// block (unused).
+// the neighbours in this order: row pairs in for_each_row_pair order, row A
+// before row B, each row's candidates (fluid range, then the others) in
+// ascending order.  The candidates are tested 32 at a time into a per-lane
+// hit mask; when both rows of a pair have at most 32 candidates their hits
+// are visited as one sequence, else row by row, chunk by chunk.  The wave
+// runs f once per hit of its busiest lane in each sequence.  rec(r, c, m)
+// receives the hit mask m of chunk c of box row r.
 template <int WALK, bool FLUID_J, bool DUMMY_J, class F, class Rec = NoRec>
 __device__ __forceinline__ void for_each_neighbor(const Prm &q, int i, float4 xi, uint32_t cell,
-                                                  const int *__restrict__ cs, const float4 *__restrict__ posr,
-                                                  int *__restrict__ qbuf, F &&f, Rec &&rec = Rec{}) {
-    (void)qbuf;
+                                                  const int *__restrict__ cs, const float4 *__restrict__ posr, F &&f,
+                                                  Rec &&rec = Rec{}) {
     auto visit = [&](int j) {
         const float4 xj = ld(posr, j);
         const float dx = xi.x - xj.x, dy = xi.y - xj.y, dz = xi.z - xj.z;

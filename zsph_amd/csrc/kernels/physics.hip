@@ -48,7 +48,6 @@ template <int WALK, bool BATCH>
 __global__ __launch_bounds__(ZS_BLOCK) void k_wall(Prm q0, const Prm *__restrict__ sp, Buf b,
         const uint32_t *__restrict__ skey,
                                                    const int *__restrict__ cs) {
-    __shared__ int qbuf[WalkLds<WALK>::ints];
     const int nf = cs[q0.G];
     // dummy blocks round-robin over the XCDs: dummies far from the fluid
     // cost little, so contiguous ranges of them would load the XCDs unevenly
@@ -58,7 +57,7 @@ __global__ __launch_bounds__(ZS_BLOCK) void k_wall(Prm q0, const Prm *__restrict
     const float4 xi = b.posr[i];
     const float4 aw = b.tvelp[i];
     float P = 0.0f, S = 0.0f;
-    for_each_neighbor<WALK, true, false>(q, i, xi, cell_of(q, skey[i]), cs, b.posr, qbuf,
+    for_each_neighbor<WALK, true, false>(q, i, xi, cell_of(q, skey[i]), cs, b.posr,
                                    [&](int j, float dx, float dy, float dz, float r2, float4 xj) {
                                        const float qq = hw_sqrt(r2) * q.inv_h;
                                        const float t = 1.0f - 0.5f * qq;
@@ -87,7 +86,6 @@ __global__ __launch_bounds__(ZS_BLOCK, ((WALK != WALK_LIST && !BATCH && !FEAT) ?
                                                   const uint32_t *__restrict__ skey, const int *__restrict__ cs,
                                                   float4 *__restrict__ gd, int *__restrict__ nlist,
                                                   int *__restrict__ ncount, float *__restrict__ feat) {
-    __shared__ int qbuf[WalkLds<WALK>::ints];
     const int nf = cs[q0.G];
     const int i = xcd_block(blocks_of(nf)) * ZS_BLOCK + threadIdx.x;
     if (i >= q0.N) return;
@@ -149,7 +147,7 @@ __global__ __launch_bounds__(ZS_BLOCK, ((WALK != WALK_LIST && !BATCH && !FEAT) ?
     const uint32_t cell = cell_of(q, skey[i]);
     if (WALK == WALK_LIST) {  // the mask walk, recording what it visits
         ListWriter lw(nlist, i);
-        for_each_neighbor<WALK_MASK, true, true>(q, i, xi, cell, cs, b.posr, qbuf,
+        for_each_neighbor<WALK_MASK, true, true>(q, i, xi, cell, cs, b.posr,
                                                  [&](int j, float dx, float dy, float dz, float r2, float4 xj) {
                                                      lw.put(j);
                                                      walk_body(j, dx, dy, dz, r2, xj);
@@ -158,14 +156,14 @@ __global__ __launch_bounds__(ZS_BLOCK, ((WALK != WALK_LIST && !BATCH && !FEAT) ?
     } else if (WALK == WALK_HITS) {  // the mask walk, recording each chunk's hit mask
         uint32_t *hits = reinterpret_cast<uint32_t *>(nlist);
         bool over = false;  // a row with more than 64 candidates: not recorded
-        for_each_neighbor<WALK_MASK, true, true>(q, i, xi, cell, cs, b.posr, qbuf, walk_body,
+        for_each_neighbor<WALK_MASK, true, true>(q, i, xi, cell, cs, b.posr, walk_body,
                                                  [&](int r, int c, uint32_t m) {
                                                      if (c < 2) hits[(size_t)(2 * r + c) * (size_t)q0.N + (size_t)i] = m;
                                                      else over = true;
                                                  });
         ncount[i] = over ? 1 : 0;
     } else {
-        for_each_neighbor<WALK, true, true>(q, i, xi, cell, cs, b.posr, qbuf, walk_body);
+        for_each_neighbor<WALK, true, true>(q, i, xi, cell, cs, b.posr, walk_body);
     }
     const float trL = lxx + lyy + lzz;
     if (ell && q.dim == 2) lyy += 1.0f;  // x-z plane: unit yy entry keeps L invertible
@@ -237,7 +235,6 @@ __global__ __launch_bounds__(ZS_BLOCK, ((WALK != WALK_LIST && !BATCH && MODE == 
                                                         const float4 *__restrict__ gd, float4 *__restrict__ acc,
                                                         const int *__restrict__ nlist, const int *__restrict__ ncount,
                                                         const float2 *__restrict__ scale, float *__restrict__ terms) {
-    __shared__ int qbuf[WalkLds<WALK>::ints];
     const int nf = cs[q0.G];
     const int i = xcd_block(blocks_of(nf)) * ZS_BLOCK + threadIdx.x;
     if (i >= nf) return;
@@ -326,7 +323,7 @@ __global__ __launch_bounds__(ZS_BLOCK, ((WALK != WALK_LIST && !BATCH && MODE == 
             }
         }
     } else {
-        for_each_neighbor<WALK_MASK, true, true>(q, i, xi, cell, cs, b.posr, qbuf, walk_body);
+        for_each_neighbor<WALK_MASK, true, true>(q, i, xi, cell, cs, b.posr, walk_body);
     }
     if constexpr (MODE == 0) {
         acc[i] = make_float4(ax, ay, az + q.g, dr);
@@ -348,7 +345,6 @@ template <int WALK, bool BATCH>
 __global__ __launch_bounds__(ZS_BLOCK) void k_governing_dummy(Prm q0, const Prm *__restrict__ sp, Buf b,
         const uint32_t *__restrict__ skey,
                                                               const int *__restrict__ cs, float4 *__restrict__ acc) {
-    __shared__ int qbuf[WalkLds<WALK>::ints];
     const int nf = cs[q0.G];
     const int i = nf + blockIdx.x * ZS_BLOCK + threadIdx.x;
     if (i >= q0.N) return;
@@ -359,7 +355,7 @@ __global__ __launch_bounds__(ZS_BLOCK) void k_governing_dummy(Prm q0, const Prm 
         const float4 xi = b.posr[i];
         const float4 vi = b.velp[i];
         const float rhoi = xi.w, pi = vi.w;
-        for_each_neighbor<WALK, true, false>(q, i, xi, cell_of(q, skey[i]), cs, b.posr, qbuf,
+        for_each_neighbor<WALK, true, false>(q, i, xi, cell_of(q, skey[i]), cs, b.posr,
                                        [&](int j, float dx, float dy, float dz, float r2, float4 xj) {
             const float qq = hw_sqrt(r2) * q.inv_h;
             const float t = 1.0f - 0.5f * qq;
@@ -430,14 +426,13 @@ __global__ __launch_bounds__(ZS_BLOCK) void k_nbr_stats(Prm q0, const Prm *__res
         const uint32_t *__restrict__ skey,
                                                         const int *__restrict__ cs, int *__restrict__ count,
                                                         uint64_t *__restrict__ digest) {
-    __shared__ int qbuf[WalkLds<WALK>::ints];
     const int i = blockIdx.x * ZS_BLOCK + threadIdx.x;
     if (i >= q0.N) return;
     const Prm q = scene_prm<BATCH>(q0, sp, b, i);
     const float4 xi = b.posr[i];
     int c = 0;
     uint64_t h = 0;
-    for_each_neighbor<WALK, true, true>(q, i, xi, cell_of(q, skey[i]), cs, b.posr, qbuf,
+    for_each_neighbor<WALK, true, true>(q, i, xi, cell_of(q, skey[i]), cs, b.posr,
                                   [&](int j, float, float, float, float, float4) {
                                       ++c;
                                       h += mix64((uint64_t)j);
