@@ -103,6 +103,29 @@ def test_batch_steps_match_standalone_scenes(walk):
     sb.close()
 
 
+@pytest.mark.parametrize("walk", [z.WALK_MASK, z.WALK_HITS, z.WALK_LIST])
+def test_mixed_diffusion_batch_matches_standalone_scenes(walk):
+    # Antuono and Molteni-Colagrossi scenes in one batch: the batched kernels
+    # read each particle's diffusion form from its scene's parameters, the
+    # standalone ones have it fixed at compile time; the bits must agree
+    anto = cases.ensemble2d(n_scenes=3, seed=5).cases
+    mc = cases.ensemble2d(n_scenes=3, seed=6, diffusion=z.DIFF_MC).cases
+    b = z.Batch([c for pair in zip(anto, mc) for c in pair])
+    assert {c.params.diffusion for c in b.cases} == {z.DIFF_ANTUONO, z.DIFF_MC}
+    say(f"[mixed diffusion batch] {len(b.cases)} scenes, N={b.n}")
+    steps = 8
+    sb = z.Solver(b)
+    sb.upload(b.host)
+    sb.set_tuning(walk, sb.get_tuning()["radix_bits"])
+    sb.step(steps)
+    sb.sync()
+    for k, c in enumerate(b.cases):
+        ss = standalone(c, steps, walk)
+        compare_scene(sb, b, k, ss)
+        ss.close()
+    sb.close()
+
+
 def test_batch_graph_replay_is_deterministic_and_tuning_independent():
     b = uneven_batch(6, seed=3)
     runs = []
