@@ -216,10 +216,33 @@ __global__ __launch_bounds__(64) void k_mlp(const float *__restrict__ P, const b
     const int l = lane();
     if (n_dev) n = *n_dev;
 
-    // weights (16-byte copies), zeroed activations and gradient accumulators
-    for (int e = 8 * l; e < PACK; e += 8 * 64)
-        *reinterpret_cast<bf16x8 *>(W + e) = *reinterpret_cast<const bf16x8 *>(pack + e);
-    for (int e = l; e < (TRAIN ? A_ALL : A_INF); e += 64) act[e] = (bf16)0.0f;
+    // weights (16-byte copies, seven loads in flight before their stores),
+    // zeroed activations (16-byte stores) and gradient accumulators
+    {
+        constexpr int STEP = 8 * 64, PASSES = (PACK + STEP - 1) / STEP, BATCH = 7;
+        static_assert(PACK % 8 == 0, "weights copy in 8-element pieces");
+        for (int p0 = 0; p0 < PASSES; p0 += BATCH) {
+            bf16x8 tmp[BATCH];
+#pragma unroll
+            for (int k = 0; k < BATCH; ++k) {
+                const int e = (p0 + k) * STEP + 8 * l;
+                if (e < PACK) tmp[k] = *reinterpret_cast<const bf16x8 *>(pack + e);
+            }
+#pragma unroll
+            for (int k = 0; k < BATCH; ++k) {
+                const int e = (p0 + k) * STEP + 8 * l;
+                if (e < PACK) *reinterpret_cast<bf16x8 *>(W + e) = tmp[k];
+            }
+        }
+    }
+    {
+        constexpr int ACT = TRAIN ? A_ALL : A_INF;
+        static_assert(ACT % 8 == 0, "activations cleared in 8-element pieces");
+        bf16x8 zero;
+#pragma unroll
+        for (int k = 0; k < 8; ++k) zero[k] = (bf16)0.0f;
+        for (int e = 8 * l; e < ACT; e += 8 * 64) *reinterpret_cast<bf16x8 *>(act + e) = zero;
+    }
     if (TRAIN)
         for (int e = l; e < MLP_NPARAM; e += 64) dW[e] = 0.0f;
     if (l == 0) lsum[0] = 0.0f;
