@@ -38,9 +38,11 @@ void launch_ldg(hipStream_t s, int walk, const Prm &q, const Prm *sp, const Buf 
 // mode (one scene only): 0 plain, 1 closure-scaled (scale: 2 floats per
 // particle), 2 plain + term decomposition (terms: 8 floats per particle).
 // clock (may be nullptr): device stamps around the continuity+momentum kernel (launch_clock).
+// moving: the context has moving dummies (a body or a piston), whose
+// pressure force k_governing_dummy computes; fixed walls get zero either way.
 void launch_governing(hipStream_t s, int walk, const Prm &q, const Prm *sp, const Buf &b, const uint32_t *skey,
                       const int *cs, const float4 *gd, float4 *acc, const int *nlist, const int *ncount, int mode,
-                      const float2 *scale, float *terms, unsigned long long *clock = nullptr);
+                      const float2 *scale, float *terms, unsigned long long *clock, bool moving);
 void launch_predict(hipStream_t s, const Prm &q, const Prm *sp, const Buf &b, const int *cs, const float4 *acc);
 void launch_correct(hipStream_t s, const Prm &q, const Prm *sp, const Buf &b, const int *cs, const float4 *acc);
 void launch_nbr_stats(hipStream_t s, int walk, const Prm &q, const Prm *sp, const Buf &b, const uint32_t *skey,
@@ -89,9 +91,8 @@ struct MlpAux {
     float out_scale, w_rho, w_acc;
 };
 size_t mlp_pack_elems();          // BF16 elements of the packed weight image
-int mlp_blocks(int n);
+int mlp_blocks(int n);        // blocks of a training launch over n samples (partials buffer size)
 int mlp_infer_blocks(int n);  // blocks for launch_mlp_infer over n samples
-           // blocks of a training launch over n samples (partials buffer size)
 hipError_t mlp_prepare();         // allows the kernels' dynamic LDS (call once per device)
 void launch_mlp_pack(hipStream_t s, const float *P, void *pack);
 // n samples, or *n_dev when n_dev is given; out: n x 2 scale factors.

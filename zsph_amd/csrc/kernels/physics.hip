@@ -237,7 +237,12 @@ __global__ __launch_bounds__(ZS_BLOCK, ((WALK != WALK_LIST && !BATCH && MODE == 
                                                         const float2 *__restrict__ scale, float *__restrict__ terms) {
     const int nf = cs[q0.G];
     const int i = xcd_block(blocks_of(nf)) * ZS_BLOCK + threadIdx.x;
-    if (i >= nf) return;
+    if (i >= nf) {
+        // dummies and removed particles: 0 (k_governing_dummy, launched after
+        // this kernel when there are moving dummies, overwrites theirs)
+        if (i < q0.N) acc[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        return;
+    }
     const Prm q = scene_prm<BATCH>(q0, sp, b, i);
     const bool delta = is_antuono<DF>(q);
     const float4 xi = b.posr[i];
@@ -519,7 +524,7 @@ static void gov_walk(hipStream_t s, int mode, int grid, const Prm &q, const Prm 
 
 void launch_governing(hipStream_t s, int walk, const Prm &q, const Prm *sp, const Buf &b, const uint32_t *skey,
                       const int *cs, const float4 *gd, float4 *acc, const int *nlist, const int *ncount, int mode,
-                      const float2 *scale, float *terms, unsigned long long *clock) {
+                      const float2 *scale, float *terms, unsigned long long *clock, bool moving) {
     if (!q.N) return;
     const int g = nblocks(q.N);
     if (clock) launch_clock(s, clock, true);
@@ -534,7 +539,7 @@ void launch_governing(hipStream_t s, int walk, const Prm &q, const Prm *sp, cons
         else gov_walk<WALK_MASK, false>(s, mode, g, q, sp, b, skey, cs, gd, acc, nlist, ncount, scale, terms);
     }
     if (clock) launch_clock(s, clock, false);
-    ZS_WALK_LAUNCH(k_governing_dummy, nblocks(q.N), q, sp, b, skey, cs, acc);
+    if (moving) ZS_WALK_LAUNCH(k_governing_dummy, nblocks(q.N), q, sp, b, skey, cs, acc);
 }
 void launch_predict(hipStream_t s, const Prm &q, const Prm *sp, const Buf &b, const int *cs, const float4 *acc) {
     if (!q.N) return;

@@ -213,7 +213,7 @@ int stage_governing(zsph_ctx *c, int b, Prof *pf = nullptr) {
                                  c->scale, zs::mlp_infer_blocks(c->q.N));
         zs::launch_governing(c->stream, walk, c->q, c->scene_prm, c->buf[b], c->key[c->sorted_slot], c->cell_start,
                              c->gd, c->acc, walk_record(c), c->ncount, c->closure_on ? 1 : 0, c->scale, c->terms,
-                             c->clock_on ? c->clock : nullptr);
+                             c->clock_on ? c->clock : nullptr, c->has_body || c->has_piston);
     }
     return check_launch(c, "governing");
 }
@@ -930,7 +930,7 @@ int zsph_closure_sample(zsph_ctx *c, int b, float *features, float *terms) {
     zs::launch_wall(c->stream, walk, c->q, nullptr, c->buf[b], sk, c->cell_start);
     zs::launch_ldg(c->stream, walk, c->q, nullptr, c->buf[b], sk, c->cell_start, c->gd, c->nlist, c->ncount, c->feat);
     zs::launch_governing(c->stream, walk, c->q, nullptr, c->buf[b], sk, c->cell_start, c->gd, c->acc, c->nlist,
-                         c->ncount, 2, c->scale, c->terms);
+                         c->ncount, 2, c->scale, c->terms, nullptr, c->has_body || c->has_piston);
     if ((r = check_launch(c, "closure sample"))) return r;
     c->list_valid = false;
     if ((r = copy_sync(c, features, c->feat, N * ZSPH_NF * sizeof(float), hipMemcpyDeviceToHost))) return r;
