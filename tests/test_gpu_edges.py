@@ -13,6 +13,9 @@
   walk takes its fallback. Hash, sort and neighbour sets are bit-exact; the
   passes are within the tolerances of tests/helpers.py; the walks give the
   same bits over one step (both half steps).
+* A crowded cell: 5,000 particles inside one cube of edge h/4, more than
+  the cell-table kernel stages per block, so its entries take the search in
+  global memory; the table equals the oracle's bit for bit.
 * Particles on and beside every cell face: the cell hash equals the
   oracle's, i.e. the floor of the correctly rounded FP32 quotient
   (x - o) / size, also where that quotient rounds up to an integer from an
@@ -28,7 +31,7 @@ import pytest
 import oracle
 import zsph_amd as z
 from oracle import A_ACC_SCALE, A_GD_SCALE
-from zsph_amd import A_ACC, A_GD, A_INDEX, A_KEY, A_POSR, A_TYPE, A_VELP, FLUID, cases
+from zsph_amd import A_ACC, A_CELL_START, A_GD, A_INDEX, A_KEY, A_POSR, A_TYPE, A_VELP, FLUID, cases
 from tests.helpers import assert_close_scaled, jitter
 
 pytestmark = pytest.mark.gpu
@@ -254,3 +257,28 @@ def p_flag(c):
     while b < 30 and (1 << b) < g:
         b += 1
     return 1 << b
+
+
+def test_cell_table_with_a_crowded_cell():
+    c = jitter(cases.dambreak3d(dx=0.03, grid_size_factor=2.0), seed=8, vmax=0.01, drho=1e-4)
+    f = fluid_ids(c)
+    assert f.size > 5000
+    rng = np.random.default_rng(9)
+    h = c.params.h
+    pick = f[:5000]
+    c.host.pos[pick] = (np.array((1.5, 0.5, 0.8)) + rng.uniform(-0.125 * h, 0.125 * h, (pick.size, 3))).astype(np.float32)
+    say(f"[crowded cell] N={c.n}, 5000 particles in a cube of edge h/4")
+    s = z.Solver(c)
+    s.upload(c.host)
+    o = oracle.Oracle(c.params)
+    o.upload(c.host)
+    for x in (s, o):
+        x.hash(0); x.sort_index(); x.sort_data(0, 1)
+    assert np.array_equal(s.get(A_KEY), o.get(A_KEY).astype(np.uint32))
+    assert np.array_equal(s.get(A_CELL_START), o.get(A_CELL_START).astype(np.int32))
+    cg, dg = s.neighbor_stats(1)
+    co, do = o.neighbor_stats(1)
+    assert np.array_equal(cg, co) and np.array_equal(dg, do)
+    assert cg.max() >= 4999
+    s.close()
+    o.close()

@@ -221,115 +221,82 @@ __global__ __launch_bounds__(ZS_BLOCK) void k_radix_scatter(const uint32_t *__re
 // cell c; cs[2G+1] = N).  Empty cells have equal bounds.  Every entry is
 // rewritten on every call.
 // ---------------------------------------------------------------------------
-// Built in O(N + G) rather than a binary search per entry: every entry is
-// first set to n, the first particle of each key writes its index into the
-// key's entry, and a suffix-minimum scan then gives each empty cell the
-// start of the next occupied one (which is exactly its lower bound).  Entry
-// of a key: key for fluid (key < flag), G + 1 + (key - flag) otherwise.
-constexpr int CS_ITEMS = 16;
+// One pass: the key of entry e, ek(e) = e for e <= G and flag + (e - G - 1)
+// above, is non-decreasing in e (flag >= G), so cs[e] is the lower bound of
+// ek(e) in the sorted keys.  A block owns CS_TILE consecutive entries; one
+// wave finds the sorted positions of its first and last entry keys with
+// 64-way searches (4 dependent rounds for 4 M keys), the keys between them
+// are staged in LDS, and each entry is a binary search there (in global
+// memory when they do not fit, e.g. hundreds of particles in one cell).
+constexpr int CS_ITEMS = 4;
 constexpr int CS_TILE = ZS_BLOCK * CS_ITEMS;
-constexpr int CS_PAD = CS_ITEMS + 1;  // LDS row stride: 17 is coprime with the 64 banks
+constexpr int CS_KEYS = 4096;  // staged keys per block (16 KB of LDS)
 
-__global__ __launch_bounds__(ZS_BLOCK) void k_cs_fill(int E, int n, int *__restrict__ cs) {
-    for (int e = blockIdx.x * ZS_BLOCK + threadIdx.x; e < E; e += gridDim.x * ZS_BLOCK) cs[e] = n;
+__device__ __forceinline__ uint32_t cs_entry_key(int e, int G, uint32_t flag) {
+    return (e <= G) ? (uint32_t)e : flag + (uint32_t)(e - G - 1);
 }
 
-__global__ __launch_bounds__(ZS_BLOCK) void k_cs_mark(const uint32_t *__restrict__ skey, int n, int G,
-                                                      uint32_t flag, int *__restrict__ cs) {
-    const int i = blockIdx.x * ZS_BLOCK + threadIdx.x;
-    if (i >= n) return;
-    const uint32_t k = skey[i];
-    if (i > 0 && skey[i - 1] == k) return;
-    cs[k < flag ? (int)k : G + 1 + (int)(k - flag)] = i;
-}
-
-__device__ __forceinline__ int wave_min(int v) {
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) v = min(v, __shfl_xor(v, o, 64));
-    return v;
-}
-
-// Minimum of each tile of CS_TILE entries.
-__global__ __launch_bounds__(ZS_BLOCK) void k_cs_tile_min(const int *__restrict__ cs, int E,
-                                                          int *__restrict__ tmin) {
-    __shared__ int wm[ZS_BLOCK / 64];
-    const int base = blockIdx.x * CS_TILE;
-    int m = INT_MAX;
-#pragma unroll
-    for (int j = 0; j < CS_ITEMS; ++j) {
-        const int e = base + j * ZS_BLOCK + (int)threadIdx.x;
-        if (e < E) m = min(m, cs[e]);
+// Lower bound of k in a[lo, hi) by one whole wave, 64 probes per round: the
+// probes below k form a prefix, so their count narrows [lo, hi] to one probe
+// interval.  Every lane returns the result.
+__device__ __forceinline__ int wave_lower_bound(const uint32_t *__restrict__ a, int lo, int hi, uint32_t k) {
+    const int lane = (int)threadIdx.x & 63;
+    while (lo < hi) {
+        const int step = (hi - lo + 63) / 64;
+        const int s = lo + lane * step;
+        const bool below = s < hi && a[s] < k;
+        const int c = __popcll(__ballot(below));
+        if (c == 0) {
+            hi = lo;
+        } else {
+            const int sc = lo + c * step;
+            lo += (c - 1) * step + 1;
+            if (c < 64 && sc < hi) hi = sc;
+        }
     }
-    m = wave_min(m);
-    if ((threadIdx.x & 63) == 0) wm[threadIdx.x >> 6] = m;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        for (int w = 1; w < ZS_BLOCK / 64; ++w) m = min(m, wm[w]);
-        tmin[blockIdx.x] = m;
-    }
+    return lo;
 }
 
-// Inclusive suffix minimum of v over the block (LDS Hillis-Steele from the right).
-__device__ __forceinline__ int block_suffix_min(int v, int *s) {
-    s[threadIdx.x] = v;
-    __syncthreads();
-    for (int o = 1; o < ZS_BLOCK; o <<= 1) {
-        const int t = ((int)threadIdx.x + o < ZS_BLOCK) ? s[threadIdx.x + o] : INT_MAX;
-        __syncthreads();
-        v = min(v, t);
-        s[threadIdx.x] = v;
-        __syncthreads();
+// Lower bound of k in a[lo, hi) by one thread.
+template <class A>
+__device__ __forceinline__ int lower_bound_in(const A *a, int lo, int hi, uint32_t k) {
+    while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (a[mid] < k) lo = mid + 1;
+        else hi = mid;
     }
-    return v;
+    return lo;
 }
 
-// One block: carry[t] = min(n, tmin[t+1..T-1]), chunks of ZS_BLOCK from the right.
-__global__ __launch_bounds__(ZS_BLOCK) void k_cs_carry(const int *__restrict__ tmin, int T, int n,
-                                                       int *__restrict__ carry) {
-    __shared__ int s[ZS_BLOCK];
-    int right = n;  // minimum of everything right of the current chunk
-    for (int hi = T; hi > 0; hi -= ZS_BLOCK) {
-        const int lo = max(0, hi - ZS_BLOCK);
-        const int t = lo + (int)threadIdx.x;
-        block_suffix_min(t < hi ? tmin[t] : INT_MAX, s);
-        const int next = ((int)threadIdx.x + 1 < ZS_BLOCK) ? s[threadIdx.x + 1] : INT_MAX;
-        if (t < hi) carry[t] = min(right, next);
-        right = min(right, s[0]);
-        __syncthreads();
-    }
-}
-
-// Suffix minimum inside each tile, seeded with the tile's carry; in place.
-__global__ __launch_bounds__(ZS_BLOCK) void k_cs_apply(int *__restrict__ cs, int E,
-                                                       const int *__restrict__ carry) {
-    __shared__ int tile[ZS_BLOCK * CS_PAD];
-    __shared__ int s[ZS_BLOCK];
-    const int base = blockIdx.x * CS_TILE;
-    // coalesced load; tile-local entry l belongs to thread l / 16, item l % 16
-#pragma unroll
-    for (int j = 0; j < CS_ITEMS; ++j) {
-        const int l = j * ZS_BLOCK + (int)threadIdx.x;
-        const int e = base + l;
-        tile[(l / CS_ITEMS) * CS_PAD + l % CS_ITEMS] = (e < E) ? cs[e] : INT_MAX;
+__global__ __launch_bounds__(ZS_BLOCK) void k_cell_start(const uint32_t *__restrict__ skey, int n, int G,
+                                                         uint32_t flag, int *__restrict__ cs) {
+    __shared__ int bound[2];
+    __shared__ uint32_t keys[CS_KEYS];
+    const int E = 2 * G + 2;
+    const int e0 = blockIdx.x * CS_TILE;
+    const int e1 = min(e0 + CS_TILE, E);
+    if (threadIdx.x < 64) {  // the first wave, whole: the searches use ballots
+        const int a = wave_lower_bound(skey, 0, n, cs_entry_key(e0, G, flag));
+        const int b = wave_lower_bound(skey, a, n, cs_entry_key(e1 - 1, G, flag));
+        if (threadIdx.x == 0) {
+            bound[0] = a;
+            bound[1] = b;
+        }
     }
     __syncthreads();
-    int *mine = tile + threadIdx.x * CS_PAD;
-    int m = INT_MAX;
-#pragma unroll
-    for (int j = 0; j < CS_ITEMS; ++j) m = min(m, mine[j]);
-    block_suffix_min(m, s);
-    int run = min(carry[blockIdx.x], ((int)threadIdx.x + 1 < ZS_BLOCK) ? s[threadIdx.x + 1] : INT_MAX);
-#pragma unroll
-    for (int j = CS_ITEMS - 1; j >= 0; --j) {
-        run = min(run, mine[j]);
-        mine[j] = run;
-    }
+    // every entry of the tile has its lower bound in [lo, lo + len]
+    const int lo = bound[0], len = bound[1] - bound[0];
+    const bool staged = len <= CS_KEYS;
+    if (staged)
+        for (int j = (int)threadIdx.x; j < len; j += ZS_BLOCK) keys[j] = skey[lo + j];
     __syncthreads();
 #pragma unroll
     for (int j = 0; j < CS_ITEMS; ++j) {
-        const int l = j * ZS_BLOCK + (int)threadIdx.x;
-        const int e = base + l;
-        if (e < E) cs[e] = tile[(l / CS_ITEMS) * CS_PAD + l % CS_ITEMS];
+        const int e = e0 + j * ZS_BLOCK + (int)threadIdx.x;
+        if (e < e1) {
+            const uint32_t k = cs_entry_key(e, G, flag);
+            cs[e] = staged ? lo + lower_bound_in(keys, 0, len, k) : lower_bound_in(skey, lo, lo + len, k);
+        }
     }
 }
 
@@ -394,17 +361,9 @@ int launch_sort(hipStream_t s, int n, int keybits, int bits, uint32_t *key[2], i
     }
 }
 
-static int cs_tiles(int G) { return (int)((2 * (size_t)G + 2 + CS_TILE - 1) / CS_TILE); }
-size_t cell_start_scratch(int G) { return 2 * (size_t)cs_tiles(G); }
-
-void launch_cell_start(hipStream_t s, const uint32_t *skey, int n, int G, uint32_t flag, int *cs, int *scratch) {
+void launch_cell_start(hipStream_t s, const uint32_t *skey, int n, int G, uint32_t flag, int *cs) {
     const int E = 2 * G + 2;
-    const int T = cs_tiles(G);
-    k_cs_fill<<<std::min((E + ZS_BLOCK - 1) / ZS_BLOCK, 4096), ZS_BLOCK, 0, s>>>(E, n, cs);
-    if (n > 0) k_cs_mark<<<nblocks(n), ZS_BLOCK, 0, s>>>(skey, n, G, flag, cs);
-    k_cs_tile_min<<<T, ZS_BLOCK, 0, s>>>(cs, E, scratch);
-    k_cs_carry<<<1, ZS_BLOCK, 0, s>>>(scratch, T, n, scratch + T);
-    k_cs_apply<<<T, ZS_BLOCK, 0, s>>>(cs, E, scratch + T);
+    k_cell_start<<<(E + CS_TILE - 1) / CS_TILE, ZS_BLOCK, 0, s>>>(skey, n, G, flag, cs);
 }
 
 void launch_reorder(hipStream_t s, int n, const int *perm, const Buf &a, const Buf &b) {
