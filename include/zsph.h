@@ -225,7 +225,8 @@ void zsph_rigid_euler_step(zsph_rigid *r, float *pts, int n, double dt);
  * checks of lib/check.cu:19-59.  device < 0 keeps the current device.
  * ptc_num must be at least 1 and below 2^28 - 64 (particle arrays, with 64
  * spare elements, are addressed by 32-bit byte offsets), as must the total of
- * a batch; every scene of a batch holds at least one particle.  Other sizes
+ * a batch; every scene of a batch holds at least one particle.  The grid may
+ * have at most 2^29 cells (all scenes of a batch together).  Other sizes
  * return ZSPH_E_ARG before any device call. */
 int zsph_create(const zsph_params *p, int device, zsph_ctx **out);
 

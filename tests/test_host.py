@@ -168,3 +168,13 @@ def test_invalid_params_rejected_without_gpu():
     big = (abi.Params * 2)(abi.Params.from_buffer_copy(good), abi.Params.from_buffer_copy(good))
     big[0].ptc_num = big[1].ptc_num = (1 << 27) - 32
     assert z.lib().zsph_create_batch(big, 2, 0, C.byref(ctx)) == abi.E_ARG
+    # grids: at most 2^29 cells, alone and summed over a batch
+    q = abi.Params.from_buffer_copy(good)
+    q.grid_xdim, q.grid_ydim, q.grid_zdim = 1024, 1024, 513
+    q.grid_hash_max = 1024 * 1024 * 513 - 1
+    assert z.lib().zsph_create(C.byref(q), 0, C.byref(ctx)) == abi.E_ARG
+    half = abi.Params.from_buffer_copy(good)
+    half.grid_xdim, half.grid_ydim, half.grid_zdim = 1024, 1024, 257
+    half.grid_hash_max = 1024 * 1024 * 257 - 1
+    scenes = (abi.Params * 2)(half, abi.Params.from_buffer_copy(half))
+    assert z.lib().zsph_create_batch(scenes, 2, 0, C.byref(ctx)) == abi.E_ARG

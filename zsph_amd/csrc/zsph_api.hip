@@ -262,8 +262,9 @@ bool valid_params(const zsph_params *p) {
         !(p->grid_size > 0.0f) || p->grid_factor < 1 || (p->dim != 2 && p->dim != 3) || !(p->gamma > 0.0f) ||
         !(p->rc2 > 0.0f) || !(p->rho0 > 0.0f) || !(p->cs2 > 0.0f))
         return false;
+    // at most 2^29 cells: the cell table's 2G + 2 entries and its indices stay in int
     const long long G = (long long)p->grid_xdim * p->grid_ydim * p->grid_zdim;
-    return G <= (1ll << 30) && G - 1 == (long long)p->grid_hash_max;
+    return G <= (1ll << 29) && G - 1 == (long long)p->grid_hash_max;
 }
 
 // Context for N particles over G cells (all scenes), c->p / c->q already set.
@@ -339,7 +340,7 @@ int zsph_create_batch(const zsph_params *scenes, int nscenes, int device, zsph_c
         N += scenes[k].ptc_num;
         G += (long long)scenes[k].grid_xdim * scenes[k].grid_ydim * scenes[k].grid_zdim;
     }
-    if (N >= zs::MAX_PARTICLES || G > (1ll << 30)) return ZSPH_E_ARG;
+    if (N >= zs::MAX_PARTICLES || G > (1ll << 29)) return ZSPH_E_ARG;
     zsph_ctx *c = new zsph_ctx();
     c->nscenes = nscenes;
     int cb = 1;
