@@ -19,7 +19,9 @@ size_t radix_hist_words(int n, int bits);
 int launch_sort(hipStream_t s, int n, int keybits, int bits, uint32_t *key[2], int *val[2], uint32_t *hist,
                 bool identity_vals);
 // cs[0..G]: fluid cell starts; cs[G+1..2G+1]: dummy cell starts (key flag + c).
-void launch_cell_start(hipStream_t s, const uint32_t *skey, int n, int G, uint32_t flag, int *cs);
+// scratch: cell_start_scratch(G) ints (tile bounds).  n >= 1.
+size_t cell_start_scratch(int G);
+void launch_cell_start(hipStream_t s, const uint32_t *skey, int n, int G, uint32_t flag, int *cs, int *scratch);
 void launch_reorder(hipStream_t s, int n, const int *perm, const Buf &a, const Buf &b);
 
 // physics.hip; walk = WALK_MASK, WALK_HITS or WALK_LIST (common.hpp).

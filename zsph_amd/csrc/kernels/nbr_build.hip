@@ -221,13 +221,17 @@ __global__ __launch_bounds__(ZS_BLOCK) void k_radix_scatter(const uint32_t *__re
 // cell c; cs[2G+1] = N).  Empty cells have equal bounds.  Every entry is
 // rewritten on every call.
 // ---------------------------------------------------------------------------
-// One pass: the key of entry e, ek(e) = e for e <= G and flag + (e - G - 1)
+// Two passes.  The key of entry e, ek(e) = e for e <= G and flag + (e - G - 1)
 // above, is non-decreasing in e (flag >= G), so cs[e] is the lower bound of
-// ek(e) in the sorted keys.  A block owns CS_TILE consecutive entries; one
-// wave finds the sorted positions of its first and last entry keys with
-// 64-way searches (4 dependent rounds for 4 M keys), the keys between them
-// are staged in LDS, and each entry is a binary search there (in global
-// memory when they do not fit, e.g. hundreds of particles in one cell).
+// ek(e) in the sorted keys, i.e. the first particle whose entry index
+// ent(key) (key for fluid, G + 1 + key - flag for the others) is >= e.
+// k_cs_bounds: bound[t] = the first particle whose entry lies in tile t or
+// later (tiles of CS_TILE entries, t = 0..T): the particle where the entry
+// tile changes writes the bounds of the tiles it skips, so each is written
+// once.  k_cell_start: a block owns one tile; its entries' lower bounds lie in
+// [bound[t], bound[t+1]]; the keys between are staged in LDS and each entry
+// is a binary search there (in global memory when they do not fit, e.g.
+// hundreds of particles in one cell).
 constexpr int CS_ITEMS = 4;
 constexpr int CS_TILE = ZS_BLOCK * CS_ITEMS;
 constexpr int CS_KEYS = 4096;  // staged keys per block (16 KB of LDS)
@@ -235,26 +239,8 @@ constexpr int CS_KEYS = 4096;  // staged keys per block (16 KB of LDS)
 __device__ __forceinline__ uint32_t cs_entry_key(int e, int G, uint32_t flag) {
     return (e <= G) ? (uint32_t)e : flag + (uint32_t)(e - G - 1);
 }
-
-// Lower bound of k in a[lo, hi) by one whole wave, 64 probes per round: the
-// probes below k form a prefix, so their count narrows [lo, hi] to one probe
-// interval.  Every lane returns the result.
-__device__ __forceinline__ int wave_lower_bound(const uint32_t *__restrict__ a, int lo, int hi, uint32_t k) {
-    const int lane = (int)threadIdx.x & 63;
-    while (lo < hi) {
-        const int step = (hi - lo + 63) / 64;
-        const int s = lo + lane * step;
-        const bool below = s < hi && a[s] < k;
-        const int c = __popcll(__ballot(below));
-        if (c == 0) {
-            hi = lo;
-        } else {
-            const int sc = lo + c * step;
-            lo += (c - 1) * step + 1;
-            if (c < 64 && sc < hi) hi = sc;
-        }
-    }
-    return lo;
+__device__ __forceinline__ int cs_tile_of(uint32_t key, int G, uint32_t flag) {
+    return ((key < flag) ? (int)key : G + 1 + (int)(key - flag)) / CS_TILE;
 }
 
 // Lower bound of k in a[lo, hi) by one thread.
@@ -268,24 +254,44 @@ __device__ __forceinline__ int lower_bound_in(const A *a, int lo, int hi, uint32
     return lo;
 }
 
-__global__ __launch_bounds__(ZS_BLOCK) void k_cell_start(const uint32_t *__restrict__ skey, int n, int G,
-                                                         uint32_t flag, int *__restrict__ cs) {
-    __shared__ int bound[2];
+// bound[ta..tb] = v for every lane's (ta, tb, v), the wave writing 64 tiles
+// at a time (a particle before a run of empty cells skips many tiles).
+__device__ __forceinline__ void wave_fill(int *__restrict__ bound, int ta, int tb, int v) {
+    const int lane = (int)threadIdx.x & 63;
+    uint64_t m = __ballot(tb >= ta);
+    while (m) {
+        const int k = __builtin_ctzll(m);
+        m &= m - 1u;
+        const int a = __shfl(ta, k), b = __shfl(tb, k), w = __shfl(v, k);
+        for (int t = a + lane; t <= b; t += 64) bound[t] = w;
+    }
+}
+
+__global__ __launch_bounds__(ZS_BLOCK) void k_cs_bounds(const uint32_t *__restrict__ skey, int n, int G,
+                                                        uint32_t flag, int T, int *__restrict__ bound) {
+    const int i = blockIdx.x * ZS_BLOCK + (int)threadIdx.x;
+    int ta = 0, tb = -1, ua = 0, ub = -1;
+    if (i < n) {
+        const int t = cs_tile_of(skey[i], G, flag);
+        ta = (i == 0) ? 0 : cs_tile_of(skey[i - 1], G, flag) + 1;
+        tb = t;
+        if (i == n - 1) {  // the tiles after the last particle's
+            ua = t + 1;
+            ub = T;
+        }
+    }
+    wave_fill(bound, ta, tb, i);  // every lane of the wave takes part
+    wave_fill(bound, ua, ub, n);
+}
+
+__global__ __launch_bounds__(ZS_BLOCK) void k_cell_start(const uint32_t *__restrict__ skey, int G, uint32_t flag,
+                                                         const int *__restrict__ bound, int *__restrict__ cs) {
     __shared__ uint32_t keys[CS_KEYS];
     const int E = 2 * G + 2;
     const int e0 = blockIdx.x * CS_TILE;
     const int e1 = min(e0 + CS_TILE, E);
-    if (threadIdx.x < 64) {  // the first wave, whole: the searches use ballots
-        const int a = wave_lower_bound(skey, 0, n, cs_entry_key(e0, G, flag));
-        const int b = wave_lower_bound(skey, a, n, cs_entry_key(e1 - 1, G, flag));
-        if (threadIdx.x == 0) {
-            bound[0] = a;
-            bound[1] = b;
-        }
-    }
-    __syncthreads();
     // every entry of the tile has its lower bound in [lo, lo + len]
-    const int lo = bound[0], len = bound[1] - bound[0];
+    const int lo = bound[blockIdx.x], len = bound[blockIdx.x + 1] - lo;
     const bool staged = len <= CS_KEYS;
     if (staged)
         for (int j = (int)threadIdx.x; j < len; j += ZS_BLOCK) keys[j] = skey[lo + j];
@@ -361,9 +367,13 @@ int launch_sort(hipStream_t s, int n, int keybits, int bits, uint32_t *key[2], i
     }
 }
 
-void launch_cell_start(hipStream_t s, const uint32_t *skey, int n, int G, uint32_t flag, int *cs) {
-    const int E = 2 * G + 2;
-    k_cell_start<<<(E + CS_TILE - 1) / CS_TILE, ZS_BLOCK, 0, s>>>(skey, n, G, flag, cs);
+static int cs_tiles(int G) { return (2 * G + 2 + CS_TILE - 1) / CS_TILE; }
+size_t cell_start_scratch(int G) { return (size_t)cs_tiles(G) + 1; }
+
+void launch_cell_start(hipStream_t s, const uint32_t *skey, int n, int G, uint32_t flag, int *cs, int *scratch) {
+    const int T = cs_tiles(G);
+    k_cs_bounds<<<nblocks(n), ZS_BLOCK, 0, s>>>(skey, n, G, flag, T, scratch);
+    k_cell_start<<<T, ZS_BLOCK, 0, s>>>(skey, G, flag, scratch, cs);
 }
 
 void launch_reorder(hipStream_t s, int n, const int *perm, const Buf &a, const Buf &b) {

@@ -28,6 +28,7 @@ struct zsph_ctx {
     int sorted_slot = -1;  // slot of (key, idx) holding the sorted result
     int hashed = 0;        // key[0]/idx[0] hold a fresh (unsorted) hash
     int *cell_start = nullptr;
+    int *cs_scratch = nullptr;  // tile bounds of the cell-table build
     float4 *acc = nullptr;
     float4 *gd = nullptr;
     uint32_t *hist = nullptr;
@@ -171,7 +172,8 @@ int stage_sort_data(zsph_ctx *c, int from, int to, Prof *pf = nullptr) {
     if (from == to) return set_err(c, ZSPH_E_ARG, "zsph_sort_data needs two different buffers");
     {
         Scope sc(pf, ZSPH_ST_CELLS);
-        zs::launch_cell_start(c->stream, c->key[c->sorted_slot], c->q.N, c->q.G, c->q.flag, c->cell_start);
+        zs::launch_cell_start(c->stream, c->key[c->sorted_slot], c->q.N, c->q.G, c->q.flag, c->cell_start,
+                              c->cs_scratch);
     }
     {
         Scope sc(pf, ZSPH_ST_REORDER);
@@ -295,7 +297,7 @@ int create_ctx(zsph_ctx *c, int device, size_t N, size_t G, bool batch) {
     // fewest passes of at most 11-bit digits, spread evenly (19 bits: 2 x 10)
     const int passes = (kb + 10) / 11;
     c->radix_bits = std::max(8, (kb + passes - 1) / passes);
-    if ((r = dalloc(c, &c->cell_start, 2 * (G + 1))) || (r = dalloc(c, &c->acc, N)) || (r = dalloc(c, &c->gd, N)) ||
+    if ((r = dalloc(c, &c->cell_start, 2 * (G + 1))) || (r = dalloc(c, &c->cs_scratch, zs::cell_start_scratch((int)G))) || (r = dalloc(c, &c->acc, N)) || (r = dalloc(c, &c->gd, N)) ||
         (r = dalloc(c, &c->hist, zs::radix_hist_words((int)N, 11))) || (r = dalloc(c, &c->killed, 1)) ||
         (r = dalloc(c, &c->diag_part, (size_t)zs::diag_blocks((int)N) * zs::DIAG_D)) ||
         (r = dalloc(c, &c->diag_max, (size_t)zs::diag_blocks((int)N) * 2)) || (r = dalloc(c, &c->body, 1)) ||
