@@ -136,6 +136,9 @@ __global__ __launch_bounds__(ZS_BLOCK) void k_radix_scatter(const uint32_t *__re
     __shared__ uint32_t cnt[RS_WAVES][R];
     __shared__ uint32_t goff[R];
     __shared__ uint32_t part[ZS_BLOCK];
+    __shared__ uint32_t dstart[R];     // start of each digit's run in the tile
+    __shared__ uint32_t sk[RS_TILE];   // the tile in digit order
+    __shared__ int sv[RS_TILE];
     // digit base = exclusive scan of the digit totals; thread t owns digits [t*PER, t*PER+PER)
     {
         const int t = threadIdx.x;
@@ -192,6 +195,7 @@ __global__ __launch_bounds__(ZS_BLOCK) void k_radix_scatter(const uint32_t *__re
         loc[it] = before + r;
     }
     __syncthreads();
+    // per digit: the waves' exclusive prefix (cnt) and the tile's count
     for (int d = threadIdx.x; d < R; d += ZS_BLOCK) {
         uint32_t run = 0;
 #pragma unroll
@@ -200,17 +204,58 @@ __global__ __launch_bounds__(ZS_BLOCK) void k_radix_scatter(const uint32_t *__re
             cnt[ww][d] = run;
             run += c;
         }
+        dstart[d] = run;
     }
     __syncthreads();
+    // exclusive scan of the tile's digit counts; thread t owns digits [t*PER, t*PER+PER)
+    {
+        const int t = threadIdx.x;
+        uint32_t s = 0;
+#pragma unroll
+        for (int k2 = 0; k2 < PER; ++k2) {
+            const int d = t * PER + k2;
+            if (d < R) s += dstart[d];
+        }
+        part[t] = s;
+        __syncthreads();
+        for (int off = 1; off < ZS_BLOCK; off <<= 1) {
+            const uint32_t v2 = (t >= off) ? part[t - off] : 0u;
+            __syncthreads();
+            part[t] += v2;
+            __syncthreads();
+        }
+        uint32_t run = part[t] - s;
+#pragma unroll
+        for (int k2 = 0; k2 < PER; ++k2) {
+            const int d = t * PER + k2;
+            if (d < R) {
+                const uint32_t c = dstart[d];
+                dstart[d] = run;
+                run += c;
+            }
+        }
+    }
+    __syncthreads();
+    // the tile in digit order in LDS (within a digit: wave, item, lane order)
 #pragma unroll
     for (int it = 0; it < RS_ITEMS; ++it) {
         const int e = base + it * 64 + lane;
         if (e < n) {
             const uint32_t d = (k[it] >> shift) & (R - 1);
-            const uint32_t dst = goff[d] + cnt[w][d] + loc[it];
-            kout[dst] = k[it];
-            vout[dst] = v[it];
+            const uint32_t p = dstart[d] + cnt[w][d] + loc[it];
+            sk[p] = k[it];
+            sv[p] = v[it];
         }
+    }
+    __syncthreads();
+    // consecutive positions of one digit's run go to consecutive outputs
+    const int valid_n = min(RS_TILE, n - (int)blockIdx.x * RS_TILE);
+    for (int p = threadIdx.x; p < valid_n; p += ZS_BLOCK) {
+        const uint32_t kk = sk[p];
+        const uint32_t d = (kk >> shift) & (R - 1);
+        const uint32_t dst = goff[d] + ((uint32_t)p - dstart[d]);
+        kout[dst] = kk;
+        vout[dst] = sv[p];
     }
 }
 
