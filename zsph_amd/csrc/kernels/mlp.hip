@@ -253,20 +253,61 @@ __global__ __launch_bounds__(64) void k_mlp(const float *__restrict__ P, const b
     bf16 *H2T = TRAIN ? act + A_H2T : nullptr, *H3T = TRAIN ? act + A_H3T : nullptr;
     const float os = aux->out_scale;
 
+    // Inference: lane l holds tile elements e = l + 64 k (k < FL), of feature
+    // column (l + 64 k) % MLP_NF, which repeats with period FP in k: the
+    // normalisation constants of a lane's FP columns stay in registers.
+    // (Training keeps the plain loop: its registers are all in use.)
+    constexpr int FL = MT * MLP_NF / 64, FP = 3;
+    static_assert(MT * MLP_NF % 64 == 0 && 64 * FP % MLP_NF == 0 && FL % FP == 0, "feature tile layout");
+    float fmean[FP], fstd[FP], flo[FP], fhi[FP];
+#pragma unroll
+    for (int j = 0; j < FP; ++j) {
+        const int f = (l + 64 * j) % MLP_NF;
+        fmean[j] = aux->mean[f];
+        fstd[j] = aux->inv_std[f];
+        flo[j] = aux->lo[f];
+        fhi[j] = aux->hi[f];
+    }
+    // Raw features of a tile: a lane's FL loads are issued together, the next
+    // tile's while the current one runs through the network.
+    float fv[FL];
+    auto load_tile = [&](int t) {
+#pragma unroll
+        for (int k = 0; k < FL; ++k) {
+            const int e = l + 64 * k;
+            fv[k] = (t + e / MLP_NF < n) ? feat[(size_t)t * MLP_NF + e] : 0.0f;
+        }
+    };
+    if (!TRAIN && blockIdx.x * MT < n) load_tile(blockIdx.x * MT);
+
     for (int t0 = blockIdx.x * MT; t0 < n; t0 += gridDim.x * MT) {
         // normalised features of the tile; out-of-range samples flagged
         if (l < MT) keep[l] = 1;
         __syncthreads();
-        for (int e = l; e < MT * MLP_NF; e += 64) {
-            const int s = e / MLP_NF, f = e - s * MLP_NF;
-            float v = 0.0f;
-            if (t0 + s < n) {
-                v = (feat[(size_t)(t0 + s) * MLP_NF + f] - aux->mean[f]) * aux->inv_std[f];
-                if (!(v >= aux->lo[f] && v <= aux->hi[f])) keep[s] = 0;
+        if constexpr (TRAIN) {
+            for (int e = l; e < MT * MLP_NF; e += 64) {
+                const int s = e / MLP_NF, f = e - s * MLP_NF;
+                float v = 0.0f;
+                if (t0 + s < n) {
+                    v = (feat[(size_t)(t0 + s) * MLP_NF + f] - aux->mean[f]) * aux->inv_std[f];
+                    if (!(v >= aux->lo[f] && v <= aux->hi[f])) keep[s] = 0;
+                }
+                const bf16 h = (bf16)v;
+                X[s * LDI + f] = h;
+                XT[f * LDI + s] = h;
             }
-            const bf16 h = (bf16)v;
-            X[s * LDI + f] = h;
-            if (TRAIN) XT[f * LDI + s] = h;
+        } else {
+#pragma unroll
+            for (int k = 0; k < FL; ++k) {
+                const int e = l + 64 * k, s = e / MLP_NF, f = e - s * MLP_NF;
+                float v = 0.0f;
+                if (t0 + s < n) {
+                    v = (fv[k] - fmean[k % FP]) * fstd[k % FP];
+                    if (!(v >= flo[k % FP] && v <= fhi[k % FP])) keep[s] = 0;
+                }
+                X[s * LDI + f] = (bf16)v;
+            }
+            if (t0 + gridDim.x * MT < n) load_tile(t0 + gridDim.x * MT);
         }
         __syncthreads();
 
