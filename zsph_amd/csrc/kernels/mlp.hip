@@ -1,9 +1,11 @@
 // Per-particle closure network on gfx950 MFMA: 24 features -> 64 -> 64 -> 64
 // (SiLU) -> 2 outputs, BF16 operands with FP32 accumulation
-// (v_mfma_f32_16x16x32_bf16).  One wave per block processes tiles of 32
-// samples: activations live in LDS as BF16 in both orientations ([sample]
-// [neuron] feeds the next layer, [neuron][sample] feeds the weight
-// gradient), weights are copied once per block from a pre-packed BF16 image.
+// (v_mfma_f32_16x16x32_bf16).  A wave processes tiles of 32 samples:
+// activations live in LDS as BF16 in both orientations ([sample][neuron]
+// feeds the next layer, [neuron][sample] feeds the weight gradient), weights
+// are copied once per block from a pre-packed BF16 image.  Training runs one
+// wave per block; inference runs INFER_WAVES waves per block, which share the
+// weights and each keep their own activations.
 //
 // MFMA operand layout (16x16x32, lane l, g = l / 16):
 //   A[m][k]: m = l % 16, k = 8 g .. 8 g + 7   (8 contiguous BF16)
@@ -76,11 +78,19 @@ constexpr int A_DZ = A_DOT + 16 * LDI;  // [32][LDH]
 constexpr int A_DZT = A_DZ + MT * LDH;  // [64][LDI]
 constexpr int A_ALL = A_DZT + 64 * LDI;
 
-template <bool TRAIN>
+// Waves per inference block: they share one copy of the weights, each runs
+// its own tiles.  73.5 KB of LDS per block, two blocks (eight waves) per CU.
+constexpr int INFER_WAVES = 4;
+
+template <bool TRAIN, int NW>
 constexpr size_t lds_bytes() {
     return TRAIN ? (size_t)(PACK_ALL + A_ALL) * 2 + (size_t)(MLP_NPARAM + MT + 8) * 4
-                 : (size_t)(PACK_FWD + A_INF) * 2 + (size_t)(MT + 8) * 4;
+                 : (size_t)(PACK_FWD + NW * A_INF) * 2 + (size_t)(NW * MT + 8) * 4;
 }
+constexpr int LDS_PER_CU = 160 * 1024;
+constexpr int CUS = 256;  // MI355X
+constexpr int INFER_BLOCKS_PER_CU = LDS_PER_CU / (int)lds_bytes<false, INFER_WAVES>();
+static_assert(INFER_BLOCKS_PER_CU >= 1, "inference block exceeds the LDS");
 
 __device__ __forceinline__ int lane() { return (int)(threadIdx.x & 63); }
 
@@ -127,6 +137,20 @@ __device__ __forceinline__ void grad_acc(const bf16 *XT, int ldx, const bf16 *DZ
                 if (row < n_in && col < n_out) dW[row * n_out + col] += acc[i];
             }
         }
+}
+
+// Orders a tile's LDS writes before the reads that follow: the whole block
+// when it is one wave (training), else the calling wave only (each inference
+// wave works on tiles of its own; a wave's LDS operations complete in order).
+template <int NW>
+__device__ __forceinline__ void tile_sync() {
+    if constexpr (NW == 1) {
+        __syncthreads();
+    } else {
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    }
 }
 
 __device__ __forceinline__ float sigm(float z) { return 1.0f / (1.0f + __expf(-z)); }
@@ -193,59 +217,63 @@ __device__ __forceinline__ void silu_back(f32x4 (&d)[2][NB], const f32x4 (&z)[2]
 
 }  // namespace mlp
 
-// One wave per block; grid-stride over 32-sample tiles of [0, n) (n = *n_dev
-// when n_dev is given: the fluid count of a sorted buffer).
+// NW waves per block; wave w of block b takes tiles b NW + w, then every
+// (grid NW)-th, of the 32-sample tiles of [0, n) (n = *n_dev when n_dev is
+// given: the fluid count of a sorted buffer).
 //   feat:   n x MLP_NF raw features (FP32)
 //   TRAIN = false: out = n x 2 scale factors (1, 1 where a normalised
 //                  feature leaves [lo, hi])
 //   TRAIN = true:  rec = n x MLP_NT training records; part = per-block
 //                  gradient partials (MLP_NPARAM) + loss partial
-template <bool TRAIN>
-__global__ __launch_bounds__(64) void k_mlp(const float *__restrict__ P, const bf16 *__restrict__ pack,
-                                           const MlpAux *__restrict__ aux, const float *__restrict__ feat, int n,
-                                           const int *__restrict__ n_dev, float2 *__restrict__ out,
-                                           const float *__restrict__ rec, float *__restrict__ part) {
+template <bool TRAIN, int NW>
+__global__ __launch_bounds__(64 * NW) void k_mlp(const float *__restrict__ P, const bf16 *__restrict__ pack,
+                                                const MlpAux *__restrict__ aux, const float *__restrict__ feat,
+                                                int n, const int *__restrict__ n_dev, float2 *__restrict__ out,
+                                                const float *__restrict__ rec, float *__restrict__ part) {
     using namespace mlp;
+    static_assert(!TRAIN || NW == 1, "training runs one wave per block");
     extern __shared__ __align__(16) unsigned char smem[];
     constexpr int PACK = TRAIN ? PACK_ALL : PACK_FWD;
+    constexpr int ACT = TRAIN ? A_ALL : A_INF;  // activations of one wave
+    const int tid = (int)threadIdx.x, wv = tid >> 6;
     bf16 *W = reinterpret_cast<bf16 *>(smem);
-    bf16 *act = W + PACK;
-    float *dW = reinterpret_cast<float *>(act + (TRAIN ? A_ALL : A_INF));
-    int *keep = reinterpret_cast<int *>(dW + (TRAIN ? MLP_NPARAM : 0));  // per sample: inside the range
-    float *lsum = reinterpret_cast<float *>(keep + MT);
+    bf16 *act = W + PACK + wv * ACT;
+    float *dW = reinterpret_cast<float *>(W + PACK + NW * ACT);
+    int *keep0 = reinterpret_cast<int *>(dW + (TRAIN ? MLP_NPARAM : 0));
+    int *keep = keep0 + wv * MT;  // per sample of the wave's tile: inside the range
+    float *lsum = reinterpret_cast<float *>(keep0 + NW * MT);
     const int l = lane();
     if (n_dev) n = *n_dev;
 
     // weights (16-byte copies, seven loads in flight before their stores),
     // zeroed activations (16-byte stores) and gradient accumulators
     {
-        constexpr int STEP = 8 * 64, PASSES = (PACK + STEP - 1) / STEP, BATCH = 7;
+        constexpr int STEP = 8 * 64 * NW, PASSES = (PACK + STEP - 1) / STEP, BATCH = 7;
         static_assert(PACK % 8 == 0, "weights copy in 8-element pieces");
         for (int p0 = 0; p0 < PASSES; p0 += BATCH) {
             bf16x8 tmp[BATCH];
 #pragma unroll
             for (int k = 0; k < BATCH; ++k) {
-                const int e = (p0 + k) * STEP + 8 * l;
+                const int e = (p0 + k) * STEP + 8 * tid;
                 if (e < PACK) tmp[k] = *reinterpret_cast<const bf16x8 *>(pack + e);
             }
 #pragma unroll
             for (int k = 0; k < BATCH; ++k) {
-                const int e = (p0 + k) * STEP + 8 * l;
+                const int e = (p0 + k) * STEP + 8 * tid;
                 if (e < PACK) *reinterpret_cast<bf16x8 *>(W + e) = tmp[k];
             }
         }
     }
     {
-        constexpr int ACT = TRAIN ? A_ALL : A_INF;
-        static_assert(ACT % 8 == 0, "activations cleared in 8-element pieces");
+        static_assert(ACT % 8 == 0 && PACK % 8 == 0, "activations cleared in 8-element pieces");
         bf16x8 zero;
 #pragma unroll
         for (int k = 0; k < 8; ++k) zero[k] = (bf16)0.0f;
-        for (int e = 8 * l; e < ACT; e += 8 * 64) *reinterpret_cast<bf16x8 *>(act + e) = zero;
+        for (int e = 8 * tid; e < NW * ACT; e += 8 * 64 * NW) *reinterpret_cast<bf16x8 *>(W + PACK + e) = zero;
     }
     if (TRAIN)
         for (int e = l; e < MLP_NPARAM; e += 64) dW[e] = 0.0f;
-    if (l == 0) lsum[0] = 0.0f;
+    if (tid == 0) lsum[0] = 0.0f;
     __syncthreads();
 
     bf16 *X = act + A_X, *H1 = act + A_H1, *H2 = act + A_H2, *H3 = act + (TRAIN ? A_H3 : A_H1);
@@ -278,12 +306,13 @@ __global__ __launch_bounds__(64) void k_mlp(const float *__restrict__ P, const b
             fv[k] = (t + e / MLP_NF < n) ? feat[(size_t)t * MLP_NF + e] : 0.0f;
         }
     };
-    if (!TRAIN && blockIdx.x * MT < n) load_tile(blockIdx.x * MT);
+    const int first = ((int)blockIdx.x * NW + wv) * MT, stride = (int)gridDim.x * NW * MT;
+    if (!TRAIN && first < n) load_tile(first);
 
-    for (int t0 = blockIdx.x * MT; t0 < n; t0 += gridDim.x * MT) {
+    for (int t0 = first; t0 < n; t0 += stride) {
         // normalised features of the tile; out-of-range samples flagged
         if (l < MT) keep[l] = 1;
-        __syncthreads();
+        tile_sync<NW>();
         if constexpr (TRAIN) {
             for (int e = l; e < MT * MLP_NF; e += 64) {
                 const int s = e / MLP_NF, f = e - s * MLP_NF;
@@ -307,23 +336,23 @@ __global__ __launch_bounds__(64) void k_mlp(const float *__restrict__ P, const b
                 }
                 X[s * LDI + f] = (bf16)v;
             }
-            if (t0 + gridDim.x * MT < n) load_tile(t0 + gridDim.x * MT);
+            if (t0 + stride < n) load_tile(t0 + stride);
         }
-        __syncthreads();
+        tile_sync<NW>();
 
         f32x4 c4[2][4], z1[2][4], z2[2][4], z3[2][4];
         gemm<4, 1>(X, LDI, W + O_W0T, LDI, c4);
         bias_silu<4>(c4, z1, P + P_B0);
         store_act<4>(c4, H1, LDH, H1T, LDI);
-        __syncthreads();
+        tile_sync<NW>();
         gemm<4, 2>(H1, LDH, W + O_W1T, LDH, c4);
         bias_silu<4>(c4, z2, P + P_B1);
         store_act<4>(c4, H2, LDH, H2T, LDI);
-        __syncthreads();
+        tile_sync<NW>();
         gemm<4, 2>(H2, LDH, W + O_W2T, LDH, c4);
         bias_silu<4>(c4, z3, P + P_B2);
         store_act<4>(c4, H3, LDH, H3T, LDI);
-        __syncthreads();
+        tile_sync<NW>();
         f32x4 o[2][1];
         gemm<1, 2>(H3, LDH, W + O_W3T, LDH, o);
 
@@ -339,7 +368,7 @@ __global__ __launch_bounds__(64) void k_mlp(const float *__restrict__ P, const b
                         const float sk = keep[s] ? 1.0f + os * tanhf(o[mb][0][i] + P[P_B3 + k]) : 1.0f;
                         reinterpret_cast<float *>(out)[(size_t)(t0 + s) * 2 + k] = sk;
                     }
-            __syncthreads();
+            tile_sync<NW>();
             continue;
         }
 
@@ -477,19 +506,24 @@ __global__ __launch_bounds__(ZS_BLOCK) void k_mlp_pack(const float *__restrict__
 size_t mlp_pack_elems() { return mlp::PACK_ALL; }
 
 int mlp_blocks(int n) { return std::max(1, std::min(256, (n + mlp::MT - 1) / mlp::MT)); }
-// Inference has no cross-block reduction: one-wave blocks, at most 1024 (the
-// 4 per CU that the LDS allows, in a single round; each block copies the
-// weights once).  Measured against 2048 and 512: 1.2 % and 8 % slower on the
-// learned coarse step.  Each sample's result does not depend on which block
-// computes it.
-int mlp_infer_blocks(int n) { return std::max(1, std::min(1024, (n + mlp::MT - 1) / mlp::MT)); }
+// Inference has no cross-block reduction: blocks of INFER_WAVES waves, as
+// many as the LDS lets run at once on every CU (one round; each block copies
+// the weights once), fewer when the tiles do not fill them.  Each sample's
+// result does not depend on which block or wave computes it.
+int mlp_infer_blocks(int n) {
+    const int tiles = (n + mlp::MT - 1) / mlp::MT;
+    const int want = (tiles + mlp::INFER_WAVES - 1) / mlp::INFER_WAVES;
+    return std::max(1, std::min(mlp::CUS * mlp::INFER_BLOCKS_PER_CU, want));
+}
 
 hipError_t mlp_prepare() {
-    hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(&k_mlp<true>),
-                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)mlp::lds_bytes<true>());
+    using mlp::INFER_WAVES;
+    hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(&k_mlp<true, 1>),
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)mlp::lds_bytes<true, 1>());
     if (e == hipSuccess)
-        e = hipFuncSetAttribute(reinterpret_cast<const void *>(&k_mlp<false>),
-                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)mlp::lds_bytes<false>());
+        e = hipFuncSetAttribute(reinterpret_cast<const void *>(&k_mlp<false, INFER_WAVES>),
+                                hipFuncAttributeMaxDynamicSharedMemorySize,
+                                (int)mlp::lds_bytes<false, INFER_WAVES>());
     return e;
 }
 
@@ -499,15 +533,16 @@ void launch_mlp_pack(hipStream_t s, const float *P, void *pack) {
 
 void launch_mlp_infer(hipStream_t s, const float *P, const void *pack, const MlpAux *aux, const float *feat, int n,
                       const int *n_dev, float2 *out, int nblk) {
-    k_mlp<false><<<nblk, 64, mlp::lds_bytes<false>(), s>>>(P, static_cast<const bf16 *>(pack), aux, feat, n, n_dev,
-                                                          out, nullptr, nullptr);
+    using mlp::INFER_WAVES;
+    k_mlp<false, INFER_WAVES><<<nblk, 64 * INFER_WAVES, mlp::lds_bytes<false, INFER_WAVES>(), s>>>(
+        P, static_cast<const bf16 *>(pack), aux, feat, n, n_dev, out, nullptr, nullptr);
 }
 
 void launch_mlp_grad(hipStream_t s, const float *P, const void *pack, const MlpAux *aux, const float *feat,
                      const float *rec, int n, float *part, float *grad) {
     const int nblk = mlp_blocks(n);
-    k_mlp<true><<<nblk, 64, mlp::lds_bytes<true>(), s>>>(P, static_cast<const bf16 *>(pack), aux, feat, n, nullptr,
-                                                        nullptr, rec, part);
+    k_mlp<true, 1><<<nblk, 64, mlp::lds_bytes<true, 1>(), s>>>(P, static_cast<const bf16 *>(pack), aux, feat, n,
+                                                              nullptr, nullptr, rec, part);
     k_mlp_reduce<<<(MLP_NPARAM + 1 + ZS_BLOCK - 1) / ZS_BLOCK, ZS_BLOCK, 0, s>>>(part, nblk, 1.0f / (float)n, grad);
 }
 
