@@ -21,8 +21,14 @@ state (force/torque reduction, body update and dummy kinematics every half step)
 height U(0.3,0.8), dx U(0.004,0.01) per scene; 3.8 M particles) advanced
 together in one context, each on its own cell grid with its own dt.
 
+--closure NPZ (one-scene cases): every step runs with the learned closure
+network stored in NPZ (e.g. zsph_amd/data/closure_dambreak3d.npz, trained for
+the coarse dam break --dx 0.016): features in the gradient pass, MFMA
+inference over the fluid, scaled diffusion and viscous terms.  The CPU
+baseline then runs the oracle with the same network.
+
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--case dambreak3d|wavetank3d|ensemble2d]
-                    [--no-cpu-baseline]
+                    [--dx DX] [--closure NPZ] [--no-cpu-baseline]
 
 Prints one JSON line.  Only one GPU is used: --gpus above 1 exits.
 """
@@ -105,6 +111,8 @@ def main():
     ap.add_argument("--no-autotune", dest="autotune", action="store_false")
     ap.add_argument("--walk", type=int, default=None, help="force a neighbour walk (0 mask, 1 hit masks, 2 lists)")
     ap.add_argument("--radix-bits", type=int, default=None, help="force the sort's digit width (8 to 11 bits)")
+    ap.add_argument("--closure", default=None, metavar="NPZ",
+                    help="run every step with this learned closure network (one-scene cases)")
     ap.add_argument("--tune-at", type=int, default=None,
                     help="autotune on the state after this many steps, then restart from the initial state "
                          "(default: 30%% of the timed steps)")
@@ -147,6 +155,20 @@ def main():
                     f"{case.n} particles ({case.json.water_ptc_num} fluid), delta-SPH Antuono, "
                     f"Tait gamma 7, CFL 0.2, cell edge 2h/{args.gsf:g}")
         metric = "particle-steps/s, 1.3M 3D dam-break"
+    if args.closure:
+        if isinstance(case, z.Batch):
+            print("bench.py: --closure needs a one-scene case", file=sys.stderr)
+            sys.exit(2)
+        from zsph_amd import closure
+        net = closure.load_network(args.closure)
+        case_setup = setup
+
+        def setup(x):  # solver and CPU baseline alike
+            if case_setup:
+                case_setup(x)
+            x.set_closure(net)
+        workload += f", learned closure {os.path.basename(args.closure)}"
+        metric += " (learned closure)"
     if args.steps is None:  # the configured run length (the ensemble and the tank: a bounded sample)
         args.steps = {"dambreak3d": 10000, "wavetank3d": 1000, "ensemble2d": 3000}[args.case]
     batch = isinstance(case, z.Batch)
