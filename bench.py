@@ -10,7 +10,21 @@ sort, cell table + reorder, wall pass, gradient pass, continuity+momentum,
 integrator) for every particle; particles are resident in HBM before timing.
 The default timed run is the configured 10,000 steps (t = 0 to 0.63 s, through
 the column's collapse and the run-up to the far wall), so the figure is the
-whole run's throughput, not the compact initial state's.
+whole run's throughput, not the compact initial state's.  A shorter --steps
+times a window of the developed flow instead: it starts at the tuning step
+(--start-at), not at t = 0.  The autotuner always runs on the state after a
+fixed number of steps (--tune-at: 3000 for the dam break), whatever --steps is.
+
+Besides the whole-step figure the line reports, device-clocked over the timed
+region, the gradient pass (the dominant kernel) as `roofline` and the
+continuity+momentum pass as `roofline_governing`: algorithmic bytes per launch
+over the average launch time, against 8 TB/s.  Both passes are VALU-bound
+(candidate tests and pair arithmetic), so each also carries `valu_frac`, the
+VALU issue share implied by its recorded SQ_INSTS_VALU; that count and the
+measured HBM traffic are recorded from rocprofv3 counter runs of the same
+workload (`RECORDED`, with the profile file named), not read in the run.
+The CPU baseline is the FP32 oracle on the host's cores: median of 5 timings
+after 2 warm-up steps with all threads, and a one-thread sample.
 
 --case wavetank3d: the floating-body tank, 4.0 x 1.0 x 1.2 m filled to 0.6 m
 at dx 0.006 m (11.1 M fluid, 12.5 M particles), piston 0.025 m amplitude at
@@ -42,16 +56,19 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
+# VALU issue capacity: 256 CUs x 4 SIMDs, one wave64 FP32 instruction per SIMD every 4 cycles at 2.4 GHz
+SIMDS, VALU_CYCLES, CLOCK_HZ = 1024, 4, 2.4e9
 
-# Measured HBM bytes of one continuity+momentum launch on the default 3-D dam
-# break (dx 0.008, cell edge h), per neighbour walk (0 mask, 1 hit masks, 2 list):
-# rocprofv3 FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0 (KiB per dispatch; the factors
-# make a 512 MiB device copy read 512 MiB and write 512 MiB).  Other workloads
-# report no measured traffic.
-GOVERNING_TRAFFIC_BYTES = {
-    ("dambreak3d", 0.008, 2.0, 0): (252352.5 * 2.0 + 20214.8) * 1024.0,
-    ("dambreak3d", 0.008, 2.0, 1): (329708.6 * 2.0 + 20214.8) * 1024.0,
-    ("dambreak3d", 0.008, 2.0, 2): (337566.0 * 2.0 + 20214.8) * 1024.0,
+# Recorded hardware counters of the neighbour passes per launch (rocprofv3 --pmc, one counter per
+# pass; the table's profile file gives the commands): HBM traffic = FETCH_SIZE x 2 + WRITE_SIZE x 1
+# (KiB, factors from a device copy of known size) and SQ_INSTS_VALU (wave instructions).  Keyed by
+# (case, dx, grid_size_factor, walk); other workloads report none.
+RECORDED = {
+    ("dambreak3d", 0.008, 2.0, 1): {
+        "profile": "profiles/counters_neighbour_kernels_dambreak3d.md",
+        "k_ldg": {"traffic": (135497.5 * 2.0 + 160981.1) * 1024.0, "valu_insts": 2.15e8},
+        "k_governing": {"traffic": (329708.6 * 2.0 + 20214.8) * 1024.0, "valu_insts": 1.57e8},
+    },
 }
 
 
@@ -63,33 +80,97 @@ def step_bytes(n, G, n_fluid):
     return 640.0 + 48.0 * G / n + 16.0 * (1.0 - ff) + 204.0 * ff
 
 
-def governing_bytes(n, G):
-    """Algorithmic bytes of one continuity+momentum launch: per particle it
-    reads (x,rho) 16, (v,p) 16, (grad rho, div v) 16, type 4, hash 4 and writes
-    (a, drho/dt) 16 = 72 B; plus one pass over the cell table, 4 (G+1) B."""
-    return 72.0 * n + 4.0 * (G + 1)
+def ldg_bytes(n, n_fluid, G):
+    """Algorithmic bytes of one gradient-pass launch: each fluid particle reads (x, rho) 16,
+    (v, p) 16 and its sort key 4 and writes (grad rho^L, div v) 16 = 52 B; each other particle
+    writes its zero record, 16 B; plus one pass over the cell table's 2 (G + 1) entries."""
+    return 52.0 * n_fluid + 16.0 * (n - n_fluid) + 8.0 * (G + 1)
 
 
-def cpu_baseline(scenes, steps=3, warmup=1, setup=None, what="the same case"):
-    """FP32 CPU oracle (OpenMP over particles) on this host's cores: `steps`
-    steps of each case in `scenes` (a bounded sample of the workload)."""
+def governing_bytes(n, n_fluid, G):
+    """Algorithmic bytes of one continuity+momentum launch: each fluid particle reads (x, rho)
+    16, (v, p) 16, (grad rho^L, div v) 16 and its key 4 and writes (a, drho/dt) 16 = 68 B; each
+    other particle writes its zero record, 16 B; plus one pass over the cell table."""
+    return 68.0 * n_fluid + 16.0 * (n - n_fluid) + 8.0 * (G + 1)
+
+
+def kernel_roofline(name, stage_ms, launches, nbytes, rec, bound):
+    """Roofline entry of one neighbour pass from its device-clocked average launch time."""
+    achieved = nbytes / (stage_ms * 1e-3) / 1e9
+    out = {"bound": bound, "kernel": name, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+           "frac": achieved / HBM_PEAK_GBS, "traffic": None, "bytes_per_launch": nbytes,
+           "avg_launch_ms": stage_ms, "launches": launches,
+           "timing": "device wall-clock stamps around every launch of the timed region"}
+    if rec and name in rec:
+        r = rec[name]
+        out["traffic"] = r["traffic"]
+        # VALU issue: the recorded instruction count over this run's launch time
+        out["valu_frac"] = r["valu_insts"] * VALU_CYCLES / (SIMDS * CLOCK_HZ * stage_ms * 1e-3)
+        out["recorded"] = (f"traffic and SQ_INSTS_VALU per launch recorded in {rec['profile']} "
+                           "(rocprofv3 --pmc of the same workload and walk)")
+    return out
+
+
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            names = [l.split(":", 1)[1].strip() for l in f if l.startswith("model name")]
+            f.seek(0)
+            sockets = {l.split(":", 1)[1].strip() for l in f if l.startswith("physical id")}
+        return (names[0] if names else "unknown"), max(1, len(sockets))
+    except OSError:
+        return "unknown", 1
+
+
+def cpu_baseline(scenes, steps=1, repeats=5, warmup=2, single=(1, 3), setup=None, what="the same case"):
+    """FP32 CPU oracle (OpenMP over particles) on this host's cores: after `warmup` steps,
+    the median of `repeats` timings of `steps` steps, with all threads; then the same with
+    one thread, (steps, repeats) = `single` after one warm-up step (a bounded sample:
+    a one-thread step of the full case takes about as long as the whole threaded sample)."""
+    import statistics
+
     import oracle
     threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
-    work, elapsed = 0, 0.0
-    for case in scenes:
-        o = oracle.Oracle(case.params, fp64=False)
-        o.upload(case.host)
-        if setup:
-            setup(o)
-        o.step(warmup)
-        t0 = time.perf_counter()
-        o.step(steps)
-        elapsed += time.perf_counter() - t0
-        work += case.n * steps
-        o.close()
-    return {"value": work / elapsed, "unit": "particle-steps/s", "cores": threads, "kind": "port",
-            "sample": f"{steps} steps (after {warmup} warm-up) of {what} ({work // steps} particles), "
-                      f"FP32 oracle, OpenMP {threads} threads"}
+
+    def run(nthreads, k, reps, warm):
+        prev = oracle.set_threads(nthreads)
+        rates = []
+        try:
+            objs = []
+            for case in scenes:
+                o = oracle.Oracle(case.params, fp64=False)
+                o.upload(case.host)
+                if setup:
+                    setup(o)
+                o.step(warm)
+                objs.append(o)
+            for _ in range(reps):
+                work, elapsed = 0, 0.0
+                for case, o in zip(scenes, objs):
+                    t0 = time.perf_counter()
+                    o.step(k)
+                    elapsed += time.perf_counter() - t0
+                    work += case.n * k
+                rates.append(work / elapsed)
+            for o in objs:
+                o.close()
+        finally:
+            oracle.set_threads(prev)
+        return statistics.median(rates), rates
+
+    model, sockets = cpu_model()
+    value, rates = run(threads, steps, repeats, warmup)
+    print(f"[bench] CPU baseline {threads} threads: {value:.4g} particle-steps/s", file=sys.stderr, flush=True)
+    one, one_rates = run(1, single[0], single[1], 1)
+    print(f"[bench] CPU baseline 1 thread: {one:.4g} particle-steps/s", file=sys.stderr, flush=True)
+    n = sum(c.n for c in scenes)
+    return {"value": value, "unit": "particle-steps/s", "cores": threads, "kind": "port",
+            "sample": f"median of {repeats} timings of {steps} step(s) after {warmup} warm-up steps of {what} "
+                      f"({n} particles), FP32 oracle, OpenMP {threads} threads",
+            "repeats": rates, "cpu_model": model, "sockets": sockets, "nproc": os.cpu_count(),
+            "single_thread": {"value": one, "cores": 1, "repeats": one_rates,
+                              "sample": f"median of {single[1]} timings of {single[0]} step(s) after 1 warm-up "
+                                        f"step, one OpenMP thread"}}
 
 
 def main():
@@ -114,8 +195,11 @@ def main():
     ap.add_argument("--closure", default=None, metavar="NPZ",
                     help="run every step with this learned closure network (one-scene cases)")
     ap.add_argument("--tune-at", type=int, default=None,
-                    help="autotune on the state after this many steps, then restart from the initial state "
-                         "(default: 30%% of the timed steps)")
+                    help="autotune on the developed state after this many steps of the run (default: 3000 for "
+                         "the 3-D dam break, 1000 for the ensemble, 300 for the wave tank, whatever --steps is)")
+    ap.add_argument("--start-at", type=int, default=None,
+                    help="first step of the timed window (default: 0 when the timed steps cover the configured "
+                         "run, else the tuning step: a short window is timed in the developed flow)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); the step is single-GPU.",
@@ -169,8 +253,13 @@ def main():
             x.set_closure(net)
         workload += f", learned closure {os.path.basename(args.closure)}"
         metric += " (learned closure)"
+    run_len = {"dambreak3d": 10000, "wavetank3d": 1000, "ensemble2d": 3000}[args.case]
     if args.steps is None:  # the configured run length (the ensemble and the tank: a bounded sample)
-        args.steps = {"dambreak3d": 10000, "wavetank3d": 1000, "ensemble2d": 3000}[args.case]
+        args.steps = run_len
+    if args.tune_at is None:
+        args.tune_at = {"dambreak3d": 3000, "wavetank3d": 300, "ensemble2d": 1000}[args.case]
+    if args.start_at is None:
+        args.start_at = 0 if args.steps >= run_len else args.tune_at
     batch = isinstance(case, z.Batch)
     n_fluid = int((case.host.type == z.FLUID).sum())
     G = case.grid_num if batch else case.cpu.grid_num
@@ -179,40 +268,46 @@ def main():
     s.upload(case.host)
     if setup:
         setup(s)
-    # pick the neighbour-walk variant and sort digit width on this case (state restored after)
+    # pick the neighbour-walk variant and sort digit width on this case
     if args.walk is not None or args.radix_bits is not None:
         t = s.get_tuning()
         s.set_tuning(t["walk"] if args.walk is None else args.walk,
                      t["radix_bits"] if args.radix_bits is None else args.radix_bits)
         args.autotune = False
-    if args.autotune:
-        # The walks' relative speed changes as the flow develops, so the
-        # candidates are timed on a developed state: checkpoint the initial
-        # state, step, autotune there, then restore the initial state bit for
-        # bit (all outside the timed region).
-        import tempfile
-        tune_at = args.tune_at if args.tune_at is not None else (3 * args.steps) // 10
-        with tempfile.TemporaryDirectory() as tmp:
-            ckpt = os.path.join(tmp, "initial.bin")
+    import tempfile
+    with tempfile.TemporaryDirectory() as tmp:
+        ckpt = os.path.join(tmp, "initial.bin")
+        # The walks' relative speed changes as the flow develops, so the candidates are
+        # timed on a developed state (a fixed step of the run, not a fraction of --steps):
+        # checkpoint the initial state, step there, autotune (it restores the state it
+        # timed from), then either restart from the initial state bit for bit or time the
+        # window from there (all outside the timed region).
+        if args.autotune or args.start_at > 0:
             s.checkpoint(ckpt, 0)
-            s.step(tune_at)
+            s.step(args.tune_at)
             s.sync()
+        if args.autotune:
             tune = s.autotune(steps=3)
+            tune["tuned_at_step"] = args.tune_at
+        else:
+            tune = s.get_tuning()
+        if args.start_at != args.tune_at and (args.autotune or args.start_at > 0):
             s.restore(ckpt)
-        tune["tuned_at_step"] = tune_at
-    else:
-        tune = s.get_tuning()
-    print(f"[bench] tuning: {tune}", file=sys.stderr, flush=True)
+            if args.start_at:
+                s.step(args.start_at)
+    print(f"[bench] tuning: {tune}; timed window starts at step {args.start_at}", file=sys.stderr, flush=True)
     s.set_graph(args.graph)
-    # device clock around every continuity+momentum launch, inside the captured step
-    s.clock_governing(True)
+    # device clock around every launch of the three neighbour passes, inside the captured step
+    for st in (z.ST_WALL, z.ST_LDG, z.ST_GOVERNING):
+        s.clock_stage(st, True)
     s.step(args.warmup)
     s.sync()
     print(f"[bench] warm-up {args.warmup} steps done", file=sys.stderr, flush=True)
 
     # timed region: K graph-replayed steps, synchronised on both sides
     s.sync()
-    s.clock_read()  # reset
+    for st in (z.ST_WALL, z.ST_LDG, z.ST_GOVERNING):
+        s.clock_read_stage(st)  # reset
     t0 = time.perf_counter()
     # replayed in chunks with a host sync between them: bounds the launches in
     # flight (profilers that trace every dispatch keep a record per launch)
@@ -220,8 +315,9 @@ def main():
         s.step(min(args.chunk, args.steps - k0))
         s.sync()
     wall = time.perf_counter() - t0
-    gov_total_ms, gov_launches = s.clock_read()
-    s.clock_governing(False)
+    clocks = {st: s.clock_read_stage(st) for st in (z.ST_WALL, z.ST_LDG, z.ST_GOVERNING)}
+    for st in (z.ST_WALL, z.ST_LDG, z.ST_GOVERNING):
+        s.clock_stage(st, False)
     # a sample of further steps between HIP events on the solver's stream (cross-check; 0: none)
     ev_ms = s.time_steps(args.event_steps) if args.event_steps > 0 else None
     if ev_ms is not None:
@@ -230,7 +326,6 @@ def main():
 
     # per-kernel times (HIP events around every stage on the same stream)
     prof = s.profile_steps(args.profile_steps)
-    gov_ms = gov_total_ms / max(gov_launches, 1)  # average over the timed region's launches
     killed = s.killed()
     sim_t = s.time()
     extra = {}
@@ -241,9 +336,13 @@ def main():
 
     ms_step = wall * 1e3 / args.steps
     value = case.n * args.steps / wall
-    gbytes = governing_bytes(case.n, G)
-    gov_gbs = gbytes / (gov_ms * 1e-3) / 1e9
     b_step = step_bytes(case.n, G, n_fluid)
+    rec = RECORDED.get((args.case, args.dx, args.gsf, tune["walk"])) if not args.closure else None
+    avg = {st: (ms / max(n, 1), n) for st, (ms, n) in clocks.items()}
+    # the dominant kernel: the gradient pass (VALU-bound: candidate tests and pair arithmetic)
+    roof = kernel_roofline("k_ldg", *avg[z.ST_LDG], ldg_bytes(case.n, n_fluid, G), rec, "valu")
+    roof_gov = kernel_roofline("k_governing", *avg[z.ST_GOVERNING], governing_bytes(case.n, n_fluid, G), rec,
+                               "valu")
     out = {
         "metric": metric,
         "value": value,
@@ -260,14 +359,13 @@ def main():
         "config": {"workload": workload,
                    "particles": case.n, "fluid_particles": n_fluid, "cells": G,
                    "fluid_particle_steps_per_s": n_fluid * args.steps / wall,
+                   "timed_steps": [args.start_at, args.start_at + args.steps], "run_steps": run_len,
                    "event_ms_per_step": ev_ms / args.event_steps if ev_ms is not None else None, "event_steps": args.event_steps,
                    "sim_time_s": sim_t, "removed_particles": killed,
                    "tuning": tune, "graph": args.graph, **extra},
-        "roofline": {"bound": "hbm", "kernel": "k_governing", "achieved": gov_gbs, "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": gov_gbs / HBM_PEAK_GBS,
-                     "traffic": GOVERNING_TRAFFIC_BYTES.get((args.case, args.dx, args.gsf, tune["walk"])),
-                     "bytes_per_launch": gbytes, "avg_launch_ms": gov_ms, "launches": gov_launches,
-                     "timing": "device wall-clock stamps around every launch of the timed region"},
+        "roofline": roof,
+        "roofline_governing": roof_gov,
+        "wall_pass_ms_per_launch": avg[z.ST_WALL][0],
         "step_roofline": {"bytes_per_particle_step": b_step, "achieved": b_step * value / 1e9,
                           "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": b_step * value / 1e9 / HBM_PEAK_GBS},
         "stage_ms_per_step": {k: v / args.profile_steps for k, v in prof.items()},
@@ -275,10 +373,13 @@ def main():
     if not args.no_cpu_baseline:
         print("[bench] CPU baseline (FP32 oracle)...", file=sys.stderr, flush=True)
         if batch:  # every 16th scene: a bounded, size-representative sample
-            out["cpu_baseline"] = cpu_baseline(case.cases[::16], steps=20,
+            out["cpu_baseline"] = cpu_baseline(case.cases[::16], steps=10, single=(2, 3),
                                                what=f"every 16th scene of the {case.nscenes}-scene ensemble")
+        elif floating:  # 12.5 M particles: one step per timing, fewer repeats
+            out["cpu_baseline"] = cpu_baseline([case], steps=1, repeats=3, warmup=1, single=(1, 1), setup=setup,
+                                               what=f"the same {case.n}-particle case")
         else:
-            out["cpu_baseline"] = cpu_baseline([case], steps=1 if floating else 3, setup=setup,
+            out["cpu_baseline"] = cpu_baseline([case], steps=1, setup=setup,
                                                what=f"the same {case.n}-particle case")
     print(json.dumps(out), flush=True)
 

@@ -533,6 +533,13 @@ int zsph_set_graph(zsph_ctx *ctx, int on);
  * may be NULL); zsph_clock_governing also resets. */
 int zsph_clock_governing(zsph_ctx *ctx, int on);
 int zsph_clock_read(zsph_ctx *ctx, double *ms, long long *launches);
+/* The same clock for any of the three neighbour passes (no counterpart in the
+ * reference): stage = ZSPH_ST_WALL, ZSPH_ST_LDG (the gradient pass) or
+ * ZSPH_ST_GOVERNING; several stages may be clocked together, each with its
+ * own accumulator.  zsph_clock_governing / zsph_clock_read are the
+ * ZSPH_ST_GOVERNING forms.  Other stages are ZSPH_E_ARG. */
+int zsph_clock_stage(zsph_ctx *ctx, int stage, int on);
+int zsph_clock_read_stage(zsph_ctx *ctx, int stage, double *ms, long long *launches);
 
 /* Number of particles removed by zsph_hash since zsph_create. */
 int zsph_killed(zsph_ctx *ctx, int *killed);

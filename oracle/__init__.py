@@ -44,6 +44,7 @@ def lib():
                   "orc_correction", "orc_step"):
             sig(n, C.c_int, [vp, C.c_int])
         sig("orc_sort_index", C.c_int, [vp])
+        sig("orc_set_threads", C.c_int, [C.c_int])
         sig("orc_sort_data", C.c_int, [vp, C.c_int, C.c_int])
         sig("orc_killed", C.c_int, [vp, P(C.c_longlong)])
         sig("orc_array_get", C.c_int, [vp, C.c_int, C.c_int, P(C.c_double), C.c_size_t])
@@ -203,6 +204,11 @@ class Oracle:
         T = (C.c_double * 3)()
         self._ok(lib().orc_body_reduce(self._h, buf, F, T), "body_reduce")
         return np.array(F[:]), np.array(T[:])
+
+
+def set_threads(n):
+    """OpenMP threads of the oracle's later steps (0: keep); returns the previous count."""
+    return lib().orc_set_threads(int(n))
 
 
 def kernel_W(params, r): return lib().orc_kernel_W(C.byref(params), r)

@@ -1,6 +1,7 @@
 // C entry points of the CPU oracle (TEST INFRASTRUCTURE ONLY): the same
 // call sequence as libzsph's C ABI, so tests drive both identically.
 // fp64 = 1 builds the double-precision parity oracle, 0 the FP32 baseline.
+#include <omp.h>
 #include <cstring>
 #include <memory>
 #include <vector>
@@ -147,6 +148,13 @@ int orc_governing(void *h, int b) { return ORC_CALL(o.governing(b)); }
 int orc_prediction(void *h, int b) { return ORC_CALL(o.prediction(b)); }
 int orc_correction(void *h, int b) { return ORC_CALL(o.correction(b)); }
 int orc_step(void *h, int n) { return ORC_CALL(o.step(n)); }
+// OpenMP threads of the calling thread's later parallel regions (the CPU
+// baseline's one-thread run); returns the previous setting.
+int orc_set_threads(int n) {
+    const int prev = omp_get_max_threads();
+    if (n > 0) omp_set_num_threads(n);
+    return prev;
+}
 int orc_killed(void *h, long long *k) { return ORC_CALL(*k = o.killed); }
 int orc_array_get(void *h, int which, int b, double *out, size_t count) {
     return dispatch<double>(static_cast<Handle *>(h), [&](auto &o) { return array_get(o, which, b, out, count); });

@@ -279,7 +279,7 @@ struct Oracle {
             if (b.type[i] != ZSPH_FLUID) continue;
             R L[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
             R g[3] = {0, 0, 0};
-            R D = 0, gs = 0, ds = 0;
+            R D = 0, gs = 0, ds = 0, ls = 0;
             const R rhoi = b.posr[4 * i + 3];
             walk(b, i, [&](int j) {
                 if (j == i || !is_nbr(b, i, j)) return;
@@ -293,6 +293,7 @@ struct Oracle {
                     for (int c = 0; c < 3; ++c) L[a][c] -= F * Vj * x[a] * x[c];
                 for (int a = 0; a < 3; ++a) g[a] += F * (rhoj - rhoi) * Vj * x[a];
                 gs += std::fabs(F * (rhoj - rhoi) * Vj) * r;
+                ls += std::fabs(F * Vj) * r * r;
                 if (b.type[j] == ZSPH_FLUID) {
                     R dvdx = 0;
                     for (int a = 0; a < 3; ++a) dvdx += (b.velp[4 * i + a] - b.velp[4 * j + a]) * x[a];
@@ -301,17 +302,35 @@ struct Oracle {
                 }
             });
             R G[3] = {0, 0, 0};
+            // Error scale of G = L^-1 g: to first order dG = L^-1 (dg - dL G), with
+            // |dg| <~ u gs and |dL| <~ u ls (u: unit round-off of the summed terms),
+            // so |dG| <~ u |L^-1| (gs + ls |G|) in the max norm; with L singular G = g.
+            R gscale = gs;
             if (delta) {
                 if (p.dim == 2) L[1][1] += R(1);
                 R Lc[3][3];
                 std::memcpy(Lc, L, sizeof(L));
                 R sol[3] = {0, 0, 0};
                 const R det = solve3(Lc, g, sol);
-                if (std::fabs(det) > R(1.8e-8)) { G[0] = sol[0]; G[1] = sol[1]; G[2] = sol[2]; }
-                else { G[0] = g[0]; G[1] = g[1]; G[2] = g[2]; }
+                if (std::fabs(det) > R(1.8e-8)) {
+                    G[0] = sol[0]; G[1] = sol[1]; G[2] = sol[2];
+                    R inv_norm = 0;  // max row sum of |L^-1|, from L^-1 e_c column by column
+                    R col[3][3];
+                    for (int c = 0; c < 3; ++c) {
+                        R e[3] = {R(c == 0), R(c == 1), R(c == 2)};
+                        std::memcpy(Lc, L, sizeof(L));
+                        solve3(Lc, e, col[c]);
+                    }
+                    for (int r = 0; r < 3; ++r)
+                        inv_norm = std::max(inv_norm, std::fabs(col[0][r]) + std::fabs(col[1][r]) + std::fabs(col[2][r]));
+                    const R gmax = std::max(std::fabs(G[0]), std::max(std::fabs(G[1]), std::fabs(G[2])));
+                    gscale = inv_norm * (gs + ls * gmax);
+                } else {
+                    G[0] = g[0]; G[1] = g[1]; G[2] = g[2];
+                }
             }
             gd[4 * i] = G[0]; gd[4 * i + 1] = G[1]; gd[4 * i + 2] = G[2]; gd[4 * i + 3] = D;
-            gd_scale[2 * i] = delta ? gs : R(0);
+            gd_scale[2 * i] = delta ? gscale : R(0);
             gd_scale[2 * i + 1] = ds;
         }
     }

@@ -175,8 +175,9 @@ def test_dense_cluster_stages(gsf):
     o.ldg(1)
     gg, og = s.get(A_GD), o.get(A_GD)
     sc = o.get(A_GD_SCALE)
-    gsc = np.maximum(sc[:, 0:1], np.abs(og[:, :3]).max(axis=1, keepdims=True))
-    assert_close_scaled(gg[f, :3], og[f, :3], np.broadcast_to(gsc[f], gg[f, :3].shape), "grad rho^L", rel=2e-5)
+    # sc[:, 0] bounds the FP32 error of G = L^-1 g to first order:
+    # |L^-1| (sum |g terms| + sum |L terms| |G|), so the 1e-5 gate holds where L is ill-conditioned
+    assert_close_scaled(gg[f, :3], og[f, :3], np.broadcast_to(sc[f, 0:1], gg[f, :3].shape), "grad rho^L")
     assert_close_scaled(gg[f, 3], og[f, 3], sc[f, 1], "div v")
     o.set(A_GD, gg)
     s.governing(1)

@@ -14,7 +14,8 @@ import numpy as np
 from . import _abi as abi
 from ._abi import (A_ACC, A_CELL_START, A_GD, A_INDEX, A_KEY, A_POSR, A_TABLE, A_TPOSR, A_TVELP,
                    A_TYPE, A_VELP, BODY, DIFF_ANTUONO, DIFF_MC, F_ACC, F_ALL, F_HASH, F_POS, F_RHOP,
-                   F_TABLE, F_TYPE, F_VEL, FLUID, KILLED, PISTON, WALL, WALK_LIST, WALK_MASK, WALK_HITS)
+                   F_TABLE, F_TYPE, F_VEL, FLUID, KILLED, PISTON, WALL, WALK_LIST, WALK_MASK, WALK_HITS,
+                   ST_GOVERNING, ST_LDG, ST_WALL)
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libzsph.so")
@@ -393,6 +394,17 @@ class Solver:
         ms = C.c_double(0.0)
         n = C.c_longlong(0)
         self._chk(lib().zsph_clock_read(self._ctx, C.byref(ms), C.byref(n)), "zsph_clock_read")
+        return ms.value, n.value
+
+    def clock_stage(self, stage, on=True):
+        """Device clock around every launch of one neighbour pass (ST_WALL, ST_LDG, ST_GOVERNING)."""
+        self._chk(lib().zsph_clock_stage(self._ctx, stage, 1 if on else 0), "zsph_clock_stage")
+
+    def clock_read_stage(self, stage):
+        """(milliseconds, launches) of that pass since the last read; resets."""
+        ms = C.c_double(0.0)
+        n = C.c_longlong(0)
+        self._chk(lib().zsph_clock_read_stage(self._ctx, stage, C.byref(ms), C.byref(n)), "zsph_clock_read_stage")
         return ms.value, n.value
 
     def filter_fields(self, rows, radius, hc, buf=1):

@@ -208,6 +208,8 @@ def load(path):
     _sig(lib, "zsph_clock_governing", _i, [vp, _i])
     _sig(lib, "zsph_set_graph", _i, [vp, _i])
     _sig(lib, "zsph_clock_read", _i, [vp, P(C.c_double), P(C.c_longlong)])
+    _sig(lib, "zsph_clock_stage", _i, [vp, _i, _i])
+    _sig(lib, "zsph_clock_read_stage", _i, [vp, _i, P(C.c_double), P(C.c_longlong)])
     _sig(lib, "zsph_checkpoint_write", _i, [vp, C.c_char_p, C.c_longlong])
     _sig(lib, "zsph_checkpoint_read", _i, [vp, C.c_char_p, P(C.c_longlong)])
     _sig(lib, "zsph_checkpoint_inspect", _i, [C.c_char_p, _i, P(CheckpointInfo)])
@@ -227,8 +229,9 @@ EXPORTS = (
     "zsph_set_piston", "zsph_set_time", "zsph_get_time", "zsph_body_reduce", "zsph_case_wavetank",
     "zsph_create_batch", "zsph_scene_diagnostics", "zsph_mlp_create", "zsph_mlp_destroy", "zsph_mlp_last_error",
     "zsph_mlp_set", "zsph_mlp_get", "zsph_mlp_data", "zsph_mlp_eval", "zsph_mlp_adam", "zsph_mlp_infer",
-    "zsph_set_closure", "zsph_closure_sample", "zsph_filter_fields", "zsph_clock_governing", "zsph_clock_read",
+    "zsph_set_closure", "zsph_closure_sample", "zsph_filter_fields", "zsph_clock_governing", "zsph_clock_read", "zsph_clock_stage", "zsph_clock_read_stage",
     "zsph_set_graph", "zsph_checkpoint_write", "zsph_checkpoint_read", "zsph_checkpoint_inspect",
 )
 
 STAGES = ("hash", "sort", "cells", "reorder", "wall", "ldg", "governing", "integrate")
+ST_HASH, ST_SORT, ST_CELLS, ST_REORDER, ST_WALL, ST_LDG, ST_GOVERNING, ST_INTEGRATE = range(8)

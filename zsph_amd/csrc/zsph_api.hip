@@ -48,8 +48,8 @@ struct zsph_ctx {
     std::vector<int> scene_off;       // first particle of each scene in upload order (+ N)
     int cells_buf = -1;               // buffer the cell table describes
     double *scene_part = nullptr;     // per-scene diagnostics (batches only)
-    unsigned long long *clock = nullptr;  // continuity+momentum kernel clock {start, ticks, launches}
-    bool clock_on = false;
+    unsigned long long *clock = nullptr;  // per stage {start, ticks, launches} (3 x ZSPH_ST_COUNT words)
+    unsigned clock_mask = 0;              // stages (1 << ZSPH_ST_*) bracketed by clock stamps
     bool closure_on = false;      // learned closure: features, network, scaled terms
     float *feat = nullptr;        // N x ZSPH_NF closure features
     float2 *scale = nullptr;      // N scale factors (diffusion, viscous)
@@ -183,6 +183,11 @@ int stage_sort_data(zsph_ctx *c, int from, int to, Prof *pf = nullptr) {
     c->cells_buf = to;
     return check_launch(c, "sort_data");
 }
+// Clock words of stage st when that stage is clocked, else nullptr.
+unsigned long long *stage_clock(zsph_ctx *c, int st) {
+    return (c->clock && (c->clock_mask >> st & 1u)) ? c->clock + 3 * st : nullptr;
+}
+
 // The record a walk's gradient pass writes and its continuity+momentum pass reads.
 int *walk_record(zsph_ctx *c) { return c->walk == zs::WALK_HITS ? c->hits : c->nlist; }
 bool walk_records(int walk) { return walk == zs::WALK_LIST || walk == zs::WALK_HITS; }
@@ -192,12 +197,18 @@ int stage_boundary(zsph_ctx *c, int b, Prof *pf = nullptr) {
     const uint32_t *sk = c->key[c->sorted_slot];
     {
         Scope sc(pf, ZSPH_ST_WALL);
+        unsigned long long *ck = stage_clock(c, ZSPH_ST_WALL);
+        if (ck) zs::launch_clock(c->stream, ck, true);
         zs::launch_wall(c->stream, c->walk, c->q, c->scene_prm, c->buf[b], sk, c->cell_start);
+        if (ck) zs::launch_clock(c->stream, ck, false);
     }
     {
         Scope sc(pf, ZSPH_ST_LDG);
+        unsigned long long *ck = stage_clock(c, ZSPH_ST_LDG);
+        if (ck) zs::launch_clock(c->stream, ck, true);
         zs::launch_ldg(c->stream, c->walk, c->q, c->scene_prm, c->buf[b], sk, c->cell_start, c->gd,
                        walk_record(c), c->ncount, c->closure_on ? c->feat : nullptr);
+        if (ck) zs::launch_clock(c->stream, ck, false);
     }
     c->list_valid = walk_records(c->walk);
     return check_launch(c, "boundary");
@@ -213,7 +224,7 @@ int stage_governing(zsph_ctx *c, int b, Prof *pf = nullptr) {
                                  c->scale, zs::mlp_infer_blocks(c->q.N));
         zs::launch_governing(c->stream, walk, c->q, c->scene_prm, c->buf[b], c->key[c->sorted_slot], c->cell_start,
                              c->gd, c->acc, walk_record(c), c->ncount, c->closure_on ? 1 : 0, c->scale, c->terms,
-                             c->clock_on ? c->clock : nullptr, c->has_body || c->has_piston);
+                             stage_clock(c, ZSPH_ST_GOVERNING), c->has_body || c->has_piston);
     }
     return check_launch(c, "governing");
 }
@@ -889,10 +900,12 @@ int zsph_set_closure(zsph_ctx *c, const zsph_closure *net) {
     }
     const size_t N = (size_t)c->q.N;
     int r;
+    // feat may already exist (zsph_closure_sample allocates it): allocate only what is missing
+    if (!c->feat && (r = dalloc(c, &c->feat, N * ZSPH_NF))) return r;
+    if (!c->scale && (r = dalloc(c, &c->scale, N))) return r;
     if (!c->mlp_P) {
         uint16_t *pack = nullptr;
-        if ((r = dalloc(c, &c->feat, N * ZSPH_NF)) || (r = dalloc(c, &c->scale, N)) ||
-            (r = dalloc(c, &c->mlp_P, (size_t)ZSPH_NPARAM)) || (r = dalloc(c, &pack, zs::mlp_pack_elems())) ||
+        if ((r = dalloc(c, &c->mlp_P, (size_t)ZSPH_NPARAM)) || (r = dalloc(c, &pack, zs::mlp_pack_elems())) ||
             (r = dalloc(c, &c->mlp_aux, 1)))
             return r;
         c->mlp_pack = pack;
@@ -977,27 +990,42 @@ int zsph_set_graph(zsph_ctx *c, int on) {
     return ZSPH_OK;
 }
 
-int zsph_clock_governing(zsph_ctx *c, int on) {
+namespace {
+bool clockable(int st) { return st == ZSPH_ST_WALL || st == ZSPH_ST_LDG || st == ZSPH_ST_GOVERNING; }
+}  // namespace
+
+int zsph_clock_stage(zsph_ctx *c, int stage, int on) {
     if (!c) return ZSPH_E_ARG;
+    if (!clockable(stage)) return set_err(c, ZSPH_E_ARG, "zsph_clock_stage: stage %d has no clock (wall, gradient "
+                                                        "or continuity+momentum pass only)", stage);
     int r;
-    if (!c->clock && (r = dalloc(c, &c->clock, 3))) return r;
-    if ((on != 0) != c->clock_on) drop_graph(c);  // the captured step gains / loses the stamps
-    c->clock_on = on != 0;
-    return zsph_clock_read(c, nullptr, nullptr);
+    if (!c->clock && (r = dalloc(c, &c->clock, 3 * ZSPH_ST_COUNT))) return r;
+    const unsigned bit = 1u << stage;
+    const unsigned mask = on ? (c->clock_mask | bit) : (c->clock_mask & ~bit);
+    if (mask != c->clock_mask) drop_graph(c);  // the captured step gains / loses the stamps
+    c->clock_mask = mask;
+    return zsph_clock_read_stage(c, stage, nullptr, nullptr);
 }
 
-int zsph_clock_read(zsph_ctx *c, double *ms, long long *launches) {
+int zsph_clock_read_stage(zsph_ctx *c, int stage, double *ms, long long *launches) {
     if (!c) return ZSPH_E_ARG;
-    if (!c->clock) return set_err(c, ZSPH_E_STATE, "zsph_clock_read: clock not enabled");
+    if (!clockable(stage)) return set_err(c, ZSPH_E_ARG, "zsph_clock_read_stage: stage %d has no clock", stage);
+    if (!c->clock) return set_err(c, ZSPH_E_STATE, "zsph_clock_read_stage: clock not enabled");
     unsigned long long v[3] = {0, 0, 0};
-    int r = copy_sync(c, v, c->clock, sizeof(v), hipMemcpyDeviceToHost);
+    int r = copy_sync(c, v, c->clock + 3 * stage, sizeof(v), hipMemcpyDeviceToHost);
     if (r) return r;
     int khz = 0;
     HIPCHK(c, hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, c->device));
     if (ms) *ms = khz > 0 ? (double)v[1] / (double)khz : 0.0;
     if (launches) *launches = (long long)v[2];
     const unsigned long long zero[3] = {0, 0, 0};
-    return copy_sync(c, c->clock, zero, sizeof(zero), hipMemcpyHostToDevice);
+    return copy_sync(c, c->clock + 3 * stage, zero, sizeof(zero), hipMemcpyHostToDevice);
+}
+
+int zsph_clock_governing(zsph_ctx *c, int on) { return zsph_clock_stage(c, ZSPH_ST_GOVERNING, on); }
+
+int zsph_clock_read(zsph_ctx *c, double *ms, long long *launches) {
+    return zsph_clock_read_stage(c, ZSPH_ST_GOVERNING, ms, launches);
 }
 
 int zsph_killed(zsph_ctx *c, int *killed) {
