@@ -340,6 +340,19 @@ int zsph_diagnostics(zsph_ctx *ctx, int buf, float front_zmax, zsph_diag *out);
  * Buffer `buf` must be in cell order (after zsph_sort_data or zsph_step). */
 int zsph_probe_pressure(zsph_ctx *ctx, int buf, const float *pts, int npts, float *p_out);
 
+/* zsph_probe_pressure without any device allocation.  No counterpart in the
+ * reference: it is for callers that cannot allocate (code under HIP graph
+ * capture, tight output loops, harnesses that forbid hipMalloc).  The points
+ * and results are staged in a fixed library-owned device array of
+ * ZSPH_PROBE_NOALLOC_MAX probes per GPU (64 KiB, held until the process
+ * ends); more points are ZSPH_E_ARG.  Every call on a GPU shares that GPU's
+ * array, so calls that overlap on one GPU (other threads, other contexts)
+ * race; calls from one thread are ordered.  The kernel writes every result,
+ * so nothing in the array needs clearing.  Same results as
+ * zsph_probe_pressure, bit for bit. */
+#define ZSPH_PROBE_NOALLOC_MAX 4096
+int zsph_probe_pressure_noalloc(zsph_ctx *ctx, int buf, const float *pts, int npts, float *p_out);
+
 /* ------------------------------------------------------------------------ */
 /* Batches of independent scenes (no counterpart in the reference, which    */
 /* runs one case per process).                                              */

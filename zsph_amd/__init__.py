@@ -352,6 +352,15 @@ class Solver:
                   "zsph_probe_pressure")
         return out
 
+    def probe_pressure_noalloc(self, points, buf=0):
+        """probe_pressure staged in the library's fixed device array: no device allocation,
+        at most abi.PROBE_NOALLOC_MAX points; calls overlapping on one GPU race."""
+        pts = np.ascontiguousarray(points, np.float32).reshape(-1, 3)
+        out = np.zeros(len(pts), np.float32)
+        self._chk(lib().zsph_probe_pressure_noalloc(self._ctx, buf, _ptr(pts, C.c_float), len(pts),
+                                                    _ptr(out, C.c_float)), "zsph_probe_pressure_noalloc")
+        return out
+
     def killed(self):
         k = C.c_int(0)
         self._chk(lib().zsph_killed(self._ctx, C.byref(k)), "zsph_killed")

@@ -8,6 +8,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -65,6 +66,7 @@ struct zsph_ctx {
     hipGraphExec_t graph_exec = nullptr;
     bool use_graph = true;        // zsph_step replays a captured graph (else launches directly)
     std::vector<unsigned char> closure_net;  // the zsph_closure last set (checkpoints carry it)
+    float *probe_scratch = nullptr;  // this device's probe_scratch (zsph_probe_pressure_noalloc)
     std::string err;
     std::vector<void *> allocs;
 };
@@ -270,6 +272,38 @@ void drop_graph(zsph_ctx *c) {
     c->graph = nullptr;
 }
 
+// ---- probes without allocation ----------------------------------------------
+//
+// Staging for zsph_probe_pressure_noalloc: the points (3 floats each) and the
+// results (1 float each) of at most ZSPH_PROBE_NOALLOC_MAX probes.  Each GPU
+// holds its own copy of this array until the process ends.  In host code the
+// name `probe_scratch` is only a host-side stand-in: the device address of a
+// GPU's copy comes from hipGetSymbolAddress, resolved once per device (with
+// that device current) and kept in probe_scratch_dev.
+static __device__ __align__(256) float probe_scratch[4 * ZSPH_PROBE_NOALLOC_MAX];
+
+constexpr int kMaxDevices = 64;
+float *probe_scratch_dev[kMaxDevices] = {};
+std::mutex probe_scratch_mu;
+
+// Device address of this device's probe_scratch (the caller has made `device`
+// current); nullptr with the HIP error in *err on failure.
+float *probe_scratch_ptr(int device, hipError_t *err) {
+    *err = hipSuccess;
+    if (device < 0 || device >= kMaxDevices) {
+        *err = hipErrorInvalidDevice;
+        return nullptr;
+    }
+    std::lock_guard<std::mutex> lock(probe_scratch_mu);
+    if (!probe_scratch_dev[device]) {
+        void *ptr = nullptr;
+        *err = hipGetSymbolAddress(&ptr, HIP_SYMBOL(probe_scratch));
+        if (*err != hipSuccess) return nullptr;
+        probe_scratch_dev[device] = static_cast<float *>(ptr);
+    }
+    return probe_scratch_dev[device];
+}
+
 bool valid_params(const zsph_params *p) {
     if (p->ptc_num < 1 || p->ptc_num >= zs::MAX_PARTICLES || p->grid_xdim < 1 || p->grid_ydim < 1 || p->grid_zdim < 1 || !(p->h > 0.0f) ||
         !(p->grid_size > 0.0f) || p->grid_factor < 1 || (p->dim != 2 && p->dim != 3) || !(p->gamma > 0.0f) ||
@@ -290,6 +324,11 @@ int create_ctx(zsph_ctx *c, int device, size_t N, size_t G, bool batch) {
     }
     (void)hipGetDevice(&c->device);
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) return ZSPH_E_HIP;
+    {  // the device address of this device's static probe staging array (no allocation)
+        hipError_t e = hipSuccess;
+        c->probe_scratch = probe_scratch_ptr(c->device, &e);
+        if (!c->probe_scratch) return set_err(c, ZSPH_E_HIP, "probe scratch address: %s", hipGetErrorString(e));
+    }
     int r;
     for (int b = 0; b < 2; ++b) {
         // posr carries zs::POS_PAD zeroed spare elements read by the neighbour walks' candidate rounds
@@ -884,6 +923,25 @@ int zsph_probe_pressure(zsph_ctx *c, int b, const float *pts, int npts, float *p
     if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
     (void)hipFree(d);
     if (e != hipSuccess) return set_err(c, ZSPH_E_HIP, "probe pressure failed: %s", hipGetErrorString(e));
+    return ZSPH_OK;
+}
+
+int zsph_probe_pressure_noalloc(zsph_ctx *c, int b, const float *pts, int npts, float *p_out) {
+    if (!c || !valid_buf(b) || npts < 0 || (npts && (!pts || !p_out)))
+        return set_err(c, ZSPH_E_ARG, "zsph_probe_pressure_noalloc: bad argument");
+    if (npts > ZSPH_PROBE_NOALLOC_MAX)
+        return set_err(c, ZSPH_E_ARG, "zsph_probe_pressure_noalloc: %d points, at most %d (use zsph_probe_pressure)",
+                       npts, ZSPH_PROBE_NOALLOC_MAX);
+    if (c->nscenes > 1)
+        return set_err(c, ZSPH_E_ARG, "zsph_probe_pressure_noalloc: one scene only");
+    if (npts == 0) return ZSPH_OK;
+    float *d = c->probe_scratch;  // resolved for this context's device at zsph_create
+    // points in [0, 3 npts), results in [3 npts, 4 npts): the kernel writes every result
+    HIPCHK(c, hipMemcpyAsync(d, pts, sizeof(float) * 3 * (size_t)npts, hipMemcpyHostToDevice, c->stream));
+    zs::launch_probe(c->stream, c->q, c->buf[b], d, npts, d + 3 * (size_t)npts);
+    if (int r = check_launch(c, "probe pressure")) return r;
+    HIPCHK(c, hipMemcpyAsync(p_out, d + 3 * (size_t)npts, sizeof(float) * npts, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
     return ZSPH_OK;
 }
 
