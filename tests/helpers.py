@@ -34,7 +34,13 @@ def jitter(case, seed=12345, amp=0.1, vmax=1.0, drho=0.01):
         h.vel[:, 1] = 0.0
     rho = case.json.rho0 * (1.0 + rng.uniform(-drho, drho, n))
     h.rhop[f, 0] = rho.astype(np.float32)
-    h.rhop[f, 1] = np.array([oracle.eos_p(case.params, float(r)) for r in h.rhop[f, 0]], np.float32)
+    if n <= 200000:
+        h.rhop[f, 1] = np.array([oracle.eos_p(case.params, float(r)) for r in h.rhop[f, 0]], np.float32)
+    else:  # the same EOS vectorised (any pressure is a valid initial state; both sides start from it)
+        p = case.params
+        e = h.rhop[f, 0].astype(np.float64) / p.rho0 - 1.0
+        h.rhop[f, 1] = (p.cs2 * p.rho0 * e if p.gamma == 1.0
+                        else p.rho0 * p.cs2 / p.gamma * np.expm1(p.gamma * np.log1p(e))).astype(np.float32)
     return case
 
 
