@@ -246,6 +246,25 @@ int zsph_upload(zsph_ctx *ctx, int buf, const zsph_host_state *st);
  * the `fields` mask (ZSPH_F_HASH reads the sorted hash of the last sort). */
 int zsph_download(zsph_ctx *ctx, int buf, zsph_host_state *st, int fields);
 
+/* gpu_to_cpu without the solver stall (lib/gmemcpy.cu:3-36 feeding the
+ * writer threads of io/mul_thread.cu:50-80).  zsph_download_async snapshots
+ * the `fields` of buffer `buf` on the device, in stream order after the
+ * steps already issued (a device-to-device copy, ~0.05 ms per million
+ * particles), and copies the snapshot into pinned host memory on a separate
+ * copy stream, so later zsph_step calls run while it crosses PCIe; it
+ * returns at once with a frame handle.  zsph_frame_wait blocks until that
+ * copy is done, unpacks it into `st` exactly as zsph_download would (same
+ * bits) and releases the frame; it may be called from another thread (a
+ * writer), once per frame.  A context holds ZSPH_FRAME_SLOTS frames in
+ * flight; zsph_download_async waits for the oldest when all are taken.  The
+ * snapshot and pinned buffers are allocated at the first call (ZSPH_E_NOMEM
+ * if that fails) and freed by zsph_destroy, which must come after every
+ * frame has been waited for. */
+#define ZSPH_FRAME_SLOTS 3
+typedef struct zsph_frame zsph_frame;
+int zsph_download_async(zsph_ctx *ctx, int buf, int fields, zsph_frame **frame);
+int zsph_frame_wait(zsph_frame *frame, zsph_host_state *st);
+
 /* lib/hash.cu:3-26 cuda_ptc_hash: cell hash of buffer `buf`, index = i;
  * particles outside the grid are removed (type 100, parked at the grid origin). */
 int zsph_hash(zsph_ctx *ctx, int buf);

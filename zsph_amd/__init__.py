@@ -201,6 +201,24 @@ def checkpoint_info(path, verify=True):
     return d
 
 
+class Frame:
+    """An asynchronous download in flight (zsph_download_async); wait() once."""
+
+    def __init__(self, handle, n):
+        self._h, self.n = handle, n
+
+    def wait(self):
+        if self._h is None:
+            raise ZsphError(abi.E_STATE, "frame already waited for")
+        h = HostArrays(self.n)
+        st = h.state()
+        rc = lib().zsph_frame_wait(self._h, C.byref(st))
+        self._h = None
+        if rc != abi.OK:
+            raise ZsphError(rc, "zsph_frame_wait")
+        return h
+
+
 class Solver:
     """One device context (zsph_ctx): the reference's device buffers and kernels.
     `case` is a Case (one scene) or a Batch (independent scenes in one context)."""
@@ -255,6 +273,14 @@ class Solver:
         st = h.state()
         self._chk(lib().zsph_download(self._ctx, buf, C.byref(st), fields), "zsph_download")
         return h
+
+    def download_async(self, buf=0, fields=F_ALL):
+        """Start an asynchronous download of buffer `buf` (zsph_download_async): returns a
+        Frame whose wait() gives the HostArrays, equal to download() at this point of the
+        stream.  At most abi.FRAME_SLOTS frames may be in flight per context."""
+        h = C.c_void_p()
+        self._chk(lib().zsph_download_async(self._ctx, buf, fields, C.byref(h)), "zsph_download_async")
+        return Frame(h, self.n)
 
     # the reference's per-stage entry points
     def hash(self, buf=0):

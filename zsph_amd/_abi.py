@@ -96,6 +96,7 @@ WALK_MASK, WALK_HITS, WALK_LIST = 0, 1, 2
 
 NF, NH, NT = 24, 64, 12
 PROBE_NOALLOC_MAX = 4096  # ZSPH_PROBE_NOALLOC_MAX
+FRAME_SLOTS = 3  # ZSPH_FRAME_SLOTS
 NPARAM = NF * NH + NH + 2 * (NH * NH + NH) + NH * 2 + 2
 
 
@@ -181,6 +182,8 @@ def load(path):
     _sig(lib, "zsph_diagnostics", _i, [vp, _i, _f, P(Diag)])
     _sig(lib, "zsph_probe_pressure", _i, [vp, _i, P(_f), _i, P(_f)])
     _sig(lib, "zsph_probe_pressure_noalloc", _i, [vp, _i, P(_f), _i, P(_f)])
+    _sig(lib, "zsph_download_async", _i, [vp, _i, _i, P(vp)])
+    _sig(lib, "zsph_frame_wait", _i, [vp, P(HostState)])
     _sig(lib, "zsph_profile_steps", _i, [vp, _i, P(_f)])
     _sig(lib, "zsph_set_tuning", _i, [vp, P(Tuning)])
     _sig(lib, "zsph_get_tuning", _i, [vp, P(Tuning)])
@@ -226,7 +229,7 @@ EXPORTS = (
     "zsph_sort_index", "zsph_sort_data", "zsph_boundary", "zsph_governing", "zsph_prediction",
     "zsph_correction", "zsph_step", "zsph_array_get", "zsph_array_set", "zsph_neighbor_stats",
     "zsph_killed", "zsph_sizeof", "zsph_time_steps", "zsph_profile_steps", "zsph_diagnostics",
-    "zsph_probe_pressure", "zsph_probe_pressure_noalloc", "zsph_rigid_mass_properties", "zsph_rigid_acc_alpha", "zsph_rigid_euler_step",
+    "zsph_probe_pressure", "zsph_probe_pressure_noalloc", "zsph_download_async", "zsph_frame_wait", "zsph_rigid_mass_properties", "zsph_rigid_acc_alpha", "zsph_rigid_euler_step",
     "zsph_set_tuning", "zsph_get_tuning", "zsph_autotune", "zsph_set_body", "zsph_get_body",
     "zsph_set_piston", "zsph_set_time", "zsph_get_time", "zsph_body_reduce", "zsph_case_wavetank",
     "zsph_create_batch", "zsph_scene_diagnostics", "zsph_mlp_create", "zsph_mlp_destroy", "zsph_mlp_last_error",

@@ -2,7 +2,10 @@
 // the C ABI.  Reads a case file (io/read_json.cu keys) and its
 // <input_path>/input.vtk, or generates a dam break; steps on the GPU and
 // hands output frames to a pool of writer threads (io/mul_thread.cu:3-80),
-// here a blocking queue that is drained and joined before exit.
+// here a blocking queue that is drained and joined before exit.  Output
+// steps do not stall the solver: zsph_download_async snapshots the state on
+// the device and copies it to pinned host memory on its own stream while
+// the next steps run; a writer waits for the copy and unpacks it.
 // Checkpoints: --checkpoint-step K writes <output_path>/zsph_ckpt<s>.bin
 // after every K-th step (s = steps done); --restart FILE takes the state of
 // such a file and continues the loop at its step (the start_step /
@@ -31,6 +34,15 @@ struct Frame {
     int n = 0;
     std::vector<float> pos, vel, rhop, acc;
     std::vector<int> type, table, hash;
+    zsph_frame *pending = nullptr;  // an asynchronous download still to be waited for (the writer does it)
+
+    void alloc() {
+        pos.resize(3 * (size_t)n); vel.resize(3 * (size_t)n); rhop.resize(2 * (size_t)n);
+        acc.resize(4 * (size_t)n); type.resize(n); table.resize(n); hash.resize(n);
+    }
+    zsph_host_state state() {
+        return zsph_host_state{pos.data(), vel.data(), rhop.data(), acc.data(), type.data(), table.data(), hash.data()};
+    }
 };
 
 class Writers {
@@ -69,8 +81,16 @@ class Writers {
                 q_.pop_front();
             }
             room_.notify_one();
-            zsph_host_state st{f.pos.data(), f.vel.data(), f.rhop.data(), f.acc.data(),
-                               f.type.data(), f.table.data(), f.hash.data()};
+            if (f.pending) {  // unpack the asynchronous download here, off the stepping thread
+                f.alloc();
+                zsph_host_state st = f.state();
+                if (zsph_frame_wait(f.pending, &st) != ZSPH_OK) {
+                    std::lock_guard<std::mutex> g(m_);
+                    ++errors_;
+                    continue;
+                }
+            }
+            zsph_host_state st = f.state();
             const std::string path = dir_ + "/zsph" + std::to_string(f.step) + ".vtk";
             if (zsph_write_vtk(path.c_str(), &st, f.n, fields_, binary_) != ZSPH_OK) {
                 std::lock_guard<std::mutex> g(m_);
@@ -90,9 +110,9 @@ class Writers {
 
 int usage() {
     std::fprintf(stderr,
-                 "usage: zsph_run case.json [--checkpoint-step K] [--restart FILE]\n"
-                 "       zsph_run --dambreak {2,3} [--dx DX] [--steps N] [--output-step K] [--out DIR] [--gamma G]\n"
-                 "                [--checkpoint-step K] [--restart FILE]\n");
+                 "usage: zsph_run case.json [--checkpoint-step K] [--restart FILE] [--sync-output 1] [--writers N] [--walk W]\n"
+                 "       zsph_run --dambreak {2,3} [--dx DX] [--steps N] [--output-step K] [--out DIR] [--gamma G] [--gsf F]\n"
+                 "                [--checkpoint-step K] [--restart FILE] [--sync-output 1] [--writers N] [--walk W]\n");
     return 2;
 }
 
@@ -106,11 +126,17 @@ int main(int argc, char **argv) {
     std::vector<int> type, table;
     int n = 0;
     int ckpt_every = 0;
+    bool sync_output = false;
+    int writers = 0;  // writer threads (0: the case's thread_num)
+    int walk = -1;    // neighbour walk (zsph_tuning.walk; -1: the library default)
     std::string restart;
     // options common to both forms; true if argv[a] was one
     auto common = [&](const std::string &k, const char *v) {
         if (k == "--checkpoint-step") ckpt_every = std::atoi(v);
         else if (k == "--restart") restart = v;
+        else if (k == "--sync-output") sync_output = std::atoi(v) != 0;
+        else if (k == "--writers") writers = std::atoi(v);
+        else if (k == "--walk") walk = std::atoi(v);
         else return false;
         return true;
     };
@@ -126,6 +152,7 @@ int main(int argc, char **argv) {
         }
         j.cfl = 0.2f;
         j.gamma = 7.0f;
+        j.grid_size_factor = 2.0f;  // cells of edge h, as the benchmark runs
         j.end_step = 100;
         j.output_step = 50;
         for (int a = 3; a + 1 < argc; a += 2) {
@@ -136,6 +163,7 @@ int main(int argc, char **argv) {
             else if (k == "--output-step") j.output_step = std::atoi(v);
             else if (k == "--out") std::snprintf(j.output_path, sizeof(j.output_path), "%s", v);
             else if (k == "--gamma") j.gamma = (float)std::atof(v);
+            else if (k == "--gsf") j.grid_size_factor = (float)std::atof(v);
             else if (!common(k, v)) return usage();
         }
         if ((argc - 3) % 2) return usage();
@@ -182,6 +210,16 @@ int main(int argc, char **argv) {
         return 1;
     }
     zsph_host_state st{pos.data(), vel.data(), rhop.data(), acc.data(), type.data(), table.data(), nullptr};
+    if (walk >= 0) {
+        zsph_tuning t;
+        zsph_get_tuning(ctx, &t);
+        t.walk = walk;
+        if (zsph_set_tuning(ctx, &t) != ZSPH_OK) {
+            std::fprintf(stderr, "%s\n", zsph_last_error(ctx));
+            zsph_destroy(ctx);
+            return 1;
+        }
+    }
     int first = cp.start_step;
     if (restart.empty() ? zsph_upload(ctx, 0, &st) != ZSPH_OK : zsph_checkpoint_read(ctx, restart.c_str(), nullptr) != ZSPH_OK) {
         std::fprintf(stderr, "%s\n", zsph_last_error(ctx));
@@ -198,8 +236,11 @@ int main(int argc, char **argv) {
         std::printf("* restart from %s at step %d (t = %.6g s)\n", restart.c_str(), first, info.time);
     }
     int status = 0;
+    double loop_s = 0.0;
+    int loop_steps = 0;
+    auto t_drain = std::chrono::steady_clock::now();
     {
-        Writers w(cp.thread_num, cp.output_path, cp.output_fields | ZSPH_F_POS, cp.binary);
+        Writers w(writers > 0 ? writers : cp.thread_num, cp.output_path, cp.output_fields | ZSPH_F_POS, cp.binary);
         auto t0 = std::chrono::steady_clock::now();
         for (int step = first; step < cp.end_step; ++step) {
             if (zsph_step(ctx, 1) != ZSPH_OK) {
@@ -211,11 +252,12 @@ int main(int argc, char **argv) {
                 Frame f;
                 f.step = step;
                 f.n = n;
-                f.pos.resize(3 * (size_t)n); f.vel.resize(3 * (size_t)n); f.rhop.resize(2 * (size_t)n);
-                f.acc.resize(4 * (size_t)n); f.type.resize(n); f.table.resize(n); f.hash.resize(n);
-                zsph_host_state fs{f.pos.data(), f.vel.data(), f.rhop.data(), f.acc.data(),
-                                   f.type.data(), f.table.data(), f.hash.data()};
-                if (zsph_download(ctx, 0, &fs, ZSPH_F_ALL) != ZSPH_OK) {
+                // asynchronous: a device snapshot now, the PCIe copy overlapping the next
+                // steps, unpacked by the writer; --sync-output 1: the blocking download
+                const int rc = sync_output ? (f.alloc(), [&] { zsph_host_state fs = f.state();
+                                                             return zsph_download(ctx, 0, &fs, ZSPH_F_ALL); }())
+                                           : zsph_download_async(ctx, 0, ZSPH_F_ALL, &f.pending);
+                if (rc != ZSPH_OK) {
                     std::fprintf(stderr, "download: %s\n", zsph_last_error(ctx));
                     status = 1;
                     break;
@@ -239,8 +281,14 @@ int main(int argc, char **argv) {
             std::fprintf(stderr, "%s\n", zsph_last_error(ctx));
             status = 1;
         }
+        loop_s = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        loop_steps = cp.end_step - first;
+        t_drain = std::chrono::steady_clock::now();
         if (w.errors()) status = 1;
-    }
+    }  // the writers drain their queue and join here
+    const double drain_s = std::chrono::duration<double>(std::chrono::steady_clock::now() - t_drain).count();
+    std::printf("* stepping loop %.3f s for %d steps (%.4f ms/step); writers drained %.3f s later\n", loop_s,
+                loop_steps, loop_steps > 0 ? 1e3 * loop_s / loop_steps : 0.0, drain_s);
     int killed = 0;
     zsph_killed(ctx, &killed);
     std::printf("* done, removed particles: %d\n", killed);

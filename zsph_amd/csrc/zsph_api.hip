@@ -3,6 +3,8 @@
 // graph-captured full step.  Host code only; kernels live in kernels/.
 #include <hip/hip_runtime.h>
 
+#include <atomic>
+#include <condition_variable>
 #include <cstdarg>
 #include <cstdio>
 #include <algorithm>
@@ -17,6 +19,19 @@
 #include "kernels/common.hpp"
 #include "kernels/launch.hpp"
 #include "zsph.h"
+
+// One in-flight asynchronous download (zsph_download_async): a device
+// snapshot of the requested arrays and its pinned host copy.
+struct zsph_frame {
+    zsph_ctx *ctx = nullptr;
+    void *dev = nullptr;                     // snapshot: posr, velp, acc (16 B), type, table, key (4 B) per particle
+    void *host = nullptr;                    // pinned copy of the same layout
+    hipEvent_t snap_done = nullptr, copy_done = nullptr;
+    int fields = 0, n = 0;
+    uint32_t cell_mask = 0;                  // the cell bits of a sort key (ZSPH_F_HASH)
+    std::atomic<int> busy{0};
+    unsigned long long seq = 0;              // issue order (the oldest is waited for first)
+};
 
 struct zsph_ctx {
     zsph_params p{};
@@ -67,6 +82,11 @@ struct zsph_ctx {
     bool use_graph = true;        // zsph_step replays a captured graph (else launches directly)
     std::vector<unsigned char> closure_net;  // the zsph_closure last set (checkpoints carry it)
     float *probe_scratch = nullptr;  // this device's probe_scratch (zsph_probe_pressure_noalloc)
+    hipStream_t copy_stream = nullptr;  // device-to-host copies of asynchronous downloads
+    zsph_frame frames[ZSPH_FRAME_SLOTS];
+    unsigned long long frame_seq = 0;
+    std::mutex frame_mu;
+    std::condition_variable frame_cv;
     std::string err;
     std::vector<void *> allocs;
 };
@@ -429,6 +449,15 @@ int zsph_create_batch(const zsph_params *scenes, int nscenes, int device, zsph_c
 void zsph_destroy(zsph_ctx *c) {
     if (!c) return;
     if (c->stream) (void)hipStreamSynchronize(c->stream);
+    if (c->copy_stream) {
+        (void)hipStreamSynchronize(c->copy_stream);
+        (void)hipStreamDestroy(c->copy_stream);
+    }
+    for (zsph_frame &f : c->frames) {
+        if (f.snap_done) (void)hipEventDestroy(f.snap_done);
+        if (f.copy_done) (void)hipEventDestroy(f.copy_done);
+        if (f.host) (void)hipHostFree(f.host);  // f.dev is one of c->allocs
+    }
     if (c->graph_exec) (void)hipGraphExecDestroy(c->graph_exec);
     if (c->graph) (void)hipGraphDestroy(c->graph);
     for (void *d : c->allocs) (void)hipFree(d);
@@ -536,6 +565,131 @@ int zsph_download(zsph_ctx *c, int b, zsph_host_state *st, int fields) {
         for (int i = 0; i < N; ++i) st->hash[i] &= mask;
     }
     return ZSPH_OK;
+}
+
+namespace {
+constexpr size_t kFrameBytesPerParticle = 3 * 16 + 3 * 4;  // posr, velp, acc; type, table, key
+
+// Allocates the copy stream and the frame slots (snapshot + pinned copy) at first use.
+int frames_init(zsph_ctx *c) {
+    if (c->copy_stream) return ZSPH_OK;
+    const size_t bytes = (size_t)c->q.N * kFrameBytesPerParticle;
+    for (int k = 0; k < ZSPH_FRAME_SLOTS; ++k) {
+        zsph_frame &f = c->frames[k];
+        f.ctx = c;
+        char *d = nullptr;
+        if (int r = dalloc(c, &d, bytes)) return r;
+        f.dev = d;
+        if (hipHostMalloc(&f.host, bytes, hipHostMallocDefault) != hipSuccess) {
+            f.host = nullptr;
+            return set_err(c, ZSPH_E_NOMEM, "zsph_download_async: %zu bytes of pinned host memory", bytes);
+        }
+        HIPCHK(c, hipEventCreateWithFlags(&f.snap_done, hipEventDisableTiming));
+        HIPCHK(c, hipEventCreateWithFlags(&f.copy_done, hipEventDisableTiming));
+    }
+    HIPCHK(c, hipStreamCreateWithFlags(&c->copy_stream, hipStreamNonBlocking));
+    return ZSPH_OK;
+}
+
+// Byte offsets of the arrays inside a frame's snapshot.
+struct FrameLayout {
+    size_t posr, velp, acc, type, table, key;
+    explicit FrameLayout(size_t n)
+        : posr(0), velp(16 * n), acc(32 * n), type(48 * n), table(52 * n), key(56 * n) {}
+};
+}  // namespace
+
+int zsph_download_async(zsph_ctx *c, int b, int fields, zsph_frame **out) {
+    if (!c || !out || !valid_buf(b)) return set_err(c, ZSPH_E_ARG, "zsph_download_async: bad argument");
+    *out = nullptr;
+    if (int r = frames_init(c)) return r;
+    zsph_frame *f = nullptr;
+    {  // a free slot, else wait for a writer to release one
+        std::unique_lock<std::mutex> lock(c->frame_mu);
+        for (;;) {
+            for (zsph_frame &x : c->frames)
+                if (!x.busy.load() && (!f || x.seq < f->seq)) f = &x;
+            if (f) break;
+            c->frame_cv.wait(lock);
+        }
+        f->busy.store(1);
+        f->seq = ++c->frame_seq;
+    }
+    const size_t N = (size_t)c->q.N;
+    const FrameLayout L(N);
+    const zs::Buf &d = c->buf[b];
+    char *dev = static_cast<char *>(f->dev), *host = static_cast<char *>(f->host);
+    f->fields = fields;
+    f->n = (int)N;
+    f->cell_mask = c->q.flag - 1u;
+    struct Part { bool on; const void *src; size_t off, bytes; };
+    const Part parts[6] = {
+        {(fields & (ZSPH_F_POS | ZSPH_F_RHOP)) != 0, d.posr, L.posr, 16 * N},
+        {(fields & (ZSPH_F_VEL | ZSPH_F_RHOP)) != 0, d.velp, L.velp, 16 * N},
+        {(fields & ZSPH_F_ACC) != 0, c->acc, L.acc, 16 * N},
+        {(fields & ZSPH_F_TYPE) != 0, d.type, L.type, 4 * N},
+        {(fields & ZSPH_F_TABLE) != 0, d.table, L.table, 4 * N},
+        {(fields & ZSPH_F_HASH) != 0, c->key[c->sorted_slot >= 0 ? c->sorted_slot : 0], L.key, 4 * N}};
+    hipError_t e = hipSuccess;
+    // snapshot in stream order after the issued steps, then the host copy on the copy stream
+    for (const Part &p : parts)
+        if (p.on && e == hipSuccess) e = hipMemcpyAsync(dev + p.off, p.src, p.bytes, hipMemcpyDeviceToDevice, c->stream);
+    if (e == hipSuccess) e = hipEventRecord(f->snap_done, c->stream);
+    if (e == hipSuccess) e = hipStreamWaitEvent(c->copy_stream, f->snap_done, 0);
+    for (const Part &p : parts)
+        if (p.on && e == hipSuccess)
+            e = hipMemcpyAsync(host + p.off, dev + p.off, p.bytes, hipMemcpyDeviceToHost, c->copy_stream);
+    if (e == hipSuccess) e = hipEventRecord(f->copy_done, c->copy_stream);
+    if (e != hipSuccess) {
+        (void)hipStreamSynchronize(c->copy_stream);
+        {
+            std::lock_guard<std::mutex> lock(c->frame_mu);
+            f->busy.store(0);
+        }
+        c->frame_cv.notify_all();
+        return set_err(c, ZSPH_E_HIP, "zsph_download_async: %s", hipGetErrorString(e));
+    }
+    *out = f;
+    return ZSPH_OK;
+}
+
+int zsph_frame_wait(zsph_frame *f, zsph_host_state *st) {
+    if (!f || !f->ctx || !f->busy.load() || !st) return ZSPH_E_ARG;
+    zsph_ctx *c = f->ctx;
+    const hipError_t e = hipEventSynchronize(f->copy_done);
+    if (e == hipSuccess) {
+        const size_t N = (size_t)f->n;
+        const FrameLayout L(N);
+        const char *host = static_cast<const char *>(f->host);
+        const float4 *posr = reinterpret_cast<const float4 *>(host + L.posr);
+        const float4 *velp = reinterpret_cast<const float4 *>(host + L.velp);
+        const int fields = f->fields;
+        for (size_t i = 0; i < N; ++i) {  // the same unpacking as zsph_download
+            if ((fields & ZSPH_F_POS) && st->pos) {
+                st->pos[3 * i] = posr[i].x; st->pos[3 * i + 1] = posr[i].y; st->pos[3 * i + 2] = posr[i].z;
+            }
+            if ((fields & ZSPH_F_VEL) && st->vel) {
+                st->vel[3 * i] = velp[i].x; st->vel[3 * i + 1] = velp[i].y; st->vel[3 * i + 2] = velp[i].z;
+            }
+            if ((fields & ZSPH_F_RHOP) && st->rhop) {
+                st->rhop[2 * i] = posr[i].w;
+                st->rhop[2 * i + 1] = velp[i].w;
+            }
+        }
+        if ((fields & ZSPH_F_ACC) && st->acc_drhodt) std::memcpy(st->acc_drhodt, host + L.acc, 16 * N);
+        if ((fields & ZSPH_F_TYPE) && st->type) std::memcpy(st->type, host + L.type, 4 * N);
+        if ((fields & ZSPH_F_TABLE) && st->table) std::memcpy(st->table, host + L.table, 4 * N);
+        if ((fields & ZSPH_F_HASH) && st->hash) {
+            const uint32_t *key = reinterpret_cast<const uint32_t *>(host + L.key);
+            for (size_t i = 0; i < N; ++i) st->hash[i] = (int)(key[i] & f->cell_mask);
+        }
+    }
+    {
+        std::lock_guard<std::mutex> lock(c->frame_mu);
+        f->busy.store(0);
+    }
+    c->frame_cv.notify_all();
+    return e == hipSuccess ? ZSPH_OK : ZSPH_E_HIP;
 }
 
 int zsph_hash(zsph_ctx *c, int b) {
