@@ -192,6 +192,8 @@ def main():
     ap.add_argument("--no-autotune", dest="autotune", action="store_false")
     ap.add_argument("--walk", type=int, default=None, help="force a neighbour walk (0 mask, 1 hit masks, 2 lists)")
     ap.add_argument("--radix-bits", type=int, default=None, help="force the sort's digit width (8 to 11 bits)")
+    ap.add_argument("--block", type=int, default=None, help="force the neighbour passes' block size (128/256/512)")
+    ap.add_argument("--sort-tile", type=int, default=None, help="force the radix tile's keys per thread (8/16)")
     ap.add_argument("--closure", default=None, metavar="NPZ",
                     help="run every step with this learned closure network (one-scene cases)")
     ap.add_argument("--tune-at", type=int, default=None,
@@ -269,10 +271,11 @@ def main():
     if setup:
         setup(s)
     # pick the neighbour-walk variant and sort digit width on this case
-    if args.walk is not None or args.radix_bits is not None:
+    if any(v is not None for v in (args.walk, args.radix_bits, args.block, args.sort_tile)):
         t = s.get_tuning()
         s.set_tuning(t["walk"] if args.walk is None else args.walk,
-                     t["radix_bits"] if args.radix_bits is None else args.radix_bits)
+                     t["radix_bits"] if args.radix_bits is None else args.radix_bits,
+                     args.block, args.sort_tile)
         args.autotune = False
     import tempfile
     with tempfile.TemporaryDirectory() as tmp:

@@ -540,14 +540,22 @@ int zsph_filter_fields(zsph_ctx *ctx, int buf, float radius, float hc, const int
                            * order) that the governing pass reuses; takes
                            * 512 B of device memory per particle          */
 typedef struct zsph_tuning {
-    int walk;        /* ZSPH_WALK_*                         */
-    int radix_bits;  /* sort digit width, 8..11             */
+    int walk;        /* ZSPH_WALK_*                                          */
+    int radix_bits;  /* sort digit width, 8..11                              */
+    int block;       /* threads per block of the three neighbour passes:
+                      * 128, 256 or 512 (0: 256; the closure's feature and
+                      * scaled passes always run 256)                        */
+    int sort_tile;   /* keys per thread of the radix tile, 8 or 16 (0: 16);
+                      * the scatter stages the whole tile in LDS (8 B per
+                      * key: 16 or 32 KB)                                    */
 } zsph_tuning;
 int zsph_set_tuning(zsph_ctx *ctx, const zsph_tuning *t);
 int zsph_get_tuning(zsph_ctx *ctx, zsph_tuning *t);
-/* Times every tuning candidate over `steps` graph-replayed steps starting
- * from a saved copy of buffer 0, restores that state, and keeps the
- * fastest; *best and *best_ms (may be NULL) receive it. */
+/* Times tuning candidates over `steps` graph-replayed steps each, every one
+ * starting from a saved copy of buffer 0, restores that state, and keeps the
+ * fastest; *best and *best_ms (may be NULL) receive it.  The search is by
+ * coordinates: walk x digit width first (at the current block and tile),
+ * then the block size, then the sort tile, each from the best so far. */
 int zsph_autotune(zsph_ctx *ctx, int steps, zsph_tuning *best, float *best_ms);
 
 /* zsph_step launches each step's kernels directly instead of replaying the

@@ -129,10 +129,11 @@ def test_mixed_diffusion_batch_matches_standalone_scenes(walk):
 def test_batch_graph_replay_is_deterministic_and_tuning_independent():
     b = uneven_batch(6, seed=3)
     runs = []
-    for walk in (z.WALK_MASK, z.WALK_MASK, z.WALK_HITS, z.WALK_LIST):
+    for walk, block, tile in ((z.WALK_MASK, 256, 16), (z.WALK_MASK, 256, 16), (z.WALK_HITS, 512, 8),
+                              (z.WALK_LIST, 128, 16)):
         s = z.Solver(b)
         s.upload(b.host)
-        s.set_tuning(walk, 9)
+        s.set_tuning(walk, 9, block, tile)
         s.step(5)
         s.sync()
         runs.append((s.get(A_POSR), s.get(A_VELP)))

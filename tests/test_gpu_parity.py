@@ -282,26 +282,30 @@ def test_results_do_not_depend_on_tuning(name):
         # candidates the hit-mask record covers, so its fallback to testing runs here too
         assert c.params.grid_factor == 1 and max_row_window(c) > 64
     ref = None
-    for walk in (z._abi.WALK_MASK, z._abi.WALK_HITS, z._abi.WALK_LIST):
-        for bits in (8, 11):
-            s = z.Solver(c)
-            s.set_tuning(walk, bits)
-            s.upload(c.host)
-            s.step(3)
-            s.sync()
-            got = (s.get(A_POSR, 0), s.get(A_VELP, 0), s.get(A_ACC), s.get(A_TABLE, 0))
-            s.close()
-            if ref is None:
-                ref = got
-            else:
-                for a, b in zip(ref, got):
-                    assert np.array_equal(a, b), (walk, bits)
+    # every walk x digit width at the default block and tile, then the other block sizes
+    # and the smaller sort tile (each particle's arithmetic does not depend on its block)
+    tunings = [(w, bits, 256, 16) for w in (z._abi.WALK_MASK, z._abi.WALK_HITS, z._abi.WALK_LIST) for bits in (8, 11)]
+    tunings += [(z._abi.WALK_HITS, 8, 128, 16), (z._abi.WALK_MASK, 9, 512, 8), (z._abi.WALK_LIST, 10, 128, 8)]
+    for walk, bits, block, tile in tunings:
+        s = z.Solver(c)
+        s.set_tuning(walk, bits, block, tile)
+        s.upload(c.host)
+        s.step(3)
+        s.sync()
+        got = (s.get(A_POSR, 0), s.get(A_VELP, 0), s.get(A_ACC), s.get(A_TABLE, 0))
+        s.close()
+        if ref is None:
+            ref = got
+        else:
+            for a, b in zip(ref, got):
+                assert np.array_equal(a, b), (walk, bits, block, tile)
     # the autotuner restores the state it timed from
     s = z.Solver(c)
     s.upload(c.host)
     before = s.get(A_POSR, 0)
     t = s.autotune(steps=2)
     assert t["walk"] in (z._abi.WALK_MASK, z._abi.WALK_HITS, z._abi.WALK_LIST) and 8 <= t["radix_bits"] <= 11
+    assert t["block"] in (128, 256, 512) and t["sort_tile"] in (8, 16)
     assert np.array_equal(s.get(A_POSR, 0), before) and s.killed() == 0
 
 

@@ -332,21 +332,26 @@ class Solver:
                   "zsph_neighbor_stats")
         return cnt, dig
 
-    def set_tuning(self, walk, radix_bits):
-        t = abi.Tuning(walk, radix_bits)
+    def set_tuning(self, walk, radix_bits, block=None, sort_tile=None):
+        """Neighbour walk, sort digit width, neighbour-pass block size and sort tile
+        (None keeps the current block size / tile)."""
+        cur = self.get_tuning()
+        t = abi.Tuning(walk, radix_bits, cur["block"] if block is None else block,
+                       cur["sort_tile"] if sort_tile is None else sort_tile)
         self._chk(lib().zsph_set_tuning(self._ctx, C.byref(t)), "zsph_set_tuning")
 
     def get_tuning(self):
         t = abi.Tuning()
         self._chk(lib().zsph_get_tuning(self._ctx, C.byref(t)), "zsph_get_tuning")
-        return {"walk": t.walk, "radix_bits": t.radix_bits}
+        return {"walk": t.walk, "radix_bits": t.radix_bits, "block": t.block, "sort_tile": t.sort_tile}
 
     def autotune(self, steps=5):
-        """Time every tuning candidate from the current buffer-0 state (restored afterwards); keep the fastest."""
+        """Time tuning candidates from the current buffer-0 state (restored afterwards); keep the fastest."""
         t = abi.Tuning()
         ms = C.c_float(0.0)
         self._chk(lib().zsph_autotune(self._ctx, steps, C.byref(t), C.byref(ms)), "zsph_autotune")
-        return {"walk": t.walk, "radix_bits": t.radix_bits, "ms_per_step": ms.value / steps}
+        return {"walk": t.walk, "radix_bits": t.radix_bits, "block": t.block, "sort_tile": t.sort_tile,
+                "ms_per_step": ms.value / steps}
 
     def time_steps(self, nsteps):
         """Milliseconds of `nsteps` graph-replayed steps, between two HIP events on the solver's stream."""

@@ -89,7 +89,7 @@ class Rigid(C.Structure):
 
 
 class Tuning(C.Structure):
-    _fields_ = [("walk", _i), ("radix_bits", _i)]
+    _fields_ = [("walk", _i), ("radix_bits", _i), ("block", _i), ("sort_tile", _i)]
 
 
 WALK_MASK, WALK_HITS, WALK_LIST = 0, 1, 2

@@ -3,6 +3,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <type_traits>
+
 #include "zsph.h"
 
 namespace zs {
@@ -116,6 +118,18 @@ __device__ __forceinline__ int xcd_block(int nblocks) {
 }
 // Blocks of ZS_BLOCK particles that cover n particles.
 __device__ __forceinline__ int blocks_of(int n) { return (n + ZS_BLOCK - 1) / ZS_BLOCK; }
+// Blocks of BS particles that cover n particles (the neighbour passes take their
+// block size as a tuning parameter: 128, 256 or 512 threads).
+template <int BS>
+__device__ __forceinline__ int blocks_of_bs(int n) { return (n + BS - 1) / BS; }
+// Runs f(std::integral_constant<int, BS>) for the tuned block size bs of the
+// neighbour passes (any other value runs ZS_BLOCK).
+template <class F>
+inline void with_block(int bs, F &&f) {
+    if (bs == 128) f(std::integral_constant<int, 128>{});
+    else if (bs == 512) f(std::integral_constant<int, 512>{});
+    else f(std::integral_constant<int, ZS_BLOCK>{});
+}
 
 // Cell coordinate along one axis: floor of the FP32 quotient (x - o) / size
 // rounded to nearest; returns false outside [0, dim).  NaN positions fail.

@@ -11,13 +11,15 @@ namespace zs {
 // sp: per-scene parameter table of a batch (device memory), nullptr for one
 // scene; b.scene must then hold each particle's scene.
 void launch_hash(hipStream_t s, const Prm &q, const Prm *sp, const Buf &b, uint32_t *key, int *idx, int *killed);
-int radix_blocks(int n);
+int radix_blocks(int n, int items);
+// Histogram buffer words for any digit width up to `bits` and any tile.
 size_t radix_hist_words(int n, int bits);
 // Sorts (key[0], val[0]) over the low `keybits` bits; returns the slot (0/1)
-// holding the result, -1 for an unsupported digit width.  identity_vals:
-// val[0] is known to be 0..n-1 and is not read.
-int launch_sort(hipStream_t s, int n, int keybits, int bits, uint32_t *key[2], int *val[2], uint32_t *hist,
-                bool identity_vals);
+// holding the result, -1 for an unsupported digit width (8..11) or tile
+// (items = keys per thread, 8 or 16).  identity_vals: val[0] is known to be
+// 0..n-1 and is not read.
+int launch_sort(hipStream_t s, int n, int keybits, int bits, int items, uint32_t *key[2], int *val[2],
+                uint32_t *hist, bool identity_vals);
 // cs[0..G]: fluid cell starts; cs[G+1..2G+1]: dummy cell starts (key flag + c).
 // scratch: cell_start_scratch(G) ints (tile bounds).  n >= 1.
 size_t cell_start_scratch(int G);
@@ -30,17 +32,19 @@ void launch_reorder(hipStream_t s, int n, const int *perm, const Buf &a, const B
 // same with the chunk hit masks (nlist: hits_slots(k) x N words; ncount:
 // overflow flags).  A caller without a current record passes WALK_MASK to
 // launch_governing.
-void launch_wall(hipStream_t s, int walk, const Prm &q, const Prm *sp, const Buf &b, const uint32_t *skey,
+// block: threads per block of the neighbour passes (128, 256 or 512; the
+// closure's feature and scaled variants always run 256).
+void launch_wall(hipStream_t s, int walk, int block, const Prm &q, const Prm *sp, const Buf &b, const uint32_t *skey,
                  const int *cs);
 // feat (one scene only; may be nullptr): the closure features, ZSPH_NF floats per particle.
-void launch_ldg(hipStream_t s, int walk, const Prm &q, const Prm *sp, const Buf &b, const uint32_t *skey,
+void launch_ldg(hipStream_t s, int walk, int block, const Prm &q, const Prm *sp, const Buf &b, const uint32_t *skey,
                 const int *cs, float4 *gd, int *nlist, int *ncount, float *feat);
 // mode (one scene only): 0 plain, 1 closure-scaled (scale: 2 floats per
 // particle), 2 plain + term decomposition (terms: 8 floats per particle).
 // clock (may be nullptr): device stamps around the continuity+momentum kernel (launch_clock).
 // moving: the context has moving dummies (a body or a piston), whose
 // pressure force k_governing_dummy computes; fixed walls get zero either way.
-void launch_governing(hipStream_t s, int walk, const Prm &q, const Prm *sp, const Buf &b, const uint32_t *skey,
+void launch_governing(hipStream_t s, int walk, int block, const Prm &q, const Prm *sp, const Buf &b, const uint32_t *skey,
                       const int *cs, const float4 *gd, float4 *acc, const int *nlist, const int *ncount, int mode,
                       const float2 *scale, float *terms, unsigned long long *clock, bool moving);
 void launch_predict(hipStream_t s, const Prm &q, const Prm *sp, const Buf &b, const int *cs, const float4 *acc);

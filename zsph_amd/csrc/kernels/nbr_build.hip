@@ -50,13 +50,14 @@ __global__ __launch_bounds__(ZS_BLOCK) void k_hash(Prm q0, const Prm *__restrict
 }
 
 // ---------------------------------------------------------------------------
-// Radix sort.  Tile = 256 threads x RS_ITEMS keys; wave w of a block owns the
-// contiguous keys [tile + w*64*RS_ITEMS, +64*RS_ITEMS) and walks them 64 at a
-// time, so (wave, item, lane) order equals input order and the scatter is
-// stable.  Equal digits inside a wave are found with BITS ballots.
+// Radix sort.  Tile = 256 threads x ITEMS keys (ITEMS = 8 or 16, a tuning
+// parameter: the scatter stages its whole tile in LDS, 8 bytes per key);
+// wave w of a block owns the contiguous keys [tile + w*64*ITEMS, +64*ITEMS)
+// and walks them 64 at a time, so (wave, item, lane) order equals input order
+// and the scatter is stable.  Equal digits inside a wave are found with BITS
+// ballots.
 // ---------------------------------------------------------------------------
-constexpr int RS_ITEMS = 16;
-constexpr int RS_TILE = ZS_BLOCK * RS_ITEMS;
+constexpr int RS_ITEMS_MIN = 8;   // smallest tile: sizes the histogram buffer
 constexpr int RS_WAVES = ZS_BLOCK / 64;
 
 __device__ __forceinline__ uint32_t lane_rank(uint64_t m) {
@@ -76,9 +77,10 @@ __device__ __forceinline__ uint64_t match_digit(uint32_t d, uint64_t valid) {
     return m;
 }
 
-template <int BITS>
+template <int BITS, int RS_ITEMS>
 __global__ __launch_bounds__(ZS_BLOCK) void k_radix_hist(const uint32_t *__restrict__ kin, int n, int shift,
                                                          uint32_t *__restrict__ hist, int nblk) {
+    constexpr int RS_TILE = ZS_BLOCK * RS_ITEMS;
     constexpr int R = 1 << BITS;
     __shared__ uint32_t h[R];
     for (int d = threadIdx.x; d < R; d += ZS_BLOCK) h[d] = 0;
@@ -124,13 +126,14 @@ __global__ __launch_bounds__(ZS_BLOCK) void k_radix_rowscan(uint32_t *__restrict
     if (t == ZS_BLOCK - 1) total[blockIdx.x] = part[t];
 }
 
-template <int BITS>
+template <int BITS, int RS_ITEMS>
 __global__ __launch_bounds__(ZS_BLOCK) void k_radix_scatter(const uint32_t *__restrict__ kin,
                                                             const int *__restrict__ vin,
                                                             uint32_t *__restrict__ kout,
                                                             int *__restrict__ vout, int n, int shift,
                                                             const uint32_t *__restrict__ scanned, int nblk,
                                                             const uint32_t *__restrict__ total) {
+    constexpr int RS_TILE = ZS_BLOCK * RS_ITEMS;
     constexpr int R = 1 << BITS;
     constexpr int PER = (R + ZS_BLOCK - 1) / ZS_BLOCK;
     __shared__ uint32_t cnt[RS_WAVES][R];
@@ -379,37 +382,47 @@ void launch_hash(hipStream_t s, const Prm &q, const Prm *sp, const Buf &b, uint3
         k_hash<false><<<nblocks(q.N), ZS_BLOCK, 0, s>>>(q, sp, b, key, idx, killed);
 }
 
-int radix_blocks(int n) { return (n + RS_TILE - 1) / RS_TILE; }
+int radix_blocks(int n, int items) { return (n + ZS_BLOCK * items - 1) / (ZS_BLOCK * items); }
 
-size_t radix_hist_words(int n, int bits) { return ((size_t)1 << bits) * ((size_t)radix_blocks(n) + 1); }
+size_t radix_hist_words(int n, int bits) {
+    return ((size_t)1 << bits) * ((size_t)radix_blocks(n, RS_ITEMS_MIN) + 1);
+}
 
-template <int BITS>
+template <int BITS, int ITEMS>
 static int sort_passes(hipStream_t s, int n, int keybits, uint32_t *key[2], int *val[2], uint32_t *hist,
                        bool identity_vals) {
-    const int nblk = radix_blocks(n);
+    const int nblk = radix_blocks(n, ITEMS);
     const int R = 1 << BITS;
     uint32_t *total = hist + (size_t)R * nblk;
     int src = 0;
     for (int shift = 0; shift < keybits; shift += BITS) {
-        k_radix_hist<BITS><<<nblk, ZS_BLOCK, 0, s>>>(key[src], n, shift, hist, nblk);
+        k_radix_hist<BITS, ITEMS><<<nblk, ZS_BLOCK, 0, s>>>(key[src], n, shift, hist, nblk);
         k_radix_rowscan<<<R, ZS_BLOCK, 0, s>>>(hist, nblk, total);
-        k_radix_scatter<BITS><<<nblk, ZS_BLOCK, 0, s>>>(key[src], (identity_vals && shift == 0) ? nullptr : val[src],
+        k_radix_scatter<BITS, ITEMS><<<nblk, ZS_BLOCK, 0, s>>>(key[src], (identity_vals && shift == 0) ? nullptr : val[src],
                                                         key[1 - src], val[1 - src], n, shift, hist, nblk, total);
         src = 1 - src;
     }
     return src;
 }
 
-int launch_sort(hipStream_t s, int n, int keybits, int bits, uint32_t *key[2], int *val[2], uint32_t *hist,
-                bool identity_vals) {
-    if (n == 0) return 0;
+template <int ITEMS>
+static int sort_bits(hipStream_t s, int n, int keybits, int bits, uint32_t *key[2], int *val[2], uint32_t *hist,
+                     bool identity_vals) {
     switch (bits) {
-    case 8: return sort_passes<8>(s, n, keybits, key, val, hist, identity_vals);
-    case 9: return sort_passes<9>(s, n, keybits, key, val, hist, identity_vals);
-    case 10: return sort_passes<10>(s, n, keybits, key, val, hist, identity_vals);
-    case 11: return sort_passes<11>(s, n, keybits, key, val, hist, identity_vals);
+    case 8: return sort_passes<8, ITEMS>(s, n, keybits, key, val, hist, identity_vals);
+    case 9: return sort_passes<9, ITEMS>(s, n, keybits, key, val, hist, identity_vals);
+    case 10: return sort_passes<10, ITEMS>(s, n, keybits, key, val, hist, identity_vals);
+    case 11: return sort_passes<11, ITEMS>(s, n, keybits, key, val, hist, identity_vals);
     default: return -1;
     }
+}
+
+int launch_sort(hipStream_t s, int n, int keybits, int bits, int items, uint32_t *key[2], int *val[2],
+                uint32_t *hist, bool identity_vals) {
+    if (n == 0) return 0;
+    if (items == 8) return sort_bits<8>(s, n, keybits, bits, key, val, hist, identity_vals);
+    if (items == 16) return sort_bits<16>(s, n, keybits, bits, key, val, hist, identity_vals);
+    return -1;
 }
 
 static int cs_tiles(int G) { return (2 * G + 2 + CS_TILE - 1) / CS_TILE; }

@@ -44,14 +44,13 @@ __device__ __forceinline__ bool is_antuono(const Prm &q) {
 // rho_w from the inverse EOS (delta_ns.cu:81-85, :116-131; a_w = 0 there).
 // a_w is the dummy's prescribed acceleration, kept in tvelp.xyz (zero for
 // fixed walls).
-template <int WALK, bool BATCH>
-__global__ __launch_bounds__(ZS_BLOCK) void k_wall(Prm q0, const Prm *__restrict__ sp, Buf b,
-        const uint32_t *__restrict__ skey,
-                                                   const int *__restrict__ cs) {
+template <int WALK, bool BATCH, int BS>
+__global__ __launch_bounds__(BS) void k_wall(Prm q0, const Prm *__restrict__ sp, Buf b,
+                                             const uint32_t *__restrict__ skey, const int *__restrict__ cs) {
     const int nf = cs[q0.G];
     // dummy blocks round-robin over the XCDs: dummies far from the fluid
     // cost little, so contiguous ranges of them would load the XCDs unevenly
-    const int i = nf + blockIdx.x * ZS_BLOCK + threadIdx.x;
+    const int i = nf + blockIdx.x * BS + threadIdx.x;
     if (i >= q0.N) return;
     const Prm q = scene_prm<BATCH>(q0, sp, b, i);
     const float4 xi = b.posr[i];
@@ -81,13 +80,13 @@ __global__ __launch_bounds__(ZS_BLOCK) void k_wall(Prm q0, const Prm *__restrict
 // Compiled for at least 7 waves per SIMD (at most 72 VGPRs; the compiler's
 // own choice, 77, allows 6) in the single-scene mask and hit-mask walks:
 // measured 5 % faster.  8 waves (64 VGPRs) spill and measured slower.
-template <int WALK, bool BATCH, bool FEAT, int DF>
-__global__ __launch_bounds__(ZS_BLOCK, ((WALK != WALK_LIST && !BATCH && !FEAT) ? 7 : 1)) void k_ldg(Prm q0, const Prm *__restrict__ sp, Buf b,
+template <int WALK, bool BATCH, bool FEAT, int DF, int BS>
+__global__ __launch_bounds__(BS, ((WALK != WALK_LIST && !BATCH && !FEAT) ? 7 : 1)) void k_ldg(Prm q0, const Prm *__restrict__ sp, Buf b,
                                                   const uint32_t *__restrict__ skey, const int *__restrict__ cs,
                                                   float4 *__restrict__ gd, int *__restrict__ nlist,
                                                   int *__restrict__ ncount, float *__restrict__ feat) {
     const int nf = cs[q0.G];
-    const int i = xcd_block(blocks_of(nf)) * ZS_BLOCK + threadIdx.x;
+    const int i = xcd_block(blocks_of_bs<BS>(nf)) * BS + threadIdx.x;
     if (i >= q0.N) return;
     if (i >= nf) {
         gd[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
@@ -229,14 +228,14 @@ __global__ __launch_bounds__(ZS_BLOCK, ((WALK != WALK_LIST && !BATCH && !FEAT) ?
 // are written to terms[i * 8 ...] = (continuity, diffusion, pressure + g (3),
 // viscous (3)) for training.
 // At least 7 waves per SIMD in the single-scene plain walks (as k_ldg).
-template <int WALK, bool BATCH, int MODE, int DF>
-__global__ __launch_bounds__(ZS_BLOCK, ((WALK != WALK_LIST && !BATCH && MODE == 0) ? 7 : 1)) void k_governing(Prm q0, const Prm *__restrict__ sp, Buf b,
+template <int WALK, bool BATCH, int MODE, int DF, int BS>
+__global__ __launch_bounds__(BS, ((WALK != WALK_LIST && !BATCH && MODE == 0) ? 7 : 1)) void k_governing(Prm q0, const Prm *__restrict__ sp, Buf b,
                                                         const uint32_t *__restrict__ skey, const int *__restrict__ cs,
                                                         const float4 *__restrict__ gd, float4 *__restrict__ acc,
                                                         const int *__restrict__ nlist, const int *__restrict__ ncount,
                                                         const float2 *__restrict__ scale, float *__restrict__ terms) {
     const int nf = cs[q0.G];
-    const int i = xcd_block(blocks_of(nf)) * ZS_BLOCK + threadIdx.x;
+    const int i = xcd_block(blocks_of_bs<BS>(nf)) * BS + threadIdx.x;
     if (i >= nf) {
         // dummies and removed particles: 0 (k_governing_dummy, launched after
         // this kernel when there are moving dummies, overwrites theirs)
@@ -446,100 +445,128 @@ __global__ __launch_bounds__(ZS_BLOCK) void k_nbr_stats(Prm q0, const Prm *__res
     digest[i] = h;
 }
 
-// Walks other than the queue use the mask walk where no list applies; a
-// batch (sp != nullptr) runs the per-scene-parameter instantiation.
-// Kernels that only ever take the mask walk (the walk variants differ in the
-// gradient and continuity+momentum passes alone).
-#define ZS_WALK_LAUNCH(kern, grid, ...)                                                         \
-    do {                                                                                        \
-        (void)walk;                                                                             \
-        if (sp)                                                                                 \
-            kern<WALK_MASK, true><<<(grid), ZS_BLOCK, 0, s>>>(__VA_ARGS__);                     \
-        else                                                                                    \
-            kern<WALK_MASK, false><<<(grid), ZS_BLOCK, 0, s>>>(__VA_ARGS__);                    \
-    } while (0)
+// A batch (sp != nullptr) runs the per-scene-parameter instantiation.  The
+// wall pass and the moving-dummy pass only ever take the mask walk (the walk
+// variants differ in the gradient and continuity+momentum passes alone).
+static inline int nblocks_bs(int n, int bs) { return (n + bs - 1) / bs; }
 
-void launch_wall(hipStream_t s, int walk, const Prm &q, const Prm *sp, const Buf &b, const uint32_t *skey,
+void launch_wall(hipStream_t s, int walk, int block, const Prm &q, const Prm *sp, const Buf &b, const uint32_t *skey,
                  const int *cs) {
-    if (q.N) ZS_WALK_LAUNCH(k_wall, nblocks(q.N), q, sp, b, skey, cs);
+    (void)walk;
+    if (!q.N) return;
+    with_block(block, [&](auto BSc) {
+        constexpr int BS = decltype(BSc)::value;
+        if (sp)
+            k_wall<WALK_MASK, true, BS><<<nblocks_bs(q.N, BS), BS, 0, s>>>(q, sp, b, skey, cs);
+        else
+            k_wall<WALK_MASK, false, BS><<<nblocks_bs(q.N, BS), BS, 0, s>>>(q, sp, b, skey, cs);
+    });
 }
-template <int WALK, bool BATCH>
-static void ldg_walk(hipStream_t s, bool feat, int grid, const Prm &q, const Prm *sp, const Buf &b,
+template <int WALK, bool BATCH, int BS>
+static void ldg_walk(hipStream_t s, bool feat, const Prm &q, const Prm *sp, const Buf &b,
                      const uint32_t *skey, const int *cs, float4 *gd, int *nlist, int *ncount, float *f) {
+    const int grid = nblocks_bs(q.N, BS);
     if constexpr (!BATCH) {
         const bool a = q.diffusion == ZSPH_DIFF_ANTUONO;
-        if (feat && a)
-            k_ldg<WALK, false, true, ZSPH_DIFF_ANTUONO><<<grid, ZS_BLOCK, 0, s>>>(q, sp, b, skey, cs, gd, nlist, ncount, f);
-        else if (feat)
-            k_ldg<WALK, false, true, ZSPH_DIFF_MC><<<grid, ZS_BLOCK, 0, s>>>(q, sp, b, skey, cs, gd, nlist, ncount, f);
-        else if (a)
-            k_ldg<WALK, false, false, ZSPH_DIFF_ANTUONO><<<grid, ZS_BLOCK, 0, s>>>(q, sp, b, skey, cs, gd, nlist, ncount, f);
+        if (a)
+            k_ldg<WALK, false, false, ZSPH_DIFF_ANTUONO, BS><<<grid, BS, 0, s>>>(q, sp, b, skey, cs, gd, nlist, ncount, f);
         else
-            k_ldg<WALK, false, false, ZSPH_DIFF_MC><<<grid, ZS_BLOCK, 0, s>>>(q, sp, b, skey, cs, gd, nlist, ncount, f);
+            k_ldg<WALK, false, false, ZSPH_DIFF_MC, BS><<<grid, BS, 0, s>>>(q, sp, b, skey, cs, gd, nlist, ncount, f);
     } else {
-        k_ldg<WALK, true, false, DIFF_PER_SCENE><<<grid, ZS_BLOCK, 0, s>>>(q, sp, b, skey, cs, gd, nlist, ncount, f);
+        k_ldg<WALK, true, false, DIFF_PER_SCENE, BS><<<grid, BS, 0, s>>>(q, sp, b, skey, cs, gd, nlist, ncount, f);
     }
 }
+// The feature-writing gradient pass of the learned closure (one scene): 256 threads.
+template <int WALK>
+static void ldg_feat(hipStream_t s, const Prm &q, const Prm *sp, const Buf &b, const uint32_t *skey, const int *cs,
+                     float4 *gd, int *nlist, int *ncount, float *f) {
+    const int grid = nblocks_bs(q.N, ZS_BLOCK);
+    if (q.diffusion == ZSPH_DIFF_ANTUONO)
+        k_ldg<WALK, false, true, ZSPH_DIFF_ANTUONO, ZS_BLOCK><<<grid, ZS_BLOCK, 0, s>>>(q, sp, b, skey, cs, gd, nlist, ncount, f);
+    else
+        k_ldg<WALK, false, true, ZSPH_DIFF_MC, ZS_BLOCK><<<grid, ZS_BLOCK, 0, s>>>(q, sp, b, skey, cs, gd, nlist, ncount, f);
+}
 
-void launch_ldg(hipStream_t s, int walk, const Prm &q, const Prm *sp, const Buf &b, const uint32_t *skey,
+void launch_ldg(hipStream_t s, int walk, int block, const Prm &q, const Prm *sp, const Buf &b, const uint32_t *skey,
                 const int *cs, float4 *gd, int *nlist, int *ncount, float *feat) {
     if (!q.N) return;
-    const int g = nblocks(q.N);
-    const bool f = feat != nullptr;
-    if (walk == WALK_LIST) {
-        if (sp) ldg_walk<WALK_LIST, true>(s, f, g, q, sp, b, skey, cs, gd, nlist, ncount, feat);
-        else ldg_walk<WALK_LIST, false>(s, f, g, q, sp, b, skey, cs, gd, nlist, ncount, feat);
-    } else if (walk == WALK_HITS) {
-        if (sp) ldg_walk<WALK_HITS, true>(s, f, g, q, sp, b, skey, cs, gd, nlist, ncount, feat);
-        else ldg_walk<WALK_HITS, false>(s, f, g, q, sp, b, skey, cs, gd, nlist, ncount, feat);
-    } else {
-        if (sp) ldg_walk<WALK_MASK, true>(s, f, g, q, sp, b, skey, cs, gd, nlist, ncount, feat);
-        else ldg_walk<WALK_MASK, false>(s, f, g, q, sp, b, skey, cs, gd, nlist, ncount, feat);
+    if (feat) {
+        if (walk == WALK_LIST) ldg_feat<WALK_LIST>(s, q, sp, b, skey, cs, gd, nlist, ncount, feat);
+        else if (walk == WALK_HITS) ldg_feat<WALK_HITS>(s, q, sp, b, skey, cs, gd, nlist, ncount, feat);
+        else ldg_feat<WALK_MASK>(s, q, sp, b, skey, cs, gd, nlist, ncount, feat);
+        return;
     }
+    with_block(block, [&](auto BSc) {
+        constexpr int BS = decltype(BSc)::value;
+        if (walk == WALK_LIST) {
+            if (sp) ldg_walk<WALK_LIST, true, BS>(s, false, q, sp, b, skey, cs, gd, nlist, ncount, nullptr);
+            else ldg_walk<WALK_LIST, false, BS>(s, false, q, sp, b, skey, cs, gd, nlist, ncount, nullptr);
+        } else if (walk == WALK_HITS) {
+            if (sp) ldg_walk<WALK_HITS, true, BS>(s, false, q, sp, b, skey, cs, gd, nlist, ncount, nullptr);
+            else ldg_walk<WALK_HITS, false, BS>(s, false, q, sp, b, skey, cs, gd, nlist, ncount, nullptr);
+        } else {
+            if (sp) ldg_walk<WALK_MASK, true, BS>(s, false, q, sp, b, skey, cs, gd, nlist, ncount, nullptr);
+            else ldg_walk<WALK_MASK, false, BS>(s, false, q, sp, b, skey, cs, gd, nlist, ncount, nullptr);
+        }
+    });
 }
 
+// Plain continuity+momentum (mode 0) at the tuned block size; the closure
+// modes (1, 2: one scene) at 256 threads.
 template <int WALK, bool BATCH>
-static void gov_walk(hipStream_t s, int mode, int grid, const Prm &q, const Prm *sp, const Buf &b,
+static void gov_walk(hipStream_t s, int mode, int block, const Prm &q, const Prm *sp, const Buf &b,
                      const uint32_t *skey, const int *cs, const float4 *gd, float4 *acc, const int *nlist,
                      const int *ncount, const float2 *scale, float *terms) {
-#define ZS_GOV(MODE_, DF_) \
-    k_governing<WALK, BATCH, MODE_, DF_><<<grid, ZS_BLOCK, 0, s>>>(q, sp, b, skey, cs, gd, acc, nlist, ncount, scale, terms)
+#define ZS_GOV(MODE_, DF_, BS_)                                                                  \
+    k_governing<WALK, BATCH, MODE_, DF_, BS_><<<nblocks_bs(q.N, BS_), BS_, 0, s>>>(              \
+        q, sp, b, skey, cs, gd, acc, nlist, ncount, scale, terms)
     if constexpr (!BATCH) {
         const bool a = q.diffusion == ZSPH_DIFF_ANTUONO;
         if (mode == 1) {
-            if (a) ZS_GOV(1, ZSPH_DIFF_ANTUONO);
-            else ZS_GOV(1, ZSPH_DIFF_MC);
+            if (a) ZS_GOV(1, ZSPH_DIFF_ANTUONO, ZS_BLOCK);
+            else ZS_GOV(1, ZSPH_DIFF_MC, ZS_BLOCK);
         } else if (mode == 2) {
-            if (a) ZS_GOV(2, ZSPH_DIFF_ANTUONO);
-            else ZS_GOV(2, ZSPH_DIFF_MC);
+            if (a) ZS_GOV(2, ZSPH_DIFF_ANTUONO, ZS_BLOCK);
+            else ZS_GOV(2, ZSPH_DIFF_MC, ZS_BLOCK);
         } else {
-            if (a) ZS_GOV(0, ZSPH_DIFF_ANTUONO);
-            else ZS_GOV(0, ZSPH_DIFF_MC);
+            with_block(block, [&](auto BSc) {
+                constexpr int BS = decltype(BSc)::value;
+                if (a) ZS_GOV(0, ZSPH_DIFF_ANTUONO, BS);
+                else ZS_GOV(0, ZSPH_DIFF_MC, BS);
+            });
         }
     } else {
-        ZS_GOV(0, DIFF_PER_SCENE);
+        with_block(block, [&](auto BSc) {
+            constexpr int BS = decltype(BSc)::value;
+            ZS_GOV(0, DIFF_PER_SCENE, BS);
+        });
     }
 #undef ZS_GOV
 }
 
-void launch_governing(hipStream_t s, int walk, const Prm &q, const Prm *sp, const Buf &b, const uint32_t *skey,
-                      const int *cs, const float4 *gd, float4 *acc, const int *nlist, const int *ncount, int mode,
-                      const float2 *scale, float *terms, unsigned long long *clock, bool moving) {
+void launch_governing(hipStream_t s, int walk, int block, const Prm &q, const Prm *sp, const Buf &b,
+                      const uint32_t *skey, const int *cs, const float4 *gd, float4 *acc, const int *nlist,
+                      const int *ncount, int mode, const float2 *scale, float *terms, unsigned long long *clock,
+                      bool moving) {
     if (!q.N) return;
-    const int g = nblocks(q.N);
     if (clock) launch_clock(s, clock, true);
     if (walk == WALK_LIST) {
-        if (sp) gov_walk<WALK_LIST, true>(s, mode, g, q, sp, b, skey, cs, gd, acc, nlist, ncount, scale, terms);
-        else gov_walk<WALK_LIST, false>(s, mode, g, q, sp, b, skey, cs, gd, acc, nlist, ncount, scale, terms);
+        if (sp) gov_walk<WALK_LIST, true>(s, mode, block, q, sp, b, skey, cs, gd, acc, nlist, ncount, scale, terms);
+        else gov_walk<WALK_LIST, false>(s, mode, block, q, sp, b, skey, cs, gd, acc, nlist, ncount, scale, terms);
     } else if (walk == WALK_HITS) {
-        if (sp) gov_walk<WALK_HITS, true>(s, mode, g, q, sp, b, skey, cs, gd, acc, nlist, ncount, scale, terms);
-        else gov_walk<WALK_HITS, false>(s, mode, g, q, sp, b, skey, cs, gd, acc, nlist, ncount, scale, terms);
+        if (sp) gov_walk<WALK_HITS, true>(s, mode, block, q, sp, b, skey, cs, gd, acc, nlist, ncount, scale, terms);
+        else gov_walk<WALK_HITS, false>(s, mode, block, q, sp, b, skey, cs, gd, acc, nlist, ncount, scale, terms);
     } else {
-        if (sp) gov_walk<WALK_MASK, true>(s, mode, g, q, sp, b, skey, cs, gd, acc, nlist, ncount, scale, terms);
-        else gov_walk<WALK_MASK, false>(s, mode, g, q, sp, b, skey, cs, gd, acc, nlist, ncount, scale, terms);
+        if (sp) gov_walk<WALK_MASK, true>(s, mode, block, q, sp, b, skey, cs, gd, acc, nlist, ncount, scale, terms);
+        else gov_walk<WALK_MASK, false>(s, mode, block, q, sp, b, skey, cs, gd, acc, nlist, ncount, scale, terms);
     }
     if (clock) launch_clock(s, clock, false);
-    if (moving) ZS_WALK_LAUNCH(k_governing_dummy, nblocks(q.N), q, sp, b, skey, cs, acc);
+    if (moving) {
+        if (sp)
+            k_governing_dummy<WALK_MASK, true><<<nblocks(q.N), ZS_BLOCK, 0, s>>>(q, sp, b, skey, cs, acc);
+        else
+            k_governing_dummy<WALK_MASK, false><<<nblocks(q.N), ZS_BLOCK, 0, s>>>(q, sp, b, skey, cs, acc);
+    }
 }
 void launch_predict(hipStream_t s, const Prm &q, const Prm *sp, const Buf &b, const int *cs, const float4 *acc) {
     if (!q.N) return;
@@ -557,7 +584,12 @@ void launch_correct(hipStream_t s, const Prm &q, const Prm *sp, const Buf &b, co
 }
 void launch_nbr_stats(hipStream_t s, int walk, const Prm &q, const Prm *sp, const Buf &b, const uint32_t *skey,
                       const int *cs, int *count, uint64_t *digest) {
-    if (q.N) ZS_WALK_LAUNCH(k_nbr_stats, nblocks(q.N), q, sp, b, skey, cs, count, digest);
+    (void)walk;
+    if (!q.N) return;
+    if (sp)
+        k_nbr_stats<WALK_MASK, true><<<nblocks(q.N), ZS_BLOCK, 0, s>>>(q, sp, b, skey, cs, count, digest);
+    else
+        k_nbr_stats<WALK_MASK, false><<<nblocks(q.N), ZS_BLOCK, 0, s>>>(q, sp, b, skey, cs, count, digest);
 }
 
 }  // namespace zs

@@ -54,6 +54,8 @@ struct zsph_ctx {
     int keybits = 1;
     int radix_bits = 8;
     int walk = zs::WALK_MASK;  // neighbour-walk variant (tuning)
+    int block = ZS_BLOCK;      // threads per block of the neighbour passes (tuning)
+    int sort_tile = 16;        // keys per thread of the radix tile (tuning)
     int *nlist = nullptr;      // WALK_LIST neighbour lists (list_words(N) ints) and counts
     int *ncount = nullptr;     // WALK_LIST counts / WALK_HITS overflow flags
     int *hits = nullptr;       // WALK_HITS hit masks (hits_slots(k) x N words)
@@ -182,9 +184,10 @@ int stage_sort(zsph_ctx *c, Prof *pf = nullptr) {
     int slot;
     {
         Scope sc(pf, ZSPH_ST_SORT);
-        slot = zs::launch_sort(c->stream, c->q.N, c->keybits, c->radix_bits, c->key, c->idx, c->hist, true);
+        slot = zs::launch_sort(c->stream, c->q.N, c->keybits, c->radix_bits, c->sort_tile, c->key, c->idx, c->hist,
+                               true);
     }
-    if (slot < 0) return set_err(c, ZSPH_E_ARG, "unsupported radix width %d", c->radix_bits);
+    if (slot < 0) return set_err(c, ZSPH_E_ARG, "unsupported radix width %d / tile %d", c->radix_bits, c->sort_tile);
     c->sorted_slot = slot;
     c->hashed = 0;
     return check_launch(c, "radix sort");
@@ -221,14 +224,14 @@ int stage_boundary(zsph_ctx *c, int b, Prof *pf = nullptr) {
         Scope sc(pf, ZSPH_ST_WALL);
         unsigned long long *ck = stage_clock(c, ZSPH_ST_WALL);
         if (ck) zs::launch_clock(c->stream, ck, true);
-        zs::launch_wall(c->stream, c->walk, c->q, c->scene_prm, c->buf[b], sk, c->cell_start);
+        zs::launch_wall(c->stream, c->walk, c->block, c->q, c->scene_prm, c->buf[b], sk, c->cell_start);
         if (ck) zs::launch_clock(c->stream, ck, false);
     }
     {
         Scope sc(pf, ZSPH_ST_LDG);
         unsigned long long *ck = stage_clock(c, ZSPH_ST_LDG);
         if (ck) zs::launch_clock(c->stream, ck, true);
-        zs::launch_ldg(c->stream, c->walk, c->q, c->scene_prm, c->buf[b], sk, c->cell_start, c->gd,
+        zs::launch_ldg(c->stream, c->walk, c->block, c->q, c->scene_prm, c->buf[b], sk, c->cell_start, c->gd,
                        walk_record(c), c->ncount, c->closure_on ? c->feat : nullptr);
         if (ck) zs::launch_clock(c->stream, ck, false);
     }
@@ -244,7 +247,7 @@ int stage_governing(zsph_ctx *c, int b, Prof *pf = nullptr) {
         if (c->closure_on)  // scale factors of the fluid particles [0, cs[G])
             zs::launch_mlp_infer(c->stream, c->mlp_P, c->mlp_pack, c->mlp_aux, c->feat, 0, c->cell_start + c->q.G,
                                  c->scale, zs::mlp_infer_blocks(c->q.N));
-        zs::launch_governing(c->stream, walk, c->q, c->scene_prm, c->buf[b], c->key[c->sorted_slot], c->cell_start,
+        zs::launch_governing(c->stream, walk, c->block, c->q, c->scene_prm, c->buf[b], c->key[c->sorted_slot], c->cell_start,
                              c->gd, c->acc, walk_record(c), c->ncount, c->closure_on ? 1 : 0, c->scale, c->terms,
                              stage_clock(c, ZSPH_ST_GOVERNING), c->has_body || c->has_piston);
     }
@@ -767,9 +770,14 @@ int zsph_step(zsph_ctx *c, int nsteps) {
 
 int zsph_set_tuning(zsph_ctx *c, const zsph_tuning *t) {
     if (!c || !t) return ZSPH_E_ARG;
-    if (t->walk < ZSPH_WALK_MASK || t->walk > ZSPH_WALK_LIST || t->radix_bits < 8 || t->radix_bits > 11)
-        return set_err(c, ZSPH_E_ARG, "zsph_set_tuning: walk %d / radix_bits %d out of range", t->walk, t->radix_bits);
-    if (t->walk == c->walk && t->radix_bits == c->radix_bits) return ZSPH_OK;
+    const int block = t->block ? t->block : ZS_BLOCK;
+    const int tile = t->sort_tile ? t->sort_tile : 16;
+    if (t->walk < ZSPH_WALK_MASK || t->walk > ZSPH_WALK_LIST || t->radix_bits < 8 || t->radix_bits > 11 ||
+        (block != 128 && block != 256 && block != 512) || (tile != 8 && tile != 16))
+        return set_err(c, ZSPH_E_ARG, "zsph_set_tuning: walk %d / radix_bits %d / block %d / sort_tile %d out of range",
+                       t->walk, t->radix_bits, t->block, t->sort_tile);
+    if (t->walk == c->walk && t->radix_bits == c->radix_bits && block == c->block && tile == c->sort_tile)
+        return ZSPH_OK;
     int r;
     if (walk_records(t->walk) && !c->ncount && (r = dalloc(c, &c->ncount, (size_t)c->q.N))) return r;
     if (t->walk == ZSPH_WALK_LIST && !c->nlist && (r = dalloc(c, &c->nlist, zs::list_words(c->q.N)))) return r;
@@ -781,6 +789,8 @@ int zsph_set_tuning(zsph_ctx *c, const zsph_tuning *t) {
     c->list_valid = false;
     c->walk = t->walk;
     c->radix_bits = t->radix_bits;
+    c->block = block;
+    c->sort_tile = tile;
     drop_graph(c);  // the captured step embeds the old launches
     c->sorted_slot = -1;
     c->hashed = 0;
@@ -791,6 +801,8 @@ int zsph_get_tuning(zsph_ctx *c, zsph_tuning *t) {
     if (!c || !t) return ZSPH_E_ARG;
     t->walk = c->walk;
     t->radix_bits = c->radix_bits;
+    t->block = c->block;
+    t->sort_tile = c->sort_tile;
     return ZSPH_OK;
 }
 
@@ -831,30 +843,38 @@ int zsph_autotune(zsph_ctx *c, int steps, zsph_tuning *best, float *best_ms) {
         (void)hipFree(save);
         return set_err(c, ZSPH_E_HIP, "autotune save: %s", hipGetErrorString(e));
     }
-    zsph_tuning cand[8];
-    int nc = 0;
     const int passes = (c->keybits + 10) / 11;  // fewest passes of <= 11-bit digits
     const int even = std::max(8, (c->keybits + passes - 1) / passes);
-    for (int walk = ZSPH_WALK_MASK; walk <= ZSPH_WALK_LIST; ++walk) {
-        cand[nc++] = {walk, even};
-        if (even != 8) cand[nc++] = {walk, 8};
-    }
-    zsph_tuning keep{c->walk, c->radix_bits};
+    zsph_tuning keep{c->walk, c->radix_bits, c->block, c->sort_tile};
     float best_t = 1e30f;
     int r = ZSPH_OK;
-    for (int k = 0; k < nc && !r; ++k) {
-        if ((r = zsph_set_tuning(c, &cand[k]))) break;
+    // times one candidate from the saved state; keeps it if it is the fastest so far
+    auto trial = [&](const zsph_tuning &t) {
+        if ((r = zsph_set_tuning(c, &t))) return;
         if ((e = copy_all(false)) != hipSuccess) {
             r = set_err(c, ZSPH_E_HIP, "autotune restore: %s", hipGetErrorString(e));
-            break;
+            return;
         }
         float ms = 0.0f;
-        if ((r = zsph_time_steps(c, steps, &ms))) break;
+        if ((r = zsph_time_steps(c, steps, &ms))) return;
         if (ms < best_t) {
             best_t = ms;
-            keep = cand[k];
+            keep = t;
         }
+    };
+    // coordinate search: walk x digit width, then block size, then sort tile
+    const zsph_tuning start = keep;
+    for (int walk = ZSPH_WALK_MASK; walk <= ZSPH_WALK_LIST && !r; ++walk) {
+        trial({walk, even, start.block, start.sort_tile});
+        if (even != 8 && !r) trial({walk, 8, start.block, start.sort_tile});
     }
+    const int blocks[3] = {128, 256, 512};
+    zsph_tuning base = keep;
+    for (int k = 0; k < 3 && !r; ++k)
+        if (blocks[k] != base.block) trial({base.walk, base.radix_bits, blocks[k], base.sort_tile});
+    base = keep;
+    for (int tile = 8; tile <= 16 && !r; tile *= 2)
+        if (tile != base.sort_tile) trial({base.walk, base.radix_bits, base.block, tile});
     if (!r) r = zsph_set_tuning(c, &keep);
     e = copy_all(false);
     (void)hipFree(save);
@@ -1153,9 +1173,10 @@ int zsph_closure_sample(zsph_ctx *c, int b, float *features, float *terms) {
     HIPCHK(c, hipMemsetAsync(c->feat, 0, N * ZSPH_NF * sizeof(float), c->stream));
     HIPCHK(c, hipMemsetAsync(c->terms, 0, N * 8 * sizeof(float), c->stream));
     const int walk = ZSPH_WALK_MASK;  // every walk visits the same neighbours in the same order
-    zs::launch_wall(c->stream, walk, c->q, nullptr, c->buf[b], sk, c->cell_start);
-    zs::launch_ldg(c->stream, walk, c->q, nullptr, c->buf[b], sk, c->cell_start, c->gd, c->nlist, c->ncount, c->feat);
-    zs::launch_governing(c->stream, walk, c->q, nullptr, c->buf[b], sk, c->cell_start, c->gd, c->acc, c->nlist,
+    zs::launch_wall(c->stream, walk, c->block, c->q, nullptr, c->buf[b], sk, c->cell_start);
+    zs::launch_ldg(c->stream, walk, c->block, c->q, nullptr, c->buf[b], sk, c->cell_start, c->gd, c->nlist, c->ncount,
+                   c->feat);
+    zs::launch_governing(c->stream, walk, c->block, c->q, nullptr, c->buf[b], sk, c->cell_start, c->gd, c->acc, c->nlist,
                          c->ncount, 2, c->scale, c->terms, nullptr, c->has_body || c->has_piston);
     if ((r = check_launch(c, "closure sample"))) return r;
     c->list_valid = false;
