@@ -194,6 +194,10 @@ def main():
     ap.add_argument("--radix-bits", type=int, default=None, help="force the sort's digit width (8 to 11 bits)")
     ap.add_argument("--block", type=int, default=None, help="force the neighbour passes' block size (128/256/512)")
     ap.add_argument("--sort-tile", type=int, default=None, help="force the radix tile's keys per thread (8/16)")
+    ap.add_argument("--skin", type=float, default=0.0,
+                    help="neighbour skin in units of h (0: rebuild every half step, the reference schedule; > 0: "
+                         "rebuild only after a particle moved more than skin/2; the autotuner then also tries "
+                         "skins of 0.05, 0.1, 0.2 and 0.3)")
     ap.add_argument("--closure", default=None, metavar="NPZ",
                     help="run every step with this learned closure network (one-scene cases)")
     ap.add_argument("--tune-at", type=int, default=None,
@@ -270,6 +274,11 @@ def main():
     s.upload(case.host)
     if setup:
         setup(s)
+    if args.skin > 0:
+        if batch:
+            print("bench.py: --skin needs a one-scene case", file=sys.stderr)
+            sys.exit(2)
+        s.set_skin(args.skin)
     # pick the neighbour-walk variant and sort digit width on this case
     if any(v is not None for v in (args.walk, args.radix_bits, args.block, args.sort_tile)):
         t = s.get_tuning()
@@ -311,6 +320,7 @@ def main():
     s.sync()
     for st in (z.ST_WALL, z.ST_LDG, z.ST_GOVERNING):
         s.clock_read_stage(st)  # reset
+    rebuilds0 = s.rebuilds()
     t0 = time.perf_counter()
     # replayed in chunks with a host sync between them: bounds the launches in
     # flight (profilers that trace every dispatch keep a record per launch)
@@ -318,6 +328,7 @@ def main():
         s.step(min(args.chunk, args.steps - k0))
         s.sync()
     wall = time.perf_counter() - t0
+    rebuilds = s.rebuilds() - rebuilds0
     clocks = {st: s.clock_read_stage(st) for st in (z.ST_WALL, z.ST_LDG, z.ST_GOVERNING)}
     for st in (z.ST_WALL, z.ST_LDG, z.ST_GOVERNING):
         s.clock_stage(st, False)
@@ -365,7 +376,9 @@ def main():
                    "timed_steps": [args.start_at, args.start_at + args.steps], "run_steps": run_len,
                    "event_ms_per_step": ev_ms / args.event_steps if ev_ms is not None else None, "event_steps": args.event_steps,
                    "sim_time_s": sim_t, "removed_particles": killed,
-                   "tuning": tune, "graph": args.graph, **extra},
+                   "tuning": tune, "graph": args.graph,
+                   "neighbour_rebuilds": (rebuilds if tune.get("skin", 0.0) > 0 else 2 * args.steps),
+                   **extra},
         "roofline": roof,
         "roofline_governing": roof_gov,
         "wall_pass_ms_per_launch": avg[z.ST_WALL][0],

@@ -548,14 +548,30 @@ typedef struct zsph_tuning {
     int sort_tile;   /* keys per thread of the radix tile, 8 or 16 (0: 16);
                       * the scatter stages the whole tile in LDS (8 B per
                       * key: 16 or 32 KB)                                    */
+    float skin;      /* zsph_step's neighbour skin s in units of h, 0..1 (0:
+                      * the reference schedule: hash, sort, cell table and
+                      * reorder every half step).  s > 0 (one-scene contexts):
+                      * zsph_step keeps the cell structures of its last
+                      * rebuild, on a grid of cells (2h + s)/grid_factor with
+                      * rows trimmed to 2h + s/2, and rebuilds them only after
+                      * some particle has moved more than s/2 since (or left
+                      * the grid).  Neighbour sets are the same (r^2 <= (2h)^2
+                      * decides every pair); the particle order, and with it
+                      * the FP32 summation order, follows the rebuilds.      */
 } zsph_tuning;
 int zsph_set_tuning(zsph_ctx *ctx, const zsph_tuning *t);
 int zsph_get_tuning(zsph_ctx *ctx, zsph_tuning *t);
+/* Rebuilds of the neighbour structures done by zsph_step in skin mode since
+ * zsph_create (no counterpart in the reference, which rebuilds every half
+ * step). */
+int zsph_rebuilds(zsph_ctx *ctx, long long *n);
 /* Times tuning candidates over `steps` graph-replayed steps each, every one
  * starting from a saved copy of buffer 0, restores that state, and keeps the
  * fastest; *best and *best_ms (may be NULL) receive it.  The search is by
  * coordinates: walk x digit width first (at the current block and tile),
- * then the block size, then the sort tile, each from the best so far. */
+ * then the block size, then the sort tile, each from the best so far; in
+ * skin mode (skin > 0 set beforehand) then skins of 0.05, 0.1, 0.2 and 0.3 h, over
+ * four times `steps` (a skin's saving shows over several steps). */
 int zsph_autotune(zsph_ctx *ctx, int steps, zsph_tuning *best, float *best_ms);
 
 /* zsph_step launches each step's kernels directly instead of replaying the

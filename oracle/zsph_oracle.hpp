@@ -791,8 +791,9 @@ struct Oracle {
         z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
         return z ^ (z >> 31);
     }
-    // Neighbour count and digest sum mix64(j): by the cell walk, or by brute
-    // force over all j (O(N^2), small cases only).
+    // Neighbour count and digest sum mix64(id_j) over the neighbours' original
+    // ids id_j = table[j] + j (independent of the particle order): by the cell
+    // walk, or by brute force over all j (O(N^2), small cases only).
     void nbr_stats(int bi, bool brute, int *count, uint64_t *digest) const {
         const Buffer<R> &b = buf[bi];
 #pragma omp parallel for schedule(dynamic, 64)
@@ -802,7 +803,7 @@ struct Oracle {
             auto visit = [&](int j) {
                 if (j == i || !is_nbr(b, i, j)) return;
                 ++c;
-                h += mix64((uint64_t)j);
+                h += mix64((uint64_t)(b.table[j] + j));
             };
             if (brute)
                 for (int j = 0; j < N; ++j) visit(j);

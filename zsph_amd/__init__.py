@@ -332,18 +332,25 @@ class Solver:
                   "zsph_neighbor_stats")
         return cnt, dig
 
-    def set_tuning(self, walk, radix_bits, block=None, sort_tile=None):
-        """Neighbour walk, sort digit width, neighbour-pass block size and sort tile
-        (None keeps the current block size / tile)."""
+    def set_tuning(self, walk, radix_bits, block=None, sort_tile=None, skin=None):
+        """Neighbour walk, sort digit width, neighbour-pass block size, sort tile and
+        zsph_step's neighbour skin in h (None keeps the current block size / tile / skin)."""
         cur = self.get_tuning()
         t = abi.Tuning(walk, radix_bits, cur["block"] if block is None else block,
-                       cur["sort_tile"] if sort_tile is None else sort_tile)
+                       cur["sort_tile"] if sort_tile is None else sort_tile,
+                       cur["skin"] if skin is None else skin)
         self._chk(lib().zsph_set_tuning(self._ctx, C.byref(t)), "zsph_set_tuning")
+
+    def set_skin(self, skin):
+        """zsph_step's neighbour skin in units of h (0: rebuild every half step, the reference schedule)."""
+        t = self.get_tuning()
+        self.set_tuning(t["walk"], t["radix_bits"], skin=skin)
 
     def get_tuning(self):
         t = abi.Tuning()
         self._chk(lib().zsph_get_tuning(self._ctx, C.byref(t)), "zsph_get_tuning")
-        return {"walk": t.walk, "radix_bits": t.radix_bits, "block": t.block, "sort_tile": t.sort_tile}
+        return {"walk": t.walk, "radix_bits": t.radix_bits, "block": t.block, "sort_tile": t.sort_tile,
+                "skin": round(t.skin, 6)}
 
     def autotune(self, steps=5):
         """Time tuning candidates from the current buffer-0 state (restored afterwards); keep the fastest."""
@@ -351,7 +358,13 @@ class Solver:
         ms = C.c_float(0.0)
         self._chk(lib().zsph_autotune(self._ctx, steps, C.byref(t), C.byref(ms)), "zsph_autotune")
         return {"walk": t.walk, "radix_bits": t.radix_bits, "block": t.block, "sort_tile": t.sort_tile,
-                "ms_per_step": ms.value / steps}
+                "skin": round(t.skin, 6), "ms_per_step": ms.value / steps}
+
+    def rebuilds(self):
+        """Neighbour-structure rebuilds done by zsph_step in skin mode since creation."""
+        n = C.c_longlong(0)
+        self._chk(lib().zsph_rebuilds(self._ctx, C.byref(n)), "zsph_rebuilds")
+        return n.value
 
     def time_steps(self, nsteps):
         """Milliseconds of `nsteps` graph-replayed steps, between two HIP events on the solver's stream."""

@@ -89,7 +89,7 @@ class Rigid(C.Structure):
 
 
 class Tuning(C.Structure):
-    _fields_ = [("walk", _i), ("radix_bits", _i), ("block", _i), ("sort_tile", _i)]
+    _fields_ = [("walk", _i), ("radix_bits", _i), ("block", _i), ("sort_tile", _i), ("skin", _f)]
 
 
 WALK_MASK, WALK_HITS, WALK_LIST = 0, 1, 2
@@ -188,6 +188,7 @@ def load(path):
     _sig(lib, "zsph_set_tuning", _i, [vp, P(Tuning)])
     _sig(lib, "zsph_get_tuning", _i, [vp, P(Tuning)])
     _sig(lib, "zsph_autotune", _i, [vp, _i, P(Tuning), P(_f)])
+    _sig(lib, "zsph_rebuilds", _i, [vp, P(C.c_longlong)])
     _sig(lib, "zsph_set_body", _i, [vp, P(Rigid)])
     _sig(lib, "zsph_get_body", _i, [vp, P(Rigid)])
     _sig(lib, "zsph_set_piston", _i, [vp, P(Piston)])
@@ -234,7 +235,7 @@ EXPORTS = (
     "zsph_set_piston", "zsph_set_time", "zsph_get_time", "zsph_body_reduce", "zsph_case_wavetank",
     "zsph_create_batch", "zsph_scene_diagnostics", "zsph_mlp_create", "zsph_mlp_destroy", "zsph_mlp_last_error",
     "zsph_mlp_set", "zsph_mlp_get", "zsph_mlp_data", "zsph_mlp_eval", "zsph_mlp_adam", "zsph_mlp_infer",
-    "zsph_set_closure", "zsph_closure_sample", "zsph_filter_fields", "zsph_clock_governing", "zsph_clock_read", "zsph_clock_stage", "zsph_clock_read_stage",
+    "zsph_set_closure", "zsph_closure_sample", "zsph_filter_fields", "zsph_clock_governing", "zsph_clock_read", "zsph_rebuilds", "zsph_clock_stage", "zsph_clock_read_stage",
     "zsph_set_graph", "zsph_checkpoint_write", "zsph_checkpoint_read", "zsph_checkpoint_inspect",
 )
 

@@ -90,7 +90,7 @@ __global__ __launch_bounds__(64) void k_body_update(Prm q, DevBody *__restrict__
 // Body dummies: r = R r0, x = c + r, v = V + w x r, a_w = A + alpha x r + w x (w x r).
 // Piston dummies: x = x0 + d(t) e, v = d'(t) e, a_w = d''(t) e along the axis e.
 __global__ __launch_bounds__(ZS_BLOCK) void k_kinematics(Prm q, Buf b, const int *__restrict__ cs,
-                                                         const DevBody *__restrict__ body) {
+                                                         const DevBody *__restrict__ body, SkinCheck chk) {
     const int i = cs[q.G] + xcd_block(blocks_of(q.N - cs[q.G])) * ZS_BLOCK + threadIdx.x;
     if (i >= q.N) return;
     const int ti = b.type[i];
@@ -125,6 +125,7 @@ __global__ __launch_bounds__(ZS_BLOCK) void k_kinematics(Prm q, Buf b, const int
     b.posr[i] = x;
     b.velp[i] = v;
     b.tvelp[i] = a;
+    if (chk.xref && skin_moved(q, chk, i, x)) *chk.flag = 1;  // moving dummies are few: no wave vote
 }
 
 static inline int nblocks(int n) { return (n + ZS_BLOCK - 1) / ZS_BLOCK; }
@@ -136,8 +137,9 @@ void launch_body_reduce(hipStream_t s, const Prm &q, const Buf &b, const int *cs
 void launch_body_update(hipStream_t s, const Prm &q, DevBody *body, const double *part, bool predictor) {
     k_body_update<<<1, 64, 0, s>>>(q, body, part, predictor ? 1 : 0);
 }
-void launch_kinematics(hipStream_t s, const Prm &q, const Buf &b, const int *cs, const DevBody *body) {
-    if (q.N) k_kinematics<<<nblocks(q.N), ZS_BLOCK, 0, s>>>(q, b, cs, body);
+void launch_kinematics(hipStream_t s, const Prm &q, const Buf &b, const int *cs, const DevBody *body,
+                       const SkinCheck &chk) {
+    if (q.N) k_kinematics<<<nblocks(q.N), ZS_BLOCK, 0, s>>>(q, b, cs, body, chk);
 }
 
 }  // namespace zs

@@ -3,6 +3,7 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <cmath>
 #include <type_traits>
 
 #include "zsph.h"
@@ -29,6 +30,12 @@ struct Prm {
     int N;
     int dim;
     int diffusion;
+    // The grid whose outside removes particles (the case's grid).  It equals the
+    // neighbour grid above except in skin mode, where the neighbour grid has
+    // larger cells (make_skin_prm) and particles are still removed exactly where
+    // the case's grid ends (lib/hash.cu:16-23).
+    float kill_size;
+    int kill_X, kill_Y, kill_Z;
 };
 
 // Host-side construction (also used by the C ABI layer).
@@ -75,7 +82,39 @@ inline Prm make_prm(const zsph_params &p) {
     q.N = p.ptc_num;
     q.dim = p.dim;
     q.diffusion = p.diffusion;
+    q.kill_size = q.grid_size;
+    q.kill_X = q.X;
+    q.kill_Y = q.Y;
+    q.kill_Z = q.Z;
     return q;
+}
+
+// Skin-mode neighbour grid for a skin of `skin` h (> 0): cells of edge
+// e = 1.001 (rc + s) / k with s = skin h, so the (2k+1)^3 cell box around a
+// particle's cell at the last rebuild reaches rc + s; rows are trimmed to
+// rc + s/2 around the particle's current position.  While no particle has
+// moved more than s/2 since the rebuild, every pair within rc has each
+// partner inside a trimmed row of the other's box, so the r^2 <= rc2
+// predicate alone still decides the neighbour set.  The grid has the same
+// origin and covers the case's grid (X' = floor(X size / e) + 1 <= X cells
+// per axis, so G' <= G and the flag bit and cell-table sizes still hold);
+// removal stays on the case's grid (kill_*).
+inline Prm make_skin_prm(const Prm &q, float skin) {
+    Prm s = q;
+    const double rc = std::sqrt((double)q.rc2);
+    const double sk = (double)skin * (double)q.h;
+    const double e = 1.001 * (rc + sk) / (double)q.k;
+    s.grid_size = (float)e;
+    s.inv_cell = (float)(1.0 / e);
+    s.X = (int)std::floor((double)q.X * (double)q.grid_size / e) + 1;
+    s.Y = (int)std::floor((double)q.Y * (double)q.grid_size / e) + 1;
+    s.Z = (int)std::floor((double)q.Z * (double)q.grid_size / e) + 1;
+    if (q.dim == 2) s.Y = q.Y;  // the x-z plane keeps its single row of cells in y
+    s.XY = s.X * s.Y;
+    s.G = s.XY * s.Z;
+    const double rt = (rc + 0.5 * sk) / e;  // trimming radius in cell units
+    s.trim2 = (float)(rt * rt * 1.002);
+    return s;
 }
 
 // One particle buffer (the reference's gpu_ptc_t, data_structure.cuh:158-168,
@@ -145,6 +184,16 @@ __host__ __device__ inline bool cell_axis(float x, float o, float size, int dim,
     return true;
 }
 
+// Skin mode's displacement test of the integrators (xref == nullptr: off):
+// a particle that has moved more than s/2 from its position at the last
+// rebuild, or has left the case's grid (to be removed by the next hash),
+// sets *flag so that the next half step rebuilds the neighbour structures.
+struct SkinCheck {
+    const float4 *xref = nullptr;
+    int *flag = nullptr;
+    float lim2 = 0.0f;  // (s/2)^2, shrunk by 0.1 %
+};
+
 // EOS p(rho).  gamma == 1 is the reference's linear form c^2 (rho - rho0)
 // (lib/time_int.cu:39-40); gamma == 7 uses the expanded (1+e)^7 - 1 so the
 // small-compression regime keeps full relative precision.
@@ -192,6 +241,21 @@ __device__ __forceinline__ float hw_sqrt(float x) { return __builtin_amdgcn_sqrt
 // r^2 in one fixed FP32 form (shared with the oracle): fma(dz,dz,fma(dy,dy,dx*dx)).
 __host__ __device__ inline float dist2(float dx, float dy, float dz) {
     return __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx));
+}
+
+// True when particle i at x needs a rebuild (see SkinCheck); NaN positions do.
+__device__ __forceinline__ bool skin_moved(const Prm &q, const SkinCheck &k, int i, float4 x) {
+    const float4 r = k.xref[i];
+    if (!(dist2(x.x - r.x, x.y - r.y, x.z - r.z) <= k.lim2)) return true;
+    int c;
+    return !(cell_axis(x.x, q.gx0, q.kill_size, q.kill_X, &c) && cell_axis(x.y, q.gy0, q.kill_size, q.kill_Y, &c) &&
+             cell_axis(x.z, q.gz0, q.kill_size, q.kill_Z, &c));
+}
+
+// One store of the flag per wave whose lanes found a particle that moved.
+__device__ __forceinline__ void skin_flag_wave(const SkinCheck &k, bool moved) {
+    const uint64_t m = __ballot(moved);
+    if (m && (int)(threadIdx.x & 63) == __builtin_ctzll(m)) *k.flag = 1;
 }
 
 // Decode a linear cell index into the clamped search box.

@@ -10,7 +10,10 @@ namespace zs {
 // nbr_build.hip
 // sp: per-scene parameter table of a batch (device memory), nullptr for one
 // scene; b.scene must then hold each particle's scene.
-void launch_hash(hipStream_t s, const Prm &q, const Prm *sp, const Buf &b, uint32_t *key, int *idx, int *killed);
+// gate (every neighbour-build launcher; nullptr: always run): skin mode's
+// rebuild flag; the kernels do nothing while *gate == 0.
+void launch_hash(hipStream_t s, const Prm &q, const Prm *sp, const Buf &b, uint32_t *key, int *idx, int *killed,
+                 const int *gate = nullptr);
 int radix_blocks(int n, int items);
 // Histogram buffer words for any digit width up to `bits` and any tile.
 size_t radix_hist_words(int n, int bits);
@@ -19,12 +22,17 @@ size_t radix_hist_words(int n, int bits);
 // (items = keys per thread, 8 or 16).  identity_vals: val[0] is known to be
 // 0..n-1 and is not read.
 int launch_sort(hipStream_t s, int n, int keybits, int bits, int items, uint32_t *key[2], int *val[2],
-                uint32_t *hist, bool identity_vals);
+                uint32_t *hist, bool identity_vals, const int *gate = nullptr);
 // cs[0..G]: fluid cell starts; cs[G+1..2G+1]: dummy cell starts (key flag + c).
 // scratch: cell_start_scratch(G) ints (tile bounds).  n >= 1.
 size_t cell_start_scratch(int G);
-void launch_cell_start(hipStream_t s, const uint32_t *skey, int n, int G, uint32_t flag, int *cs, int *scratch);
-void launch_reorder(hipStream_t s, int n, const int *perm, const Buf &a, const Buf &b);
+void launch_cell_start(hipStream_t s, const uint32_t *skey, int n, int G, uint32_t flag, int *cs, int *scratch,
+                       const int *gate = nullptr);
+void launch_reorder(hipStream_t s, int n, const int *perm, const Buf &a, const Buf &b, const int *gate = nullptr);
+// Skin mode: copy the reordered buffer a into the working buffer b and keep
+// the positions in xref (gated); count and clear the rebuild flag.
+void launch_skin_commit(hipStream_t s, int n, const Buf &a, const Buf &b, float4 *xref, const int *gate);
+void launch_skin_flag(hipStream_t s, int *flag, unsigned long long *rebuilds);
 
 // physics.hip; walk = WALK_MASK, WALK_HITS or WALK_LIST (common.hpp).
 // WALK_LIST: launch_ldg records the neighbour lists (nlist: list_words(N)
@@ -47,8 +55,11 @@ void launch_ldg(hipStream_t s, int walk, int block, const Prm &q, const Prm *sp,
 void launch_governing(hipStream_t s, int walk, int block, const Prm &q, const Prm *sp, const Buf &b, const uint32_t *skey,
                       const int *cs, const float4 *gd, float4 *acc, const int *nlist, const int *ncount, int mode,
                       const float2 *scale, float *terms, unsigned long long *clock, bool moving);
-void launch_predict(hipStream_t s, const Prm &q, const Prm *sp, const Buf &b, const int *cs, const float4 *acc);
-void launch_correct(hipStream_t s, const Prm &q, const Prm *sp, const Buf &b, const int *cs, const float4 *acc);
+// chk (skin mode): the displacement test after the update (chk.xref == nullptr: none).
+void launch_predict(hipStream_t s, const Prm &q, const Prm *sp, const Buf &b, const int *cs, const float4 *acc,
+                    const SkinCheck &chk = SkinCheck{});
+void launch_correct(hipStream_t s, const Prm &q, const Prm *sp, const Buf &b, const int *cs, const float4 *acc,
+                    const SkinCheck &chk = SkinCheck{});
 void launch_nbr_stats(hipStream_t s, int walk, const Prm &q, const Prm *sp, const Buf &b, const uint32_t *skey,
                       const int *cs, int *count, uint64_t *digest);
 
@@ -82,7 +93,8 @@ constexpr int BODY_D = 6;         // F x/y/z, T x/y/z
 void launch_body_reduce(hipStream_t s, const Prm &q, const Buf &b, const int *cs, const float4 *acc,
                         const DevBody *body, double *part);
 void launch_body_update(hipStream_t s, const Prm &q, DevBody *body, const double *part, bool predictor);
-void launch_kinematics(hipStream_t s, const Prm &q, const Buf &b, const int *cs, const DevBody *body);
+void launch_kinematics(hipStream_t s, const Prm &q, const Buf &b, const int *cs, const DevBody *body,
+                       const SkinCheck &chk = SkinCheck{});
 
 // mlp.hip: closure network 24 -> 64 -> 64 -> 64 -> 2 on MFMA (BF16, FP32 accumulate).
 constexpr int MLP_NF = ZSPH_NF;          // features

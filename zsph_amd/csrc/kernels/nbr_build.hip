@@ -19,21 +19,40 @@ namespace zs {
 // v = 0, (rho, p) = (rho0, 0) and H = 0 (hash.cu:16-23).  The sort key adds
 // the non-fluid flag above the cell bits, so fluid sorts first.
 // ---------------------------------------------------------------------------
+// Cell coordinate on the neighbour grid when it differs from the case's grid
+// (skin mode), clamped to the grid: a particle inside the case's grid lies
+// inside the neighbour grid, so the clamp only absorbs a last-bit rounding of
+// the quotient at the far end.
+__device__ __forceinline__ int skin_axis(float x, float o, float size, int dim) {
+    const float t = floorf((float)((double)(x - o) / (double)size));
+    return min(max((int)t, 0), dim - 1);
+}
+
+// gate (skin mode; nullptr otherwise): the kernel does nothing unless *gate
+// is set (the step's rebuild flag, written by the previous integrator).
+__device__ __forceinline__ bool gated_off(const int *gate) { return gate && *gate == 0; }
+
 template <bool BATCH>
 __global__ __launch_bounds__(ZS_BLOCK) void k_hash(Prm q0, const Prm *__restrict__ sp, Buf b,
                                                    uint32_t *__restrict__ key, int *__restrict__ idx,
-                                                   int *__restrict__ killed) {
+                                                   int *__restrict__ killed, const int *__restrict__ gate) {
+    if (gated_off(gate)) return;
     const int i = xcd_block(gridDim.x) * ZS_BLOCK + threadIdx.x;
     if (i >= q0.N) return;
     const Prm q = scene_prm<BATCH>(q0, sp, b, i);
     const float4 x = b.posr[i];
     int cx, cy, cz;
-    const bool in = cell_axis(x.x, q.gx0, q.grid_size, q.X, &cx) &&
-                    cell_axis(x.y, q.gy0, q.grid_size, q.Y, &cy) &&
-                    cell_axis(x.z, q.gz0, q.grid_size, q.Z, &cz);
+    const bool in = cell_axis(x.x, q.gx0, q.kill_size, q.kill_X, &cx) &&
+                    cell_axis(x.y, q.gy0, q.kill_size, q.kill_Y, &cy) &&
+                    cell_axis(x.z, q.gz0, q.kill_size, q.kill_Z, &cz);
     uint32_t h = 0;
     int t;
     if (in) {
+        if (q.kill_size != q.grid_size) {  // skin mode: the key on the neighbour grid
+            cx = skin_axis(x.x, q.gx0, q.grid_size, q.X);
+            cy = skin_axis(x.y, q.gy0, q.grid_size, q.Y);
+            cz = skin_axis(x.z, q.gz0, q.grid_size, q.Z);
+        }
         h = (uint32_t)((cz * q.Y + cy) * q.X + cx);
         t = b.type[i];
     } else {
@@ -79,7 +98,9 @@ __device__ __forceinline__ uint64_t match_digit(uint32_t d, uint64_t valid) {
 
 template <int BITS, int RS_ITEMS>
 __global__ __launch_bounds__(ZS_BLOCK) void k_radix_hist(const uint32_t *__restrict__ kin, int n, int shift,
-                                                         uint32_t *__restrict__ hist, int nblk) {
+                                                         uint32_t *__restrict__ hist, int nblk,
+                                                         const int *__restrict__ gate) {
+    if (gated_off(gate)) return;
     constexpr int RS_TILE = ZS_BLOCK * RS_ITEMS;
     constexpr int R = 1 << BITS;
     __shared__ uint32_t h[R];
@@ -101,7 +122,9 @@ __global__ __launch_bounds__(ZS_BLOCK) void k_radix_hist(const uint32_t *__restr
 // Row scan: block d turns hist[d][0..nblk) into its exclusive prefix (the
 // offset of each tile among digit-d keys) and writes the digit total.
 __global__ __launch_bounds__(ZS_BLOCK) void k_radix_rowscan(uint32_t *__restrict__ hist, int nblk,
-                                                            uint32_t *__restrict__ total) {
+                                                            uint32_t *__restrict__ total,
+                                                            const int *__restrict__ gate) {
+    if (gated_off(gate)) return;
     __shared__ uint32_t part[ZS_BLOCK];
     uint32_t *row = hist + (size_t)blockIdx.x * nblk;
     const int t = threadIdx.x;
@@ -132,7 +155,9 @@ __global__ __launch_bounds__(ZS_BLOCK) void k_radix_scatter(const uint32_t *__re
                                                             uint32_t *__restrict__ kout,
                                                             int *__restrict__ vout, int n, int shift,
                                                             const uint32_t *__restrict__ scanned, int nblk,
-                                                            const uint32_t *__restrict__ total) {
+                                                            const uint32_t *__restrict__ total,
+                                                            const int *__restrict__ gate) {
+    if (gated_off(gate)) return;
     constexpr int RS_TILE = ZS_BLOCK * RS_ITEMS;
     constexpr int R = 1 << BITS;
     constexpr int PER = (R + ZS_BLOCK - 1) / ZS_BLOCK;
@@ -316,7 +341,9 @@ __device__ __forceinline__ void wave_fill(int *__restrict__ bound, int ta, int t
 }
 
 __global__ __launch_bounds__(ZS_BLOCK) void k_cs_bounds(const uint32_t *__restrict__ skey, int n, int G,
-                                                        uint32_t flag, int T, int *__restrict__ bound) {
+                                                        uint32_t flag, int T, int *__restrict__ bound,
+                                                        const int *__restrict__ gate) {
+    if (gated_off(gate)) return;
     const int i = blockIdx.x * ZS_BLOCK + (int)threadIdx.x;
     int ta = 0, tb = -1, ua = 0, ub = -1;
     if (i < n) {
@@ -333,7 +360,9 @@ __global__ __launch_bounds__(ZS_BLOCK) void k_cs_bounds(const uint32_t *__restri
 }
 
 __global__ __launch_bounds__(ZS_BLOCK) void k_cell_start(const uint32_t *__restrict__ skey, int G, uint32_t flag,
-                                                         const int *__restrict__ bound, int *__restrict__ cs) {
+                                                         const int *__restrict__ bound, int *__restrict__ cs,
+                                                         const int *__restrict__ gate) {
+    if (gated_off(gate)) return;
     __shared__ uint32_t keys[CS_KEYS];
     const int E = 2 * G + 2;
     const int e0 = blockIdx.x * CS_TILE;
@@ -356,7 +385,9 @@ __global__ __launch_bounds__(ZS_BLOCK) void k_cell_start(const uint32_t *__restr
 
 // Gather buffer `a` into cell order in buffer `b`; table'[k] = table[p] + p - k
 // keeps table + index equal to the original particle id (sort.cu:107).
-__global__ __launch_bounds__(ZS_BLOCK) void k_reorder(int n, const int *__restrict__ perm, Buf a, Buf b) {
+__global__ __launch_bounds__(ZS_BLOCK) void k_reorder(int n, const int *__restrict__ perm, Buf a, Buf b,
+                                                      const int *__restrict__ gate) {
+    if (gated_off(gate)) return;
     const int k = xcd_block(gridDim.x) * ZS_BLOCK + threadIdx.x;
     if (k >= n) return;
     const int p = perm[k];
@@ -369,17 +400,44 @@ __global__ __launch_bounds__(ZS_BLOCK) void k_reorder(int n, const int *__restri
     if (a.scene) b.scene[k] = a.scene[p];
 }
 
+// Skin mode, end of a rebuild: the reordered state in buffer `a` becomes the
+// working buffer `b` (the step's passes always run on one buffer), and each
+// particle's position is kept as its reference position for the displacement
+// test of the integrators.
+__global__ __launch_bounds__(ZS_BLOCK) void k_skin_commit(int n, Buf a, Buf b, float4 *__restrict__ xref,
+                                                          const int *__restrict__ gate) {
+    if (gated_off(gate)) return;
+    const int k = xcd_block(gridDim.x) * ZS_BLOCK + threadIdx.x;
+    if (k >= n) return;
+    const float4 x = a.posr[k];
+    b.posr[k] = x;
+    xref[k] = x;
+    b.velp[k] = a.velp[k];
+    b.tposr[k] = a.tposr[k];
+    b.tvelp[k] = a.tvelp[k];
+    b.type[k] = a.type[k];
+    b.table[k] = a.table[k];
+}
+
+// Skin mode, after the rebuild kernels of a half step: count a rebuild that
+// happened and clear the flag (the integrator sets it again when needed).
+__global__ void k_skin_flag(int *__restrict__ flag, unsigned long long *__restrict__ rebuilds) {
+    if (*flag) ++*rebuilds;
+    *flag = 0;
+}
+
 // ---------------------------------------------------------------------------
 // Launchers
 // ---------------------------------------------------------------------------
 static inline int nblocks(int n) { return (n + ZS_BLOCK - 1) / ZS_BLOCK; }
 
-void launch_hash(hipStream_t s, const Prm &q, const Prm *sp, const Buf &b, uint32_t *key, int *idx, int *killed) {
+void launch_hash(hipStream_t s, const Prm &q, const Prm *sp, const Buf &b, uint32_t *key, int *idx, int *killed,
+                 const int *gate) {
     if (q.N == 0) return;
     if (sp)
-        k_hash<true><<<nblocks(q.N), ZS_BLOCK, 0, s>>>(q, sp, b, key, idx, killed);
+        k_hash<true><<<nblocks(q.N), ZS_BLOCK, 0, s>>>(q, sp, b, key, idx, killed, gate);
     else
-        k_hash<false><<<nblocks(q.N), ZS_BLOCK, 0, s>>>(q, sp, b, key, idx, killed);
+        k_hash<false><<<nblocks(q.N), ZS_BLOCK, 0, s>>>(q, sp, b, key, idx, killed, gate);
 }
 
 int radix_blocks(int n, int items) { return (n + ZS_BLOCK * items - 1) / (ZS_BLOCK * items); }
@@ -390,16 +448,16 @@ size_t radix_hist_words(int n, int bits) {
 
 template <int BITS, int ITEMS>
 static int sort_passes(hipStream_t s, int n, int keybits, uint32_t *key[2], int *val[2], uint32_t *hist,
-                       bool identity_vals) {
+                       bool identity_vals, const int *gate) {
     const int nblk = radix_blocks(n, ITEMS);
     const int R = 1 << BITS;
     uint32_t *total = hist + (size_t)R * nblk;
     int src = 0;
     for (int shift = 0; shift < keybits; shift += BITS) {
-        k_radix_hist<BITS, ITEMS><<<nblk, ZS_BLOCK, 0, s>>>(key[src], n, shift, hist, nblk);
-        k_radix_rowscan<<<R, ZS_BLOCK, 0, s>>>(hist, nblk, total);
+        k_radix_hist<BITS, ITEMS><<<nblk, ZS_BLOCK, 0, s>>>(key[src], n, shift, hist, nblk, gate);
+        k_radix_rowscan<<<R, ZS_BLOCK, 0, s>>>(hist, nblk, total, gate);
         k_radix_scatter<BITS, ITEMS><<<nblk, ZS_BLOCK, 0, s>>>(key[src], (identity_vals && shift == 0) ? nullptr : val[src],
-                                                        key[1 - src], val[1 - src], n, shift, hist, nblk, total);
+                                                        key[1 - src], val[1 - src], n, shift, hist, nblk, total, gate);
         src = 1 - src;
     }
     return src;
@@ -407,36 +465,46 @@ static int sort_passes(hipStream_t s, int n, int keybits, uint32_t *key[2], int 
 
 template <int ITEMS>
 static int sort_bits(hipStream_t s, int n, int keybits, int bits, uint32_t *key[2], int *val[2], uint32_t *hist,
-                     bool identity_vals) {
+                     bool identity_vals, const int *gate) {
     switch (bits) {
-    case 8: return sort_passes<8, ITEMS>(s, n, keybits, key, val, hist, identity_vals);
-    case 9: return sort_passes<9, ITEMS>(s, n, keybits, key, val, hist, identity_vals);
-    case 10: return sort_passes<10, ITEMS>(s, n, keybits, key, val, hist, identity_vals);
-    case 11: return sort_passes<11, ITEMS>(s, n, keybits, key, val, hist, identity_vals);
+    case 8: return sort_passes<8, ITEMS>(s, n, keybits, key, val, hist, identity_vals, gate);
+    case 9: return sort_passes<9, ITEMS>(s, n, keybits, key, val, hist, identity_vals, gate);
+    case 10: return sort_passes<10, ITEMS>(s, n, keybits, key, val, hist, identity_vals, gate);
+    case 11: return sort_passes<11, ITEMS>(s, n, keybits, key, val, hist, identity_vals, gate);
     default: return -1;
     }
 }
 
 int launch_sort(hipStream_t s, int n, int keybits, int bits, int items, uint32_t *key[2], int *val[2],
-                uint32_t *hist, bool identity_vals) {
+                uint32_t *hist, bool identity_vals, const int *gate) {
     if (n == 0) return 0;
-    if (items == 8) return sort_bits<8>(s, n, keybits, bits, key, val, hist, identity_vals);
-    if (items == 16) return sort_bits<16>(s, n, keybits, bits, key, val, hist, identity_vals);
+    if (items == 8) return sort_bits<8>(s, n, keybits, bits, key, val, hist, identity_vals, gate);
+    if (items == 16) return sort_bits<16>(s, n, keybits, bits, key, val, hist, identity_vals, gate);
     return -1;
 }
 
 static int cs_tiles(int G) { return (2 * G + 2 + CS_TILE - 1) / CS_TILE; }
 size_t cell_start_scratch(int G) { return (size_t)cs_tiles(G) + 1; }
 
-void launch_cell_start(hipStream_t s, const uint32_t *skey, int n, int G, uint32_t flag, int *cs, int *scratch) {
+void launch_cell_start(hipStream_t s, const uint32_t *skey, int n, int G, uint32_t flag, int *cs, int *scratch,
+                       const int *gate) {
     const int T = cs_tiles(G);
-    k_cs_bounds<<<nblocks(n), ZS_BLOCK, 0, s>>>(skey, n, G, flag, T, scratch);
-    k_cell_start<<<T, ZS_BLOCK, 0, s>>>(skey, G, flag, scratch, cs);
+    k_cs_bounds<<<nblocks(n), ZS_BLOCK, 0, s>>>(skey, n, G, flag, T, scratch, gate);
+    k_cell_start<<<T, ZS_BLOCK, 0, s>>>(skey, G, flag, scratch, cs, gate);
 }
 
-void launch_reorder(hipStream_t s, int n, const int *perm, const Buf &a, const Buf &b) {
+void launch_reorder(hipStream_t s, int n, const int *perm, const Buf &a, const Buf &b, const int *gate) {
     if (n == 0) return;
-    k_reorder<<<nblocks(n), ZS_BLOCK, 0, s>>>(n, perm, a, b);
+    k_reorder<<<nblocks(n), ZS_BLOCK, 0, s>>>(n, perm, a, b, gate);
+}
+
+void launch_skin_commit(hipStream_t s, int n, const Buf &a, const Buf &b, float4 *xref, const int *gate) {
+    if (n == 0) return;
+    k_skin_commit<<<nblocks(n), ZS_BLOCK, 0, s>>>(n, a, b, xref, gate);
+}
+
+void launch_skin_flag(hipStream_t s, int *flag, unsigned long long *rebuilds) {
+    k_skin_flag<<<1, 1, 0, s>>>(flag, rebuilds);
 }
 
 }  // namespace zs

@@ -379,8 +379,8 @@ __global__ __launch_bounds__(ZS_BLOCK) void k_governing_dummy(Prm q0, const Prm 
 // x += dt/2 v, v += dt/2 a, rho = max(rho + dt/2 drho, rho0), p = EOS(rho).
 template <bool BATCH>
 __global__ __launch_bounds__(ZS_BLOCK) void k_predict(Prm q0, const Prm *__restrict__ sp, Buf b,
-        const int *__restrict__ cs,
-                                                      const float4 *__restrict__ acc) {
+                                                      const int *__restrict__ cs, const float4 *__restrict__ acc,
+                                                      SkinCheck chk) {
     const int i = blockIdx.x * ZS_BLOCK + threadIdx.x;
     if (i >= cs[q0.G]) return;
     const Prm q = scene_prm<BATCH>(q0, sp, b, i);
@@ -394,14 +394,15 @@ __global__ __launch_bounds__(ZS_BLOCK) void k_predict(Prm q0, const Prm *__restr
     v.w = eos_p(q, x.w);
     b.posr[i] = x;
     b.velp[i] = v;
+    if (chk.xref) skin_flag_wave(chk, skin_moved(q, chk, i, x));
 }
 
 // Midpoint corrector (time_int.cu:50-91): x = x^n + dt v*, v = v^n + dt a*,
 // rho = max(rho^n + dt drho*, rho0), p = EOS(rho).
 template <bool BATCH>
 __global__ __launch_bounds__(ZS_BLOCK) void k_correct(Prm q0, const Prm *__restrict__ sp, Buf b,
-        const int *__restrict__ cs,
-                                                      const float4 *__restrict__ acc) {
+                                                      const int *__restrict__ cs, const float4 *__restrict__ acc,
+                                                      SkinCheck chk) {
     const int i = blockIdx.x * ZS_BLOCK + threadIdx.x;
     if (i >= cs[q0.G]) return;
     const Prm q = scene_prm<BATCH>(q0, sp, b, i);
@@ -414,6 +415,7 @@ __global__ __launch_bounds__(ZS_BLOCK) void k_correct(Prm q0, const Prm *__restr
     v.w = eos_p(q, x.w);
     b.posr[i] = x;
     b.velp[i] = v;
+    if (chk.xref) skin_flag_wave(chk, skin_moved(q, chk, i, x));
 }
 
 __device__ __forceinline__ uint64_t mix64(uint64_t z) {
@@ -423,8 +425,10 @@ __device__ __forceinline__ uint64_t mix64(uint64_t z) {
     return z ^ (z >> 31);
 }
 
-// Neighbour count and set digest sum_j mix64(j) (mod 2^64) per particle,
-// through the same walk as the physics kernels.
+// Neighbour count and set digest sum_j mix64(id_j) (mod 2^64) per particle
+// over the neighbours' original ids id_j = table[j] + j, so the digest does
+// not depend on the particle order; through the same walk as the physics
+// kernels.
 template <int WALK, bool BATCH>
 __global__ __launch_bounds__(ZS_BLOCK) void k_nbr_stats(Prm q0, const Prm *__restrict__ sp, Buf b,
         const uint32_t *__restrict__ skey,
@@ -439,7 +443,7 @@ __global__ __launch_bounds__(ZS_BLOCK) void k_nbr_stats(Prm q0, const Prm *__res
     for_each_neighbor<WALK, true, true>(q, i, xi, cell_of(q, skey[i]), cs, b.posr,
                                   [&](int j, float, float, float, float, float4) {
                                       ++c;
-                                      h += mix64((uint64_t)j);
+                                      h += mix64((uint64_t)(b.table[j] + j));
                                   });
     count[i] = c;
     digest[i] = h;
@@ -568,19 +572,21 @@ void launch_governing(hipStream_t s, int walk, int block, const Prm &q, const Pr
             k_governing_dummy<WALK_MASK, false><<<nblocks(q.N), ZS_BLOCK, 0, s>>>(q, sp, b, skey, cs, acc);
     }
 }
-void launch_predict(hipStream_t s, const Prm &q, const Prm *sp, const Buf &b, const int *cs, const float4 *acc) {
+void launch_predict(hipStream_t s, const Prm &q, const Prm *sp, const Buf &b, const int *cs, const float4 *acc,
+                    const SkinCheck &chk) {
     if (!q.N) return;
     if (sp)
-        k_predict<true><<<nblocks(q.N), ZS_BLOCK, 0, s>>>(q, sp, b, cs, acc);
+        k_predict<true><<<nblocks(q.N), ZS_BLOCK, 0, s>>>(q, sp, b, cs, acc, chk);
     else
-        k_predict<false><<<nblocks(q.N), ZS_BLOCK, 0, s>>>(q, sp, b, cs, acc);
+        k_predict<false><<<nblocks(q.N), ZS_BLOCK, 0, s>>>(q, sp, b, cs, acc, chk);
 }
-void launch_correct(hipStream_t s, const Prm &q, const Prm *sp, const Buf &b, const int *cs, const float4 *acc) {
+void launch_correct(hipStream_t s, const Prm &q, const Prm *sp, const Buf &b, const int *cs, const float4 *acc,
+                    const SkinCheck &chk) {
     if (!q.N) return;
     if (sp)
-        k_correct<true><<<nblocks(q.N), ZS_BLOCK, 0, s>>>(q, sp, b, cs, acc);
+        k_correct<true><<<nblocks(q.N), ZS_BLOCK, 0, s>>>(q, sp, b, cs, acc, chk);
     else
-        k_correct<false><<<nblocks(q.N), ZS_BLOCK, 0, s>>>(q, sp, b, cs, acc);
+        k_correct<false><<<nblocks(q.N), ZS_BLOCK, 0, s>>>(q, sp, b, cs, acc, chk);
 }
 void launch_nbr_stats(hipStream_t s, int walk, const Prm &q, const Prm *sp, const Buf &b, const uint32_t *skey,
                       const int *cs, int *count, uint64_t *digest) {

@@ -56,6 +56,14 @@ struct zsph_ctx {
     int walk = zs::WALK_MASK;  // neighbour-walk variant (tuning)
     int block = ZS_BLOCK;      // threads per block of the neighbour passes (tuning)
     int sort_tile = 16;        // keys per thread of the radix tile (tuning)
+    float skin = 0.0f;         // zsph_step's neighbour skin in h (tuning; 0: rebuild every half step)
+    zs::Prm qs{};              // the skin-mode neighbour grid (make_skin_prm)
+    int *rebuild = nullptr;    // device flag: the next skin-mode half step rebuilds
+    unsigned long long *rebuilds = nullptr;  // device count of skin-mode rebuilds
+    float4 *xref = nullptr;    // positions at the last skin-mode rebuild
+    bool skin_valid = false;   // buffer 0 is as zsph_step left it (or a skin rebuild is flagged)
+    bool skin_pending = false; // a rebuild is flagged: the skin structures do not describe buffer 0 yet
+    int skin_slot = 0;         // key slot holding the skin grid's sorted keys
     int *nlist = nullptr;      // WALK_LIST neighbour lists (list_words(N) ints) and counts
     int *ncount = nullptr;     // WALK_LIST counts / WALK_HITS overflow flags
     int *hits = nullptr;       // WALK_HITS hit masks (hits_slots(k) x N words)
@@ -217,37 +225,37 @@ unsigned long long *stage_clock(zsph_ctx *c, int st) {
 int *walk_record(zsph_ctx *c) { return c->walk == zs::WALK_HITS ? c->hits : c->nlist; }
 bool walk_records(int walk) { return walk == zs::WALK_LIST || walk == zs::WALK_HITS; }
 
-int stage_boundary(zsph_ctx *c, int b, Prof *pf = nullptr) {
+int stage_boundary(zsph_ctx *c, const zs::Prm &q, int b, Prof *pf = nullptr) {
     if (c->sorted_slot < 0) return set_err(c, ZSPH_E_STATE, "zsph_boundary called before zsph_sort_data");
     const uint32_t *sk = c->key[c->sorted_slot];
     {
         Scope sc(pf, ZSPH_ST_WALL);
         unsigned long long *ck = stage_clock(c, ZSPH_ST_WALL);
         if (ck) zs::launch_clock(c->stream, ck, true);
-        zs::launch_wall(c->stream, c->walk, c->block, c->q, c->scene_prm, c->buf[b], sk, c->cell_start);
+        zs::launch_wall(c->stream, c->walk, c->block, q, c->scene_prm, c->buf[b], sk, c->cell_start);
         if (ck) zs::launch_clock(c->stream, ck, false);
     }
     {
         Scope sc(pf, ZSPH_ST_LDG);
         unsigned long long *ck = stage_clock(c, ZSPH_ST_LDG);
         if (ck) zs::launch_clock(c->stream, ck, true);
-        zs::launch_ldg(c->stream, c->walk, c->block, c->q, c->scene_prm, c->buf[b], sk, c->cell_start, c->gd,
+        zs::launch_ldg(c->stream, c->walk, c->block, q, c->scene_prm, c->buf[b], sk, c->cell_start, c->gd,
                        walk_record(c), c->ncount, c->closure_on ? c->feat : nullptr);
         if (ck) zs::launch_clock(c->stream, ck, false);
     }
     c->list_valid = walk_records(c->walk);
     return check_launch(c, "boundary");
 }
-int stage_governing(zsph_ctx *c, int b, Prof *pf = nullptr) {
+int stage_governing(zsph_ctx *c, const zs::Prm &q, int b, Prof *pf = nullptr) {
     if (c->sorted_slot < 0) return set_err(c, ZSPH_E_STATE, "zsph_governing called before zsph_sort_data");
     {
         Scope sc(pf, ZSPH_ST_GOVERNING);
         // the recording walks need this half step's record; otherwise walk the cells
         const int walk = (walk_records(c->walk) && !c->list_valid) ? zs::WALK_MASK : c->walk;
         if (c->closure_on)  // scale factors of the fluid particles [0, cs[G])
-            zs::launch_mlp_infer(c->stream, c->mlp_P, c->mlp_pack, c->mlp_aux, c->feat, 0, c->cell_start + c->q.G,
-                                 c->scale, zs::mlp_infer_blocks(c->q.N));
-        zs::launch_governing(c->stream, walk, c->block, c->q, c->scene_prm, c->buf[b], c->key[c->sorted_slot], c->cell_start,
+            zs::launch_mlp_infer(c->stream, c->mlp_P, c->mlp_pack, c->mlp_aux, c->feat, 0, c->cell_start + q.G,
+                                 c->scale, zs::mlp_infer_blocks(q.N));
+        zs::launch_governing(c->stream, walk, c->block, q, c->scene_prm, c->buf[b], c->key[c->sorted_slot], c->cell_start,
                              c->gd, c->acc, walk_record(c), c->ncount, c->closure_on ? 1 : 0, c->scale, c->terms,
                              stage_clock(c, ZSPH_ST_GOVERNING), c->has_body || c->has_piston);
     }
@@ -257,17 +265,18 @@ int stage_governing(zsph_ctx *c, int b, Prof *pf = nullptr) {
 // Integrator of one half step on buffer b: the body's force sum and FP64
 // update, the fluid's predictor or corrector, then the moving dummies placed
 // at the new time.  Without moving dummies only the fluid kernel runs.
-int stage_integrate(zsph_ctx *c, int b, bool predictor, Prof *pf = nullptr) {
+int stage_integrate(zsph_ctx *c, const zs::Prm &q, int b, bool predictor, Prof *pf = nullptr,
+                    const zs::SkinCheck &chk = zs::SkinCheck{}) {
     Scope sc(pf, ZSPH_ST_INTEGRATE);
     const bool moving = c->has_body || c->has_piston;
     if (c->has_body)
-        zs::launch_body_reduce(c->stream, c->q, c->buf[b], c->cell_start, c->acc, c->body, c->body_part);
-    if (moving) zs::launch_body_update(c->stream, c->q, c->body, c->body_part, predictor);
+        zs::launch_body_reduce(c->stream, q, c->buf[b], c->cell_start, c->acc, c->body, c->body_part);
+    if (moving) zs::launch_body_update(c->stream, q, c->body, c->body_part, predictor);
     if (predictor)
-        zs::launch_predict(c->stream, c->q, c->scene_prm, c->buf[b], c->cell_start, c->acc);
+        zs::launch_predict(c->stream, q, c->scene_prm, c->buf[b], c->cell_start, c->acc, chk);
     else
-        zs::launch_correct(c->stream, c->q, c->scene_prm, c->buf[b], c->cell_start, c->acc);
-    if (moving) zs::launch_kinematics(c->stream, c->q, c->buf[b], c->cell_start, c->body);
+        zs::launch_correct(c->stream, q, c->scene_prm, c->buf[b], c->cell_start, c->acc, chk);
+    if (moving) zs::launch_kinematics(c->stream, q, c->buf[b], c->cell_start, c->body, chk);
     return check_launch(c, predictor ? "prediction" : "correction");
 }
 
@@ -283,9 +292,65 @@ int half_step(zsph_ctx *c, int from, int to, bool predictor, Prof *pf = nullptr)
     if ((r = stage_hash(c, from, pf))) return r;
     if ((r = stage_sort(c, pf))) return r;
     if ((r = stage_sort_data(c, from, to, pf))) return r;
-    if ((r = stage_boundary(c, to, pf))) return r;
-    if ((r = stage_governing(c, to, pf))) return r;
-    return stage_integrate(c, to, predictor, pf);
+    if ((r = stage_boundary(c, c->q, to, pf))) return r;
+    if ((r = stage_governing(c, c->q, to, pf))) return r;
+    return stage_integrate(c, c->q, to, predictor, pf);
+}
+
+// Skin mode's half step, always on buffer 0.  The rebuild kernels (hash on
+// the skin grid, sort, cell table, reorder into buffer 1 and back with the
+// reference positions) run only when the rebuild flag is set; the passes use
+// the skin grid; the integrator sets the flag again when a particle has
+// moved more than s/2 since the rebuild or left the grid.
+int skin_half_step(zsph_ctx *c, bool predictor, Prof *pf = nullptr) {
+    const zs::Prm &q = c->qs;
+    const int *gate = c->rebuild;
+    {
+        Scope sc(pf, ZSPH_ST_HASH);
+        zs::launch_hash(c->stream, q, nullptr, c->buf[0], c->key[0], c->idx[0], c->killed, gate);
+    }
+    int slot;
+    {
+        Scope sc(pf, ZSPH_ST_SORT);
+        slot = zs::launch_sort(c->stream, q.N, c->keybits, c->radix_bits, c->sort_tile, c->key, c->idx, c->hist,
+                               true, gate);
+    }
+    if (slot < 0) return set_err(c, ZSPH_E_ARG, "unsupported radix width %d / tile %d", c->radix_bits, c->sort_tile);
+    {
+        Scope sc(pf, ZSPH_ST_CELLS);
+        zs::launch_cell_start(c->stream, c->key[slot], q.N, q.G, q.flag, c->cell_start, c->cs_scratch, gate);
+    }
+    {
+        Scope sc(pf, ZSPH_ST_REORDER);
+        zs::launch_reorder(c->stream, q.N, c->idx[slot], c->buf[0], c->buf[1], gate);
+        zs::launch_skin_commit(c->stream, q.N, c->buf[1], c->buf[0], c->xref, gate);
+        zs::launch_skin_flag(c->stream, c->rebuild, c->rebuilds);
+    }
+    if (int r = check_launch(c, "skin rebuild")) return r;
+    c->sorted_slot = slot;
+    c->skin_slot = slot;
+    c->list_valid = false;
+    int r;
+    if ((r = stage_boundary(c, q, 0, pf))) return r;
+    if ((r = stage_governing(c, q, 0, pf))) return r;
+    zs::SkinCheck chk;
+    chk.xref = c->xref;
+    chk.flag = c->rebuild;
+    const float half = 0.5f * c->skin * c->q.h * 0.999f;
+    chk.lim2 = half * half;
+    return stage_integrate(c, q, 0, predictor, pf, chk);
+}
+
+// Skin mode: make the next half step rebuild (the state in buffer 0 changed
+// outside zsph_step, or its structures describe another grid).
+int skin_force_rebuild(zsph_ctx *c) {
+    static const int one = 1;
+    if (!c->rebuild) return ZSPH_OK;
+    HIPCHK(c, hipMemcpyAsync(c->rebuild, &one, sizeof(int), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    c->skin_valid = true;
+    c->skin_pending = true;
+    return ZSPH_OK;
 }
 
 void drop_graph(zsph_ctx *c) {
@@ -479,6 +544,7 @@ const char *zsph_last_error(zsph_ctx *c) { return c ? c->err.c_str() : "null con
 
 int zsph_upload(zsph_ctx *c, int b, const zsph_host_state *st) {
     if (!c || !st || !valid_buf(b) || !st->pos || !st->type) return set_err(c, ZSPH_E_ARG, "zsph_upload: bad argument");
+    c->skin_valid = false;  // buffer 0 changes (or, for a checkpoint, the next step rebuilds)
     const int N = c->q.N;
     std::vector<float4> posr(N), velp(N);
     for (int i = 0; i < N; ++i) {
@@ -697,54 +763,81 @@ int zsph_frame_wait(zsph_frame *f, zsph_host_state *st) {
 
 int zsph_hash(zsph_ctx *c, int b) {
     if (!c || !valid_buf(b)) return set_err(c, ZSPH_E_ARG, "zsph_hash: bad buffer");
+    c->skin_valid = false;  // buffer 0 changes (or, for a checkpoint, the next step rebuilds)
     return stage_hash(c, b);
 }
 int zsph_sort_index(zsph_ctx *c) {
     if (!c) return ZSPH_E_ARG;
+    c->skin_valid = false;  // buffer 0 changes (or, for a checkpoint, the next step rebuilds)
     return stage_sort(c);
 }
 int zsph_sort_data(zsph_ctx *c, int from, int to) {
     if (!c || !valid_buf(from) || !valid_buf(to)) return set_err(c, ZSPH_E_ARG, "zsph_sort_data: bad buffer");
+    c->skin_valid = false;  // buffer 0 changes (or, for a checkpoint, the next step rebuilds)
     return stage_sort_data(c, from, to);
 }
 int zsph_boundary(zsph_ctx *c, int b) {
     if (!c || !valid_buf(b)) return set_err(c, ZSPH_E_ARG, "zsph_boundary: bad buffer");
-    return stage_boundary(c, b);
+    c->skin_valid = false;  // buffer 0 changes (or, for a checkpoint, the next step rebuilds)
+    return stage_boundary(c, c->q, b);
 }
 int zsph_governing(zsph_ctx *c, int b) {
     if (!c || !valid_buf(b)) return set_err(c, ZSPH_E_ARG, "zsph_governing: bad buffer");
-    return stage_governing(c, b);
+    c->skin_valid = false;  // buffer 0 changes (or, for a checkpoint, the next step rebuilds)
+    return stage_governing(c, c->q, b);
 }
 int zsph_prediction(zsph_ctx *c, int b) {
     if (!c || !valid_buf(b)) return set_err(c, ZSPH_E_ARG, "zsph_prediction: bad buffer");
+    c->skin_valid = false;  // buffer 0 changes (or, for a checkpoint, the next step rebuilds)
     if (c->sorted_slot < 0) return set_err(c, ZSPH_E_STATE, "zsph_prediction called before zsph_sort_data");
     host_time(c, true);
-    return stage_integrate(c, b, true);
+    return stage_integrate(c, c->q, b, true);
 }
 int zsph_correction(zsph_ctx *c, int b) {
     if (!c || !valid_buf(b)) return set_err(c, ZSPH_E_ARG, "zsph_correction: bad buffer");
+    c->skin_valid = false;  // buffer 0 changes (or, for a checkpoint, the next step rebuilds)
     if (c->sorted_slot < 0) return set_err(c, ZSPH_E_STATE, "zsph_correction called before zsph_sort_data");
     host_time(c, false);
-    return stage_integrate(c, b, false);
+    return stage_integrate(c, c->q, b, false);
 }
 
 int zsph_step(zsph_ctx *c, int nsteps) {
     if (!c || nsteps < 0) return set_err(c, ZSPH_E_ARG, "zsph_step: bad argument");
     if (nsteps == 0 || c->q.N == 0) return ZSPH_OK;
-    if (!c->use_graph) {  // the same kernels, launched one by one
-        for (int s = 0; s < nsteps; ++s) {
-            host_time(c, true);
-            int r = half_step(c, 0, 1, true);
-            host_time(c, false);
-            if (!r) r = half_step(c, 1, 0, false);
-            if (r) return r;
+    const bool skin = c->skin > 0.0f;
+    if (skin && !c->skin_valid) {  // the first step after any outside change rebuilds
+        if (int r = skin_force_rebuild(c)) return r;
+    }
+    // one step: two half steps (skin mode: both on buffer 0)
+    auto one_step = [&](bool host_clock) {
+        if (host_clock) host_time(c, true);
+        int r = skin ? skin_half_step(c, true) : half_step(c, 0, 1, true);
+        if (host_clock) host_time(c, false);
+        if (!r) r = skin ? skin_half_step(c, false) : half_step(c, 1, 0, false);
+        return r;
+    };
+    auto finish = [&]() {
+        // after a step the sorted tables describe buffer 0 (skin mode: on the skin grid)
+        c->hashed = 0;
+        if (skin) {
+            c->cells_buf = -1;
+            c->sorted_slot = -1;
+            c->skin_valid = true;
+            c->skin_pending = false;
+        } else {
+            c->cells_buf = 0;
+            c->sorted_slot = (c->keybits + c->radix_bits - 1) / c->radix_bits % 2;
         }
+    };
+    if (!c->use_graph) {  // the same kernels, launched one by one
+        for (int s = 0; s < nsteps; ++s)
+            if (int r = one_step(true)) return r;
+        finish();
         return ZSPH_OK;
     }
     if (!c->graph_exec) {
         HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-        int r = half_step(c, 0, 1, true);
-        if (!r) r = half_step(c, 1, 0, false);
+        int r = one_step(false);
         hipGraph_t g = nullptr;
         hipError_t e = hipStreamEndCapture(c->stream, &g);
         if (r) {
@@ -761,10 +854,17 @@ int zsph_step(zsph_ctx *c, int nsteps) {
         host_time(c, true);
         host_time(c, false);
     }
-    // After a step the sorted tables describe buffer 0.
-    c->hashed = 0;
-    c->cells_buf = 0;
-    c->sorted_slot = (c->keybits + c->radix_bits - 1) / c->radix_bits % 2;
+    finish();
+    return ZSPH_OK;
+}
+
+int zsph_rebuilds(zsph_ctx *c, long long *n) {
+    if (!c || !n) return ZSPH_E_ARG;
+    *n = 0;
+    if (!c->rebuilds) return ZSPH_OK;
+    unsigned long long v = 0;
+    if (int r = copy_sync(c, &v, c->rebuilds, sizeof(v), hipMemcpyDeviceToHost)) return r;
+    *n = (long long)v;
     return ZSPH_OK;
 }
 
@@ -773,12 +873,26 @@ int zsph_set_tuning(zsph_ctx *c, const zsph_tuning *t) {
     const int block = t->block ? t->block : ZS_BLOCK;
     const int tile = t->sort_tile ? t->sort_tile : 16;
     if (t->walk < ZSPH_WALK_MASK || t->walk > ZSPH_WALK_LIST || t->radix_bits < 8 || t->radix_bits > 11 ||
-        (block != 128 && block != 256 && block != 512) || (tile != 8 && tile != 16))
-        return set_err(c, ZSPH_E_ARG, "zsph_set_tuning: walk %d / radix_bits %d / block %d / sort_tile %d out of range",
-                       t->walk, t->radix_bits, t->block, t->sort_tile);
-    if (t->walk == c->walk && t->radix_bits == c->radix_bits && block == c->block && tile == c->sort_tile)
+        (block != 128 && block != 256 && block != 512) || (tile != 8 && tile != 16) || !(t->skin >= 0.0f) ||
+        t->skin > 1.0f)
+        return set_err(c, ZSPH_E_ARG,
+                       "zsph_set_tuning: walk %d / radix_bits %d / block %d / sort_tile %d / skin %g out of range",
+                       t->walk, t->radix_bits, t->block, t->sort_tile, (double)t->skin);
+    if (t->skin > 0.0f && c->nscenes > 1)
+        return set_err(c, ZSPH_E_ARG, "zsph_set_tuning: the skin is for one-scene contexts");
+    if (t->walk == c->walk && t->radix_bits == c->radix_bits && block == c->block && tile == c->sort_tile &&
+        t->skin == c->skin)
         return ZSPH_OK;
     int r;
+    if (t->skin > 0.0f && !c->rebuild &&
+        ((r = dalloc(c, &c->rebuild, 1)) || (r = dalloc(c, &c->rebuilds, 1)) ||
+         (r = dalloc(c, &c->xref, (size_t)c->q.N))))
+        return r;
+    if (t->skin != c->skin) {
+        c->skin = t->skin;
+        if (c->skin > 0.0f) c->qs = zs::make_skin_prm(c->q, c->skin);
+        c->skin_valid = false;
+    }
     if (walk_records(t->walk) && !c->ncount && (r = dalloc(c, &c->ncount, (size_t)c->q.N))) return r;
     if (t->walk == ZSPH_WALK_LIST && !c->nlist && (r = dalloc(c, &c->nlist, zs::list_words(c->q.N)))) return r;
     if (t->walk == ZSPH_WALK_HITS && !c->hits) {
@@ -803,6 +917,7 @@ int zsph_get_tuning(zsph_ctx *c, zsph_tuning *t) {
     t->radix_bits = c->radix_bits;
     t->block = c->block;
     t->sort_tile = c->sort_tile;
+    t->skin = c->skin;
     return ZSPH_OK;
 }
 
@@ -845,18 +960,20 @@ int zsph_autotune(zsph_ctx *c, int steps, zsph_tuning *best, float *best_ms) {
     }
     const int passes = (c->keybits + 10) / 11;  // fewest passes of <= 11-bit digits
     const int even = std::max(8, (c->keybits + passes - 1) / passes);
-    zsph_tuning keep{c->walk, c->radix_bits, c->block, c->sort_tile};
+    zsph_tuning keep{c->walk, c->radix_bits, c->block, c->sort_tile, c->skin};
     float best_t = 1e30f;
     int r = ZSPH_OK;
-    // times one candidate from the saved state; keeps it if it is the fastest so far
-    auto trial = [&](const zsph_tuning &t) {
+    // times one candidate from the saved state over n steps; keeps it if it is the fastest so far
+    auto trial = [&](const zsph_tuning &t, int n) {
         if ((r = zsph_set_tuning(c, &t))) return;
         if ((e = copy_all(false)) != hipSuccess) {
             r = set_err(c, ZSPH_E_HIP, "autotune restore: %s", hipGetErrorString(e));
             return;
         }
+        c->skin_valid = false;  // the restored state rebuilds first
         float ms = 0.0f;
-        if ((r = zsph_time_steps(c, steps, &ms))) return;
+        if ((r = zsph_time_steps(c, n, &ms))) return;
+        ms *= (float)steps / (float)n;
         if (ms < best_t) {
             best_t = ms;
             keep = t;
@@ -864,17 +981,24 @@ int zsph_autotune(zsph_ctx *c, int steps, zsph_tuning *best, float *best_ms) {
     };
     // coordinate search: walk x digit width, then block size, then sort tile
     const zsph_tuning start = keep;
+    const int nskin = start.skin > 0.0f ? 4 * steps : steps;  // skin mode: time over more steps
     for (int walk = ZSPH_WALK_MASK; walk <= ZSPH_WALK_LIST && !r; ++walk) {
-        trial({walk, even, start.block, start.sort_tile});
-        if (even != 8 && !r) trial({walk, 8, start.block, start.sort_tile});
+        trial({walk, even, start.block, start.sort_tile, start.skin}, nskin);
+        if (even != 8 && !r) trial({walk, 8, start.block, start.sort_tile, start.skin}, nskin);
     }
     const int blocks[3] = {128, 256, 512};
     zsph_tuning base = keep;
     for (int k = 0; k < 3 && !r; ++k)
-        if (blocks[k] != base.block) trial({base.walk, base.radix_bits, blocks[k], base.sort_tile});
+        if (blocks[k] != base.block) trial({base.walk, base.radix_bits, blocks[k], base.sort_tile, base.skin}, nskin);
     base = keep;
     for (int tile = 8; tile <= 16 && !r; tile *= 2)
-        if (tile != base.sort_tile) trial({base.walk, base.radix_bits, base.block, tile});
+        if (tile != base.sort_tile) trial({base.walk, base.radix_bits, base.block, tile, base.skin}, nskin);
+    if (start.skin > 0.0f) {
+        base = keep;
+        const float skins[4] = {0.05f, 0.1f, 0.2f, 0.3f};
+        for (int k = 0; k < 4 && !r; ++k)
+            if (skins[k] != base.skin) trial({base.walk, base.radix_bits, base.block, base.sort_tile, skins[k]}, nskin);
+    }
     if (!r) r = zsph_set_tuning(c, &keep);
     e = copy_all(false);
     (void)hipFree(save);
@@ -883,6 +1007,7 @@ int zsph_autotune(zsph_ctx *c, int steps, zsph_tuning *best, float *best_ms) {
     c->sorted_slot = -1;  // buffer 0 is back to its unsorted starting state
     c->hashed = 0;
     c->cells_buf = -1;
+    c->skin_valid = false;
     if (r) return r;
     if (e != hipSuccess) return set_err(c, ZSPH_E_HIP, "autotune restore: %s", hipGetErrorString(e));
     if (best) *best = keep;
@@ -917,6 +1042,7 @@ int zsph_array_get(zsph_ctx *c, int which, int b, void *host, size_t bytes) {
 
 int zsph_array_set(zsph_ctx *c, int which, int b, const void *host, size_t bytes) {
     if (!c || !host || !valid_buf(b)) return set_err(c, ZSPH_E_ARG, "zsph_array_set: bad argument");
+    c->skin_valid = false;  // buffer 0 changes (or, for a checkpoint, the next step rebuilds)
     const size_t N = (size_t)c->q.N;
     void *dst = nullptr;
     size_t need = 0;
@@ -939,7 +1065,10 @@ int zsph_array_set(zsph_ctx *c, int which, int b, const void *host, size_t bytes
 
 int zsph_neighbor_stats(zsph_ctx *c, int b, int *count, uint64_t *digest) {
     if (!c || !valid_buf(b) || !count || !digest) return set_err(c, ZSPH_E_ARG, "zsph_neighbor_stats: bad argument");
-    if (c->sorted_slot < 0) return set_err(c, ZSPH_E_STATE, "zsph_neighbor_stats called before zsph_sort_data");
+    // after skin-mode steps: buffer 0 on the skin grid, the structures of the last rebuild
+    const bool skin = c->skin > 0.0f && c->skin_valid && !c->skin_pending && c->sorted_slot < 0 && c->hashed == 0;
+    if (skin && b != 0) return set_err(c, ZSPH_E_STATE, "zsph_neighbor_stats: skin-mode state is in buffer 0");
+    if (!skin && c->sorted_slot < 0) return set_err(c, ZSPH_E_STATE, "zsph_neighbor_stats called before zsph_sort_data");
     const size_t N = (size_t)c->q.N;
     int *dc = nullptr;
     uint64_t *dd = nullptr;
@@ -949,8 +1078,8 @@ int zsph_neighbor_stats(zsph_ctx *c, int b, int *count, uint64_t *digest) {
         (void)hipFree(dc);
         return set_err(c, ZSPH_E_NOMEM, "hipMalloc failed");
     }
-    zs::launch_nbr_stats(c->stream, c->walk, c->q, c->scene_prm, c->buf[b], c->key[c->sorted_slot], c->cell_start,
-                         dc, dd);
+    zs::launch_nbr_stats(c->stream, c->walk, skin ? c->qs : c->q, c->scene_prm, c->buf[b],
+                         c->key[skin ? c->skin_slot : c->sorted_slot], c->cell_start, dc, dd);
     e = hipGetLastError();
     if (e == hipSuccess && N) e = hipMemcpyAsync(count, dc, N * 4, hipMemcpyDeviceToHost, c->stream);
     if (e == hipSuccess && N) e = hipMemcpyAsync(digest, dd, N * 8, hipMemcpyDeviceToHost, c->stream);
@@ -984,11 +1113,19 @@ int zsph_profile_steps(zsph_ctx *c, int nsteps, float *stage_ms) {
     Prof pf;
     pf.s = c->stream;
     int r = ZSPH_OK;
+    const bool skin = c->skin > 0.0f;
+    if (skin && !c->skin_valid) r = skin_force_rebuild(c);
     for (int s = 0; s < nsteps && !r; ++s) {
         host_time(c, true);
-        r = half_step(c, 0, 1, true, &pf);
+        r = skin ? skin_half_step(c, true, &pf) : half_step(c, 0, 1, true, &pf);
         host_time(c, false);
-        if (!r) r = half_step(c, 1, 0, false, &pf);
+        if (!r) r = skin ? skin_half_step(c, false, &pf) : half_step(c, 1, 0, false, &pf);
+    }
+    if (skin && !r) {  // as after zsph_step
+        c->hashed = 0;
+        c->cells_buf = -1;
+        c->sorted_slot = -1;
+        c->skin_pending = false;
     }
     hipError_t e = hipStreamSynchronize(c->stream);
     for (int k = 0; k < ZSPH_ST_COUNT; ++k) stage_ms[k] = 0.0f;
@@ -1309,6 +1446,7 @@ int store_arrays(zsph_ctx *c, const HostArrays &h, const zs::DevBody &hb) {
     if (int e_ = copy_sync(c, d.tvelp, h.tvelp.data(), N * sizeof(float4), hipMemcpyHostToDevice)) return e_;
     if (int e_ = copy_sync(c, c->body, &hb, sizeof(hb), hipMemcpyHostToDevice)) return e_;
     drop_graph(c);  // the captured step does not contain the moving-dummy kernels
+    c->skin_valid = false;
     return ZSPH_OK;
 }
 
@@ -1502,6 +1640,7 @@ extern "C" {
 
 int zsph_checkpoint_write(zsph_ctx *c, const char *path, long long step) {
     if (!c || !path || !*path || step < 0) return set_err(c, ZSPH_E_ARG, "zsph_checkpoint_write: bad argument");
+    c->skin_valid = false;  // buffer 0 changes (or, for a checkpoint, the next step rebuilds)
     std::vector<CkptSection> secs = ckpt_layout(c, c->closure_on);
     size_t total = 0;
     for (const CkptSection &s : secs) total += s.bytes;
@@ -1602,6 +1741,7 @@ int zsph_checkpoint_read(zsph_ctx *c, const char *path, long long *step) {
     c->hashed = 0;
     c->cells_buf = -1;
     c->list_valid = false;
+    c->skin_valid = false;
     drop_graph(c);  // the captured step may lack (or hold) the moving-dummy kernels
     if (step) *step = m.step;
     return ZSPH_OK;
