@@ -539,6 +539,17 @@ int zsph_filter_fields(zsph_ctx *ctx, int buf, float radius, float hc, const int
 #define ZSPH_WALK_LIST 2  /* gradient pass records neighbour lists (in walk
                            * order) that the governing pass reuses; takes
                            * 512 B of device memory per particle          */
+#define ZSPH_WALK_SKIN 3  /* skin mode (skin > 0): each rebuild records
+                           * every particle's candidates within 2h + s as
+                           * hit masks; until the next rebuild the gradient
+                           * pass replays them with the exact predicate
+                           * instead of testing, and records the exact
+                           * masks the governing pass replays (another
+                           * 200 B per particle); elsewhere (skin 0, staged
+                           * calls, the closure) it runs as ZSPH_WALK_HITS.
+                           * Its rows lie around the rebuild positions, so
+                           * its neighbour order, and FP32 summation order,
+                           * differs from the other walks'                 */
 typedef struct zsph_tuning {
     int walk;        /* ZSPH_WALK_*                                          */
     int radix_bits;  /* sort digit width, 8..11                              */

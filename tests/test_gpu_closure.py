@@ -244,10 +244,13 @@ def test_pipeline_small_scale_learns_something():
 
 def test_trained_closure_gains_over_plain_coarse():
     # The committed network (trained on seeded columns other than this one) on the
-    # held-out 3-D dam break: coarse (dx 0.016) runs against the fine plain run
-    # (dx 0.008) up to t = 0.45 s.  The learned closure must reduce the front
-    # error of the plain coarse method and keep it within 2 %, at >= 5x less
-    # stepping time than the fine run.
+    # held-out 3-D dam break, through the first far-wall impact and run-up (t = 0.9 s;
+    # the surge reaches the wall near 0.62 s): coarse (dx 0.016) runs against the fine
+    # plain run (dx 0.008).  Gate: front error <= 2 % at >= 5x less stepping time,
+    # which the plain coarse method fails (2.14 %), and a smoothed (4 ms) peak
+    # pressure no further from the fine run's than the plain coarse one at every
+    # far-wall probe.  Measured: front 1.14 % (plain 2.14 %); peak-pressure errors
+    # 5 / 419 / 11 / 16 % at z = 0.01 / 0.05 / 0.10 / 0.20 m (plain 23 / 746 / 44 / 28 %).
     import os
     from zsph_amd import cases
     path = os.path.join(os.path.dirname(closure.__file__), "data", "closure_dambreak3d.npz")
@@ -255,14 +258,20 @@ def test_trained_closure_gains_over_plain_coarse():
     fine = cases.dambreak3d(dx=0.008, grid_size_factor=2.0)
     coarse = cases.dambreak3d(dx=0.016, grid_size_factor=2.0)
     xw = closure.far_wall(fine)
-    say(f"[closure gain] fine N={fine.n}, coarse N={coarse.n}")
-    hf = closure.run_history(fine, 0.45, sample_dt=2e-3)
+    t_end = 0.9
+    say(f"[closure gain] fine N={fine.n}, coarse N={coarse.n}, to t = {t_end} s")
+    hf = closure.run_history(fine, t_end, log=say)
     say(f"  fine: {hf['steps']} steps, {hf['ms_per_step']:.3f} ms/step")
-    hp = closure.run_history(coarse, 0.45, sample_dt=2e-3)
-    hl = closure.run_history(coarse, 0.45, net=net, sample_dt=2e-3)
+    hp = closure.run_history(coarse, t_end, log=say)
+    hl = closure.run_history(coarse, t_end, net=net, log=say)
     plain, learned = closure.compare(hf, hp, xw), closure.compare(hf, hl, xw)
-    say(f"  front error: plain {plain['front_err_max']:.4f}, learned {learned['front_err_max']:.4f}; "
-        f"stepping time ratio: plain {plain['speedup']:.1f}x, learned {learned['speedup']:.1f}x")
-    assert learned["front_err_max"] < plain["front_err_max"]
-    assert learned["front_err_max"] <= 0.02
-    assert learned["speedup"] >= 5.0
+    for name, r in (("plain", plain), ("learned", learned)):
+        say(f"  {name}: front error {r['front_err_max']:.4f}, smoothed peak p error per probe "
+            f"{np.round(r['smoothed_peak_p_err'], 4).tolist()}, {r['speedup']:.1f}x less stepping time")
+
+    def gate(r):
+        return r["front_err_max"] <= 0.02 and r["speedup"] >= 5.0
+
+    assert gate(learned)
+    assert not gate(plain)
+    assert np.all(np.asarray(learned["smoothed_peak_p_err"]) <= np.asarray(plain["smoothed_peak_p_err"]))

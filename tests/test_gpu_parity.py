@@ -286,6 +286,7 @@ def test_results_do_not_depend_on_tuning(name):
     # and the smaller sort tile (each particle's arithmetic does not depend on its block)
     tunings = [(w, bits, 256, 16) for w in (z._abi.WALK_MASK, z._abi.WALK_HITS, z._abi.WALK_LIST) for bits in (8, 11)]
     tunings += [(z._abi.WALK_HITS, 8, 128, 16), (z._abi.WALK_MASK, 9, 512, 8), (z._abi.WALK_LIST, 10, 128, 8)]
+    tunings += [(z._abi.WALK_SKIN, 8, 256, 16)]  # without a skin the skin walk runs as the hit-mask walk
     for walk, bits, block, tile in tunings:
         s = z.Solver(c)
         s.set_tuning(walk, bits, block, tile)

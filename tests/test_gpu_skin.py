@@ -13,6 +13,9 @@ to 2h + s/2.
 * A particle leaving the grid is removed at the same step as with skin 0.
 * Deterministic: graph replay equals direct launches bit for bit; a restart
   from a checkpoint equals the writing run continued.
+* The recorded skin walk (ZSPH_WALK_SKIN: skin masks recorded at a rebuild,
+  replayed with the exact predicate until the next) matches the oracle after
+  one step and tracks the tested walk over ten steps.
 * With a floating body and a piston (moving dummies) the coupled state stays
   within tolerance of the reference schedule.
 """
@@ -52,10 +55,14 @@ def skin_solver(c, skin=SKIN, walk=None):
     return s
 
 
+@pytest.mark.parametrize("walk", [z.WALK_HITS, z.WALK_SKIN])
 @pytest.mark.parametrize("name", ["2d", "3d-small", "3d-k2"])
-def test_skin_one_step_matches_oracle(name):
+def test_skin_one_step_matches_oracle(name, walk):
+    # WALK_SKIN: the first half step records the skin masks (and tests), the second
+    # replays them; "3d-small" (cells of 2h) has rows beyond the record, whose
+    # particles take the tested walk
     c = CASES[name]()
-    s = skin_solver(c)
+    s = skin_solver(c, walk=walk)
     s.step(1)
     s.sync()
     assert s.rebuilds() == 1, "the second half step should run on the first one's structures"
@@ -154,11 +161,32 @@ def test_skin_removal_at_the_reference_step():
     assert runs[0.0] == runs[SKIN] and runs[0.0][-1] >= 1
 
 
+def test_skin_walk_tracks_the_tested_walk():
+    # the recorded skin walk visits the same neighbours as the tested walks (other order):
+    # over 10 skin-mode steps both stay within FP32 summation-order noise of each other
+    c = CASES["3d-k2"]()
+    out = {}
+    for walk in (z.WALK_HITS, z.WALK_SKIN):
+        s = skin_solver(c, walk=walk)
+        s.step(10)
+        s.sync()
+        h = s.download(0)
+        order = by_original_id(h)
+        out[walk] = (h.pos[order], h.rhop[order], s.rebuilds())
+        s.close()
+    say(f"  rebuilds: hits {out[z.WALK_HITS][2]}, skin walk {out[z.WALK_SKIN][2]} in 20 half steps")
+    assert out[z.WALK_SKIN][2] < 20
+    dx = np.abs(out[z.WALK_SKIN][0] - out[z.WALK_HITS][0]).max()
+    drho = np.abs(out[z.WALK_SKIN][1][:, 0] - out[z.WALK_HITS][1][:, 0]).max()
+    say(f"  max |dx| {dx:.3e} m ({dx / c.json.dx:.2e} dx), max |drho| {drho:.3e}")
+    assert dx < 1e-5 * c.json.dx and drho < 1e-5 * c.json.rho0
+
+
 def test_skin_graph_replay_equals_direct_launches_and_restart(tmp_path):
     c = CASES["3d-k2"]()
     out = []
     for graph in (True, False):
-        s = skin_solver(c, walk=z.WALK_HITS)
+        s = skin_solver(c, walk=z.WALK_SKIN)
         s.set_graph(graph)
         s.step(6)
         s.sync()
@@ -169,14 +197,14 @@ def test_skin_graph_replay_equals_direct_launches_and_restart(tmp_path):
         assert np.array_equal(a, b)
     # restart: a checkpoint makes the writer's next step rebuild, as the restored context's first
     path = str(tmp_path / "skin.bin")
-    s = skin_solver(c)
+    s = skin_solver(c, walk=z.WALK_SKIN)
     s.step(5)
     s.checkpoint(path, 5)
     s.step(7)
     s.sync()
     ref = (s.get(A_POSR, 0), s.get(A_VELP, 0))
     s.close()
-    t = skin_solver(c)
+    t = skin_solver(c, walk=z.WALK_SKIN)
     t.restore(path)
     t.step(7)
     t.sync()

@@ -14,7 +14,7 @@ import numpy as np
 from . import _abi as abi
 from ._abi import (A_ACC, A_CELL_START, A_GD, A_INDEX, A_KEY, A_POSR, A_TABLE, A_TPOSR, A_TVELP,
                    A_TYPE, A_VELP, BODY, DIFF_ANTUONO, DIFF_MC, F_ACC, F_ALL, F_HASH, F_POS, F_RHOP,
-                   F_TABLE, F_TYPE, F_VEL, FLUID, KILLED, PISTON, WALL, WALK_LIST, WALK_MASK, WALK_HITS,
+                   F_TABLE, F_TYPE, F_VEL, FLUID, KILLED, PISTON, WALL, WALK_LIST, WALK_MASK, WALK_HITS, WALK_SKIN,
                    ST_GOVERNING, ST_LDG, ST_WALL)
 
 _HERE = os.path.dirname(os.path.abspath(__file__))

@@ -92,7 +92,7 @@ class Tuning(C.Structure):
     _fields_ = [("walk", _i), ("radix_bits", _i), ("block", _i), ("sort_tile", _i), ("skin", _f)]
 
 
-WALK_MASK, WALK_HITS, WALK_LIST = 0, 1, 2
+WALK_MASK, WALK_HITS, WALK_LIST, WALK_SKIN = 0, 1, 2, 3
 
 NF, NH, NT = 24, 64, 12
 PROBE_NOALLOC_MAX = 4096  # ZSPH_PROBE_NOALLOC_MAX

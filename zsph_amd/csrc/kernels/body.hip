@@ -6,7 +6,9 @@
 //                       zero there; prescribed here by the body / piston motion)
 // Per half step, after the continuity+momentum pass:
 //   k_body_reduce   fluid force and torque on the body, FP64 per-block partials
-//                   over the dummy segment (fixed block count, fixed order);
+//                   over the dummy segment (fixed block count; each wave sums
+//                   its lanes with a wave64 shuffle butterfly, then the block
+//                   sums its waves in order: a fixed order throughout);
 //   k_body_update   one wave: sums the partials in block order, adds M g,
 //                   accelerations, midpoint predictor / corrector of the FP64
 //                   state, device time;
@@ -28,7 +30,7 @@ __device__ __forceinline__ V3 f3(float4 a) { return {(double)a.x, (double)a.y, (
 __global__ __launch_bounds__(ZS_BLOCK) void k_body_reduce(Prm q, Buf b, const int *__restrict__ cs,
                                                           const float4 *__restrict__ acc,
                                                           const DevBody *__restrict__ body, double *__restrict__ part) {
-    __shared__ double red[BODY_D][ZS_BLOCK];
+    __shared__ double red[BODY_D][ZS_BLOCK / 64];
     const int nf = cs[q.G];
     const V3 c = rb::v3(body->s.c);
     const double m = (double)q.m;
@@ -40,14 +42,21 @@ __global__ __launch_bounds__(ZS_BLOCK) void k_body_reduce(Prm q, Buf b, const in
         s[0] += f.x; s[1] += f.y; s[2] += f.z;
         s[3] += t.x; s[4] += t.y; s[5] += t.z;
     }
-    for (int k = 0; k < BODY_D; ++k) red[k][threadIdx.x] = s[k];
+    // wave64 butterfly in FP64 (a fixed pattern: bitwise reproducible), then the
+    // block's waves in order
+#pragma unroll
+    for (int k = 0; k < BODY_D; ++k)
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) s[k] += __shfl_xor(s[k], o, 64);
+    const int w = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0)
+        for (int k = 0; k < BODY_D; ++k) red[k][w] = s[k];
     __syncthreads();
-    for (int w = ZS_BLOCK / 2; w > 0; w >>= 1) {
-        if ((int)threadIdx.x < w)
-            for (int k = 0; k < BODY_D; ++k) red[k][threadIdx.x] += red[k][threadIdx.x + w];
-        __syncthreads();
+    if (threadIdx.x < BODY_D) {
+        double t = 0.0;
+        for (int ww = 0; ww < ZS_BLOCK / 64; ++ww) t += red[threadIdx.x][ww];
+        part[blockIdx.x * BODY_D + threadIdx.x] = t;
     }
-    if (threadIdx.x < BODY_D) part[blockIdx.x * BODY_D + threadIdx.x] = red[threadIdx.x][0];
 }
 
 // One wave.  Lane k < 6 sums partial k over the blocks in block order; lane 0

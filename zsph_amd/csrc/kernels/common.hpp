@@ -36,6 +36,11 @@ struct Prm {
     // the case's grid ends (lib/hash.cu:16-23).
     float kill_size;
     int kill_X, kill_Y, kill_Z;
+    // Skin mode: the rows of the recorded skin walk (WALK_SKIN) are trimmed to
+    // rc + s around a particle's rebuild position (trim2_skin, cell units,
+    // padded like trim2), and its rebuild records every candidate within
+    // rc + s (rcs2 = (rc + s)^2); 0 outside skin mode.
+    float trim2_skin, rcs2;
 };
 
 // Host-side construction (also used by the C ABI layer).
@@ -86,6 +91,8 @@ inline Prm make_prm(const zsph_params &p) {
     q.kill_X = q.X;
     q.kill_Y = q.Y;
     q.kill_Z = q.Z;
+    q.trim2_skin = 0.0f;
+    q.rcs2 = 0.0f;
     return q;
 }
 
@@ -114,6 +121,9 @@ inline Prm make_skin_prm(const Prm &q, float skin) {
     s.G = s.XY * s.Z;
     const double rt = (rc + 0.5 * sk) / e;  // trimming radius in cell units
     s.trim2 = (float)(rt * rt * 1.002);
+    const double rs = (rc + sk) / e;  // the recorded skin walk's rows
+    s.trim2_skin = (float)(rs * rs * 1.002);
+    s.rcs2 = (float)((rc + sk) * (rc + sk));
     return s;
 }
 
@@ -183,6 +193,15 @@ __host__ __device__ inline bool cell_axis(float x, float o, float size, int dim,
     *c = (int)t;
     return true;
 }
+
+// WALK_SKIN's extra inputs of the gradient pass: the rebuild positions, the
+// skin hit masks of the last rebuild (slot-major like the exact record) and
+// the flag telling whether this half step rebuilt (gates the two forms).
+struct SkinRec {
+    const float4 *xref = nullptr;
+    uint32_t *skin = nullptr;
+    const int *rebuilt = nullptr;
+};
 
 // Skin mode's displacement test of the integrators (xref == nullptr: off):
 // a particle that has moved more than s/2 from its position at the last
@@ -386,6 +405,17 @@ __device__ __forceinline__ void for_each_row(const Prm &q, float4 xi, uint32_t c
 constexpr int WALK_MASK = 0;
 constexpr int WALK_HITS = 1;
 constexpr int WALK_LIST = 2;
+// WALK_SKIN (skin mode only): the gradient pass of a rebuild half step tests
+// each particle's candidates against (rc + s)^2 from its rebuild position,
+// records those skin hit masks, and visits them with the exact predicate;
+// between rebuilds it replays the skin masks instead of testing.  Either way
+// it records the exact hit masks, which the continuity+momentum pass replays
+// as in WALK_HITS.  All its rows lie around the rebuild position, trimmed to
+// rc + s.  The two gradient-pass forms are separate launches gated on
+// whether this half step rebuilt (WALK_SKIN_REC, WALK_SKIN_REPLAY).
+constexpr int WALK_SKIN = 3;
+constexpr int WALK_SKIN_REC = 4;
+constexpr int WALK_SKIN_REPLAY = 5;
 constexpr int LIST_MAX = 128;
 // List layout: 4-entry pieces (16 bytes), interleaved per wave: quad q
 // (entries 4q .. 4q+3) of particle i is at int4 index
@@ -523,9 +553,21 @@ __device__ __forceinline__ void for_each_neighbor(const Prm &q, int i, float4 xi
 // (row pairs merged the same way).  Every row of the particle must have at
 // most 64 candidates.
 template <bool FLUID_J, bool DUMMY_J, class F>
+__device__ __forceinline__ void for_each_recorded_at(const Prm &q, int i, float4 xi, float4 xc, uint32_t cell,
+                                                     const int *__restrict__ cs, const float4 *__restrict__ posr,
+                                                     const uint32_t *__restrict__ hits, size_t n, F &&f);
+template <bool FLUID_J, bool DUMMY_J, class F>
 __device__ __forceinline__ void for_each_recorded(const Prm &q, int i, float4 xi, uint32_t cell,
                                                   const int *__restrict__ cs, const float4 *__restrict__ posr,
                                                   const uint32_t *__restrict__ hits, size_t n, F &&f) {
+    for_each_recorded_at<FLUID_J, DUMMY_J>(q, i, xi, xi, cell, cs, posr, hits, n, f);
+}
+// The same with the rows trimmed around xc (WALK_SKIN: the rebuild position)
+// while the pair geometry uses the current position xi.
+template <bool FLUID_J, bool DUMMY_J, class F>
+__device__ __forceinline__ void for_each_recorded_at(const Prm &q, int i, float4 xi, float4 xc, uint32_t cell,
+                                                     const int *__restrict__ cs, const float4 *__restrict__ posr,
+                                                     const uint32_t *__restrict__ hits, size_t n, F &&f) {
     auto visit = [&](int j) {
         const float4 xj = ld(posr, j);
         const float dx = xi.x - xj.x, dy = xi.y - xj.y, dz = xi.z - xj.z;
@@ -547,7 +589,7 @@ __device__ __forceinline__ void for_each_recorded(const Prm &q, int i, float4 xi
     // the first masks of the pair's rows, loaded with their cell-table entries
     uint32_t mA = 0, mB = 0;
     for_each_row_pair<FLUID_J, DUMMY_J>(
-        q, xi, cell, cs,
+        q, xc, cell, cs,
         [&](const RowRange &A, int rA, const RowRange &B, int rB) {
             const int nA = A.fe - A.fb, tA = nA + (A.de - A.db);
             const int nB = B.fe - B.fb, tB = nB + (B.de - B.db);
@@ -565,6 +607,125 @@ __device__ __forceinline__ void for_each_recorded(const Prm &q, int i, float4 xi
             const uint32_t m = hits[(size_t)(2 * r) * n + (size_t)i];
             if (side) mB = m;
             else mA = m;
+        });
+}
+
+// WALK_SKIN's gradient-pass walk of particle i (current position xi, rebuild
+// position xc): the rows of the box around its rebuild cell trimmed to
+// rc + s around xc (qr.trim2 = trim2_skin).  REPLAY = false (the rebuild
+// half step): each chunk's candidates are tested against qr.rcs2 from xc and
+// the skin mask goes to rec_skin(r, c, m); REPLAY = true: the skin mask is
+// read from skin[(2 r + c) n + i] (every row of the particle within 64
+// candidates).  Every skin hit is visited: f(j, dx, dy, dz, r2, xj) returns
+// whether j is a neighbour (the caller applies r^2 <= rc2), and the chunk's
+// exact mask goes to rec_exact(r, c, m).  Row pairs and chunks are visited
+// as in for_each_neighbor, so the exact record replays in the same order.
+template <bool REPLAY, class F, class RS, class RE>
+__device__ __forceinline__ void skin_walk(const Prm &qr, int i, float4 xi, float4 xc, uint32_t cell,
+                                          const int *__restrict__ cs, const float4 *__restrict__ posr,
+                                          const uint32_t *__restrict__ skin, size_t n, F &&f, RS &&rec_skin,
+                                          RE &&rec_exact) {
+    auto visit = [&](int j) -> bool {
+        const float4 xj = ld(posr, j);
+        const float dx = xi.x - xj.x, dy = xi.y - xj.y, dz = xi.z - xj.z;
+        return f(j, dx, dy, dz, dist2(dx, dy, dz), xj);
+    };
+    // skin mask of candidates [base, base + cnt) (cnt <= 32), tested from xc
+    auto test_chunk = [&](int base, int cnt, int fb, int nfl, int dbo) {
+        uint32_t m = 0;
+        for (int t = 0; t < cnt; t += 4) {
+            float3 xs[4];
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                const int u = base + t + e;
+                xs[e] = ld_xyz(posr, ((u < nfl) ? fb : dbo) + u);
+            }
+            __builtin_amdgcn_sched_barrier(0);
+            uint32_t nib = 0;
+#pragma unroll
+            for (int e = 0; e < 4; ++e)
+                nib |= (dist2(xc.x - xs[e].x, xc.y - xs[e].y, xc.z - xs[e].z) <= qr.rcs2) ? (1u << e) : 0u;
+            m |= nib << t;
+        }
+        return (cnt < 32) ? m & ((1u << cnt) - 1u) : m;
+    };
+    auto self_place = [&](const RowRange &w) {
+        return (i >= w.fb && i < w.fe) ? i - w.fb : (i >= w.db && i < w.de) ? (w.fe - w.fb) + (i - w.db) : -1;
+    };
+    auto row_chunks = [&](const RowRange &w, int r, uint32_t m0) {
+        const int nfl = w.fe - w.fb;
+        const int total = nfl + (w.de - w.db);
+        const int dbo = w.db - nfl;
+        const int us = self_place(w);
+        for (int base = 0, c = 0; base < total; base += 32, ++c) {
+            uint32_t m;
+            if constexpr (REPLAY) {
+                m = c ? skin[(size_t)(2 * r + c) * n + (size_t)i] : m0;
+            } else {
+                m = test_chunk(base, min(32, total - base), w.fb, nfl, dbo);
+                if ((uint32_t)(us - base) < 32u) m &= ~(1u << (us - base));
+                rec_skin(r, c, m);
+            }
+            uint32_t e = 0;
+            while (m) {
+                const int b = __builtin_ctz(m);
+                m &= m - 1u;
+                const int u = base + b;
+                if (visit(((u < nfl) ? w.fb : dbo) + u)) e |= 1u << b;
+            }
+            rec_exact(r, c, e);
+        }
+    };
+    uint32_t pA = 0, pB = 0;  // REPLAY: the first skin masks of the pair's rows
+    for_each_row_pair<true, true>(
+        qr, xc, cell, cs,
+        [&](const RowRange &A, int rA, const RowRange &B, int rB) {
+            const int nA = A.fe - A.fb, tA = nA + (A.de - A.db);
+            const int nB = B.fe - B.fb, tB = nB + (B.de - B.db);
+            uint32_t mA = 0, mB = 0;
+            if constexpr (REPLAY) {  // a row without candidates has no record
+                mA = tA ? pA : 0u;
+                mB = tB ? pB : 0u;
+            }
+            if (tA <= 32 && tB <= 32) {
+                const int dA = A.db - nA, dB = B.db - nB;
+                if constexpr (!REPLAY) {
+                    if (tA) {
+                        mA = test_chunk(0, tA, A.fb, nA, dA);
+                        const int us = self_place(A);
+                        if ((uint32_t)us < 32u) mA &= ~(1u << us);
+                        rec_skin(rA, 0, mA);
+                    }
+                    if (tB) {
+                        mB = test_chunk(0, tB, B.fb, nB, dB);
+                        const int us = self_place(B);
+                        if ((uint32_t)us < 32u) mB &= ~(1u << us);
+                        rec_skin(rB, 0, mB);
+                    }
+                }
+                uint64_t m = (uint64_t)mA | ((uint64_t)mB << 32), e = 0;
+                while (m) {
+                    const int bit = __builtin_ctzll(m);
+                    m &= m - 1u;
+                    const bool hi = bit >= 32;
+                    const int u = bit & 31;
+                    const int nfl = hi ? nB : nA;
+                    const int j = ((u < nfl) ? (hi ? B.fb : A.fb) : (hi ? dB : dA)) + u;
+                    if (visit(j)) e |= 1ull << bit;
+                }
+                if (tA) rec_exact(rA, 0, (uint32_t)e);
+                if (tB) rec_exact(rB, 0, (uint32_t)(e >> 32));
+            } else {
+                row_chunks(A, rA, mA);
+                if (rB >= 0) row_chunks(B, rB, mB);
+            }
+        },
+        [&](int r, int side) {
+            if constexpr (REPLAY) {
+                const uint32_t m = skin[(size_t)(2 * r) * n + (size_t)i];
+                if (side) pB = m;
+                else pA = m;
+            }
         });
 }
 

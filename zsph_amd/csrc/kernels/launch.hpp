@@ -32,7 +32,7 @@ void launch_reorder(hipStream_t s, int n, const int *perm, const Buf &a, const B
 // Skin mode: copy the reordered buffer a into the working buffer b and keep
 // the positions in xref (gated); count and clear the rebuild flag.
 void launch_skin_commit(hipStream_t s, int n, const Buf &a, const Buf &b, float4 *xref, const int *gate);
-void launch_skin_flag(hipStream_t s, int *flag, unsigned long long *rebuilds);
+void launch_skin_flag(hipStream_t s, int *flag, unsigned long long *rebuilds, int *rebuilt);
 
 // physics.hip; walk = WALK_MASK, WALK_HITS or WALK_LIST (common.hpp).
 // WALK_LIST: launch_ldg records the neighbour lists (nlist: list_words(N)
@@ -52,9 +52,16 @@ void launch_ldg(hipStream_t s, int walk, int block, const Prm &q, const Prm *sp,
 // clock (may be nullptr): device stamps around the continuity+momentum kernel (launch_clock).
 // moving: the context has moving dummies (a body or a piston), whose
 // pressure force k_governing_dummy computes; fixed walls get zero either way.
+// xref (WALK_SKIN only): the rebuild positions its rows are centred on.
 void launch_governing(hipStream_t s, int walk, int block, const Prm &q, const Prm *sp, const Buf &b, const uint32_t *skey,
                       const int *cs, const float4 *gd, float4 *acc, const int *nlist, const int *ncount, int mode,
-                      const float2 *scale, float *terms, unsigned long long *clock, bool moving);
+                      const float2 *scale, float *terms, unsigned long long *clock, bool moving,
+                      const float4 *xref = nullptr);
+// WALK_SKIN's gradient pass (one scene, plain): the recording form on rebuild
+// half steps, the replaying form on the others (gated on *sr.rebuilt);
+// hits/ncount: the exact record and the per-particle "not recorded" flags.
+void launch_ldg_skin(hipStream_t s, int block, const Prm &q, const Buf &b, const uint32_t *skey, const int *cs,
+                     float4 *gd, int *hits, int *ncount, const SkinRec &sr);
 // chk (skin mode): the displacement test after the update (chk.xref == nullptr: none).
 void launch_predict(hipStream_t s, const Prm &q, const Prm *sp, const Buf &b, const int *cs, const float4 *acc,
                     const SkinCheck &chk = SkinCheck{});

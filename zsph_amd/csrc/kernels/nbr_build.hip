@@ -420,9 +420,13 @@ __global__ __launch_bounds__(ZS_BLOCK) void k_skin_commit(int n, Buf a, Buf b, f
 }
 
 // Skin mode, after the rebuild kernels of a half step: count a rebuild that
-// happened and clear the flag (the integrator sets it again when needed).
-__global__ void k_skin_flag(int *__restrict__ flag, unsigned long long *__restrict__ rebuilds) {
-    if (*flag) ++*rebuilds;
+// happened, tell the passes whether it did, and clear the flag (the
+// integrator sets it again when needed).
+__global__ void k_skin_flag(int *__restrict__ flag, unsigned long long *__restrict__ rebuilds,
+                            int *__restrict__ rebuilt) {
+    const int f = *flag;
+    if (f) ++*rebuilds;
+    *rebuilt = f;  // read by the passes of this half step (WALK_SKIN)
     *flag = 0;
 }
 
@@ -503,8 +507,8 @@ void launch_skin_commit(hipStream_t s, int n, const Buf &a, const Buf &b, float4
     k_skin_commit<<<nblocks(n), ZS_BLOCK, 0, s>>>(n, a, b, xref, gate);
 }
 
-void launch_skin_flag(hipStream_t s, int *flag, unsigned long long *rebuilds) {
-    k_skin_flag<<<1, 1, 0, s>>>(flag, rebuilds);
+void launch_skin_flag(hipStream_t s, int *flag, unsigned long long *rebuilds, int *rebuilt) {
+    k_skin_flag<<<1, 1, 0, s>>>(flag, rebuilds, rebuilt);
 }
 
 }  // namespace zs

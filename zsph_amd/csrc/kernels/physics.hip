@@ -84,7 +84,13 @@ template <int WALK, bool BATCH, bool FEAT, int DF, int BS>
 __global__ __launch_bounds__(BS, ((WALK != WALK_LIST && !BATCH && !FEAT) ? 7 : 1)) void k_ldg(Prm q0, const Prm *__restrict__ sp, Buf b,
                                                   const uint32_t *__restrict__ skey, const int *__restrict__ cs,
                                                   float4 *__restrict__ gd, int *__restrict__ nlist,
-                                                  int *__restrict__ ncount, float *__restrict__ feat) {
+                                                  int *__restrict__ ncount, float *__restrict__ feat, SkinRec sr) {
+    // WALK_SKIN: the recording form runs on rebuild half steps, the replaying form on the others
+    if constexpr (WALK == WALK_SKIN_REC) {
+        if (*sr.rebuilt == 0) return;
+    } else if constexpr (WALK == WALK_SKIN_REPLAY) {
+        if (*sr.rebuilt != 0) return;
+    }
     const int nf = cs[q0.G];
     const int i = xcd_block(blocks_of_bs<BS>(nf)) * BS + threadIdx.x;
     if (i >= q0.N) return;
@@ -152,6 +158,35 @@ __global__ __launch_bounds__(BS, ((WALK != WALK_LIST && !BATCH && !FEAT) ? 7 : 1
                                                      walk_body(j, dx, dy, dz, r2, xj);
                                                  });
         ncount[i] = lw.finish();
+    } else if constexpr (WALK == WALK_SKIN_REC || WALK == WALK_SKIN_REPLAY) {
+        uint32_t *hits = reinterpret_cast<uint32_t *>(nlist);  // this half step's exact masks
+        const size_t n = (size_t)q0.N;
+        Prm qr = q;  // rows around the rebuild position, trimmed to rc + s
+        qr.trim2 = q.trim2_skin;
+        const float4 xc = sr.xref[i];
+        auto exact = [&](int j, float dx, float dy, float dz, float r2, float4 xj) -> bool {
+            if (!(r2 <= q.rc2)) return false;
+            walk_body(j, dx, dy, dz, r2, xj);
+            return true;
+        };
+        auto rec_exact = [&](int r, int c, uint32_t m) {
+            if (c < 2) hits[(size_t)(2 * r + c) * n + (size_t)i] = m;
+        };
+        if constexpr (WALK == WALK_SKIN_REC) {
+            bool over = false;  // a row of more than 64 candidates: this particle is not recorded
+            skin_walk<false>(qr, i, xi, xc, cell, cs, b.posr, sr.skin, n, exact,
+                             [&](int r, int c, uint32_t m) {
+                                 if (c < 2) sr.skin[(size_t)(2 * r + c) * n + (size_t)i] = m;
+                                 else over = true;
+                             },
+                             rec_exact);
+            ncount[i] = over ? 1 : 0;
+        } else if (ncount[i]) {  // not recorded at the rebuild: the mask walk (and so its successor)
+            for_each_neighbor<WALK_MASK, true, true>(q, i, xi, cell, cs, b.posr, walk_body);
+        } else {
+            skin_walk<true>(qr, i, xi, xc, cell, cs, b.posr, sr.skin, n, exact, [](int, int, uint32_t) {},
+                            rec_exact);
+        }
     } else if (WALK == WALK_HITS) {  // the mask walk, recording each chunk's hit mask
         uint32_t *hits = reinterpret_cast<uint32_t *>(nlist);
         bool over = false;  // a row with more than 64 candidates: not recorded
@@ -233,7 +268,8 @@ __global__ __launch_bounds__(BS, ((WALK != WALK_LIST && !BATCH && MODE == 0) ? 7
                                                         const uint32_t *__restrict__ skey, const int *__restrict__ cs,
                                                         const float4 *__restrict__ gd, float4 *__restrict__ acc,
                                                         const int *__restrict__ nlist, const int *__restrict__ ncount,
-                                                        const float2 *__restrict__ scale, float *__restrict__ terms) {
+                                                        const float2 *__restrict__ scale, float *__restrict__ terms,
+                                                        const float4 *__restrict__ xref) {
     const int nf = cs[q0.G];
     const int i = xcd_block(blocks_of_bs<BS>(nf)) * BS + threadIdx.x;
     if (i >= nf) {
@@ -290,8 +326,13 @@ __global__ __launch_bounds__(BS, ((WALK != WALK_LIST && !BATCH && MODE == 0) ? 7
         body(j, dx, dy, dz, r2, xj, vj, gj);
     };
     const uint32_t cell = cell_of(q, skey[i]);
-    const int n = (WALK == WALK_LIST || WALK == WALK_HITS) ? ncount[i] : 0;
-    if (WALK == WALK_HITS && n == 0) {
+    const int n = (WALK == WALK_LIST || WALK == WALK_HITS || WALK == WALK_SKIN) ? ncount[i] : 0;
+    if (WALK == WALK_SKIN && n == 0) {  // the exact record, rows around the rebuild position
+        Prm qr = q;
+        qr.trim2 = q.trim2_skin;
+        for_each_recorded_at<true, true>(qr, i, xi, xref[i], cell, cs, b.posr,
+                                         reinterpret_cast<const uint32_t *>(nlist), (size_t)q0.N, walk_body);
+    } else if (WALK == WALK_HITS && n == 0) {
         for_each_recorded<true, true>(q, i, xi, cell, cs, b.posr, reinterpret_cast<const uint32_t *>(nlist),
                                       (size_t)q0.N, walk_body);
     } else if (WALK == WALK_LIST && n <= LIST_MAX) {
@@ -468,16 +509,17 @@ void launch_wall(hipStream_t s, int walk, int block, const Prm &q, const Prm *sp
 }
 template <int WALK, bool BATCH, int BS>
 static void ldg_walk(hipStream_t s, bool feat, const Prm &q, const Prm *sp, const Buf &b,
-                     const uint32_t *skey, const int *cs, float4 *gd, int *nlist, int *ncount, float *f) {
+                     const uint32_t *skey, const int *cs, float4 *gd, int *nlist, int *ncount, float *f,
+                     const SkinRec &sr = SkinRec{}) {
     const int grid = nblocks_bs(q.N, BS);
     if constexpr (!BATCH) {
         const bool a = q.diffusion == ZSPH_DIFF_ANTUONO;
         if (a)
-            k_ldg<WALK, false, false, ZSPH_DIFF_ANTUONO, BS><<<grid, BS, 0, s>>>(q, sp, b, skey, cs, gd, nlist, ncount, f);
+            k_ldg<WALK, false, false, ZSPH_DIFF_ANTUONO, BS><<<grid, BS, 0, s>>>(q, sp, b, skey, cs, gd, nlist, ncount, f, sr);
         else
-            k_ldg<WALK, false, false, ZSPH_DIFF_MC, BS><<<grid, BS, 0, s>>>(q, sp, b, skey, cs, gd, nlist, ncount, f);
+            k_ldg<WALK, false, false, ZSPH_DIFF_MC, BS><<<grid, BS, 0, s>>>(q, sp, b, skey, cs, gd, nlist, ncount, f, sr);
     } else {
-        k_ldg<WALK, true, false, DIFF_PER_SCENE, BS><<<grid, BS, 0, s>>>(q, sp, b, skey, cs, gd, nlist, ncount, f);
+        k_ldg<WALK, true, false, DIFF_PER_SCENE, BS><<<grid, BS, 0, s>>>(q, sp, b, skey, cs, gd, nlist, ncount, f, sr);
     }
 }
 // The feature-writing gradient pass of the learned closure (one scene): 256 threads.
@@ -486,9 +528,20 @@ static void ldg_feat(hipStream_t s, const Prm &q, const Prm *sp, const Buf &b, c
                      float4 *gd, int *nlist, int *ncount, float *f) {
     const int grid = nblocks_bs(q.N, ZS_BLOCK);
     if (q.diffusion == ZSPH_DIFF_ANTUONO)
-        k_ldg<WALK, false, true, ZSPH_DIFF_ANTUONO, ZS_BLOCK><<<grid, ZS_BLOCK, 0, s>>>(q, sp, b, skey, cs, gd, nlist, ncount, f);
+        k_ldg<WALK, false, true, ZSPH_DIFF_ANTUONO, ZS_BLOCK><<<grid, ZS_BLOCK, 0, s>>>(q, sp, b, skey, cs, gd, nlist, ncount, f, SkinRec{});
     else
-        k_ldg<WALK, false, true, ZSPH_DIFF_MC, ZS_BLOCK><<<grid, ZS_BLOCK, 0, s>>>(q, sp, b, skey, cs, gd, nlist, ncount, f);
+        k_ldg<WALK, false, true, ZSPH_DIFF_MC, ZS_BLOCK><<<grid, ZS_BLOCK, 0, s>>>(q, sp, b, skey, cs, gd, nlist, ncount, f, SkinRec{});
+}
+
+void launch_ldg_skin(hipStream_t s, int block, const Prm &q, const Buf &b, const uint32_t *skey, const int *cs,
+                     float4 *gd, int *hits, int *ncount, const SkinRec &sr) {
+    if (!q.N) return;
+    with_block(block, [&](auto BSc) {
+        constexpr int BS = decltype(BSc)::value;
+        // exactly one of the two forms does the work (the other returns at once)
+        ldg_walk<WALK_SKIN_REC, false, BS>(s, false, q, nullptr, b, skey, cs, gd, hits, ncount, nullptr, sr);
+        ldg_walk<WALK_SKIN_REPLAY, false, BS>(s, false, q, nullptr, b, skey, cs, gd, hits, ncount, nullptr, sr);
+    });
 }
 
 void launch_ldg(hipStream_t s, int walk, int block, const Prm &q, const Prm *sp, const Buf &b, const uint32_t *skey,
@@ -520,10 +573,10 @@ void launch_ldg(hipStream_t s, int walk, int block, const Prm &q, const Prm *sp,
 template <int WALK, bool BATCH>
 static void gov_walk(hipStream_t s, int mode, int block, const Prm &q, const Prm *sp, const Buf &b,
                      const uint32_t *skey, const int *cs, const float4 *gd, float4 *acc, const int *nlist,
-                     const int *ncount, const float2 *scale, float *terms) {
+                     const int *ncount, const float2 *scale, float *terms, const float4 *xref = nullptr) {
 #define ZS_GOV(MODE_, DF_, BS_)                                                                  \
     k_governing<WALK, BATCH, MODE_, DF_, BS_><<<nblocks_bs(q.N, BS_), BS_, 0, s>>>(              \
-        q, sp, b, skey, cs, gd, acc, nlist, ncount, scale, terms)
+        q, sp, b, skey, cs, gd, acc, nlist, ncount, scale, terms, xref)
     if constexpr (!BATCH) {
         const bool a = q.diffusion == ZSPH_DIFF_ANTUONO;
         if (mode == 1) {
@@ -551,10 +604,12 @@ static void gov_walk(hipStream_t s, int mode, int block, const Prm &q, const Prm
 void launch_governing(hipStream_t s, int walk, int block, const Prm &q, const Prm *sp, const Buf &b,
                       const uint32_t *skey, const int *cs, const float4 *gd, float4 *acc, const int *nlist,
                       const int *ncount, int mode, const float2 *scale, float *terms, unsigned long long *clock,
-                      bool moving) {
+                      bool moving, const float4 *xref) {
     if (!q.N) return;
     if (clock) launch_clock(s, clock, true);
-    if (walk == WALK_LIST) {
+    if (walk == WALK_SKIN) {  // one scene, plain terms (the caller falls back otherwise)
+        gov_walk<WALK_SKIN, false>(s, 0, block, q, nullptr, b, skey, cs, gd, acc, nlist, ncount, scale, terms, xref);
+    } else if (walk == WALK_LIST) {
         if (sp) gov_walk<WALK_LIST, true>(s, mode, block, q, sp, b, skey, cs, gd, acc, nlist, ncount, scale, terms);
         else gov_walk<WALK_LIST, false>(s, mode, block, q, sp, b, skey, cs, gd, acc, nlist, ncount, scale, terms);
     } else if (walk == WALK_HITS) {

@@ -66,7 +66,8 @@ struct zsph_ctx {
     int skin_slot = 0;         // key slot holding the skin grid's sorted keys
     int *nlist = nullptr;      // WALK_LIST neighbour lists (list_words(N) ints) and counts
     int *ncount = nullptr;     // WALK_LIST counts / WALK_HITS overflow flags
-    int *hits = nullptr;       // WALK_HITS hit masks (hits_slots(k) x N words)
+    int *hits = nullptr;       // WALK_HITS / WALK_SKIN exact hit masks (hits_slots(k) x N words)
+    uint32_t *skin_hits = nullptr;  // WALK_SKIN skin hit masks of the last rebuild (same layout)
     bool list_valid = false;   // the lists / hit masks describe the current sorted buffer
     int nscenes = 1;                  // > 1: a batch of independent scenes
     zs::Prm *scene_prm = nullptr;     // per-scene parameters (device, batches only)
@@ -221,43 +222,60 @@ unsigned long long *stage_clock(zsph_ctx *c, int st) {
     return (c->clock && (c->clock_mask >> st & 1u)) ? c->clock + 3 * st : nullptr;
 }
 
+// The walk a stage runs: WALK_SKIN only on skin mode's plain path (its rows
+// live around the rebuild positions); elsewhere (staged reference calls, the
+// closure's passes) the hit-mask walk stands in for it.
+int eff_walk(const zsph_ctx *c, bool skin_path) {
+    if (c->walk == zs::WALK_SKIN && (!skin_path || c->closure_on)) return zs::WALK_HITS;
+    return c->walk;
+}
 // The record a walk's gradient pass writes and its continuity+momentum pass reads.
-int *walk_record(zsph_ctx *c) { return c->walk == zs::WALK_HITS ? c->hits : c->nlist; }
-bool walk_records(int walk) { return walk == zs::WALK_LIST || walk == zs::WALK_HITS; }
+int *walk_record(zsph_ctx *c) { return (c->walk == zs::WALK_HITS || c->walk == zs::WALK_SKIN) ? c->hits : c->nlist; }
+bool walk_records(int walk) { return walk == zs::WALK_LIST || walk == zs::WALK_HITS || walk == zs::WALK_SKIN; }
 
-int stage_boundary(zsph_ctx *c, const zs::Prm &q, int b, Prof *pf = nullptr) {
+int stage_boundary(zsph_ctx *c, const zs::Prm &q, int b, Prof *pf = nullptr, bool skin_path = false) {
     if (c->sorted_slot < 0) return set_err(c, ZSPH_E_STATE, "zsph_boundary called before zsph_sort_data");
     const uint32_t *sk = c->key[c->sorted_slot];
+    const int walk = eff_walk(c, skin_path);
     {
         Scope sc(pf, ZSPH_ST_WALL);
         unsigned long long *ck = stage_clock(c, ZSPH_ST_WALL);
         if (ck) zs::launch_clock(c->stream, ck, true);
-        zs::launch_wall(c->stream, c->walk, c->block, q, c->scene_prm, c->buf[b], sk, c->cell_start);
+        zs::launch_wall(c->stream, walk, c->block, q, c->scene_prm, c->buf[b], sk, c->cell_start);
         if (ck) zs::launch_clock(c->stream, ck, false);
     }
     {
         Scope sc(pf, ZSPH_ST_LDG);
         unsigned long long *ck = stage_clock(c, ZSPH_ST_LDG);
         if (ck) zs::launch_clock(c->stream, ck, true);
-        zs::launch_ldg(c->stream, c->walk, c->block, q, c->scene_prm, c->buf[b], sk, c->cell_start, c->gd,
-                       walk_record(c), c->ncount, c->closure_on ? c->feat : nullptr);
+        if (walk == zs::WALK_SKIN) {
+            zs::SkinRec sr;
+            sr.xref = c->xref;
+            sr.skin = c->skin_hits;
+            sr.rebuilt = c->rebuild + 1;
+            zs::launch_ldg_skin(c->stream, c->block, q, c->buf[b], sk, c->cell_start, c->gd, c->hits, c->ncount, sr);
+        } else {
+            zs::launch_ldg(c->stream, walk, c->block, q, c->scene_prm, c->buf[b], sk, c->cell_start, c->gd,
+                           walk_record(c), c->ncount, c->closure_on ? c->feat : nullptr);
+        }
         if (ck) zs::launch_clock(c->stream, ck, false);
     }
-    c->list_valid = walk_records(c->walk);
+    c->list_valid = walk_records(walk);
     return check_launch(c, "boundary");
 }
-int stage_governing(zsph_ctx *c, const zs::Prm &q, int b, Prof *pf = nullptr) {
+int stage_governing(zsph_ctx *c, const zs::Prm &q, int b, Prof *pf = nullptr, bool skin_path = false) {
     if (c->sorted_slot < 0) return set_err(c, ZSPH_E_STATE, "zsph_governing called before zsph_sort_data");
     {
         Scope sc(pf, ZSPH_ST_GOVERNING);
         // the recording walks need this half step's record; otherwise walk the cells
-        const int walk = (walk_records(c->walk) && !c->list_valid) ? zs::WALK_MASK : c->walk;
+        const int ew = eff_walk(c, skin_path);
+        const int walk = (walk_records(ew) && !c->list_valid) ? zs::WALK_MASK : ew;
         if (c->closure_on)  // scale factors of the fluid particles [0, cs[G])
             zs::launch_mlp_infer(c->stream, c->mlp_P, c->mlp_pack, c->mlp_aux, c->feat, 0, c->cell_start + q.G,
                                  c->scale, zs::mlp_infer_blocks(q.N));
         zs::launch_governing(c->stream, walk, c->block, q, c->scene_prm, c->buf[b], c->key[c->sorted_slot], c->cell_start,
                              c->gd, c->acc, walk_record(c), c->ncount, c->closure_on ? 1 : 0, c->scale, c->terms,
-                             stage_clock(c, ZSPH_ST_GOVERNING), c->has_body || c->has_piston);
+                             stage_clock(c, ZSPH_ST_GOVERNING), c->has_body || c->has_piston, c->xref);
     }
     return check_launch(c, "governing");
 }
@@ -324,15 +342,15 @@ int skin_half_step(zsph_ctx *c, bool predictor, Prof *pf = nullptr) {
         Scope sc(pf, ZSPH_ST_REORDER);
         zs::launch_reorder(c->stream, q.N, c->idx[slot], c->buf[0], c->buf[1], gate);
         zs::launch_skin_commit(c->stream, q.N, c->buf[1], c->buf[0], c->xref, gate);
-        zs::launch_skin_flag(c->stream, c->rebuild, c->rebuilds);
+        zs::launch_skin_flag(c->stream, c->rebuild, c->rebuilds, c->rebuild + 1);
     }
     if (int r = check_launch(c, "skin rebuild")) return r;
     c->sorted_slot = slot;
     c->skin_slot = slot;
     c->list_valid = false;
     int r;
-    if ((r = stage_boundary(c, q, 0, pf))) return r;
-    if ((r = stage_governing(c, q, 0, pf))) return r;
+    if ((r = stage_boundary(c, q, 0, pf, true))) return r;
+    if ((r = stage_governing(c, q, 0, pf, true))) return r;
     zs::SkinCheck chk;
     chk.xref = c->xref;
     chk.flag = c->rebuild;
@@ -872,7 +890,7 @@ int zsph_set_tuning(zsph_ctx *c, const zsph_tuning *t) {
     if (!c || !t) return ZSPH_E_ARG;
     const int block = t->block ? t->block : ZS_BLOCK;
     const int tile = t->sort_tile ? t->sort_tile : 16;
-    if (t->walk < ZSPH_WALK_MASK || t->walk > ZSPH_WALK_LIST || t->radix_bits < 8 || t->radix_bits > 11 ||
+    if (t->walk < ZSPH_WALK_MASK || t->walk > ZSPH_WALK_SKIN || t->radix_bits < 8 || t->radix_bits > 11 ||
         (block != 128 && block != 256 && block != 512) || (tile != 8 && tile != 16) || !(t->skin >= 0.0f) ||
         t->skin > 1.0f)
         return set_err(c, ZSPH_E_ARG,
@@ -885,7 +903,7 @@ int zsph_set_tuning(zsph_ctx *c, const zsph_tuning *t) {
         return ZSPH_OK;
     int r;
     if (t->skin > 0.0f && !c->rebuild &&
-        ((r = dalloc(c, &c->rebuild, 1)) || (r = dalloc(c, &c->rebuilds, 1)) ||
+        ((r = dalloc(c, &c->rebuild, 2)) || (r = dalloc(c, &c->rebuilds, 1)) ||
          (r = dalloc(c, &c->xref, (size_t)c->q.N))))
         return r;
     if (t->skin != c->skin) {
@@ -895,11 +913,15 @@ int zsph_set_tuning(zsph_ctx *c, const zsph_tuning *t) {
     }
     if (walk_records(t->walk) && !c->ncount && (r = dalloc(c, &c->ncount, (size_t)c->q.N))) return r;
     if (t->walk == ZSPH_WALK_LIST && !c->nlist && (r = dalloc(c, &c->nlist, zs::list_words(c->q.N)))) return r;
-    if (t->walk == ZSPH_WALK_HITS && !c->hits) {
+    if ((t->walk == ZSPH_WALK_HITS || t->walk == ZSPH_WALK_SKIN) && !c->hits) {
         int k = c->q.k;  // the widest search of any scene
         for (const zs::Prm &p : c->scene_host) k = std::max(k, p.k);
         if ((r = dalloc(c, &c->hits, (size_t)zs::hits_slots(k) * (size_t)c->q.N))) return r;
     }
+    if (t->walk == ZSPH_WALK_SKIN && !c->skin_hits &&
+        (r = dalloc(c, &c->skin_hits, (size_t)zs::hits_slots(c->q.k) * (size_t)c->q.N)))
+        return r;
+    if (t->walk == ZSPH_WALK_SKIN && c->walk != ZSPH_WALK_SKIN) c->skin_valid = false;  // record at the next step
     c->list_valid = false;
     c->walk = t->walk;
     c->radix_bits = t->radix_bits;
@@ -982,7 +1004,8 @@ int zsph_autotune(zsph_ctx *c, int steps, zsph_tuning *best, float *best_ms) {
     // coordinate search: walk x digit width, then block size, then sort tile
     const zsph_tuning start = keep;
     const int nskin = start.skin > 0.0f ? 4 * steps : steps;  // skin mode: time over more steps
-    for (int walk = ZSPH_WALK_MASK; walk <= ZSPH_WALK_LIST && !r; ++walk) {
+    const int last_walk = (start.skin > 0.0f && c->nscenes == 1) ? ZSPH_WALK_SKIN : ZSPH_WALK_LIST;
+    for (int walk = ZSPH_WALK_MASK; walk <= last_walk && !r; ++walk) {
         trial({walk, even, start.block, start.sort_tile, start.skin}, nskin);
         if (even != 8 && !r) trial({walk, 8, start.block, start.sort_tile, start.skin}, nskin);
     }
