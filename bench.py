@@ -26,6 +26,12 @@ workload (`RECORDED`, with the profile file named), not read in the run.
 The CPU baseline is the FP32 oracle on the host's cores: median of 5 timings
 after 2 warm-up steps with all threads, and a one-thread sample.
 
+One-scene cases run zsph_step in skin mode by default (--skin 0.1, tuned among
+0.05 to 0.3 h): the neighbour structures are rebuilt only after a particle has
+moved more than half the skin since the last rebuild, instead of every half
+step; neighbour sets are unchanged (the r^2 <= (2h)^2 predicate decides every
+pair).  --skin 0 times the reference schedule.
+
 --case wavetank3d: the floating-body tank, 4.0 x 1.0 x 1.2 m filled to 0.6 m
 at dx 0.006 m (11.1 M fluid, 12.5 M particles), piston 0.025 m amplitude at
 1 Hz, a 0.3 x 0.3 x 0.2 m box of density 500 coupled through its FP64 rigid
@@ -61,13 +67,21 @@ SIMDS, VALU_CYCLES, CLOCK_HZ = 1024, 4, 2.4e9
 
 # Recorded hardware counters of the neighbour passes per launch (rocprofv3 --pmc, one counter per
 # pass; the table's profile file gives the commands): HBM traffic = FETCH_SIZE x 2 + WRITE_SIZE x 1
-# (KiB, factors from a device copy of known size) and SQ_INSTS_VALU (wave instructions).  Keyed by
-# (case, dx, grid_size_factor, walk); other workloads report none.
+# (KiB, factors from a device copy of known size) and SQ_INSTS_VALU (wave instructions), measured on the
+# developed flow (from step 3000).  Keyed by (case, dx, grid_size_factor, walk, skin, block); other
+# workloads and tunings report none.
 RECORDED = {
-    ("dambreak3d", 0.008, 2.0, 1): {
-        "profile": "profiles/counters_neighbour_kernels_dambreak3d.md",
-        "k_ldg": {"traffic": (135497.5 * 2.0 + 160981.1) * 1024.0, "valu_insts": 2.15e8},
-        "k_governing": {"traffic": (329708.6 * 2.0 + 20214.8) * 1024.0, "valu_insts": 1.57e8},
+    ("dambreak3d", 0.008, 2.0, 1, 0.1, 128): {
+        "profile": "profiles/counters_neighbour_kernels_dambreak3d.md and "
+                   "profiles/traffic_neighbour_kernels_dambreak3d_dx0.008.md",
+        "k_ldg": {"traffic": (139940.0 * 2.0 + 166313.0) * 1024.0, "valu_insts": 2.70221e8},
+        "k_governing": {"traffic": (347830.0 * 2.0 + 28966.3) * 1024.0, "valu_insts": 1.88672e8},
+    },
+    ("dambreak3d", 0.008, 2.0, 1, 0.0, 128): {
+        "profile": "profiles/counters_neighbour_kernels_dambreak3d.md and "
+                   "profiles/traffic_neighbour_kernels_dambreak3d_dx0.008.md",
+        "k_ldg": {"traffic": (147093.0 * 2.0 + 167160.0) * 1024.0, "valu_insts": 2.60343e8},
+        "k_governing": {"traffic": (362084.0 * 2.0 + 28966.3) * 1024.0, "valu_insts": 1.88264e8},
     },
 }
 
@@ -107,7 +121,7 @@ def kernel_roofline(name, stage_ms, launches, nbytes, rec, bound):
         # VALU issue: the recorded instruction count over this run's launch time
         out["valu_frac"] = r["valu_insts"] * VALU_CYCLES / (SIMDS * CLOCK_HZ * stage_ms * 1e-3)
         out["recorded"] = (f"traffic and SQ_INSTS_VALU per launch recorded in {rec['profile']} "
-                           "(rocprofv3 --pmc of the same workload and walk)")
+                           "(rocprofv3 --pmc of the same workload and tuning)")
     return out
 
 
@@ -194,15 +208,20 @@ def main():
     ap.add_argument("--radix-bits", type=int, default=None, help="force the sort's digit width (8 to 11 bits)")
     ap.add_argument("--block", type=int, default=None, help="force the neighbour passes' block size (128/256/512)")
     ap.add_argument("--sort-tile", type=int, default=None, help="force the radix tile's keys per thread (8/16)")
-    ap.add_argument("--skin", type=float, default=0.0,
+    ap.add_argument("--skin", type=float, default=None,
                     help="neighbour skin in units of h (0: rebuild every half step, the reference schedule; > 0: "
-                         "rebuild only after a particle moved more than skin/2; the autotuner then also tries "
-                         "skins of 0.05, 0.1, 0.2 and 0.3)")
+                         "rebuild only after a particle moved more than skin/2, the autotuner then also tries "
+                         "skins of 0.05, 0.1, 0.2 and 0.3; default 0.1 for one-scene cases, 0 for the ensemble)")
     ap.add_argument("--closure", default=None, metavar="NPZ",
                     help="run every step with this learned closure network (one-scene cases)")
     ap.add_argument("--tune-at", type=int, default=None,
                     help="autotune on the developed state after this many steps of the run (default: 3000 for "
                          "the 3-D dam break, 1000 for the ensemble, 300 for the wave tank, whatever --steps is)")
+    ap.add_argument("--state-out", default=None, metavar="PATH",
+                    help="write the state at the start of the timed window to this checkpoint file")
+    ap.add_argument("--state-in", default=None, metavar="PATH",
+                    help="start the timed window from this checkpoint (written by --state-out with the same "
+                         "case) instead of stepping there; needs a fixed tuning (--walk etc. or --no-autotune)")
     ap.add_argument("--start-at", type=int, default=None,
                     help="first step of the timed window (default: 0 when the timed steps cover the configured "
                          "run, else the tuning step: a short window is timed in the developed flow)")
@@ -274,6 +293,8 @@ def main():
     s.upload(case.host)
     if setup:
         setup(s)
+    if args.skin is None:
+        args.skin = 0.0 if batch else 0.1
     if args.skin > 0:
         if batch:
             print("bench.py: --skin needs a one-scene case", file=sys.stderr)
@@ -287,26 +308,38 @@ def main():
                      args.block, args.sort_tile)
         args.autotune = False
     import tempfile
-    with tempfile.TemporaryDirectory() as tmp:
-        ckpt = os.path.join(tmp, "initial.bin")
+    if args.state_in:
+        if args.autotune:
+            print("bench.py: --state-in needs a fixed tuning (--walk / --no-autotune)", file=sys.stderr)
+            sys.exit(2)
+        info = z.checkpoint_info(args.state_in)
+        s.restore(args.state_in)
+        args.start_at = int(info["step"])
+        tune = s.get_tuning()
+    elif args.autotune or args.start_at > 0:
         # The walks' relative speed changes as the flow develops, so the candidates are
         # timed on a developed state (a fixed step of the run, not a fraction of --steps):
         # checkpoint the initial state, step there, autotune (it restores the state it
         # timed from), then either restart from the initial state bit for bit or time the
         # window from there (all outside the timed region).
-        if args.autotune or args.start_at > 0:
+        with tempfile.TemporaryDirectory() as tmp:
+            ckpt = os.path.join(tmp, "initial.bin")
             s.checkpoint(ckpt, 0)
             s.step(args.tune_at)
             s.sync()
-        if args.autotune:
-            tune = s.autotune(steps=3)
-            tune["tuned_at_step"] = args.tune_at
-        else:
-            tune = s.get_tuning()
-        if args.start_at != args.tune_at and (args.autotune or args.start_at > 0):
-            s.restore(ckpt)
-            if args.start_at:
-                s.step(args.start_at)
+            if args.autotune:
+                tune = s.autotune(steps=3)
+                tune["tuned_at_step"] = args.tune_at
+            else:
+                tune = s.get_tuning()
+            if args.start_at != args.tune_at:
+                s.restore(ckpt)
+                if args.start_at:
+                    s.step(args.start_at)
+    else:
+        tune = s.get_tuning()
+    if args.state_out:
+        s.checkpoint(args.state_out, args.start_at)
     print(f"[bench] tuning: {tune}; timed window starts at step {args.start_at}", file=sys.stderr, flush=True)
     s.set_graph(args.graph)
     # device clock around every launch of the three neighbour passes, inside the captured step
@@ -351,7 +384,9 @@ def main():
     ms_step = wall * 1e3 / args.steps
     value = case.n * args.steps / wall
     b_step = step_bytes(case.n, G, n_fluid)
-    rec = RECORDED.get((args.case, args.dx, args.gsf, tune["walk"])) if not args.closure else None
+    rec = RECORDED.get((args.case, args.dx, args.gsf, tune["walk"], round(tune.get("skin", 0.0), 3),
+                        tune.get("block"))) \
+        if not args.closure else None
     avg = {st: (ms / max(n, 1), n) for st, (ms, n) in clocks.items()}
     # the dominant kernel: the gradient pass (VALU-bound: candidate tests and pair arithmetic)
     roof = kernel_roofline("k_ldg", *avg[z.ST_LDG], ldg_bytes(case.n, n_fluid, G), rec, "valu")
