@@ -429,20 +429,89 @@ inline size_t list_words(int n) { return ((size_t)(n + 63) >> 6) * 64 * LIST_MAX
 // slot-major over the particles (hits[slot * N + i]), so a wave's lanes
 // storing or loading the same slot touch 256 contiguous bytes.
 inline int hits_slots(int k) { return 2 * (2 * k + 1) * (2 * k + 1); }
-// Visits the hits of two one-chunk rows as one sequence: bit u of mA is
-// candidate u of row A (fb_a + u for u < nfl_a, else dbo_a + u), bit u of mB
-// the same in row B; A's hits first, each row's in ascending order.
-template <class V>
-__device__ __forceinline__ void visit_pair(uint32_t mA, uint32_t mB, int fbA, int nA, int dA, int fbB, int nB, int dB,
-                                           V &visit) {
+// Kind of a visited neighbour j, known from the part of its row it came from:
+// the visits of one-chunk row pairs pass FluidJ or DummyJ, the chunked rows
+// AnyJ (then j < cs[G] tells).
+template <int K>
+struct JKind {
+    static constexpr int value = K;
+};
+using FluidJ = JKind<0>;
+using DummyJ = JKind<1>;
+using AnyJ = JKind<2>;
+// Whether neighbour j of kind K is a fluid particle (nf = cs[G]).
+template <class K>
+__device__ __forceinline__ bool fluid_j(K, int j, int nf) {
+    if constexpr (K::value == 0) return true;
+    else if constexpr (K::value == 1) return false;
+    else return j < nf;
+}
+// Per-neighbour data a walk gathers besides the position: gath(j, kind)
+// returns it (NoGather: nothing); its loads are issued with the position's.
+struct Empty {};
+struct VelGrad {  // velp[j] and gd[j]
+    float4 v, g;
+};
+struct NoGather {
+    template <class K>
+    __device__ __forceinline__ Empty operator()(int, K) const { return {}; }
+};
+// Body call: f(j, dx, dy, dz, r2, xj, kind, ext), f(j, dx, dy, dz, r2, xj,
+// kind) or f(j, dx, dy, dz, r2, xj), whichever the body takes.
+template <class F, class K, class E>
+__device__ __forceinline__ decltype(auto) call_body(F &f, int j, float dx, float dy, float dz, float r2, float4 xj,
+                                                    K k, const E &e) {
+    if constexpr (std::is_invocable_v<F &, int, float, float, float, float, float4, K, const E &>)
+        return f(j, dx, dy, dz, r2, xj, k, e);
+    else if constexpr (std::is_invocable_v<F &, int, float, float, float, float, float4, K>)
+        return f(j, dx, dy, dz, r2, xj, k);
+    else
+        return f(j, dx, dy, dz, r2, xj);
+}
+// A hit's index, position and gathered data.
+template <class E>
+struct Hit {
+    int j;
+    float4 x;
+    E e;
+};
+// The hit j of kind K: position and gathered data, loads issued together.
+template <class G, class K>
+__device__ __forceinline__ auto load_hit(const float4 *__restrict__ posr, G &gath, int j, K k) {
+    using E = decltype(gath(j, k));
+    return Hit<E>{j, ld(posr, j), gath(j, k)};
+}
+// Runs the body on hit h of particle i at xi.
+template <class F, class K, class E>
+__device__ __forceinline__ decltype(auto) run_hit(F &f, float4 xi, const Hit<E> &h, K k) {
+    const float dx = xi.x - h.x.x, dy = xi.y - h.x.y, dz = xi.z - h.x.z;
+    return call_body(f, h.j, dx, dy, dz, dist2(dx, dy, dz), h.x, k, h.e);
+}
+
+// Bits [0, n) of m, and m shifted down by n (n <= 32).
+__device__ __forceinline__ uint32_t low_bits(uint32_t m, int n) { return n >= 32 ? m : m & ((1u << n) - 1u); }
+__device__ __forceinline__ uint32_t high_bits(uint32_t m, int n) { return n >= 32 ? 0u : m >> n; }
+
+// Visits the hits of a pair of one-chunk rows A, B (at most 32 candidates
+// each) as one sequence: A's hits, then B's, each row's in candidate order.
+// Bit u of mA is candidate u of row A: the fluid particle A.fb + u for
+// u < nA = A.fe - A.fb, else the other particle A.db + u - nA; the same for
+// B.  For each hit, in that order, use(load(j, kind), kind, side, u) runs
+// (kind AnyJ; side 0 for row A, 1 for B; u = the hit's bit in its row's
+// mask).
+template <bool FLUID_J, bool DUMMY_J, class L, class U>
+__device__ __forceinline__ void visit_rows(uint32_t mA, uint32_t mB, const RowRange &A, const RowRange &B, L &&load,
+                                           U &&use) {
+    const int nA = A.fe - A.fb, nB = B.fe - B.fb;
+    const int dA = A.db - nA, dB = B.db - nB;
     uint64_t m = (uint64_t)mA | ((uint64_t)mB << 32);
     while (m) {
-        const int bit = __builtin_ctzll(m);
+        const int t = __builtin_ctzll(m);
         m &= m - 1u;
-        const bool hi = bit >= 32;
-        const int u = bit & 31;
-        const int nfl = hi ? nB : nA;
-        visit(((u < nfl) ? (hi ? fbB : fbA) : (hi ? dB : dA)) + u);
+        const bool hi = t >= 32;
+        const int u = t & 31;
+        const int j = ((u < (hi ? nB : nA)) ? (hi ? B.fb : A.fb) : (hi ? dB : dA)) + u;
+        use(load(j, AnyJ{}), AnyJ{}, (int)hi, u);
     }
 }
 
@@ -451,31 +520,161 @@ struct NoRec {
     __device__ __forceinline__ void operator()(int, int, uint32_t) const {}
 };
 
+// LDS staging of a wave's candidate rows.  The lanes of a wave are
+// consecutive particles in cell order, so when every active lane lies in one
+// cell row (the same y and z cell), the candidates of all lanes in the box
+// row at offset (dz, dy) lie in that row's cells [cx_lo - k, cx_hi + k]
+// (cx_lo, cx_hi: the first and last active lanes' x cells): one contiguous
+// range of the fluid segment and one of the others.  The wave copies those
+// ranges of both rows of a pair into its LDS slice with coalesced loads
+// (their bounds are read once per wave from the cell table), and each lane
+// tests its own candidates there instead of gathering them from L2.  The
+// staged records are the positions themselves, so the hit masks, and
+// everything after them, are those of the unstaged test.
+constexpr int STAGE_SLOTS = 256;  // records per wave: the four ranges of a row pair
+constexpr int STAGE_PAD = 4;      // a test round reads up to three records past a row's end
+struct NoStage {
+    float4 *lds = nullptr;  // unused
+};
+struct LdsStage {
+    float4 *lds;  // this wave's STAGE_SLOTS + STAGE_PAD records
+};
+// The wave-uniform cell row of the walk (ok: every active lane lies in it).
+struct WaveRow {
+    bool ok;
+    int cz, cy, x0, x1;  // the row's z and y cell; the first and last lanes' x cells
+};
+__device__ __forceinline__ WaveRow wave_row(const Prm &q, uint32_t cell) {
+    const uint64_t ex = __builtin_amdgcn_read_exec();
+    const int last = 63 - __builtin_clzll(ex);
+    const uint32_t c0 = (uint32_t)__builtin_amdgcn_readfirstlane((int)cell);
+    const uint32_t c1 = (uint32_t)__builtin_amdgcn_readlane((int)cell, last);
+    const int r0 = (int)(c0 / (uint32_t)q.X), r1 = (int)(c1 / (uint32_t)q.X);
+    WaveRow w;
+    w.ok = r0 == r1;
+    w.x0 = (int)c0 - r0 * q.X;
+    w.x1 = (int)c1 - r1 * q.X;
+    w.cz = r0 / q.Y;
+    w.cy = r0 - w.cz * q.Y;
+    return w;
+}
+// Staged ranges of a row pair: segment starts (particle index) and LDS slots.
+struct StagedPair {
+    bool ok;
+    int sAf, sAd, sBf, sBd;  // first particle of each staged range
+    int oAf, oAd, oBf, oBd;  // its first LDS slot
+};
+// The uniform fluid and other ranges [s, e) of box row (dz, dy) of the wave's row.
+template <bool DUMMY_J>
+__device__ __forceinline__ void stage_range(const Prm &q, const WaveRow &w, const int *__restrict__ cs, int dz, int dy,
+                                            int &sf, int &ef, int &sd, int &ed) {
+    sf = ef = sd = ed = 0;
+    const int z = w.cz + dz, y = w.cy + dy;
+    if (z < 0 || z >= q.Z || y < 0 || y >= q.Y) return;
+    const int xa = max(w.x0 - q.k, 0), xb = min(w.x1 + q.k, q.X - 1);
+    const int row = q.cell_base + z * q.XY + y * q.X;
+    sf = __builtin_amdgcn_readfirstlane(cs[row + xa]);
+    ef = __builtin_amdgcn_readfirstlane(cs[row + xb + 1]);
+    if (DUMMY_J) {
+        sd = __builtin_amdgcn_readfirstlane(cs[q.G + 1 + row + xa]);
+        ed = __builtin_amdgcn_readfirstlane(cs[q.G + 1 + row + xb + 1]);
+    }
+}
+// Copies the ranges of box rows (dz, dy) (when useA) and (-dz, -dy) (when
+// useB) into the wave's LDS slice; ok = false (nothing copied) when they
+// exceed STAGE_SLOTS.  Every active lane of the wave takes part.
+template <bool DUMMY_J>
+__device__ __forceinline__ StagedPair stage_pair(const Prm &q, const WaveRow &w, const int *__restrict__ cs,
+                                                 const float4 *__restrict__ posr, float4 *lds, int dz, int dy,
+                                                 bool useA, bool useB) {
+    StagedPair p;
+    int eAf, eAd, eBf, eBd;
+    p.sAf = p.sAd = p.sBf = p.sBd = 0;
+    eAf = eAd = eBf = eBd = 0;
+    if (useA) stage_range<DUMMY_J>(q, w, cs, dz, dy, p.sAf, eAf, p.sAd, eAd);
+    if (useB) stage_range<DUMMY_J>(q, w, cs, -dz, -dy, p.sBf, eBf, p.sBd, eBd);
+    const int lAf = eAf - p.sAf, lAd = eAd - p.sAd, lBf = eBf - p.sBf, lBd = eBd - p.sBd;
+    p.oAf = 0;
+    p.oAd = lAf;
+    p.oBf = lAf + lAd;
+    p.oBd = p.oBf + lBf;
+    const int total = p.oBd + lBd;
+    p.ok = total <= STAGE_SLOTS;
+    if (!p.ok) return p;
+    const uint64_t ex = __builtin_amdgcn_read_exec();
+    const int rank = (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(ex >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)ex, 0u));
+    const int nact = __builtin_popcountll(ex);
+    // the previous pair's reads of the slice come before these writes
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    for (int t = rank; t < total; t += nact) {
+        const int src = (t < p.oAd) ? p.sAf + t
+                        : (t < p.oBf) ? p.sAd + (t - p.oAd)
+                        : (t < p.oBd) ? p.sBf + (t - p.oBf)
+                                      : p.sBd + (t - p.oBd);
+        lds[t] = ld(posr, src);
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    return p;
+}
+// Hit mask of the cnt (<= 32) consecutive staged records from slot s against
+// c: bit u set when record s + u lies within r2max of c.  A round past cnt
+// reads up to three records that follow (STAGE_PAD) and the final mask
+// clears their bits.
+__device__ __forceinline__ uint32_t test_run_lds(const float4 *lds, int s, int cnt, float4 c, float r2max) {
+    uint32_t m = 0;
+    const float4 *p = lds + s;
+    for (int t = 0; t < cnt; t += 4, p += 4) {
+        float3 xs[4];
+#pragma unroll
+        for (int e = 0; e < 4; ++e) xs[e] = make_float3(p[e].x, p[e].y, p[e].z);
+        uint32_t nib = 0;
+#pragma unroll
+        for (int e = 0; e < 4; ++e) nib |= (dist2(c.x - xs[e].x, c.y - xs[e].y, c.z - xs[e].z) <= r2max) ? (1u << e) : 0u;
+        m |= nib << t;  // t <= 28
+    }
+    return (cnt < 32) ? m & ((1u << cnt) - 1u) : m;
+}
+// Hit mask of a row (at most 32 candidates) from its staged ranges: fluid
+// candidates from slot so + (w.fb - sf), the others from sd's.
+template <bool FLUID_J, bool DUMMY_J>
+__device__ __forceinline__ uint32_t test_row_lds(const float4 *lds, const RowRange &w, int sf, int of, int sd, int od,
+                                                 float4 c, float r2max) {
+    const int nfl = w.fe - w.fb;
+    uint32_t m = FLUID_J ? test_run_lds(lds, of + (w.fb - sf), nfl, c, r2max) : 0u;
+    if (DUMMY_J && w.de > w.db) m |= test_run_lds(lds, od + (w.db - sd), w.de - w.db, c, r2max) << nfl;
+    return m;
+}
+
 // Visit every neighbour j != i (FP32 r^2 <= rc2) of particle i at xi whose
 // segment is enabled (FLUID_J: j < cs[G]; DUMMY_J: j >= cs[G]).
-// f(j, dx, dy, dz, r2, xj) with x_ij = xi - xj.  Every walk variant visits
-// the neighbours in this order: row pairs in for_each_row_pair order, row A
-// before row B, each row's candidates (fluid range, then the others) in
-// ascending order.  The candidates are tested 32 at a time into a per-lane
-// hit mask; when both rows of a pair have at most 32 candidates their hits
-// are visited as one sequence, else row by row, chunk by chunk.  The wave
-// runs f once per hit of its busiest lane in each sequence.  rec(r, c, m)
-// receives the hit mask m of chunk c of box row r.
-template <int WALK, bool FLUID_J, bool DUMMY_J, class F, class Rec = NoRec>
+// f(j, dx, dy, dz, r2, xj[, kind]) with x_ij = xi - xj (kind: FluidJ, DummyJ
+// or AnyJ).  Every walk variant visits the neighbours in this order: row
+// pairs in for_each_row_pair order, row A before row B, each row's
+// candidates (fluid range, then the others) in ascending order.  A row's
+// candidates are tested into a per-lane hit mask; when both rows of a pair
+// have at most 32 candidates their hits are visited as one sequence
+// (visit_rows), else row by row, 32 candidates at a time.  The wave runs f
+// once per hit of its busiest lane in each sequence.  rec(r, c, m) receives
+// the hit mask m of chunk c of box row r; gath(j, kind) the data gathered
+// with each hit's position (passed to f as ext).
+template <int WALK, bool FLUID_J, bool DUMMY_J, class F, class Rec = NoRec, class Gath = NoGather,
+          class Stg = NoStage>
 __device__ __forceinline__ void for_each_neighbor(const Prm &q, int i, float4 xi, uint32_t cell,
                                                   const int *__restrict__ cs, const float4 *__restrict__ posr, F &&f,
-                                                  Rec &&rec = Rec{}) {
-    auto visit = [&](int j) {
-        const float4 xj = ld(posr, j);
-        const float dx = xi.x - xj.x, dy = xi.y - xj.y, dz = xi.z - xj.z;
-        f(j, dx, dy, dz, dist2(dx, dy, dz), xj);
-    };
+                                                  Rec &&rec = Rec{}, Gath &&gath = Gath{}, Stg stg = Stg{}) {
+    constexpr bool STAGED = std::is_same_v<Stg, LdsStage>;
+    WaveRow wr{};
+    if constexpr (STAGED) wr = wave_row(q, cell);
+    auto load = [&](int j, auto kind) { return load_hit(posr, gath, j, kind); };
+    auto use = [&](const auto &h, auto kind, int, int) { run_hit(f, xi, h, kind); };
+    auto visit = [&](int j, auto kind) { use(load(j, kind), kind, 0, 0); };
     // Hit mask of candidates [base, base + cnt) (cnt <= 32) of a row whose
-    // candidate u is off(u) + u, off(u) = fb for u < nfl, else dbo = db - nfl.
-    // Four candidates per round, their loads issued together.  Past the
-    // chunk's end a round reads the particles that follow the row's range
-    // (at most index N + 2: position arrays carry POS_PAD spare elements) and
-    // the final mask clears those bits; i itself is cleared by the caller.
+    // candidate u is off(u) + u, off(u) = fb for u < nfl, else dbo = db - nfl
+    // (rows of more than 32 candidates); a round past cnt reads up to three
+    // particles that follow (POS_PAD) and the final mask clears their bits.
     auto test_chunk = [&](int base, int cnt, int fb, int nfl, int dbo) {
         uint32_t m = 0;
         for (int t = 0; t < cnt; t += 4) {
@@ -485,8 +684,6 @@ __device__ __forceinline__ void for_each_neighbor(const Prm &q, int i, float4 xi
                 const int u = base + t + e;
                 xs[e] = ld_xyz(posr, ((u < nfl) ? fb : dbo) + u);
             }
-            // keep all four loads ahead of the tests (the scheduler would
-            // otherwise interleave them and wait after two)
             __builtin_amdgcn_sched_barrier(0);
             uint32_t nib = 0;
 #pragma unroll
@@ -515,30 +712,39 @@ __device__ __forceinline__ void for_each_neighbor(const Prm &q, int i, float4 xi
             while (m) {
                 const int u = base + __builtin_ctz(m);
                 m &= m - 1u;
-                visit(((u < nfl) ? w.fb : dbo) + u);
+                visit(((u < nfl) ? w.fb : dbo) + u, AnyJ{});
             }
         }
     };
     for_each_row_pair<FLUID_J, DUMMY_J>(q, xi, cell, cs, [&](const RowRange &A, int rA, const RowRange &B, int rB) {
-        const int nA = A.fe - A.fb, tA = nA + (A.de - A.db);
-        const int nB = B.fe - B.fb, tB = nB + (B.de - B.db);
+        const int tA = (A.fe - A.fb) + (A.de - A.db);
+        const int tB = (B.fe - B.fb) + (B.de - B.db);
+        StagedPair sp{};
+        if constexpr (STAGED) {
+            // wave-uniform: the pair's box-row offsets, and whether any lane has candidates
+            const int K = 2 * q.k + 1;
+            const bool useA = __any(tA > 0 && tA <= 32 && tB <= 32);
+            const bool useB = __any(tB > 0 && tA <= 32 && tB <= 32);
+            if (wr.ok && (useA || useB))
+                sp = stage_pair<DUMMY_J>(q, wr, cs, posr, stg.lds, rA / K - q.k, rA % K - q.k, useA, useB);
+        }
         if (tA <= 32 && tB <= 32) {
-            // both rows in one chunk: one 64-bit hit mask, A's hits first
-            const int dA = A.db - nA, dB = B.db - nB;
             uint32_t mA = 0, mB = 0;
             if (tA) {
-                mA = test_chunk(0, tA, A.fb, nA, dA);
+                if (STAGED && sp.ok) mA = test_row_lds<FLUID_J, DUMMY_J>(stg.lds, A, sp.sAf, sp.oAf, sp.sAd, sp.oAd, xi, q.rc2);
+                else mA = test_chunk(0, tA, A.fb, A.fe - A.fb, A.db - (A.fe - A.fb));
                 const int us = self_place(A);
                 if ((uint32_t)us < 32u) mA &= ~(1u << us);
                 rec(rA, 0, mA);
             }
             if (tB) {
-                mB = test_chunk(0, tB, B.fb, nB, dB);
+                if (STAGED && sp.ok) mB = test_row_lds<FLUID_J, DUMMY_J>(stg.lds, B, sp.sBf, sp.oBf, sp.sBd, sp.oBd, xi, q.rc2);
+                else mB = test_chunk(0, tB, B.fb, B.fe - B.fb, B.db - (B.fe - B.fb));
                 const int us = self_place(B);
                 if ((uint32_t)us < 32u) mB &= ~(1u << us);
                 rec(rB, 0, mB);
             }
-            visit_pair(mA, mB, A.fb, nA, dA, B.fb, nB, dB, visit);
+            visit_rows<FLUID_J, DUMMY_J>(mA, mB, A, B, load, use);
         } else {
             row_chunks(A, rA);
             if (rB >= 0) row_chunks(B, rB);
@@ -552,27 +758,28 @@ __device__ __forceinline__ void for_each_neighbor(const Prm &q, int i, float4 xi
 // their candidate ranges are the same) instead of tested, in the same order
 // (row pairs merged the same way).  Every row of the particle must have at
 // most 64 candidates.
-template <bool FLUID_J, bool DUMMY_J, class F>
+template <bool FLUID_J, bool DUMMY_J, class F, class Gath = NoGather>
 __device__ __forceinline__ void for_each_recorded_at(const Prm &q, int i, float4 xi, float4 xc, uint32_t cell,
                                                      const int *__restrict__ cs, const float4 *__restrict__ posr,
-                                                     const uint32_t *__restrict__ hits, size_t n, F &&f);
-template <bool FLUID_J, bool DUMMY_J, class F>
+                                                     const uint32_t *__restrict__ hits, size_t n, F &&f,
+                                                     Gath &&gath = Gath{});
+template <bool FLUID_J, bool DUMMY_J, class F, class Gath = NoGather>
 __device__ __forceinline__ void for_each_recorded(const Prm &q, int i, float4 xi, uint32_t cell,
                                                   const int *__restrict__ cs, const float4 *__restrict__ posr,
-                                                  const uint32_t *__restrict__ hits, size_t n, F &&f) {
-    for_each_recorded_at<FLUID_J, DUMMY_J>(q, i, xi, xi, cell, cs, posr, hits, n, f);
+                                                  const uint32_t *__restrict__ hits, size_t n, F &&f,
+                                                  Gath &&gath = Gath{}) {
+    for_each_recorded_at<FLUID_J, DUMMY_J>(q, i, xi, xi, cell, cs, posr, hits, n, f, gath);
 }
 // The same with the rows trimmed around xc (WALK_SKIN: the rebuild position)
 // while the pair geometry uses the current position xi.
-template <bool FLUID_J, bool DUMMY_J, class F>
+template <bool FLUID_J, bool DUMMY_J, class F, class Gath>
 __device__ __forceinline__ void for_each_recorded_at(const Prm &q, int i, float4 xi, float4 xc, uint32_t cell,
                                                      const int *__restrict__ cs, const float4 *__restrict__ posr,
-                                                     const uint32_t *__restrict__ hits, size_t n, F &&f) {
-    auto visit = [&](int j) {
-        const float4 xj = ld(posr, j);
-        const float dx = xi.x - xj.x, dy = xi.y - xj.y, dz = xi.z - xj.z;
-        f(j, dx, dy, dz, dist2(dx, dy, dz), xj);
-    };
+                                                     const uint32_t *__restrict__ hits, size_t n, F &&f,
+                                                     Gath &&gath) {
+    auto load = [&](int j, auto kind) { return load_hit(posr, gath, j, kind); };
+    auto use = [&](const auto &h, auto kind, int, int) { run_hit(f, xi, h, kind); };
+    auto visit = [&](int j, auto kind) { use(load(j, kind), kind, 0, 0); };
     // one row, chunk by chunk (m0: its first chunk's mask)
     auto row_chunks = [&](const RowRange &w, int r, uint32_t m0) {
         const int nfl = w.fe - w.fb;
@@ -582,7 +789,7 @@ __device__ __forceinline__ void for_each_recorded_at(const Prm &q, int i, float4
             while (m) {
                 const int u = base + __builtin_ctz(m);
                 m &= m - 1u;
-                visit(((u < nfl) ? w.fb : w.db - nfl) + u);
+                visit(((u < nfl) ? w.fb : w.db - nfl) + u, AnyJ{});
             }
         }
     };
@@ -591,13 +798,13 @@ __device__ __forceinline__ void for_each_recorded_at(const Prm &q, int i, float4
     for_each_row_pair<FLUID_J, DUMMY_J>(
         q, xc, cell, cs,
         [&](const RowRange &A, int rA, const RowRange &B, int rB) {
-            const int nA = A.fe - A.fb, tA = nA + (A.de - A.db);
-            const int nB = B.fe - B.fb, tB = nB + (B.de - B.db);
+            const int tA = (A.fe - A.fb) + (A.de - A.db);
+            const int tB = (B.fe - B.fb) + (B.de - B.db);
             // a row without candidates has no record (its load, if any, is stale)
             const uint32_t a = tA ? mA : 0u, b = tB ? mB : 0u;
             if (tA <= 32 && tB <= 32) {
                 if ((a | b) == 0u) return;  // no neighbour in either row
-                visit_pair(a, b, A.fb, nA, A.db - nA, B.fb, nB, B.db - nB, visit);
+                visit_rows<FLUID_J, DUMMY_J>(a, b, A, B, load, use);
             } else {
                 row_chunks(A, rA, a);
                 if (rB >= 0) row_chunks(B, rB, b);
@@ -616,21 +823,19 @@ __device__ __forceinline__ void for_each_recorded_at(const Prm &q, int i, float4
 // half step): each chunk's candidates are tested against qr.rcs2 from xc and
 // the skin mask goes to rec_skin(r, c, m); REPLAY = true: the skin mask is
 // read from skin[(2 r + c) n + i] (every row of the particle within 64
-// candidates).  Every skin hit is visited: f(j, dx, dy, dz, r2, xj) returns
-// whether j is a neighbour (the caller applies r^2 <= rc2), and the chunk's
+// candidates).  Every skin hit is visited: f(j, dx, dy, dz, r2, xj[, kind])
+// returns whether j is a neighbour (the caller applies r^2 <= rc2), and the chunk's
 // exact mask goes to rec_exact(r, c, m).  Row pairs and chunks are visited
 // as in for_each_neighbor, so the exact record replays in the same order.
-template <bool REPLAY, class F, class RS, class RE>
+template <bool REPLAY, class F, class RS, class RE, class Gath = NoGather>
 __device__ __forceinline__ void skin_walk(const Prm &qr, int i, float4 xi, float4 xc, uint32_t cell,
                                           const int *__restrict__ cs, const float4 *__restrict__ posr,
                                           const uint32_t *__restrict__ skin, size_t n, F &&f, RS &&rec_skin,
-                                          RE &&rec_exact) {
-    auto visit = [&](int j) -> bool {
-        const float4 xj = ld(posr, j);
-        const float dx = xi.x - xj.x, dy = xi.y - xj.y, dz = xi.z - xj.z;
-        return f(j, dx, dy, dz, dist2(dx, dy, dz), xj);
-    };
+                                          RE &&rec_exact, Gath &&gath = Gath{}) {
+    auto load = [&](int j, auto kind) { return load_hit(posr, gath, j, kind); };
+    auto visit = [&](int j, auto kind) -> bool { return run_hit(f, xi, load(j, kind), kind); };
     // skin mask of candidates [base, base + cnt) (cnt <= 32), tested from xc
+    // (rows of more than 32 candidates)
     auto test_chunk = [&](int base, int cnt, int fb, int nfl, int dbo) {
         uint32_t m = 0;
         for (int t = 0; t < cnt; t += 4) {
@@ -671,7 +876,7 @@ __device__ __forceinline__ void skin_walk(const Prm &qr, int i, float4 xi, float
                 const int b = __builtin_ctz(m);
                 m &= m - 1u;
                 const int u = base + b;
-                if (visit(((u < nfl) ? w.fb : dbo) + u)) e |= 1u << b;
+                if (visit(((u < nfl) ? w.fb : dbo) + u, AnyJ{})) e |= 1u << b;
             }
             rec_exact(r, c, e);
         }
@@ -680,41 +885,37 @@ __device__ __forceinline__ void skin_walk(const Prm &qr, int i, float4 xi, float
     for_each_row_pair<true, true>(
         qr, xc, cell, cs,
         [&](const RowRange &A, int rA, const RowRange &B, int rB) {
-            const int nA = A.fe - A.fb, tA = nA + (A.de - A.db);
-            const int nB = B.fe - B.fb, tB = nB + (B.de - B.db);
+            const int tA = (A.fe - A.fb) + (A.de - A.db);
+            const int tB = (B.fe - B.fb) + (B.de - B.db);
             uint32_t mA = 0, mB = 0;
             if constexpr (REPLAY) {  // a row without candidates has no record
                 mA = tA ? pA : 0u;
                 mB = tB ? pB : 0u;
             }
             if (tA <= 32 && tB <= 32) {
-                const int dA = A.db - nA, dB = B.db - nB;
                 if constexpr (!REPLAY) {
                     if (tA) {
-                        mA = test_chunk(0, tA, A.fb, nA, dA);
+                        mA = test_chunk(0, tA, A.fb, A.fe - A.fb, A.db - (A.fe - A.fb));
                         const int us = self_place(A);
                         if ((uint32_t)us < 32u) mA &= ~(1u << us);
                         rec_skin(rA, 0, mA);
                     }
                     if (tB) {
-                        mB = test_chunk(0, tB, B.fb, nB, dB);
+                        mB = test_chunk(0, tB, B.fb, B.fe - B.fb, B.db - (B.fe - B.fb));
                         const int us = self_place(B);
                         if ((uint32_t)us < 32u) mB &= ~(1u << us);
                         rec_skin(rB, 0, mB);
                     }
                 }
-                uint64_t m = (uint64_t)mA | ((uint64_t)mB << 32), e = 0;
-                while (m) {
-                    const int bit = __builtin_ctzll(m);
-                    m &= m - 1u;
-                    const bool hi = bit >= 32;
-                    const int u = bit & 31;
-                    const int nfl = hi ? nB : nA;
-                    const int j = ((u < nfl) ? (hi ? B.fb : A.fb) : (hi ? dB : dA)) + u;
-                    if (visit(j)) e |= 1ull << bit;
-                }
-                if (tA) rec_exact(rA, 0, (uint32_t)e);
-                if (tB) rec_exact(rB, 0, (uint32_t)(e >> 32));
+                uint32_t eA = 0, eB = 0;  // the exact masks, in the rows' candidate bits
+                visit_rows<true, true>(mA, mB, A, B, load, [&](const auto &h, auto kind, int side, int u) {
+                    if (run_hit(f, xi, h, kind)) {
+                        if (side) eB |= 1u << u;
+                        else eA |= 1u << u;
+                    }
+                });
+                if (tA) rec_exact(rA, 0, eA);
+                if (tB) rec_exact(rB, 0, eB);
             } else {
                 row_chunks(A, rA, mA);
                 if (rB >= 0) row_chunks(B, rB, mB);

@@ -20,6 +20,15 @@ namespace zs {
 
 static inline int nblocks(int n) { return (n + ZS_BLOCK - 1) / ZS_BLOCK; }
 
+// Waves per SIMD the single-scene plain gradient and continuity+momentum
+// passes are compiled for (a register budget of 512 / waves VGPRs).
+#ifndef ZS_LDG_WAVES
+#define ZS_LDG_WAVES 7
+#endif
+#ifndef ZS_GOV_WAVES
+#define ZS_GOV_WAVES 7
+#endif
+
 __device__ __forceinline__ float *wptr(float4 *a, int i) { return reinterpret_cast<float *>(a + i) + 3; }
 
 // Cell of a sort key, relative to the particle's scene (cell_base = 0 for one scene).
@@ -81,7 +90,7 @@ __global__ __launch_bounds__(BS) void k_wall(Prm q0, const Prm *__restrict__ sp,
 // own choice, 77, allows 6) in the single-scene mask and hit-mask walks:
 // measured 5 % faster.  8 waves (64 VGPRs) spill and measured slower.
 template <int WALK, bool BATCH, bool FEAT, int DF, int BS>
-__global__ __launch_bounds__(BS, ((WALK != WALK_LIST && !BATCH && !FEAT) ? 7 : 1)) void k_ldg(Prm q0, const Prm *__restrict__ sp, Buf b,
+__global__ __launch_bounds__(BS, ((WALK != WALK_LIST && !BATCH && !FEAT) ? ZS_LDG_WAVES : 1)) void k_ldg(Prm q0, const Prm *__restrict__ sp, Buf b,
                                                   const uint32_t *__restrict__ skey, const int *__restrict__ cs,
                                                   float4 *__restrict__ gd, int *__restrict__ nlist,
                                                   int *__restrict__ ncount, float *__restrict__ feat, SkinRec sr) {
@@ -111,7 +120,7 @@ __global__ __launch_bounds__(BS, ((WALK != WALK_LIST && !BATCH && !FEAT) ? 7 : 1
     float ws = 0, wv2 = 0, wdr = 0;
     int nnb = 0, ndum = 0;
     // vj: velp[j] (read only for fluid j)
-    auto body = [&](int j, float dx, float dy, float dz, float r2, float4 xj, float4 vj) {
+    auto body = [&](int j, float dx, float dy, float dz, float r2, float4 xj, float4 vj, auto kind) {
         const float qq = hw_sqrt(r2) * q.inv_h;
         const float t = 1.0f - 0.5f * qq;
         const float Vj = q.m * hw_rcp(xj.w);
@@ -131,7 +140,7 @@ __global__ __launch_bounds__(BS, ((WALK != WALK_LIST && !BATCH && !FEAT) ? 7 : 1
             wdr += WV * (xj.w - xi.w);
             ++nnb;
         }
-        if (j < nf) {
+        if (fluid_j(kind, j, nf)) {
             const float ux = vj.x - vi.x, uy = vj.y - vi.y, uz = vj.z - vi.z;
             D -= FV * ((vi.x - vj.x) * dx + (vi.y - vj.y) * dy + (vi.z - vj.z) * dz);
             if (FEAT) {
@@ -144,19 +153,40 @@ __global__ __launch_bounds__(BS, ((WALK != WALK_LIST && !BATCH && !FEAT) ? 7 : 1
             ++ndum;
         }
     };
-    auto walk_body = [&](int j, float dx, float dy, float dz, float r2, float4 xj) {
-        float4 vj;  // loaded, and read by the body, for fluid neighbours only
-        if (j < nf) vj = ld(b.velp, j);
-        body(j, dx, dy, dz, r2, xj, vj);
+    // gathered per neighbour besides its position: velp[j] for fluid j
+    auto gath = [&](int j, auto kind) {
+        if constexpr (decltype(kind)::value == 1) {
+            return Empty{};
+        } else if constexpr (decltype(kind)::value == 0) {
+            return ld(b.velp, j);
+        } else {
+            float4 v;  // loaded, and read by the body, for fluid neighbours only
+            if (j < nf) v = ld(b.velp, j);
+            return v;
+        }
+    };
+    auto walk_body = [&](int j, float dx, float dy, float dz, float r2, float4 xj, auto kind, const auto &e) {
+        float4 vj;
+        if constexpr (!std::is_same_v<std::decay_t<decltype(e)>, Empty>) vj = e;
+        body(j, dx, dy, dz, r2, xj, vj, kind);
     };
     const uint32_t cell = cell_of(q, skey[i]);
+    // the wave's LDS slice for staging its candidate rows (one scene: a batch's
+    // waves can span scenes with different grids)
+    __shared__ float4 stage_lds[BATCH ? 1 : (BS / 64) * (STAGE_SLOTS + STAGE_PAD)];
+    const auto stg = [&]() {
+        if constexpr (BATCH) return NoStage{};
+        else return LdsStage{stage_lds + (threadIdx.x >> 6) * (STAGE_SLOTS + STAGE_PAD)};
+    }();
     if (WALK == WALK_LIST) {  // the mask walk, recording what it visits
         ListWriter lw(nlist, i);
         for_each_neighbor<WALK_MASK, true, true>(q, i, xi, cell, cs, b.posr,
-                                                 [&](int j, float dx, float dy, float dz, float r2, float4 xj) {
+                                                 [&](int j, float dx, float dy, float dz, float r2, float4 xj,
+                                                     auto kind, const auto &e) {
                                                      lw.put(j);
-                                                     walk_body(j, dx, dy, dz, r2, xj);
-                                                 });
+                                                     walk_body(j, dx, dy, dz, r2, xj, kind, e);
+                                                 },
+                                                 NoRec{}, gath);
         ncount[i] = lw.finish();
     } else if constexpr (WALK == WALK_SKIN_REC || WALK == WALK_SKIN_REPLAY) {
         uint32_t *hits = reinterpret_cast<uint32_t *>(nlist);  // this half step's exact masks
@@ -164,9 +194,10 @@ __global__ __launch_bounds__(BS, ((WALK != WALK_LIST && !BATCH && !FEAT) ? 7 : 1
         Prm qr = q;  // rows around the rebuild position, trimmed to rc + s
         qr.trim2 = q.trim2_skin;
         const float4 xc = sr.xref[i];
-        auto exact = [&](int j, float dx, float dy, float dz, float r2, float4 xj) -> bool {
+        auto exact = [&](int j, float dx, float dy, float dz, float r2, float4 xj, auto kind,
+                         const auto &e) -> bool {
             if (!(r2 <= q.rc2)) return false;
-            walk_body(j, dx, dy, dz, r2, xj);
+            walk_body(j, dx, dy, dz, r2, xj, kind, e);
             return true;
         };
         auto rec_exact = [&](int r, int c, uint32_t m) {
@@ -179,13 +210,13 @@ __global__ __launch_bounds__(BS, ((WALK != WALK_LIST && !BATCH && !FEAT) ? 7 : 1
                                  if (c < 2) sr.skin[(size_t)(2 * r + c) * n + (size_t)i] = m;
                                  else over = true;
                              },
-                             rec_exact);
+                             rec_exact, gath);
             ncount[i] = over ? 1 : 0;
         } else if (ncount[i]) {  // not recorded at the rebuild: the mask walk (and so its successor)
-            for_each_neighbor<WALK_MASK, true, true>(q, i, xi, cell, cs, b.posr, walk_body);
+            for_each_neighbor<WALK_MASK, true, true>(q, i, xi, cell, cs, b.posr, walk_body, NoRec{}, gath);
         } else {
             skin_walk<true>(qr, i, xi, xc, cell, cs, b.posr, sr.skin, n, exact, [](int, int, uint32_t) {},
-                            rec_exact);
+                            rec_exact, gath);
         }
     } else if (WALK == WALK_HITS) {  // the mask walk, recording each chunk's hit mask
         uint32_t *hits = reinterpret_cast<uint32_t *>(nlist);
@@ -194,10 +225,11 @@ __global__ __launch_bounds__(BS, ((WALK != WALK_LIST && !BATCH && !FEAT) ? 7 : 1
                                                  [&](int r, int c, uint32_t m) {
                                                      if (c < 2) hits[(size_t)(2 * r + c) * (size_t)q0.N + (size_t)i] = m;
                                                      else over = true;
-                                                 });
+                                                 },
+                                                 gath, stg);
         ncount[i] = over ? 1 : 0;
     } else {
-        for_each_neighbor<WALK, true, true>(q, i, xi, cell, cs, b.posr, walk_body);
+        for_each_neighbor<WALK, true, true>(q, i, xi, cell, cs, b.posr, walk_body, NoRec{}, gath, stg);
     }
     const float trL = lxx + lyy + lzz;
     if (ell && q.dim == 2) lyy += 1.0f;  // x-z plane: unit yy entry keeps L invertible
@@ -264,7 +296,7 @@ __global__ __launch_bounds__(BS, ((WALK != WALK_LIST && !BATCH && !FEAT) ? 7 : 1
 // viscous (3)) for training.
 // At least 7 waves per SIMD in the single-scene plain walks (as k_ldg).
 template <int WALK, bool BATCH, int MODE, int DF, int BS>
-__global__ __launch_bounds__(BS, ((WALK != WALK_LIST && !BATCH && MODE == 0) ? 7 : 1)) void k_governing(Prm q0, const Prm *__restrict__ sp, Buf b,
+__global__ __launch_bounds__(BS, ((WALK != WALK_LIST && !BATCH && MODE == 0) ? ZS_GOV_WAVES : 1)) void k_governing(Prm q0, const Prm *__restrict__ sp, Buf b,
                                                         const uint32_t *__restrict__ skey, const int *__restrict__ cs,
                                                         const float4 *__restrict__ gd, float4 *__restrict__ acc,
                                                         const int *__restrict__ nlist, const int *__restrict__ ncount,
@@ -288,7 +320,7 @@ __global__ __launch_bounds__(BS, ((WALK != WALK_LIST && !BATCH && MODE == 0) ? 7
     float ax = 0, ay = 0, az = 0, dr = 0;
     float dd = 0, vx = 0, vy = 0, vz = 0;  // diffusion and viscous parts (MODE != 0)
     // gj: gd[j] (read only for fluid j)
-    auto body = [&](int j, float dx, float dy, float dz, float r2, float4 xj, float4 vj, float4 gj) {
+    auto body = [&](int j, float dx, float dy, float dz, float r2, float4 xj, float4 vj, float4 gj, auto kind) {
         const float qq = hw_sqrt(r2) * q.inv_h;
         const float t = 1.0f - 0.5f * qq;
         const float F = q.fk * t * t * t;
@@ -300,7 +332,7 @@ __global__ __launch_bounds__(BS, ((WALK != WALK_LIST && !BATCH && MODE == 0) ? 7
         const float irr = irhoi * irhoj;
         float pt = -(pi + pj) * q.m * F * irr;
         float pv = 0.0f;
-        if (j < nf) {
+        if (fluid_j(kind, j, nf)) {
             const float ire = hw_rcp(r2 + q.eta2);
             if (delta) {
                 const float psi = 2.0f * (rhoi - rhoj) * r2 * ire -
@@ -319,11 +351,26 @@ __global__ __launch_bounds__(BS, ((WALK != WALK_LIST && !BATCH && MODE == 0) ? 7
         ax += pt * dx; ay += pt * dy; az += pt * dz;
         if (MODE) { vx += pv * dx; vy += pv * dy; vz += pv * dz; }
     };
-    auto walk_body = [&](int j, float dx, float dy, float dz, float r2, float4 xj) {
-        const float4 vj = ld(b.velp, j);
-        float4 gj;  // loaded, and read by the body, for fluid neighbours only
-        if (j < nf) gj = ld(gd, j);
-        body(j, dx, dy, dz, r2, xj, vj, gj);
+    // gathered per neighbour besides its position: velp[j], and gd[j] for fluid j
+    auto gath = [&](int j, auto kind) {
+        if constexpr (decltype(kind)::value == 1) {
+            return ld(b.velp, j);
+        } else if constexpr (decltype(kind)::value == 0) {
+            return VelGrad{ld(b.velp, j), ld(gd, j)};
+        } else {
+            VelGrad e;
+            e.v = ld(b.velp, j);
+            if (j < nf) e.g = ld(gd, j);  // read by the body for fluid neighbours only
+            return e;
+        }
+    };
+    auto walk_body = [&](int j, float dx, float dy, float dz, float r2, float4 xj, auto kind, const auto &e) {
+        if constexpr (std::is_same_v<std::decay_t<decltype(e)>, float4>) {
+            float4 gj;  // not read for a dummy neighbour
+            body(j, dx, dy, dz, r2, xj, e, gj, kind);
+        } else {
+            body(j, dx, dy, dz, r2, xj, e.v, e.g, kind);
+        }
     };
     const uint32_t cell = cell_of(q, skey[i]);
     const int n = (WALK == WALK_LIST || WALK == WALK_HITS || WALK == WALK_SKIN) ? ncount[i] : 0;
@@ -331,10 +378,10 @@ __global__ __launch_bounds__(BS, ((WALK != WALK_LIST && !BATCH && MODE == 0) ? 7
         Prm qr = q;
         qr.trim2 = q.trim2_skin;
         for_each_recorded_at<true, true>(qr, i, xi, xref[i], cell, cs, b.posr,
-                                         reinterpret_cast<const uint32_t *>(nlist), (size_t)q0.N, walk_body);
+                                         reinterpret_cast<const uint32_t *>(nlist), (size_t)q0.N, walk_body, gath);
     } else if (WALK == WALK_HITS && n == 0) {
         for_each_recorded<true, true>(q, i, xi, cell, cs, b.posr, reinterpret_cast<const uint32_t *>(nlist),
-                                      (size_t)q0.N, walk_body);
+                                      (size_t)q0.N, walk_body, gath);
     } else if (WALK == WALK_LIST && n <= LIST_MAX) {
         // R quads (4R recorded neighbours) per round: the round's list reads
         // and all its gathers are issued before its first interaction, so
@@ -363,12 +410,12 @@ __global__ __launch_bounds__(BS, ((WALK != WALK_LIST && !BATCH && MODE == 0) ? 7
             for (int e = 0; e < 4 * R; ++e) {
                 if (k + e < n) {
                     const float dx = xi.x - xs[e].x, dy = xi.y - xs[e].y, dz = xi.z - xs[e].z;
-                    body(js[e], dx, dy, dz, dist2(dx, dy, dz), xs[e], vs[e], gs[e]);
+                    body(js[e], dx, dy, dz, dist2(dx, dy, dz), xs[e], vs[e], gs[e], AnyJ{});
                 }
             }
         }
     } else {
-        for_each_neighbor<WALK_MASK, true, true>(q, i, xi, cell, cs, b.posr, walk_body);
+        for_each_neighbor<WALK_MASK, true, true>(q, i, xi, cell, cs, b.posr, walk_body, NoRec{}, gath);
     }
     if constexpr (MODE == 0) {
         acc[i] = make_float4(ax, ay, az + q.g, dr);
