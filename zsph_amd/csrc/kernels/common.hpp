@@ -497,12 +497,28 @@ __device__ __forceinline__ uint32_t high_bits(uint32_t m, int n) { return n >= 3
 // Bit u of mA is candidate u of row A: the fluid particle A.fb + u for
 // u < nA = A.fe - A.fb, else the other particle A.db + u - nA; the same for
 // B.  For each hit, in that order, use(load(j, kind), kind, side, u) runs
-// (kind AnyJ; side 0 for row A, 1 for B; u = the hit's bit in its row's
-// mask).
+// (side 0 for row A, 1 for B; u = the hit's bit in its row's mask).  When
+// no lane of the wave has a non-fluid candidate in either row and every
+// lane's hits fit in 32 bits with B's placed above A's candidates, the
+// sequence runs on that 32-bit mask with kind FluidJ; otherwise on the
+// 64-bit mask (mA | mB << 32) with kind AnyJ.  Both give the same order.
 template <bool FLUID_J, bool DUMMY_J, class L, class U>
 __device__ __forceinline__ void visit_rows(uint32_t mA, uint32_t mB, const RowRange &A, const RowRange &B, L &&load,
                                            U &&use) {
     const int nA = A.fe - A.fb, nB = B.fe - B.fb;
+    const int bw = mB ? 32 - __clz(mB) : 0;  // B's highest hit + 1
+    const bool fluid_only = !DUMMY_J || (A.de == A.db && B.de == B.db);
+    if (FLUID_J && __all(fluid_only && nA + bw <= 32)) {
+        uint32_t m = mA | (mB ? mB << nA : 0u);
+        const int ob = B.fb - nA;
+        while (m) {
+            const int t = __builtin_ctz(m);
+            m &= m - 1u;
+            const bool hi = t >= nA;
+            use(load((hi ? ob : A.fb) + t, FluidJ{}), FluidJ{}, (int)hi, hi ? t - nA : t);
+        }
+        return;
+    }
     const int dA = A.db - nA, dB = B.db - nB;
     uint64_t m = (uint64_t)mA | ((uint64_t)mB << 32);
     while (m) {
@@ -512,6 +528,53 @@ __device__ __forceinline__ void visit_rows(uint32_t mA, uint32_t mB, const RowRa
         const int u = t & 31;
         const int j = ((u < (hi ? nB : nA)) ? (hi ? B.fb : A.fb) : (hi ? dB : dA)) + u;
         use(load(j, AnyJ{}), AnyJ{}, (int)hi, u);
+    }
+}
+
+// visit_rows with the positions read from the staged rows: the hit of row A's
+// fluid range at u is LDS slot fA + u, of its other range slot dA + u (the
+// same for B).  The hit's gathered data is gath(j, kind), or, when the slice
+// also holds the records of src2, gath_v(j, kind, staged record).  Same order
+// and kinds as visit_rows.
+template <bool FLUID_J, bool DUMMY_J, class Stg, class G, class GV, class U>
+__device__ __forceinline__ void visit_rows_lds(uint32_t mA, uint32_t mB, const RowRange &A, const RowRange &B,
+                                               const Stg &stg, int fA, int dA_, int fB, int dB_, G &gath, GV &gath_v,
+                                               U &&use) {
+    const int nA = A.fe - A.fb, nB = B.fe - B.fb;
+    auto hit = [&](int j, int slot, auto kind) {
+        if constexpr (std::is_invocable_v<GV &, int, decltype(kind), float4>) {
+            using E = decltype(gath_v(j, kind, float4{}));
+            return Hit<E>{j, stg.lds[slot], gath_v(j, kind, stg.lds2[slot])};
+        } else {
+            using E = decltype(gath(j, kind));
+            return Hit<E>{j, stg.lds[slot], gath(j, kind)};
+        }
+    };
+    const int bw = mB ? 32 - __clz(mB) : 0;  // B's highest hit + 1
+    const bool fluid_only = !DUMMY_J || (A.de == A.db && B.de == B.db);
+    if (FLUID_J && __all(fluid_only && nA + bw <= 32)) {
+        uint32_t m = mA | (mB ? mB << nA : 0u);
+        const int ob = B.fb - nA, sob = fB - nA;
+        while (m) {
+            const int t = __builtin_ctz(m);
+            m &= m - 1u;
+            const bool hi = t >= nA;
+            use(hit((hi ? ob : A.fb) + t, (hi ? sob : fA) + t, FluidJ{}), FluidJ{}, (int)hi, hi ? t - nA : t);
+        }
+        return;
+    }
+    const int dA = A.db - nA, dB = B.db - nB;
+    const int sdA = dA_ - nA, sdB = dB_ - nB;
+    uint64_t m = (uint64_t)mA | ((uint64_t)mB << 32);
+    while (m) {
+        const int t = __builtin_ctzll(m);
+        m &= m - 1u;
+        const bool hi = t >= 32;
+        const int u = t & 31;
+        const bool fl = u < (hi ? nB : nA);
+        const int j = (fl ? (hi ? B.fb : A.fb) : (hi ? dB : dA)) + u;
+        const int slot = (fl ? (hi ? fB : fA) : (hi ? sdB : sdA)) + u;
+        use(hit(j, slot, AnyJ{}), AnyJ{}, (int)hi, u);
     }
 }
 
@@ -535,9 +598,18 @@ constexpr int STAGE_SLOTS = 256;  // records per wave: the four ranges of a row 
 constexpr int STAGE_PAD = 4;      // a test round reads up to three records past a row's end
 struct NoStage {
     float4 *lds = nullptr;  // unused
+    float4 *lds2 = nullptr;
+    const float4 *src2 = nullptr;
+    int cap = 0;
 };
+// A wave's LDS slice: cap + STAGE_PAD position records (lds) and, when lds2
+// is set, the same records of the array src2 (the continuity+momentum pass
+// stages velocities and pressures with the positions).
 struct LdsStage {
-    float4 *lds;  // this wave's STAGE_SLOTS + STAGE_PAD records
+    float4 *lds;
+    float4 *lds2 = nullptr;
+    const float4 *src2 = nullptr;
+    int cap = STAGE_SLOTS;
 };
 // The wave-uniform cell row of the walk (ok: every active lane lies in it).
 struct WaveRow {
@@ -585,8 +657,8 @@ __device__ __forceinline__ void stage_range(const Prm &q, const WaveRow &w, cons
 // exceed STAGE_SLOTS.  Every active lane of the wave takes part.
 template <bool DUMMY_J>
 __device__ __forceinline__ StagedPair stage_pair(const Prm &q, const WaveRow &w, const int *__restrict__ cs,
-                                                 const float4 *__restrict__ posr, float4 *lds, int dz, int dy,
-                                                 bool useA, bool useB) {
+                                                 const float4 *__restrict__ posr, const LdsStage &stg, int dz,
+                                                 int dy, bool useA, bool useB) {
     StagedPair p;
     int eAf, eAd, eBf, eBd;
     p.sAf = p.sAd = p.sBf = p.sBd = 0;
@@ -599,7 +671,7 @@ __device__ __forceinline__ StagedPair stage_pair(const Prm &q, const WaveRow &w,
     p.oBf = lAf + lAd;
     p.oBd = p.oBf + lBf;
     const int total = p.oBd + lBd;
-    p.ok = total <= STAGE_SLOTS;
+    p.ok = total <= stg.cap;
     if (!p.ok) return p;
     const uint64_t ex = __builtin_amdgcn_read_exec();
     const int rank = (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(ex >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)ex, 0u));
@@ -612,7 +684,8 @@ __device__ __forceinline__ StagedPair stage_pair(const Prm &q, const WaveRow &w,
                         : (t < p.oBf) ? p.sAd + (t - p.oAd)
                         : (t < p.oBd) ? p.sBf + (t - p.oBf)
                                       : p.sBd + (t - p.oBd);
-        lds[t] = ld(posr, src);
+        stg.lds[t] = ld(posr, src);
+        if (stg.lds2) stg.lds2[t] = ld(stg.src2, src);
     }
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
@@ -726,7 +799,7 @@ __device__ __forceinline__ void for_each_neighbor(const Prm &q, int i, float4 xi
             const bool useA = __any(tA > 0 && tA <= 32 && tB <= 32);
             const bool useB = __any(tB > 0 && tA <= 32 && tB <= 32);
             if (wr.ok && (useA || useB))
-                sp = stage_pair<DUMMY_J>(q, wr, cs, posr, stg.lds, rA / K - q.k, rA % K - q.k, useA, useB);
+                sp = stage_pair<DUMMY_J>(q, wr, cs, posr, stg, rA / K - q.k, rA % K - q.k, useA, useB);
         }
         if (tA <= 32 && tB <= 32) {
             uint32_t mA = 0, mB = 0;
@@ -758,25 +831,28 @@ __device__ __forceinline__ void for_each_neighbor(const Prm &q, int i, float4 xi
 // their candidate ranges are the same) instead of tested, in the same order
 // (row pairs merged the same way).  Every row of the particle must have at
 // most 64 candidates.
-template <bool FLUID_J, bool DUMMY_J, class F, class Gath = NoGather>
+template <bool FLUID_J, bool DUMMY_J, class F, class Gath = NoGather, class Stg = NoStage, class GV = NoGather>
 __device__ __forceinline__ void for_each_recorded_at(const Prm &q, int i, float4 xi, float4 xc, uint32_t cell,
                                                      const int *__restrict__ cs, const float4 *__restrict__ posr,
                                                      const uint32_t *__restrict__ hits, size_t n, F &&f,
-                                                     Gath &&gath = Gath{});
-template <bool FLUID_J, bool DUMMY_J, class F, class Gath = NoGather>
+                                                     Gath &&gath = Gath{}, Stg stg = Stg{}, GV &&gath_v = GV{});
+template <bool FLUID_J, bool DUMMY_J, class F, class Gath = NoGather, class Stg = NoStage, class GV = NoGather>
 __device__ __forceinline__ void for_each_recorded(const Prm &q, int i, float4 xi, uint32_t cell,
                                                   const int *__restrict__ cs, const float4 *__restrict__ posr,
                                                   const uint32_t *__restrict__ hits, size_t n, F &&f,
-                                                  Gath &&gath = Gath{}) {
-    for_each_recorded_at<FLUID_J, DUMMY_J>(q, i, xi, xi, cell, cs, posr, hits, n, f, gath);
+                                                  Gath &&gath = Gath{}, Stg stg = Stg{}, GV &&gath_v = GV{}) {
+    for_each_recorded_at<FLUID_J, DUMMY_J>(q, i, xi, xi, cell, cs, posr, hits, n, f, gath, stg, gath_v);
 }
 // The same with the rows trimmed around xc (WALK_SKIN: the rebuild position)
 // while the pair geometry uses the current position xi.
-template <bool FLUID_J, bool DUMMY_J, class F, class Gath>
+template <bool FLUID_J, bool DUMMY_J, class F, class Gath, class Stg, class GV>
 __device__ __forceinline__ void for_each_recorded_at(const Prm &q, int i, float4 xi, float4 xc, uint32_t cell,
                                                      const int *__restrict__ cs, const float4 *__restrict__ posr,
                                                      const uint32_t *__restrict__ hits, size_t n, F &&f,
-                                                     Gath &&gath) {
+                                                     Gath &&gath, Stg stg, GV &&gath_v) {
+    constexpr bool STAGED = std::is_same_v<Stg, LdsStage>;
+    WaveRow wr{};
+    if constexpr (STAGED) wr = wave_row(q, cell);
     auto load = [&](int j, auto kind) { return load_hit(posr, gath, j, kind); };
     auto use = [&](const auto &h, auto kind, int, int) { run_hit(f, xi, h, kind); };
     auto visit = [&](int j, auto kind) { use(load(j, kind), kind, 0, 0); };
@@ -802,9 +878,22 @@ __device__ __forceinline__ void for_each_recorded_at(const Prm &q, int i, float4
             const int tB = (B.fe - B.fb) + (B.de - B.db);
             // a row without candidates has no record (its load, if any, is stale)
             const uint32_t a = tA ? mA : 0u, b = tB ? mB : 0u;
-            if (tA <= 32 && tB <= 32) {
+            const bool fast = tA <= 32 && tB <= 32;
+            StagedPair sp{};
+            if constexpr (STAGED) {  // wave-uniform, before any lane leaves
+                const int K = 2 * q.k + 1;
+                const bool useA = __any(fast && a != 0u), useB = __any(fast && b != 0u);
+                if (wr.ok && (useA || useB))
+                    sp = stage_pair<DUMMY_J>(q, wr, cs, posr, stg, rA / K - q.k, rA % K - q.k, useA, useB);
+            }
+            if (fast) {
                 if ((a | b) == 0u) return;  // no neighbour in either row
-                visit_rows<FLUID_J, DUMMY_J>(a, b, A, B, load, use);
+                if (STAGED && sp.ok)
+                    visit_rows_lds<FLUID_J, DUMMY_J>(a, b, A, B, stg, sp.oAf + (A.fb - sp.sAf), sp.oAd + (A.db - sp.sAd),
+                                                     sp.oBf + (B.fb - sp.sBf), sp.oBd + (B.db - sp.sBd), gath, gath_v,
+                                                     use);
+                else
+                    visit_rows<FLUID_J, DUMMY_J>(a, b, A, B, load, use);
             } else {
                 row_chunks(A, rA, a);
                 if (rB >= 0) row_chunks(B, rB, b);

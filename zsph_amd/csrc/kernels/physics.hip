@@ -21,12 +21,20 @@ namespace zs {
 static inline int nblocks(int n) { return (n + ZS_BLOCK - 1) / ZS_BLOCK; }
 
 // Waves per SIMD the single-scene plain gradient and continuity+momentum
-// passes are compiled for (a register budget of 512 / waves VGPRs).
+// passes are compiled for (a register budget of 512 / waves VGPRs), and the
+// records per wave the continuity+momentum pass stages in LDS (positions and
+// velocities of a row pair; larger pairs are gathered from L2): 6 waves and
+// 192 records fill the CU's 160 KB of LDS without spilling registers
+// (measured 8 % faster than 7 waves with 160 records, which spills, and than
+// 5 waves with 240).
 #ifndef ZS_LDG_WAVES
 #define ZS_LDG_WAVES 7
 #endif
 #ifndef ZS_GOV_WAVES
-#define ZS_GOV_WAVES 7
+#define ZS_GOV_WAVES 6
+#endif
+#ifndef ZS_GOV_SLOTS
+#define ZS_GOV_SLOTS 192
 #endif
 
 __device__ __forceinline__ float *wptr(float4 *a, int i) { return reinterpret_cast<float *>(a + i) + 3; }
@@ -374,14 +382,37 @@ __global__ __launch_bounds__(BS, ((WALK != WALK_LIST && !BATCH && MODE == 0) ? Z
     };
     const uint32_t cell = cell_of(q, skey[i]);
     const int n = (WALK == WALK_LIST || WALK == WALK_HITS || WALK == WALK_SKIN) ? ncount[i] : 0;
+    // the wave's LDS slices for staging its rows' positions and velocities (one scene)
+    constexpr int GS = ZS_GOV_SLOTS + STAGE_PAD;
+    __shared__ float4 stage_x[BATCH ? 1 : (BS / 64) * GS];
+    __shared__ float4 stage_v[BATCH ? 1 : (BS / 64) * GS];
+    const auto stg = [&]() {
+        if constexpr (BATCH) return NoStage{};
+        else return LdsStage{stage_x + (threadIdx.x >> 6) * GS, stage_v + (threadIdx.x >> 6) * GS, b.velp,
+                             ZS_GOV_SLOTS};
+    }();
+    // the gathered data of a hit whose velocity is staged: gd[j] for fluid j
+    auto gath_v = [&](int j, auto kind, float4 v) {
+        if constexpr (decltype(kind)::value == 1) {
+            return v;
+        } else if constexpr (decltype(kind)::value == 0) {
+            return VelGrad{v, ld(gd, j)};
+        } else {
+            VelGrad e;
+            e.v = v;
+            if (j < nf) e.g = ld(gd, j);  // read by the body for fluid neighbours only
+            return e;
+        }
+    };
     if (WALK == WALK_SKIN && n == 0) {  // the exact record, rows around the rebuild position
         Prm qr = q;
         qr.trim2 = q.trim2_skin;
         for_each_recorded_at<true, true>(qr, i, xi, xref[i], cell, cs, b.posr,
-                                         reinterpret_cast<const uint32_t *>(nlist), (size_t)q0.N, walk_body, gath);
+                                         reinterpret_cast<const uint32_t *>(nlist), (size_t)q0.N, walk_body, gath,
+                                         stg, gath_v);
     } else if (WALK == WALK_HITS && n == 0) {
         for_each_recorded<true, true>(q, i, xi, cell, cs, b.posr, reinterpret_cast<const uint32_t *>(nlist),
-                                      (size_t)q0.N, walk_body, gath);
+                                      (size_t)q0.N, walk_body, gath, stg, gath_v);
     } else if (WALK == WALK_LIST && n <= LIST_MAX) {
         // R quads (4R recorded neighbours) per round: the round's list reads
         // and all its gathers are issued before its first interaction, so
