@@ -243,18 +243,18 @@ def test_pipeline_small_scale_learns_something():
 
 
 def test_trained_closure_gains_over_plain_coarse():
-    # The committed network (trained on seeded columns other than this one) on the
-    # held-out 3-D dam break, through the first far-wall impact and run-up (t = 0.9 s;
-    # the surge reaches the wall near 0.62 s): coarse (dx 0.016) runs against the fine
-    # plain run (dx 0.008).  Gate: front error <= 2 % at >= 5x less stepping time,
-    # which the plain coarse method fails (2.14 %), and smoothed (4 ms) peak
-    # pressures closer to the fine run's than the plain coarse ones, on average over
-    # the four far-wall probes.  Measured: front 1.32 % (plain 2.14 %); peak-pressure
-    # errors 16 / 182 / 7 / 23 % at z = 0.01 / 0.05 / 0.10 / 0.20 m (plain 24 / 734 /
-    # 41 / 17 %).  A single probe's error moves with the FP32 summation order of the
-    # neighbour sums (the impact is sensitive to it): with the fluid and dummy
-    # neighbours visited interleaved the same runs gave 5 / 419 / 11 / 16 % (plain 23 /
-    # 746 / 44 / 28 %), so no probe is gated alone.
+    # The committed network (trained on seeded columns other than this one, on snapshots
+    # through the first far-wall impact) on the held-out 3-D dam break, through the impact
+    # and run-up (t = 0.9 s; the surge reaches the wall near 0.62 s): coarse (dx 0.016) runs
+    # against the fine plain run (dx 0.008).  Gate: front error <= 2 % at >= 5x less
+    # stepping time, which the plain coarse method fails (2.14 %), and smoothed (4 ms) peak
+    # pressures closer to the fine run's than the plain coarse ones, on average over the
+    # four far-wall probes.  Measured: front 0.87 % (plain 2.14 %); peak-pressure errors
+    # 34 / 173 / 21 / 21 % at z = 0.01 / 0.05 / 0.10 / 0.20 m (plain 23 / 746 / 44 / 28 %).
+    # A single probe's error moves with the FP32 summation order of the neighbour sums: the
+    # fine run itself, rerun with its sums in another order (skin mode), moves its peaks by
+    # 0.2 / 3.2 / 1.6 / 8.7 %, and the learned coarse run by 7 / 7 / 15 / 22 %, so no probe
+    # is gated alone.
     import os
     from zsph_amd import cases
     path = os.path.join(os.path.dirname(closure.__file__), "data", "closure_dambreak3d.npz")

@@ -356,12 +356,15 @@ def far_wall_probes(case):
     return np.array([[far_wall(case), y, z] for z in FAR_WALL_PROBES_Z], np.float32)
 
 
-def run_history(case, t_end, net=None, sample_dt=2.5e-4, front_zmax=0.05, log=print):
-    """Front x(t) (fluid below front_zmax) and far-wall probe pressures of one run."""
+def run_history(case, t_end, net=None, sample_dt=2.5e-4, front_zmax=0.05, log=print, skin=0.0):
+    """Front x(t) (fluid below front_zmax) and far-wall probe pressures of one run
+    (skin > 0: zsph_step's skin mode, the same physics summed in another order)."""
     import time as _time
     from . import Solver
     s = Solver(case)
     s.upload(case.host)
+    if skin > 0.0:
+        s.set_skin(skin)
     if net is not None:
         s.set_closure(net)
     dt = 2.0 * float(np.float32(case.params.half_dt))
