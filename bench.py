@@ -120,7 +120,7 @@ def kernel_roofline(name, stage_ms, launches, nbytes, rec, bound):
     out = {"bound": bound, "kernel": name, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
            "frac": achieved / HBM_PEAK_GBS, "traffic": None, "bytes_per_launch": nbytes,
            "avg_launch_ms": stage_ms, "launches": launches,
-           "timing": "device wall-clock stamps around every launch of the timed region"}
+           "timing": "device wall clock over every launch of the timed region, stamped by the kernels' first blocks"}
     if rec and name in rec:
         r = rec[name]
         out["traffic"] = r["traffic"]

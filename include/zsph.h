@@ -592,10 +592,13 @@ int zsph_set_graph(zsph_ctx *ctx, int on);
 
 /* Kernel clock of the continuity+momentum kernel (no counterpart in the
  * reference): with it on, every launch of that kernel -- in zsph_step's
- * captured graph too -- is bracketed by two one-thread kernels that read the
- * device's constant-rate wall clock, so its average duration over a long
- * graph-replayed run can be measured without a profiler.  Each stamp pair
- * adds about two microseconds per half step.  zsph_clock_read returns the
+ * captured graph too -- is timed on the device's constant-rate wall clock, so
+ * its average duration over a long graph-replayed run can be measured without
+ * a profiler.  The kernels stamp the clock themselves (the first thread of the
+ * first block of the pass and of the kernel that follows it), so no kernel is
+ * added to the stream; an interval includes one launch gap.  With the learned
+ * closure on, the gradient pass's interval also holds the network's
+ * inference, which runs between the two passes.  zsph_clock_read returns the
  * accumulated milliseconds and launch count and resets them (either pointer
  * may be NULL); zsph_clock_governing also resets. */
 int zsph_clock_governing(zsph_ctx *ctx, int on);

@@ -30,6 +30,7 @@ __device__ __forceinline__ V3 f3(float4 a) { return {(double)a.x, (double)a.y, (
 __global__ __launch_bounds__(ZS_BLOCK) void k_body_reduce(Prm q, Buf b, const int *__restrict__ cs,
                                                           const float4 *__restrict__ acc,
                                                           const DevBody *__restrict__ body, double *__restrict__ part) {
+    stage_stamp(q, ZSPH_ST_GOVERNING, -1);  // the integrator follows the continuity+momentum pass
     __shared__ double red[BODY_D][ZS_BLOCK / 64];
     const int nf = cs[q.G];
     const V3 c = rb::v3(body->s.c);

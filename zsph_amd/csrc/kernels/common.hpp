@@ -41,6 +41,11 @@ struct Prm {
     // padded like trim2), and its rebuild records every candidate within
     // rc + s (rcs2 = (rc + s)^2); 0 outside skin mode.
     float trim2_skin, rcs2;
+    // Stage clocks of the neighbour passes (clk nullptr: off): three words per
+    // stage ZSPH_ST_* (open time, ticks, launches) for the stages whose bit is
+    // set in clk_mask; see stage_stamp.  (clk_mask first: no padding.)
+    unsigned clk_mask;
+    unsigned long long *clk;
 };
 
 // Host-side construction (also used by the C ABI layer).
@@ -269,6 +274,29 @@ __device__ __forceinline__ bool skin_moved(const Prm &q, const SkinCheck &k, int
     int c;
     return !(cell_axis(x.x, q.gx0, q.kill_size, q.kill_X, &c) && cell_axis(x.y, q.gy0, q.kill_size, q.kill_Y, &c) &&
              cell_axis(x.z, q.gz0, q.kill_size, q.kill_Z, &c));
+}
+
+// Stage clock stamp, taken by the first thread of the first block of a kernel
+// (blocks are dispatched in index order, so block 0 starts when the kernel
+// does, right after the previous kernel on the stream has finished): closes
+// the interval of stage close_st opened by an earlier kernel (its ticks and
+// launch count grow; a stage not open is left alone) and opens stage
+// open_st (-1: none).  A stage's interval thus runs from the start of its
+// first kernel to the start of the kernel after its last one: the kernels'
+// time plus one launch gap, with no clock kernels of its own in the stream.
+__device__ __forceinline__ void stage_stamp(const Prm &q, int close_st, int open_st) {
+    if (!q.clk || blockIdx.x != 0 || threadIdx.x != 0) return;
+    const unsigned long long now = (unsigned long long)wall_clock64();
+    if (close_st >= 0 && (q.clk_mask >> close_st & 1u)) {
+        unsigned long long *w = q.clk + 3 * close_st;
+        const unsigned long long t0 = w[0];
+        if (t0) {
+            w[1] += now - t0;
+            w[2] += 1ull;
+            w[0] = 0ull;
+        }
+    }
+    if (open_st >= 0 && (q.clk_mask >> open_st & 1u)) q.clk[3 * open_st] = now;
 }
 
 // One store of the flag per wave whose lanes found a particle that moved.

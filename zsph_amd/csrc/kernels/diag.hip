@@ -196,18 +196,6 @@ void launch_filter(hipStream_t s, const Prm &qf, const Buf &b, const uint32_t *s
                                                                      radius * radius, out);
 }
 
-// Kernel clock: device wall-clock stamps around one kernel inside the
-// captured step (one thread each; c = {start, accumulated ticks, launches}).
-__global__ void k_clock_start(unsigned long long *__restrict__ c) { c[0] = (unsigned long long)wall_clock64(); }
-__global__ void k_clock_stop(unsigned long long *__restrict__ c) {
-    c[1] += (unsigned long long)wall_clock64() - c[0];
-    c[2] += 1ull;
-}
-void launch_clock(hipStream_t s, unsigned long long *c, bool start) {
-    if (start) k_clock_start<<<1, 1, 0, s>>>(c);
-    else k_clock_stop<<<1, 1, 0, s>>>(c);
-}
-
 int diag_blocks(int n) { return (n + ZS_BLOCK - 1) / ZS_BLOCK; }
 
 void launch_diag(hipStream_t s, const Prm &q, const Buf &b, float front_zmax, double *part, float *pmax) {

@@ -49,13 +49,12 @@ void launch_ldg(hipStream_t s, int walk, int block, const Prm &q, const Prm *sp,
                 const int *cs, float4 *gd, int *nlist, int *ncount, float *feat);
 // mode (one scene only): 0 plain, 1 closure-scaled (scale: 2 floats per
 // particle), 2 plain + term decomposition (terms: 8 floats per particle).
-// clock (may be nullptr): device stamps around the continuity+momentum kernel (launch_clock).
 // moving: the context has moving dummies (a body or a piston), whose
 // pressure force k_governing_dummy computes; fixed walls get zero either way.
 // xref (WALK_SKIN only): the rebuild positions its rows are centred on.
 void launch_governing(hipStream_t s, int walk, int block, const Prm &q, const Prm *sp, const Buf &b, const uint32_t *skey,
                       const int *cs, const float4 *gd, float4 *acc, const int *nlist, const int *ncount, int mode,
-                      const float2 *scale, float *terms, unsigned long long *clock, bool moving,
+                      const float2 *scale, float *terms, bool moving,
                       const float4 *xref = nullptr);
 // WALK_SKIN's gradient pass (one scene, plain): the recording form on rebuild
 // half steps, the replaying form on the others (gated on *sr.rebuilt);
@@ -79,8 +78,6 @@ void launch_probe(hipStream_t s, const Prm &q, const Buf &b, const float *pts, i
 // buffer's grid with k and trim2 covering `radius`); out: nq x 9 floats.
 void launch_filter(hipStream_t s, const Prm &qf, const Buf &b, const uint32_t *skey, const int *cs, const float4 *acc,
                    const int *rows, int nq, float hc, float radius, float *out);
-// Device wall-clock stamp (start) or accumulation (stop) on stream s.
-void launch_clock(hipStream_t s, unsigned long long *c, bool start);
 // Per-scene diagnostics from the cell table of buffer b (one block per scene).
 constexpr int SCENE_D = 8;  // count, kinetic, momentum x/y/z, front, zmax, removed
 void launch_scene_diag(hipStream_t s, const Prm &q, const Prm *sp, int nscenes, const Buf &b, const int *cs,

@@ -64,6 +64,7 @@ __device__ __forceinline__ bool is_antuono(const Prm &q) {
 template <int WALK, bool BATCH, int BS>
 __global__ __launch_bounds__(BS) void k_wall(Prm q0, const Prm *__restrict__ sp, Buf b,
                                              const uint32_t *__restrict__ skey, const int *__restrict__ cs) {
+    stage_stamp(q0, -1, ZSPH_ST_WALL);
     const int nf = cs[q0.G];
     // dummy blocks round-robin over the XCDs: dummies far from the fluid
     // cost little, so contiguous ranges of them would load the XCDs unevenly
@@ -102,6 +103,8 @@ __global__ __launch_bounds__(BS, ((WALK != WALK_LIST && !BATCH && !FEAT) ? ZS_LD
                                                   const uint32_t *__restrict__ skey, const int *__restrict__ cs,
                                                   float4 *__restrict__ gd, int *__restrict__ nlist,
                                                   int *__restrict__ ncount, float *__restrict__ feat, SkinRec sr) {
+    // the stage clocks: the recording form of WALK_SKIN is launched first, the replaying form right after it
+    if constexpr (WALK != WALK_SKIN_REPLAY) stage_stamp(q0, ZSPH_ST_WALL, ZSPH_ST_LDG);
     // WALK_SKIN: the recording form runs on rebuild half steps, the replaying form on the others
     if constexpr (WALK == WALK_SKIN_REC) {
         if (*sr.rebuilt == 0) return;
@@ -310,6 +313,7 @@ __global__ __launch_bounds__(BS, ((WALK != WALK_LIST && !BATCH && MODE == 0) ? Z
                                                         const int *__restrict__ nlist, const int *__restrict__ ncount,
                                                         const float2 *__restrict__ scale, float *__restrict__ terms,
                                                         const float4 *__restrict__ xref) {
+    stage_stamp(q0, ZSPH_ST_LDG, ZSPH_ST_GOVERNING);
     const int nf = cs[q0.G];
     const int i = xcd_block(blocks_of_bs<BS>(nf)) * BS + threadIdx.x;
     if (i >= nf) {
@@ -468,6 +472,7 @@ template <int WALK, bool BATCH>
 __global__ __launch_bounds__(ZS_BLOCK) void k_governing_dummy(Prm q0, const Prm *__restrict__ sp, Buf b,
         const uint32_t *__restrict__ skey,
                                                               const int *__restrict__ cs, float4 *__restrict__ acc) {
+    stage_stamp(q0, ZSPH_ST_GOVERNING, -1);
     const int nf = cs[q0.G];
     const int i = nf + blockIdx.x * ZS_BLOCK + threadIdx.x;
     if (i >= q0.N) return;
@@ -500,6 +505,7 @@ template <bool BATCH>
 __global__ __launch_bounds__(ZS_BLOCK) void k_predict(Prm q0, const Prm *__restrict__ sp, Buf b,
                                                       const int *__restrict__ cs, const float4 *__restrict__ acc,
                                                       SkinCheck chk) {
+    stage_stamp(q0, ZSPH_ST_GOVERNING, -1);  // the integrator follows the continuity+momentum pass
     const int i = blockIdx.x * ZS_BLOCK + threadIdx.x;
     if (i >= cs[q0.G]) return;
     const Prm q = scene_prm<BATCH>(q0, sp, b, i);
@@ -522,6 +528,7 @@ template <bool BATCH>
 __global__ __launch_bounds__(ZS_BLOCK) void k_correct(Prm q0, const Prm *__restrict__ sp, Buf b,
                                                       const int *__restrict__ cs, const float4 *__restrict__ acc,
                                                       SkinCheck chk) {
+    stage_stamp(q0, ZSPH_ST_GOVERNING, -1);  // the integrator follows the continuity+momentum pass
     const int i = blockIdx.x * ZS_BLOCK + threadIdx.x;
     if (i >= cs[q0.G]) return;
     const Prm q = scene_prm<BATCH>(q0, sp, b, i);
@@ -681,10 +688,9 @@ static void gov_walk(hipStream_t s, int mode, int block, const Prm &q, const Prm
 
 void launch_governing(hipStream_t s, int walk, int block, const Prm &q, const Prm *sp, const Buf &b,
                       const uint32_t *skey, const int *cs, const float4 *gd, float4 *acc, const int *nlist,
-                      const int *ncount, int mode, const float2 *scale, float *terms, unsigned long long *clock,
-                      bool moving, const float4 *xref) {
+                      const int *ncount, int mode, const float2 *scale, float *terms, bool moving,
+                      const float4 *xref) {
     if (!q.N) return;
-    if (clock) launch_clock(s, clock, true);
     if (walk == WALK_SKIN) {  // one scene, plain terms (the caller falls back otherwise)
         gov_walk<WALK_SKIN, false>(s, 0, block, q, nullptr, b, skey, cs, gd, acc, nlist, ncount, scale, terms, xref);
     } else if (walk == WALK_LIST) {
@@ -697,7 +703,6 @@ void launch_governing(hipStream_t s, int walk, int block, const Prm &q, const Pr
         if (sp) gov_walk<WALK_MASK, true>(s, mode, block, q, sp, b, skey, cs, gd, acc, nlist, ncount, scale, terms);
         else gov_walk<WALK_MASK, false>(s, mode, block, q, sp, b, skey, cs, gd, acc, nlist, ncount, scale, terms);
     }
-    if (clock) launch_clock(s, clock, false);
     if (moving) {
         if (sp)
             k_governing_dummy<WALK_MASK, true><<<nblocks(q.N), ZS_BLOCK, 0, s>>>(q, sp, b, skey, cs, acc);

@@ -6,6 +6,7 @@
 #include <atomic>
 #include <condition_variable>
 #include <cstdarg>
+#include <cstddef>
 #include <cstdio>
 #include <algorithm>
 #include <cmath>
@@ -217,10 +218,6 @@ int stage_sort_data(zsph_ctx *c, int from, int to, Prof *pf = nullptr) {
     c->cells_buf = to;
     return check_launch(c, "sort_data");
 }
-// Clock words of stage st when that stage is clocked, else nullptr.
-unsigned long long *stage_clock(zsph_ctx *c, int st) {
-    return (c->clock && (c->clock_mask >> st & 1u)) ? c->clock + 3 * st : nullptr;
-}
 
 // The walk a stage runs: WALK_SKIN only on skin mode's plain path (its rows
 // live around the rebuild positions); elsewhere (staged reference calls, the
@@ -239,15 +236,10 @@ int stage_boundary(zsph_ctx *c, const zs::Prm &q, int b, Prof *pf = nullptr, boo
     const int walk = eff_walk(c, skin_path);
     {
         Scope sc(pf, ZSPH_ST_WALL);
-        unsigned long long *ck = stage_clock(c, ZSPH_ST_WALL);
-        if (ck) zs::launch_clock(c->stream, ck, true);
         zs::launch_wall(c->stream, walk, c->block, q, c->scene_prm, c->buf[b], sk, c->cell_start);
-        if (ck) zs::launch_clock(c->stream, ck, false);
     }
     {
         Scope sc(pf, ZSPH_ST_LDG);
-        unsigned long long *ck = stage_clock(c, ZSPH_ST_LDG);
-        if (ck) zs::launch_clock(c->stream, ck, true);
         if (walk == zs::WALK_SKIN) {
             zs::SkinRec sr;
             sr.xref = c->xref;
@@ -258,7 +250,6 @@ int stage_boundary(zsph_ctx *c, const zs::Prm &q, int b, Prof *pf = nullptr, boo
             zs::launch_ldg(c->stream, walk, c->block, q, c->scene_prm, c->buf[b], sk, c->cell_start, c->gd,
                            walk_record(c), c->ncount, c->closure_on ? c->feat : nullptr);
         }
-        if (ck) zs::launch_clock(c->stream, ck, false);
     }
     c->list_valid = walk_records(walk);
     return check_launch(c, "boundary");
@@ -275,7 +266,7 @@ int stage_governing(zsph_ctx *c, const zs::Prm &q, int b, Prof *pf = nullptr, bo
                                  c->scale, zs::mlp_infer_blocks(q.N));
         zs::launch_governing(c->stream, walk, c->block, q, c->scene_prm, c->buf[b], c->key[c->sorted_slot], c->cell_start,
                              c->gd, c->acc, walk_record(c), c->ncount, c->closure_on ? 1 : 0, c->scale, c->terms,
-                             stage_clock(c, ZSPH_ST_GOVERNING), c->has_body || c->has_piston, c->xref);
+                             c->has_body || c->has_piston, c->xref);
     }
     return check_launch(c, "governing");
 }
@@ -1337,7 +1328,7 @@ int zsph_closure_sample(zsph_ctx *c, int b, float *features, float *terms) {
     zs::launch_ldg(c->stream, walk, c->block, c->q, nullptr, c->buf[b], sk, c->cell_start, c->gd, c->nlist, c->ncount,
                    c->feat);
     zs::launch_governing(c->stream, walk, c->block, c->q, nullptr, c->buf[b], sk, c->cell_start, c->gd, c->acc, c->nlist,
-                         c->ncount, 2, c->scale, c->terms, nullptr, c->has_body || c->has_piston);
+                         c->ncount, 2, c->scale, c->terms, c->has_body || c->has_piston);
     if ((r = check_launch(c, "closure sample"))) return r;
     c->list_valid = false;
     if ((r = copy_sync(c, features, c->feat, N * ZSPH_NF * sizeof(float), hipMemcpyDeviceToHost))) return r;
@@ -1395,8 +1386,13 @@ int zsph_clock_stage(zsph_ctx *c, int stage, int on) {
     if (!c->clock && (r = dalloc(c, &c->clock, 3 * ZSPH_ST_COUNT))) return r;
     const unsigned bit = 1u << stage;
     const unsigned mask = on ? (c->clock_mask | bit) : (c->clock_mask & ~bit);
-    if (mask != c->clock_mask) drop_graph(c);  // the captured step gains / loses the stamps
+    if (mask != c->clock_mask) drop_graph(c);  // the captured step's kernel arguments change
     c->clock_mask = mask;
+    // the kernels stamp through their parameter block (zs::stage_stamp)
+    for (zs::Prm *q : {&c->q, &c->qs}) {
+        q->clk = mask ? c->clock : nullptr;
+        q->clk_mask = mask;
+    }
     return zsph_clock_read_stage(c, stage, nullptr, nullptr);
 }
 
@@ -1611,11 +1607,15 @@ int zsph_body_reduce(zsph_ctx *c, int b, double F[3], double T[3]) {
 
 namespace {
 
-// FNV-1a 64 of the parameter blocks (all 4-byte fields, no padding): a
-// restart must use the parameters the checkpoint was written with.
-static_assert(sizeof(zs::Prm) % 4 == 0, "Prm is a block of 4-byte fields");
+// FNV-1a 64 of the parameter blocks (4-byte fields and the 8-byte clock
+// pointer, no padding), without the stage clocks (measurement state, not
+// parameters): a restart must use the parameters the checkpoint was written with.
+static_assert(sizeof(zs::Prm) == offsetof(zs::Prm, clk) + sizeof(void *), "Prm has no padding");
 uint64_t params_hash(const zsph_ctx *c) {
-    uint64_t h = zsph_fnv1a64(&c->q, sizeof(c->q));
+    zs::Prm q = c->q;
+    q.clk = nullptr;
+    q.clk_mask = 0;
+    uint64_t h = zsph_fnv1a64(&q, sizeof(q));
     for (const zs::Prm &p : c->scene_host) h = zsph_fnv1a64(&p, sizeof(p), h);
     return h;
 }
