@@ -29,11 +29,12 @@ file named), not read in the run.
 The CPU baseline is the FP32 oracle on the host's cores: median of 5 timings
 after 2 warm-up steps with all threads, and a one-thread sample.
 
-One-scene cases run zsph_step in skin mode by default (--skin 0.1, tuned among
-0.05 to 0.3 h): the neighbour structures are rebuilt only after a particle has
-moved more than half the skin since the last rebuild, instead of every half
-step; neighbour sets are unchanged (the r^2 <= (2h)^2 predicate decides every
-pair).  --skin 0 times the reference schedule.
+Every case runs zsph_step in skin mode by default (--skin 0.1, tuned among
+0.05 to 0.3 h; in the ensemble each scene's skin is in its own h): the
+neighbour structures are rebuilt only after a particle has moved more than half
+the skin since the last rebuild, instead of every half step; neighbour sets are
+unchanged (the r^2 <= (2h)^2 predicate decides every pair).  --skin 0 times
+the reference schedule.
 
 --case wavetank3d: the floating-body tank, 4.0 x 1.0 x 1.2 m filled to 0.6 m
 at dx 0.006 m (11.1 M fluid, 12.5 M particles), piston 0.025 m amplitude at
@@ -217,7 +218,7 @@ def main():
     ap.add_argument("--skin", type=float, default=None,
                     help="neighbour skin in units of h (0: rebuild every half step, the reference schedule; > 0: "
                          "rebuild only after a particle moved more than skin/2, the autotuner then also tries "
-                         "skins of 0.05, 0.1, 0.2 and 0.3; default 0.1 for one-scene cases, 0 for the ensemble)")
+                         "skins of 0.05, 0.1, 0.2 and 0.3; default 0.1)")
     ap.add_argument("--closure", default=None, metavar="NPZ",
                     help="run every step with this learned closure network (one-scene cases)")
     ap.add_argument("--tune-at", type=int, default=None,
@@ -300,11 +301,8 @@ def main():
     if setup:
         setup(s)
     if args.skin is None:
-        args.skin = 0.0 if batch else 0.1
+        args.skin = 0.1
     if args.skin > 0:
-        if batch:
-            print("bench.py: --skin needs a one-scene case", file=sys.stderr)
-            sys.exit(2)
         s.set_skin(args.skin)
     # pick the neighbour-walk variant and sort digit width on this case
     if any(v is not None for v in (args.walk, args.radix_bits, args.block, args.sort_tile)):

@@ -239,7 +239,11 @@ int zsph_sync(zsph_ctx *ctx);
 const char *zsph_last_error(zsph_ctx *ctx);
 
 /* lib/init.cu:136-151 cpu_to_gpu: pos, vel, rhop, type, table into buffer
- * `buf` (0 = "old", 1 = "new").  Saved predictor state is cleared. */
+ * `buf` (0 = "old", 1 = "new").  Saved predictor state is cleared.  In a
+ * batch, a particle belongs to the scene of its original index i + table[i]
+ * (table NULL: i), counted in zsph_create_batch's scene order, so a state
+ * downloaded in cell order uploads with its scenes; an original index
+ * outside [0, N) is ZSPH_E_ARG. */
 int zsph_upload(zsph_ctx *ctx, int buf, const zsph_host_state *st);
 
 /* lib/gmemcpy.cu:3-36 gpu_to_cpu, with the ZSPH_OUTPUT_* macros replaced by
@@ -319,7 +323,10 @@ int zsph_array_set(zsph_ctx *ctx, int which, int buf, const void *host, size_t b
 
 /* Per-particle neighbour count and an order-independent 64-bit digest of
  * the neighbour-index set of buffer `buf` (after zsph_sort_data), using the
- * same cell walk and r^2 <= rc2 predicate as the physics kernels. */
+ * same cell walk and r^2 <= rc2 predicate as the physics kernels.  After
+ * skin-mode steps (buffer 0): the structures the next half step's passes
+ * would use, so a rebuild that is due runs first (buffer 0 is then in the
+ * rebuild's order, as after that half step's rebuild). */
 int zsph_neighbor_stats(zsph_ctx *ctx, int buf, int *count, uint64_t *digest);
 
 /* Timing on the context's stream with HIP events (no counterpart in the
@@ -549,7 +556,8 @@ int zsph_filter_fields(zsph_ctx *ctx, int buf, float radius, float hc, const int
                            * calls, the closure) it runs as ZSPH_WALK_HITS.
                            * Its rows lie around the rebuild positions, so
                            * its neighbour order, and FP32 summation order,
-                           * differs from the other walks'                 */
+                           * differs from the other walks'.  One-scene
+                           * contexts only (ZSPH_E_ARG for a batch)        */
 typedef struct zsph_tuning {
     int walk;        /* ZSPH_WALK_*                                          */
     int radix_bits;  /* sort digit width, 8..11                              */
@@ -561,14 +569,17 @@ typedef struct zsph_tuning {
                       * key: 16 or 32 KB)                                    */
     float skin;      /* zsph_step's neighbour skin s in units of h, 0..1 (0:
                       * the reference schedule: hash, sort, cell table and
-                      * reorder every half step).  s > 0 (one-scene contexts):
-                      * zsph_step keeps the cell structures of its last
-                      * rebuild, on a grid of cells (2h + s)/grid_factor with
-                      * rows trimmed to 2h + s/2, and rebuilds them only after
-                      * some particle has moved more than s/2 since (or left
-                      * the grid).  Neighbour sets are the same (r^2 <= (2h)^2
-                      * decides every pair); the particle order, and with it
-                      * the FP32 summation order, follows the rebuilds.      */
+                      * reorder every half step).  s > 0: zsph_step keeps the
+                      * cell structures of its last rebuild, on a grid of
+                      * cells (2h + s)/grid_factor with rows trimmed to
+                      * 2h + s/2, and rebuilds them only after some particle
+                      * has moved more than s/2 since (or left the grid).  In
+                      * a batch, s is in each scene's own h and each scene has
+                      * its own skin grid; any scene's particle triggers the
+                      * rebuild of all.  Neighbour sets are the same
+                      * (r^2 <= (2h)^2 decides every pair); the particle
+                      * order, and with it the FP32 summation order, follows
+                      * the rebuilds.                                        */
 } zsph_tuning;
 int zsph_set_tuning(zsph_ctx *ctx, const zsph_tuning *t);
 int zsph_get_tuning(zsph_ctx *ctx, zsph_tuning *t);

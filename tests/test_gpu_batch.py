@@ -169,6 +169,36 @@ def test_batched_scene_matches_oracle_after_one_step():
     sb.close()
 
 
+def test_batch_reupload_of_a_sorted_state_keeps_scenes():
+    # A state downloaded in cell order and uploaded again: each particle's scene
+    # follows its original index (row + table), so the run continues bit for bit.
+    b = uneven_batch(5, seed=17)
+    s1 = z.Solver(b)
+    s1.upload(b.host)
+    s1.step(5)
+    s1.sync()
+    h = s1.download(0)
+    assert not np.array_equal(h.table, np.zeros_like(h.table))  # in cell order, not upload order
+    s2 = z.Solver(b)
+    s2.upload(h)
+    for s in (s1, s2):
+        s.step(5)
+        s.sync()
+    g1, g2 = s1.download(0), s2.download(0)
+    for f in ("pos", "vel", "rhop", "type", "table"):
+        assert np.array_equal(getattr(g1, f), getattr(g2, f)), f
+    d1, d2 = s1.scene_diagnostics(0), s2.scene_diagnostics(0)
+    assert [d["n_fluid"] for d in d1] == [d["n_fluid"] for d in d2] == [c.json.water_ptc_num for c in b.cases]
+    bad = h.table.copy()
+    bad[0] = b.n  # original index outside [0, N)
+    h.table[:] = bad
+    with pytest.raises(z.ZsphError) as e:
+        s2.upload(h)
+    assert e.value.code == z.abi.E_ARG
+    s1.close()
+    s2.close()
+
+
 def test_batch_rejects_single_scene_calls():
     b = uneven_batch(2, seed=9)
     s = z.Solver(b)

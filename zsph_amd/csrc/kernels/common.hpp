@@ -215,7 +215,7 @@ struct SkinRec {
 struct SkinCheck {
     const float4 *xref = nullptr;
     int *flag = nullptr;
-    float lim2 = 0.0f;  // (s/2)^2, shrunk by 0.1 %
+    float lim_h = 0.0f;  // s/2 in units of the particle's scene h, shrunk by 0.1 %
 };
 
 // EOS p(rho).  gamma == 1 is the reference's linear form c^2 (rho - rho0)
@@ -270,7 +270,8 @@ __host__ __device__ inline float dist2(float dx, float dy, float dz) {
 // True when particle i at x needs a rebuild (see SkinCheck); NaN positions do.
 __device__ __forceinline__ bool skin_moved(const Prm &q, const SkinCheck &k, int i, float4 x) {
     const float4 r = k.xref[i];
-    if (!(dist2(x.x - r.x, x.y - r.y, x.z - r.z) <= k.lim2)) return true;
+    const float lim = k.lim_h * q.h;  // the scene's own s/2 (a batch's scenes differ in h)
+    if (!(dist2(x.x - r.x, x.y - r.y, x.z - r.z) <= lim * lim)) return true;
     int c;
     return !(cell_axis(x.x, q.gx0, q.kill_size, q.kill_X, &c) && cell_axis(x.y, q.gy0, q.kill_size, q.kill_Y, &c) &&
              cell_axis(x.z, q.gz0, q.kill_size, q.kill_Z, &c));

@@ -417,6 +417,7 @@ __global__ __launch_bounds__(ZS_BLOCK) void k_skin_commit(int n, Buf a, Buf b, f
     b.tvelp[k] = a.tvelp[k];
     b.type[k] = a.type[k];
     b.table[k] = a.table[k];
+    if (a.scene) b.scene[k] = a.scene[k];
 }
 
 // Skin mode, after the rebuild kernels of a half step: count a rebuild that

@@ -73,6 +73,8 @@ struct zsph_ctx {
     int nscenes = 1;                  // > 1: a batch of independent scenes
     zs::Prm *scene_prm = nullptr;     // per-scene parameters (device, batches only)
     std::vector<zs::Prm> scene_host;  // the same on the host
+    zs::Prm *scene_prm_skin = nullptr;     // skin mode: per-scene skin grids (device, batches only)
+    std::vector<zs::Prm> scene_skin_host;  // the same on the host
     std::vector<int> scene_off;       // first particle of each scene in upload order (+ N)
     int cells_buf = -1;               // buffer the cell table describes
     double *scene_part = nullptr;     // per-scene diagnostics (batches only)
@@ -219,6 +221,12 @@ int stage_sort_data(zsph_ctx *c, int from, int to, Prof *pf = nullptr) {
     return check_launch(c, "sort_data");
 }
 
+// The per-scene parameter table of a batch's kernels (nullptr for one
+// scene): in skin mode's half step (skin_path) each scene's skin grid.
+const zs::Prm *scene_table(const zsph_ctx *c, bool skin_path) {
+    return (skin_path && c->scene_prm_skin && c->skin > 0.0f) ? c->scene_prm_skin : c->scene_prm;
+}
+
 // The walk a stage runs: WALK_SKIN only on skin mode's plain path (its rows
 // live around the rebuild positions); elsewhere (staged reference calls, the
 // closure's passes) the hit-mask walk stands in for it.
@@ -236,7 +244,7 @@ int stage_boundary(zsph_ctx *c, const zs::Prm &q, int b, Prof *pf = nullptr, boo
     const int walk = eff_walk(c, skin_path);
     {
         Scope sc(pf, ZSPH_ST_WALL);
-        zs::launch_wall(c->stream, walk, c->block, q, c->scene_prm, c->buf[b], sk, c->cell_start);
+        zs::launch_wall(c->stream, walk, c->block, q, scene_table(c, skin_path), c->buf[b], sk, c->cell_start);
     }
     {
         Scope sc(pf, ZSPH_ST_LDG);
@@ -247,7 +255,7 @@ int stage_boundary(zsph_ctx *c, const zs::Prm &q, int b, Prof *pf = nullptr, boo
             sr.rebuilt = c->rebuild + 1;
             zs::launch_ldg_skin(c->stream, c->block, q, c->buf[b], sk, c->cell_start, c->gd, c->hits, c->ncount, sr);
         } else {
-            zs::launch_ldg(c->stream, walk, c->block, q, c->scene_prm, c->buf[b], sk, c->cell_start, c->gd,
+            zs::launch_ldg(c->stream, walk, c->block, q, scene_table(c, skin_path), c->buf[b], sk, c->cell_start, c->gd,
                            walk_record(c), c->ncount, c->closure_on ? c->feat : nullptr);
         }
     }
@@ -264,7 +272,7 @@ int stage_governing(zsph_ctx *c, const zs::Prm &q, int b, Prof *pf = nullptr, bo
         if (c->closure_on)  // scale factors of the fluid particles [0, cs[G])
             zs::launch_mlp_infer(c->stream, c->mlp_P, c->mlp_pack, c->mlp_aux, c->feat, 0, c->cell_start + q.G,
                                  c->scale, zs::mlp_infer_blocks(q.N));
-        zs::launch_governing(c->stream, walk, c->block, q, c->scene_prm, c->buf[b], c->key[c->sorted_slot], c->cell_start,
+        zs::launch_governing(c->stream, walk, c->block, q, scene_table(c, skin_path), c->buf[b], c->key[c->sorted_slot], c->cell_start,
                              c->gd, c->acc, walk_record(c), c->ncount, c->closure_on ? 1 : 0, c->scale, c->terms,
                              c->has_body || c->has_piston, c->xref);
     }
@@ -281,10 +289,11 @@ int stage_integrate(zsph_ctx *c, const zs::Prm &q, int b, bool predictor, Prof *
     if (c->has_body)
         zs::launch_body_reduce(c->stream, q, c->buf[b], c->cell_start, c->acc, c->body, c->body_part);
     if (moving) zs::launch_body_update(c->stream, q, c->body, c->body_part, predictor);
+    const zs::Prm *sp = scene_table(c, chk.xref != nullptr);  // skin mode's half step checks displacements
     if (predictor)
-        zs::launch_predict(c->stream, q, c->scene_prm, c->buf[b], c->cell_start, c->acc, chk);
+        zs::launch_predict(c->stream, q, sp, c->buf[b], c->cell_start, c->acc, chk);
     else
-        zs::launch_correct(c->stream, q, c->scene_prm, c->buf[b], c->cell_start, c->acc, chk);
+        zs::launch_correct(c->stream, q, sp, c->buf[b], c->cell_start, c->acc, chk);
     if (moving) zs::launch_kinematics(c->stream, q, c->buf[b], c->cell_start, c->body, chk);
     return check_launch(c, predictor ? "prediction" : "correction");
 }
@@ -311,12 +320,15 @@ int half_step(zsph_ctx *c, int from, int to, bool predictor, Prof *pf = nullptr)
 // reference positions) run only when the rebuild flag is set; the passes use
 // the skin grid; the integrator sets the flag again when a particle has
 // moved more than s/2 since the rebuild or left the grid.
-int skin_half_step(zsph_ctx *c, bool predictor, Prof *pf = nullptr) {
+// Skin mode's rebuild kernels (hash on the skin grids, sort, cell table,
+// reorder into buffer 1 and back with the reference positions), gated on the
+// device's rebuild flag, which they then count and clear.
+int skin_rebuild(zsph_ctx *c, Prof *pf = nullptr) {
     const zs::Prm &q = c->qs;
     const int *gate = c->rebuild;
     {
         Scope sc(pf, ZSPH_ST_HASH);
-        zs::launch_hash(c->stream, q, nullptr, c->buf[0], c->key[0], c->idx[0], c->killed, gate);
+        zs::launch_hash(c->stream, q, scene_table(c, true), c->buf[0], c->key[0], c->idx[0], c->killed, gate);
     }
     int slot;
     {
@@ -339,14 +351,19 @@ int skin_half_step(zsph_ctx *c, bool predictor, Prof *pf = nullptr) {
     c->sorted_slot = slot;
     c->skin_slot = slot;
     c->list_valid = false;
+    return ZSPH_OK;
+}
+
+int skin_half_step(zsph_ctx *c, bool predictor, Prof *pf = nullptr) {
+    const zs::Prm &q = c->qs;
     int r;
+    if ((r = skin_rebuild(c, pf))) return r;
     if ((r = stage_boundary(c, q, 0, pf, true))) return r;
     if ((r = stage_governing(c, q, 0, pf, true))) return r;
     zs::SkinCheck chk;
     chk.xref = c->xref;
     chk.flag = c->rebuild;
-    const float half = 0.5f * c->skin * c->q.h * 0.999f;
-    chk.lim2 = half * half;
+    chk.lim_h = 0.5f * c->skin * 0.999f;
     return stage_integrate(c, q, 0, predictor, pf, chk);
 }
 
@@ -553,8 +570,23 @@ const char *zsph_last_error(zsph_ctx *c) { return c ? c->err.c_str() : "null con
 
 int zsph_upload(zsph_ctx *c, int b, const zsph_host_state *st) {
     if (!c || !st || !valid_buf(b) || !st->pos || !st->type) return set_err(c, ZSPH_E_ARG, "zsph_upload: bad argument");
-    c->skin_valid = false;  // buffer 0 changes (or, for a checkpoint, the next step rebuilds)
     const int N = c->q.N;
+    // A batch's particle belongs to the scene of its original index i + table[i]
+    // (the upload order of zsph_create_batch's scenes): a downloaded state in
+    // cell order keeps its scenes.  Every original index must lie in [0, N).
+    std::vector<int> sid;
+    if (c->buf[b].scene) {
+        sid.resize(N);
+        for (int i = 0; i < N; ++i) {
+            const long long o = (long long)i + (st->table ? st->table[i] : 0);
+            if (o < 0 || o >= N)
+                return set_err(c, ZSPH_E_ARG, "zsph_upload: particle %d has original index %lld outside [0, %d)", i,
+                               o, N);
+            sid[i] = (int)(std::upper_bound(c->scene_off.begin(), c->scene_off.end(), (int)o) -
+                           c->scene_off.begin()) - 1;
+        }
+    }
+    c->skin_valid = false;  // buffer 0 changes (or, for a checkpoint, the next step rebuilds)
     std::vector<float4> posr(N), velp(N);
     for (int i = 0; i < N; ++i) {
         posr[i] = make_float4(st->pos[3 * i], st->pos[3 * i + 1], st->pos[3 * i + 2],
@@ -576,13 +608,7 @@ int zsph_upload(zsph_ctx *c, int b, const zsph_host_state *st) {
     }
     if (st->acc_drhodt)
         HIPCHK(c, hipMemcpyAsync(c->acc, st->acc_drhodt, N * sizeof(float4), hipMemcpyHostToDevice, c->stream));
-    std::vector<int> sid;
-    if (d.scene) {  // a batch: particles arrive scene by scene
-        sid.resize(N);
-        for (int k = 0; k < c->nscenes; ++k)
-            std::fill(sid.begin() + c->scene_off[k], sid.begin() + c->scene_off[k + 1], k);
-        HIPCHK(c, hipMemcpyAsync(d.scene, sid.data(), N * sizeof(int), hipMemcpyHostToDevice, c->stream));
-    }
+    if (d.scene) HIPCHK(c, hipMemcpyAsync(d.scene, sid.data(), N * sizeof(int), hipMemcpyHostToDevice, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
     c->hashed = 0;
     c->sorted_slot = -1;
@@ -877,6 +903,39 @@ int zsph_rebuilds(zsph_ctx *c, long long *n) {
     return ZSPH_OK;
 }
 
+namespace {
+// Skin-mode neighbour grids for a skin of `skin` h: the context's (qs) and,
+// for a batch, each scene's own (make_skin_prm on the scene's grid), laid out
+// scene after scene in the key space like the case grids (cell_base), with
+// the batch's particle count and sort-key flag (a skin grid has no more cells
+// than its case grid, so the keys still fit below the flag).
+int set_skin_grids(zsph_ctx *c, float skin) {
+    if (c->nscenes == 1) {
+        c->qs = zs::make_skin_prm(c->q, skin);
+        return ZSPH_OK;
+    }
+    std::vector<zs::Prm> v;
+    long long base = 0;
+    for (const zs::Prm &p : c->scene_host) {
+        zs::Prm sk = zs::make_skin_prm(p, skin);
+        sk.cell_base = (int)base;
+        base += sk.G;  // the scene's own skin cells
+        v.push_back(sk);
+    }
+    for (zs::Prm &sk : v) sk.G = (int)base;
+    int r;
+    if (!c->scene_prm_skin && (r = dalloc(c, &c->scene_prm_skin, v.size()))) return r;
+    HIPCHK(c, hipMemcpy(c->scene_prm_skin, v.data(), v.size() * sizeof(zs::Prm), hipMemcpyHostToDevice));
+    c->scene_skin_host = v;
+    zs::Prm qs = v[0];  // context-wide view, as c->q: scene 0's constants with the batch's sizes
+    qs.cell_base = 0;
+    qs.clk = c->q.clk;
+    qs.clk_mask = c->q.clk_mask;
+    c->qs = qs;
+    return ZSPH_OK;
+}
+}  // namespace
+
 int zsph_set_tuning(zsph_ctx *c, const zsph_tuning *t) {
     if (!c || !t) return ZSPH_E_ARG;
     const int block = t->block ? t->block : ZS_BLOCK;
@@ -887,8 +946,8 @@ int zsph_set_tuning(zsph_ctx *c, const zsph_tuning *t) {
         return set_err(c, ZSPH_E_ARG,
                        "zsph_set_tuning: walk %d / radix_bits %d / block %d / sort_tile %d / skin %g out of range",
                        t->walk, t->radix_bits, t->block, t->sort_tile, (double)t->skin);
-    if (t->skin > 0.0f && c->nscenes > 1)
-        return set_err(c, ZSPH_E_ARG, "zsph_set_tuning: the skin is for one-scene contexts");
+    if (t->walk == ZSPH_WALK_SKIN && c->nscenes > 1)
+        return set_err(c, ZSPH_E_ARG, "zsph_set_tuning: the recorded skin walk is for one-scene contexts");
     if (t->walk == c->walk && t->radix_bits == c->radix_bits && block == c->block && tile == c->sort_tile &&
         t->skin == c->skin)
         return ZSPH_OK;
@@ -898,8 +957,8 @@ int zsph_set_tuning(zsph_ctx *c, const zsph_tuning *t) {
          (r = dalloc(c, &c->xref, (size_t)c->q.N))))
         return r;
     if (t->skin != c->skin) {
+        if (t->skin > 0.0f && (r = set_skin_grids(c, t->skin))) return r;
         c->skin = t->skin;
-        if (c->skin > 0.0f) c->qs = zs::make_skin_prm(c->q, c->skin);
         c->skin_valid = false;
     }
     if (walk_records(t->walk) && !c->ncount && (r = dalloc(c, &c->ncount, (size_t)c->q.N))) return r;
@@ -1083,6 +1142,14 @@ int zsph_neighbor_stats(zsph_ctx *c, int b, int *count, uint64_t *digest) {
     const bool skin = c->skin > 0.0f && c->skin_valid && !c->skin_pending && c->sorted_slot < 0 && c->hashed == 0;
     if (skin && b != 0) return set_err(c, ZSPH_E_STATE, "zsph_neighbor_stats: skin-mode state is in buffer 0");
     if (!skin && c->sorted_slot < 0) return set_err(c, ZSPH_E_STATE, "zsph_neighbor_stats called before zsph_sort_data");
+    if (skin) {
+        // the structures the next half step's passes would use: its rebuild, if one is due
+        // (some particle moved more than s/2 in the last half step), runs first
+        int r = skin_rebuild(c);
+        if (!r && c->walk == ZSPH_WALK_SKIN) r = skin_force_rebuild(c);  // its skin record is of the old order
+        c->sorted_slot = -1;
+        if (r) return r;
+    }
     const size_t N = (size_t)c->q.N;
     int *dc = nullptr;
     uint64_t *dd = nullptr;
@@ -1092,7 +1159,7 @@ int zsph_neighbor_stats(zsph_ctx *c, int b, int *count, uint64_t *digest) {
         (void)hipFree(dc);
         return set_err(c, ZSPH_E_NOMEM, "hipMalloc failed");
     }
-    zs::launch_nbr_stats(c->stream, c->walk, skin ? c->qs : c->q, c->scene_prm, c->buf[b],
+    zs::launch_nbr_stats(c->stream, c->walk, skin ? c->qs : c->q, scene_table(c, skin), c->buf[b],
                          c->key[skin ? c->skin_slot : c->sorted_slot], c->cell_start, dc, dd);
     e = hipGetLastError();
     if (e == hipSuccess && N) e = hipMemcpyAsync(count, dc, N * 4, hipMemcpyDeviceToHost, c->stream);
@@ -1206,10 +1273,14 @@ int zsph_diagnostics(zsph_ctx *c, int b, float front_zmax, zsph_diag *out) {
 
 int zsph_scene_diagnostics(zsph_ctx *c, int b, float front_zmax, zsph_diag *out) {
     if (!c || !out || !valid_buf(b)) return set_err(c, ZSPH_E_ARG, "zsph_scene_diagnostics: bad argument");
-    if (c->cells_buf != b)
+    // after skin-mode steps the cell table is that of the last rebuild, on the
+    // skin grids: it still gives each scene's particle ranges in buffer 0
+    const bool skin = c->skin > 0.0f && c->skin_valid && !c->skin_pending && c->cells_buf < 0 && c->hashed == 0;
+    if (c->cells_buf != b && !(skin && b == 0))
         return set_err(c, ZSPH_E_STATE, "zsph_scene_diagnostics: buffer %d is not in cell order (step or sort first)", b);
     const int S = c->nscenes;
-    zs::launch_scene_diag(c->stream, c->q, c->scene_prm, S, c->buf[b], c->cell_start, front_zmax, c->scene_part);
+    zs::launch_scene_diag(c->stream, skin ? c->qs : c->q, scene_table(c, skin), S, c->buf[b], c->cell_start,
+                          front_zmax, c->scene_part);
     int r = check_launch(c, "scene diagnostics");
     if (r) return r;
     std::vector<double> part((size_t)S * zs::SCENE_D);
