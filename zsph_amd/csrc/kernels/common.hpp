@@ -152,6 +152,33 @@ __device__ __forceinline__ Prm scene_prm(const Prm &q0, const Prm *__restrict__ 
     else return q0;
 }
 
+// Runs f(q) with the parameters of particle i's scene as a wave-uniform
+// value: q0 itself for one scene; in a batch, once per distinct scene among
+// the wave's active lanes (scenes are contiguous in cell order, so nearly
+// every wave has one) with only that scene's lanes active, the scene's
+// parameters read once for the wave instead of per lane (they stay out of
+// the vector registers the neighbour walks need).
+template <bool BATCH, class F>
+__device__ __forceinline__ void with_scene(const Prm &q0, const Prm *__restrict__ sp, const Buf &b, int i, F &&f) {
+    if constexpr (!BATCH) {
+        f(q0);
+    } else {
+        const int sc = b.scene[i];
+        uint64_t todo = __ballot(1);
+        while (todo) {
+            const int s = __builtin_amdgcn_readlane(sc, (int)__builtin_ctzll(todo));
+            const bool mine = sc == s;
+            todo &= ~__ballot(mine);
+            if (mine) {
+                // the scene index restated as wave-uniform where it is used, so the
+                // parameter block is read with scalar loads into scalar registers
+                const Prm q = sp[__builtin_amdgcn_readfirstlane(s)];
+                f(q);
+            }
+        }
+    }
+}
+
 #define ZS_BLOCK 256
 
 // Blocks are dispatched round-robin over the 8 XCDs; remap blocks
@@ -633,26 +660,33 @@ struct NoStage {
 };
 // A wave's LDS slice: cap + STAGE_PAD position records (lds) and, when lds2
 // is set, the same records of the array src2 (the continuity+momentum pass
-// stages velocities and pressures with the positions).
+// stages velocities and pressures with the positions).  tag: the lane's
+// scene in a batch (0 for one scene); a wave stages only when its lanes share
+// one scene (and so one grid), as they nearly always do: scenes are
+// contiguous in cell order.
 struct LdsStage {
     float4 *lds;
     float4 *lds2 = nullptr;
     const float4 *src2 = nullptr;
     int cap = STAGE_SLOTS;
+    int tag = 0;
 };
 // The wave-uniform cell row of the walk (ok: every active lane lies in it).
 struct WaveRow {
     bool ok;
     int cz, cy, x0, x1;  // the row's z and y cell; the first and last lanes' x cells
 };
-__device__ __forceinline__ WaveRow wave_row(const Prm &q, uint32_t cell) {
+__device__ __forceinline__ WaveRow wave_row(const Prm &q, uint32_t cell, int tag) {
     const uint64_t ex = __builtin_amdgcn_read_exec();
     const int last = 63 - __builtin_clzll(ex);
     const uint32_t c0 = (uint32_t)__builtin_amdgcn_readfirstlane((int)cell);
     const uint32_t c1 = (uint32_t)__builtin_amdgcn_readlane((int)cell, last);
+    // scenes are non-decreasing along the sorted particles: equal tags at the
+    // first and last active lanes put the whole wave in one scene
+    const bool one_scene = __builtin_amdgcn_readfirstlane(tag) == __builtin_amdgcn_readlane(tag, last);
     const int r0 = (int)(c0 / (uint32_t)q.X), r1 = (int)(c1 / (uint32_t)q.X);
     WaveRow w;
-    w.ok = r0 == r1;
+    w.ok = one_scene && r0 == r1;
     w.x0 = (int)c0 - r0 * q.X;
     w.x1 = (int)c1 - r1 * q.X;
     w.cz = r0 / q.Y;
@@ -769,7 +803,7 @@ __device__ __forceinline__ void for_each_neighbor(const Prm &q, int i, float4 xi
                                                   Rec &&rec = Rec{}, Gath &&gath = Gath{}, Stg stg = Stg{}) {
     constexpr bool STAGED = std::is_same_v<Stg, LdsStage>;
     WaveRow wr{};
-    if constexpr (STAGED) wr = wave_row(q, cell);
+    if constexpr (STAGED) wr = wave_row(q, cell, stg.tag);
     auto load = [&](int j, auto kind) { return load_hit(posr, gath, j, kind); };
     auto use = [&](const auto &h, auto kind, int, int) { run_hit(f, xi, h, kind); };
     auto visit = [&](int j, auto kind) { use(load(j, kind), kind, 0, 0); };
@@ -881,7 +915,7 @@ __device__ __forceinline__ void for_each_recorded_at(const Prm &q, int i, float4
                                                      Gath &&gath, Stg stg, GV &&gath_v) {
     constexpr bool STAGED = std::is_same_v<Stg, LdsStage>;
     WaveRow wr{};
-    if constexpr (STAGED) wr = wave_row(q, cell);
+    if constexpr (STAGED) wr = wave_row(q, cell, stg.tag);
     auto load = [&](int j, auto kind) { return load_hit(posr, gath, j, kind); };
     auto use = [&](const auto &h, auto kind, int, int) { run_hit(f, xi, h, kind); };
     auto visit = [&](int j, auto kind) { use(load(j, kind), kind, 0, 0); };

@@ -21,7 +21,9 @@ namespace zs {
 static inline int nblocks(int n) { return (n + ZS_BLOCK - 1) / ZS_BLOCK; }
 
 // Waves per SIMD the single-scene plain gradient and continuity+momentum
-// passes are compiled for (a register budget of 512 / waves VGPRs), and the
+// passes are compiled for (a register budget of 512 / waves VGPRs; the batch
+// forms keep the compiler's own budget, about 90 VGPRs and 5 waves: capped
+// like the single-scene forms they spill and measured slower), and the
 // records per wave the continuity+momentum pass stages in LDS (positions and
 // velocities of a row pair; larger pairs are gathered from L2): 6 waves and
 // 192 records fill the CU's 160 KB of LDS without spilling registers
@@ -118,183 +120,182 @@ __global__ __launch_bounds__(BS, ((WALK != WALK_LIST && !BATCH && !FEAT) ? ZS_LD
         gd[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
         return;
     }
-    const Prm q = scene_prm<BATCH>(q0, sp, b, i);
-    const bool delta = is_antuono<DF>(q);
-    const bool ell = delta || FEAT;  // L is needed
-    const float4 xi = b.posr[i];
-    const float4 vi = b.velp[i];
-    float lxx = 0, lxy = 0, lxz = 0, lyy = 0, lyz = 0, lzz = 0;
-    float gx = 0, gy = 0, gz = 0, D = 0;
-    // FEAT only: velocity-gradient tensor K_ab = sum F V (v_j - v_i)_a x_b (fluid
-    // neighbours), kernel sum, neighbour counts, velocity spread, density contrast
-    float kxx = 0, kxy = 0, kxz = 0, kyx = 0, kyy = 0, kyz = 0, kzx = 0, kzy = 0, kzz = 0;
-    float ws = 0, wv2 = 0, wdr = 0;
-    int nnb = 0, ndum = 0;
-    // vj: velp[j] (read only for fluid j)
-    auto body = [&](int j, float dx, float dy, float dz, float r2, float4 xj, float4 vj, auto kind) {
-        const float qq = hw_sqrt(r2) * q.inv_h;
-        const float t = 1.0f - 0.5f * qq;
-        const float Vj = q.m * hw_rcp(xj.w);
-        const float FV = q.fk * t * t * t * Vj;
-        if (ell) {
-            lxx -= FV * dx * dx; lxy -= FV * dx * dy; lxz -= FV * dx * dz;
-            lyy -= FV * dy * dy; lyz -= FV * dy * dz; lzz -= FV * dz * dz;
-        }
-        if (delta) {
-            const float c = FV * (xj.w - xi.w);
-            gx += c * dx; gy += c * dy; gz += c * dz;
-        }
-        float WV = 0.0f;
-        if (FEAT) {
-            WV = q.adh * t * t * t * t * (2.0f * qq + 1.0f) * Vj;
-            ws += WV;
-            wdr += WV * (xj.w - xi.w);
-            ++nnb;
-        }
-        if (fluid_j(kind, j, nf)) {
-            const float ux = vj.x - vi.x, uy = vj.y - vi.y, uz = vj.z - vi.z;
-            D -= FV * ((vi.x - vj.x) * dx + (vi.y - vj.y) * dy + (vi.z - vj.z) * dz);
-            if (FEAT) {
-                kxx += FV * ux * dx; kxy += FV * ux * dy; kxz += FV * ux * dz;
-                kyx += FV * uy * dx; kyy += FV * uy * dy; kyz += FV * uy * dz;
-                kzx += FV * uz * dx; kzy += FV * uz * dy; kzz += FV * uz * dz;
-                wv2 += WV * (ux * ux + uy * uy + uz * uz);
+    // the wave's LDS slice for staging its candidate rows (a batch's wave
+    // stages only when its lanes share one scene's grid)
+    __shared__ float4 stage_lds[(BS / 64) * (STAGE_SLOTS + STAGE_PAD)];
+    with_scene<BATCH>(q0, sp, b, i, [&](const Prm &q) {
+        const bool delta = is_antuono<DF>(q);
+        const bool ell = delta || FEAT;  // L is needed
+        const float4 xi = b.posr[i];
+        const float4 vi = b.velp[i];
+        float lxx = 0, lxy = 0, lxz = 0, lyy = 0, lyz = 0, lzz = 0;
+        float gx = 0, gy = 0, gz = 0, D = 0;
+        // FEAT only: velocity-gradient tensor K_ab = sum F V (v_j - v_i)_a x_b (fluid
+        // neighbours), kernel sum, neighbour counts, velocity spread, density contrast
+        float kxx = 0, kxy = 0, kxz = 0, kyx = 0, kyy = 0, kyz = 0, kzx = 0, kzy = 0, kzz = 0;
+        float ws = 0, wv2 = 0, wdr = 0;
+        int nnb = 0, ndum = 0;
+        // vj: velp[j] (read only for fluid j)
+        auto body = [&](int j, float dx, float dy, float dz, float r2, float4 xj, float4 vj, auto kind) {
+            const float qq = hw_sqrt(r2) * q.inv_h;
+            const float t = 1.0f - 0.5f * qq;
+            const float Vj = q.m * hw_rcp(xj.w);
+            const float FV = q.fk * t * t * t * Vj;
+            if (ell) {
+                lxx -= FV * dx * dx; lxy -= FV * dx * dy; lxz -= FV * dx * dz;
+                lyy -= FV * dy * dy; lyz -= FV * dy * dz; lzz -= FV * dz * dz;
             }
-        } else if (FEAT) {
-            ++ndum;
-        }
-    };
-    // gathered per neighbour besides its position: velp[j] for fluid j
-    auto gath = [&](int j, auto kind) {
-        if constexpr (decltype(kind)::value == 1) {
-            return Empty{};
-        } else if constexpr (decltype(kind)::value == 0) {
-            return ld(b.velp, j);
-        } else {
-            float4 v;  // loaded, and read by the body, for fluid neighbours only
-            if (j < nf) v = ld(b.velp, j);
-            return v;
-        }
-    };
-    auto walk_body = [&](int j, float dx, float dy, float dz, float r2, float4 xj, auto kind, const auto &e) {
-        float4 vj;
-        if constexpr (!std::is_same_v<std::decay_t<decltype(e)>, Empty>) vj = e;
-        body(j, dx, dy, dz, r2, xj, vj, kind);
-    };
-    const uint32_t cell = cell_of(q, skey[i]);
-    // the wave's LDS slice for staging its candidate rows (one scene: a batch's
-    // waves can span scenes with different grids)
-    __shared__ float4 stage_lds[BATCH ? 1 : (BS / 64) * (STAGE_SLOTS + STAGE_PAD)];
-    const auto stg = [&]() {
-        if constexpr (BATCH) return NoStage{};
-        else return LdsStage{stage_lds + (threadIdx.x >> 6) * (STAGE_SLOTS + STAGE_PAD)};
-    }();
-    if (WALK == WALK_LIST) {  // the mask walk, recording what it visits
-        ListWriter lw(nlist, i);
-        for_each_neighbor<WALK_MASK, true, true>(q, i, xi, cell, cs, b.posr,
-                                                 [&](int j, float dx, float dy, float dz, float r2, float4 xj,
-                                                     auto kind, const auto &e) {
-                                                     lw.put(j);
-                                                     walk_body(j, dx, dy, dz, r2, xj, kind, e);
-                                                 },
-                                                 NoRec{}, gath);
-        ncount[i] = lw.finish();
-    } else if constexpr (WALK == WALK_SKIN_REC || WALK == WALK_SKIN_REPLAY) {
-        uint32_t *hits = reinterpret_cast<uint32_t *>(nlist);  // this half step's exact masks
-        const size_t n = (size_t)q0.N;
-        Prm qr = q;  // rows around the rebuild position, trimmed to rc + s
-        qr.trim2 = q.trim2_skin;
-        const float4 xc = sr.xref[i];
-        auto exact = [&](int j, float dx, float dy, float dz, float r2, float4 xj, auto kind,
-                         const auto &e) -> bool {
-            if (!(r2 <= q.rc2)) return false;
-            walk_body(j, dx, dy, dz, r2, xj, kind, e);
-            return true;
+            if (delta) {
+                const float c = FV * (xj.w - xi.w);
+                gx += c * dx; gy += c * dy; gz += c * dz;
+            }
+            float WV = 0.0f;
+            if (FEAT) {
+                WV = q.adh * t * t * t * t * (2.0f * qq + 1.0f) * Vj;
+                ws += WV;
+                wdr += WV * (xj.w - xi.w);
+                ++nnb;
+            }
+            if (fluid_j(kind, j, nf)) {
+                const float ux = vj.x - vi.x, uy = vj.y - vi.y, uz = vj.z - vi.z;
+                D -= FV * ((vi.x - vj.x) * dx + (vi.y - vj.y) * dy + (vi.z - vj.z) * dz);
+                if (FEAT) {
+                    kxx += FV * ux * dx; kxy += FV * ux * dy; kxz += FV * ux * dz;
+                    kyx += FV * uy * dx; kyy += FV * uy * dy; kyz += FV * uy * dz;
+                    kzx += FV * uz * dx; kzy += FV * uz * dy; kzz += FV * uz * dz;
+                    wv2 += WV * (ux * ux + uy * uy + uz * uz);
+                }
+            } else if (FEAT) {
+                ++ndum;
+            }
         };
-        auto rec_exact = [&](int r, int c, uint32_t m) {
-            if (c < 2) hits[(size_t)(2 * r + c) * n + (size_t)i] = m;
+        // gathered per neighbour besides its position: velp[j] for fluid j
+        auto gath = [&](int j, auto kind) {
+            if constexpr (decltype(kind)::value == 1) {
+                return Empty{};
+            } else if constexpr (decltype(kind)::value == 0) {
+                return ld(b.velp, j);
+            } else {
+                float4 v;  // loaded, and read by the body, for fluid neighbours only
+                if (j < nf) v = ld(b.velp, j);
+                return v;
+            }
         };
-        if constexpr (WALK == WALK_SKIN_REC) {
-            bool over = false;  // a row of more than 64 candidates: this particle is not recorded
-            skin_walk<false>(qr, i, xi, xc, cell, cs, b.posr, sr.skin, n, exact,
-                             [&](int r, int c, uint32_t m) {
-                                 if (c < 2) sr.skin[(size_t)(2 * r + c) * n + (size_t)i] = m;
-                                 else over = true;
-                             },
-                             rec_exact, gath);
+        auto walk_body = [&](int j, float dx, float dy, float dz, float r2, float4 xj, auto kind, const auto &e) {
+            float4 vj;
+            if constexpr (!std::is_same_v<std::decay_t<decltype(e)>, Empty>) vj = e;
+            body(j, dx, dy, dz, r2, xj, vj, kind);
+        };
+        const uint32_t cell = cell_of(q, skey[i]);
+        LdsStage stg{stage_lds + (threadIdx.x >> 6) * (STAGE_SLOTS + STAGE_PAD)};
+        if constexpr (BATCH) stg.tag = b.scene[i];
+        if (WALK == WALK_LIST) {  // the mask walk, recording what it visits
+            ListWriter lw(nlist, i);
+            for_each_neighbor<WALK_MASK, true, true>(q, i, xi, cell, cs, b.posr,
+                                                     [&](int j, float dx, float dy, float dz, float r2, float4 xj,
+                                                         auto kind, const auto &e) {
+                                                         lw.put(j);
+                                                         walk_body(j, dx, dy, dz, r2, xj, kind, e);
+                                                     },
+                                                     NoRec{}, gath);
+            ncount[i] = lw.finish();
+        } else if constexpr (WALK == WALK_SKIN_REC || WALK == WALK_SKIN_REPLAY) {
+            uint32_t *hits = reinterpret_cast<uint32_t *>(nlist);  // this half step's exact masks
+            const size_t n = (size_t)q0.N;
+            Prm qr = q;  // rows around the rebuild position, trimmed to rc + s
+            qr.trim2 = q.trim2_skin;
+            const float4 xc = sr.xref[i];
+            auto exact = [&](int j, float dx, float dy, float dz, float r2, float4 xj, auto kind,
+                             const auto &e) -> bool {
+                if (!(r2 <= q.rc2)) return false;
+                walk_body(j, dx, dy, dz, r2, xj, kind, e);
+                return true;
+            };
+            auto rec_exact = [&](int r, int c, uint32_t m) {
+                if (c < 2) hits[(size_t)(2 * r + c) * n + (size_t)i] = m;
+            };
+            if constexpr (WALK == WALK_SKIN_REC) {
+                bool over = false;  // a row of more than 64 candidates: this particle is not recorded
+                skin_walk<false>(qr, i, xi, xc, cell, cs, b.posr, sr.skin, n, exact,
+                                 [&](int r, int c, uint32_t m) {
+                                     if (c < 2) sr.skin[(size_t)(2 * r + c) * n + (size_t)i] = m;
+                                     else over = true;
+                                 },
+                                 rec_exact, gath);
+                ncount[i] = over ? 1 : 0;
+            } else if (ncount[i]) {  // not recorded at the rebuild: the mask walk (and so its successor)
+                for_each_neighbor<WALK_MASK, true, true>(q, i, xi, cell, cs, b.posr, walk_body, NoRec{}, gath);
+            } else {
+                skin_walk<true>(qr, i, xi, xc, cell, cs, b.posr, sr.skin, n, exact, [](int, int, uint32_t) {},
+                                rec_exact, gath);
+            }
+        } else if (WALK == WALK_HITS) {  // the mask walk, recording each chunk's hit mask
+            uint32_t *hits = reinterpret_cast<uint32_t *>(nlist);
+            bool over = false;  // a row with more than 64 candidates: not recorded
+            for_each_neighbor<WALK_MASK, true, true>(q, i, xi, cell, cs, b.posr, walk_body,
+                                                     [&](int r, int c, uint32_t m) {
+                                                         if (c < 2) hits[(size_t)(2 * r + c) * (size_t)q0.N + (size_t)i] = m;
+                                                         else over = true;
+                                                     },
+                                                     gath, stg);
             ncount[i] = over ? 1 : 0;
-        } else if (ncount[i]) {  // not recorded at the rebuild: the mask walk (and so its successor)
-            for_each_neighbor<WALK_MASK, true, true>(q, i, xi, cell, cs, b.posr, walk_body, NoRec{}, gath);
         } else {
-            skin_walk<true>(qr, i, xi, xc, cell, cs, b.posr, sr.skin, n, exact, [](int, int, uint32_t) {},
-                            rec_exact, gath);
+            for_each_neighbor<WALK, true, true>(q, i, xi, cell, cs, b.posr, walk_body, NoRec{}, gath, stg);
         }
-    } else if (WALK == WALK_HITS) {  // the mask walk, recording each chunk's hit mask
-        uint32_t *hits = reinterpret_cast<uint32_t *>(nlist);
-        bool over = false;  // a row with more than 64 candidates: not recorded
-        for_each_neighbor<WALK_MASK, true, true>(q, i, xi, cell, cs, b.posr, walk_body,
-                                                 [&](int r, int c, uint32_t m) {
-                                                     if (c < 2) hits[(size_t)(2 * r + c) * (size_t)q0.N + (size_t)i] = m;
-                                                     else over = true;
-                                                 },
-                                                 gath, stg);
-        ncount[i] = over ? 1 : 0;
-    } else {
-        for_each_neighbor<WALK, true, true>(q, i, xi, cell, cs, b.posr, walk_body, NoRec{}, gath, stg);
-    }
-    const float trL = lxx + lyy + lzz;
-    if (ell && q.dim == 2) lyy += 1.0f;  // x-z plane: unit yy entry keeps L invertible
-    const float c00 = lyy * lzz - lyz * lyz;
-    const float c01 = lxz * lyz - lxy * lzz;
-    const float c02 = lxy * lyz - lxz * lyy;
-    const float det = lxx * c00 + lxy * c01 + lxz * c02;
-    if (delta) {
-        if (fabsf(det) > 1.8e-8f) {
-            const float c11 = lxx * lzz - lxz * lxz;
-            const float c12 = lxy * lxz - lxx * lyz;
-            const float c22 = lxx * lyy - lxy * lxy;
-            const float id = 1.0f / det;
-            const float ox = (c00 * gx + c01 * gy + c02 * gz) * id;
-            const float oy = (c01 * gx + c11 * gy + c12 * gz) * id;
-            const float oz = (c02 * gx + c12 * gy + c22 * gz) * id;
-            gx = ox; gy = oy; gz = oz;
+        const float trL = lxx + lyy + lzz;
+        if (ell && q.dim == 2) lyy += 1.0f;  // x-z plane: unit yy entry keeps L invertible
+        const float c00 = lyy * lzz - lyz * lyz;
+        const float c01 = lxz * lyz - lxy * lzz;
+        const float c02 = lxy * lyz - lxz * lyy;
+        const float det = lxx * c00 + lxy * c01 + lxz * c02;
+        if (delta) {
+            if (fabsf(det) > 1.8e-8f) {
+                const float c11 = lxx * lzz - lxz * lxz;
+                const float c12 = lxy * lxz - lxx * lyz;
+                const float c22 = lxx * lyy - lxy * lxy;
+                const float id = 1.0f / det;
+                const float ox = (c00 * gx + c01 * gy + c02 * gz) * id;
+                const float oy = (c01 * gx + c11 * gy + c12 * gz) * id;
+                const float oz = (c02 * gx + c12 * gy + c22 * gz) * id;
+                gx = ox; gy = oy; gz = oz;
+            }
+        } else {
+            gx = gy = gz = 0.0f;
         }
-    } else {
-        gx = gy = gz = 0.0f;
-    }
-    gd[i] = make_float4(gx, gy, gz, D);
-    if constexpr (FEAT) {
-        const float ic0 = rsqrtf(q.cs2), hc = q.h * ic0, hr = q.h / q.rho0;
-        const float self = q.adh * q.m / xi.w;  // W(0) V_i
-        const float wx = kzy - kyz, wy = kxz - kzx, wz = kyx - kxy;
-        const float dv = kxx + kyy + kzz, third = dv / (float)q.dim;
-        const float sxx = kxx - third, syy = (q.dim == 3) ? kyy - third : 0.0f, szz = kzz - third;
-        const float sxy = 0.5f * (kxy + kyx), sxz = 0.5f * (kxz + kzx), syz = 0.5f * (kyz + kzy);
-        const float g2 = gx * gx + gy * gy + gz * gz;
-        const float v2 = vi.x * vi.x + vi.y * vi.y + vi.z * vi.z;
-        float f[ZSPH_NF];
-        f[0] = 1.0f - (ws + self);
-        f[1] = trL / (float)q.dim;
-        f[2] = det;
-        f[3] = sqrtf(lxy * lxy + lxz * lxz + lyz * lyz);
-        f[4] = gx * hr; f[5] = gy * hr; f[6] = gz * hr;
-        f[7] = sqrtf(g2) * hr;
-        f[8] = D * hc;
-        f[9] = wx * hc; f[10] = wy * hc; f[11] = wz * hc;
-        f[12] = sqrtf(wx * wx + wy * wy + wz * wz) * hc;
-        f[13] = sqrtf(sxx * sxx + syy * syy + szz * szz + 2.0f * (sxy * sxy + sxz * sxz + syz * syz)) * hc;
-        f[14] = vi.x * ic0; f[15] = vi.y * ic0; f[16] = vi.z * ic0;
-        f[17] = sqrtf(v2) * ic0;
-        f[18] = (xi.w - q.rho0) / q.rho0;
-        f[19] = vi.w / (q.rho0 * q.cs2);
-        f[20] = (float)nnb;
-        f[21] = (ws > 0.0f) ? sqrtf(wv2 / ws) * ic0 : 0.0f;
-        f[22] = (ws > 0.0f) ? wdr / (ws * q.rho0) : 0.0f;
-        f[23] = nnb ? (float)ndum / (float)nnb : 0.0f;
-        float *o = feat + (size_t)i * ZSPH_NF;
-#pragma unroll
-        for (int k = 0; k < ZSPH_NF; k += 4) *reinterpret_cast<float4 *>(o + k) = make_float4(f[k], f[k + 1], f[k + 2], f[k + 3]);
-    }
+        gd[i] = make_float4(gx, gy, gz, D);
+        if constexpr (FEAT) {
+            const float ic0 = rsqrtf(q.cs2), hc = q.h * ic0, hr = q.h / q.rho0;
+            const float self = q.adh * q.m / xi.w;  // W(0) V_i
+            const float wx = kzy - kyz, wy = kxz - kzx, wz = kyx - kxy;
+            const float dv = kxx + kyy + kzz, third = dv / (float)q.dim;
+            const float sxx = kxx - third, syy = (q.dim == 3) ? kyy - third : 0.0f, szz = kzz - third;
+            const float sxy = 0.5f * (kxy + kyx), sxz = 0.5f * (kxz + kzx), syz = 0.5f * (kyz + kzy);
+            const float g2 = gx * gx + gy * gy + gz * gz;
+            const float v2 = vi.x * vi.x + vi.y * vi.y + vi.z * vi.z;
+            float f[ZSPH_NF];
+            f[0] = 1.0f - (ws + self);
+            f[1] = trL / (float)q.dim;
+            f[2] = det;
+            f[3] = sqrtf(lxy * lxy + lxz * lxz + lyz * lyz);
+            f[4] = gx * hr; f[5] = gy * hr; f[6] = gz * hr;
+            f[7] = sqrtf(g2) * hr;
+            f[8] = D * hc;
+            f[9] = wx * hc; f[10] = wy * hc; f[11] = wz * hc;
+            f[12] = sqrtf(wx * wx + wy * wy + wz * wz) * hc;
+            f[13] = sqrtf(sxx * sxx + syy * syy + szz * szz + 2.0f * (sxy * sxy + sxz * sxz + syz * syz)) * hc;
+            f[14] = vi.x * ic0; f[15] = vi.y * ic0; f[16] = vi.z * ic0;
+            f[17] = sqrtf(v2) * ic0;
+            f[18] = (xi.w - q.rho0) / q.rho0;
+            f[19] = vi.w / (q.rho0 * q.cs2);
+            f[20] = (float)nnb;
+            f[21] = (ws > 0.0f) ? sqrtf(wv2 / ws) * ic0 : 0.0f;
+            f[22] = (ws > 0.0f) ? wdr / (ws * q.rho0) : 0.0f;
+            f[23] = nnb ? (float)ndum / (float)nnb : 0.0f;
+            float *o = feat + (size_t)i * ZSPH_NF;
+    #pragma unroll
+            for (int k = 0; k < ZSPH_NF; k += 4) *reinterpret_cast<float4 *>(o + k) = make_float4(f[k], f[k + 1], f[k + 2], f[k + 3]);
+        }
+    });
 }
 
 // Continuity + momentum of fluid particles (delta_ns.cu:204-265; MC form
@@ -322,147 +323,145 @@ __global__ __launch_bounds__(BS, ((WALK != WALK_LIST && !BATCH && MODE == 0) ? Z
         if (i < q0.N) acc[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
         return;
     }
-    const Prm q = scene_prm<BATCH>(q0, sp, b, i);
-    const bool delta = is_antuono<DF>(q);
-    const float4 xi = b.posr[i];
-    const float4 vi = b.velp[i];
-    const float4 gi = gd[i];
-    const float rhoi = xi.w, pi = vi.w;
-    const float irhoi = hw_rcp(rhoi);  // 1/(rho_i rho_j) = irhoi irhoj: one reciprocal per pair
-    float ax = 0, ay = 0, az = 0, dr = 0;
-    float dd = 0, vx = 0, vy = 0, vz = 0;  // diffusion and viscous parts (MODE != 0)
-    // gj: gd[j] (read only for fluid j)
-    auto body = [&](int j, float dx, float dy, float dz, float r2, float4 xj, float4 vj, float4 gj, auto kind) {
-        const float qq = hw_sqrt(r2) * q.inv_h;
-        const float t = 1.0f - 0.5f * qq;
-        const float F = q.fk * t * t * t;
-        const float rhoj = xj.w, pj = vj.w;
-        const float irhoj = hw_rcp(rhoj);
-        const float Vj = q.m * irhoj;
-        const float dvdx = (vi.x - vj.x) * dx + (vi.y - vj.y) * dy + (vi.z - vj.z) * dz;
-        dr += rhoi * dvdx * F * Vj;
-        const float irr = irhoi * irhoj;
-        float pt = -(pi + pj) * q.m * F * irr;
-        float pv = 0.0f;
-        if (fluid_j(kind, j, nf)) {
-            const float ire = hw_rcp(r2 + q.eta2);
-            if (delta) {
-                const float psi = 2.0f * (rhoi - rhoj) * r2 * ire -
-                                  ((gi.x + gj.x) * dx + (gi.y + gj.y) * dy + (gi.z + gj.z) * dz);
-                if (MODE) dd += q.delta_h_cs * F * psi * Vj;
-                else dr += q.delta_h_cs * F * psi * Vj;
-            }
-            if (MODE) pv = q.h_cs_rho_m * F * (gi.w + gj.w + q.alpha * dvdx * ire) * irr;
-            else pt += q.h_cs_rho_m * F * (gi.w + gj.w + q.alpha * dvdx * ire) * irr;
-        }
-        if (!delta) {
-            const float ire = hw_rcp(r2 + q.eta2);
-            if (MODE) dd += (rhoi - rhoj) * r2 * F * q.delta_h_cs * ire * Vj;
-            else dr += (rhoi - rhoj) * r2 * F * q.delta_h_cs * ire * Vj;
-        }
-        ax += pt * dx; ay += pt * dy; az += pt * dz;
-        if (MODE) { vx += pv * dx; vy += pv * dy; vz += pv * dz; }
-    };
-    // gathered per neighbour besides its position: velp[j], and gd[j] for fluid j
-    auto gath = [&](int j, auto kind) {
-        if constexpr (decltype(kind)::value == 1) {
-            return ld(b.velp, j);
-        } else if constexpr (decltype(kind)::value == 0) {
-            return VelGrad{ld(b.velp, j), ld(gd, j)};
-        } else {
-            VelGrad e;
-            e.v = ld(b.velp, j);
-            if (j < nf) e.g = ld(gd, j);  // read by the body for fluid neighbours only
-            return e;
-        }
-    };
-    auto walk_body = [&](int j, float dx, float dy, float dz, float r2, float4 xj, auto kind, const auto &e) {
-        if constexpr (std::is_same_v<std::decay_t<decltype(e)>, float4>) {
-            float4 gj;  // not read for a dummy neighbour
-            body(j, dx, dy, dz, r2, xj, e, gj, kind);
-        } else {
-            body(j, dx, dy, dz, r2, xj, e.v, e.g, kind);
-        }
-    };
-    const uint32_t cell = cell_of(q, skey[i]);
-    const int n = (WALK == WALK_LIST || WALK == WALK_HITS || WALK == WALK_SKIN) ? ncount[i] : 0;
-    // the wave's LDS slices for staging its rows' positions and velocities (one scene)
+    // the wave's LDS slices for staging its rows' positions and velocities
     constexpr int GS = ZS_GOV_SLOTS + STAGE_PAD;
-    __shared__ float4 stage_x[BATCH ? 1 : (BS / 64) * GS];
-    __shared__ float4 stage_v[BATCH ? 1 : (BS / 64) * GS];
-    const auto stg = [&]() {
-        if constexpr (BATCH) return NoStage{};
-        else return LdsStage{stage_x + (threadIdx.x >> 6) * GS, stage_v + (threadIdx.x >> 6) * GS, b.velp,
-                             ZS_GOV_SLOTS};
-    }();
-    // the gathered data of a hit whose velocity is staged: gd[j] for fluid j
-    auto gath_v = [&](int j, auto kind, float4 v) {
-        if constexpr (decltype(kind)::value == 1) {
-            return v;
-        } else if constexpr (decltype(kind)::value == 0) {
-            return VelGrad{v, ld(gd, j)};
-        } else {
-            VelGrad e;
-            e.v = v;
-            if (j < nf) e.g = ld(gd, j);  // read by the body for fluid neighbours only
-            return e;
-        }
-    };
-    if (WALK == WALK_SKIN && n == 0) {  // the exact record, rows around the rebuild position
-        Prm qr = q;
-        qr.trim2 = q.trim2_skin;
-        for_each_recorded_at<true, true>(qr, i, xi, xref[i], cell, cs, b.posr,
-                                         reinterpret_cast<const uint32_t *>(nlist), (size_t)q0.N, walk_body, gath,
-                                         stg, gath_v);
-    } else if (WALK == WALK_HITS && n == 0) {
-        for_each_recorded<true, true>(q, i, xi, cell, cs, b.posr, reinterpret_cast<const uint32_t *>(nlist),
-                                      (size_t)q0.N, walk_body, gath, stg, gath_v);
-    } else if (WALK == WALK_LIST && n <= LIST_MAX) {
-        // R quads (4R recorded neighbours) per round: the round's list reads
-        // and all its gathers are issued before its first interaction, so
-        // they are in flight together (this pass is latency-bound; 8 per
-        // round measured best).  gd of a dummy is zero (k_ldg writes it), so
-        // every gather is unconditional.
-        constexpr int R = 2;
-        const int4 *list = reinterpret_cast<const int4 *>(nlist);
-        for (int k = 0; k < n; k += 4 * R) {
-            int js[4 * R];
-#pragma unroll
-            for (int r = 0; r < R; ++r) {
-                const int4 jj = (k + 4 * r < n) ? nt_load(&list[list_quad(i, (k >> 2) + r)]) : make_int4(0, 0, 0, 0);
-                js[4 * r] = jj.x; js[4 * r + 1] = jj.y; js[4 * r + 2] = jj.z; js[4 * r + 3] = jj.w;
+    __shared__ float4 stage_x[(BS / 64) * GS];
+    __shared__ float4 stage_v[(BS / 64) * GS];
+    with_scene<BATCH>(q0, sp, b, i, [&](const Prm &q) {
+        const bool delta = is_antuono<DF>(q);
+        const float4 xi = b.posr[i];
+        const float4 vi = b.velp[i];
+        const float4 gi = gd[i];
+        const float rhoi = xi.w, pi = vi.w;
+        const float irhoi = hw_rcp(rhoi);  // 1/(rho_i rho_j) = irhoi irhoj: one reciprocal per pair
+        float ax = 0, ay = 0, az = 0, dr = 0;
+        float dd = 0, vx = 0, vy = 0, vz = 0;  // diffusion and viscous parts (MODE != 0)
+        // gj: gd[j] (read only for fluid j)
+        auto body = [&](int j, float dx, float dy, float dz, float r2, float4 xj, float4 vj, float4 gj, auto kind) {
+            const float qq = hw_sqrt(r2) * q.inv_h;
+            const float t = 1.0f - 0.5f * qq;
+            const float F = q.fk * t * t * t;
+            const float rhoj = xj.w, pj = vj.w;
+            const float irhoj = hw_rcp(rhoj);
+            const float Vj = q.m * irhoj;
+            const float dvdx = (vi.x - vj.x) * dx + (vi.y - vj.y) * dy + (vi.z - vj.z) * dz;
+            dr += rhoi * dvdx * F * Vj;
+            const float irr = irhoi * irhoj;
+            float pt = -(pi + pj) * q.m * F * irr;
+            float pv = 0.0f;
+            if (fluid_j(kind, j, nf)) {
+                const float ire = hw_rcp(r2 + q.eta2);
+                if (delta) {
+                    const float psi = 2.0f * (rhoi - rhoj) * r2 * ire -
+                                      ((gi.x + gj.x) * dx + (gi.y + gj.y) * dy + (gi.z + gj.z) * dz);
+                    if (MODE) dd += q.delta_h_cs * F * psi * Vj;
+                    else dr += q.delta_h_cs * F * psi * Vj;
+                }
+                if (MODE) pv = q.h_cs_rho_m * F * (gi.w + gj.w + q.alpha * dvdx * ire) * irr;
+                else pt += q.h_cs_rho_m * F * (gi.w + gj.w + q.alpha * dvdx * ire) * irr;
             }
-#pragma unroll
-            for (int e = 1; e < 4 * R; ++e) js[e] = (k + e < n) ? js[e] : js[0];  // past the end: a valid index
-            float4 xs[4 * R], vs[4 * R], gs[4 * R];
-#pragma unroll
-            for (int e = 0; e < 4 * R; ++e) {
-                xs[e] = ld(b.posr, js[e]);
-                vs[e] = ld(b.velp, js[e]);
-                gs[e] = ld(gd, js[e]);
+            if (!delta) {
+                const float ire = hw_rcp(r2 + q.eta2);
+                if (MODE) dd += (rhoi - rhoj) * r2 * F * q.delta_h_cs * ire * Vj;
+                else dr += (rhoi - rhoj) * r2 * F * q.delta_h_cs * ire * Vj;
             }
-#pragma unroll
-            for (int e = 0; e < 4 * R; ++e) {
-                if (k + e < n) {
-                    const float dx = xi.x - xs[e].x, dy = xi.y - xs[e].y, dz = xi.z - xs[e].z;
-                    body(js[e], dx, dy, dz, dist2(dx, dy, dz), xs[e], vs[e], gs[e], AnyJ{});
+            ax += pt * dx; ay += pt * dy; az += pt * dz;
+            if (MODE) { vx += pv * dx; vy += pv * dy; vz += pv * dz; }
+        };
+        // gathered per neighbour besides its position: velp[j], and gd[j] for fluid j
+        auto gath = [&](int j, auto kind) {
+            if constexpr (decltype(kind)::value == 1) {
+                return ld(b.velp, j);
+            } else if constexpr (decltype(kind)::value == 0) {
+                return VelGrad{ld(b.velp, j), ld(gd, j)};
+            } else {
+                VelGrad e;
+                e.v = ld(b.velp, j);
+                if (j < nf) e.g = ld(gd, j);  // read by the body for fluid neighbours only
+                return e;
+            }
+        };
+        auto walk_body = [&](int j, float dx, float dy, float dz, float r2, float4 xj, auto kind, const auto &e) {
+            if constexpr (std::is_same_v<std::decay_t<decltype(e)>, float4>) {
+                float4 gj;  // not read for a dummy neighbour
+                body(j, dx, dy, dz, r2, xj, e, gj, kind);
+            } else {
+                body(j, dx, dy, dz, r2, xj, e.v, e.g, kind);
+            }
+        };
+        const uint32_t cell = cell_of(q, skey[i]);
+        const int n = (WALK == WALK_LIST || WALK == WALK_HITS || WALK == WALK_SKIN) ? ncount[i] : 0;
+        LdsStage stg{stage_x + (threadIdx.x >> 6) * GS, stage_v + (threadIdx.x >> 6) * GS, b.velp, ZS_GOV_SLOTS};
+        if constexpr (BATCH) stg.tag = b.scene[i];
+        // the gathered data of a hit whose velocity is staged: gd[j] for fluid j
+        auto gath_v = [&](int j, auto kind, float4 v) {
+            if constexpr (decltype(kind)::value == 1) {
+                return v;
+            } else if constexpr (decltype(kind)::value == 0) {
+                return VelGrad{v, ld(gd, j)};
+            } else {
+                VelGrad e;
+                e.v = v;
+                if (j < nf) e.g = ld(gd, j);  // read by the body for fluid neighbours only
+                return e;
+            }
+        };
+        if (WALK == WALK_SKIN && n == 0) {  // the exact record, rows around the rebuild position
+            Prm qr = q;
+            qr.trim2 = q.trim2_skin;
+            for_each_recorded_at<true, true>(qr, i, xi, xref[i], cell, cs, b.posr,
+                                             reinterpret_cast<const uint32_t *>(nlist), (size_t)q0.N, walk_body, gath,
+                                             stg, gath_v);
+        } else if (WALK == WALK_HITS && n == 0) {
+            for_each_recorded<true, true>(q, i, xi, cell, cs, b.posr, reinterpret_cast<const uint32_t *>(nlist),
+                                          (size_t)q0.N, walk_body, gath, stg, gath_v);
+        } else if (WALK == WALK_LIST && n <= LIST_MAX) {
+            // R quads (4R recorded neighbours) per round: the round's list reads
+            // and all its gathers are issued before its first interaction, so
+            // they are in flight together (this pass is latency-bound; 8 per
+            // round measured best).  gd of a dummy is zero (k_ldg writes it), so
+            // every gather is unconditional.
+            constexpr int R = 2;
+            const int4 *list = reinterpret_cast<const int4 *>(nlist);
+            for (int k = 0; k < n; k += 4 * R) {
+                int js[4 * R];
+    #pragma unroll
+                for (int r = 0; r < R; ++r) {
+                    const int4 jj = (k + 4 * r < n) ? nt_load(&list[list_quad(i, (k >> 2) + r)]) : make_int4(0, 0, 0, 0);
+                    js[4 * r] = jj.x; js[4 * r + 1] = jj.y; js[4 * r + 2] = jj.z; js[4 * r + 3] = jj.w;
+                }
+    #pragma unroll
+                for (int e = 1; e < 4 * R; ++e) js[e] = (k + e < n) ? js[e] : js[0];  // past the end: a valid index
+                float4 xs[4 * R], vs[4 * R], gs[4 * R];
+    #pragma unroll
+                for (int e = 0; e < 4 * R; ++e) {
+                    xs[e] = ld(b.posr, js[e]);
+                    vs[e] = ld(b.velp, js[e]);
+                    gs[e] = ld(gd, js[e]);
+                }
+    #pragma unroll
+                for (int e = 0; e < 4 * R; ++e) {
+                    if (k + e < n) {
+                        const float dx = xi.x - xs[e].x, dy = xi.y - xs[e].y, dz = xi.z - xs[e].z;
+                        body(js[e], dx, dy, dz, dist2(dx, dy, dz), xs[e], vs[e], gs[e], AnyJ{});
+                    }
                 }
             }
+        } else {
+            for_each_neighbor<WALK_MASK, true, true>(q, i, xi, cell, cs, b.posr, walk_body, NoRec{}, gath);
         }
-    } else {
-        for_each_neighbor<WALK_MASK, true, true>(q, i, xi, cell, cs, b.posr, walk_body, NoRec{}, gath);
-    }
-    if constexpr (MODE == 0) {
-        acc[i] = make_float4(ax, ay, az + q.g, dr);
-    } else if constexpr (MODE == 1) {
-        const float2 sc = scale[i];
-        acc[i] = make_float4(ax + sc.y * vx, ay + sc.y * vy, az + q.g + sc.y * vz, dr + sc.x * dd);
-    } else {
-        float4 *o = reinterpret_cast<float4 *>(terms + (size_t)i * 8);
-        o[0] = make_float4(dr, dd, ax, ay);
-        o[1] = make_float4(az + q.g, vx, vy, vz);
-        acc[i] = make_float4(ax + vx, ay + vy, az + q.g + vz, dr + dd);
-    }
+        if constexpr (MODE == 0) {
+            acc[i] = make_float4(ax, ay, az + q.g, dr);
+        } else if constexpr (MODE == 1) {
+            const float2 sc = scale[i];
+            acc[i] = make_float4(ax + sc.y * vx, ay + sc.y * vy, az + q.g + sc.y * vz, dr + sc.x * dd);
+        } else {
+            float4 *o = reinterpret_cast<float4 *>(terms + (size_t)i * 8);
+            o[0] = make_float4(dr, dd, ax, ay);
+            o[1] = make_float4(az + q.g, vx, vy, vz);
+            acc[i] = make_float4(ax + vx, ay + vy, az + q.g + vz, dr + dd);
+        }
+    });
 }
 
 // Dummies: moving ones (body, piston) get the pressure force of their fluid
