@@ -80,6 +80,11 @@ SIMDS, VALU_CYCLES, CLOCK_HZ = 1024, 2, 2.4e9
 _PROFILES = ("profiles/counters_neighbour_kernels_dambreak3d.md and "
              "profiles/traffic_neighbour_kernels_dambreak3d_dx0.008.md")
 RECORDED = {
+    ("dambreak3d", 0.008, 2.0, 1, 0.05, 128): {
+        "profile": _PROFILES,
+        "k_ldg": {"traffic": (139411.0 * 2.0 + 171444.0) * 1024.0, "valu_insts": 2.50797e8},
+        "k_governing": {"traffic": (355986.0 * 2.0 + 28966.3) * 1024.0, "valu_insts": 1.92998e8},
+    },
     ("dambreak3d", 0.008, 2.0, 1, 0.1, 128): {
         "profile": _PROFILES,
         "k_ldg": {"traffic": (136702.0 * 2.0 + 170848.0) * 1024.0, "valu_insts": 2.57490e8},
@@ -89,6 +94,11 @@ RECORDED = {
         "profile": _PROFILES,
         "k_ldg": {"traffic": (131766.0 * 2.0 + 170162.0) * 1024.0, "valu_insts": 2.67643e8},
         "k_governing": {"traffic": (337919.0 * 2.0 + 28966.3) * 1024.0, "valu_insts": 1.96918e8},
+    },
+    ("dambreak3d", 0.008, 2.0, 1, 0.3, 128): {
+        "profile": _PROFILES,
+        "k_ldg": {"traffic": (129085.0 * 2.0 + 168821.0) * 1024.0, "valu_insts": 2.75235e8},
+        "k_governing": {"traffic": (329900.0 * 2.0 + 28966.3) * 1024.0, "valu_insts": 1.96657e8},
     },
 }
 
@@ -285,6 +295,23 @@ def main():
             x.set_closure(net)
         workload += f", learned closure {os.path.basename(args.closure)}"
         metric += " (learned closure)"
+        # the network's accuracy against the fine plain run, as its evaluation report recorded
+        # it (python -m zsph_amd.train_closure --noise-floor writes <network>_eval.json)
+        accuracy = None
+        report = os.path.splitext(args.closure)[0] + "_eval.json"
+        if os.path.exists(report):
+            with open(report) as f:
+                ev = json.load(f).get("evaluation", {})
+            if "coarse_learned" in ev:
+                lr, pl = ev["coarse_learned"], ev["coarse_plain"]
+                accuracy = {"recorded_in": os.path.relpath(report, ROOT), "t_end_s": ev["t_end"],
+                            "probe_z_m": ev["probe_z"],
+                            "front_err": lr["front_err_max"], "front_err_plain_coarse": pl["front_err_max"],
+                            "smoothed_peak_p_err": lr["smoothed_peak_p_err"],
+                            "smoothed_peak_p_err_plain_coarse": pl["smoothed_peak_p_err"],
+                            "stepping_time_ratio_fine_over_coarse": lr["speedup"]}
+                if "reordered_sums" in ev:
+                    accuracy["fine_run_noise_floor_p_err"] = ev["reordered_sums"]["fine_vs_fine"]["smoothed_peak_p_err"]
     run_len = {"dambreak3d": 10000, "wavetank3d": 1000, "ensemble2d": 3000}[args.case]
     if args.steps is None:  # the configured run length (the ensemble and the tank: a bounded sample)
         args.steps = run_len
@@ -390,9 +417,16 @@ def main():
     ms_step = wall * 1e3 / args.steps
     value = case.n * args.steps / wall
     b_step = step_bytes(case.n, G, n_fluid)
-    rec = RECORDED.get((args.case, args.dx, args.gsf, tune["walk"], round(tune.get("skin", 0.0), 3),
-                        tune.get("block"))) \
-        if not args.closure else None
+    rec = None
+    if not args.closure:
+        key = (args.case, args.dx, args.gsf, tune["walk"], round(tune.get("skin", 0.0), 3))
+        rec = RECORDED.get(key + (tune.get("block"),))
+        if rec is None:  # the same walk and skin recorded at another block size
+            other = [k for k in RECORDED if k[:5] == key]
+            if other:
+                rec = dict(RECORDED[other[0]])
+                rec["profile"] += (f" (recorded with {other[0][5]}-thread blocks; this run's "
+                                   f"{tune.get('block')}-thread blocks do the same per-particle work)")
     avg = {st: (ms / max(n, 1), n) for st, (ms, n) in clocks.items()}
     # the dominant kernel: the gradient pass (VALU-bound: candidate tests and pair arithmetic)
     roof = kernel_roofline("k_ldg", *avg[z.ST_LDG], ldg_bytes(case.n, n_fluid, G), rec, "valu")
@@ -412,6 +446,7 @@ def main():
         "dtype": "f32",
         "data": "synthetic (lattice case generated in-process, no input files)",
         "config": {"workload": workload,
+                   **({"closure_accuracy": accuracy} if args.closure and accuracy else {}),
                    "particles": case.n, "fluid_particles": n_fluid, "cells": G,
                    "fluid_particle_steps_per_s": n_fluid * args.steps / wall,
                    "timed_steps": [args.start_at, args.start_at + args.steps], "run_steps": run_len,
