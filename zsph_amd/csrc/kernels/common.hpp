@@ -537,6 +537,17 @@ __device__ __forceinline__ auto load_hit(const float4 *__restrict__ posr, G &gat
     using E = decltype(gath(j, k));
     return Hit<E>{j, ld(posr, j), gath(j, k)};
 }
+// Keeps a loaded value live at this point.  A body that tests its hit before
+// using it (the skin walk's exact predicate) reads the density and the
+// gathered record only inside the test's branch, and the compiler then issues
+// those loads inside it: a second L2 round trip per hit after the one the
+// test waits for.  Holding them before the test issues every load of the hit
+// together.
+__device__ __forceinline__ void hold(float v) { asm volatile("" ::"v"(v)); }
+__device__ __forceinline__ void hold_xyz(const float4 &v) {
+    hold(v.x); hold(v.y); hold(v.z);
+}
+__device__ __forceinline__ void hold_xyz(const Empty &) {}
 // Runs the body on hit h of particle i at xi.
 template <class F, class K, class E>
 __device__ __forceinline__ decltype(auto) run_hit(F &f, float4 xi, const Hit<E> &h, K k) {

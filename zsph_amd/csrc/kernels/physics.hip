@@ -207,6 +207,8 @@ __global__ __launch_bounds__(BS, ((WALK != WALK_LIST && !BATCH && !FEAT) ? ZS_LD
             const float4 xc = sr.xref[i];
             auto exact = [&](int j, float dx, float dy, float dz, float r2, float4 xj, auto kind,
                              const auto &e) -> bool {
+                hold(xj.w);  // every load of the hit before the test (one L2 round trip)
+                hold_xyz(e);
                 if (!(r2 <= q.rc2)) return false;
                 walk_body(j, dx, dy, dz, r2, xj, kind, e);
                 return true;
