@@ -389,13 +389,14 @@ __device__ __forceinline__ RowRange box_row(const Prm &q, const CellBox &b, floa
     const float rem = remz - dyc * dyc;
     if (rem < 0.0f) return w;
     pre(r, s);
-    const float wd = sqrtf(rem) + q.trim_pad;
+    // the hardware square root (about 1 ulp) is far inside the trimming pad
+    const float wd = hw_sqrt(rem) + q.trim_pad;
     const int xa = max(b.xlo, (int)floorf(fx - wd));
     const int xb = min(b.xhi, (int)floorf(fx + wd));
     if (xa > xb) return w;
     const int row = q.cell_base + cz * q.XY + cy * q.X;
-    if (FLUID_J) { w.fb = cs[row + xa]; w.fe = cs[row + xb + 1]; }
-    if (DUMMY_J) { w.db = cs[q.G + 1 + row + xa]; w.de = cs[q.G + 1 + row + xb + 1]; }
+    if (FLUID_J) { w.fb = ld(cs, row + xa); w.fe = ld(cs, row + xb + 1); }
+    if (DUMMY_J) { w.db = ld(cs, q.G + 1 + row + xa); w.de = ld(cs, q.G + 1 + row + xb + 1); }
     return w;
 }
 
