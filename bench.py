@@ -100,6 +100,14 @@ RECORDED = {
         "k_ldg": {"traffic": (129085.0 * 2.0 + 168821.0) * 1024.0, "valu_insts": 2.75235e8},
         "k_governing": {"traffic": (329900.0 * 2.0 + 28966.3) * 1024.0, "valu_insts": 1.96657e8},
     },
+    # the recorded skin walk's gradient pass is two gated launches per half step
+    # (recording and replaying forms): their per-launch figures are summed
+    ("wavetank3d", 0.006, 2.0, 3, 0.05, 256): {
+        "profile": "profiles/kernel_stats_wavetank3d_dx0.006.md",
+        "k_ldg": {"traffic": ((258166.0 + 2756080.0) * 2.0 + (202550.0 + 1459580.0)) * 1024.0,
+                  "valu_insts": 1.10794e8 + 1.04900e9},
+        "k_governing": {"traffic": (3853700.0 * 2.0 + 195312.0) * 1024.0, "valu_insts": 1.31611e9},
+    },
 }
 
 
