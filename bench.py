@@ -75,34 +75,34 @@ SIMDS, VALU_CYCLES, CLOCK_HZ = 1024, 2, 2.4e9
 # Recorded hardware counters of the neighbour passes per launch (rocprofv3 --pmc, one counter per
 # pass; the table's profile file gives the commands): HBM traffic = FETCH_SIZE x 2 + WRITE_SIZE x 1
 # (KiB, factors from a device copy of known size) and SQ_INSTS_VALU (wave instructions), measured on the
-# developed flow (from step 3000).  Keyed by (case, dx, grid_size_factor, walk, skin, block); other
-# workloads and tunings report none.
+# developed flow (from step 3000).  Keyed by (case, dx, grid_size_factor, walk, skin, block,
+# tile_rows); other workloads and tunings report none.
 _PROFILES = ("profiles/counters_neighbour_kernels_dambreak3d.md and "
              "profiles/traffic_neighbour_kernels_dambreak3d_dx0.008.md")
 RECORDED = {
-    ("dambreak3d", 0.008, 2.0, 1, 0.05, 128): {
+    ("dambreak3d", 0.008, 2.0, 1, 0.05, 128, 0): {
         "profile": _PROFILES,
         "k_ldg": {"traffic": (139411.0 * 2.0 + 171444.0) * 1024.0, "valu_insts": 2.50797e8},
         "k_governing": {"traffic": (355986.0 * 2.0 + 28966.3) * 1024.0, "valu_insts": 1.92998e8},
     },
-    ("dambreak3d", 0.008, 2.0, 1, 0.1, 128): {
+    ("dambreak3d", 0.008, 2.0, 1, 0.1, 128, 0): {
         "profile": _PROFILES,
         "k_ldg": {"traffic": (136702.0 * 2.0 + 170848.0) * 1024.0, "valu_insts": 2.57490e8},
         "k_governing": {"traffic": (348649.0 * 2.0 + 28966.3) * 1024.0, "valu_insts": 1.95183e8},
     },
-    ("dambreak3d", 0.008, 2.0, 1, 0.2, 128): {
+    ("dambreak3d", 0.008, 2.0, 1, 0.2, 128, 0): {
         "profile": _PROFILES,
         "k_ldg": {"traffic": (131766.0 * 2.0 + 170162.0) * 1024.0, "valu_insts": 2.67643e8},
         "k_governing": {"traffic": (337919.0 * 2.0 + 28966.3) * 1024.0, "valu_insts": 1.96918e8},
     },
-    ("dambreak3d", 0.008, 2.0, 1, 0.3, 128): {
+    ("dambreak3d", 0.008, 2.0, 1, 0.3, 128, 0): {
         "profile": _PROFILES,
         "k_ldg": {"traffic": (129085.0 * 2.0 + 168821.0) * 1024.0, "valu_insts": 2.75235e8},
         "k_governing": {"traffic": (329900.0 * 2.0 + 28966.3) * 1024.0, "valu_insts": 1.96657e8},
     },
     # the recorded skin walk's gradient pass is two gated launches per half step
     # (recording and replaying forms): their per-launch figures are summed
-    ("wavetank3d", 0.006, 2.0, 3, 0.05, 256): {
+    ("wavetank3d", 0.006, 2.0, 3, 0.05, 256, 0): {
         "profile": "profiles/kernel_stats_wavetank3d_dx0.006.md",
         "k_ldg": {"traffic": ((258166.0 + 2756080.0) * 2.0 + (202550.0 + 1459580.0)) * 1024.0,
                   "valu_insts": 1.10794e8 + 1.04900e9},
@@ -233,6 +233,8 @@ def main():
     ap.add_argument("--radix-bits", type=int, default=None, help="force the sort's digit width (8 to 11 bits)")
     ap.add_argument("--block", type=int, default=None, help="force the neighbour passes' block size (128/256/512)")
     ap.add_argument("--sort-tile", type=int, default=None, help="force the radix tile's keys per thread (8/16)")
+    ap.add_argument("--tile-rows", type=int, default=None,
+                    help="force skin mode's wave order: cell rows per unit (0: cell order; default 16)")
     ap.add_argument("--skin", type=float, default=None,
                     help="neighbour skin in units of h (0: rebuild every half step, the reference schedule; > 0: "
                          "rebuild only after a particle moved more than skin/2, the autotuner then also tries "
@@ -343,11 +345,11 @@ def main():
     if args.skin > 0:
         s.set_skin(args.skin)
     # pick the neighbour-walk variant and sort digit width on this case
-    if any(v is not None for v in (args.walk, args.radix_bits, args.block, args.sort_tile)):
+    if any(v is not None for v in (args.walk, args.radix_bits, args.block, args.sort_tile, args.tile_rows)):
         t = s.get_tuning()
         s.set_tuning(t["walk"] if args.walk is None else args.walk,
                      t["radix_bits"] if args.radix_bits is None else args.radix_bits,
-                     args.block, args.sort_tile)
+                     args.block, args.sort_tile, tile_rows=args.tile_rows)
         args.autotune = False
     import tempfile
     if args.state_in:
@@ -428,9 +430,10 @@ def main():
     rec = None
     if not args.closure:
         key = (args.case, args.dx, args.gsf, tune["walk"], round(tune.get("skin", 0.0), 3))
-        rec = RECORDED.get(key + (tune.get("block"),))
-        if rec is None:  # the same walk and skin recorded at another block size
-            other = [k for k in RECORDED if k[:5] == key]
+        rows = tune.get("tile_rows", 0) if tune.get("skin", 0.0) > 0 else 0
+        rec = RECORDED.get(key + (tune.get("block"), rows))
+        if rec is None:  # the same walk, skin and wave order recorded at another block size
+            other = [k for k in RECORDED if k[:5] == key and k[6] == rows]
             if other:
                 rec = dict(RECORDED[other[0]])
                 rec["profile"] += (f" (recorded with {other[0][5]}-thread blocks; this run's "

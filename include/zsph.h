@@ -580,6 +580,14 @@ typedef struct zsph_tuning {
                       * (r^2 <= (2h)^2 decides every pair); the particle
                       * order, and with it the FP32 summation order, follows
                       * the rebuilds.                                        */
+    int tile_rows;   /* skin mode, one scene: the order in which the gradient
+                      * and continuity+momentum passes take their waves.
+                      * > 0: units of tile_rows cell rows of one plane, tile
+                      * by tile and plane by plane inside a tile, so the
+                      * neighbour rows in use on an XCD fit its L2; 0: cell
+                      * order.  Each particle's results are the same in any
+                      * order (bit for bit).  Default 16 (-1 in
+                      * zsph_set_tuning: keep the current value)           */
 } zsph_tuning;
 int zsph_set_tuning(zsph_ctx *ctx, const zsph_tuning *t);
 int zsph_get_tuning(zsph_ctx *ctx, zsph_tuning *t);

@@ -332,13 +332,15 @@ class Solver:
                   "zsph_neighbor_stats")
         return cnt, dig
 
-    def set_tuning(self, walk, radix_bits, block=None, sort_tile=None, skin=None):
-        """Neighbour walk, sort digit width, neighbour-pass block size, sort tile and
-        zsph_step's neighbour skin in h (None keeps the current block size / tile / skin)."""
+    def set_tuning(self, walk, radix_bits, block=None, sort_tile=None, skin=None, tile_rows=None):
+        """Neighbour walk, sort digit width, neighbour-pass block size, sort tile, zsph_step's
+        neighbour skin in h and skin mode's wave order (cell rows per unit, 0: cell order);
+        None keeps the current block size / tile / skin / wave order."""
         cur = self.get_tuning()
         t = abi.Tuning(walk, radix_bits, cur["block"] if block is None else block,
                        cur["sort_tile"] if sort_tile is None else sort_tile,
-                       cur["skin"] if skin is None else skin)
+                       cur["skin"] if skin is None else skin,
+                       cur["tile_rows"] if tile_rows is None else tile_rows)
         self._chk(lib().zsph_set_tuning(self._ctx, C.byref(t)), "zsph_set_tuning")
 
     def set_skin(self, skin):
@@ -350,7 +352,7 @@ class Solver:
         t = abi.Tuning()
         self._chk(lib().zsph_get_tuning(self._ctx, C.byref(t)), "zsph_get_tuning")
         return {"walk": t.walk, "radix_bits": t.radix_bits, "block": t.block, "sort_tile": t.sort_tile,
-                "skin": round(t.skin, 6)}
+                "skin": round(t.skin, 6), "tile_rows": t.tile_rows}
 
     def autotune(self, steps=5):
         """Time tuning candidates from the current buffer-0 state (restored afterwards); keep the fastest."""
@@ -358,7 +360,7 @@ class Solver:
         ms = C.c_float(0.0)
         self._chk(lib().zsph_autotune(self._ctx, steps, C.byref(t), C.byref(ms)), "zsph_autotune")
         return {"walk": t.walk, "radix_bits": t.radix_bits, "block": t.block, "sort_tile": t.sort_tile,
-                "skin": round(t.skin, 6), "ms_per_step": ms.value / steps}
+                "skin": round(t.skin, 6), "tile_rows": t.tile_rows, "ms_per_step": ms.value / steps}
 
     def rebuilds(self):
         """Neighbour-structure rebuilds done by zsph_step in skin mode since creation."""

@@ -89,7 +89,8 @@ class Rigid(C.Structure):
 
 
 class Tuning(C.Structure):
-    _fields_ = [("walk", _i), ("radix_bits", _i), ("block", _i), ("sort_tile", _i), ("skin", _f)]
+    _fields_ = [("walk", _i), ("radix_bits", _i), ("block", _i), ("sort_tile", _i), ("skin", _f),
+                ("tile_rows", _i)]
 
 
 WALK_MASK, WALK_HITS, WALK_LIST, WALK_SKIN = 0, 1, 2, 3

@@ -43,9 +43,16 @@ struct Prm {
     float trim2_skin, rcs2;
     // Stage clocks of the neighbour passes (clk nullptr: off): three words per
     // stage ZSPH_ST_* (open time, ticks, launches) for the stages whose bit is
-    // set in clk_mask; see stage_stamp.  (clk_mask first: no padding.)
+    // set in clk_mask; see stage_stamp.
     unsigned clk_mask;
+    // Wave order of the gradient and continuity+momentum passes (one scene):
+    // wtab[w] = (first, end) of the particles of wave w, for wt_cap waves, in
+    // units of wt_rows cell rows (k_wave_table); wtab nullptr: wave w of the
+    // XCD-dealt blocks takes particles [64 w, 64 w + 64).  (An even number of
+    // 4-byte fields before the pointers: no padding.)
+    int wt_cap, wt_rows;
     unsigned long long *clk;
+    const int2 *wtab;
 };
 
 // Host-side construction (also used by the C ABI layer).
@@ -203,6 +210,28 @@ __device__ __forceinline__ int blocks_of(int n) { return (n + ZS_BLOCK - 1) / ZS
 // block size as a tuning parameter: 128, 256 or 512 threads).
 template <int BS>
 __device__ __forceinline__ int blocks_of_bs(int n) { return (n + BS - 1) / BS; }
+// The particle of this thread in a gradient or continuity+momentum pass of
+// BS-thread blocks (nf = cs[G]): with a wave table (q.wtab, k_wave_table)
+// lane l of the block's wave in that order, the fluid's blocks dealt to the
+// XCDs in contiguous ranges; otherwise particle (block) * BS + thread with the
+// fluid's blocks dealt the same way.  q.N for a lane without a particle.
+template <int BS>
+__device__ __forceinline__ int pass_particle(const Prm &q, int nf) {
+    if (q.wtab) {
+        constexpr int WPB = BS / 64;
+        const int2 h = q.wtab[0];  // (fluid waves, all waves)
+        const int w = xcd_block((h.x + WPB - 1) / WPB) * WPB + (int)(threadIdx.x >> 6);
+        if (w >= h.y) return q.N;
+        const int2 e = q.wtab[1 + w];
+        const int i = e.x + (int)(threadIdx.x & 63);
+        return i < e.y ? i : q.N;
+    }
+    return xcd_block(blocks_of_bs<BS>(nf)) * BS + (int)threadIdx.x;
+}
+// Blocks of a gradient or continuity+momentum pass of BS-thread blocks.
+inline int pass_grid(const Prm &q, int bs) {
+    return q.wtab ? (q.wt_cap + bs / 64 - 1) / (bs / 64) : (q.N + bs - 1) / bs;
+}
 // Runs f(std::integral_constant<int, BS>) for the tuned block size bs of the
 // neighbour passes (any other value runs ZS_BLOCK).
 template <class F>

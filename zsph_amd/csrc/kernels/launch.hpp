@@ -33,6 +33,11 @@ void launch_reorder(hipStream_t s, int n, const int *perm, const Buf &a, const B
 // the positions in xref (gated); count and clear the rebuild flag.
 void launch_skin_commit(hipStream_t s, int n, const Buf &a, const Buf &b, float4 *xref, const int *gate);
 void launch_skin_flag(hipStream_t s, int *flag, unsigned long long *rebuilds, int *rebuilt);
+// Wave order of the neighbour passes (one scene, gated): units of yt cell rows
+// of a plane, tile by tile (k_wave_table).  wtab holds 1 + cap entries, cap =
+// wave_table_cap(N, Y, Z, yt) for a grid of at most Y x Z rows.
+int wave_table_cap(int N, int Y, int Z, int yt);
+void launch_wave_table(hipStream_t s, const Prm &q, const int *cs, int yt, int2 *wtab, int cap, const int *gate);
 
 // physics.hip; walk = WALK_MASK, WALK_HITS or WALK_LIST (common.hpp).
 // WALK_LIST: launch_ldg records the neighbour lists (nlist: list_words(N)

@@ -431,9 +431,71 @@ __global__ void k_skin_flag(int *__restrict__ flag, unsigned long long *__restri
     *flag = 0;
 }
 
+// Wave order of the gradient and continuity+momentum passes (one scene).  In
+// cell order a sweep over whole planes keeps about five planes of neighbour
+// rows in use, more than one XCD's L2 holds.  Here the fluid goes in units of
+// yt consecutive cell rows of one plane (rows [t yt, (t + 1) yt) of plane z:
+// one contiguous range of the fluid), tile by tile and, inside a tile, plane
+// by plane, so the waves in flight on an XCD read a compact block of rows.
+// Each unit's fluid makes waves of up to 64 consecutive particles; the other
+// particles [nf, N) follow in waves of 64.  wtab[0] = (fluid waves, all
+// waves); wave w is wtab[1 + w] = (first, end); the entries up to cap are
+// filled with empty waves.  Every particle lies in exactly one wave, and a
+// particle's own arithmetic does not depend on its wave, so the passes'
+// results do not depend on the order.  One block of 1024 threads.
+__global__ __launch_bounds__(1024) void k_wave_table(Prm q, const int *__restrict__ cs, int yt,
+                                                     int2 *__restrict__ wtab, int cap,
+                                                     const int *__restrict__ gate) {
+    if (gated_off(gate)) return;
+    __shared__ int wsum[1024 / 64];
+    const int tid = (int)threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    const int T = (q.Y + yt - 1) / yt;
+    const int U = T * q.Z;
+    const int nf = cs[q.G];
+    int base = 0;  // waves of the units before this round
+    for (int u0 = 0; u0 < U; u0 += 1024) {
+        const int u = u0 + tid;
+        int a = 0, e = 0, nw = 0;
+        if (u < U) {
+            const int t = u / q.Z, z = u - t * q.Z;
+            const int y0 = t * yt, y1 = min(y0 + yt, q.Y);
+            a = cs[z * q.XY + y0 * q.X];
+            e = cs[z * q.XY + y1 * q.X];
+            nw = (e - a + 63) >> 6;
+        }
+        int s = nw;  // inclusive scan over the wave, then over the block's waves
+#pragma unroll
+        for (int o = 1; o < 64; o <<= 1) {
+            const int v = __shfl_up(s, o, 64);
+            if (lane >= o) s += v;
+        }
+        if (lane == 63) wsum[wv] = s;
+        __syncthreads();
+        int pre = 0, total = 0;
+        for (int k = 0; k < 1024 / 64; ++k) {
+            pre += (k < wv) ? wsum[k] : 0;
+            total += wsum[k];
+        }
+        const int off = base + pre + s - nw;
+        for (int k = 0; k < nw; ++k) wtab[1 + off + k] = make_int2(a + 64 * k, min(a + 64 * (k + 1), e));
+        base += total;
+        __syncthreads();  // wsum is written again next round
+    }
+    const int nd = (q.N - nf + 63) >> 6;
+    for (int k = tid; k < nd; k += 1024) wtab[1 + base + k] = make_int2(nf + 64 * k, min(nf + 64 * (k + 1), q.N));
+    for (int w = base + nd + tid; w < cap; w += 1024) wtab[1 + w] = make_int2(q.N, q.N);
+    if (tid == 0) wtab[0] = make_int2(base, base + nd);
+}
+
 // ---------------------------------------------------------------------------
 // Launchers
 // ---------------------------------------------------------------------------
+
+int wave_table_cap(int N, int Y, int Z, int yt) { return (N + 63) / 64 + ((Y + yt - 1) / yt) * Z + 2; }
+
+void launch_wave_table(hipStream_t s, const Prm &q, const int *cs, int yt, int2 *wtab, int cap, const int *gate) {
+    k_wave_table<<<1, 1024, 0, s>>>(q, cs, yt, wtab, cap, gate);
+}
 static inline int nblocks(int n) { return (n + ZS_BLOCK - 1) / ZS_BLOCK; }
 
 void launch_hash(hipStream_t s, const Prm &q, const Prm *sp, const Buf &b, uint32_t *key, int *idx, int *killed,

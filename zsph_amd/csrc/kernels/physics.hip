@@ -114,7 +114,7 @@ __global__ __launch_bounds__(BS, ((WALK != WALK_LIST && !BATCH && !FEAT) ? ZS_LD
         if (*sr.rebuilt != 0) return;
     }
     const int nf = cs[q0.G];
-    const int i = xcd_block(blocks_of_bs<BS>(nf)) * BS + threadIdx.x;
+    const int i = pass_particle<BS>(q0, nf);
     if (i >= q0.N) return;
     if (i >= nf) {
         gd[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
@@ -318,7 +318,7 @@ __global__ __launch_bounds__(BS, ((WALK != WALK_LIST && !BATCH && MODE == 0) ? Z
                                                         const float4 *__restrict__ xref) {
     stage_stamp(q0, ZSPH_ST_LDG, ZSPH_ST_GOVERNING);
     const int nf = cs[q0.G];
-    const int i = xcd_block(blocks_of_bs<BS>(nf)) * BS + threadIdx.x;
+    const int i = pass_particle<BS>(q0, nf);
     if (i >= nf) {
         // dummies and removed particles: 0 (k_governing_dummy, launched after
         // this kernel when there are moving dummies, overwrites theirs)
@@ -597,7 +597,7 @@ template <int WALK, bool BATCH, int BS>
 static void ldg_walk(hipStream_t s, bool feat, const Prm &q, const Prm *sp, const Buf &b,
                      const uint32_t *skey, const int *cs, float4 *gd, int *nlist, int *ncount, float *f,
                      const SkinRec &sr = SkinRec{}) {
-    const int grid = nblocks_bs(q.N, BS);
+    const int grid = pass_grid(q, BS);
     if constexpr (!BATCH) {
         const bool a = q.diffusion == ZSPH_DIFF_ANTUONO;
         if (a)
@@ -612,7 +612,7 @@ static void ldg_walk(hipStream_t s, bool feat, const Prm &q, const Prm *sp, cons
 template <int WALK>
 static void ldg_feat(hipStream_t s, const Prm &q, const Prm *sp, const Buf &b, const uint32_t *skey, const int *cs,
                      float4 *gd, int *nlist, int *ncount, float *f) {
-    const int grid = nblocks_bs(q.N, ZS_BLOCK);
+    const int grid = pass_grid(q, ZS_BLOCK);
     if (q.diffusion == ZSPH_DIFF_ANTUONO)
         k_ldg<WALK, false, true, ZSPH_DIFF_ANTUONO, ZS_BLOCK><<<grid, ZS_BLOCK, 0, s>>>(q, sp, b, skey, cs, gd, nlist, ncount, f, SkinRec{});
     else
@@ -661,7 +661,7 @@ static void gov_walk(hipStream_t s, int mode, int block, const Prm &q, const Prm
                      const uint32_t *skey, const int *cs, const float4 *gd, float4 *acc, const int *nlist,
                      const int *ncount, const float2 *scale, float *terms, const float4 *xref = nullptr) {
 #define ZS_GOV(MODE_, DF_, BS_)                                                                  \
-    k_governing<WALK, BATCH, MODE_, DF_, BS_><<<nblocks_bs(q.N, BS_), BS_, 0, s>>>(              \
+    k_governing<WALK, BATCH, MODE_, DF_, BS_><<<pass_grid(q, BS_), BS_, 0, s>>>(              \
         q, sp, b, skey, cs, gd, acc, nlist, ncount, scale, terms, xref)
     if constexpr (!BATCH) {
         const bool a = q.diffusion == ZSPH_DIFF_ANTUONO;

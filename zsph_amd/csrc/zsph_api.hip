@@ -70,6 +70,9 @@ struct zsph_ctx {
     int *hits = nullptr;       // WALK_HITS / WALK_SKIN exact hit masks (hits_slots(k) x N words)
     uint32_t *skin_hits = nullptr;  // WALK_SKIN skin hit masks of the last rebuild (same layout)
     bool list_valid = false;   // the lists / hit masks describe the current sorted buffer
+    int tile_rows = 0;         // skin mode's wave order: cell rows per unit (0: cell order; one scene)
+    int2 *wtab = nullptr;      // its wave table, rebuilt with the cell table (room for any tile_rows)
+    int wt_cap = 0;            // waves of the table for tile_rows
     int nscenes = 1;                  // > 1: a batch of independent scenes
     zs::Prm *scene_prm = nullptr;     // per-scene parameters (device, batches only)
     std::vector<zs::Prm> scene_host;  // the same on the host
@@ -340,6 +343,8 @@ int skin_rebuild(zsph_ctx *c, Prof *pf = nullptr) {
     {
         Scope sc(pf, ZSPH_ST_CELLS);
         zs::launch_cell_start(c->stream, c->key[slot], q.N, q.G, q.flag, c->cell_start, c->cs_scratch, gate);
+        if (c->tile_rows && c->wtab)
+            zs::launch_wave_table(c->stream, q, c->cell_start, c->tile_rows, c->wtab, c->wt_cap, gate);
     }
     {
         Scope sc(pf, ZSPH_ST_REORDER);
@@ -355,7 +360,12 @@ int skin_rebuild(zsph_ctx *c, Prof *pf = nullptr) {
 }
 
 int skin_half_step(zsph_ctx *c, bool predictor, Prof *pf = nullptr) {
-    const zs::Prm &q = c->qs;
+    zs::Prm q = c->qs;
+    if (c->tile_rows && c->wtab) {  // the neighbour passes take the wave table's order
+        q.wtab = c->wtab;
+        q.wt_cap = c->wt_cap;
+        q.wt_rows = c->tile_rows;
+    }
     int r;
     if ((r = skin_rebuild(c, pf))) return r;
     if ((r = stage_boundary(c, q, 0, pf, true))) return r;
@@ -488,6 +498,7 @@ int zsph_create(const zsph_params *p, int device, zsph_ctx **out) {
     c->p = *p;
     c->q = zs::make_prm(*p);
     c->scene_off = {0, p->ptc_num};
+    c->tile_rows = 16;  // skin mode's wave order (one scene)
     const int r = create_ctx(c, device, (size_t)p->ptc_num, (size_t)c->q.G, false);
     if (r) {
         zsph_destroy(c);
@@ -946,16 +957,28 @@ int zsph_set_tuning(zsph_ctx *c, const zsph_tuning *t) {
         return set_err(c, ZSPH_E_ARG,
                        "zsph_set_tuning: walk %d / radix_bits %d / block %d / sort_tile %d / skin %g out of range",
                        t->walk, t->radix_bits, t->block, t->sort_tile, (double)t->skin);
+    const int rows = t->tile_rows < 0 ? c->tile_rows : t->tile_rows;
+    if (rows > 4096) return set_err(c, ZSPH_E_ARG, "zsph_set_tuning: tile_rows %d out of range", t->tile_rows);
     if (t->walk == ZSPH_WALK_SKIN && c->nscenes > 1)
         return set_err(c, ZSPH_E_ARG, "zsph_set_tuning: the recorded skin walk is for one-scene contexts");
+    if (rows > 0 && c->nscenes > 1)
+        return set_err(c, ZSPH_E_ARG, "zsph_set_tuning: the tiled wave order is for one-scene contexts");
     if (t->walk == c->walk && t->radix_bits == c->radix_bits && block == c->block && tile == c->sort_tile &&
-        t->skin == c->skin)
+        t->skin == c->skin && rows == c->tile_rows)
         return ZSPH_OK;
     int r;
     if (t->skin > 0.0f && !c->rebuild &&
         ((r = dalloc(c, &c->rebuild, 2)) || (r = dalloc(c, &c->rebuilds, 1)) ||
          (r = dalloc(c, &c->xref, (size_t)c->q.N))))
         return r;
+    // the wave table: room for units of one row (any tile_rows fits); the case's
+    // grid bounds the skin grids' rows and planes
+    if (t->skin > 0.0f && rows > 0 && !c->wtab &&
+        (r = dalloc(c, &c->wtab, (size_t)zs::wave_table_cap(c->q.N, c->q.Y, c->q.Z, 1) + 1)))
+        return r;
+    if (rows != c->tile_rows) c->skin_valid = false;  // the table is built by the next rebuild
+    c->tile_rows = rows;
+    c->wt_cap = rows > 0 ? zs::wave_table_cap(c->q.N, c->q.Y, c->q.Z, rows) : 0;
     if (t->skin != c->skin) {
         if (t->skin > 0.0f && (r = set_skin_grids(c, t->skin))) return r;
         c->skin = t->skin;
@@ -990,6 +1013,7 @@ int zsph_get_tuning(zsph_ctx *c, zsph_tuning *t) {
     t->block = c->block;
     t->sort_tile = c->sort_tile;
     t->skin = c->skin;
+    t->tile_rows = c->tile_rows;
     return ZSPH_OK;
 }
 
@@ -1032,7 +1056,7 @@ int zsph_autotune(zsph_ctx *c, int steps, zsph_tuning *best, float *best_ms) {
     }
     const int passes = (c->keybits + 10) / 11;  // fewest passes of <= 11-bit digits
     const int even = std::max(8, (c->keybits + passes - 1) / passes);
-    zsph_tuning keep{c->walk, c->radix_bits, c->block, c->sort_tile, c->skin};
+    zsph_tuning keep{c->walk, c->radix_bits, c->block, c->sort_tile, c->skin, c->tile_rows};
     float best_t = 1e30f;
     int r = ZSPH_OK;
     // times one candidate from the saved state over n steps; keeps it if it is the fastest so far
@@ -1056,21 +1080,28 @@ int zsph_autotune(zsph_ctx *c, int steps, zsph_tuning *best, float *best_ms) {
     const int nskin = start.skin > 0.0f ? 4 * steps : steps;  // skin mode: time over more steps
     const int last_walk = (start.skin > 0.0f && c->nscenes == 1) ? ZSPH_WALK_SKIN : ZSPH_WALK_LIST;
     for (int walk = ZSPH_WALK_MASK; walk <= last_walk && !r; ++walk) {
-        trial({walk, even, start.block, start.sort_tile, start.skin}, nskin);
-        if (even != 8 && !r) trial({walk, 8, start.block, start.sort_tile, start.skin}, nskin);
+        trial({walk, even, start.block, start.sort_tile, start.skin, start.tile_rows}, nskin);
+        if (even != 8 && !r) trial({walk, 8, start.block, start.sort_tile, start.skin, start.tile_rows}, nskin);
     }
     const int blocks[3] = {128, 256, 512};
     zsph_tuning base = keep;
     for (int k = 0; k < 3 && !r; ++k)
-        if (blocks[k] != base.block) trial({base.walk, base.radix_bits, blocks[k], base.sort_tile, base.skin}, nskin);
+        if (blocks[k] != base.block)
+            trial({base.walk, base.radix_bits, blocks[k], base.sort_tile, base.skin, base.tile_rows}, nskin);
     base = keep;
     for (int tile = 8; tile <= 16 && !r; tile *= 2)
-        if (tile != base.sort_tile) trial({base.walk, base.radix_bits, base.block, tile, base.skin}, nskin);
+        if (tile != base.sort_tile)
+            trial({base.walk, base.radix_bits, base.block, tile, base.skin, base.tile_rows}, nskin);
     if (start.skin > 0.0f) {
         base = keep;
         const float skins[4] = {0.05f, 0.1f, 0.2f, 0.3f};
         for (int k = 0; k < 4 && !r; ++k)
-            if (skins[k] != base.skin) trial({base.walk, base.radix_bits, base.block, base.sort_tile, skins[k]}, nskin);
+            if (skins[k] != base.skin)
+                trial({base.walk, base.radix_bits, base.block, base.sort_tile, skins[k], base.tile_rows}, nskin);
+        if (c->nscenes == 1 && !r) {  // the wave order: cell order against units of 16 rows
+            base = keep;
+            trial({base.walk, base.radix_bits, base.block, base.sort_tile, base.skin, base.tile_rows ? 0 : 16}, nskin);
+        }
     }
     if (!r) r = zsph_set_tuning(c, &keep);
     e = copy_all(false);
@@ -1678,14 +1709,19 @@ int zsph_body_reduce(zsph_ctx *c, int b, double F[3], double T[3]) {
 
 namespace {
 
-// FNV-1a 64 of the parameter blocks (4-byte fields and the 8-byte clock
-// pointer, no padding), without the stage clocks (measurement state, not
-// parameters): a restart must use the parameters the checkpoint was written with.
-static_assert(sizeof(zs::Prm) == offsetof(zs::Prm, clk) + sizeof(void *), "Prm has no padding");
+// FNV-1a 64 of the parameter blocks (4-byte fields and the two 8-byte
+// pointers, no padding), without the stage clocks and the wave order
+// (measurement and scheduling state, not parameters): a restart must use the
+// parameters the checkpoint was written with.
+static_assert(sizeof(zs::Prm) == offsetof(zs::Prm, wtab) + sizeof(void *) &&
+                  offsetof(zs::Prm, wtab) == offsetof(zs::Prm, clk) + sizeof(void *),
+              "Prm has no padding");
 uint64_t params_hash(const zsph_ctx *c) {
     zs::Prm q = c->q;
     q.clk = nullptr;
     q.clk_mask = 0;
+    q.wtab = nullptr;
+    q.wt_cap = q.wt_rows = 0;
     uint64_t h = zsph_fnv1a64(&q, sizeof(q));
     for (const zs::Prm &p : c->scene_host) h = zsph_fnv1a64(&p, sizeof(p), h);
     return h;
