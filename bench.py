@@ -100,6 +100,27 @@ RECORDED = {
         "k_ldg": {"traffic": (129085.0 * 2.0 + 168821.0) * 1024.0, "valu_insts": 2.75235e8},
         "k_governing": {"traffic": (329900.0 * 2.0 + 28966.3) * 1024.0, "valu_insts": 1.96657e8},
     },
+    # the default wave order: units of 16 cell rows of a plane, tile by tile
+    ("dambreak3d", 0.008, 2.0, 1, 0.05, 128, 16): {
+        "profile": _PROFILES,
+        "k_ldg": {"traffic": (76376.9 * 2.0 + 178441.0) * 1024.0, "valu_insts": 2.46811e+08},
+        "k_governing": {"traffic": (191424.0 * 2.0 + 29539.5) * 1024.0, "valu_insts": 1.89055e+08},
+    },
+    ("dambreak3d", 0.008, 2.0, 1, 0.1, 128, 16): {
+        "profile": _PROFILES,
+        "k_ldg": {"traffic": (70036.7 * 2.0 + 178111.0) * 1024.0, "valu_insts": 2.53138e+08},
+        "k_governing": {"traffic": (179546.0 * 2.0 + 29548.6) * 1024.0, "valu_insts": 1.90899e+08},
+    },
+    ("dambreak3d", 0.008, 2.0, 1, 0.2, 128, 16): {
+        "profile": _PROFILES,
+        "k_ldg": {"traffic": (77368.8 * 2.0 + 176659.0) * 1024.0, "valu_insts": 2.63319e+08},
+        "k_governing": {"traffic": (181200.0 * 2.0 + 29544.6) * 1024.0, "valu_insts": 1.92546e+08},
+    },
+    ("dambreak3d", 0.008, 2.0, 1, 0.3, 128, 16): {
+        "profile": _PROFILES,
+        "k_ldg": {"traffic": (85604.8 * 2.0 + 174888.0) * 1024.0, "valu_insts": 2.70571e+08},
+        "k_governing": {"traffic": (181395.0 * 2.0 + 29542.6) * 1024.0, "valu_insts": 1.92063e+08},
+    },
     # the recorded skin walk's gradient pass is two gated launches per half step
     # (recording and replaying forms): their per-launch figures are summed
     ("wavetank3d", 0.006, 2.0, 3, 0.05, 256, 0): {
@@ -107,6 +128,12 @@ RECORDED = {
         "k_ldg": {"traffic": ((258166.0 + 2756080.0) * 2.0 + (202550.0 + 1459580.0)) * 1024.0,
                   "valu_insts": 1.10794e8 + 1.04900e9},
         "k_governing": {"traffic": (3853700.0 * 2.0 + 195312.0) * 1024.0, "valu_insts": 1.31611e9},
+    },
+    ("wavetank3d", 0.006, 2.0, 3, 0.05, 256, 16): {
+        "profile": "profiles/kernel_stats_wavetank3d_dx0.006.md",
+        "k_ldg": {"traffic": ((186650.0 + 1848610.0) * 2.0 + (203892.0 + 1462990.0)) * 1024.0,
+                  "valu_insts": 1.11066e8 + 1.05222e9},
+        "k_governing": {"traffic": (2364560.0 * 2.0 + 197813.0) * 1024.0, "valu_insts": 1.32075e9},
     },
 }
 
