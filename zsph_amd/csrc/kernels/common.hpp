@@ -688,9 +688,10 @@ struct NoRec {
 // range of the fluid segment and one of the others.  The wave copies those
 // ranges of both rows of a pair into its LDS slice with coalesced loads
 // (their bounds are read once per wave from the cell table), and each lane
-// tests its own candidates there instead of gathering them from L2.  The
-// staged records are the positions themselves, so the hit masks, and
-// everything after them, are those of the unstaged test.
+// tests its own candidates there instead of gathering them from L2, then
+// reads its hits' positions there too.  The staged records are the positions
+// themselves, so the hit masks, and everything after them, are those of the
+// unstaged walk.
 constexpr int STAGE_SLOTS = 256;  // records per wave: the four ranges of a row pair
 constexpr int STAGE_PAD = 4;      // a test round reads up to three records past a row's end
 struct NoStage {
@@ -921,7 +922,13 @@ __device__ __forceinline__ void for_each_neighbor(const Prm &q, int i, float4 xi
                 if ((uint32_t)us < 32u) mB &= ~(1u << us);
                 rec(rB, 0, mB);
             }
-            visit_rows<FLUID_J, DUMMY_J>(mA, mB, A, B, load, use);
+            if (STAGED && sp.ok) {  // the hits' positions from the staged rows, their other data gathered
+                NoGather none;
+                visit_rows_lds<FLUID_J, DUMMY_J>(mA, mB, A, B, stg, sp.oAf + (A.fb - sp.sAf), sp.oAd + (A.db - sp.sAd),
+                                                 sp.oBf + (B.fb - sp.sBf), sp.oBd + (B.db - sp.sBd), gath, none, use);
+            } else {
+                visit_rows<FLUID_J, DUMMY_J>(mA, mB, A, B, load, use);
+            }
         } else {
             row_chunks(A, rA);
             if (rB >= 0) row_chunks(B, rB);
