@@ -362,9 +362,13 @@ __device__ __forceinline__ void skin_flag_wave(const SkinCheck &k, bool moved) {
     if (m && (int)(threadIdx.x & 63) == __builtin_ctzll(m)) *k.flag = 1;
 }
 
-// Decode a linear cell index into the clamped search box.
+// Decode a linear cell index into the particle's cell and the x range of
+// its search box (clamped to the grid).  A box row (cz + dz, cy + dy) with
+// |dz|, |dy| <= k lies outside the clamped box exactly when it lies outside
+// the grid, so the rows are checked against the grid's bounds instead of
+// keeping the box's y and z limits.
 struct CellBox {
-    int xlo, xhi, ylo, yhi, zlo, zhi;
+    int xlo, xhi;
     int cy, cz;  // the particle's own cell row
 };
 __device__ __forceinline__ CellBox cell_box(const Prm &q, uint32_t key) {
@@ -374,8 +378,6 @@ __device__ __forceinline__ CellBox cell_box(const Prm &q, uint32_t key) {
     const int cx = rem - cy * q.X;
     CellBox b;
     b.xlo = max(cx - q.k, 0); b.xhi = min(cx + q.k, q.X - 1);
-    b.ylo = max(cy - q.k, 0); b.yhi = min(cy + q.k, q.Y - 1);
-    b.zlo = max(cz - q.k, 0); b.zhi = min(cz + q.k, q.Z - 1);
     b.cy = cy;
     b.cz = cz;
     return b;
@@ -411,7 +413,7 @@ __device__ __forceinline__ RowRange box_row(const Prm &q, const CellBox &b, floa
                                             const int *__restrict__ cs, int dz, int dy, int r, int s, P &pre) {
     RowRange w;
     const int cz = b.cz + dz, cy = b.cy + dy;
-    if (cz < b.zlo || cz > b.zhi || cy < b.ylo || cy > b.yhi) return w;
+    if ((unsigned)cz >= (unsigned)q.Z || (unsigned)cy >= (unsigned)q.Y) return w;
     const float dzc = fmaxf(0.0f, fmaxf((float)cz - fz, fz - (float)(cz + 1)));
     const float dyc = fmaxf(0.0f, fmaxf((float)cy - fy, fy - (float)(cy + 1)));
     const float remz = q.trim2 - dzc * dzc;
