@@ -103,23 +103,23 @@ RECORDED = {
     # the default wave order: units of 16 cell rows of a plane, tile by tile
     ("dambreak3d", 0.008, 2.0, 1, 0.05, 128, 16): {
         "profile": _PROFILES,
-        "k_ldg": {"traffic": (106172.0 * 2.0 + 192666.0) * 1024.0, "valu_insts": 2.56263e+08},
-        "k_governing": {"traffic": (212416.0 * 2.0 + 29507.7) * 1024.0, "valu_insts": 1.89055e+08},
+        "k_ldg": {"traffic": (84936.7 * 2.0 + 182540.0) * 1024.0, "valu_insts": 2.55211e+08},
+        "k_governing": {"traffic": (209375.0 * 2.0 + 29515.8) * 1024.0, "valu_insts": 1.88239e+08},
     },
     ("dambreak3d", 0.008, 2.0, 1, 0.1, 128, 16): {
         "profile": _PROFILES,
-        "k_ldg": {"traffic": (97337.8 * 2.0 + 192372.0) * 1024.0, "valu_insts": 2.62532e+08},
-        "k_governing": {"traffic": (191839.0 * 2.0 + 29530.8) * 1024.0, "valu_insts": 1.90899e+08},
+        "k_ldg": {"traffic": (78966.5 * 2.0 + 182108.0) * 1024.0, "valu_insts": 2.61419e+08},
+        "k_governing": {"traffic": (198885.0 * 2.0 + 29529.8) * 1024.0, "valu_insts": 1.90084e+08},
     },
     ("dambreak3d", 0.008, 2.0, 1, 0.2, 128, 16): {
         "profile": _PROFILES,
-        "k_ldg": {"traffic": (93174.2 * 2.0 + 191834.0) * 1024.0, "valu_insts": 2.72587e+08},
-        "k_governing": {"traffic": (184659.0 * 2.0 + 29536.2) * 1024.0, "valu_insts": 1.92546e+08},
+        "k_ldg": {"traffic": (86605.6 * 2.0 + 180882.0) * 1024.0, "valu_insts": 2.71388e+08},
+        "k_governing": {"traffic": (212369.0 * 2.0 + 29486.6) * 1024.0, "valu_insts": 1.91732e+08},
     },
     ("dambreak3d", 0.008, 2.0, 1, 0.3, 128, 16): {
         "profile": _PROFILES,
-        "k_ldg": {"traffic": (93194.8 * 2.0 + 190398.0) * 1024.0, "valu_insts": 2.79621e+08},
-        "k_governing": {"traffic": (184894.0 * 2.0 + 29535.1) * 1024.0, "valu_insts": 1.92063e+08},
+        "k_ldg": {"traffic": (93798.4 * 2.0 + 179186.0) * 1024.0, "valu_insts": 2.78358e+08},
+        "k_governing": {"traffic": (221221.0 * 2.0 + 29480.9) * 1024.0, "valu_insts": 1.9125e+08},
     },
     # the recorded skin walk's gradient pass is two gated launches per half step
     # (recording and replaying forms): their per-launch figures are summed
