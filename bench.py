@@ -129,6 +129,11 @@ RECORDED = {
                   "valu_insts": 1.10794e8 + 1.04900e9},
         "k_governing": {"traffic": (3853700.0 * 2.0 + 195312.0) * 1024.0, "valu_insts": 1.31611e9},
     },
+    ("wavetank3d", 0.006, 2.0, 1, 0.05, 256, 16): {
+        "profile": "profiles/kernel_stats_wavetank3d_dx0.006.md",
+        "k_ldg": {"traffic": (1097950.0 * 2.0 + 1414740.0) * 1024.0, "valu_insts": 1.74543e9},
+        "k_governing": {"traffic": (2230220.0 * 2.0 + 197827.0) * 1024.0, "valu_insts": 1.31261e9},
+    },
     ("wavetank3d", 0.006, 2.0, 3, 0.05, 256, 16): {
         "profile": "profiles/kernel_stats_wavetank3d_dx0.006.md",
         "k_ldg": {"traffic": ((186650.0 + 1848610.0) * 2.0 + (203892.0 + 1462990.0)) * 1024.0,
