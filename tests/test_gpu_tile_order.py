@@ -4,8 +4,9 @@ of one plane, tile by tile (k_wave_table), instead of in cell order.  A
 particle's arithmetic and neighbour order do not depend on which wave runs
 it, so every order gives the same state bit for bit:
 
-* 3-D dam breaks over 150 steps with rebuilds, for the hit-mask walk and the
-  recorded skin walk, at 0 (cell order), 1, 7 and 16 rows per unit;
+* 3-D dam breaks over 150 steps with rebuilds, for every walk (mask, hit
+  masks, lists, recorded skin walk), at 0 (cell order), 1, 7 and 16 rows per
+  unit;
 * graph replay and direct launches with the tiled order;
 * a wave tank with a floating body and a piston (moving dummies, body
   coupling), cell order against 16 rows;
@@ -54,7 +55,7 @@ def run(c, walk, rows, steps, graph=True, setup=None):
     return out
 
 
-@pytest.mark.parametrize("walk", [z.WALK_HITS, z.WALK_SKIN])
+@pytest.mark.parametrize("walk", [z.WALK_MASK, z.WALK_HITS, z.WALK_LIST, z.WALK_SKIN])
 def test_wave_orders_give_identical_states(walk):
     c = dam()
     ref, nreb, s = run(c, walk, 0, 150)
