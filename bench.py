@@ -432,9 +432,15 @@ def main():
     t0 = time.perf_counter()
     # replayed in chunks with a host sync between them: bounds the launches in
     # flight (profilers that trace every dispatch keep a record per launch)
+    last_note = t0
     for k0 in range(0, args.steps, args.chunk):
         s.step(min(args.chunk, args.steps - k0))
         s.sync()
+        now = time.perf_counter()
+        if now - last_note > 30.0:  # a progress line about every 30 s of a long timed region
+            print(f"[bench] {k0 + min(args.chunk, args.steps - k0)} of {args.steps} timed steps done",
+                  file=sys.stderr, flush=True)
+            last_note = now
     wall = time.perf_counter() - t0
     rebuilds = s.rebuilds() - rebuilds0
     clocks = {st: s.clock_read_stage(st) for st in (z.ST_WALL, z.ST_LDG, z.ST_GOVERNING)}
