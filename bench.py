@@ -261,7 +261,7 @@ def main():
     ap.add_argument("--profile-steps", type=int, default=3)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-autotune", dest="autotune", action="store_false")
-    ap.add_argument("--walk", type=int, default=None, help="force a neighbour walk (0 mask, 1 hit masks, 2 lists)")
+    ap.add_argument("--walk", type=int, default=None, help="force a neighbour walk (0 mask, 1 hit masks, 2 lists, 3 recorded skin walk)")
     ap.add_argument("--radix-bits", type=int, default=None, help="force the sort's digit width (8 to 11 bits)")
     ap.add_argument("--block", type=int, default=None, help="force the neighbour passes' block size (128/256/512)")
     ap.add_argument("--sort-tile", type=int, default=None, help="force the radix tile's keys per thread (8/16)")
