@@ -34,7 +34,9 @@ Every case runs zsph_step in skin mode by default (--skin 0.1, tuned among
 neighbour structures are rebuilt only after a particle has moved more than half
 the skin since the last rebuild, instead of every half step; neighbour sets are
 unchanged (the r^2 <= (2h)^2 predicate decides every pair).  --skin 0 times
-the reference schedule.
+the reference schedule.  In skin mode a one-scene case's neighbour passes take
+their waves in units of 16 cell rows of a plane, tile by tile, for L2 locality
+(--tile-rows; 0: cell order; results are the same bit for bit).
 
 --case wavetank3d: the floating-body tank, 4.0 x 1.0 x 1.2 m filled to 0.6 m
 at dx 0.006 m (11.1 M fluid, 12.5 M particles), piston 0.025 m amplitude at
