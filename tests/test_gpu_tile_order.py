@@ -10,6 +10,7 @@ it, so every order gives the same state bit for bit:
 * graph replay and direct launches with the tiled order;
 * a wave tank with a floating body and a piston (moving dummies, body
   coupling), cell order against 16 rows;
+* the learned closure's passes, cell order against 16 rows;
 * the tuning's range and one-scene checks.
 """
 import sys
@@ -98,6 +99,23 @@ def test_tiled_order_with_body_and_piston():
     for x, y in zip(res[0][0], res[1][0]):
         assert np.array_equal(x, y)
     assert np.array_equal(res[0][1], res[1][1]) and np.array_equal(res[0][2], res[1][2])
+
+
+def test_tiled_order_with_learned_closure():
+    # the closure's feature-writing gradient pass and scaled continuity+momentum
+    # pass take their waves from the same table
+    import os
+    from zsph_amd import closure
+    net = closure.load_network(os.path.join(os.path.dirname(closure.__file__), "data", "closure_dambreak3d.npz"))
+    c = dam()
+    res = []
+    for rows in (0, 16):
+        st, nreb, s = run(c, z.WALK_HITS, rows, 60, setup=lambda x: x.set_closure(net))
+        res.append((st, nreb))
+        s.close()
+    assert res[0][1] == res[1][1]
+    for x, y in zip(res[0][0], res[1][0]):
+        assert np.array_equal(x, y)
 
 
 def test_tile_rows_tuning_checks():
