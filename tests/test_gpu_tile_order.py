@@ -11,8 +11,11 @@ it, so every order gives the same state bit for bit:
 * a wave tank with a floating body and a piston (moving dummies, body
   coupling), cell order against 16 rows;
 * the learned closure's passes, cell order against 16 rows;
-* the tuning's range and one-scene checks.
+* the tuning's range and one-scene checks;
+* the stand-alone driver's frames under --skin with --tile-rows 16 and 0.
 """
+import os
+import subprocess
 import sys
 
 import numpy as np
@@ -24,6 +27,7 @@ from tests.helpers import jitter
 
 pytestmark = pytest.mark.gpu
 SKIN = 0.2
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 
 def say(*a):
@@ -137,3 +141,23 @@ def test_tile_rows_tuning_checks():
     with pytest.raises(z.ZsphError):
         s.set_tuning(t["walk"], t["radix_bits"], skin=SKIN, tile_rows=16)
     s.close()
+
+
+def test_driver_tile_rows_frames_identical(tmp_path):
+    exe = os.path.join(ROOT, "zsph_amd", "zsph_run")
+    base = [exe, "--dambreak", "3", "--dx", "0.03", "--output-step", "20", "--steps", "60", "--skin", str(SKIN)]
+    outs = {}
+    for rows in (0, 16):
+        d = tmp_path / f"r{rows}"
+        d.mkdir()
+        say(f"[driver tile rows] {rows}")
+        subprocess.run(base + ["--tile-rows", str(rows), "--out", str(d)], check=True, capture_output=True,
+                       text=True, timeout=120)
+        outs[rows] = d
+    for k in (0, 20, 40):
+        a = (outs[0] / f"zsph{k}.vtk").read_bytes()
+        b = (outs[16] / f"zsph{k}.vtk").read_bytes()
+        assert a == b, f"frame {k} differs between cell order and 16-row units"
+    with pytest.raises(subprocess.CalledProcessError):  # an out-of-range order is refused before any step
+        subprocess.run(base + ["--tile-rows", "5000", "--out", str(tmp_path)], check=True, capture_output=True,
+                       text=True, timeout=120)

@@ -10,10 +10,12 @@
 // after every K-th step (s = steps done); --restart FILE takes the state of
 // such a file and continues the loop at its step (the start_step /
 // current_step the reference's case file carries, io/read_json.cu:32-33).
+// --walk, --skin and --tile-rows set the matching zsph_tuning fields before
+// the first step (each left at the library default when not given).
 //
-//   zsph_run case.json [--checkpoint-step K] [--restart FILE]
+//   zsph_run case.json [--checkpoint-step K] [--restart FILE] [--walk W] [--skin S] [--tile-rows R]
 //   zsph_run --dambreak {2,3} [--dx DX] [--steps N] [--output-step K] [--out DIR] [--gamma G]
-//            [--checkpoint-step K] [--restart FILE]
+//            [--checkpoint-step K] [--restart FILE] [--walk W] [--skin S] [--tile-rows R]
 #include <chrono>
 #include <condition_variable>
 #include <cstdio>
@@ -111,8 +113,10 @@ class Writers {
 int usage() {
     std::fprintf(stderr,
                  "usage: zsph_run case.json [--checkpoint-step K] [--restart FILE] [--sync-output 1] [--writers N] [--walk W]\n"
+                 "                [--skin S] [--tile-rows R]\n"
                  "       zsph_run --dambreak {2,3} [--dx DX] [--steps N] [--output-step K] [--out DIR] [--gamma G] [--gsf F]\n"
-                 "                [--checkpoint-step K] [--restart FILE] [--sync-output 1] [--writers N] [--walk W]\n");
+                 "                [--checkpoint-step K] [--restart FILE] [--sync-output 1] [--writers N] [--walk W]\n"
+                 "                [--skin S] [--tile-rows R]\n");
     return 2;
 }
 
@@ -129,6 +133,8 @@ int main(int argc, char **argv) {
     bool sync_output = false;
     int writers = 0;  // writer threads (0: the case's thread_num)
     int walk = -1;    // neighbour walk (zsph_tuning.walk; -1: the library default)
+    float skin = -1;  // neighbour skin in units of h (zsph_tuning.skin; < 0: the library default)
+    int tile_rows = -1;  // wave order of skin mode (zsph_tuning.tile_rows; -1: the library default)
     std::string restart;
     // options common to both forms; true if argv[a] was one
     auto common = [&](const std::string &k, const char *v) {
@@ -137,6 +143,8 @@ int main(int argc, char **argv) {
         else if (k == "--sync-output") sync_output = std::atoi(v) != 0;
         else if (k == "--writers") writers = std::atoi(v);
         else if (k == "--walk") walk = std::atoi(v);
+        else if (k == "--skin") skin = (float)std::atof(v);
+        else if (k == "--tile-rows") tile_rows = std::atoi(v);
         else return false;
         return true;
     };
@@ -210,10 +218,12 @@ int main(int argc, char **argv) {
         return 1;
     }
     zsph_host_state st{pos.data(), vel.data(), rhop.data(), acc.data(), type.data(), table.data(), nullptr};
-    if (walk >= 0) {
+    if (walk >= 0 || skin >= 0 || tile_rows >= 0) {
         zsph_tuning t;
         zsph_get_tuning(ctx, &t);
-        t.walk = walk;
+        if (walk >= 0) t.walk = walk;
+        if (skin >= 0) t.skin = skin;
+        if (tile_rows >= 0) t.tile_rows = tile_rows;
         if (zsph_set_tuning(ctx, &t) != ZSPH_OK) {
             std::fprintf(stderr, "%s\n", zsph_last_error(ctx));
             zsph_destroy(ctx);
