@@ -76,11 +76,12 @@ __global__ __launch_bounds__(BS) void k_wall(Prm q0, const Prm *__restrict__ sp,
     const float4 xi = b.posr[i];
     const float4 aw = b.tvelp[i];
     float P = 0.0f, S = 0.0f;
+    const float nhh = -0.5f * q.inv_h, ih2 = 2.0f * q.inv_h;  // t = 1 - r/(2h), 1 + 2r/h: one multiply-add each
     for_each_neighbor<WALK, true, false>(q, i, xi, cell_of(q, skey[i]), cs, b.posr,
                                    [&](int j, float dx, float dy, float dz, float r2, float4 xj) {
-                                       const float qq = hw_sqrt(r2) * q.inv_h;
-                                       const float t = 1.0f - 0.5f * qq;
-                                       const float W = q.adh * t * t * t * t * (2.0f * qq + 1.0f);
+                                       const float r = hw_sqrt(r2);
+                                       const float t = fmaf(r, nhh, 1.0f);
+                                       const float W = q.adh * t * t * t * t * fmaf(r, ih2, 1.0f);
                                        const float ax = aw.x * dx + aw.y * dy + (aw.z - q.g) * dz;
                                        P += (ld(b.velp, j).w - xj.w * ax) * W;
                                        S += W;
@@ -135,30 +136,38 @@ __global__ __launch_bounds__(BS, ((WALK != WALK_LIST && !BATCH && !FEAT) ? ZS_LD
         float kxx = 0, kxy = 0, kxz = 0, kyx = 0, kyy = 0, kyz = 0, kzx = 0, kzy = 0, kzz = 0;
         float ws = 0, wv2 = 0, wdr = 0;
         int nnb = 0, ndum = 0;
+        // t = 1 - r/(2h) in one multiply-add; F V_j = (fk m) t^3 / rho_j
+        const float nhh = -0.5f * q.inv_h;
+        const float fkm = q.fk * q.m;
         // vj: velp[j] (read only for fluid j)
         auto body = [&](int j, float dx, float dy, float dz, float r2, float4 xj, float4 vj, auto kind) {
-            const float qq = hw_sqrt(r2) * q.inv_h;
-            const float t = 1.0f - 0.5f * qq;
-            const float Vj = q.m * hw_rcp(xj.w);
-            const float FV = q.fk * t * t * t * Vj;
+            const float r = hw_sqrt(r2);
+            const float t = fmaf(r, nhh, 1.0f);
+            const float FV = fkm * t * t * t * hw_rcp(xj.w);
+            // F V_j x_ij, shared by L, the density gradient and D
+            const float fx = FV * dx, fy = FV * dy, fz = FV * dz;
             if (ell) {
-                lxx -= FV * dx * dx; lxy -= FV * dx * dy; lxz -= FV * dx * dz;
-                lyy -= FV * dy * dy; lyz -= FV * dy * dz; lzz -= FV * dz * dz;
+                lxx -= fx * dx; lxy -= fx * dy; lxz -= fx * dz;
+                lyy -= fy * dy; lyz -= fy * dz; lzz -= fz * dz;
             }
             if (delta) {
-                const float c = FV * (xj.w - xi.w);
-                gx += c * dx; gy += c * dy; gz += c * dz;
+                const float drho = xj.w - xi.w;
+                gx += fx * drho; gy += fy * drho; gz += fz * drho;
             }
             float WV = 0.0f;
             if (FEAT) {
-                WV = q.adh * t * t * t * t * (2.0f * qq + 1.0f) * Vj;
+                const float Vj = q.m * hw_rcp(xj.w);
+                WV = q.adh * t * t * t * t * (2.0f * r * q.inv_h + 1.0f) * Vj;
                 ws += WV;
                 wdr += WV * (xj.w - xi.w);
                 ++nnb;
             }
             if (fluid_j(kind, j, nf)) {
+                // D -= F V_j (v_i - v_j) . x_ij, one multiply-add per component
                 const float ux = vj.x - vi.x, uy = vj.y - vi.y, uz = vj.z - vi.z;
-                D -= FV * ((vi.x - vj.x) * dx + (vi.y - vj.y) * dy + (vi.z - vj.z) * dz);
+                D += fx * ux;
+                D += fy * uy;
+                D += fz * uz;
                 if (FEAT) {
                     kxx += FV * ux * dx; kxy += FV * ux * dy; kxz += FV * ux * dz;
                     kyx += FV * uy * dx; kyy += FV * uy * dy; kyz += FV * uy * dz;
@@ -338,10 +347,10 @@ __global__ __launch_bounds__(BS, ((WALK != WALK_LIST && !BATCH && MODE == 0) ? Z
         const float irhoi = hw_rcp(rhoi);  // 1/(rho_i rho_j) = irhoi irhoj: one reciprocal per pair
         float ax = 0, ay = 0, az = 0, dr = 0;
         float dd = 0, vx = 0, vy = 0, vz = 0;  // diffusion and viscous parts (MODE != 0)
+        const float nhh = -0.5f * q.inv_h;       // t = 1 - r/(2h) in one multiply-add
         // gj: gd[j] (read only for fluid j)
         auto body = [&](int j, float dx, float dy, float dz, float r2, float4 xj, float4 vj, float4 gj, auto kind) {
-            const float qq = hw_sqrt(r2) * q.inv_h;
-            const float t = 1.0f - 0.5f * qq;
+            const float t = fmaf(hw_sqrt(r2), nhh, 1.0f);
             const float F = q.fk * t * t * t;
             const float rhoj = xj.w, pj = vj.w;
             const float irhoj = hw_rcp(rhoj);
