@@ -694,7 +694,16 @@ struct NoRec {
 // reads its hits' positions there too.  The staged records are the positions
 // themselves, so the hit masks, and everything after them, are those of the
 // unstaged walk.
-constexpr int STAGE_SLOTS = 256;  // records per wave: the four ranges of a row pair
+// Records per wave: the four ranges of a row pair; a larger pair is gathered
+// from L2.  Staging a very wide pair costs more than gathering its hits, so
+// the best cap is below the LDS limit (about 350 at 7 waves per SIMD).
+// Gradient pass per launch, developed flows: dam break 0.376 / 0.372 / 0.367
+// / 0.374 / 0.377 ms at 256 / 288 / 316 / 332 / 352 records, wave tank 2.42 /
+// 2.38 / 2.37 ms at 256 / 316 / 352.
+#ifndef ZS_STAGE_SLOTS
+#define ZS_STAGE_SLOTS 316
+#endif
+constexpr int STAGE_SLOTS = ZS_STAGE_SLOTS;
 constexpr int STAGE_PAD = 4;      // a test round reads up to three records past a row's end
 struct NoStage {
     float4 *lds = nullptr;  // unused
