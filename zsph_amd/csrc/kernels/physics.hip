@@ -25,10 +25,11 @@ static inline int nblocks(int n) { return (n + ZS_BLOCK - 1) / ZS_BLOCK; }
 // forms keep the compiler's own budget, about 90 VGPRs and 5 waves: capped
 // like the single-scene forms they spill and measured slower), and the
 // records per wave the continuity+momentum pass stages in LDS (positions and
-// velocities of a row pair; larger pairs are gathered from L2): 6 waves and
-// 192 records fill the CU's 160 KB of LDS without spilling registers
-// (measured 8 % faster than 7 waves with 160 records, which spills, and than
-// 5 waves with 240).
+// velocities of a row pair; larger pairs are gathered from L2): 6 waves
+// without spilling registers, with 176 records (192 measured 8 % faster than
+// 7 waves with 160 records, which spills, and than 5 waves with 240; 176 runs
+// as fast as 192 on the dam break and 0.8 % faster on the wave tank, 160
+// slower).
 #ifndef ZS_LDG_WAVES
 #define ZS_LDG_WAVES 7
 #endif
@@ -36,7 +37,7 @@ static inline int nblocks(int n) { return (n + ZS_BLOCK - 1) / ZS_BLOCK; }
 #define ZS_GOV_WAVES 6
 #endif
 #ifndef ZS_GOV_SLOTS
-#define ZS_GOV_SLOTS 192
+#define ZS_GOV_SLOTS 176
 #endif
 
 __device__ __forceinline__ float *wptr(float4 *a, int i) { return reinterpret_cast<float *>(a + i) + 3; }
