@@ -105,23 +105,23 @@ RECORDED = {
     # the default wave order: units of 16 cell rows of a plane, tile by tile
     ("dambreak3d", 0.008, 2.0, 1, 0.05, 128, 16): {
         "profile": _PROFILES,
-        "k_ldg": {"traffic": (84936.7 * 2.0 + 182540.0) * 1024.0, "valu_insts": 2.55211e+08},
-        "k_governing": {"traffic": (209375.0 * 2.0 + 29515.8) * 1024.0, "valu_insts": 1.88239e+08},
+        "k_ldg": {"traffic": (93810.5 * 2.0 + 187465.0) * 1024.0, "valu_insts": 2.46819e+08},
+        "k_governing": {"traffic": (193329.0 * 2.0 + 29546.5) * 1024.0, "valu_insts": 1.85636e+08},
     },
     ("dambreak3d", 0.008, 2.0, 1, 0.1, 128, 16): {
         "profile": _PROFILES,
-        "k_ldg": {"traffic": (78966.5 * 2.0 + 182108.0) * 1024.0, "valu_insts": 2.61419e+08},
-        "k_governing": {"traffic": (198885.0 * 2.0 + 29529.8) * 1024.0, "valu_insts": 1.90084e+08},
+        "k_ldg": {"traffic": (87634.9 * 2.0 + 186941.0) * 1024.0, "valu_insts": 2.52817e+08},
+        "k_governing": {"traffic": (181999.0 * 2.0 + 29555.9) * 1024.0, "valu_insts": 1.8741e+08},
     },
     ("dambreak3d", 0.008, 2.0, 1, 0.2, 128, 16): {
         "profile": _PROFILES,
-        "k_ldg": {"traffic": (86605.6 * 2.0 + 180882.0) * 1024.0, "valu_insts": 2.71388e+08},
-        "k_governing": {"traffic": (212369.0 * 2.0 + 29486.6) * 1024.0, "valu_insts": 1.91732e+08},
+        "k_ldg": {"traffic": (94168.3 * 2.0 + 185524.0) * 1024.0, "valu_insts": 2.62347e+08},
+        "k_governing": {"traffic": (183317.0 * 2.0 + 29539.5) * 1024.0, "valu_insts": 1.88895e+08},
     },
     ("dambreak3d", 0.008, 2.0, 1, 0.3, 128, 16): {
         "profile": _PROFILES,
-        "k_ldg": {"traffic": (93798.4 * 2.0 + 179186.0) * 1024.0, "valu_insts": 2.78358e+08},
-        "k_governing": {"traffic": (221221.0 * 2.0 + 29480.9) * 1024.0, "valu_insts": 1.9125e+08},
+        "k_ldg": {"traffic": (100933.0 * 2.0 + 183734.0) * 1024.0, "valu_insts": 2.69057e+08},
+        "k_governing": {"traffic": (183096.0 * 2.0 + 29546.1) * 1024.0, "valu_insts": 1.87955e+08},
     },
     # the recorded skin walk's gradient pass is two gated launches per half step
     # (recording and replaying forms): their per-launch figures are summed
