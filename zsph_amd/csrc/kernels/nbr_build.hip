@@ -32,13 +32,11 @@ __device__ __forceinline__ int skin_axis(float x, float o, float size, int dim) 
 // is set (the step's rebuild flag, written by the previous integrator).
 __device__ __forceinline__ bool gated_off(const int *gate) { return gate && *gate == 0; }
 
+// One particle's cell key (k_hash's body).
 template <bool BATCH>
-__global__ __launch_bounds__(ZS_BLOCK) void k_hash(Prm q0, const Prm *__restrict__ sp, Buf b,
-                                                   uint32_t *__restrict__ key, int *__restrict__ idx,
-                                                   int *__restrict__ killed, const int *__restrict__ gate) {
-    if (gated_off(gate)) return;
-    const int i = xcd_block(gridDim.x) * ZS_BLOCK + threadIdx.x;
-    if (i >= q0.N) return;
+__device__ __forceinline__ void hash_one(const Prm &q0, const Prm *__restrict__ sp, const Buf &b,
+                                         uint32_t *__restrict__ key, int *__restrict__ idx, int *__restrict__ killed,
+                                         int i) {
     const Prm q = scene_prm<BATCH>(q0, sp, b, i);
     const float4 x = b.posr[i];
     int cx, cy, cz;
@@ -66,6 +64,17 @@ __global__ __launch_bounds__(ZS_BLOCK) void k_hash(Prm q0, const Prm *__restrict
     // a removed particle stays in its own scene's cell 0 (dummy segment)
     key[i] = ((uint32_t)q.cell_base + h) | (t == ZSPH_FLUID ? 0u : q.flag);
     idx[i] = i;
+}
+
+// Gated (skin mode) launches cap the grid so that a skipped rebuild costs a
+// small launch; each thread then takes every (grid)-th particle.
+template <bool BATCH>
+__global__ __launch_bounds__(ZS_BLOCK) void k_hash(Prm q0, const Prm *__restrict__ sp, Buf b,
+                                                   uint32_t *__restrict__ key, int *__restrict__ idx,
+                                                   int *__restrict__ killed, const int *__restrict__ gate) {
+    if (gated_off(gate)) return;
+    for (int i = xcd_block(gridDim.x) * ZS_BLOCK + threadIdx.x; i < q0.N; i += gridDim.x * ZS_BLOCK)
+        hash_one<BATCH>(q0, sp, b, key, idx, killed, i);
 }
 
 // ---------------------------------------------------------------------------
@@ -344,19 +353,22 @@ __global__ __launch_bounds__(ZS_BLOCK) void k_cs_bounds(const uint32_t *__restri
                                                         uint32_t flag, int T, int *__restrict__ bound,
                                                         const int *__restrict__ gate) {
     if (gated_off(gate)) return;
-    const int i = blockIdx.x * ZS_BLOCK + (int)threadIdx.x;
-    int ta = 0, tb = -1, ua = 0, ub = -1;
-    if (i < n) {
-        const int t = cs_tile_of(skey[i], G, flag);
-        ta = (i == 0) ? 0 : cs_tile_of(skey[i - 1], G, flag) + 1;
-        tb = t;
-        if (i == n - 1) {  // the tiles after the last particle's
-            ua = t + 1;
-            ub = T;
+    // the block's base is uniform, so every lane of a wave makes the same trips
+    for (int base = blockIdx.x * ZS_BLOCK; base < n; base += gridDim.x * ZS_BLOCK) {
+        const int i = base + (int)threadIdx.x;
+        int ta = 0, tb = -1, ua = 0, ub = -1;
+        if (i < n) {
+            const int t = cs_tile_of(skey[i], G, flag);
+            ta = (i == 0) ? 0 : cs_tile_of(skey[i - 1], G, flag) + 1;
+            tb = t;
+            if (i == n - 1) {  // the tiles after the last particle's
+                ua = t + 1;
+                ub = T;
+            }
         }
+        wave_fill(bound, ta, tb, i);  // every lane of the wave takes part
+        wave_fill(bound, ua, ub, n);
     }
-    wave_fill(bound, ta, tb, i);  // every lane of the wave takes part
-    wave_fill(bound, ua, ub, n);
 }
 
 __global__ __launch_bounds__(ZS_BLOCK) void k_cell_start(const uint32_t *__restrict__ skey, int G, uint32_t flag,
@@ -388,16 +400,16 @@ __global__ __launch_bounds__(ZS_BLOCK) void k_cell_start(const uint32_t *__restr
 __global__ __launch_bounds__(ZS_BLOCK) void k_reorder(int n, const int *__restrict__ perm, Buf a, Buf b,
                                                       const int *__restrict__ gate) {
     if (gated_off(gate)) return;
-    const int k = xcd_block(gridDim.x) * ZS_BLOCK + threadIdx.x;
-    if (k >= n) return;
-    const int p = perm[k];
-    b.posr[k] = a.posr[p];
-    b.velp[k] = a.velp[p];
-    b.tposr[k] = a.tposr[p];
-    b.tvelp[k] = a.tvelp[p];
-    b.type[k] = a.type[p];
-    b.table[k] = a.table[p] + p - k;
-    if (a.scene) b.scene[k] = a.scene[p];
+    for (int k = xcd_block(gridDim.x) * ZS_BLOCK + threadIdx.x; k < n; k += gridDim.x * ZS_BLOCK) {
+        const int p = perm[k];
+        b.posr[k] = a.posr[p];
+        b.velp[k] = a.velp[p];
+        b.tposr[k] = a.tposr[p];
+        b.tvelp[k] = a.tvelp[p];
+        b.type[k] = a.type[p];
+        b.table[k] = a.table[p] + p - k;
+        if (a.scene) b.scene[k] = a.scene[p];
+    }
 }
 
 // Skin mode, end of a rebuild: the reordered state in buffer `a` becomes the
@@ -407,17 +419,17 @@ __global__ __launch_bounds__(ZS_BLOCK) void k_reorder(int n, const int *__restri
 __global__ __launch_bounds__(ZS_BLOCK) void k_skin_commit(int n, Buf a, Buf b, float4 *__restrict__ xref,
                                                           const int *__restrict__ gate) {
     if (gated_off(gate)) return;
-    const int k = xcd_block(gridDim.x) * ZS_BLOCK + threadIdx.x;
-    if (k >= n) return;
-    const float4 x = a.posr[k];
-    b.posr[k] = x;
-    xref[k] = x;
-    b.velp[k] = a.velp[k];
-    b.tposr[k] = a.tposr[k];
-    b.tvelp[k] = a.tvelp[k];
-    b.type[k] = a.type[k];
-    b.table[k] = a.table[k];
-    if (a.scene) b.scene[k] = a.scene[k];
+    for (int k = xcd_block(gridDim.x) * ZS_BLOCK + threadIdx.x; k < n; k += gridDim.x * ZS_BLOCK) {
+        const float4 x = a.posr[k];
+        b.posr[k] = x;
+        xref[k] = x;
+        b.velp[k] = a.velp[k];
+        b.tposr[k] = a.tposr[k];
+        b.tvelp[k] = a.tvelp[k];
+        b.type[k] = a.type[k];
+        b.table[k] = a.table[k];
+        if (a.scene) b.scene[k] = a.scene[k];
+    }
 }
 
 // Skin mode, after the rebuild kernels of a half step: count a rebuild that
@@ -497,14 +509,25 @@ void launch_wave_table(hipStream_t s, const Prm &q, const int *cs, int yt, int2 
     k_wave_table<<<1, 1024, 0, s>>>(q, cs, yt, wtab, cap, gate);
 }
 static inline int nblocks(int n) { return (n + ZS_BLOCK - 1) / ZS_BLOCK; }
+// Gated (skin mode) launches of the per-particle rebuild kernels use at most
+// ZS_GATED_BLOCKS blocks (8 per CU), so that a half step without a rebuild
+// costs a small launch; ungated launches take one block per ZS_BLOCK
+// particles.
+#ifndef ZS_GATED_BLOCKS
+#define ZS_GATED_BLOCKS 2048
+#endif
+static inline int gated_blocks(int n, const int *gate) {
+    const int b = nblocks(n);
+    return (gate && b > ZS_GATED_BLOCKS) ? ZS_GATED_BLOCKS : b;
+}
 
 void launch_hash(hipStream_t s, const Prm &q, const Prm *sp, const Buf &b, uint32_t *key, int *idx, int *killed,
                  const int *gate) {
     if (q.N == 0) return;
     if (sp)
-        k_hash<true><<<nblocks(q.N), ZS_BLOCK, 0, s>>>(q, sp, b, key, idx, killed, gate);
+        k_hash<true><<<gated_blocks(q.N, gate), ZS_BLOCK, 0, s>>>(q, sp, b, key, idx, killed, gate);
     else
-        k_hash<false><<<nblocks(q.N), ZS_BLOCK, 0, s>>>(q, sp, b, key, idx, killed, gate);
+        k_hash<false><<<gated_blocks(q.N, gate), ZS_BLOCK, 0, s>>>(q, sp, b, key, idx, killed, gate);
 }
 
 int radix_blocks(int n, int items) { return (n + ZS_BLOCK * items - 1) / (ZS_BLOCK * items); }
@@ -556,18 +579,18 @@ size_t cell_start_scratch(int G) { return (size_t)cs_tiles(G) + 1; }
 void launch_cell_start(hipStream_t s, const uint32_t *skey, int n, int G, uint32_t flag, int *cs, int *scratch,
                        const int *gate) {
     const int T = cs_tiles(G);
-    k_cs_bounds<<<nblocks(n), ZS_BLOCK, 0, s>>>(skey, n, G, flag, T, scratch, gate);
+    k_cs_bounds<<<gated_blocks(n, gate), ZS_BLOCK, 0, s>>>(skey, n, G, flag, T, scratch, gate);
     k_cell_start<<<T, ZS_BLOCK, 0, s>>>(skey, G, flag, scratch, cs, gate);
 }
 
 void launch_reorder(hipStream_t s, int n, const int *perm, const Buf &a, const Buf &b, const int *gate) {
     if (n == 0) return;
-    k_reorder<<<nblocks(n), ZS_BLOCK, 0, s>>>(n, perm, a, b, gate);
+    k_reorder<<<gated_blocks(n, gate), ZS_BLOCK, 0, s>>>(n, perm, a, b, gate);
 }
 
 void launch_skin_commit(hipStream_t s, int n, const Buf &a, const Buf &b, float4 *xref, const int *gate) {
     if (n == 0) return;
-    k_skin_commit<<<nblocks(n), ZS_BLOCK, 0, s>>>(n, a, b, xref, gate);
+    k_skin_commit<<<gated_blocks(n, gate), ZS_BLOCK, 0, s>>>(n, a, b, xref, gate);
 }
 
 void launch_skin_flag(hipStream_t s, int *flag, unsigned long long *rebuilds, int *rebuilt) {
