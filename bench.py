@@ -133,8 +133,8 @@ RECORDED = {
     },
     ("wavetank3d", 0.006, 2.0, 1, 0.05, 256, 16): {
         "profile": "profiles/kernel_stats_wavetank3d_dx0.006.md",
-        "k_ldg": {"traffic": (1097950.0 * 2.0 + 1414740.0) * 1024.0, "valu_insts": 1.74543e9},
-        "k_governing": {"traffic": (2230220.0 * 2.0 + 197827.0) * 1024.0, "valu_insts": 1.31261e9},
+        "k_ldg": {"traffic": (1.19967e+06 * 2.0 + 1.45045e+06) * 1024.0, "valu_insts": 1.6886e+09},
+        "k_governing": {"traffic": (2.22977e+06 * 2.0 + 197816) * 1024.0, "valu_insts": 1.29785e+09},
     },
     ("wavetank3d", 0.006, 2.0, 3, 0.05, 256, 16): {
         "profile": "profiles/kernel_stats_wavetank3d_dx0.006.md",
